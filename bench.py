@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""MPPI benchmark on one MI355X.
+
+Workload (default): 2-D double integrator with power-law (colored) noise and a 1024x1024 cost map, K = 8192 samples,
+T = 100, C = 2 -- one step is the full GPU work of one computeControl call with num_iters = 1 (colored sampling,
+rollout with cost-map lookups, weights, weighted update). The timed region replays device-resident steps (x0, the
+nominal sequence and the map already in HBM) from a captured hipGraph, bracketed by device synchronisations.
+
+Also reported: the host-inclusive control rate (median wall time of computeControl(x, 1), which adds the 16 B x0
+upload, the 800 B control download, SG smoothing and the host state propagation), per-kernel HIP-event durations
+with the roofline of the dominant kernel, and the CPU restatement of the reference rollout loop timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MPPI control Hz + rollout-steps/sec; learned vs plain cost at equal time"
+HBM_PEAK_GBS = 8000.0
+
+
+def build_problem(name: str):
+    from mppi_generic_vendor_amd import capi, workloads
+
+    if name == "di_colored_map":
+        return workloads.double_integrator_map(num_rollouts=8192, num_timesteps=100, colored=True,
+                                               fft_len_mode=capi.FFT_LEN_2T)
+    if name == "cartpole":
+        return workloads.cartpole(num_rollouts=2048, num_timesteps=100)
+    if name == "bicycle_nn":
+        return workloads.bicycle_nn(num_rollouts=16384, num_timesteps=100)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def time_kernels(prob, reps: int):
+    """Average duration of each kernel of one iteration, HIP events on the stream the kernel is launched on."""
+    import ctypes as ct
+
+    from mppi_generic_vendor_amd import capi
+    from mppi_generic_vendor_amd.hip import DeviceBuffer, Event, Stream
+
+    K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
+    s = Stream()
+    dyn = capi.DynamicsParams.from_buffer_copy(prob.dyn)
+    cost = capi.CostParams.from_buffer_copy(prob.cost)
+    keep = []
+    if prob.cost_map is not None:
+        mb = DeviceBuffer.from_numpy(prob.cost_map)
+        keep.append(mb)
+        cost.map = mb.ptr
+    if prob.nn_theta is not None:
+        tb = DeviceBuffer.from_numpy(prob.nn_theta)
+        keep.append(tb)
+        dyn.nn_theta = tb.ptr
+    x0 = DeviceBuffer.from_numpy(prob.x0)
+    mean = DeviceBuffer.zeros((T, C))
+    samples = DeviceBuffer(K * T * C * 4)
+    costs = DeviceBuffer(K * 4)
+    stats = DeviceBuffer(64)
+    out = DeviceBuffer(T * C * 4)
+    sampler_name = "sample_colored" if prob.sampler.kind == capi.SAMPLER_COLORED else "sample_gaussian"
+
+    def sample(i):
+        capi.call("mppi_" + sampler_name, ct.byref(prob.sampler), K, T, mean.ptr, samples.ptr, 1, 0,
+                  ct.c_uint64(i), None, s.ptr)
+
+    def rollout(i):
+        capi.call("mppi_rollout", ct.byref(dyn), ct.byref(cost), ct.byref(prob.sampler), K, T, prob.dt, prob.lambda_,
+                  prob.alpha, x0.ptr, mean.ptr, samples.ptr, costs.ptr, s.ptr)
+
+    def weights(i):
+        capi.call("mppi_compute_weights", costs.ptr, K, prob.lambda_, stats.ptr, s.ptr)
+
+    def update(i):
+        capi.call("mppi_weighted_update", costs.ptr, stats.ptr, samples.ptr, K, T, C, out.ptr, s.ptr)
+
+    res = {}
+    for name, fn in ((sampler_name, sample), ("rollout", rollout), ("weights", weights), ("weighted_update", update)):
+        for i in range(3):
+            sample(i) if name != sampler_name else None
+            fn(i)
+        s.synchronize()
+        e0, e1 = Event(), Event()
+        total = 0.0
+        for i in range(reps):
+            if name == "rollout":  # the rollout clamps its input in place: time it on fresh samples each rep
+                sample(i)
+            e0.record(s)
+            fn(i)
+            e1.record(s)
+            e1.synchronize()
+            total += e0.elapsed_ms(e1)
+        res[name] = total / reps
+    return res
+
+
+def samples_for(prob, k: int, mean: np.ndarray) -> np.ndarray:
+    """The first k control samples of the GPU sampler's buffer for this problem (same seed, rng offset 0)."""
+    import ctypes as ct
+
+    from mppi_generic_vendor_amd import capi
+    from mppi_generic_vendor_amd.hip import DeviceBuffer, synchronize
+
+    T, C = prob.num_timesteps, prob.C
+    mb = DeviceBuffer.from_numpy(mean)
+    out = DeviceBuffer(k * T * C * 4)
+    name = "mppi_sample_colored" if prob.sampler.kind == capi.SAMPLER_COLORED else "mppi_sample_gaussian"
+    capi.call(name, ct.byref(prob.sampler), k, T, mb.ptr, out.ptr, 1, 0, ct.c_uint64(0), None, None)
+    synchronize()
+    return out.to_numpy((k, T, C))
+
+
+def cpu_baseline(prob, seconds: float, threads: int):
+    """The CPU restatement of the reference rollout loop (+ weights and update) on a bounded sample of the same
+    workload and noise, timed on this host; scaled to iterations/s of the full K."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_lib as ol
+
+    K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
+    mean = np.zeros((T, C), np.float32)
+    k_probe = 256
+    u = samples_for(prob, k_probe, mean)
+    t0 = time.perf_counter()
+    costs, clamped = ol.rollout_f32(prob, u, mean, nthreads=threads)
+    w, eta, *_ = ol.weights(costs, prob.lambda_)
+    ol.weighted_update(w, eta, clamped)
+    dt_probe = max(time.perf_counter() - t0, 1e-6)
+    k_sample = int(min(K, max(k_probe, k_probe * seconds / dt_probe / 3)))
+    u = samples_for(prob, k_sample, mean)
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        costs, clamped = ol.rollout_f32(prob, u, mean, nthreads=threads)
+        w, eta, *_ = ol.weights(costs, prob.lambda_)
+        ol.weighted_update(w, eta, clamped)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > seconds * 0.5 or reps >= 50:
+            break
+    per_iter_full = el / reps * (K / k_sample)
+    return {
+        "value": 1.0 / per_iter_full,
+        "unit": "Hz",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{k_sample} of {K} samples x {T} steps, {reps} reps, rollout+weights+update, scaled to K={K}",
+        "rollout_steps_per_s": k_sample * T * reps / el,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="di_colored_map", choices=["di_colored_map", "cartpole", "bicycle_nn"])
+    ap.add_argument("--host-calls", type=int, default=200)
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: there is no multi-GPU path (pass --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mppi_generic_vendor_amd import capi, hip
+    from mppi_generic_vendor_amd.controller import VanillaMPPIController
+
+    capi.load()
+    if hip.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    prob = build_problem(args.workload)
+    K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
+    print(f"[bench] workload {args.workload} K={K} T={T} C={C}", file=sys.stderr, flush=True)
+
+    ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+    x = prob.x0.copy()
+    # host-inclusive control rate: computeControl(x, 1) from the x0 upload to the smoothed host control
+    for _ in range(args.warmup):
+        ctrl.computeControl(x, 1)
+    host_ms = []
+    for _ in range(args.host_calls):
+        t0 = time.perf_counter()
+        ctrl.computeControl(x, 1)
+        host_ms.append((time.perf_counter() - t0) * 1e3)
+    control_hz_host = 1e3 / statistics.median(host_ms)
+    print(f"[bench] host-inclusive computeControl median {statistics.median(host_ms):.4f} ms", file=sys.stderr,
+          flush=True)
+
+    # device-resident steps (the timed region of `value`)
+    ctrl.stepDevice(args.warmup)
+    hip.synchronize()
+    stream = hip.Stream.__new__(hip.Stream)
+    stream.ptr = ctrl.stream()
+    e0, e1 = hip.Event(), hip.Event()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    ctrl.stepDevice(args.steps)
+    e1.record(stream)
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    stream.ptr = None
+    ev_ms = e0.elapsed_ms(e1)
+    value = args.steps / wall
+    ms_per_step = wall * 1e3 / args.steps
+    print(f"[bench] device-resident {ms_per_step:.4f} ms/step (events {ev_ms / args.steps:.4f})", file=sys.stderr,
+          flush=True)
+    st = ctrl.getWeightStats()
+    if not np.isfinite(st.normalizer) or st.normalizer <= 0:
+        raise SystemExit("benchmark iteration produced an invalid normalizer")
+    ctrl.close()
+
+    kern = time_kernels(prob, args.kernel_reps)
+    print(f"[bench] kernel ms {kern}", file=sys.stderr, flush=True)
+    # algorithmic bytes per launch (4 B per FP32): sampler writes K*T*C, rollout reads + writes K*T*C and writes K
+    alg = {
+        "sample_colored": 4 * K * T * C, "sample_gaussian": 4 * K * T * C,
+        "rollout": 8 * K * T * C + 4 * K,
+        "weights": 8 * K + 16,
+        "weighted_update": 4 * K * T * C + 4 * K + 4 * T * C,
+    }
+    dom = max((k for k in kern if k != "weights"), key=lambda k: kern[k])
+    achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "algorithmic_bytes": alg[dom], "avg_ms": round(kern[dom], 5)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(prob, args.cpu_seconds, 1)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Hz",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded cost map, Philox noise)",
+        "config": {"workload": f"{args.workload}: K={K} samples, T={T}, S={prob.S}, C={C}, dt={prob.dt}, "
+                               f"num_iters=1, device-resident graph replay"},
+        "rollout_steps_per_s": round(value * K * T, 1),
+        "control_hz_host": round(control_hz_host, 3),
+        "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,307 @@
+/*
+ * mppi_amd.h -- C-ABI of the MI355X-native MPPI engine.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers, sizes and POD parameter blocks, and returns an int status
+ * (MPPI_OK == 0, negative MPPI_E_* on failure; the message of the last failure on the calling thread is available
+ * from mppi_last_error()). Nothing here ever calls exit().
+ *
+ * Two levels are exported:
+ *   1. kernel-level entry points that work on caller-owned device buffers and a caller-supplied hipStream_t
+ *      (passed as void* so this header needs no HIP include). They replace the reference's kernel launchers.
+ *   2. a controller object (mppi_controller) that owns its device workspace and runs whole MPPI iterations
+ *      (sample -> rollout -> weights -> update), optionally replayed from a captured hipGraph. It replaces
+ *      VanillaMPPIController::computeControl and the getters around it.
+ *
+ * Buffer layouts (all FP32, row-major, C order):
+ *   samples / control samples   [K][T][C]   (reference: sampling_distributions/sampling_distribution.cu:176-177)
+ *   mean / nominal control      [T][C]      (reference: Eigen col-major C x T, i.e. the same bytes)
+ *   costs / weights             [K]
+ *   x0                          [S]
+ *   cost map                    [height][width]
+ *
+ * Reference citations are into MPPI-Generic 0.9.0 (include/mppi/...).
+ */
+#ifndef MPPI_AMD_H
+#define MPPI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * Status codes. The reference exits the process on CUDA errors (utils/gpu_err_chk.cuh:32-40) and on bad launch shapes
+ * (core/mppi_common.cu:1302-1313); here every failure is a return code.
+ * ---------------------------------------------------------------------------------------------------------------- */
+#define MPPI_OK 0
+#define MPPI_E_INVALID -1     /* bad argument, shape or parameter block */
+#define MPPI_E_HIP -2         /* a HIP runtime call failed */
+#define MPPI_E_UNSUPPORTED -3 /* model/cost/sampler combination not compiled into the kernel library */
+#define MPPI_E_NOMEM -4       /* device allocation failed */
+#define MPPI_E_SIZE -5        /* size exceeds a compiled limit (MPPI_MAX_*) */
+
+#define MPPI_MAX_STATE 8
+#define MPPI_MAX_CONTROL 4
+#define MPPI_MAX_OUTPUT 8
+#define MPPI_MAX_TIMESTEPS 256
+#define MPPI_MAX_NN_LAYERS 4
+
+/* Dynamics models (reference: dynamics/cartpole, dynamics/double_integrator, dynamics/racer_dubins,
+ * dynamics/autorally/ar_nn_model for the residual-MLP pattern). */
+#define MPPI_MODEL_CARTPOLE 1
+#define MPPI_MODEL_DOUBLE_INTEGRATOR 2
+#define MPPI_MODEL_RACER_DUBINS 3
+#define MPPI_MODEL_RACER_DUBINS_NN 4
+
+/* Cost functions (reference: cost_functions/cartpole/cartpole_quadratic_cost, cost_functions/quadratic_cost plus the
+ * cost-map and speed terms patterned on cost_functions/autorally/ar_standard_cost). */
+#define MPPI_COST_CARTPOLE_QUADRATIC 1
+#define MPPI_COST_QUADRATIC_MAP 2
+
+/* Sampling distributions (reference: sampling_distributions/gaussian, sampling_distributions/colored_noise). */
+#define MPPI_SAMPLER_GAUSSIAN 1
+#define MPPI_SAMPLER_COLORED 2
+
+/* Colored-noise FFT length: the reference uses n = 2T and keeps the first T outputs
+ * (sampling_distributions/colored_noise/colored_noise.cu:296); MPPI_FFT_LEN_T uses n = T. */
+#define MPPI_FFT_LEN_2T 0
+#define MPPI_FFT_LEN_T 1
+
+/* Precision of the residual MLP inside the rollout. */
+#define MPPI_NN_FP32 0 /* FP32 weights and activations (parity mode) */
+#define MPPI_NN_BF16 1 /* BF16 weights/activations on MFMA, FP32 accumulate */
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * Parameter blocks. Fill them with mppi_*_defaults() and then edit fields.
+ * ---------------------------------------------------------------------------------------------------------------- */
+typedef struct mppi_dynamics_params {
+  int32_t model;
+  int32_t state_dim;   /* set by mppi_dynamics_defaults */
+  int32_t control_dim; /* set by mppi_dynamics_defaults */
+  int32_t output_dim;  /* set by mppi_dynamics_defaults */
+  /* Dynamics::control_rngs_ / control_deadband_ / zero_control_ (dynamics/dynamics.cuh:130,508-509) */
+  float control_min[MPPI_MAX_CONTROL];
+  float control_max[MPPI_MAX_CONTROL];
+  float control_deadband[MPPI_MAX_CONTROL];
+  float zero_control[MPPI_MAX_CONTROL];
+  /* CartpoleDynamicsParams (dynamics/cartpole/cartpole_dynamics.cuh:29-31,101) */
+  float cart_mass;
+  float pole_mass;
+  float pole_length;
+  float gravity;
+  /* RacerDubinsParams (dynamics/racer_dubins/racer_dubins.cuh:81-103) */
+  float c_t;
+  float c_b;
+  float c_v;
+  float c_0;
+  float steering_constant;
+  float steer_command_angle_scale;
+  float steer_angle_scale;
+  float max_steer_angle;
+  float max_steer_rate;
+  float brake_delay_constant;
+  float max_brake_rate_neg;
+  float max_brake_rate_pos;
+  float wheel_base;
+  int32_t gear_sign;
+  /* Residual MLP (FNNHelper layout, utils/nn_helpers/fnn_helper.cu:130-205): per layer l the weights W_l are
+   * row-major [out][in] followed by the bias [out]. nn_layers = {in, h1, h2, out}; the output is added to the
+   * derivatives of states nn_output_states[0..out). Device pointer to nn_num_params floats. */
+  int32_t nn_num_layers;
+  int32_t nn_layers[MPPI_MAX_NN_LAYERS];
+  int32_t nn_output_states[MPPI_MAX_STATE];
+  int32_t nn_precision; /* MPPI_NN_FP32 or MPPI_NN_BF16 */
+  int32_t nn_num_params;
+  const float* nn_theta;
+} mppi_dynamics_params;
+
+typedef struct mppi_cost_params {
+  int32_t cost;
+  int32_t output_dim;
+  /* Quadratic state term sum_i coeff[i] * (y[i] - goal[i])^2. For MPPI_COST_CARTPOLE_QUADRATIC the coefficients are
+   * (cart_position, cart_velocity, pole_angle, pole_angular_velocity) and goal is desired_terminal_state
+   * (cost_functions/cartpole/cartpole_quadratic_cost.cuh:10-23). */
+  float coeff[MPPI_MAX_OUTPUT];
+  float goal[MPPI_MAX_OUTPUT];
+  /* terminal cost = terminal_cost_coeff * quadratic state term (cartpole) ; 0 -> no terminal cost */
+  float terminal_cost_coeff;
+  /* Cost-map term map_coeff * bilinear(map, y[map_x_index], y[map_y_index]) with the reference's CPU bilinear
+   * semantics (utils/texture_helpers/two_d_texture_helper.cu:151-245, texture_helper.cu:94-134).
+   * World -> texel: q = ((p - origin) / resolution) / extent * extent - 0.5, clamped. map == NULL disables it. */
+  const float* map; /* device pointer [map_height][map_width] */
+  int32_t map_width;
+  int32_t map_height;
+  float map_origin_x;
+  float map_origin_y;
+  float map_resolution;
+  float map_coeff;
+  int32_t map_x_index;
+  int32_t map_y_index;
+  /* Speed term speed_coeff * (y[speed_index] - speed_target)^2 (cost_functions/autorally/ar_standard_cost.cu:284-297) */
+  float speed_coeff;
+  float speed_target;
+  int32_t speed_index;
+  int32_t _pad;
+} mppi_cost_params;
+
+typedef struct mppi_sampler_params {
+  int32_t kind;
+  int32_t control_dim;
+  /* GaussianParamsImpl (sampling_distributions/gaussian/gaussian.cuh:21-42) */
+  float std_dev[MPPI_MAX_CONTROL];
+  float control_cost_coeff[MPPI_MAX_CONTROL];
+  float pure_noise_trajectories_percentage;
+  float std_dev_decay;
+  /* ColoredNoiseParamsImpl (sampling_distributions/colored_noise/colored_noise.cuh:44-73) */
+  float exponents[MPPI_MAX_CONTROL];
+  float offset_decay_rate;
+  float fmin;
+  int32_t fft_len_mode; /* MPPI_FFT_LEN_2T (reference) or MPPI_FFT_LEN_T */
+  int32_t _pad;
+  /* Counter-based RNG (Philox4x32-10): normals are a pure function of (seed, counter, element index). */
+  uint64_t seed;
+} mppi_sampler_params;
+
+/* Per-instance result of the weight transform (device-resident, 32 bytes). */
+typedef struct mppi_weight_stats {
+  float baseline;   /* min_k cost_k (core/mppi_common.cu:938-992) */
+  int32_t argmin;   /* first strict minimum (core/mppi_common.cu:921-936) */
+  double normalizer; /* sum_k exp(-(cost_k - baseline)/lambda), accumulated in FP64 */
+  double sum_w2;     /* sum_k w_k^2, FP64 (free-energy variance) */
+  float lambda;
+  int32_t num_rollouts;
+} mppi_weight_stats;
+
+/* Free-energy statistics (core/mppi_common.cu:1101-1117, controllers/controller.cuh:22-30). */
+typedef struct mppi_free_energy {
+  float free_energy_mean;
+  float free_energy_variance;
+  float free_energy_modified_variance;
+  float normalizer_percent;
+  float increase;
+  float previous_baseline;
+} mppi_free_energy;
+
+const char* mppi_last_error(void);
+const char* mppi_status_string(int status);
+/* Version of this library, e.g. "0.1.0". */
+const char* mppi_version(void);
+
+int mppi_dynamics_defaults(int32_t model, mppi_dynamics_params* out);
+int mppi_cost_defaults(int32_t cost, const mppi_dynamics_params* dyn, mppi_cost_params* out);
+int mppi_sampler_defaults(int32_t kind, int32_t control_dim, mppi_sampler_params* out);
+
+/* Number of floats of an FNN parameter vector for the given layer sizes ((in+1)*out summed over layers). */
+int mppi_nn_num_params(int32_t num_layers, const int32_t* layers, int32_t* out_num_params);
+
+/* First sample index that is drawn as zero-mean "pure noise": the smallest k with (float)k >= (1 - p) * K evaluated in
+ * FP32, as in sampling_distributions/gaussian/gaussian.cu:96 and :658. */
+int mppi_pure_noise_start(int32_t num_rollouts, float pure_noise_percentage, int32_t* out_index);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * Kernel-level entry points (device pointers, caller stream).
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* Standard normals eps[i] = N(0,1) for i in [0, n): Philox4x32-10 keyed by `seed`, counter (i/4, offset), Box-Muller.
+ * Replaces curandGenerateNormal (sampling_distributions/gaussian/gaussian.cu:380-395). */
+int mppi_generate_normals(uint64_t seed, uint64_t offset, float* out_d, int64_t n, void* stream);
+
+/* Gaussian control samples u[k][t][c] (sampling_distributions/gaussian/gaussian.cu:17-277,375-431):
+ *   u = mean[t][c]                          if k == 0 or t < optimization_stride
+ *   u = s * eps                             if k >= pure_noise_start
+ *   u = mean[t][c] + s * eps                otherwise,     s = std_dev[c] * std_dev_decay^iteration_num
+ * with eps drawn as by mppi_generate_normals(seed, rng_offset, ...) over the K*T*C buffer.
+ * If eps_in_d is non-NULL it is used instead of the generator (a fed noise buffer). */
+int mppi_sample_gaussian(const mppi_sampler_params* sampler, int32_t num_rollouts, int32_t num_timesteps,
+                         const float* mean_d, float* samples_d, int32_t optimization_stride, int32_t iteration_num,
+                         uint64_t rng_offset, const float* eps_in_d, void* stream);
+
+/* Colored (power-law, 1/f^beta per control dimension) samples
+ * (sampling_distributions/colored_noise/colored_noise.cu:286-392): complex Gaussian spectra scaled by f^(-beta/2),
+ * inverse real DFT of length n (2T or T), first T outputs, offset-decay subtraction x[t] - x[stride]*decay^t,
+ * division by sigma_c * n, then the Gaussian transform above. If spectra_in_d is non-NULL it holds the unscaled
+ * complex spectra [K][C][n/2+1][2] used instead of the generator. */
+int mppi_sample_colored(const mppi_sampler_params* sampler, int32_t num_rollouts, int32_t num_timesteps,
+                        const float* mean_d, float* samples_d, int32_t optimization_stride, int32_t iteration_num,
+                        uint64_t rng_offset, const float* spectra_in_d, void* stream);
+
+/* Fused rollout (core/mppi_common.cu:35-153): per sample, for t < T: read u, enforce constraints, write the clamped u
+ * back in place, step the dynamics, add running cost + likelihood-ratio cost; add the terminal cost; divide by T.
+ * mean_d is the [T][C] mean the samples were drawn around (used by the likelihood-ratio control cost). */
+int mppi_rollout(const mppi_dynamics_params* dyn, const mppi_cost_params* cost, const mppi_sampler_params* sampler,
+                 int32_t num_rollouts, int32_t num_timesteps, float dt, float lambda, float alpha, const float* x0_d,
+                 const float* mean_d, float* samples_d, float* costs_d, void* stream);
+
+/* Weight transform (core/mppi_common.cu:1067-1089,1483-1506): baseline = min cost (first strict minimum), costs_d is
+ * overwritten in place with w_k = exp(-(cost_k - baseline)/lambda), the FP64 normalizer and sum of squares go to
+ * stats_d. */
+int mppi_compute_weights(float* costs_d, int32_t num_rollouts, float lambda, mppi_weight_stats* stats_d, void* stream);
+
+/* Weighted control update (core/mppi_common.cu:746-773,1151-1196): mean_out[t][c] = sum_k (w_k / normalizer) *
+ * samples[k][t][c], with the normalizer read from stats_d on the device. */
+int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_d, const float* samples_d,
+                         int32_t num_rollouts, int32_t num_timesteps, int32_t control_dim, float* mean_out_d,
+                         void* stream);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * Controller object (controllers/MPPI/mppi_controller.cu:152-237, controllers/controller.cuh:70-1058).
+ * ---------------------------------------------------------------------------------------------------------------- */
+typedef struct mppi_controller mppi_controller;
+
+typedef struct mppi_controller_desc {
+  mppi_dynamics_params dynamics;
+  mppi_cost_params cost;
+  mppi_sampler_params sampler;
+  int32_t num_rollouts;
+  int32_t num_timesteps;
+  float dt;
+  float lambda;
+  float alpha;
+  int32_t num_iters;
+  int32_t use_graph; /* 1: capture one iteration into a hipGraph on first use and replay it */
+  int32_t _pad;
+  float slide_control_scale[MPPI_MAX_CONTROL]; /* ControllerParams::slide_control_scale_ */
+  const float* init_control; /* host [T][C] initial nominal sequence or NULL for zeros */
+  const float* cost_map_host; /* optional host [map_height][map_width]; copied into controller-owned memory and
+                                 used in place of cost.map */
+  const float* nn_theta_host; /* optional host FNN parameters, copied like cost_map_host */
+} mppi_controller_desc;
+
+int mppi_controller_create(const mppi_controller_desc* desc, mppi_controller** out);
+int mppi_controller_destroy(mppi_controller* ctrl);
+
+/* One MPC solve: H2D x0, num_iters x (sample, rollout, weights, update), D2H of the new mean, Savitzky-Golay smoothing
+ * keeping u[0] (controllers/controller.cuh:573-607), host state-trajectory propagation (:664-684) and a control clamp
+ * (controllers/MPPI/mppi_controller.cu:224-227). */
+int mppi_controller_compute_control(mppi_controller* ctrl, const float* x0_host, int32_t optimization_stride);
+/* Enqueue `num_calls` device-resident repetitions of the GPU work of one computeControl call (num_iters x sample,
+ * rollout, weights, update) on the controller's stream, replayed from a captured hipGraph: x0 and the nominal
+ * sequence are the ones already in device memory, each call warm-starts from the previous mean, and the generator
+ * counter advances on the device. No host transfer and no synchronisation; for tight loops and throughput timing. */
+int mppi_controller_step_device(mppi_controller* ctrl, int32_t num_calls);
+/* getControlSeq(): [T][C] smoothed control sequence. */
+int mppi_controller_get_control_seq(const mppi_controller* ctrl, float* out_host);
+/* Raw (pre-smoothing) weighted mean of the last iteration [T][C]. */
+int mppi_controller_get_raw_mean(const mppi_controller* ctrl, float* out_host);
+/* getTargetStateSeq(): [T][S] propagated state trajectory of the returned controls. */
+int mppi_controller_get_state_seq(const mppi_controller* ctrl, float* out_host);
+int mppi_controller_get_weight_stats(const mppi_controller* ctrl, mppi_weight_stats* out_host);
+int mppi_controller_get_free_energy(const mppi_controller* ctrl, mppi_free_energy* out_host);
+/* updateImportanceSampler / setControlSeq: set the nominal [T][C] sequence. */
+int mppi_controller_set_control_seq(mppi_controller* ctrl, const float* u_host);
+/* slideControlSequence(steps) (controllers/controller.cuh:609-621). */
+int mppi_controller_slide_control_seq(mppi_controller* ctrl, int32_t steps);
+/* Device pointers to the controller's buffers (samples [K][T][C], costs/weights [K]) for inspection. */
+int mppi_controller_device_buffers(const mppi_controller* ctrl, float** samples_d, float** costs_d, float** mean_d);
+/* Reseed the generator and reset its counter (controllers/controller.cu:203-209). */
+int mppi_controller_set_seed(mppi_controller* ctrl, uint64_t seed);
+/* The stream all controller work runs on (hipStream_t as void*). */
+int mppi_controller_stream(const mppi_controller* ctrl, void** stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* MPPI_AMD_H */

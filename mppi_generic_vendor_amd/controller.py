@@ -1,0 +1,125 @@
+"""Python host wrapper of the C-ABI controller, with the reference controller's method names
+(controllers/controller.cuh:228-876, controllers/MPPI/mppi_controller.cuh:36-41)."""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import capi
+from .workloads import Problem
+
+
+class VanillaMPPIController:
+    def __init__(self, problem: Problem, num_iters: int = 1, use_graph: bool = True,
+                 init_control: np.ndarray | None = None, slide_control_scale=None):
+        lib = capi.load()
+        self.problem = problem
+        self.T = problem.num_timesteps
+        self.K = problem.num_rollouts
+        self.S = problem.S
+        self.C = problem.C
+        d = capi.ControllerDesc()
+        d.dynamics = problem.dyn
+        d.cost = problem.cost
+        d.sampler = problem.sampler
+        d.num_rollouts = self.K
+        d.num_timesteps = self.T
+        d.dt = problem.dt
+        d.lambda_ = problem.lambda_
+        d.alpha = problem.alpha
+        d.num_iters = num_iters
+        d.use_graph = 1 if use_graph else 0
+        if slide_control_scale is not None:
+            for i, v in enumerate(slide_control_scale):
+                d.slide_control_scale[i] = v
+        self._keep = []
+        if init_control is not None:
+            ic = np.ascontiguousarray(init_control, dtype=np.float32).reshape(self.T, self.C)
+            self._keep.append(ic)
+            d.init_control = ic.ctypes.data
+        if problem.cost_map is not None:
+            m = np.ascontiguousarray(problem.cost_map, dtype=np.float32)
+            self._keep.append(m)
+            d.cost_map_host = m.ctypes.data
+        if problem.nn_theta is not None:
+            th = np.ascontiguousarray(problem.nn_theta, dtype=np.float32)
+            self._keep.append(th)
+            d.nn_theta_host = th.ctypes.data
+        h = ct.c_void_p()
+        capi.call("mppi_controller_create", ct.byref(d), ct.byref(h))
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.mppi_controller_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --------------------------------------------------------------------------------------------- reference API
+    def computeControl(self, state: np.ndarray, optimization_stride: int = 1):
+        x = np.ascontiguousarray(state, dtype=np.float32)
+        capi.call("mppi_controller_compute_control", self._h, x.ctypes.data, int(optimization_stride))
+
+    def stepDevice(self, num_calls: int):
+        """Enqueue device-resident iterations (no host transfer, no synchronisation)."""
+        capi.call("mppi_controller_step_device", self._h, int(num_calls))
+
+    def getControlSeq(self) -> np.ndarray:
+        out = np.empty((self.T, self.C), dtype=np.float32)
+        capi.call("mppi_controller_get_control_seq", self._h, out.ctypes.data)
+        return out
+
+    def getRawMean(self) -> np.ndarray:
+        out = np.empty((self.T, self.C), dtype=np.float32)
+        capi.call("mppi_controller_get_raw_mean", self._h, out.ctypes.data)
+        return out
+
+    def getTargetStateSeq(self) -> np.ndarray:
+        out = np.empty((self.T, self.S), dtype=np.float32)
+        capi.call("mppi_controller_get_state_seq", self._h, out.ctypes.data)
+        return out
+
+    def getWeightStats(self) -> capi.WeightStats:
+        st = capi.WeightStats()
+        capi.call("mppi_controller_get_weight_stats", self._h, ct.byref(st))
+        return st
+
+    def getBaselineCost(self) -> float:
+        return self.getWeightStats().baseline
+
+    def getNormalizerCost(self) -> float:
+        return self.getWeightStats().normalizer
+
+    def getFreeEnergyStatistics(self) -> capi.FreeEnergy:
+        fe = capi.FreeEnergy()
+        capi.call("mppi_controller_get_free_energy", self._h, ct.byref(fe))
+        return fe
+
+    def updateImportanceSampler(self, u: np.ndarray):
+        arr = np.ascontiguousarray(u, dtype=np.float32).reshape(self.T, self.C)
+        capi.call("mppi_controller_set_control_seq", self._h, arr.ctypes.data)
+
+    setControlSeq = updateImportanceSampler
+
+    def slideControlSequence(self, steps: int):
+        capi.call("mppi_controller_slide_control_seq", self._h, int(steps))
+
+    def setSeedCUDARandomNumberGen(self, seed: int):
+        capi.call("mppi_controller_set_seed", self._h, ct.c_uint64(seed))
+
+    def device_buffers(self):
+        s, c, m = ct.c_void_p(), ct.c_void_p(), ct.c_void_p()
+        capi.call("mppi_controller_device_buffers", self._h, ct.byref(s), ct.byref(c), ct.byref(m))
+        return s.value, c.value, m.value
+
+    def stream(self):
+        s = ct.c_void_p()
+        capi.call("mppi_controller_stream", self._h, ct.byref(s))
+        return s.value

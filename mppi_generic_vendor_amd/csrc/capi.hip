@@ -1,0 +1,768 @@
+// C-ABI of the engine: parameter defaults and validation, kernel-level entry points and the controller object that
+// runs whole MPPI iterations (VanillaMPPIController::computeControl, controllers/MPPI/mppi_controller.cu:152-237).
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "models.hpp"
+
+namespace mppi_amd {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int status, const std::string& msg)
+{
+  set_last_error(msg);
+  return status;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+  set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+  return MPPI_E_HIP;
+}
+
+int pure_noise_start(int K, float pct)
+{
+  const float thr = (1.0f - pct) * (float)K;
+  // smallest k with (float)k >= thr, found without a K-long scan
+  int k = (int)std::ceil((double)thr);
+  if (k < 0) k = 0;
+  while (k > 0 && (float)(k - 1) >= thr) --k;
+  while (k < K && !((float)k >= thr)) ++k;
+  return std::min(k, K);
+}
+
+static int model_dims(int model, int* S, int* C, int* O)
+{
+  switch (model) {
+    case MPPI_MODEL_CARTPOLE: *S = 4; *C = 1; *O = 4; return MPPI_OK;
+    case MPPI_MODEL_DOUBLE_INTEGRATOR: *S = 4; *C = 2; *O = 4; return MPPI_OK;
+    case MPPI_MODEL_RACER_DUBINS:
+    case MPPI_MODEL_RACER_DUBINS_NN: *S = 7; *C = 2; *O = 7; return MPPI_OK;
+    default: return fail(MPPI_E_UNSUPPORTED, "unknown dynamics model id " + std::to_string(model));
+  }
+}
+
+static int validate_dynamics(const mppi_dynamics_params& d)
+{
+  int S, C, O;
+  MPPI_CHECK(model_dims(d.model, &S, &C, &O));
+  if (d.state_dim != S || d.control_dim != C || d.output_dim != O)
+    return fail(MPPI_E_INVALID, "dynamics params: state/control/output dims do not match the model");
+  for (int i = 0; i < C; ++i)
+    if (!(d.control_min[i] <= d.control_max[i])) return fail(MPPI_E_INVALID, "dynamics params: control_min > control_max");
+  if (d.model == MPPI_MODEL_RACER_DUBINS_NN) {
+    const int32_t want[4] = {NN_IN, NN_H1, NN_H2, NN_OUT};
+    if (d.nn_num_layers != 4 || std::memcmp(d.nn_layers, want, sizeof(want)) != 0)
+      return fail(MPPI_E_UNSUPPORTED, "residual MLP: only the 9-64-64-4 shape is compiled into the kernel library");
+    if (d.nn_num_params != NN_PARAMS) return fail(MPPI_E_INVALID, "residual MLP: nn_num_params mismatch");
+    for (int i = 0; i < NN_OUT; ++i)
+      if (d.nn_output_states[i] < 0 || d.nn_output_states[i] >= S)
+        return fail(MPPI_E_INVALID, "residual MLP: nn_output_states out of range");
+    if (d.nn_precision != MPPI_NN_FP32 && d.nn_precision != MPPI_NN_BF16)
+      return fail(MPPI_E_INVALID, "residual MLP: unknown precision");
+  }
+  return MPPI_OK;
+}
+
+static int validate_cost(const mppi_cost_params& c, const mppi_dynamics_params& d, bool need_map_ptr)
+{
+  if (c.cost != MPPI_COST_CARTPOLE_QUADRATIC && c.cost != MPPI_COST_QUADRATIC_MAP)
+    return fail(MPPI_E_UNSUPPORTED, "unknown cost id " + std::to_string(c.cost));
+  if (c.output_dim != d.output_dim) return fail(MPPI_E_INVALID, "cost params: output_dim does not match the model");
+  if (c.cost == MPPI_COST_QUADRATIC_MAP) {
+    if (c.map != nullptr || !need_map_ptr) {
+      if (c.map_coeff != 0.0f && (c.map != nullptr || !need_map_ptr)) {
+        if (c.map_width < 2 || c.map_height < 2) return fail(MPPI_E_INVALID, "cost map: width/height must be >= 2");
+        if (!(c.map_resolution > 0.0f)) return fail(MPPI_E_INVALID, "cost map: resolution must be > 0");
+        if (c.map_x_index < 0 || c.map_x_index >= c.output_dim || c.map_y_index < 0 || c.map_y_index >= c.output_dim)
+          return fail(MPPI_E_INVALID, "cost map: position indices out of range");
+      }
+    }
+    if (c.speed_coeff != 0.0f && (c.speed_index < 0 || c.speed_index >= c.output_dim))
+      return fail(MPPI_E_INVALID, "speed term: speed_index out of range");
+  }
+  return MPPI_OK;
+}
+
+static int validate_sampler(const mppi_sampler_params& s, const mppi_dynamics_params& d)
+{
+  if (s.kind != MPPI_SAMPLER_GAUSSIAN && s.kind != MPPI_SAMPLER_COLORED)
+    return fail(MPPI_E_UNSUPPORTED, "unknown sampler kind " + std::to_string(s.kind));
+  if (s.control_dim != d.control_dim) return fail(MPPI_E_INVALID, "sampler params: control_dim does not match the model");
+  for (int i = 0; i < s.control_dim; ++i)
+    if (!(s.std_dev[i] > 0.0f)) return fail(MPPI_E_INVALID, "sampler params: std_dev must be > 0");
+  if (s.kind == MPPI_SAMPLER_COLORED) {
+    if (s.fft_len_mode != MPPI_FFT_LEN_2T && s.fft_len_mode != MPPI_FFT_LEN_T)
+      return fail(MPPI_E_INVALID, "colored sampler: unknown fft_len_mode");
+    if (!(s.fmin >= 0.0f && s.fmin <= 0.5f)) return fail(MPPI_E_INVALID, "colored sampler: fmin must be in [0, 0.5]");
+  }
+  return MPPI_OK;
+}
+
+static int validate_sizes(int K, int T)
+{
+  if (K < 1) return fail(MPPI_E_INVALID, "num_rollouts must be >= 1");
+  if (T < 1) return fail(MPPI_E_INVALID, "num_timesteps must be >= 1");
+  if (T > MPPI_MAX_TIMESTEPS) return fail(MPPI_E_SIZE, "num_timesteps exceeds MPPI_MAX_TIMESTEPS");
+  if ((int64_t)K * T * MPPI_MAX_CONTROL > ((int64_t)1 << 31)) return fail(MPPI_E_SIZE, "sample buffer too large");
+  return MPPI_OK;
+}
+
+// ------------------------------------------------------------------------------------ host post-processing
+// Savitzky-Golay [-3, 12, 17, 12, -3] / 35 with edge replication, u[0] kept (controllers/controller.cuh:573-607).
+static void smooth_control(std::vector<float>& u, int T, int C)
+{
+  const float coef[5] = {-3.0f / 35.0f, 12.0f / 35.0f, 17.0f / 35.0f, 12.0f / 35.0f, -3.0f / 35.0f};
+  std::vector<float> buf((size_t)(T + 4) * C);
+  for (int c = 0; c < C; ++c) {
+    buf[c] = u[c];
+    buf[C + c] = u[c];
+    for (int t = 0; t < T; ++t) buf[(t + 2) * C + c] = u[t * C + c];
+    buf[(T + 2) * C + c] = u[(T - 1) * C + c];
+    buf[(T + 3) * C + c] = u[(T - 1) * C + c];
+  }
+  for (int i = 1; i < T; ++i)
+    for (int c = 0; c < C; ++c) {
+      float s = 0.0f;
+      for (int j = 0; j < 5; ++j) s += coef[j] * buf[(i + j) * C + c];
+      u[i * C + c] = s;
+    }
+}
+
+template <int MODEL>
+static void propagate_states(const mppi_dynamics_params& p, const float* x0, const float* u, int T, float dt,
+                             float* states)
+{
+  using M = Model<MODEL>;
+  constexpr int S = M::S, C = M::C;
+  for (int i = 0; i < S; ++i) states[i] = x0[i];
+  for (int t = 0; t + 1 < T; ++t) {
+    float uc[C], xd[S];
+    for (int c = 0; c < C; ++c) uc[c] = u[t * C + c];
+    enforce_constraints<C>(p, uc);
+    M::deriv(p, states + t * S, uc, xd);
+    M::update(p, states + t * S, states + (t + 1) * S, xd, dt);
+  }
+}
+
+// computeOutputTrajectoryHelper (controllers/controller.cuh:664-684) on the host.
+static void propagate(const mppi_dynamics_params& p, const float* x0, const float* u, int T, float dt, float* states)
+{
+  switch (p.model) {
+    case MPPI_MODEL_CARTPOLE: propagate_states<MPPI_MODEL_CARTPOLE>(p, x0, u, T, dt, states); break;
+    case MPPI_MODEL_DOUBLE_INTEGRATOR: propagate_states<MPPI_MODEL_DOUBLE_INTEGRATOR>(p, x0, u, T, dt, states); break;
+    case MPPI_MODEL_RACER_DUBINS: propagate_states<MPPI_MODEL_RACER_DUBINS>(p, x0, u, T, dt, states); break;
+    case MPPI_MODEL_RACER_DUBINS_NN: propagate_states<MPPI_MODEL_RACER_DUBINS_NN>(p, x0, u, T, dt, states); break;
+    default: break;
+  }
+}
+
+template <int C>
+static void clamp_sequence(const mppi_dynamics_params& p, float* u, int T)
+{
+  for (int t = 0; t < T; ++t) enforce_constraints<C>(p, u + t * C);
+}
+
+template <class Tp>
+static int dev_alloc(Tp** p, size_t bytes)
+{
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) return fail(MPPI_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return MPPI_OK;
+}
+
+template <class Tp>
+static int host_alloc(Tp** p, size_t bytes)
+{
+  hipError_t e = hipHostMalloc((void**)p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return fail(MPPI_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  std::memset(*p, 0, bytes);
+  return MPPI_OK;
+}
+
+}  // namespace mppi_amd
+
+using namespace mppi_amd;
+
+// ======================================================================================================= C-ABI
+struct mppi_controller {
+  mppi_controller_desc d{};
+  int S = 0, C = 0, O = 0, K = 0, T = 0;
+  hipStream_t stream = nullptr;
+  float* x0_d = nullptr;
+  float* mean_d = nullptr;
+  float* samples_d = nullptr;
+  float* costs_d = nullptr;
+  float* mean_out_d = nullptr;
+  mppi_weight_stats* stats_d = nullptr;
+  void* ws_d = nullptr;
+  IterArgs* iter_d = nullptr;
+  float* map_d = nullptr;
+  float* nn_d = nullptr;
+  IterArgs* iter_h = nullptr;
+  float* x0_h = nullptr;
+  float* nominal_h = nullptr;
+  float* mean_out_h = nullptr;
+  mppi_weight_stats* stats_h = nullptr;
+  std::vector<float> control, raw_mean, state_seq, nn_host;
+  mppi_weight_stats stats{};
+  mppi_free_energy fe{};
+  uint64_t rng_counter = 0;
+  int eager_runs = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  IterArgs* loop_iter_d = nullptr;  // free-running iteration arguments (rng offset advanced on the device)
+  hipGraph_t loop_graph = nullptr;
+  hipGraphExec_t loop_exec = nullptr;
+};
+
+extern "C" {
+
+const char* mppi_last_error(void) { return g_last_error.c_str(); }
+
+const char* mppi_status_string(int status)
+{
+  switch (status) {
+    case MPPI_OK: return "ok";
+    case MPPI_E_INVALID: return "invalid argument";
+    case MPPI_E_HIP: return "HIP runtime error";
+    case MPPI_E_UNSUPPORTED: return "unsupported combination";
+    case MPPI_E_NOMEM: return "out of device memory";
+    case MPPI_E_SIZE: return "size exceeds a compiled limit";
+    default: return "unknown status";
+  }
+}
+
+const char* mppi_version(void) { return "0.1.0"; }
+
+int mppi_dynamics_defaults(int32_t model, mppi_dynamics_params* out)
+{
+  if (!out) return fail(MPPI_E_INVALID, "null output");
+  mppi_dynamics_params d;
+  std::memset(&d, 0, sizeof(d));
+  d.model = model;
+  MPPI_CHECK(model_dims(model, &d.state_dim, &d.control_dim, &d.output_dim));
+  for (int i = 0; i < MPPI_MAX_CONTROL; ++i) {
+    d.control_min[i] = -FLT_MAX;  // Dynamics::setDefaultControlRanges (dynamics/dynamics.cu:20-27)
+    d.control_max[i] = FLT_MAX;
+  }
+  d.cart_mass = 1.0f;
+  d.pole_mass = 1.0f;
+  d.pole_length = 1.0f;
+  d.gravity = 9.81f;
+  // RacerDubinsParams defaults (dynamics/racer_dubins/racer_dubins.cuh:81-103)
+  d.c_t = 1.3f;
+  d.c_b = 2.5f;
+  d.c_v = 3.7f;
+  d.c_0 = 4.9f;
+  d.steering_constant = 0.6f;
+  d.steer_command_angle_scale = 5.0f;
+  d.steer_angle_scale = -9.1f;
+  d.max_steer_angle = 0.5f;
+  d.max_steer_rate = 5.0f;
+  d.brake_delay_constant = 6.6f;
+  d.max_brake_rate_neg = 0.9f;
+  d.max_brake_rate_pos = 0.33f;
+  d.wheel_base = 0.3f;
+  d.gear_sign = 1;
+  if (model == MPPI_MODEL_RACER_DUBINS || model == MPPI_MODEL_RACER_DUBINS_NN) {
+    // throttle/brake and steering commands in [-1, 1]; the brake state is then clamped to [0, 1] on host and device
+    for (int i = 0; i < 2; ++i) {
+      d.control_min[i] = -1.0f;
+      d.control_max[i] = 1.0f;
+    }
+  }
+  if (model == MPPI_MODEL_RACER_DUBINS_NN) {
+    d.nn_num_layers = 4;
+    d.nn_layers[0] = NN_IN;
+    d.nn_layers[1] = NN_H1;
+    d.nn_layers[2] = NN_H2;
+    d.nn_layers[3] = NN_OUT;
+    // residual on VEL_X, YAW, STEER_ANGLE, BRAKE_STATE
+    d.nn_output_states[0] = 0;
+    d.nn_output_states[1] = 1;
+    d.nn_output_states[2] = 4;
+    d.nn_output_states[3] = 5;
+    d.nn_precision = MPPI_NN_FP32;
+    d.nn_num_params = NN_PARAMS;
+  }
+  *out = d;
+  return MPPI_OK;
+}
+
+int mppi_cost_defaults(int32_t cost, const mppi_dynamics_params* dyn, mppi_cost_params* out)
+{
+  if (!out || !dyn) return fail(MPPI_E_INVALID, "null argument");
+  mppi_cost_params c;
+  std::memset(&c, 0, sizeof(c));
+  c.cost = cost;
+  c.output_dim = dyn->output_dim;
+  if (cost == MPPI_COST_CARTPOLE_QUADRATIC) {
+    // CartpoleQuadraticCostParams (cost_functions/cartpole/cartpole_quadratic_cost.cuh:10-23)
+    c.coeff[0] = 1000.0f;
+    c.coeff[1] = 100.0f;
+    c.coeff[2] = 2000.0f;
+    c.coeff[3] = 100.0f;
+    c.goal[2] = 3.14159265358979323846f;
+  } else if (cost == MPPI_COST_QUADRATIC_MAP) {
+    for (int i = 0; i < MPPI_MAX_OUTPUT; ++i) c.coeff[i] = (i < c.output_dim) ? 1.0f : 0.0f;
+    c.map_resolution = 1.0f;
+    c.map_coeff = 1.0f;
+    const bool bicycle = dyn->model == MPPI_MODEL_RACER_DUBINS || dyn->model == MPPI_MODEL_RACER_DUBINS_NN;
+    c.map_x_index = bicycle ? 2 : 0;
+    c.map_y_index = bicycle ? 3 : 1;
+    c.speed_index = 0;
+  } else {
+    return fail(MPPI_E_UNSUPPORTED, "unknown cost id");
+  }
+  *out = c;
+  return MPPI_OK;
+}
+
+int mppi_sampler_defaults(int32_t kind, int32_t control_dim, mppi_sampler_params* out)
+{
+  if (!out) return fail(MPPI_E_INVALID, "null output");
+  if (control_dim < 1 || control_dim > MPPI_MAX_CONTROL) return fail(MPPI_E_INVALID, "control_dim out of range");
+  mppi_sampler_params s;
+  std::memset(&s, 0, sizeof(s));
+  s.kind = kind;
+  s.control_dim = control_dim;
+  for (int i = 0; i < MPPI_MAX_CONTROL; ++i) s.std_dev[i] = 1.0f;
+  s.pure_noise_trajectories_percentage = 0.01f;
+  s.std_dev_decay = 1.0f;
+  s.offset_decay_rate = 0.97f;
+  s.fmin = 0.0f;
+  s.fft_len_mode = MPPI_FFT_LEN_2T;
+  *out = s;
+  return MPPI_OK;
+}
+
+int mppi_nn_num_params(int32_t num_layers, const int32_t* layers, int32_t* out_num_params)
+{
+  if (!layers || !out_num_params || num_layers < 2) return fail(MPPI_E_INVALID, "bad layer description");
+  int32_t n = 0;
+  for (int i = 1; i < num_layers; ++i) n += (layers[i - 1] + 1) * layers[i];
+  *out_num_params = n;
+  return MPPI_OK;
+}
+
+int mppi_pure_noise_start(int32_t num_rollouts, float pct, int32_t* out_index)
+{
+  if (!out_index || num_rollouts < 1) return fail(MPPI_E_INVALID, "bad argument");
+  *out_index = pure_noise_start(num_rollouts, pct);
+  return MPPI_OK;
+}
+
+int mppi_generate_normals(uint64_t seed, uint64_t offset, float* out_d, int64_t n, void* stream)
+{
+  if (n < 0 || (n > 0 && !out_d)) return fail(MPPI_E_INVALID, "bad normals buffer");
+  return launch_generate_normals(seed, offset, out_d, n, (hipStream_t)stream);
+}
+
+int mppi_sample_gaussian(const mppi_sampler_params* sampler, int32_t K, int32_t T, const float* mean_d,
+                         float* samples_d, int32_t stride, int32_t iteration, uint64_t rng_offset,
+                         const float* eps_in_d, void* stream)
+{
+  if (!sampler || !mean_d || !samples_d) return fail(MPPI_E_INVALID, "null argument");
+  MPPI_CHECK(validate_sizes(K, T));
+  if (sampler->control_dim < 1 || sampler->control_dim > MPPI_MAX_CONTROL)
+    return fail(MPPI_E_INVALID, "sampler control_dim out of range");
+  if (stride < 0) return fail(MPPI_E_INVALID, "optimization_stride must be >= 0");
+  IterRef it{nullptr, {rng_offset, stride, iteration}};
+  return launch_sample_gaussian(*sampler, K, T, sampler->control_dim, mean_d, samples_d, it, eps_in_d,
+                                (hipStream_t)stream);
+}
+
+int mppi_sample_colored(const mppi_sampler_params* sampler, int32_t K, int32_t T, const float* mean_d,
+                        float* samples_d, int32_t stride, int32_t iteration, uint64_t rng_offset,
+                        const float* spectra_in_d, void* stream)
+{
+  if (!sampler || !mean_d || !samples_d) return fail(MPPI_E_INVALID, "null argument");
+  MPPI_CHECK(validate_sizes(K, T));
+  if (sampler->control_dim < 1 || sampler->control_dim > MPPI_MAX_CONTROL)
+    return fail(MPPI_E_INVALID, "sampler control_dim out of range");
+  if (sampler->fft_len_mode != MPPI_FFT_LEN_2T && sampler->fft_len_mode != MPPI_FFT_LEN_T)
+    return fail(MPPI_E_INVALID, "colored sampler: unknown fft_len_mode");
+  const int n = (sampler->fft_len_mode == MPPI_FFT_LEN_T) ? T : 2 * T;
+  if (n < 2) return fail(MPPI_E_INVALID, "colored sampler: transform length must be >= 2");
+  if (stride < 0 || stride >= n) return fail(MPPI_E_INVALID, "colored sampler: optimization_stride out of range");
+  IterRef it{nullptr, {rng_offset, stride, iteration}};
+  return launch_sample_colored(*sampler, K, T, sampler->control_dim, mean_d, samples_d, it, spectra_in_d,
+                               (hipStream_t)stream);
+}
+
+int mppi_rollout(const mppi_dynamics_params* dyn, const mppi_cost_params* cost, const mppi_sampler_params* sampler,
+                 int32_t K, int32_t T, float dt, float lambda, float alpha, const float* x0_d, const float* mean_d,
+                 float* samples_d, float* costs_d, void* stream)
+{
+  if (!dyn || !cost || !sampler || !x0_d || !mean_d || !samples_d || !costs_d)
+    return fail(MPPI_E_INVALID, "null argument");
+  MPPI_CHECK(validate_sizes(K, T));
+  MPPI_CHECK(validate_dynamics(*dyn));
+  MPPI_CHECK(validate_cost(*cost, *dyn, true));
+  MPPI_CHECK(validate_sampler(*sampler, *dyn));
+  if (dyn->model == MPPI_MODEL_RACER_DUBINS_NN && dyn->nn_theta == nullptr)
+    return fail(MPPI_E_INVALID, "residual MLP: nn_theta is null");
+  if (dyn->model == MPPI_MODEL_RACER_DUBINS_NN && dyn->nn_precision == MPPI_NN_BF16)
+    return fail(MPPI_E_UNSUPPORTED, "residual MLP: BF16 path not available");
+  return launch_rollout(*dyn, *cost, *sampler, K, T, dt, lambda, alpha, x0_d, mean_d, samples_d, costs_d,
+                        (hipStream_t)stream);
+}
+
+int mppi_compute_weights(float* costs_d, int32_t K, float lambda, mppi_weight_stats* stats_d, void* stream)
+{
+  if (!costs_d || !stats_d) return fail(MPPI_E_INVALID, "null argument");
+  if (K < 1) return fail(MPPI_E_INVALID, "num_rollouts must be >= 1");
+  if (!(lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
+  return launch_compute_weights(costs_d, K, lambda, stats_d, (hipStream_t)stream);
+}
+
+int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_d, const float* samples_d, int32_t K,
+                         int32_t T, int32_t C, float* mean_out_d, void* stream)
+{
+  if (!weights_d || !stats_d || !samples_d || !mean_out_d) return fail(MPPI_E_INVALID, "null argument");
+  MPPI_CHECK(validate_sizes(K, T));
+  if (C < 1 || C > MPPI_MAX_CONTROL) return fail(MPPI_E_INVALID, "control_dim out of range");
+  return launch_weighted_update(weights_d, stats_d, samples_d, K, T, C, mean_out_d, (hipStream_t)stream);
+}
+
+// --------------------------------------------------------------------------------------------------- controller
+static int controller_free(mppi_controller* c)
+{
+  if (!c) return MPPI_OK;
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->loop_exec) (void)hipGraphExecDestroy(c->loop_exec);
+  if (c->loop_graph) (void)hipGraphDestroy(c->loop_graph);
+  void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->mean_out_d, c->stats_d, c->ws_d, c->iter_d,
+                 c->map_d, c->nn_d, c->loop_iter_d};
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h};
+  for (void* p : host)
+    if (p) (void)hipHostFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MPPI_OK;
+}
+
+static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc)
+{
+  c->d = *desc;
+  c->K = desc->num_rollouts;
+  c->T = desc->num_timesteps;
+  MPPI_CHECK(validate_sizes(c->K, c->T));
+  MPPI_CHECK(validate_dynamics(desc->dynamics));
+  MPPI_CHECK(validate_sampler(desc->sampler, desc->dynamics));
+  if (!(desc->lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
+  if (desc->num_iters < 1) return fail(MPPI_E_INVALID, "num_iters must be >= 1");
+  if (desc->dynamics.model == MPPI_MODEL_RACER_DUBINS_NN && desc->dynamics.nn_precision == MPPI_NN_BF16)
+    return fail(MPPI_E_UNSUPPORTED, "residual MLP: BF16 path not available");
+  c->S = desc->dynamics.state_dim;
+  c->C = desc->dynamics.control_dim;
+  c->O = desc->dynamics.output_dim;
+  const int K = c->K, T = c->T, C = c->C, S = c->S;
+  MPPI_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  MPPI_CHECK(dev_alloc(&c->x0_d, sizeof(float) * MPPI_MAX_STATE));
+  MPPI_CHECK(dev_alloc(&c->mean_d, sizeof(float) * T * C));
+  MPPI_CHECK(dev_alloc(&c->mean_out_d, sizeof(float) * T * C));
+  MPPI_CHECK(dev_alloc(&c->samples_d, sizeof(float) * (size_t)K * T * C));
+  MPPI_CHECK(dev_alloc(&c->costs_d, sizeof(float) * K));
+  MPPI_CHECK(dev_alloc(&c->stats_d, sizeof(mppi_weight_stats)));
+  const size_t ws = fused_update_workspace_bytes(K, T, C);
+  MPPI_CHECK(dev_alloc(&c->ws_d, ws));
+  MPPI_HIP_CHECK(hipMemset(c->ws_d, 0, ws));
+  MPPI_CHECK(dev_alloc(&c->iter_d, sizeof(IterArgs) * desc->num_iters));
+  MPPI_CHECK(dev_alloc(&c->loop_iter_d, sizeof(IterArgs)));
+  MPPI_CHECK(host_alloc(&c->iter_h, sizeof(IterArgs) * desc->num_iters));
+  MPPI_CHECK(host_alloc(&c->x0_h, sizeof(float) * MPPI_MAX_STATE));
+  MPPI_CHECK(host_alloc(&c->nominal_h, sizeof(float) * T * C));
+  MPPI_CHECK(host_alloc(&c->mean_out_h, sizeof(float) * T * C));
+  MPPI_CHECK(host_alloc(&c->stats_h, sizeof(mppi_weight_stats)));
+  // plugin-owned device copies of the cost map and the MLP parameters
+  if (desc->cost_map_host != nullptr) {
+    const size_t n = (size_t)desc->cost.map_width * desc->cost.map_height;
+    if (n < 4) return fail(MPPI_E_INVALID, "cost map: width/height must be >= 2");
+    MPPI_CHECK(dev_alloc(&c->map_d, sizeof(float) * n));
+    MPPI_HIP_CHECK(hipMemcpy(c->map_d, desc->cost_map_host, sizeof(float) * n, hipMemcpyHostToDevice));
+    c->d.cost.map = c->map_d;
+  }
+  MPPI_CHECK(validate_cost(c->d.cost, c->d.dynamics, true));
+  if (desc->dynamics.model == MPPI_MODEL_RACER_DUBINS_NN) {
+    if (desc->nn_theta_host == nullptr) return fail(MPPI_E_INVALID, "residual MLP: nn_theta_host is required");
+    c->nn_host.assign(desc->nn_theta_host, desc->nn_theta_host + desc->dynamics.nn_num_params);
+    MPPI_CHECK(dev_alloc(&c->nn_d, sizeof(float) * c->nn_host.size()));
+    MPPI_HIP_CHECK(hipMemcpy(c->nn_d, c->nn_host.data(), sizeof(float) * c->nn_host.size(), hipMemcpyHostToDevice));
+    c->d.dynamics.nn_theta = c->nn_d;
+  }
+  c->d.init_control = nullptr;
+  c->d.cost_map_host = nullptr;
+  c->d.nn_theta_host = nullptr;
+  c->control.assign((size_t)T * C, 0.0f);
+  if (desc->init_control) std::copy(desc->init_control, desc->init_control + (size_t)T * C, c->control.begin());
+  c->raw_mean = c->control;
+  c->state_seq.assign((size_t)T * S, 0.0f);
+  c->stats_h->baseline = 0.0f;
+  c->rng_counter = 0;
+  return MPPI_OK;
+}
+
+int mppi_controller_create(const mppi_controller_desc* desc, mppi_controller** out)
+{
+  if (!desc || !out) return fail(MPPI_E_INVALID, "null argument");
+  *out = nullptr;
+  mppi_controller* c = new (std::nothrow) mppi_controller();
+  if (!c) return fail(MPPI_E_NOMEM, "host allocation failed");
+  const int st = controller_setup(c, desc);
+  if (st != MPPI_OK) {
+    const std::string msg = g_last_error;
+    controller_free(c);
+    set_last_error(msg);
+    return st;
+  }
+  *out = c;
+  return MPPI_OK;
+}
+
+int mppi_controller_destroy(mppi_controller* ctrl)
+{
+  if (ctrl && ctrl->stream) (void)hipStreamSynchronize(ctrl->stream);
+  return controller_free(ctrl);
+}
+
+// The work of one computeControl call, enqueued on the controller's stream (captured once when use_graph is set).
+static int controller_enqueue(mppi_controller* c)
+{
+  const int K = c->K, T = c->T, C = c->C, S = c->S;
+  const hipStream_t s = c->stream;
+  const mppi_controller_desc& d = c->d;
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->iter_d, c->iter_h, sizeof(IterArgs) * d.num_iters, hipMemcpyHostToDevice, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->x0_d, c->x0_h, sizeof(float) * S, hipMemcpyHostToDevice, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->nominal_h, sizeof(float) * T * C, hipMemcpyHostToDevice, s));
+  for (int it = 0; it < d.num_iters; ++it) {
+    IterRef ref{c->iter_d + it, {0, 0, 0}};
+    if (d.sampler.kind == MPPI_SAMPLER_COLORED)
+      MPPI_CHECK(launch_sample_colored(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, s));
+    else
+      MPPI_CHECK(launch_sample_gaussian(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, s));
+    MPPI_CHECK(launch_rollout(d.dynamics, d.cost, d.sampler, K, T, d.dt, d.lambda, d.alpha, c->x0_d, c->mean_d,
+                              c->samples_d, c->costs_d, s));
+    MPPI_CHECK(launch_fused_update(c->costs_d, c->samples_d, K, T, C, d.lambda, c->ws_d, c->stats_d, c->mean_out_d, s));
+    if (it + 1 < d.num_iters)
+      MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->mean_out_d, sizeof(float) * T * C, hipMemcpyDeviceToDevice, s));
+  }
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_out_h, c->mean_out_d, sizeof(float) * T * C, hipMemcpyDeviceToHost, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->stats_h, c->stats_d, sizeof(mppi_weight_stats), hipMemcpyDeviceToHost, s));
+  return MPPI_OK;
+}
+
+int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, int32_t stride)
+{
+  if (!c || !x0_host) return fail(MPPI_E_INVALID, "null argument");
+  if (stride < 0 || stride > c->T) return fail(MPPI_E_INVALID, "optimization_stride out of range");
+  const int T = c->T, C = c->C, S = c->S, K = c->K;
+  const mppi_controller_desc& d = c->d;
+  if (d.sampler.kind == MPPI_SAMPLER_COLORED) {
+    const int n = (d.sampler.fft_len_mode == MPPI_FFT_LEN_T) ? T : 2 * T;
+    if (stride >= n) return fail(MPPI_E_INVALID, "colored sampler: optimization_stride out of range");
+  }
+  c->fe.previous_baseline = c->stats.baseline;
+  for (int i = 0; i < S; ++i) c->x0_h[i] = x0_host[i];
+  std::copy(c->control.begin(), c->control.end(), c->nominal_h);
+  for (int it = 0; it < d.num_iters; ++it) {
+    c->iter_h[it].rng_offset = c->rng_counter++;
+    c->iter_h[it].stride = stride;
+    c->iter_h[it].iteration = it;
+  }
+  if (d.use_graph && c->eager_runs > 0) {
+    if (!c->graph_exec) {
+      MPPI_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int st = controller_enqueue(c);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (st != MPPI_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      c->graph = g;
+      MPPI_HIP_CHECK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    }
+    MPPI_HIP_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  } else {
+    MPPI_CHECK(controller_enqueue(c));
+    c->eager_runs++;
+  }
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->stats = *c->stats_h;
+  c->raw_mean.assign(c->mean_out_h, c->mean_out_h + (size_t)T * C);
+  c->control = c->raw_mean;
+  // free-energy statistics from the FP64 device sums (core/mppi_common.cu:1101-1117)
+  const double norm = c->stats.normalizer / K;
+  const double lam = d.lambda;
+  c->fe.free_energy_mean = (float)(-lam * std::log(norm) + c->stats.baseline);
+  c->fe.free_energy_variance = (float)(lam * (c->stats.sum_w2 / K - norm * norm));
+  const double weird = c->fe.free_energy_variance / (norm * std::sqrt((double)K));
+  c->fe.free_energy_modified_variance = (float)(lam * (weird + 0.5 * weird * weird));
+  c->fe.normalizer_percent = (float)norm;
+  c->fe.increase = c->stats.baseline - c->fe.previous_baseline;
+  smooth_control(c->control, T, C);
+  mppi_dynamics_params hostdyn = d.dynamics;
+  if (!c->nn_host.empty()) hostdyn.nn_theta = c->nn_host.data();
+  propagate(hostdyn, c->x0_h, c->control.data(), T, d.dt, c->state_seq.data());
+  if (C == 1) clamp_sequence<1>(d.dynamics, c->control.data(), T);
+  else clamp_sequence<2>(d.dynamics, c->control.data(), T);
+  return MPPI_OK;
+}
+
+// One device-resident iteration sequence: the same kernels as computeControl, reading x0 and the mean already in
+// device memory, warm-starting the next call from the new mean, with the rng offset advanced on the device.
+static int controller_enqueue_loop(mppi_controller* c)
+{
+  const int K = c->K, T = c->T, C = c->C;
+  const hipStream_t s = c->stream;
+  const mppi_controller_desc& d = c->d;
+  for (int it = 0; it < d.num_iters; ++it) {
+    IterRef ref{c->loop_iter_d, {0, 0, 0}};
+    if (d.sampler.kind == MPPI_SAMPLER_COLORED)
+      MPPI_CHECK(launch_sample_colored(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, s));
+    else
+      MPPI_CHECK(launch_sample_gaussian(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, s));
+    MPPI_CHECK(launch_rollout(d.dynamics, d.cost, d.sampler, K, T, d.dt, d.lambda, d.alpha, c->x0_d, c->mean_d,
+                              c->samples_d, c->costs_d, s));
+    MPPI_CHECK(launch_fused_update(c->costs_d, c->samples_d, K, T, C, d.lambda, c->ws_d, c->stats_d, c->mean_out_d, s,
+                                   c->loop_iter_d));
+    MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->mean_out_d, sizeof(float) * T * C, hipMemcpyDeviceToDevice, s));
+  }
+  return MPPI_OK;
+}
+
+int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  if (num_calls < 0) return fail(MPPI_E_INVALID, "num_calls must be >= 0");
+  if (num_calls == 0) return MPPI_OK;
+  if (!c->loop_exec) {
+    // arguments of the free-running loop: x0 and the nominal sequence as last uploaded, stride 1, iteration 0
+    IterArgs a{c->rng_counter + ((uint64_t)1 << 40), 1, 0};
+    MPPI_HIP_CHECK(hipMemcpyAsync(c->loop_iter_d, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+    MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+    MPPI_CHECK(controller_enqueue_loop(c));  // eager pass (sets kernel attributes before capture)
+    MPPI_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int st = controller_enqueue_loop(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st != MPPI_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    c->loop_graph = g;
+    MPPI_HIP_CHECK(hipGraphInstantiate(&c->loop_exec, c->loop_graph, nullptr, nullptr, 0));
+    --num_calls;
+  }
+  for (int i = 0; i < num_calls; ++i) MPPI_HIP_CHECK(hipGraphLaunch(c->loop_exec, c->stream));
+  return MPPI_OK;
+}
+
+int mppi_controller_get_control_seq(const mppi_controller* c, float* out)
+{
+  if (!c || !out) return fail(MPPI_E_INVALID, "null argument");
+  std::copy(c->control.begin(), c->control.end(), out);
+  return MPPI_OK;
+}
+
+int mppi_controller_get_raw_mean(const mppi_controller* c, float* out)
+{
+  if (!c || !out) return fail(MPPI_E_INVALID, "null argument");
+  std::copy(c->raw_mean.begin(), c->raw_mean.end(), out);
+  return MPPI_OK;
+}
+
+int mppi_controller_get_state_seq(const mppi_controller* c, float* out)
+{
+  if (!c || !out) return fail(MPPI_E_INVALID, "null argument");
+  std::copy(c->state_seq.begin(), c->state_seq.end(), out);
+  return MPPI_OK;
+}
+
+int mppi_controller_get_weight_stats(const mppi_controller* c, mppi_weight_stats* out)
+{
+  if (!c || !out) return fail(MPPI_E_INVALID, "null argument");
+  *out = c->stats;
+  return MPPI_OK;
+}
+
+int mppi_controller_get_free_energy(const mppi_controller* c, mppi_free_energy* out)
+{
+  if (!c || !out) return fail(MPPI_E_INVALID, "null argument");
+  *out = c->fe;
+  return MPPI_OK;
+}
+
+int mppi_controller_set_control_seq(mppi_controller* c, const float* u)
+{
+  if (!c || !u) return fail(MPPI_E_INVALID, "null argument");
+  c->control.assign(u, u + (size_t)c->T * c->C);
+  return MPPI_OK;
+}
+
+int mppi_controller_slide_control_seq(mppi_controller* c, int32_t steps)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  if (steps < 0) return fail(MPPI_E_INVALID, "steps must be >= 0");
+  const int T = c->T, C = c->C;
+  const float* zero = c->d.dynamics.zero_control;
+  const float* scale = c->d.slide_control_scale;
+  for (int i = 0; i < T; ++i) {
+    const int ind = std::min(i + steps, T - 1);
+    for (int j = 0; j < C; ++j) {
+      float v = c->control[(size_t)ind * C + j];
+      if (i + steps > T - 1) v = (v - zero[j]) * scale[j] + zero[j];
+      c->control[(size_t)i * C + j] = v;
+    }
+  }
+  return MPPI_OK;
+}
+
+int mppi_controller_device_buffers(const mppi_controller* c, float** samples_d, float** costs_d, float** mean_d)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  if (samples_d) *samples_d = c->samples_d;
+  if (costs_d) *costs_d = c->costs_d;
+  if (mean_d) *mean_d = c->mean_d;
+  return MPPI_OK;
+}
+
+int mppi_controller_set_seed(mppi_controller* c, uint64_t seed)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  c->d.sampler.seed = seed;
+  c->rng_counter = 0;
+  // the seed is a kernel argument of the captured graph: drop the graph so the next call captures it again
+  if (c->graph_exec) {
+    (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+  }
+  if (c->graph) {
+    (void)hipGraphDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return MPPI_OK;
+}
+
+int mppi_controller_stream(const mppi_controller* c, void** stream)
+{
+  if (!c || !stream) return fail(MPPI_E_INVALID, "null argument");
+  *stream = (void*)c->stream;
+  return MPPI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// Shared definitions of the MI355X MPPI engine: error plumbing, the counter-based normal generator and wavefront
+// reductions. Everything here is compiled by hipcc for gfx950 (device) and x86-64 (host).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cfloat>
+#include <cmath>
+#include <string>
+
+#include "../../include/mppi_amd.h"
+
+#define MPPI_HD __host__ __device__ __forceinline__
+
+namespace mppi_amd {
+
+// ------------------------------------------------------------------------------------------------ error handling
+void set_last_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define MPPI_HIP_CHECK(call)                              \
+  do {                                                    \
+    hipError_t _e = (call);                               \
+    if (_e != hipSuccess) return ::mppi_amd::hip_fail(_e, #call); \
+  } while (0)
+
+#define MPPI_CHECK(status_expr)       \
+  do {                                \
+    int _s = (status_expr);           \
+    if (_s != MPPI_OK) return _s;     \
+  } while (0)
+
+// ------------------------------------------------------------------------------------------- Philox4x32-10 + BM
+// Counter-based generator (Salmon, Moraes, Dror, Shaw, SC'11). Normal i of the stream (seed, offset) comes from
+// Philox block i/4 with counter (i/4 lo, i/4 hi, offset lo, offset hi), key (seed lo, seed hi), and Box-Muller on
+// the pairs (x0, x1) and (x2, x3). Any launch shape yields the same numbers.
+struct Philox4 {
+  uint32_t v[4];
+};
+
+MPPI_HD Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  Philox4 o;
+  o.v[0] = c0;
+  o.v[1] = c1;
+  o.v[2] = c2;
+  o.v[3] = c3;
+  return o;
+}
+
+// Uniform in (0, 1]: fma((float)x, 2^-32, 2^-33) -- one rounding, identical on every IEEE platform.
+MPPI_HD float u32_to_unit(uint32_t x) { return fmaf((float)x, 2.3283064365386963e-10f, 1.1641532182693481e-10f); }
+
+struct Normal4 {
+  float z[4];
+};
+
+__device__ __forceinline__ Normal4 normals4(uint64_t seed, uint64_t offset, uint64_t block)
+{
+  const Philox4 o = philox4x32_10((uint32_t)block, (uint32_t)(block >> 32), (uint32_t)offset,
+                                  (uint32_t)(offset >> 32), (uint32_t)seed, (uint32_t)(seed >> 32));
+  Normal4 n;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const float u1 = u32_to_unit(o.v[2 * p]);
+    const float u2 = u32_to_unit(o.v[2 * p + 1]);
+    const float r = sqrtf(-2.0f * logf(u1));
+    float s, c;
+    sincospif(2.0f * u2, &s, &c);
+    n.z[2 * p] = r * c;
+    n.z[2 * p + 1] = r * s;
+  }
+  return n;
+}
+
+// --------------------------------------------------------------------------------------------- wave reductions
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// (value, index) minimum with the lower index winning ties, i.e. the first strict minimum of a scan in index order.
+__device__ __forceinline__ void argmin_combine(float& v, int& i, float v2, int i2)
+{
+  if (v2 < v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
+  }
+}
+
+__device__ __forceinline__ void wave_argmin(float& v, int& i)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float v2 = __shfl_xor(v, o, 64);
+    const int i2 = __shfl_xor(i, o, 64);
+    argmin_combine(v, i, v2, i2);
+  }
+}
+
+// A cost that is NaN or +inf carries no weight and never becomes the baseline.
+__device__ __forceinline__ float sanitize_cost(float c) { return (c <= FLT_MAX) ? c : INFINITY; }
+
+// Pure-noise start index: first k with (float)k >= (1 - p) * K in FP32 (sampling_distributions/gaussian/gaussian.cu:96).
+int pure_noise_start(int K, float pct);
+
+}  // namespace mppi_amd

@@ -1,0 +1,40 @@
+// Internal launch interface of the engine's HIP kernels (used by the C-ABI and the controller object).
+#pragma once
+
+#include "common.hpp"
+
+namespace mppi_amd {
+
+// Per-call arguments that change between MPC solves. The controller keeps them in device memory so that a captured
+// hipGraph can be replayed with new values; the kernel-level C-ABI passes them by value.
+struct IterArgs {
+  uint64_t rng_offset;
+  int32_t stride;
+  int32_t iteration;
+};
+
+struct IterRef {
+  const IterArgs* ptr;  // device pointer or nullptr
+  IterArgs val;         // used when ptr == nullptr
+};
+
+int launch_generate_normals(uint64_t seed, uint64_t offset, float* out, int64_t n, hipStream_t s);
+int launch_sample_gaussian(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
+                           IterRef it, const float* eps_in, hipStream_t s);
+int launch_sample_colored(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
+                          IterRef it, const float* spectra_in, hipStream_t s);
+int launch_rollout(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,
+                   int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
+                   float* samples, float* costs, hipStream_t s);
+int launch_compute_weights(float* costs, int K, float lambda, mppi_weight_stats* stats, hipStream_t s);
+int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,
+                           float* out, hipStream_t s);
+
+// Weights + update in one launch: every workgroup reduces its block of samples against a local baseline, the last
+// workgroup to finish combines the partials with a log-sum-exp rescaling. `ws` must be zero-initialised once.
+// When `advance` is non-null the finishing workgroup also increments advance->rng_offset (free-running iterations).
+size_t fused_update_workspace_bytes(int K, int T, int C);
+int launch_fused_update(const float* costs, const float* samples, int K, int T, int C, float lambda, void* ws,
+                        mppi_weight_stats* stats, float* mean_out, hipStream_t s, IterArgs* advance = nullptr);
+
+}  // namespace mppi_amd

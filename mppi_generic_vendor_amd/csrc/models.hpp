@@ -1,0 +1,250 @@
+// Dynamics and cost plugins of the engine, as compile-time functors over the POD parameter blocks of mppi_amd.h.
+// The same functions run per lane inside the rollout kernels and on the host for the state-trajectory propagation
+// that follows an MPC solve. Reference formulas are cited per function.
+#pragma once
+
+#include "common.hpp"
+
+namespace mppi_amd {
+
+// utils/angle_utils.cuh:23-29 (float version)
+MPPI_HD float normalize_angle(float a)
+{
+  const float pi = 3.14159265358979323846f;
+  const float r = fmodf(a + pi, 6.28318530717958647692f);
+  return (r <= 0.0f) ? r + pi : r - pi;
+}
+
+// Dynamics::enforceConstraints (dynamics/dynamics.cu:107-126): deadband, then clamp to the control range.
+// sign() is utils/math_utils.h:744-747 (sign(0) == 1).
+template <int C>
+MPPI_HD void enforce_constraints(const mppi_dynamics_params& p, float* u)
+{
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    float v = u[i];
+    if (fabsf(v) < p.control_deadband[i]) v = p.zero_control[i];
+    else v += p.control_deadband[i] * -(v >= 0.0f ? 1.0f : -1.0f);
+    u[i] = fminf(fmaxf(p.control_min[i], v), p.control_max[i]);
+  }
+}
+
+template <int MODEL>
+struct Model;
+
+// Cart-pole, dynamics/cartpole/cartpole_dynamics.cu:87-105 (with accurate sinf/cosf).
+template <>
+struct Model<MPPI_MODEL_CARTPOLE> {
+  static constexpr int S = 4, C = 1, O = 4;
+  MPPI_HD static void deriv(const mppi_dynamics_params& p, const float* x, const float* u, float* xd)
+  {
+    const float th = normalize_angle(x[2]);
+    const float s = sinf(th), c = cosf(th);
+    const float td = x[3], F = u[0];
+    const float mc = p.cart_mass, mp = p.pole_mass, l = p.pole_length, g = p.gravity;
+    xd[0] = x[1];
+    xd[1] = (F + mp * s * (l * td * td + g * c)) / (mc + mp * s * s);
+    xd[2] = td;
+    xd[3] = (-F * c - mp * l * td * td * c * s - (mc + mp) * g * s) / (l * (mc + mp * s * s));
+  }
+  MPPI_HD static void update(const mppi_dynamics_params&, const float* x, float* xn, const float* xd, float dt)
+  {
+#pragma unroll
+    for (int i = 0; i < S; ++i) xn[i] = x[i] + xd[i] * dt;
+  }
+};
+
+// Double integrator, dynamics/double_integrator/di_dynamics.cu:46-53.
+template <>
+struct Model<MPPI_MODEL_DOUBLE_INTEGRATOR> {
+  static constexpr int S = 4, C = 2, O = 4;
+  MPPI_HD static void deriv(const mppi_dynamics_params&, const float* x, const float* u, float* xd)
+  {
+    xd[0] = x[2];
+    xd[1] = x[3];
+    xd[2] = u[0];
+    xd[3] = u[1];
+  }
+  MPPI_HD static void update(const mppi_dynamics_params&, const float* x, float* xn, const float* xd, float dt)
+  {
+#pragma unroll
+    for (int i = 0; i < S; ++i) xn[i] = x[i] + xd[i] * dt;
+  }
+};
+
+// Kinematic bicycle (RacerDubins), dynamics/racer_dubins/racer_dubins.cu:5-33,138-166 (deriv) and :44-59 (update).
+// State: VEL_X, YAW, POS_X, POS_Y, STEER_ANGLE, BRAKE_STATE, STEER_ANGLE_RATE. Control: THROTTLE_BRAKE, STEER_CMD.
+struct Bicycle {
+  static constexpr int S = 7, C = 2, O = 7;
+  MPPI_HD static void deriv(const mppi_dynamics_params& p, const float* x, const float* u, float* xd)
+  {
+    const bool brake = u[0] < 0.0f;
+    xd[5] = fminf(fmaxf(((brake ? -u[0] : 0.0f) - x[5]) * p.brake_delay_constant, -p.max_brake_rate_neg),
+                  p.max_brake_rate_pos);
+    xd[0] = (brake ? 0.0f : p.c_t * u[0] * (float)p.gear_sign) + p.c_b * x[5] * (x[0] >= 0.0f ? -1.0f : 1.0f) -
+            p.c_v * x[0] + p.c_0;
+    xd[1] = (x[0] / p.wheel_base) * tanf(x[4] / p.steer_angle_scale);
+    const float yaw = normalize_angle(x[1]);
+    float sy, cy;
+    sincosf(yaw, &sy, &cy);
+    xd[2] = x[0] * cy;
+    xd[3] = x[0] * sy;
+    xd[4] = fmaxf(fminf((u[1] * p.steer_command_angle_scale - x[4]) * p.steering_constant, p.max_steer_rate),
+                  -p.max_steer_rate);
+    xd[6] = 0.0f;
+  }
+  MPPI_HD static void update(const mppi_dynamics_params& p, const float* x, float* xn, const float* xd, float dt)
+  {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) xn[i] = x[i] + xd[i] * dt;
+    xn[1] = normalize_angle(xn[1]);
+    xn[4] = fmaxf(fminf(xn[4], p.max_steer_angle), -p.max_steer_angle);
+    xn[6] = xd[4];
+    xn[5] = fminf(fmaxf(xn[5], 0.0f), -p.control_min[0]);
+  }
+};
+
+template <>
+struct Model<MPPI_MODEL_RACER_DUBINS> : Bicycle {};
+
+// Bicycle + residual MLP [x(7), u(2)] -> 64 -> 64 -> 4 (tanh hidden, linear out) added to the derivatives of
+// nn_output_states. The per-lane FP32 path below reads the uniform weights through scalar loads; the BF16 MFMA path
+// lives in the rollout kernel. Layer layout: FNNHelper (utils/nn_helpers/fnn_helper.cu:130-205).
+constexpr int NN_IN = 9, NN_H1 = 64, NN_H2 = 64, NN_OUT = 4;
+constexpr int NN_PARAMS = (NN_IN + 1) * NN_H1 + (NN_H1 + 1) * NN_H2 + (NN_H2 + 1) * NN_OUT;
+
+MPPI_HD float tanh_acc(float v) { return tanhf(v); }
+
+MPPI_HD void nn_forward_f32(const float* __restrict__ th, const float* in, float* out)
+{
+  float h1[NN_H1];
+  const float* W1 = th;
+  const float* b1 = W1 + NN_IN * NN_H1;
+#pragma unroll 4
+  for (int j = 0; j < NN_H1; ++j) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NN_IN; ++k) s += W1[j * NN_IN + k] * in[k];
+    h1[j] = tanh_acc(s + b1[j]);
+  }
+  const float* W2 = b1 + NN_H1;
+  const float* b2 = W2 + NN_H1 * NN_H2;
+  float h2[NN_H2];
+#pragma unroll 2
+  for (int j = 0; j < NN_H2; ++j) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NN_H1; ++k) s += W2[j * NN_H1 + k] * h1[k];
+    h2[j] = tanh_acc(s + b2[j]);
+  }
+  const float* W3 = b2 + NN_H2;
+  const float* b3 = W3 + NN_H2 * NN_OUT;
+#pragma unroll
+  for (int j = 0; j < NN_OUT; ++j) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NN_H2; ++k) s += W3[j * NN_H2 + k] * h2[k];
+    out[j] = s + b3[j];
+  }
+}
+
+template <>
+struct Model<MPPI_MODEL_RACER_DUBINS_NN> : Bicycle {
+  MPPI_HD static void deriv(const mppi_dynamics_params& p, const float* x, const float* u, float* xd)
+  {
+    Bicycle::deriv(p, x, u, xd);
+    float in[NN_IN], out[NN_OUT];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) in[i] = x[i];
+    in[7] = u[0];
+    in[8] = u[1];
+    nn_forward_f32(p.nn_theta, in, out);
+#pragma unroll
+    for (int i = 0; i < NN_OUT; ++i) xd[p.nn_output_states[i]] += out[i];
+  }
+};
+
+// ---------------------------------------------------------------------------------------------------------- costs
+// Bilinear cost-map query with the reference's CPU semantics (utils/texture_helpers/two_d_texture_helper.cu:151-245
+// and the world->texture transform of texture_helper.cu:94-134 with identity rotation): clamp address mode,
+// texel centres at +0.5, x0 = min(floor(q), W - 2).
+__device__ __forceinline__ float map_query(const mppi_cost_params& c, float px, float py)
+{
+  const int W = c.map_width, H = c.map_height;
+  const float nx = ((px - c.map_origin_x) / c.map_resolution) / (float)W;
+  const float ny = ((py - c.map_origin_y) / c.map_resolution) / (float)H;
+  // clamp to [0, W - 1] as the reference's if/else chain does for every non-NaN value; a NaN query maps to texel 0
+  // instead of reaching the index computation
+  const float qx = fminf(fmaxf(nx * (float)W - 0.5f, 0.0f), (float)(W - 1));
+  const float qy = fminf(fmaxf(ny * (float)H - 0.5f, 0.0f), (float)(H - 1));
+  const int x0 = min((int)floorf(qx), W - 2), x1 = x0 + 1;
+  const int y0 = min((int)floorf(qy), H - 2), y1 = y0 + 1;
+  const float* row0 = c.map + (size_t)y0 * W;
+  const float* row1 = c.map + (size_t)y1 * W;
+  const float q11 = row0[x0], q12 = row0[x1], q21 = row1[x0], q22 = row1[x1];
+  const float ya = q11 * ((float)x1 - qx) + q12 * (qx - (float)x0);
+  const float yb = q21 * ((float)x1 - qx) + q22 * (qx - (float)x0);
+  return ya * ((float)y1 - qy) + yb * (qy - (float)y0);
+}
+
+template <int O>
+MPPI_HD float quad_state(const mppi_cost_params& c, const float* y)
+{
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < O; ++i) {
+    const float e = y[i] - c.goal[i];
+    s += e * e * c.coeff[i];
+  }
+  return s;
+}
+
+template <int COST, int O>
+struct Cost;
+
+// cost_functions/cartpole/cartpole_quadratic_cost.cu:23-55
+template <int O>
+struct Cost<MPPI_COST_CARTPOLE_QUADRATIC, O> {
+  __device__ __forceinline__ static float running(const mppi_cost_params& c, const float* y) { return quad_state<O>(c, y); }
+  __device__ __forceinline__ static float terminal(const mppi_cost_params& c, const float* y)
+  {
+    return quad_state<O>(c, y) * c.terminal_cost_coeff;
+  }
+};
+
+// Quadratic goal term (cost_functions/quadratic_cost/quadratic_cost.cu:35-54) + cost-map term + speed term
+// (cost_functions/autorally/ar_standard_cost.cu:284-297); terminal = terminal_cost_coeff * quadratic term.
+template <int O>
+struct Cost<MPPI_COST_QUADRATIC_MAP, O> {
+  __device__ __forceinline__ static float running(const mppi_cost_params& c, const float* y)
+  {
+    float s = quad_state<O>(c, y);
+    if (c.map != nullptr && c.map_coeff != 0.0f) s += c.map_coeff * map_query(c, y[c.map_x_index], y[c.map_y_index]);
+    if (c.speed_coeff != 0.0f) {
+      const float e = y[c.speed_index] - c.speed_target;
+      s += c.speed_coeff * e * e;
+    }
+    return s;
+  }
+  __device__ __forceinline__ static float terminal(const mppi_cost_params& c, const float* y)
+  {
+    return (c.terminal_cost_coeff == 0.0f) ? 0.0f : quad_state<O>(c, y) * c.terminal_cost_coeff;
+  }
+};
+
+// Likelihood-ratio control cost (sampling_distributions/gaussian/gaussian.cu:491-574,650-675):
+// 0.5 * lambda * (1 - alpha) * sum_c coeff_c * mu_c * (mu_c - 2 u_c) / sigma_c^2, mu = 0 for pure-noise samples.
+template <int C>
+__device__ __forceinline__ float lr_cost(const mppi_sampler_params& s, const float* u, const float* mu, bool pure,
+                                         float lambda, float alpha)
+{
+  float cost = 0.0f;
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const float m = pure ? 0.0f : mu[i];
+    cost += s.control_cost_coeff[i] * m * (m - 2.0f * u[i]) / (s.std_dev[i] * s.std_dev[i]);
+  }
+  return 0.5f * lambda * (1.0f - alpha) * cost;
+}
+
+}  // namespace mppi_amd

@@ -1,0 +1,264 @@
+"""Parity of the HIP engine (called through the C-ABI) with the CPU oracle on identical seeded inputs.
+
+Bars: integer/index results (argmin, pure-noise split, clamping, copies of the mean) are bit-exact; FP32 costs agree
+with the oracle's FP32 loop within 1e-4 relative (tests/mppi_core/rollout_kernel_tests.cu:258), the sampler
+transforms within 1e-6 relative, the weights and the update within the tolerances stated per test. The error of both
+the GPU and the oracle's FP32 run against the oracle's FP64 run is printed for every rollout case.
+"""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_util import (banner, gpu_normals, gpu_rollout, gpu_sample_colored, gpu_sample_gaussian,
+                      gpu_weighted_update, gpu_weights, rel_err)
+from mppi_generic_vendor_amd import capi, workloads
+from mppi_generic_vendor_amd.controller import VanillaMPPIController
+
+pytestmark = pytest.mark.gpu
+
+SMALL_MAP = workloads.gaussian_blob_map(256, 256, blobs=16, seed=7)
+
+
+def test_normals_match_oracle(hip, engine):
+    banner("normals n=100003 seed=42 offset=5")
+    n = 100_003
+    z = gpu_normals(42, 5, n)
+    want = ol.normals(42, 5, n)
+    assert np.all(np.isfinite(z))
+    np.testing.assert_allclose(z, want, rtol=2e-6, atol=2e-6)
+
+
+def test_gaussian_sampler_generated_and_fed(hip, engine):
+    banner("gaussian sampler K=300 T=37 C=2")
+    K, T, C = 300, 37, 2
+    sp = capi.sampler_defaults(capi.SAMPLER_GAUSSIAN, C)
+    sp.std_dev[0], sp.std_dev[1] = 0.7, 1.9
+    sp.std_dev_decay = 0.95
+    sp.seed = 1234
+    mean = np.random.default_rng(1).normal(size=(T, C)).astype(np.float32)
+    gen = gpu_sample_gaussian(sp, K, T, C, mean, stride=2, iteration=3, rng_offset=9)
+    eps = gpu_normals(1234, 9, K * T * C).reshape(K, T, C)
+    fed = gpu_sample_gaussian(sp, K, T, C, mean, stride=2, iteration=3, eps=eps)
+    assert np.array_equal(gen, fed)
+    want = ol.gaussian_transform(sp, K, T, C, mean, eps, 2, 3)
+    pstart = capi.pure_noise_start(K, 0.01)
+    assert np.array_equal(gen[0], mean) and np.array_equal(gen[:, :2], np.broadcast_to(mean[:2], (K, 2, C)))
+    np.testing.assert_allclose(gen, want, rtol=1e-6, atol=1e-6)
+    assert np.array_equal(want[pstart:, 2:] == 0, gen[pstart:, 2:] == 0)
+
+
+@pytest.mark.parametrize("T,mode,stride,decay", [(100, capi.FFT_LEN_2T, 1, 0.97), (100, capi.FFT_LEN_T, 0, 0.0),
+                                                 (37, capi.FFT_LEN_2T, 0, 0.97), (51, capi.FFT_LEN_T, 3, 0.5)])
+def test_colored_sampler_matches_oracle(hip, engine, T, mode, stride, decay):
+    banner(f"colored sampler T={T} mode={mode} stride={stride}")
+    K, C = 200, 2
+    sp = capi.sampler_defaults(capi.SAMPLER_COLORED, C)
+    sp.exponents[0], sp.exponents[1] = 1.0, 2.0
+    sp.fft_len_mode = mode
+    sp.offset_decay_rate = decay
+    sp.std_dev[0], sp.std_dev[1] = 1.0, 0.5
+    sp.seed = 77
+    n = 2 * T if mode == capi.FFT_LEN_2T else T
+    F = n // 2 + 1
+    mean = np.random.default_rng(2).normal(size=(T, C)).astype(np.float32)
+    spectra = gpu_normals(77, 4, K * C * F * 2).reshape(K, C, F, 2)
+    gen = gpu_sample_colored(sp, K, T, C, mean, stride, 1, rng_offset=4)
+    fed = gpu_sample_colored(sp, K, T, C, mean, stride, 1, spectra=spectra)
+    assert np.array_equal(gen, fed)
+    eps, want = ol.colored_samples(sp, K, T, C, mean, spectra, stride, 1)
+    np.testing.assert_allclose(gen, want, rtol=1e-4, atol=1e-4)
+    assert np.array_equal(gen[0], mean)
+
+
+def _sample_for(prob, K, T):
+    sp = prob.sampler
+    mean = np.zeros((T, prob.C), np.float32)
+    if prob.name == "cartpole":
+        mean[:, 0] = np.sin(np.linspace(0, 3, T)).astype(np.float32)
+    if sp.kind == capi.SAMPLER_COLORED:
+        return gpu_sample_colored(sp, K, T, prob.C, mean, 1, 0, rng_offset=0), mean
+    return gpu_sample_gaussian(sp, K, T, prob.C, mean, 1, 0, rng_offset=0), mean
+
+
+def _problems():
+    return {
+        "cartpole": lambda: workloads.cartpole(num_rollouts=2048, num_timesteps=100),
+        "di_map_colored": lambda: workloads.double_integrator_map(num_rollouts=1024, num_timesteps=100,
+                                                                  cost_map=SMALL_MAP),
+        "bicycle": lambda: workloads.bicycle_nn(num_rollouts=512, num_timesteps=100, residual=False,
+                                                cost_map=SMALL_MAP),
+        "bicycle_nn_fp32": lambda: workloads.bicycle_nn(num_rollouts=512, num_timesteps=100, cost_map=SMALL_MAP),
+        "cartpole_ragged": lambda: workloads.cartpole(num_rollouts=100, num_timesteps=7),
+        "di_single": lambda: workloads.double_integrator_map(num_rollouts=1, num_timesteps=1, colored=False,
+                                                             cost_map=SMALL_MAP),
+    }
+
+
+@pytest.mark.parametrize("name", list(_problems().keys()))
+def test_rollout_matches_oracle(hip, engine, name):
+    prob = _problems()[name]()
+    if name == "cartpole":
+        prob.dyn.control_min[0], prob.dyn.control_max[0] = -2.0, 2.0
+    K, T = prob.num_rollouts, prob.num_timesteps
+    banner(f"rollout {name} K={K} T={T}")
+    samples, mean = _sample_for(prob, K, T)
+    costs_gpu, clamped_gpu = gpu_rollout(prob, samples, mean)
+    costs_cpu, clamped_cpu = ol.rollout_f32(prob, samples, mean, nthreads=8)
+    costs_64, _ = ol.rollout_f64(prob, samples, mean)
+    assert np.array_equal(clamped_gpu, clamped_cpu)
+    assert np.all(np.isfinite(costs_gpu))
+    e_gpu = rel_err(costs_gpu, costs_cpu)
+    print(f"[fp64] {name}: gpu-vs-oracle32 {e_gpu:.2e}  gpu-vs-fp64 {rel_err(costs_gpu, costs_64):.2e}  "
+          f"oracle32-vs-fp64 {rel_err(costs_cpu, costs_64):.2e}")
+    assert e_gpu < 1e-4
+    assert int(np.argmin(costs_gpu)) == int(np.argmin(costs_cpu))
+
+
+def test_weights_match_oracle_bitexact_argmin(hip, engine):
+    banner("weights K=8192")
+    rng = np.random.default_rng(3)
+    c = (rng.gamma(2.0, 20.0, 8192) + 3).astype(np.float32)
+    c[4000] = c.min() - 0.5
+    c[6000] = c[4000]
+    for lam in (1.0, 0.25, 30.0):
+        w, st = gpu_weights(c, lam)
+        wo, eta, b, idx, s2 = ol.weights(c, lam)
+        assert st.argmin == idx == 4000 and st.baseline == b
+        assert st.normalizer == pytest.approx(eta, rel=1e-6)
+        assert st.sum_w2 == pytest.approx(s2, rel=1e-6)
+        w32 = np.empty(8192, np.float32)
+        ol.load().oracle_weights_f32(c.ctypes.data, 8192, lam, w32.ctypes.data)
+        np.testing.assert_allclose(w, w32, rtol=2e-6, atol=1e-30)
+
+
+def test_weights_nonfinite_costs_get_zero_weight(hip, engine):
+    banner("weights with NaN/inf")
+    c = np.linspace(5, 10, 1000).astype(np.float32)
+    c[0] = np.nan
+    c[1] = np.inf
+    c[500] = 1.0
+    w, st = gpu_weights(c, 1.0)
+    assert st.argmin == 500 and st.baseline == 1.0
+    assert w[0] == 0 and w[1] == 0 and w[500] == 1.0
+    assert np.isfinite(st.normalizer)
+
+
+def test_all_equal_costs_give_uniform_weights(hip, engine):
+    c = np.full(4096, 7.25, np.float32)
+    w, st = gpu_weights(c, 1.0)
+    assert st.argmin == 0 and np.all(w == 1.0) and st.normalizer == 4096.0
+
+
+def test_weighted_update_matches_oracle(hip, engine):
+    banner("weighted update K=8192 T=100 C=2")
+    rng = np.random.default_rng(4)
+    K, T, C = 8192, 100, 2
+    u = rng.normal(size=(K, T, C)).astype(np.float32)
+    c = rng.uniform(0, 20, K).astype(np.float32)
+    w, st = gpu_weights(c, 2.0)
+    out = gpu_weighted_update(w, st, u)
+    wo, eta, *_ = ol.weights(c, 2.0)
+    want = ol.weighted_update(wo, eta, u)
+    np.testing.assert_allclose(out, want, rtol=1e-4, atol=2e-6)
+
+
+def _oracle_iteration(prob, samples, mean):
+    costs, clamped = ol.rollout_f32(prob, samples, mean, nthreads=8)
+    w, eta, b, idx, s2 = ol.weights(costs, prob.lambda_)
+    return ol.weighted_update(w, eta, clamped), costs, b, idx
+
+
+@pytest.mark.parametrize("name,graph", [("cartpole", False), ("cartpole", True), ("di_map_colored", True),
+                                        ("bicycle_nn_fp32", False)])
+def test_controller_iteration_matches_oracle(hip, engine, name, graph):
+    prob = _problems()[name]()
+    K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
+    banner(f"controller {name} graph={graph} K={K}")
+    ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=graph)
+    stride = 1
+    # eager first call (and, with graph, a captured replay on the second call with a new rng offset)
+    calls = 2 if graph else 1
+    for call in range(calls):
+        nominal = np.zeros((T, C), np.float32) if call == 0 else ctrl.getControlSeq()
+        ctrl.updateImportanceSampler(nominal)
+        ctrl.computeControl(prob.x0, stride)
+        raw = ctrl.getRawMean()
+        st = ctrl.getWeightStats()
+        if prob.sampler.kind == capi.SAMPLER_COLORED:
+            n = 2 * T if prob.sampler.fft_len_mode == capi.FFT_LEN_2T else T
+            spectra = gpu_normals(prob.sampler.seed, call, K * C * (n // 2 + 1) * 2)
+            samples = gpu_sample_colored(prob.sampler, K, T, C, nominal, stride, 0, spectra=spectra)
+        else:
+            eps = gpu_normals(prob.sampler.seed, call, K * T * C)
+            samples = gpu_sample_gaussian(prob.sampler, K, T, C, nominal, stride, 0, eps=eps)
+        want, costs, b, idx = _oracle_iteration(prob, samples, nominal)
+        scale = max(1e-3, float(np.max(np.abs(want))))
+        err = float(np.max(np.abs(raw - want))) / scale
+        print(f"[parity] {name} call {call}: control rel err {err:.2e} baseline {st.baseline:.6g} vs {b:.6g} "
+              f"argmin {st.argmin} vs {idx}")
+        assert err < 1e-4
+        assert st.argmin == idx
+        assert st.baseline == pytest.approx(b, rel=1e-4)
+        # host post-processing: SG smoothing keeps u[0]; the state sequence propagates the returned controls
+        smoothed = ol.smooth(raw)
+        np.testing.assert_allclose(ctrl.getControlSeq(), ol.enforce(prob, smoothed), rtol=1e-6, atol=1e-7)
+        xs = ol.state_trajectory(prob, prob.x0, smoothed)
+        np.testing.assert_allclose(ctrl.getTargetStateSeq(), xs, rtol=1e-4, atol=1e-5)
+    ctrl.close()
+
+
+def test_graph_replay_equals_eager_over_many_calls(hip, engine):
+    banner("graph replay vs eager, 6 calls")
+    prob = workloads.cartpole(num_rollouts=1024, num_timesteps=60)
+    a = VanillaMPPIController(prob, num_iters=2, use_graph=False)
+    b = VanillaMPPIController(prob, num_iters=2, use_graph=True)
+    x = prob.x0.copy()
+    for i in range(6):
+        a.computeControl(x, 1)
+        b.computeControl(x, 1)
+        ra, rb = a.getRawMean(), b.getRawMean()
+        assert np.array_equal(ra, rb), f"call {i}"
+        assert a.getWeightStats().argmin == b.getWeightStats().argmin
+        a.slideControlSequence(1)
+        b.slideControlSequence(1)
+        x = a.getTargetStateSeq()[1].copy()
+    fe = b.getFreeEnergyStatistics()
+    assert np.isfinite(fe.free_energy_mean) and fe.normalizer_percent > 0
+    a.close()
+    b.close()
+
+
+def test_cartpole_swing_up_behaviour(hip, engine):
+    """Closed-loop cart-pole (the reference's behavioural test, tests/controllers/vanilla_mppi_test.cu:79-136):
+    the controller drives the pole to the upright goal and the baseline cost falls."""
+    banner("cartpole closed loop")
+    prob = workloads.cartpole(num_rollouts=2048, num_timesteps=100)
+    prob.dt = 0.01
+    prob.lambda_ = 0.25
+    prob.alpha = 0.01
+    prob.sampler.std_dev[0] = 5.0
+    # CartpoleQuadraticCostParams of the reference test: Q = (100, 10, 200, 20), goal (-20, 0, pi, 0), no terminal
+    prob.cost.coeff[0], prob.cost.coeff[1], prob.cost.coeff[2], prob.cost.coeff[3] = 100.0, 10.0, 200.0, 20.0
+    prob.cost.goal[0] = -20.0
+    prob.cost.terminal_cost_coeff = 0.0
+    prob.sampler.control_cost_coeff[0] = 1.0
+    ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True, slide_control_scale=[1.0])
+    x = np.zeros(4, np.float32)
+    dyn = ol._host_dyn(prob)
+    baselines = []
+    for step in range(1000):
+        ctrl.computeControl(x, 1)
+        u = ctrl.getControlSeq()[0]
+        xn = np.zeros(4, np.float32)
+        xd = np.zeros(4, np.float32)
+        y = np.zeros(4, np.float32)
+        ol.load().oracle_step_f32(ct.byref(dyn), x.ctypes.data, u.ctypes.data, prob.dt, xn.ctypes.data,
+                                  xd.ctypes.data, y.ctypes.data)
+        x = xn
+        ctrl.slideControlSequence(1)
+        baselines.append(ctrl.getBaselineCost())
+    print(f"[behaviour] final state {x}, baseline {baselines[-1]:.4f}")
+    assert baselines[-1] < 1.0
+    ctrl.close()
