@@ -200,6 +200,14 @@ int mppi_nn_num_params(int32_t num_layers, const int32_t* layers, int32_t* out_n
  * FP32, as in sampling_distributions/gaussian/gaussian.cu:96 and :658. */
 int mppi_pure_noise_start(int32_t num_rollouts, float pure_noise_percentage, int32_t* out_index);
 
+/* Host-side plugin methods (the same model code the kernels run), with host pointers:
+ * Dynamics::step (dynamics/dynamics.cuh:278-285: computeStateDeriv -> updateState -> stateToOutput) and
+ * Dynamics::enforceConstraints (dynamics/dynamics.cuh:247-261). Any of x_next, x_dot, y may be NULL. For
+ * MPPI_MODEL_RACER_DUBINS_NN, p->nn_theta must point to host memory here. */
+int mppi_host_step(const mppi_dynamics_params* p, const float* x, const float* u, float dt, float* x_next,
+                   float* x_dot, float* y);
+int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u);
+
 /* ------------------------------------------------------------------------------------------------------------------
  * Kernel-level entry points (device pointers, caller stream).
  * ---------------------------------------------------------------------------------------------------------------- */
@@ -260,8 +268,8 @@ typedef struct mppi_controller_desc {
   float lambda;
   float alpha;
   int32_t num_iters;
-  int32_t use_graph; /* 1: capture one iteration into a hipGraph on first use and replay it */
-  int32_t _pad;
+  int32_t use_graph;    /* 1: capture one computeControl call into a hipGraph on its second use and replay it */
+  int32_t keep_samples; /* 1: store the clamped control samples [K][T][C] in the sample buffer every iteration */
   float slide_control_scale[MPPI_MAX_CONTROL]; /* ControllerParams::slide_control_scale_ */
   const float* init_control; /* host [T][C] initial nominal sequence or NULL for zeros */
   const float* cost_map_host; /* optional host [map_height][map_width]; copied into controller-owned memory and
@@ -293,7 +301,9 @@ int mppi_controller_get_free_energy(const mppi_controller* ctrl, mppi_free_energ
 int mppi_controller_set_control_seq(mppi_controller* ctrl, const float* u_host);
 /* slideControlSequence(steps) (controllers/controller.cuh:609-621). */
 int mppi_controller_slide_control_seq(mppi_controller* ctrl, int32_t steps);
-/* Device pointers to the controller's buffers (samples [K][T][C], costs/weights [K]) for inspection. */
+/* Device pointers to the controller's buffers for inspection: samples [K][T][C] (clamped samples of the last
+ * iteration when keep_samples is set; the colored sampler always writes its pre-clamp samples there), costs [K] of
+ * the last iteration, and the [T][C] mean the last iteration sampled around. */
 int mppi_controller_device_buffers(const mppi_controller* ctrl, float** samples_d, float** costs_d, float** mean_d);
 /* Reseed the generator and reset its counter (controllers/controller.cu:203-209). */
 int mppi_controller_set_seed(mppi_controller* ctrl, uint64_t seed);

@@ -1,0 +1,724 @@
+// mppi.hpp -- C++ template plugin API of the MI355X MPPI engine, header-only, over the C-ABI of mppi_amd.h.
+//
+// It keeps the reference's type and method names so that user code written against MPPI-Generic ports by
+// changing includes and namespaces:
+//   VanillaMPPIController<DYN_T, COST_T, FB_T, MAX_TIMESTEPS, NUM_ROLLOUTS, SAMPLING_T, PARAMS_T>
+//       (controllers/MPPI/mppi_controller.cuh:14-41, controllers/controller.cuh:70-1058)
+//   CartpoleDynamics, DoubleIntegratorDynamics, RacerDubins  (dynamics/*: Dynamics<CLASS_T, PARAMS_T>)
+//   CartpoleQuadraticCost, QuadraticMapCost<DYN_T>          (cost_functions/*: Cost<CLASS_T, PARAMS_T, DYN_PARAMS_T>)
+//   sampling_distributions::GaussianDistribution, ColoredNoiseDistribution  (sampling_distributions/*)
+// Eigen is replaced by the fixed-size column-major Mat below (col(), data(), operator()), so `control_trajectory`
+// holds the same bytes as the reference's Eigen::Matrix<float, C, T>.
+//
+// Device work never happens in these classes: the plugins are parameter holders whose `desc()` is a POD block of
+// mppi_amd.h, and the kernel library instantiates the registered (model x cost x sampler) combinations. Errors from
+// the C-ABI are thrown as std::runtime_error (the reference exits the process, utils/gpu_err_chk.cuh:32-40).
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../mppi_amd.h"
+
+namespace mppi_amd {
+
+inline void check(int status, const char* what)
+{
+  if (status != MPPI_OK) throw std::runtime_error(std::string(what) + ": " + mppi_last_error());
+}
+
+struct float2 {
+  float x, y;
+};
+
+struct dim3 {
+  unsigned x = 1, y = 1, z = 1;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+// ------------------------------------------------------------------------------------------------ fixed matrices
+template <class T, int R, int C>
+class Mat {
+ public:
+  static constexpr int RowsAtCompileTime = R, ColsAtCompileTime = C;
+
+  class ColRef {
+   public:
+    explicit ColRef(T* p) : p_(p) {}
+    T& operator[](int i) { return p_[i]; }
+    T& operator()(int i) { return p_[i]; }
+    ColRef& operator=(const Mat<T, R, 1>& v)
+    {
+      std::memcpy(p_, v.data(), sizeof(T) * R);
+      return *this;
+    }
+    operator Mat<T, R, 1>() const
+    {
+      Mat<T, R, 1> v;
+      std::memcpy(v.data(), p_, sizeof(T) * R);
+      return v;
+    }
+    T* data() { return p_; }
+
+   private:
+    T* p_;
+  };
+
+  Mat() { setZero(); }
+  static Mat Zero() { return Mat(); }
+  static Mat Ones()
+  {
+    Mat m;
+    for (auto& v : m.d_) v = T(1);
+    return m;
+  }
+  static Mat Constant(T v)
+  {
+    Mat m;
+    for (auto& x : m.d_) x = v;
+    return m;
+  }
+  void setZero()
+  {
+    for (auto& v : d_) v = T(0);
+  }
+  T& operator()(int r, int c) { return d_[c * R + r]; }
+  const T& operator()(int r, int c) const { return d_[c * R + r]; }
+  T& operator()(int i) { return d_[i]; }
+  const T& operator()(int i) const { return d_[i]; }
+  T& operator[](int i) { return d_[i]; }
+  const T& operator[](int i) const { return d_[i]; }
+  ColRef col(int c) { return ColRef(d_ + c * R); }
+  Mat<T, R, 1> col(int c) const
+  {
+    Mat<T, R, 1> v;
+    std::memcpy(v.data(), d_ + c * R, sizeof(T) * R);
+    return v;
+  }
+  T* data() { return d_; }
+  const T* data() const { return d_; }
+  static constexpr int rows() { return R; }
+  static constexpr int cols() { return C; }
+
+ private:
+  T d_[R * C];
+};
+
+// =================================================================================================== dynamics
+// Dynamics<CLASS_T, PARAMS_T> (dynamics/dynamics.cuh:68-522): host methods run the engine's model code through
+// mppi_host_step / mppi_host_enforce_constraints; the device copy is made by the controller.
+template <int MODEL, int S, int C, int O>
+class DynamicsBase {
+ public:
+  static const int STATE_DIM = S, CONTROL_DIM = C, OUTPUT_DIM = O;
+  using state_array = Mat<float, S, 1>;
+  using control_array = Mat<float, C, 1>;
+  using output_array = Mat<float, O, 1>;
+
+  DynamicsBase()
+  {
+    check(mppi_dynamics_defaults(MODEL, &desc_), "mppi_dynamics_defaults");
+    for (int i = 0; i < C; ++i) {
+      control_rngs_[i] = {desc_.control_min[i], desc_.control_max[i]};
+      control_deadband_[i] = 0.0f;
+    }
+  }
+
+  // Dynamics::control_rngs_ / control_deadband_ / zero_control_ are public in the reference (examples write them)
+  float2 control_rngs_[C];
+  float control_deadband_[C];
+  control_array zero_control_ = control_array::Zero();
+
+  void setControlRanges(const std::array<float2, C>& r)
+  {
+    for (int i = 0; i < C; ++i) control_rngs_[i] = r[i];
+  }
+  void setControlDeadbands(const std::array<float, C>& d)
+  {
+    for (int i = 0; i < C; ++i) control_deadband_[i] = d[i];
+  }
+  std::array<float2, C> getControlRanges() const
+  {
+    std::array<float2, C> r;
+    for (int i = 0; i < C; ++i) r[i] = control_rngs_[i];
+    return r;
+  }
+  state_array getZeroState() const { return state_array::Zero(); }
+
+  void enforceConstraints(state_array&, control_array& u) const
+  {
+    const mppi_dynamics_params d = desc();
+    check(mppi_host_enforce_constraints(&d, u.data()), "enforceConstraints");
+  }
+  void computeStateDeriv(const state_array& x, const control_array& u, state_array& xd) const
+  {
+    const mppi_dynamics_params d = host_desc();
+    check(mppi_host_step(&d, x.data(), u.data(), 0.0f, nullptr, xd.data(), nullptr), "computeStateDeriv");
+  }
+  void updateState(const state_array& x, state_array& xn, const state_array& xd, float dt) const
+  {
+    for (int i = 0; i < S; ++i) xn[i] = x[i] + xd[i] * dt;
+  }
+  void step(const state_array& x, state_array& xn, state_array& xd, const control_array& u, output_array& y,
+            float /*t*/, float dt) const
+  {
+    const mppi_dynamics_params d = host_desc();
+    check(mppi_host_step(&d, x.data(), u.data(), dt, xn.data(), xd.data(), y.data()), "step");
+  }
+  void initializeDynamics(const state_array&, const control_array&, output_array&, float, float) {}
+  void GPUSetup() {}
+  void freeCudaMem() {}
+  void bindToStream(void*) {}
+
+  // The POD block the kernel library consumes.
+  mppi_dynamics_params desc() const
+  {
+    mppi_dynamics_params d = desc_;
+    for (int i = 0; i < C; ++i) {
+      d.control_min[i] = control_rngs_[i].x;
+      d.control_max[i] = control_rngs_[i].y;
+      d.control_deadband[i] = control_deadband_[i];
+      d.zero_control[i] = zero_control_[i];
+    }
+    return d;
+  }
+  virtual const float* hostParameters() const { return nullptr; }
+  virtual ~DynamicsBase() = default;
+
+ protected:
+  mppi_dynamics_params host_desc() const
+  {
+    mppi_dynamics_params d = desc();
+    d.nn_theta = hostParameters();
+    return d;
+  }
+  mppi_dynamics_params desc_{};
+};
+
+struct CartpoleDynamicsParams {
+  float cart_mass = 1.0f;
+  float pole_mass = 1.0f;
+  float pole_length = 1.0f;
+  CartpoleDynamicsParams() = default;
+  CartpoleDynamicsParams(float m_c, float m_p, float l) : cart_mass(m_c), pole_mass(m_p), pole_length(l) {}
+};
+
+// dynamics/cartpole/cartpole_dynamics.cuh:40-101
+class CartpoleDynamics : public DynamicsBase<MPPI_MODEL_CARTPOLE, 4, 1, 4> {
+ public:
+  using DYN_PARAMS_T = CartpoleDynamicsParams;
+  explicit CartpoleDynamics(float cart_mass = 1.0f, float pole_mass = 1.0f, float pole_length = 1.0f,
+                            void* /*stream*/ = nullptr)
+  {
+    setParams(CartpoleDynamicsParams(cart_mass, pole_mass, pole_length));
+  }
+  void setParams(const CartpoleDynamicsParams& p)
+  {
+    params_ = p;
+    desc_.cart_mass = p.cart_mass;
+    desc_.pole_mass = p.pole_mass;
+    desc_.pole_length = p.pole_length;
+  }
+  CartpoleDynamicsParams getParams() const { return params_; }
+  float getCartMass() const { return params_.cart_mass; }
+  float getPoleMass() const { return params_.pole_mass; }
+  float getPoleLength() const { return params_.pole_length; }
+  float getGravity() const { return desc_.gravity; }
+
+ private:
+  CartpoleDynamicsParams params_;
+};
+
+struct DoubleIntegratorParams {};
+
+// dynamics/double_integrator/di_dynamics.cuh
+class DoubleIntegratorDynamics : public DynamicsBase<MPPI_MODEL_DOUBLE_INTEGRATOR, 4, 2, 4> {
+ public:
+  using DYN_PARAMS_T = DoubleIntegratorParams;
+  explicit DoubleIntegratorDynamics(float /*system_noise*/ = 1.0f, void* /*stream*/ = nullptr) {}
+};
+
+// RacerDubinsParams (dynamics/racer_dubins/racer_dubins.cuh:81-103)
+struct RacerDubinsParams {
+  float c_t[3] = {1.3f, 2.6f, 3.9f};
+  float c_b[3] = {2.5f, 3.5f, 4.5f};
+  float c_v[3] = {3.7f, 4.7f, 5.7f};
+  float c_0 = 4.9f;
+  float steering_constant = 0.6f;
+  float steer_command_angle_scale = 5.0f;
+  float steer_angle_scale = -9.1f;
+  float max_steer_angle = 0.5f;
+  float max_steer_rate = 5.0f;
+  float brake_delay_constant = 6.6f;
+  float max_brake_rate_neg = 0.9f;
+  float max_brake_rate_pos = 0.33f;
+  float wheel_base = 0.3f;
+  int gear_sign = 1;
+};
+
+template <int MODEL>
+class RacerDubinsImpl : public DynamicsBase<MODEL, 7, 2, 7> {
+ public:
+  using DYN_PARAMS_T = RacerDubinsParams;
+  RacerDubinsImpl() { setParams(RacerDubinsParams()); }
+  void setParams(const RacerDubinsParams& p)
+  {
+    params_ = p;
+    auto& d = this->desc_;
+    d.c_t = p.c_t[0];
+    d.c_b = p.c_b[0];
+    d.c_v = p.c_v[0];
+    d.c_0 = p.c_0;
+    d.steering_constant = p.steering_constant;
+    d.steer_command_angle_scale = p.steer_command_angle_scale;
+    d.steer_angle_scale = p.steer_angle_scale;
+    d.max_steer_angle = p.max_steer_angle;
+    d.max_steer_rate = p.max_steer_rate;
+    d.brake_delay_constant = p.brake_delay_constant;
+    d.max_brake_rate_neg = p.max_brake_rate_neg;
+    d.max_brake_rate_pos = p.max_brake_rate_pos;
+    d.wheel_base = p.wheel_base;
+    d.gear_sign = p.gear_sign;
+  }
+  RacerDubinsParams getParams() const { return params_; }
+
+ protected:
+  RacerDubinsParams params_;
+};
+
+// Kinematic bicycle (dynamics/racer_dubins/racer_dubins.cu).
+class RacerDubins : public RacerDubinsImpl<MPPI_MODEL_RACER_DUBINS> {};
+
+// Kinematic bicycle + 9-64-64-4 tanh residual MLP on (VEL_X, YAW, STEER_ANGLE, BRAKE_STATE) derivatives, the
+// NeuralNetModel pattern of dynamics/autorally/ar_nn_model.cu with FNNHelper's parameter layout.
+class RacerDubinsNN : public RacerDubinsImpl<MPPI_MODEL_RACER_DUBINS_NN> {
+ public:
+  RacerDubinsNN() : theta_(desc_.nn_num_params, 0.0f) {}
+  // FNNHelper::updateModel(layers, theta) (utils/nn_helpers/fnn_helper.cu:207-246)
+  void updateModel(const std::vector<int>& layers, const std::vector<float>& theta)
+  {
+    if (layers != std::vector<int>{9, 64, 64, 4})
+      throw std::runtime_error("RacerDubinsNN: only the 9-64-64-4 network is compiled into the kernel library");
+    if ((int)theta.size() != desc_.nn_num_params) throw std::runtime_error("RacerDubinsNN: parameter count mismatch");
+    theta_ = theta;
+  }
+  void setPrecision(int precision) { desc_.nn_precision = precision; }
+  const std::vector<float>& getTheta() const { return theta_; }
+  const float* hostParameters() const override { return theta_.data(); }
+
+ private:
+  std::vector<float> theta_;
+};
+
+// ======================================================================================================= costs
+// CartpoleQuadraticCostParams (cost_functions/cartpole/cartpole_quadratic_cost.cuh:10-23)
+struct CartpoleQuadraticCostParams {
+  float cart_position_coeff = 1000;
+  float cart_velocity_coeff = 100;
+  float pole_angle_coeff = 2000;
+  float pole_angular_velocity_coeff = 100;
+  float terminal_cost_coeff = 0;
+  float desired_terminal_state[4] = {0, 0, (float)M_PI, 0};
+  float control_cost_coeff[1] = {10.0f};  // inert on the device path (cost_functions/cost.cuh:205-208)
+};
+
+class CartpoleQuadraticCost {
+ public:
+  using COST_PARAMS_T = CartpoleQuadraticCostParams;
+  explicit CartpoleQuadraticCost(void* /*stream*/ = nullptr) {}
+  void setParams(const CartpoleQuadraticCostParams& p) { params_ = p; }
+  CartpoleQuadraticCostParams getParams() const { return params_; }
+  void GPUSetup() {}
+  void freeCudaMem() {}
+  void bindToStream(void*) {}
+  template <class DYN>
+  mppi_cost_params desc(const DYN& dyn) const
+  {
+    const mppi_dynamics_params d = dyn.desc();
+    mppi_cost_params c;
+    check(mppi_cost_defaults(MPPI_COST_CARTPOLE_QUADRATIC, &d, &c), "mppi_cost_defaults");
+    c.coeff[0] = params_.cart_position_coeff;
+    c.coeff[1] = params_.cart_velocity_coeff;
+    c.coeff[2] = params_.pole_angle_coeff;
+    c.coeff[3] = params_.pole_angular_velocity_coeff;
+    for (int i = 0; i < 4; ++i) c.goal[i] = params_.desired_terminal_state[i];
+    c.terminal_cost_coeff = params_.terminal_cost_coeff;
+    return c;
+  }
+  const float* costMapHost() const { return nullptr; }
+
+ private:
+  CartpoleQuadraticCostParams params_;
+};
+
+// Quadratic goal term (QuadraticCostTrajectoryParams, cost_functions/quadratic_cost/quadratic_cost.cuh:13-63 with a
+// single goal) + bilinear cost map + speed term.
+template <class DYN_T>
+struct QuadraticMapCostParams {
+  float s_coeffs[DYN_T::OUTPUT_DIM];
+  float s_goal[DYN_T::OUTPUT_DIM];
+  float terminal_cost_coeff = 0.0f;
+  float map_coeff = 1.0f;
+  float map_origin[2] = {0.0f, 0.0f};
+  float map_resolution = 1.0f;
+  int map_x_index = 0, map_y_index = 1;
+  float speed_coeff = 0.0f, speed_target = 0.0f;
+  int speed_index = 0;
+  QuadraticMapCostParams()
+  {
+    for (int i = 0; i < DYN_T::OUTPUT_DIM; ++i) {
+      s_coeffs[i] = 1.0f;
+      s_goal[i] = 0.0f;
+    }
+  }
+};
+
+template <class DYN_T>
+class QuadraticMapCost {
+ public:
+  using COST_PARAMS_T = QuadraticMapCostParams<DYN_T>;
+  explicit QuadraticMapCost(void* /*stream*/ = nullptr) {}
+  void setParams(const COST_PARAMS_T& p) { params_ = p; }
+  COST_PARAMS_T getParams() const { return params_; }
+  // A row-major [height][width] FP32 map (the TwoDTextureHelper::updateTexture role, texture_helper.cu:147-222).
+  void setCostMap(const std::vector<float>& map, int width, int height)
+  {
+    if ((int)map.size() != width * height || width < 2 || height < 2)
+      throw std::runtime_error("setCostMap: bad map size");
+    map_ = map;
+    width_ = width;
+    height_ = height;
+  }
+  void GPUSetup() {}
+  void freeCudaMem() {}
+  void bindToStream(void*) {}
+  template <class DYN>
+  mppi_cost_params desc(const DYN& dyn) const
+  {
+    const mppi_dynamics_params d = dyn.desc();
+    mppi_cost_params c;
+    check(mppi_cost_defaults(MPPI_COST_QUADRATIC_MAP, &d, &c), "mppi_cost_defaults");
+    for (int i = 0; i < DYN_T::OUTPUT_DIM; ++i) {
+      c.coeff[i] = params_.s_coeffs[i];
+      c.goal[i] = params_.s_goal[i];
+    }
+    c.terminal_cost_coeff = params_.terminal_cost_coeff;
+    c.map_width = width_;
+    c.map_height = height_;
+    c.map_origin_x = params_.map_origin[0];
+    c.map_origin_y = params_.map_origin[1];
+    c.map_resolution = params_.map_resolution;
+    c.map_coeff = map_.empty() ? 0.0f : params_.map_coeff;
+    c.map_x_index = params_.map_x_index;
+    c.map_y_index = params_.map_y_index;
+    c.speed_coeff = params_.speed_coeff;
+    c.speed_target = params_.speed_target;
+    c.speed_index = params_.speed_index;
+    return c;
+  }
+  const float* costMapHost() const { return map_.empty() ? nullptr : map_.data(); }
+
+ private:
+  COST_PARAMS_T params_;
+  std::vector<float> map_;
+  int width_ = 0, height_ = 0;
+};
+
+// ==================================================================================================== samplers
+namespace sampling_distributions {
+
+// GaussianParamsImpl (sampling_distributions/gaussian/gaussian.cuh:21-42)
+template <int C_DIM>
+struct GaussianParams {
+  static const int CONTROL_DIM = C_DIM;
+  float std_dev[C_DIM];
+  float control_cost_coeff[C_DIM];
+  float pure_noise_trajectories_percentage = 0.01f;
+  float std_dev_decay = 1.0f;
+  dim3 rewrite_controls_block_dim = dim3(32, 16, 1);  // accepted, unused: the engine picks its own launch shapes
+  int sum_strides = 32;                               // accepted, unused
+  GaussianParams()
+  {
+    for (int i = 0; i < C_DIM; ++i) {
+      std_dev[i] = 1.0f;
+      control_cost_coeff[i] = 0.0f;
+    }
+  }
+};
+
+// ColoredNoiseParamsImpl (sampling_distributions/colored_noise/colored_noise.cuh:44-73)
+template <int C_DIM>
+struct ColoredNoiseParams : GaussianParams<C_DIM> {
+  float exponents[C_DIM];
+  float offset_decay_rate = 0.97f;
+  float fmin = 0.0f;
+  int fft_len_mode = MPPI_FFT_LEN_2T;
+  ColoredNoiseParams()
+  {
+    for (int i = 0; i < C_DIM; ++i) exponents[i] = 0.0f;
+  }
+};
+
+template <class PARAMS_T, int KIND>
+class DistributionBase {
+ public:
+  using SAMPLING_PARAMS_T = PARAMS_T;
+  static const int CONTROL_DIM = PARAMS_T::CONTROL_DIM;
+  DistributionBase() = default;
+  explicit DistributionBase(const PARAMS_T& p, void* /*stream*/ = nullptr) : params_(p) {}
+  void setParams(const PARAMS_T& p) { params_ = p; }
+  PARAMS_T getParams() const { return params_; }
+  void GPUSetup() {}
+  void freeCudaMem() {}
+  void bindToStream(void*) {}
+  virtual mppi_sampler_params desc() const
+  {
+    mppi_sampler_params s;
+    check(mppi_sampler_defaults(KIND, CONTROL_DIM, &s), "mppi_sampler_defaults");
+    for (int i = 0; i < CONTROL_DIM; ++i) {
+      s.std_dev[i] = params_.std_dev[i];
+      s.control_cost_coeff[i] = params_.control_cost_coeff[i];
+    }
+    s.pure_noise_trajectories_percentage = params_.pure_noise_trajectories_percentage;
+    s.std_dev_decay = params_.std_dev_decay;
+    return s;
+  }
+  virtual ~DistributionBase() = default;
+
+ protected:
+  PARAMS_T params_;
+};
+
+template <class DYN_PARAMS_T, int C_DIM = 0>
+class GaussianDistributionT;
+
+// GaussianDistribution<DYN_PARAMS_T> (sampling_distributions/gaussian/gaussian.cuh:120-260); the control dimension
+// is taken from the dynamics model the params type belongs to.
+template <int C_DIM>
+class GaussianDistributionC : public DistributionBase<GaussianParams<C_DIM>, MPPI_SAMPLER_GAUSSIAN> {
+ public:
+  using DistributionBase<GaussianParams<C_DIM>, MPPI_SAMPLER_GAUSSIAN>::DistributionBase;
+};
+
+template <int C_DIM>
+class ColoredNoiseDistributionC : public DistributionBase<ColoredNoiseParams<C_DIM>, MPPI_SAMPLER_COLORED> {
+ public:
+  using DistributionBase<ColoredNoiseParams<C_DIM>, MPPI_SAMPLER_COLORED>::DistributionBase;
+  mppi_sampler_params desc() const override
+  {
+    mppi_sampler_params s = DistributionBase<ColoredNoiseParams<C_DIM>, MPPI_SAMPLER_COLORED>::desc();
+    for (int i = 0; i < C_DIM; ++i) s.exponents[i] = this->params_.exponents[i];
+    s.offset_decay_rate = this->params_.offset_decay_rate;
+    s.fmin = this->params_.fmin;
+    s.fft_len_mode = this->params_.fft_len_mode;
+    return s;
+  }
+};
+
+template <class DYN_PARAMS_T>
+struct control_dim_of;
+template <>
+struct control_dim_of<CartpoleDynamicsParams> {
+  static const int value = 1;
+};
+template <>
+struct control_dim_of<DoubleIntegratorParams> {
+  static const int value = 2;
+};
+template <>
+struct control_dim_of<RacerDubinsParams> {
+  static const int value = 2;
+};
+
+template <class DYN_PARAMS_T>
+using GaussianDistribution = GaussianDistributionC<control_dim_of<DYN_PARAMS_T>::value>;
+template <class DYN_PARAMS_T>
+using ColoredNoiseDistribution = ColoredNoiseDistributionC<control_dim_of<DYN_PARAMS_T>::value>;
+
+}  // namespace sampling_distributions
+
+// ==================================================================================================== feedback
+// FB_T is a required template argument of the reference controller (controllers/controller.cuh:85-89); feedback is
+// disabled by default there (:1020). NoFeedback satisfies the slot and carries no computation.
+template <class DYN_T, int FB_TIMESTEPS>
+class NoFeedback {
+ public:
+  static const int TEMPLATED_FB_TIMESTEPS = FB_TIMESTEPS;
+  NoFeedback(DYN_T* = nullptr, float = 0.0f) {}
+};
+
+// ================================================================================================== controller
+struct freeEnergyEstimate {
+  float increase = -1, previousBaseline = -1, freeEnergyMean = -1, freeEnergyVariance = -1,
+        freeEnergyModifiedVariance = -1, normalizerPercent = -1;
+};
+
+struct MPPIFreeEnergyStatistics {
+  int nominal_state_used = 0;
+  freeEnergyEstimate nominal_sys;
+  freeEnergyEstimate real_sys;
+};
+
+// ControllerParams (controllers/controller.cuh:46-68)
+template <int S_DIM, int C_DIM, int MAX_TIMESTEPS>
+struct ControllerParams {
+  float dt_ = 0.02f;
+  float lambda_ = 1.0f;
+  float alpha_ = 0.0f;
+  int num_timesteps_ = MAX_TIMESTEPS;
+  int num_iters_ = 1;
+  unsigned long long seed_ = 0;
+  dim3 dynamics_rollout_dim_;  // accepted, unused
+  dim3 cost_rollout_dim_;      // accepted, unused
+  Mat<float, C_DIM, MAX_TIMESTEPS> init_control_traj_ = Mat<float, C_DIM, MAX_TIMESTEPS>::Zero();
+  Mat<float, C_DIM, 1> slide_control_scale_ = Mat<float, C_DIM, 1>::Zero();
+  bool use_graph_ = true;
+};
+
+template <class DYN_T, class COST_T, class FB_T, int MAX_TIMESTEPS, int NUM_ROLLOUTS,
+          class SAMPLING_T = sampling_distributions::GaussianDistribution<typename DYN_T::DYN_PARAMS_T>,
+          class PARAMS_T = ControllerParams<DYN_T::STATE_DIM, DYN_T::CONTROL_DIM, MAX_TIMESTEPS>>
+class VanillaMPPIController {
+ public:
+  static const int STATE_DIM = DYN_T::STATE_DIM, CONTROL_DIM = DYN_T::CONTROL_DIM;
+  using state_array = typename DYN_T::state_array;
+  using control_array = typename DYN_T::control_array;
+  using control_trajectory = Mat<float, CONTROL_DIM, MAX_TIMESTEPS>;
+  using state_trajectory = Mat<float, STATE_DIM, MAX_TIMESTEPS>;
+
+  // controllers/MPPI/mppi_controller.cuh:36-41
+  VanillaMPPIController(DYN_T* model, COST_T* cost, FB_T* fb_controller, SAMPLING_T* sampler, float dt, int max_iter,
+                        float lambda, float alpha, int num_timesteps = MAX_TIMESTEPS,
+                        const control_trajectory& init_control_traj = control_trajectory::Zero(),
+                        void* /*stream*/ = nullptr)
+    : model_(model), cost_(cost), fb_(fb_controller), sampler_(sampler)
+  {
+    params_.dt_ = dt;
+    params_.num_iters_ = max_iter;
+    params_.lambda_ = lambda;
+    params_.alpha_ = alpha;
+    params_.num_timesteps_ = num_timesteps;
+    params_.init_control_traj_ = init_control_traj;
+    control_ = init_control_traj;
+  }
+  VanillaMPPIController(DYN_T* model, COST_T* cost, FB_T* fb_controller, SAMPLING_T* sampler, const PARAMS_T& params,
+                        void* /*stream*/ = nullptr)
+    : model_(model), cost_(cost), fb_(fb_controller), sampler_(sampler), params_(params)
+  {
+    control_ = params.init_control_traj_;
+  }
+  ~VanillaMPPIController() { release(); }
+  VanillaMPPIController(const VanillaMPPIController&) = delete;
+  VanillaMPPIController& operator=(const VanillaMPPIController&) = delete;
+
+  std::string getControllerName() const { return "Vanilla MPPI"; }
+
+  // VanillaMPPIController::computeControl (controllers/MPPI/mppi_controller.cu:152-237)
+  void computeControl(const state_array& state, int optimization_stride = 1)
+  {
+    ensure();
+    check(mppi_controller_set_control_seq(ctrl_, control_.data()), "updateImportanceSampler");
+    check(mppi_controller_compute_control(ctrl_, state.data(), optimization_stride), "computeControl");
+    check(mppi_controller_get_control_seq(ctrl_, control_.data()), "getControlSeq");
+    check(mppi_controller_get_state_seq(ctrl_, state_.data()), "getTargetStateSeq");
+    mppi_weight_stats st;
+    check(mppi_controller_get_weight_stats(ctrl_, &st), "weights");
+    baseline_ = st.baseline;
+    normalizer_ = (float)st.normalizer;
+    mppi_free_energy fe;
+    check(mppi_controller_get_free_energy(ctrl_, &fe), "free energy");
+    fe_.real_sys.freeEnergyMean = fe.free_energy_mean;
+    fe_.real_sys.freeEnergyVariance = fe.free_energy_variance;
+    fe_.real_sys.freeEnergyModifiedVariance = fe.free_energy_modified_variance;
+    fe_.real_sys.normalizerPercent = fe.normalizer_percent;
+    fe_.real_sys.increase = fe.increase;
+    fe_.real_sys.previousBaseline = fe.previous_baseline;
+  }
+
+  control_trajectory getControlSeq() const { return control_; }
+  state_trajectory getTargetStateSeq() const { return state_; }
+  float getBaselineCost(int = 0) const { return baseline_; }
+  float getNormalizerCost(int = 0) const { return normalizer_; }
+  MPPIFreeEnergyStatistics getFreeEnergyStatistics() const { return fe_; }
+  void updateImportanceSampler(const control_trajectory& u) { control_ = u; }
+  // Controller::slideControlSequence / slideControlSequenceHelper (controllers/controller.cuh:370-376,609-621)
+  void slideControlSequence(int steps)
+  {
+    const int T = params_.num_timesteps_;
+    for (int i = 0; i < T; ++i) {
+      const int ind = std::min(i + steps, T - 1);
+      for (int c = 0; c < CONTROL_DIM; ++c) {
+        float v = control_(c, ind);
+        if (i + steps > T - 1)
+          v = (v - model_->zero_control_[c]) * params_.slide_control_scale_[c] + model_->zero_control_[c];
+        control_(c, i) = v;
+      }
+    }
+  }
+  PARAMS_T getParams() const { return params_; }
+  void setParams(const PARAMS_T& p)
+  {
+    params_ = p;
+    release();
+  }
+  void setSeedCUDARandomNumberGen(unsigned long long seed)
+  {
+    params_.seed_ = seed;
+    if (ctrl_) check(mppi_controller_set_seed(ctrl_, seed), "setSeed");
+  }
+  int getNumTimesteps() const { return params_.num_timesteps_; }
+  float getDt() const { return params_.dt_; }
+  float getLambda() const { return params_.lambda_; }
+  float getAlpha() const { return params_.alpha_; }
+  // plugin parameters changed: the device copies are rebuilt on the next computeControl
+  void resetPlugins() { release(); }
+
+ private:
+  void ensure()
+  {
+    if (ctrl_) return;
+    const int T = params_.num_timesteps_;
+    if (T < 1 || T > MAX_TIMESTEPS) throw std::runtime_error("num_timesteps out of range");
+    mppi_controller_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.dynamics = model_->desc();
+    d.cost = cost_->desc(*model_);
+    d.sampler = sampler_->desc();
+    d.sampler.seed = params_.seed_;
+    d.num_rollouts = NUM_ROLLOUTS;
+    d.num_timesteps = T;
+    d.dt = params_.dt_;
+    d.lambda = params_.lambda_;
+    d.alpha = params_.alpha_;
+    d.num_iters = params_.num_iters_;
+    d.use_graph = params_.use_graph_ ? 1 : 0;
+    for (int c = 0; c < CONTROL_DIM; ++c) d.slide_control_scale[c] = params_.slide_control_scale_[c];
+    d.init_control = control_.data();  // [T][C] row-major == the first T columns of the col-major C x MAX_T matrix
+    d.cost_map_host = cost_->costMapHost();
+    d.nn_theta_host = model_->hostParameters();
+    check(mppi_controller_create(&d, &ctrl_), "mppi_controller_create");
+  }
+  void release()
+  {
+    if (ctrl_) mppi_controller_destroy(ctrl_);
+    ctrl_ = nullptr;
+  }
+
+  DYN_T* model_;
+  COST_T* cost_;
+  FB_T* fb_;
+  SAMPLING_T* sampler_;
+  PARAMS_T params_;
+  mppi_controller* ctrl_ = nullptr;
+  control_trajectory control_;
+  state_trajectory state_;
+  float baseline_ = 0.0f, normalizer_ = 0.0f;
+  MPPIFreeEnergyStatistics fe_;
+};
+
+}  // namespace mppi_amd

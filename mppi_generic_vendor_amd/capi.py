@@ -99,7 +99,7 @@ class ControllerDesc(ct.Structure):
     _fields_ = [
         ("dynamics", DynamicsParams), ("cost", CostParams), ("sampler", SamplerParams),
         ("num_rollouts", _i), ("num_timesteps", _i), ("dt", _f), ("lambda_", _f), ("alpha", _f),
-        ("num_iters", _i), ("use_graph", _i), ("_pad", _i),
+        ("num_iters", _i), ("use_graph", _i), ("keep_samples", _i),
         ("slide_control_scale", _f * MAX_CONTROL),
         ("init_control", ct.c_void_p), ("cost_map_host", ct.c_void_p), ("nn_theta_host", ct.c_void_p),
     ]
@@ -115,6 +115,8 @@ _SIGS = {
     "mppi_sampler_defaults": ([_i, _i, ct.POINTER(SamplerParams)], ct.c_int),
     "mppi_nn_num_params": ([_i, ct.POINTER(_i), ct.POINTER(_i)], ct.c_int),
     "mppi_pure_noise_start": ([_i, _f, ct.POINTER(_i)], ct.c_int),
+    "mppi_host_step": ([ct.POINTER(DynamicsParams), P, P, _f, P, P, P], ct.c_int),
+    "mppi_host_enforce_constraints": ([ct.POINTER(DynamicsParams), P], ct.c_int),
     "mppi_generate_normals": ([ct.c_uint64, ct.c_uint64, P, ct.c_int64, P], ct.c_int),
     "mppi_sample_gaussian": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),
     "mppi_sample_colored": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),

@@ -12,7 +12,7 @@ from .workloads import Problem
 
 class VanillaMPPIController:
     def __init__(self, problem: Problem, num_iters: int = 1, use_graph: bool = True,
-                 init_control: np.ndarray | None = None, slide_control_scale=None):
+                 init_control: np.ndarray | None = None, slide_control_scale=None, keep_samples: bool = False):
         lib = capi.load()
         self.problem = problem
         self.T = problem.num_timesteps
@@ -30,6 +30,7 @@ class VanillaMPPIController:
         d.alpha = problem.alpha
         d.num_iters = num_iters
         d.use_graph = 1 if use_graph else 0
+        d.keep_samples = 1 if keep_samples else 0
         if slide_control_scale is not None:
             for i, v in enumerate(slide_control_scale):
                 d.slide_control_scale[i] = v

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "fused.hpp"
 #include "kernels.hpp"
 #include "models.hpp"
 
@@ -187,6 +188,23 @@ static int host_alloc(Tp** p, size_t bytes)
   return MPPI_OK;
 }
 
+template <int MODEL>
+static void host_step_t(const mppi_dynamics_params& p, const float* x, const float* u_in, float dt, float* xn,
+                        float* xd, float* y)
+{
+  using M = Model<MODEL>;
+  float u[M::C], xdl[M::S], xnl[M::S];
+  for (int i = 0; i < M::C; ++i) u[i] = u_in[i];
+  M::deriv(p, x, u, xdl);
+  M::update(p, x, xnl, xdl, dt);
+  for (int i = 0; i < M::S; ++i) {
+    if (xd) xd[i] = xdl[i];
+    if (xn) xn[i] = xnl[i];
+  }
+  if (y)
+    for (int i = 0; i < M::O; ++i) y[i] = xnl[i];
+}
+
 }  // namespace mppi_amd
 
 using namespace mppi_amd;
@@ -360,6 +378,34 @@ int mppi_pure_noise_start(int32_t num_rollouts, float pct, int32_t* out_index)
   return MPPI_OK;
 }
 
+// ----------------------------------------------------------------------------------- host-side plugin methods
+int mppi_host_step(const mppi_dynamics_params* p, const float* x, const float* u, float dt, float* x_next,
+                   float* x_dot, float* y)
+{
+  if (!p || !x || !u) return fail(MPPI_E_INVALID, "null argument");
+  MPPI_CHECK(validate_dynamics(*p));
+  if (p->model == MPPI_MODEL_RACER_DUBINS_NN && !p->nn_theta)
+    return fail(MPPI_E_INVALID, "residual MLP: nn_theta (host pointer) is null");
+  switch (p->model) {
+    case MPPI_MODEL_CARTPOLE: host_step_t<MPPI_MODEL_CARTPOLE>(*p, x, u, dt, x_next, x_dot, y); break;
+    case MPPI_MODEL_DOUBLE_INTEGRATOR: host_step_t<MPPI_MODEL_DOUBLE_INTEGRATOR>(*p, x, u, dt, x_next, x_dot, y); break;
+    case MPPI_MODEL_RACER_DUBINS: host_step_t<MPPI_MODEL_RACER_DUBINS>(*p, x, u, dt, x_next, x_dot, y); break;
+    case MPPI_MODEL_RACER_DUBINS_NN: host_step_t<MPPI_MODEL_RACER_DUBINS_NN>(*p, x, u, dt, x_next, x_dot, y); break;
+    default: return fail(MPPI_E_UNSUPPORTED, "unknown model");
+  }
+  return MPPI_OK;
+}
+
+int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u)
+{
+  if (!p || !u) return fail(MPPI_E_INVALID, "null argument");
+  MPPI_CHECK(validate_dynamics(*p));
+  if (p->control_dim == 1) enforce_constraints<1>(*p, u);
+  else if (p->control_dim == 2) enforce_constraints<2>(*p, u);
+  else return fail(MPPI_E_UNSUPPORTED, "control_dim");
+  return MPPI_OK;
+}
+
 int mppi_generate_normals(uint64_t seed, uint64_t offset, float* out_d, int64_t n, void* stream)
 {
   if (n < 0 || (n > 0 && !out_d)) return fail(MPPI_E_INVALID, "bad normals buffer");
@@ -476,9 +522,8 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   MPPI_CHECK(dev_alloc(&c->samples_d, sizeof(float) * (size_t)K * T * C));
   MPPI_CHECK(dev_alloc(&c->costs_d, sizeof(float) * K));
   MPPI_CHECK(dev_alloc(&c->stats_d, sizeof(mppi_weight_stats)));
-  const size_t ws = fused_update_workspace_bytes(K, T, C);
+  const size_t ws = fused_iteration_workspace_bytes(K, T, C);
   MPPI_CHECK(dev_alloc(&c->ws_d, ws));
-  MPPI_HIP_CHECK(hipMemset(c->ws_d, 0, ws));
   MPPI_CHECK(dev_alloc(&c->iter_d, sizeof(IterArgs) * desc->num_iters));
   MPPI_CHECK(dev_alloc(&c->loop_iter_d, sizeof(IterArgs)));
   MPPI_CHECK(host_alloc(&c->iter_h, sizeof(IterArgs) * desc->num_iters));
@@ -537,6 +582,19 @@ int mppi_controller_destroy(mppi_controller* ctrl)
   return controller_free(ctrl);
 }
 
+// One MPPI iteration on the controller's stream: colored sampling (if configured) into the sample buffer, then the
+// fused rollout + weights + update (Gaussian samples drawn inside the rollout kernel).
+static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance)
+{
+  const mppi_controller_desc& d = c->d;
+  const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
+  if (colored) MPPI_CHECK(launch_sample_colored(d.sampler, c->K, c->T, c->C, c->mean_d, c->samples_d, ref, nullptr,
+                                                c->stream));
+  return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->K, c->T, d.dt, d.lambda, d.alpha, c->x0_d,
+                                c->mean_d, c->samples_d, !colored, d.keep_samples != 0, ref, c->costs_d, c->ws_d,
+                                c->stats_d, c->mean_out_d, advance, c->stream);
+}
+
 // The work of one computeControl call, enqueued on the controller's stream (captured once when use_graph is set).
 static int controller_enqueue(mppi_controller* c)
 {
@@ -547,14 +605,7 @@ static int controller_enqueue(mppi_controller* c)
   MPPI_HIP_CHECK(hipMemcpyAsync(c->x0_d, c->x0_h, sizeof(float) * S, hipMemcpyHostToDevice, s));
   MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->nominal_h, sizeof(float) * T * C, hipMemcpyHostToDevice, s));
   for (int it = 0; it < d.num_iters; ++it) {
-    IterRef ref{c->iter_d + it, {0, 0, 0}};
-    if (d.sampler.kind == MPPI_SAMPLER_COLORED)
-      MPPI_CHECK(launch_sample_colored(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, s));
-    else
-      MPPI_CHECK(launch_sample_gaussian(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, s));
-    MPPI_CHECK(launch_rollout(d.dynamics, d.cost, d.sampler, K, T, d.dt, d.lambda, d.alpha, c->x0_d, c->mean_d,
-                              c->samples_d, c->costs_d, s));
-    MPPI_CHECK(launch_fused_update(c->costs_d, c->samples_d, K, T, C, d.lambda, c->ws_d, c->stats_d, c->mean_out_d, s));
+    MPPI_CHECK(enqueue_iteration(c, IterRef{c->iter_d + it, {0, 0, 0}}, nullptr));
     if (it + 1 < d.num_iters)
       MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->mean_out_d, sizeof(float) * T * C, hipMemcpyDeviceToDevice, s));
   }
@@ -630,15 +681,7 @@ static int controller_enqueue_loop(mppi_controller* c)
   const hipStream_t s = c->stream;
   const mppi_controller_desc& d = c->d;
   for (int it = 0; it < d.num_iters; ++it) {
-    IterRef ref{c->loop_iter_d, {0, 0, 0}};
-    if (d.sampler.kind == MPPI_SAMPLER_COLORED)
-      MPPI_CHECK(launch_sample_colored(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, s));
-    else
-      MPPI_CHECK(launch_sample_gaussian(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, s));
-    MPPI_CHECK(launch_rollout(d.dynamics, d.cost, d.sampler, K, T, d.dt, d.lambda, d.alpha, c->x0_d, c->mean_d,
-                              c->samples_d, c->costs_d, s));
-    MPPI_CHECK(launch_fused_update(c->costs_d, c->samples_d, K, T, C, d.lambda, c->ws_d, c->stats_d, c->mean_out_d, s,
-                                   c->loop_iter_d));
+    MPPI_CHECK(enqueue_iteration(c, IterRef{c->loop_iter_d, {0, 0, 0}}, c->loop_iter_d));
     MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->mean_out_d, sizeof(float) * T * C, hipMemcpyDeviceToDevice, s));
   }
   return MPPI_OK;

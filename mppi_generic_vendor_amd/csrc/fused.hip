@@ -1,0 +1,299 @@
+// Fused MPPI iteration for the controller path:
+//   rollout_update_kernel  -- per workgroup of ROLLOUT_SPB samples: the control tile is drawn from Philox directly
+//                             into LDS (Gaussian) or loaded from the sample buffer (colored), every lane rolls its
+//                             sample out over the horizon, then the workgroup reduces its clamped tile against a
+//                             local baseline into (b_g, argmin_g, eta_g, sum w^2_g, S_g[T*C]).
+//   combine_kernel         -- log-sum-exp combination of the partials: global baseline (first strict minimum),
+//                             FP64 normalizer, new mean S / eta.
+// The result equals fullGPUcomputeWeights + weightedReductionKernelDeviceNorm (core/mppi_common.cu:746-773,
+// 1067-1089) on the same costs: sum_k exp(-(c_k - b)/lambda) u_k = sum_g exp(-(b_g - b)/lambda) S_g.
+#include <climits>
+
+#include "fused.hpp"
+#include "models.hpp"
+
+namespace mppi_amd {
+
+__device__ __forceinline__ IterArgs load_iter_f(const IterRef& r) { return r.ptr ? *r.ptr : r.val; }
+
+size_t fused_iteration_workspace_bytes(int K, int T, int C)
+{
+  const size_t G = (size_t)(K + ROLLOUT_SPB - 1) / ROLLOUT_SPB;
+  return G * sizeof(Partial) + G * (size_t)T * C * sizeof(float);
+}
+
+template <int MODEL, int COST, bool GEN, bool WRITE>
+__global__ void __launch_bounds__(ROLLOUT_SPB) rollout_update_kernel(
+    mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int K, int T, float dt, float lambda,
+    float alpha, const float* __restrict__ x0, const float* __restrict__ mean, float* __restrict__ samples,
+    IterRef itr, int pstart, float* __restrict__ costs_out, Partial* __restrict__ parts, float* __restrict__ sums)
+{
+  using M = Model<MODEL>;
+  constexpr int S = M::S, C = M::C, O = M::O;
+  extern __shared__ float lds[];
+  const int TC = T * C;
+  const int ROW = TC + 1;
+  float* tile = lds;
+  float* mu = tile + ROLLOUT_SPB * ROW;
+  float* wl = mu + TC;
+  const int lane = threadIdx.x;
+  const int g = blockIdx.x;
+  const int k0 = g * ROLLOUT_SPB;
+  const int nrows = min(ROLLOUT_SPB, K - k0);
+  for (int j = lane; j < TC; j += ROLLOUT_SPB) mu[j] = mean[j];
+  if (GEN) {
+    // u = transform(eps) for the tile's elements; the tile is a contiguous span starting at element k0*T*C, which
+    // is a multiple of 4, so Philox block b covers tile elements 4(b - base) .. 4(b - base) + 3.
+    __syncthreads();
+    const IterArgs a = load_iter_f(itr);
+    const float decay = powf(sp.std_dev_decay, (float)a.iteration);
+    const int64_t base = ((int64_t)k0 * TC) >> 2;
+    const int n = nrows * TC;
+    for (int e0 = lane * 4; e0 < n; e0 += ROLLOUT_SPB * 4) {
+      const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(base + (e0 >> 2)));
+      int r = e0 / TC;
+      int col = e0 - r * TC;
+      int t = col / C;
+      int c = col - t * C;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (e0 + j < n) {
+          const int k = k0 + r;
+          const float m = mu[col];
+          float v;
+          if (k == 0 || t < a.stride) v = m;
+          else if (k >= pstart) v = decay * sp.std_dev[c] * z.z[j];
+          else v = m + decay * sp.std_dev[c] * z.z[j];
+          tile[r * ROW + col] = v;
+        }
+        ++col;
+        if (++c == C) {
+          c = 0;
+          ++t;
+        }
+        if (col == TC) {
+          col = 0;
+          t = 0;
+          c = 0;
+          ++r;
+        }
+      }
+    }
+  } else {
+    const float* src = samples + (size_t)k0 * TC;
+    for (int r = 0; r < nrows; ++r)
+      for (int j = lane; j < TC; j += ROLLOUT_SPB) tile[r * ROW + j] = src[(size_t)r * TC + j];
+  }
+  __syncthreads();
+
+  const int k = k0 + lane;
+  const bool pure = k >= pstart;
+  float x[S], y[O];
+#pragma unroll
+  for (int i = 0; i < S; ++i) x[i] = x0[i];
+#pragma unroll
+  for (int i = 0; i < O; ++i) y[i] = x[i];
+  float* row = tile + lane * ROW;
+  float run = 0.0f;
+  for (int t = 0; t < T; ++t) {
+    float u[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) u[c] = row[t * C + c];
+    enforce_constraints<C>(dyn, u);
+#pragma unroll
+    for (int c = 0; c < C; ++c) row[t * C + c] = u[c];
+    float xd[S], xn[S];
+    M::deriv(dyn, x, u, xd);
+    M::update(dyn, x, xn, xd, dt);
+#pragma unroll
+    for (int i = 0; i < O; ++i) y[i] = xn[i];
+    run += Cost<COST, O>::running(cost, y) + lr_cost<C>(sp, u, mu + t * C, pure, lambda, alpha);
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = xn[i];
+  }
+  const bool valid = lane < nrows;
+  const float my_cost = run / (float)T + Cost<COST, O>::terminal(cost, y) / (float)T;
+  if (valid && costs_out != nullptr) costs_out[k] = my_cost;
+
+  // local weights against the workgroup's own baseline
+  const float c = valid ? sanitize_cost(my_cost) : INFINITY;
+  float b = c;
+  int bi = valid ? k : INT_MAX;
+  wave_argmin(b, bi);
+  const float w = (valid && c <= FLT_MAX) ? expf(-(1.0f / lambda) * (c - b)) : 0.0f;
+  wl[lane] = w;
+  const double e = wave_sum((double)w);
+  const double e2 = wave_sum((double)w * (double)w);
+  __syncthreads();
+  if (WRITE) {
+    float* dst = samples + (size_t)k0 * TC;
+    for (int r = 0; r < nrows; ++r)
+      for (int j = lane; j < TC; j += ROLLOUT_SPB) dst[(size_t)r * TC + j] = tile[r * ROW + j];
+  }
+  for (int j = lane; j < TC; j += ROLLOUT_SPB) {
+    float acc = 0.0f;
+    for (int r = 0; r < nrows; ++r) acc += wl[r] * tile[r * ROW + j];
+    sums[(size_t)g * TC + j] = acc;
+  }
+  if (lane == 0) {
+    Partial p;
+    p.baseline = b;
+    p.argmin = bi;
+    p.eta = e;
+    p.s2 = e2;
+    p._pad = 0.0;
+    parts[g] = p;
+  }
+}
+
+// Combination of G partials: workgroup q owns COMBINE_COLS columns; every workgroup recomputes the global baseline
+// and the scales e_g = exp(-(b_g - b)/lambda) (G <= MAX_PARTIALS values), workgroup 0 also writes the statistics.
+constexpr int COMBINE_COLS = 32;
+constexpr int COMBINE_THREADS = 256;
+constexpr int MAX_PARTIALS = 4096;
+
+__global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial* __restrict__ parts,
+                                                                  const float* __restrict__ sums, int G, int K, int TC,
+                                                                  float lambda, mppi_weight_stats* __restrict__ stats,
+                                                                  float* __restrict__ mean_out, IterArgs* advance)
+{
+  __shared__ float escale[MAX_PARTIALS];
+  __shared__ float sv[COMBINE_THREADS / 64];
+  __shared__ int si[COMBINE_THREADS / 64];
+  __shared__ double se[COMBINE_THREADS / 64], se2[COMBINE_THREADS / 64];
+  __shared__ float acc_l[COMBINE_THREADS / COMBINE_COLS][COMBINE_COLS];
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const float inv = 1.0f / lambda;
+  float v = INFINITY;
+  int idx = INT_MAX;
+  for (int q = tid; q < G; q += COMBINE_THREADS) argmin_combine(v, idx, parts[q].baseline, parts[q].argmin);
+  wave_argmin(v, idx);
+  if (lane == 0) {
+    sv[wid] = v;
+    si[wid] = idx;
+  }
+  __syncthreads();
+  float b = sv[0];
+  int bi = si[0];
+  for (int q = 1; q < COMBINE_THREADS / 64; ++q) argmin_combine(b, bi, sv[q], si[q]);
+  if (bi == INT_MAX) bi = 0;
+  double e = 0.0, e2 = 0.0;
+  for (int q = tid; q < G; q += COMBINE_THREADS) {
+    const Partial p = parts[q];
+    const double sc = (p.baseline <= FLT_MAX) ? exp(-(double)inv * ((double)p.baseline - (double)b)) : 0.0;
+    escale[q] = (float)sc;
+    e += p.eta * sc;
+    e2 += p.s2 * sc * sc;
+  }
+  e = wave_sum(e);
+  e2 = wave_sum(e2);
+  if (lane == 0) {
+    se[wid] = e;
+    se2[wid] = e2;
+  }
+  __syncthreads();
+  double eta = 0.0, s2 = 0.0;
+  for (int q = 0; q < COMBINE_THREADS / 64; ++q) {
+    eta += se[q];
+    s2 += se2[q];
+  }
+  // columns of this workgroup: thread (grp, col) sums partials g = grp, grp + NG, ...
+  constexpr int NG = COMBINE_THREADS / COMBINE_COLS;
+  const int col = tid % COMBINE_COLS, grp = tid / COMBINE_COLS;
+  const int j = blockIdx.x * COMBINE_COLS + col;
+  float acc = 0.0f;
+  if (j < TC)
+    for (int q = grp; q < G; q += NG) acc += escale[q] * sums[(size_t)q * TC + j];
+  acc_l[grp][col] = acc;
+  __syncthreads();
+  if (grp == 0 && j < TC) {
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NG; ++q) s += acc_l[q][col];
+    mean_out[j] = (float)((double)s / eta);
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    mppi_weight_stats st;
+    st.baseline = b;
+    st.argmin = bi;
+    st.normalizer = eta;
+    st.sum_w2 = s2;
+    st.lambda = lambda;
+    st.num_rollouts = K;
+    *stats = st;
+    if (advance != nullptr) advance->rng_offset += 1;
+  }
+}
+
+template <int MODEL, int COST, bool GEN, bool WRITE>
+static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,
+                     int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
+                     float* samples, IterRef it, float* costs_out, Partial* parts, float* sums, hipStream_t s)
+{
+  constexpr int C = Model<MODEL>::C;
+  const size_t lds = sizeof(float) * ((size_t)ROLLOUT_SPB * (T * C + 1) + (size_t)T * C + ROLLOUT_SPB);
+  if (lds > 160 * 1024) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
+  static bool attr_set = false;
+  if (!attr_set) {
+    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
+  const int G = (K + ROLLOUT_SPB - 1) / ROLLOUT_SPB;
+  rollout_update_kernel<MODEL, COST, GEN, WRITE><<<G, ROLLOUT_SPB, lds, s>>>(
+      dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, pstart, costs_out, parts, sums);
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
+template <int MODEL, int COST>
+static int dispatch_flags(bool gen, bool write, const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                          const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
+                          const float* x0, const float* mean, float* samples, IterRef it, float* costs_out,
+                          Partial* parts, float* sums, hipStream_t s)
+{
+  if (gen && write)
+    return launch_ru<MODEL, COST, true, true>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                              costs_out, parts, sums, s);
+  if (gen)
+    return launch_ru<MODEL, COST, true, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                               costs_out, parts, sums, s);
+  if (write)
+    return launch_ru<MODEL, COST, false, true>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                               costs_out, parts, sums, s);
+  return launch_ru<MODEL, COST, false, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                              costs_out, parts, sums, s);
+}
+
+int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                           const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
+                           const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
+                           IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
+                           IterArgs* advance, hipStream_t s)
+{
+  const int C = dyn.control_dim;
+  const int G = (K + ROLLOUT_SPB - 1) / ROLLOUT_SPB;
+  if (G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "fused iteration: too many samples");
+  Partial* parts = static_cast<Partial*>(ws);
+  float* sums = reinterpret_cast<float*>(static_cast<char*>(ws) + (size_t)G * sizeof(Partial));
+  int st = MPPI_E_UNSUPPORTED;
+#define MPPI_FUSED_CASE(MODEL, COST)                                                                            \
+  if (dyn.model == MODEL && cost.cost == COST)                                                                  \
+    st = dispatch_flags<MODEL, COST>(generate, write_samples, dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, \
+                                     samples, it, costs_out, parts, sums, s);
+  MPPI_FUSED_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
+  MPPI_FUSED_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
+  MPPI_FUSED_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)
+  MPPI_FUSED_CASE(MPPI_MODEL_RACER_DUBINS_NN, MPPI_COST_QUADRATIC_MAP)
+#undef MPPI_FUSED_CASE
+  if (st == MPPI_E_UNSUPPORTED)
+    return fail(MPPI_E_UNSUPPORTED, "fused iteration: model/cost combination is not compiled into the kernel library");
+  if (st != MPPI_OK) return st;
+  const int grid = (T * C + COMBINE_COLS - 1) / COMBINE_COLS;
+  combine_kernel<<<grid, COMBINE_THREADS, 0, s>>>(parts, sums, G, K, T * C, lambda, stats, mean_out, advance);
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
+}  // namespace mppi_amd

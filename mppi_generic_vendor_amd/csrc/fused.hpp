@@ -1,0 +1,32 @@
+// Fused iteration (sampling + rollout + local weights + partial update, then a log-sum-exp combine).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mppi_amd {
+
+// Samples per rollout workgroup (one wavefront, one sample per lane).
+constexpr int ROLLOUT_SPB = 64;
+
+// Per-workgroup partial of the weighted update, relative to the workgroup's own baseline.
+struct Partial {
+  float baseline;
+  int argmin;
+  double eta;
+  double s2;
+  double _pad;
+};
+
+size_t fused_iteration_workspace_bytes(int K, int T, int C);
+
+// One MPPI iteration in two launches. With `generate` the Gaussian control samples are drawn inside the rollout
+// kernel (rng offset, stride and iteration from `it`); otherwise they are read from `samples` (e.g. written by the
+// colored sampler). `write_samples` stores the clamped samples back to `samples`; `costs_out` (optional) receives
+// the per-sample costs. `advance` (optional) gets its rng offset incremented by the combine kernel.
+int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                           const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
+                           const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
+                           IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
+                           IterArgs* advance, hipStream_t s);
+
+}  // namespace mppi_amd

@@ -472,138 +472,4 @@ int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const
   return MPPI_OK;
 }
 
-// ========================================================================================== fused weights + update
-constexpr int FUSED_ROWS = 64;
-
-struct FusedPartial {
-  float baseline;
-  int argmin;
-  double eta;
-  double s2;
-  double _pad;
-};
-
-size_t fused_update_workspace_bytes(int K, int T, int C)
-{
-  const size_t G = (size_t)(K + FUSED_ROWS - 1) / FUSED_ROWS;
-  return 256 + G * sizeof(FusedPartial) + G * (size_t)T * C * sizeof(float);
-}
-
-__global__ void __launch_bounds__(256) fused_update_kernel(const float* __restrict__ costs,
-                                                           const float* __restrict__ samples, int K, int TC,
-                                                           float lambda, unsigned* __restrict__ ticket,
-                                                           FusedPartial* __restrict__ parts,
-                                                           float* __restrict__ sums,
-                                                           mppi_weight_stats* __restrict__ stats,
-                                                           float* __restrict__ mean_out, IterArgs* advance)
-{
-  extern __shared__ float escale[];  // [G]
-  __shared__ float w[FUSED_ROWS];
-  __shared__ float sv[4];
-  __shared__ int si[4];
-  __shared__ double se[4], se2[4];
-  __shared__ bool last;
-  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
-  const int G = gridDim.x;
-  const int g = blockIdx.x;
-  const int k0 = g * FUSED_ROWS;
-  const int nr = min(FUSED_ROWS, K - k0);
-  const float inv = 1.0f / lambda;
-  if (wid == 0) {
-    const float c = (lane < nr) ? sanitize_cost(costs[k0 + lane]) : INFINITY;
-    float v = c;
-    int i = (lane < nr) ? k0 + lane : INT_MAX;
-    wave_argmin(v, i);
-    const float wk = (lane < nr && c <= FLT_MAX) ? expf(-inv * (c - v)) : 0.0f;
-    w[lane] = wk;
-    const double e = wave_sum((double)wk);
-    const double e2 = wave_sum((double)wk * (double)wk);
-    if (lane == 0) {
-      FusedPartial p;
-      p.baseline = v;
-      p.argmin = i;
-      p.eta = e;
-      p.s2 = e2;
-      p._pad = 0.0;
-      parts[g] = p;
-    }
-  }
-  __syncthreads();
-  for (int j = tid; j < TC; j += blockDim.x) {
-    float acc = 0.0f;
-    const float* col = samples + (size_t)k0 * TC + j;
-    for (int r = 0; r < nr; ++r) acc += w[r] * col[(size_t)r * TC];
-    sums[(size_t)g * TC + j] = acc;
-  }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) last = (atomicAdd(ticket, 1u) == (unsigned)(G - 1));
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // combine: global first strict minimum, then log-sum-exp rescaling of every workgroup's partials
-  float v = INFINITY;
-  int idx = INT_MAX;
-  for (int q = tid; q < G; q += blockDim.x) argmin_combine(v, idx, parts[q].baseline, parts[q].argmin);
-  wave_argmin(v, idx);
-  if (lane == 0) {
-    sv[wid] = v;
-    si[wid] = idx;
-  }
-  __syncthreads();
-  float b = sv[0];
-  int bi = si[0];
-  for (int q = 1; q < 4; ++q) argmin_combine(b, bi, sv[q], si[q]);
-  if (bi == INT_MAX) bi = 0;
-  double e = 0.0, e2 = 0.0;
-  for (int q = tid; q < G; q += blockDim.x) {
-    const FusedPartial p = parts[q];
-    const double sc = (p.baseline <= FLT_MAX) ? exp(-(double)inv * ((double)p.baseline - (double)b)) : 0.0;
-    escale[q] = (float)sc;
-    e += p.eta * sc;
-    e2 += p.s2 * sc * sc;
-  }
-  e = wave_sum(e);
-  e2 = wave_sum(e2);
-  if (lane == 0) {
-    se[wid] = e;
-    se2[wid] = e2;
-  }
-  __syncthreads();
-  const double eta = se[0] + se[1] + se[2] + se[3];
-  const double s2 = se2[0] + se2[1] + se2[2] + se2[3];
-  for (int j = tid; j < TC; j += blockDim.x) {
-    float acc = 0.0f;
-    for (int q = 0; q < G; ++q) acc += escale[q] * sums[(size_t)q * TC + j];
-    mean_out[j] = (float)((double)acc / eta);
-  }
-  if (tid == 0) {
-    mppi_weight_stats st;
-    st.baseline = b;
-    st.argmin = bi;
-    st.normalizer = eta;
-    st.sum_w2 = s2;
-    st.lambda = lambda;
-    st.num_rollouts = K;
-    *stats = st;
-    *ticket = 0u;
-    if (advance != nullptr) advance->rng_offset += 1;
-  }
-}
-
-int launch_fused_update(const float* costs, const float* samples, int K, int T, int C, float lambda, void* ws,
-                        mppi_weight_stats* stats, float* mean_out, hipStream_t s, IterArgs* advance)
-{
-  const int G = (K + FUSED_ROWS - 1) / FUSED_ROWS;
-  if ((size_t)G * sizeof(float) > 64 * 1024) return fail(MPPI_E_SIZE, "fused update: too many samples");
-  char* base = static_cast<char*>(ws);
-  unsigned* ticket = reinterpret_cast<unsigned*>(base);
-  FusedPartial* parts = reinterpret_cast<FusedPartial*>(base + 256);
-  float* sums = reinterpret_cast<float*>(base + 256 + (size_t)G * sizeof(FusedPartial));
-  fused_update_kernel<<<G, 256, (size_t)G * sizeof(float), s>>>(costs, samples, K, T * C, lambda, ticket, parts,
-                                                                sums, stats, mean_out, advance);
-  MPPI_HIP_CHECK(hipGetLastError());
-  return MPPI_OK;
-}
-
 }  // namespace mppi_amd

@@ -30,11 +30,4 @@ int launch_compute_weights(float* costs, int K, float lambda, mppi_weight_stats*
 int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,
                            float* out, hipStream_t s);
 
-// Weights + update in one launch: every workgroup reduces its block of samples against a local baseline, the last
-// workgroup to finish combines the partials with a log-sum-exp rescaling. `ws` must be zero-initialised once.
-// When `advance` is non-null the finishing workgroup also increments advance->rng_offset (free-running iterations).
-size_t fused_update_workspace_bytes(int K, int T, int C);
-int launch_fused_update(const float* costs, const float* samples, int K, int T, int C, float lambda, void* ws,
-                        mppi_weight_stats* stats, float* mean_out, hipStream_t s, IterArgs* advance = nullptr);
-
 }  // namespace mppi_amd
