@@ -303,7 +303,8 @@ int mppi_controller_set_control_seq(mppi_controller* ctrl, const float* u_host);
 int mppi_controller_slide_control_seq(mppi_controller* ctrl, int32_t steps);
 /* Device pointers to the controller's buffers for inspection: samples [K][T][C] (clamped samples of the last
  * iteration when keep_samples is set; the colored sampler always writes its pre-clamp samples there), costs [K] of
- * the last iteration, and the [T][C] mean the last iteration sampled around. */
+ * the last iteration, and the [T][C] mean (after computeControl: the updated mean, which the next iteration samples
+ * around, as the reference's control_means_d_). */
 int mppi_controller_device_buffers(const mppi_controller* ctrl, float** samples_d, float** costs_d, float** mean_d);
 /* Reseed the generator and reset its counter (controllers/controller.cu:203-209). */
 int mppi_controller_set_seed(mppi_controller* ctrl, uint64_t seed);

@@ -218,7 +218,6 @@ struct mppi_controller {
   float* mean_d = nullptr;
   float* samples_d = nullptr;
   float* costs_d = nullptr;
-  float* mean_out_d = nullptr;
   mppi_weight_stats* stats_d = nullptr;
   void* ws_d = nullptr;
   IterArgs* iter_d = nullptr;
@@ -487,7 +486,7 @@ static int controller_free(mppi_controller* c)
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->loop_exec) (void)hipGraphExecDestroy(c->loop_exec);
   if (c->loop_graph) (void)hipGraphDestroy(c->loop_graph);
-  void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->mean_out_d, c->stats_d, c->ws_d, c->iter_d,
+  void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
                  c->map_d, c->nn_d, c->loop_iter_d};
   for (void* p : dev)
     if (p) (void)hipFree(p);
@@ -518,7 +517,6 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   MPPI_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   MPPI_CHECK(dev_alloc(&c->x0_d, sizeof(float) * MPPI_MAX_STATE));
   MPPI_CHECK(dev_alloc(&c->mean_d, sizeof(float) * T * C));
-  MPPI_CHECK(dev_alloc(&c->mean_out_d, sizeof(float) * T * C));
   MPPI_CHECK(dev_alloc(&c->samples_d, sizeof(float) * (size_t)K * T * C));
   MPPI_CHECK(dev_alloc(&c->costs_d, sizeof(float) * K));
   MPPI_CHECK(dev_alloc(&c->stats_d, sizeof(mppi_weight_stats)));
@@ -590,9 +588,11 @@ static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance)
   const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
   if (colored) MPPI_CHECK(launch_sample_colored(d.sampler, c->K, c->T, c->C, c->mean_d, c->samples_d, ref, nullptr,
                                                 c->stream));
+  // the combine kernel writes the new mean over the sampling mean, as updateDistributionParamsFromDevice writes
+  // control_means_d_ (sampling_distributions/gaussian/gaussian.cu:434-467); the next iteration samples around it
   return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->K, c->T, d.dt, d.lambda, d.alpha, c->x0_d,
                                 c->mean_d, c->samples_d, !colored, d.keep_samples != 0, ref, c->costs_d, c->ws_d,
-                                c->stats_d, c->mean_out_d, advance, c->stream);
+                                c->stats_d, c->mean_d, advance, c->stream);
 }
 
 // The work of one computeControl call, enqueued on the controller's stream (captured once when use_graph is set).
@@ -604,12 +604,8 @@ static int controller_enqueue(mppi_controller* c)
   MPPI_HIP_CHECK(hipMemcpyAsync(c->iter_d, c->iter_h, sizeof(IterArgs) * d.num_iters, hipMemcpyHostToDevice, s));
   MPPI_HIP_CHECK(hipMemcpyAsync(c->x0_d, c->x0_h, sizeof(float) * S, hipMemcpyHostToDevice, s));
   MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->nominal_h, sizeof(float) * T * C, hipMemcpyHostToDevice, s));
-  for (int it = 0; it < d.num_iters; ++it) {
-    MPPI_CHECK(enqueue_iteration(c, IterRef{c->iter_d + it, {0, 0, 0}}, nullptr));
-    if (it + 1 < d.num_iters)
-      MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->mean_out_d, sizeof(float) * T * C, hipMemcpyDeviceToDevice, s));
-  }
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_out_h, c->mean_out_d, sizeof(float) * T * C, hipMemcpyDeviceToHost, s));
+  for (int it = 0; it < d.num_iters; ++it) MPPI_CHECK(enqueue_iteration(c, IterRef{c->iter_d + it, {0, 0, 0}}, nullptr));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_out_h, c->mean_d, sizeof(float) * T * C, hipMemcpyDeviceToHost, s));
   MPPI_HIP_CHECK(hipMemcpyAsync(c->stats_h, c->stats_d, sizeof(mppi_weight_stats), hipMemcpyDeviceToHost, s));
   return MPPI_OK;
 }
@@ -680,10 +676,8 @@ static int controller_enqueue_loop(mppi_controller* c)
   const int K = c->K, T = c->T, C = c->C;
   const hipStream_t s = c->stream;
   const mppi_controller_desc& d = c->d;
-  for (int it = 0; it < d.num_iters; ++it) {
+  for (int it = 0; it < d.num_iters; ++it)
     MPPI_CHECK(enqueue_iteration(c, IterRef{c->loop_iter_d, {0, 0, 0}}, c->loop_iter_d));
-    MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->mean_out_d, sizeof(float) * T * C, hipMemcpyDeviceToDevice, s));
-  }
   return MPPI_OK;
 }
 

@@ -13,6 +13,9 @@
 
 #define MPPI_HD __host__ __device__ __forceinline__
 
+// LDS available to one workgroup on gfx950.
+constexpr size_t LDS_LIMIT = 160 * 1024;
+
 namespace mppi_amd {
 
 // ------------------------------------------------------------------------------------------------ error handling
@@ -70,6 +73,8 @@ struct Normal4 {
   float z[4];
 };
 
+// Box-Muller with the hardware transcendentals: r = sqrt(-2 ln 2 * log2(u1)) (v_log_f32, v_sqrt_f32) and
+// (cos, sin)(2 pi u2) from v_cos_f32 / v_sin_f32, which take their argument in revolutions (u2 in (0, 1]).
 __device__ __forceinline__ Normal4 normals4(uint64_t seed, uint64_t offset, uint64_t block)
 {
   const Philox4 o = philox4x32_10((uint32_t)block, (uint32_t)(block >> 32), (uint32_t)offset,
@@ -79,11 +84,9 @@ __device__ __forceinline__ Normal4 normals4(uint64_t seed, uint64_t offset, uint
   for (int p = 0; p < 2; ++p) {
     const float u1 = u32_to_unit(o.v[2 * p]);
     const float u2 = u32_to_unit(o.v[2 * p + 1]);
-    const float r = sqrtf(-2.0f * logf(u1));
-    float s, c;
-    sincospif(2.0f * u2, &s, &c);
-    n.z[2 * p] = r * c;
-    n.z[2 * p + 1] = r * s;
+    const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    n.z[2 * p] = r * __builtin_amdgcn_cosf(u2);
+    n.z[2 * p + 1] = r * __builtin_amdgcn_sinf(u2);
   }
   return n;
 }

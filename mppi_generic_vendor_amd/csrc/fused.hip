@@ -1,5 +1,5 @@
 // Fused MPPI iteration for the controller path:
-//   rollout_update_kernel  -- per workgroup of ROLLOUT_SPB samples: the control tile is drawn from Philox directly
+//   rollout_update_kernel  -- per workgroup of 64 samples: the control tile is drawn from Philox directly
 //                             into LDS (Gaussian) or loaded from the sample buffer (colored), every lane rolls its
 //                             sample out over the horizon, then the workgroup reduces its clamped tile against a
 //                             local baseline into (b_g, argmin_g, eta_g, sum w^2_g, S_g[T*C]).
@@ -11,6 +11,8 @@
 
 #include "fused.hpp"
 #include "models.hpp"
+#include "rollout_core.hpp"
+#include "tile.hpp"
 
 namespace mppi_amd {
 
@@ -18,29 +20,32 @@ __device__ __forceinline__ IterArgs load_iter_f(const IterRef& r) { return r.ptr
 
 size_t fused_iteration_workspace_bytes(int K, int T, int C)
 {
-  const size_t G = (size_t)(K + ROLLOUT_SPB - 1) / ROLLOUT_SPB;
+  const size_t G = (size_t)(K + RC_LANES - 1) / RC_LANES;
   return G * sizeof(Partial) + G * (size_t)T * C * sizeof(float);
 }
 
 template <int MODEL, int COST, bool GEN, bool WRITE>
-__global__ void __launch_bounds__(ROLLOUT_SPB) rollout_update_kernel(
+__global__ void __launch_bounds__(RC_THREADS) rollout_update_kernel(
     mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int K, int T, float dt, float lambda,
     float alpha, const float* __restrict__ x0, const float* __restrict__ mean, float* __restrict__ samples,
-    IterRef itr, int pstart, float* __restrict__ costs_out, Partial* __restrict__ parts, float* __restrict__ sums)
+    IterRef itr, int pstart, float* __restrict__ costs_out, Partial* __restrict__ parts, float* __restrict__ sums,
+    int chunk)
 {
   using M = Model<MODEL>;
-  constexpr int S = M::S, C = M::C, O = M::O;
+  constexpr int C = M::C;
   extern __shared__ float lds[];
   const int TC = T * C;
   const int ROW = TC + 1;
   float* tile = lds;
-  float* mu = tile + ROLLOUT_SPB * ROW;
-  float* wl = mu + TC;
-  const int lane = threadIdx.x;
+  float* mu = tile + RC_LANES * ROW;
+  float* wl = mu + ((TC + 3) & ~3);   // [64] local weights
+  float* scratch = wl + RC_LANES;     // pipelined-rollout buffers
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const int g = blockIdx.x;
-  const int k0 = g * ROLLOUT_SPB;
-  const int nrows = min(ROLLOUT_SPB, K - k0);
-  for (int j = lane; j < TC; j += ROLLOUT_SPB) mu[j] = mean[j];
+  const int k0 = g * RC_LANES;
+  const int nrows = min(RC_LANES, K - k0);
+  for (int j = tid; j < TC; j += RC_THREADS) mu[j] = mean[j];
   if (GEN) {
     // u = transform(eps) for the tile's elements; the tile is a contiguous span starting at element k0*T*C, which
     // is a multiple of 4, so Philox block b covers tile elements 4(b - base) .. 4(b - base) + 3.
@@ -49,7 +54,7 @@ __global__ void __launch_bounds__(ROLLOUT_SPB) rollout_update_kernel(
     const float decay = powf(sp.std_dev_decay, (float)a.iteration);
     const int64_t base = ((int64_t)k0 * TC) >> 2;
     const int n = nrows * TC;
-    for (int e0 = lane * 4; e0 < n; e0 += ROLLOUT_SPB * 4) {
+    for (int e0 = tid * 4; e0 < n; e0 += RC_THREADS * 4) {
       const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(base + (e0 >> 2)));
       int r = e0 / TC;
       int col = e0 - r * TC;
@@ -80,69 +85,42 @@ __global__ void __launch_bounds__(ROLLOUT_SPB) rollout_update_kernel(
       }
     }
   } else {
-    const float* src = samples + (size_t)k0 * TC;
-    for (int r = 0; r < nrows; ++r)
-      for (int j = lane; j < TC; j += ROLLOUT_SPB) tile[r * ROW + j] = src[(size_t)r * TC + j];
+    tile_load<RC_THREADS>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
   }
   __syncthreads();
-
-  const int k = k0 + lane;
-  const bool pure = k >= pstart;
-  float x[S], y[O];
-#pragma unroll
-  for (int i = 0; i < S; ++i) x[i] = x0[i];
-#pragma unroll
-  for (int i = 0; i < O; ++i) y[i] = x[i];
-  float* row = tile + lane * ROW;
-  float run = 0.0f;
-  for (int t = 0; t < T; ++t) {
-    float u[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) u[c] = row[t * C + c];
-    enforce_constraints<C>(dyn, u);
-#pragma unroll
-    for (int c = 0; c < C; ++c) row[t * C + c] = u[c];
-    float xd[S], xn[S];
-    M::deriv(dyn, x, u, xd);
-    M::update(dyn, x, xn, xd, dt);
-#pragma unroll
-    for (int i = 0; i < O; ++i) y[i] = xn[i];
-    run += Cost<COST, O>::running(cost, y) + lr_cost<C>(sp, u, mu + t * C, pure, lambda, alpha);
-#pragma unroll
-    for (int i = 0; i < S; ++i) x[i] = xn[i];
+  float* cost_l;
+  rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
+                            cost_l);
+  if (tid < RC_LANES) {
+    const int k = k0 + tid;
+    const bool valid = lane < nrows;
+    const float my_cost = cost_l[lane];
+    if (valid && costs_out != nullptr) costs_out[k] = my_cost;
+    // local weights against the workgroup's own baseline
+    const float c = valid ? sanitize_cost(my_cost) : INFINITY;
+    float b = c;
+    int bi = valid ? k : INT_MAX;
+    wave_argmin(b, bi);
+    const float w = (valid && c <= FLT_MAX) ? expf(-(1.0f / lambda) * (c - b)) : 0.0f;
+    wl[lane] = w;
+    const double e = wave_sum((double)w);
+    const double e2 = wave_sum((double)w * (double)w);
+    if (lane == 0) {
+      Partial p;
+      p.baseline = b;
+      p.argmin = bi;
+      p.eta = e;
+      p.s2 = e2;
+      p._pad = 0.0;
+      parts[g] = p;
+    }
   }
-  const bool valid = lane < nrows;
-  const float my_cost = run / (float)T + Cost<COST, O>::terminal(cost, y) / (float)T;
-  if (valid && costs_out != nullptr) costs_out[k] = my_cost;
-
-  // local weights against the workgroup's own baseline
-  const float c = valid ? sanitize_cost(my_cost) : INFINITY;
-  float b = c;
-  int bi = valid ? k : INT_MAX;
-  wave_argmin(b, bi);
-  const float w = (valid && c <= FLT_MAX) ? expf(-(1.0f / lambda) * (c - b)) : 0.0f;
-  wl[lane] = w;
-  const double e = wave_sum((double)w);
-  const double e2 = wave_sum((double)w * (double)w);
   __syncthreads();
-  if (WRITE) {
-    float* dst = samples + (size_t)k0 * TC;
-    for (int r = 0; r < nrows; ++r)
-      for (int j = lane; j < TC; j += ROLLOUT_SPB) dst[(size_t)r * TC + j] = tile[r * ROW + j];
-  }
-  for (int j = lane; j < TC; j += ROLLOUT_SPB) {
+  if (WRITE) tile_store<RC_THREADS>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
+  for (int j = tid; j < TC; j += RC_THREADS) {
     float acc = 0.0f;
     for (int r = 0; r < nrows; ++r) acc += wl[r] * tile[r * ROW + j];
     sums[(size_t)g * TC + j] = acc;
-  }
-  if (lane == 0) {
-    Partial p;
-    p.baseline = b;
-    p.argmin = bi;
-    p.eta = e;
-    p.s2 = e2;
-    p._pad = 0.0;
-    parts[g] = p;
   }
 }
 
@@ -162,11 +140,20 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   __shared__ int si[COMBINE_THREADS / 64];
   __shared__ double se[COMBINE_THREADS / 64], se2[COMBINE_THREADS / 64];
   __shared__ float acc_l[COMBINE_THREADS / COMBINE_COLS][COMBINE_COLS];
+  constexpr int PER = MAX_PARTIALS / COMBINE_THREADS;
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
-  const float inv = 1.0f / lambda;
+  const double inv = 1.0 / (double)lambda;
+  Partial mine[PER];
   float v = INFINITY;
   int idx = INT_MAX;
-  for (int q = tid; q < G; q += COMBINE_THREADS) argmin_combine(v, idx, parts[q].baseline, parts[q].argmin);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int q = tid + i * COMBINE_THREADS;
+    if (q < G) {
+      mine[i] = parts[q];
+      argmin_combine(v, idx, mine[i].baseline, mine[i].argmin);
+    }
+  }
   wave_argmin(v, idx);
   if (lane == 0) {
     sv[wid] = v;
@@ -178,12 +165,15 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   for (int q = 1; q < COMBINE_THREADS / 64; ++q) argmin_combine(b, bi, sv[q], si[q]);
   if (bi == INT_MAX) bi = 0;
   double e = 0.0, e2 = 0.0;
-  for (int q = tid; q < G; q += COMBINE_THREADS) {
-    const Partial p = parts[q];
-    const double sc = (p.baseline <= FLT_MAX) ? exp(-(double)inv * ((double)p.baseline - (double)b)) : 0.0;
-    escale[q] = (float)sc;
-    e += p.eta * sc;
-    e2 += p.s2 * sc * sc;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int q = tid + i * COMBINE_THREADS;
+    if (q < G) {
+      const double sc = (mine[i].baseline <= FLT_MAX) ? exp(-inv * ((double)mine[i].baseline - (double)b)) : 0.0;
+      escale[q] = (float)sc;
+      e += mine[i].eta * sc;
+      e2 += mine[i].s2 * sc * sc;
+    }
   }
   e = wave_sum(e);
   e2 = wave_sum(e2);
@@ -202,8 +192,10 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   const int col = tid % COMBINE_COLS, grp = tid / COMBINE_COLS;
   const int j = blockIdx.x * COMBINE_COLS + col;
   float acc = 0.0f;
-  if (j < TC)
+  if (j < TC) {
+#pragma unroll 8
     for (int q = grp; q < G; q += NG) acc += escale[q] * sums[(size_t)q * TC + j];
+  }
   acc_l[grp][col] = acc;
   __syncthreads();
   if (grp == 0 && j < TC) {
@@ -230,19 +222,22 @@ static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& co
                      int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
                      float* samples, IterRef it, float* costs_out, Partial* parts, float* sums, hipStream_t s)
 {
-  constexpr int C = Model<MODEL>::C;
-  const size_t lds = sizeof(float) * ((size_t)ROLLOUT_SPB * (T * C + 1) + (size_t)T * C + ROLLOUT_SPB);
-  if (lds > 160 * 1024) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
+  constexpr int C = Model<MODEL>::C, O = Model<MODEL>::O;
+  const size_t base = (size_t)RC_LANES * (T * C + 1) + (size_t)((T * C + 3) & ~3) + RC_LANES;
+  const size_t limit = LDS_LIMIT / sizeof(float);
+  const int chunk = base < limit ? rollout_chunk<O>(limit - base) : 0;
+  if (chunk == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
+  const size_t lds = sizeof(float) * (base + rollout_scratch_floats<O>(chunk));
   static bool attr_set = false;
   if (!attr_set) {
     MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
     attr_set = true;
   }
   const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
-  const int G = (K + ROLLOUT_SPB - 1) / ROLLOUT_SPB;
-  rollout_update_kernel<MODEL, COST, GEN, WRITE><<<G, ROLLOUT_SPB, lds, s>>>(
-      dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, pstart, costs_out, parts, sums);
+  const int G = (K + RC_LANES - 1) / RC_LANES;
+  rollout_update_kernel<MODEL, COST, GEN, WRITE><<<G, RC_THREADS, lds, s>>>(
+      dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, pstart, costs_out, parts, sums, chunk);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
@@ -273,7 +268,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            IterArgs* advance, hipStream_t s)
 {
   const int C = dyn.control_dim;
-  const int G = (K + ROLLOUT_SPB - 1) / ROLLOUT_SPB;
+  const int G = (K + RC_LANES - 1) / RC_LANES;
   if (G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "fused iteration: too many samples");
   Partial* parts = static_cast<Partial*>(ws);
   float* sums = reinterpret_cast<float*>(static_cast<char*>(ws) + (size_t)G * sizeof(Partial));

@@ -5,9 +5,6 @@
 
 namespace mppi_amd {
 
-// Samples per rollout workgroup (one wavefront, one sample per lane).
-constexpr int ROLLOUT_SPB = 64;
-
 // Per-workgroup partial of the weighted update, relative to the workgroup's own baseline.
 struct Partial {
   float baseline;

@@ -4,6 +4,8 @@
 
 #include "kernels.hpp"
 #include "models.hpp"
+#include "rollout_core.hpp"
+#include "tile.hpp"
 
 namespace mppi_amd {
 
@@ -117,216 +119,36 @@ int launch_sample_gaussian(const mppi_sampler_params& sp, int K, int T, int C, c
   return MPPI_OK;
 }
 
-// ============================================================================================ colored-noise samples
-// ColoredNoiseDistributionImpl::generateSamples (sampling_distributions/colored_noise/colored_noise.cu:286-392) in
-// one kernel: each workgroup owns COLORED_SPB samples (their C series), builds the power-law frequency scales and
-// sigma_c (:292-338), draws or loads the complex spectra [k][c][f] (:343), applies configureFrequencyNoise (:12-37),
-// evaluates the inverse real DFT of length n at t = 0..T-1 and at the offset index, and writes
-// (x[t] - x[stride] decay^t) / (sigma_c n) (:39-56) through the Gaussian transform.
-constexpr int COLORED_SPB = 8;
-
-__global__ void __launch_bounds__(256) sample_colored_kernel(mppi_sampler_params sp, int K, int T, int C, int n,
-                                                             int F, int spb, const float* __restrict__ mean,
-                                                             float* __restrict__ out, IterRef itr,
-                                                             const float* __restrict__ spectra_in, int pstart)
-{
-  extern __shared__ float lds[];
-  float* scale = lds;                        // [F][C]
-  float* ct = scale + F * C;                 // [n]
-  float* st = ct + n;                        // [n]
-  float* sigma = st + n;                     // [C] (padded to 4)
-  float* spec = sigma + 4;                   // [spb*C][F][2]
-  float* xs = spec + (size_t)spb * C * F * 2;  // [spb*C][T+1]
-
-  const IterArgs a = load_iter(itr);
-  const int tid = threadIdx.x;
-  const int k0 = blockIdx.x * spb;
-  const int ns = min(spb, K - k0);
-  const int nser = ns * C;
-  const int stride = min(max(a.stride, 0), n - 1);
-
-  // frequency scales: f_i = i / n, frequencies below the cutoff take the first admissible one
-  const float cutoff = fmaxf(sp.fmin, 1.0f / n);
-  int j0 = 0;
-  while (j0 < F && (j0 / (1.0f * n)) < cutoff) ++j0;
-  for (int i = tid; i < F * C; i += blockDim.x) {
-    const int fi = i / C, c = i - fi * C;
-    const int src = (fi < j0 && j0 < F) ? j0 : fi;
-    scale[i] = powf(src / (1.0f * n), -sp.exponents[c] / 2.0f);
-  }
-  for (int m = tid; m < n; m += blockDim.x) {
-    float s, c;
-    sincospif((float)(2 * m) / (float)n, &s, &c);
-    ct[m] = c;
-    st[m] = s;
-  }
-  __syncthreads();
-  if (tid < C) {
-    float sg = 0.0f;
-    for (int j = 1; j < F - 1; ++j) sg += scale[j * C + tid] * scale[j * C + tid];
-    const float last = scale[(F - 1) * C + tid] * ((1.0f + (n % 2)) / 2.0f);
-    sg += last * last;
-    sigma[tid] = 2.0f * sqrtf(sg) / n;
-  }
-  // spectra of this workgroup's series: global elements [base, base + cnt) of the [K][C][F][2] buffer
-  const int64_t base = (int64_t)k0 * C * F * 2;
-  const int64_t cnt = (int64_t)nser * F * 2;
-  for (int64_t gb = base / 4 + tid; gb * 4 < base + cnt; gb += blockDim.x) {
-    float z[4];
-    if (spectra_in != nullptr) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t g = gb * 4 + j;
-        z[j] = (g >= base && g < base + cnt) ? spectra_in[g] : 0.0f;
-      }
-    } else {
-      const Normal4 nz = normals4(sp.seed, a.rng_offset, (uint64_t)gb);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) z[j] = nz.z[j];
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t g = gb * 4 + j;
-      if (g < base || g >= base + cnt) continue;
-      const int e = (int)(g - base);
-      const int f = (e >> 1) % F;
-      const bool zero_im = (e & 1) && (f == 0 || ((F & 1) && f == F - 1));
-      spec[e] = zero_im ? 0.0f : z[j];
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < nser * F * 2; e += blockDim.x) {
-    const int f = (e >> 1) % F;
-    const int c = ((e >> 1) / F) % C;
-    spec[e] *= scale[f * C + c];
-  }
-  __syncthreads();
-  // inverse real DFT at t = 0..T-1 and at the offset index
-  for (int idx = tid; idx < nser * (T + 1); idx += blockDim.x) {
-    const int sidx = idx / (T + 1);
-    const int o = idx - sidx * (T + 1);
-    const int tt = (o < T) ? o : stride;
-    const float* X = spec + (size_t)sidx * F * 2;
-    float acc = X[0];
-    int m = 0;
-    for (int f = 1; f < F; ++f) {
-      m += tt;
-      if (m >= n) m -= n;
-      const float re = X[2 * f], im = X[2 * f + 1];
-      if (2 * f == n) acc += re * ct[m];
-      else acc += 2.0f * (re * ct[m] - im * st[m]);
-    }
-    xs[idx] = acc;
-  }
-  __syncthreads();
-  const float decay = powf(sp.std_dev_decay, (float)a.iteration);
-  const int TC = T * C;
-  for (int e = tid; e < ns * TC; e += blockDim.x) {
-    const int r = e / TC;
-    const int rem = e - r * TC;
-    const int t = rem / C;
-    const int c = rem - t * C;
-    const int k = k0 + r;
-    const float* xser = xs + (size_t)(r * C + c) * (T + 1);
-    const float dec = (sp.offset_decay_rate == 0.0f) ? 0.0f : powf(sp.offset_decay_rate, (float)t);
-    const float eps = (xser[t] - xser[T] * dec) / (sigma[c] * n);
-    const float sd = decay * sp.std_dev[c];
-    const float mu = mean[t * C + c];
-    float v;
-    if (k == 0 || t < a.stride) v = mu;
-    else if (k >= pstart) v = sd * eps;
-    else v = mu + sd * eps;
-    out[(size_t)k * TC + rem] = v;
-  }
-}
-
-static size_t colored_lds_bytes(int T, int C, int n, int F, int spb)
-{
-  return sizeof(float) * ((size_t)F * C + 2 * n + 4 + (size_t)spb * C * F * 2 + (size_t)spb * C * (T + 1));
-}
-
-int launch_sample_colored(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
-                          IterRef it, const float* spectra_in, hipStream_t s)
-{
-  const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
-  const int n = (sp.fft_len_mode == MPPI_FFT_LEN_T) ? T : 2 * T;
-  const int F = n / 2 + 1;
-  const int spb = (C > 2) ? COLORED_SPB / 2 : COLORED_SPB;
-  const size_t lds = colored_lds_bytes(T, C, n, F, spb);
-  if (lds > 160 * 1024) return fail(MPPI_E_SIZE, "colored sampler: horizon too long for the LDS tile");
-  static bool attr_set = false;
-  if (!attr_set) {
-    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)sample_colored_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_set = true;
-  }
-  const int grid = (K + spb - 1) / spb;
-  sample_colored_kernel<<<grid, 256, lds, s>>>(sp, K, T, C, n, F, spb, mean, samples, it, spectra_in, pstart);
-  MPPI_HIP_CHECK(hipGetLastError());
-  return MPPI_OK;
-}
-
 // ===================================================================================================== rollout
-// rolloutKernel (core/mppi_common.cu:35-153) with one sample per lane. The workgroup's control tile [64][T*C]
-// (a contiguous span of the sample buffer) is staged in LDS with a padded row, each lane steps its sample over the
-// whole horizon without barriers, and the clamped tile is written back with coalesced stores.
-constexpr int ROLLOUT_SPB = 64;
-
+// rolloutKernel (core/mppi_common.cu:35-153): each workgroup stages its control tile [64][T*C] (a contiguous span of
+// the sample buffer) in LDS with padded rows, runs the pipelined rollout of rollout_core.hpp, writes the costs and
+// stores the clamped tile back with 16-byte stores.
 template <int MODEL, int COST>
-__global__ void __launch_bounds__(ROLLOUT_SPB) rollout_kernel(mppi_dynamics_params dyn, mppi_cost_params cost,
-                                                              mppi_sampler_params sp, int K, int T, float dt,
-                                                              float lambda, float alpha,
-                                                              const float* __restrict__ x0,
-                                                              const float* __restrict__ mean,
-                                                              float* __restrict__ samples, float* __restrict__ costs,
-                                                              int pstart)
+__global__ void __launch_bounds__(RC_THREADS) rollout_kernel(mppi_dynamics_params dyn, mppi_cost_params cost,
+                                                             mppi_sampler_params sp, int K, int T, float dt,
+                                                             float lambda, float alpha, const float* __restrict__ x0,
+                                                             const float* __restrict__ mean,
+                                                             float* __restrict__ samples, float* __restrict__ costs,
+                                                             int pstart, int chunk)
 {
-  using M = Model<MODEL>;
-  constexpr int S = M::S, C = M::C, O = M::O;
+  constexpr int C = Model<MODEL>::C;
   extern __shared__ float lds[];
   const int TC = T * C;
   const int ROW = TC + 1;
   float* tile = lds;
-  float* mu = lds + ROLLOUT_SPB * ROW;
-  const int lane = threadIdx.x;
-  const int k0 = blockIdx.x * ROLLOUT_SPB;
-  const int nrows = min(ROLLOUT_SPB, K - k0);
-  const float* src = samples + (size_t)k0 * TC;
-  for (int r = 0; r < nrows; ++r)
-    for (int j = lane; j < TC; j += ROLLOUT_SPB) tile[r * ROW + j] = src[(size_t)r * TC + j];
-  for (int j = lane; j < TC; j += ROLLOUT_SPB) mu[j] = mean[j];
+  float* mu = tile + RC_LANES * ROW;
+  float* scratch = mu + ((TC + 3) & ~3);
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * RC_LANES;
+  const int nrows = min(RC_LANES, K - k0);
+  tile_load<RC_THREADS>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
+  for (int j = tid; j < TC; j += RC_THREADS) mu[j] = mean[j];
   __syncthreads();
-
-  const int k = k0 + lane;
-  const bool pure = k >= pstart;
-  float x[S], y[O];
-#pragma unroll
-  for (int i = 0; i < S; ++i) x[i] = x0[i];
-#pragma unroll
-  for (int i = 0; i < O; ++i) y[i] = x[i];
-  float* row = tile + lane * ROW;
-  float run = 0.0f;
-  for (int t = 0; t < T; ++t) {
-    float u[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) u[c] = row[t * C + c];
-    enforce_constraints<C>(dyn, u);
-#pragma unroll
-    for (int c = 0; c < C; ++c) row[t * C + c] = u[c];
-    float xd[S], xn[S];
-    M::deriv(dyn, x, u, xd);
-    M::update(dyn, x, xn, xd, dt);
-#pragma unroll
-    for (int i = 0; i < O; ++i) y[i] = xn[i];
-    run += Cost<COST, O>::running(cost, y) + lr_cost<C>(sp, u, mu + t * C, pure, lambda, alpha);
-#pragma unroll
-    for (int i = 0; i < S; ++i) x[i] = xn[i];
-  }
-  if (lane < nrows) costs[k] = run / (float)T + Cost<COST, O>::terminal(cost, y) / (float)T;
-  __syncthreads();
-  float* dst = samples + (size_t)k0 * TC;
-  for (int r = 0; r < nrows; ++r)
-    for (int j = lane; j < TC; j += ROLLOUT_SPB) dst[(size_t)r * TC + j] = tile[r * ROW + j];
+  float* cost_l;
+  rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
+                            cost_l);
+  if (tid < nrows) costs[k0 + tid] = cost_l[tid];
+  tile_store<RC_THREADS>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
 }
 
 template <int MODEL, int COST>
@@ -334,19 +156,22 @@ static int launch_rollout_t(const mppi_dynamics_params& dyn, const mppi_cost_par
                             const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                             const float* x0, const float* mean, float* samples, float* costs, hipStream_t s)
 {
-  constexpr int C = Model<MODEL>::C;
-  const size_t lds = sizeof(float) * ((size_t)ROLLOUT_SPB * (T * C + 1) + (size_t)T * C);
-  if (lds > 160 * 1024) return fail(MPPI_E_SIZE, "rollout: horizon too long for the LDS control tile");
+  constexpr int C = Model<MODEL>::C, O = Model<MODEL>::O;
+  const size_t base = (size_t)RC_LANES * (T * C + 1) + (size_t)((T * C + 3) & ~3);
+  const size_t limit = LDS_LIMIT / sizeof(float);
+  const int chunk = base < limit ? rollout_chunk<O>(limit - base) : 0;
+  if (chunk == 0) return fail(MPPI_E_SIZE, "rollout: horizon too long for the LDS control tile");
+  const size_t lds = sizeof(float) * (base + rollout_scratch_floats<O>(chunk));
   static bool attr_set = false;
   if (!attr_set) {
     MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_kernel<MODEL, COST>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
     attr_set = true;
   }
   const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
-  const int grid = (K + ROLLOUT_SPB - 1) / ROLLOUT_SPB;
-  rollout_kernel<MODEL, COST><<<grid, ROLLOUT_SPB, lds, s>>>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean,
-                                                            samples, costs, pstart);
+  const int grid = (K + RC_LANES - 1) / RC_LANES;
+  rollout_kernel<MODEL, COST><<<grid, RC_THREADS, lds, s>>>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean,
+                                                            samples, costs, pstart, chunk);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
@@ -441,25 +266,51 @@ int launch_compute_weights(float* costs, int K, float lambda, mppi_weight_stats*
 
 // ================================================================================================ weighted update
 // weightedReductionKernelDeviceNorm (core/mppi_common.cu:746-773): new_u[j] = sum_k w_k u_k[j] / normalizer, j = t*C+c.
-// A workgroup owns 64 consecutive columns; its 16 wavefronts take every 16th sample and are summed in fixed order.
-__global__ void __launch_bounds__(1024) weighted_update_kernel(const float* __restrict__ w,
-                                                               const mppi_weight_stats* __restrict__ stats,
-                                                               const float* __restrict__ samples, int K, int TC,
-                                                               float* __restrict__ out)
+// A workgroup owns 4 consecutive columns; its 256 threads take every 256th sample (16-byte row reads when the rows
+// are 16-byte aligned) and are combined by a fixed-order LDS tree.
+constexpr int WU_THREADS = 256;
+
+template <bool VEC4>
+__global__ void __launch_bounds__(WU_THREADS) weighted_update_kernel(const float* __restrict__ w,
+                                                                     const mppi_weight_stats* __restrict__ stats,
+                                                                     const float* __restrict__ samples, int K, int TC,
+                                                                     float* __restrict__ out)
 {
-  __shared__ float part[16][64];
-  const int tid = threadIdx.x, col = tid & 63, rg = tid >> 6;
-  const int j = blockIdx.x * 64 + col;
-  float acc = 0.0f;
-  if (j < TC)
-    for (int k = rg; k < K; k += 16) acc += w[k] * samples[(size_t)k * TC + j];
-  part[rg][col] = acc;
+  __shared__ float4 part[WU_THREADS];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * 4;
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 4
+  for (int k = tid; k < K; k += WU_THREADS) {
+    const float wk = w[k];
+    const float* row = samples + (size_t)k * TC + j0;
+    float4 u;
+    if (VEC4) {
+      u = *reinterpret_cast<const float4*>(row);
+    } else {
+      u.x = row[0];
+      u.y = (j0 + 1 < TC) ? row[1] : 0.0f;
+      u.z = (j0 + 2 < TC) ? row[2] : 0.0f;
+      u.w = (j0 + 3 < TC) ? row[3] : 0.0f;
+    }
+    acc.x += wk * u.x;
+    acc.y += wk * u.y;
+    acc.z += wk * u.z;
+    acc.w += wk * u.w;
+  }
+  part[tid] = acc;
   __syncthreads();
-  if (rg == 0 && j < TC) {
-    float s = 0.0f;
-#pragma unroll
-    for (int g = 0; g < 16; ++g) s += part[g][col];
-    out[j] = (float)((double)s / stats->normalizer);
+  for (int s = WU_THREADS / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      float4 a = part[tid], b = part[tid + s];
+      part[tid] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+    }
+    __syncthreads();
+  }
+  if (tid < 4 && j0 + tid < TC) {
+    const float4 r = part[0];
+    const float v = (tid == 0) ? r.x : (tid == 1) ? r.y : (tid == 2) ? r.z : r.w;
+    out[j0 + tid] = (float)((double)v / stats->normalizer);
   }
 }
 
@@ -467,7 +318,11 @@ int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const
                            float* out, hipStream_t s)
 {
   const int TC = T * C;
-  weighted_update_kernel<<<(TC + 63) / 64, 1024, 0, s>>>(w, stats, samples, K, TC, out);
+  const int grid = (TC + 3) / 4;
+  if ((TC % 4) == 0 && (((uintptr_t)samples) & 15) == 0)
+    weighted_update_kernel<true><<<grid, WU_THREADS, 0, s>>>(w, stats, samples, K, TC, out);
+  else
+    weighted_update_kernel<false><<<grid, WU_THREADS, 0, s>>>(w, stats, samples, K, TC, out);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }

@@ -15,6 +15,32 @@ MPPI_HD float normalize_angle(float a)
   return (r <= 0.0f) ? r + pi : r - pi;
 }
 
+// sin and cos of an angle in radians. On the device: the hardware v_sin_f32 / v_cos_f32, which take revolutions,
+// on fract(a / 2 pi) -- the role of normalizeAngle + __sinf/__cosf in the reference's device code
+// (dynamics/cartpole/cartpole_dynamics.cu:89-91); on the host: normalizeAngle + sinf/cosf.
+MPPI_HD void sincos_angle(float a, float* s, float* c)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+  const float rev = __builtin_amdgcn_fractf(a * 0.15915494309189535f);
+  *s = __builtin_amdgcn_sinf(rev);
+  *c = __builtin_amdgcn_cosf(rev);
+#else
+  const float th = normalize_angle(a);
+  *s = sinf(th);
+  *c = cosf(th);
+#endif
+}
+
+// 1 / x: v_rcp_f32 on the device (1 ulp), an IEEE division on the host.
+MPPI_HD float recip(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
 // Dynamics::enforceConstraints (dynamics/dynamics.cu:107-126): deadband, then clamp to the control range.
 // sign() is utils/math_utils.h:744-747 (sign(0) == 1).
 template <int C>
@@ -29,6 +55,14 @@ MPPI_HD void enforce_constraints(const mppi_dynamics_params& p, float* u)
   }
 }
 
+// enforce_constraints without a deadband (all control_deadband == 0): the clamp alone.
+template <int C>
+MPPI_HD void clamp_controls(const mppi_dynamics_params& p, float* u)
+{
+#pragma unroll
+  for (int i = 0; i < C; ++i) u[i] = fminf(fmaxf(p.control_min[i], u[i]), p.control_max[i]);
+}
+
 template <int MODEL>
 struct Model;
 
@@ -38,14 +72,15 @@ struct Model<MPPI_MODEL_CARTPOLE> {
   static constexpr int S = 4, C = 1, O = 4;
   MPPI_HD static void deriv(const mppi_dynamics_params& p, const float* x, const float* u, float* xd)
   {
-    const float th = normalize_angle(x[2]);
-    const float s = sinf(th), c = cosf(th);
+    float s, c;
+    sincos_angle(x[2], &s, &c);
     const float td = x[3], F = u[0];
     const float mc = p.cart_mass, mp = p.pole_mass, l = p.pole_length, g = p.gravity;
+    const float inv_den = recip(mc + mp * s * s);
     xd[0] = x[1];
-    xd[1] = (F + mp * s * (l * td * td + g * c)) / (mc + mp * s * s);
+    xd[1] = (F + mp * s * (l * td * td + g * c)) * inv_den;
     xd[2] = td;
-    xd[3] = (-F * c - mp * l * td * td * c * s - (mc + mp) * g * s) / (l * (mc + mp * s * s));
+    xd[3] = (-F * c - mp * l * td * td * c * s - (mc + mp) * g * s) * (inv_den * recip(l));
   }
   MPPI_HD static void update(const mppi_dynamics_params&, const float* x, float* xn, const float* xd, float dt)
   {
@@ -83,10 +118,11 @@ struct Bicycle {
                   p.max_brake_rate_pos);
     xd[0] = (brake ? 0.0f : p.c_t * u[0] * (float)p.gear_sign) + p.c_b * x[5] * (x[0] >= 0.0f ? -1.0f : 1.0f) -
             p.c_v * x[0] + p.c_0;
-    xd[1] = (x[0] / p.wheel_base) * tanf(x[4] / p.steer_angle_scale);
-    const float yaw = normalize_angle(x[1]);
+    float ss, cs;
+    sincos_angle(x[4] * recip(p.steer_angle_scale), &ss, &cs);
+    xd[1] = (x[0] * recip(p.wheel_base)) * (ss * recip(cs));
     float sy, cy;
-    sincosf(yaw, &sy, &cy);
+    sincos_angle(x[1], &sy, &cy);
     xd[2] = x[0] * cy;
     xd[3] = x[0] * sy;
     xd[4] = fmaxf(fminf((u[1] * p.steer_command_angle_scale - x[4]) * p.steering_constant, p.max_steer_rate),
@@ -171,12 +207,13 @@ struct Model<MPPI_MODEL_RACER_DUBINS_NN> : Bicycle {
 __device__ __forceinline__ float map_query(const mppi_cost_params& c, float px, float py)
 {
   const int W = c.map_width, H = c.map_height;
-  const float nx = ((px - c.map_origin_x) / c.map_resolution) / (float)W;
-  const float ny = ((py - c.map_origin_y) / c.map_resolution) / (float)H;
+  // texel coordinate (p - origin) / resolution - 0.5 (the reference's normalise-by-extent and scale-back cancel);
+  // the reciprocal of the uniform resolution is computed once per kernel
+  const float inv_res = 1.0f / c.map_resolution;
   // clamp to [0, W - 1] as the reference's if/else chain does for every non-NaN value; a NaN query maps to texel 0
   // instead of reaching the index computation
-  const float qx = fminf(fmaxf(nx * (float)W - 0.5f, 0.0f), (float)(W - 1));
-  const float qy = fminf(fmaxf(ny * (float)H - 0.5f, 0.0f), (float)(H - 1));
+  const float qx = fminf(fmaxf((px - c.map_origin_x) * inv_res - 0.5f, 0.0f), (float)(W - 1));
+  const float qy = fminf(fmaxf((py - c.map_origin_y) * inv_res - 0.5f, 0.0f), (float)(H - 1));
   const int x0 = min((int)floorf(qx), W - 2), x1 = x0 + 1;
   const int y0 = min((int)floorf(qy), H - 2), y1 = y0 + 1;
   const float* row0 = c.map + (size_t)y0 * W;
@@ -205,6 +242,11 @@ struct Cost;
 // cost_functions/cartpole/cartpole_quadratic_cost.cu:23-55
 template <int O>
 struct Cost<MPPI_COST_CARTPOLE_QUADRATIC, O> {
+  __device__ __forceinline__ static bool has_map(const mppi_cost_params&) { return false; }
+  __device__ __forceinline__ static float running_nomap(const mppi_cost_params& c, const float* y)
+  {
+    return quad_state<O>(c, y);
+  }
   __device__ __forceinline__ static float running(const mppi_cost_params& c, const float* y) { return quad_state<O>(c, y); }
   __device__ __forceinline__ static float terminal(const mppi_cost_params& c, const float* y)
   {
@@ -216,6 +258,20 @@ struct Cost<MPPI_COST_CARTPOLE_QUADRATIC, O> {
 // (cost_functions/autorally/ar_standard_cost.cu:284-297); terminal = terminal_cost_coeff * quadratic term.
 template <int O>
 struct Cost<MPPI_COST_QUADRATIC_MAP, O> {
+  __device__ __forceinline__ static bool has_map(const mppi_cost_params& c)
+  {
+    return c.map != nullptr && c.map_coeff != 0.0f;
+  }
+  // every term but the cost map (the map term is then added by a separate gather pass)
+  __device__ __forceinline__ static float running_nomap(const mppi_cost_params& c, const float* y)
+  {
+    float s = quad_state<O>(c, y);
+    if (c.speed_coeff != 0.0f) {
+      const float e = y[c.speed_index] - c.speed_target;
+      s += c.speed_coeff * e * e;
+    }
+    return s;
+  }
   __device__ __forceinline__ static float running(const mppi_cost_params& c, const float* y)
   {
     float s = quad_state<O>(c, y);
@@ -242,7 +298,8 @@ __device__ __forceinline__ float lr_cost(const mppi_sampler_params& s, const flo
 #pragma unroll
   for (int i = 0; i < C; ++i) {
     const float m = pure ? 0.0f : mu[i];
-    cost += s.control_cost_coeff[i] * m * (m - 2.0f * u[i]) / (s.std_dev[i] * s.std_dev[i]);
+    // coeff / sigma^2 is uniform and hoisted out of the horizon loop
+    cost += m * (m - 2.0f * u[i]) * (s.control_cost_coeff[i] / (s.std_dev[i] * s.std_dev[i]));
   }
   return 0.5f * lambda * (1.0f - alpha) * cost;
 }

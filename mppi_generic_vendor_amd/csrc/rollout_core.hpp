@@ -1,0 +1,133 @@
+// Rollout of a workgroup's 64 samples over the horizon (rolloutKernel, core/mppi_common.cu:105-152): for t < T read
+// u_t from the LDS control row, enforce constraints, write the clamped u_t back, step the dynamics, accumulate the
+// running cost of the post-step output plus the likelihood-ratio cost; add the terminal cost; divide by T.
+//
+// The work is split across the four wavefronts of the workgroup so that the serial chain only carries the dynamics:
+//   producer  (wavefront 0, one sample per lane) -- the state chain: u_t (prefetched one step ahead), constraints,
+//             write-back of the clamped u_t, dynamics step, and the output y_t stored to a double-buffered LDS chunk
+//             ybuf[t - t0][lane][O];
+//   consumers (wavefronts 1..15) -- the running + likelihood-ratio cost of every (sample, step) of the previous
+//             chunk: thread (w, lane) takes sample `lane` and the steps t0 + w, t0 + w + 15, ...; the cost-map
+//             gathers of these independent items overlap in flight.
+// One barrier per chunk hands a chunk from the producer to the consumers. The per-sample sums of the consumer
+// wavefronts are added in a fixed order, so the result is deterministic.
+#pragma once
+
+#include "models.hpp"
+
+namespace mppi_amd {
+
+constexpr int RC_LANES = 64;      // samples per workgroup (lanes of the producer wavefront)
+constexpr int RC_THREADS = 1024;  // producer + 15 consumer wavefronts (all of them stage the tile)
+constexpr int RC_CONSUMERS = RC_THREADS / 64 - 1;
+
+// LDS floats the pipelined rollout needs besides the control tile: the two output chunks, the consumer partial sums,
+// the terminal costs and the per-sample costs.
+template <int O>
+inline size_t rollout_scratch_floats(int chunk)
+{
+  return 2 * (size_t)chunk * RC_LANES * O + (size_t)RC_CONSUMERS * RC_LANES + 2 * RC_LANES;
+}
+
+// Largest chunk (a power of two, 4..32 steps) whose buffers fit in `avail` floats; 0 if none.
+template <int O>
+inline int rollout_chunk(size_t avail)
+{
+  for (int ch = 32; ch >= 4; ch >>= 1)
+    if (rollout_scratch_floats<O>(ch) <= avail) return ch;
+  return 0;
+}
+
+// Runs the pipelined rollout of the tile; on return (after a barrier) cost_l[r] holds the cost of sample k0 + r for
+// every r < 64 (garbage for r >= nrows). `scratch` provides rollout_scratch_floats<O>(chunk) floats.
+template <int MODEL, int COST>
+__device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                                             const mppi_sampler_params& sp, int T, float dt, float lambda,
+                                             float alpha, const float* __restrict__ x0, float* tile, int ROW,
+                                             const float* mu, int pstart, int k0, float* scratch, int chunk,
+                                             float*& cost_l_out)
+{
+  using M = Model<MODEL>;
+  using CF = Cost<COST, M::O>;
+  constexpr int S = M::S, C = M::C, O = M::O;
+  float* ybuf = scratch;                                  // [2][chunk][64][O]
+  float* part = ybuf + 2 * chunk * RC_LANES * O;          // [3][64]
+  float* term_l = part + RC_CONSUMERS * RC_LANES;         // [64]
+  float* cost_l = term_l + RC_LANES;                      // [64]
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int nch = (T + chunk - 1) / chunk;
+
+  // producer state
+  float x[S], y[O], un[C];
+  float* row = tile + lane * ROW;
+  if (wave == 0) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = x0[i];
+#pragma unroll
+    for (int i = 0; i < O; ++i) y[i] = x[i];
+#pragma unroll
+    for (int c = 0; c < C; ++c) un[c] = row[c];
+  }
+  bool deadband = false;
+#pragma unroll
+  for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
+  // consumer state
+  float acc = 0.0f;
+  const bool pure = (k0 + lane) >= pstart;
+  const float* crow = tile + lane * ROW;
+
+  for (int it = 0; it <= nch; ++it) {
+    if (wave == 0 && it < nch) {
+      const int t0 = it * chunk, t1 = min(T, t0 + chunk);
+      float* yb = ybuf + (it & 1) * chunk * RC_LANES * O + lane * O;
+      for (int t = t0; t < t1; ++t) {
+        float u[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) u[c] = un[c];
+        // prefetch u_{t+1}; at t = T-1 this reads the row's padding / the next LDS words, never used
+#pragma unroll
+        for (int c = 0; c < C; ++c) un[c] = row[(t + 1) * C + c];
+        if (deadband) enforce_constraints<C>(dyn, u);
+        else clamp_controls<C>(dyn, u);
+#pragma unroll
+        for (int c = 0; c < C; ++c) row[t * C + c] = u[c];
+        float xd[S], xn[S];
+        M::deriv(dyn, x, u, xd);
+        M::update(dyn, x, xn, xd, dt);
+#pragma unroll
+        for (int i = 0; i < O; ++i) {
+          y[i] = xn[i];
+          yb[(t - t0) * RC_LANES * O + i] = xn[i];
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i) x[i] = xn[i];
+      }
+      if (it == nch - 1) term_l[lane] = CF::terminal(cost, y);
+    } else if (wave > 0 && it > 0) {
+      const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
+      const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * O + lane * O;
+      for (int t = t0 + wave - 1; t < t1; t += RC_CONSUMERS) {
+        float yy[O], u[C];
+#pragma unroll
+        for (int i = 0; i < O; ++i) yy[i] = yb[(t - t0) * RC_LANES * O + i];
+#pragma unroll
+        for (int c = 0; c < C; ++c) u[c] = crow[t * C + c];
+        acc += CF::running(cost, yy) + lr_cost<C>(sp, u, mu + t * C, pure, lambda, alpha);
+      }
+    }
+    __syncthreads();
+  }
+  if (wave > 0) part[(wave - 1) * RC_LANES + lane] = acc;
+  __syncthreads();
+  if (wave == 0) {
+    float run = 0.0f;
+#pragma unroll
+    for (int w = 0; w < RC_CONSUMERS; ++w) run += part[w * RC_LANES + lane];
+    cost_l[lane] = run / (float)T + term_l[lane] / (float)T;
+  }
+  __syncthreads();
+  cost_l_out = cost_l;
+}
+
+}  // namespace mppi_amd

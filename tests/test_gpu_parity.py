@@ -27,7 +27,11 @@ def test_normals_match_oracle(hip, engine):
     z = gpu_normals(42, 5, n)
     want = ol.normals(42, 5, n)
     assert np.all(np.isfinite(z))
-    np.testing.assert_allclose(z, want, rtol=2e-6, atol=2e-6)
+    err = np.abs(z - want)
+    print(f"[normals] max abs err vs FP64 Box-Muller {err.max():.2e}, max rel {np.max(err / np.maximum(np.abs(want), 1)):.2e}")
+    # identical Philox words and uniforms; the hardware log2/sqrt/sin/cos of the GPU path differ from the FP64
+    # transform by a few FP32 ulps of the result
+    np.testing.assert_allclose(z, want, rtol=1e-5, atol=1e-5)
 
 
 def test_gaussian_sampler_generated_and_fed(hip, engine):
