@@ -27,6 +27,26 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "MPPI control Hz + rollout-steps/sec; learned vs plain cost at equal time"
 HBM_PEAK_GBS = 8000.0
+# kernels behind each timed entry point, as named in rocprofv3 traces
+KERNEL_NAMES = {"sample_colored": ["colored_mfma_kernel"], "sample_gaussian": ["sample_gaussian_kernel"],
+                "rollout": ["rollout_kernel"], "weights": ["weights_kernel"],
+                "weighted_update": ["update_partial_kernel", "update_columns_kernel"]}
+
+
+def measured_traffic(workload: str, entry: str):
+    """HBM bytes per launch of `entry` from the committed rocprofv3 counter summary of this workload (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy of known size), or None when there is none."""
+    path = ROOT / "profiles" / f"mppi_iteration_traffic_{workload}.json"
+    if not path.exists():
+        return None
+    tab = json.loads(path.read_text())["traffic"]
+    total = 0
+    for prefix in KERNEL_NAMES[entry]:
+        hits = [v for k, v in tab.items() if k.startswith(prefix + "<") or k == prefix]
+        if len(hits) != 1:
+            return None
+        total += hits[0]["total_bytes"]
+    return total
 
 
 def build_problem(name: str):
@@ -85,22 +105,20 @@ def time_kernels(prob, reps: int):
         capi.call("mppi_weighted_update", costs.ptr, stats.ptr, samples.ptr, K, T, C, out.ptr, s.ptr)
 
     res = {}
+    sample(0)
     for name, fn in ((sampler_name, sample), ("rollout", rollout), ("weights", weights), ("weighted_update", update)):
+        # back-to-back launches between two events on the launching stream (the rollout's clamping is idempotent,
+        # so repeated rollouts of one sample buffer do identical work)
         for i in range(3):
-            sample(i) if name != sampler_name else None
             fn(i)
         s.synchronize()
         e0, e1 = Event(), Event()
-        total = 0.0
+        e0.record(s)
         for i in range(reps):
-            if name == "rollout":  # the rollout clamps its input in place: time it on fresh samples each rep
-                sample(i)
-            e0.record(s)
             fn(i)
-            e1.record(s)
-            e1.synchronize()
-            total += e0.elapsed_ms(e1)
-        res[name] = total / reps
+        e1.record(s)
+        e1.synchronize()
+        res[name] = e0.elapsed_ms(e1) / reps
     return res
 
 
@@ -168,6 +186,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibration-mib", type=int, default=256,
+                    help="device-to-device copy of this size, a plain copy kernel of known traffic for counter scaling")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -223,6 +243,14 @@ def main():
         raise SystemExit("benchmark iteration produced an invalid normalizer")
     ctrl.close()
 
+    if args.calibration_mib > 0:
+        # a plain copy kernel of known size (reads and writes calibration_mib MiB), for scaling HBM counters
+        nb = args.calibration_mib << 20
+        a_buf, b_buf = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
+        hip.check(hip._load().hipMemcpy(b_buf.ptr, a_buf.ptr, nb, hip.D2D), "calibration copy")
+        hip.synchronize()
+        a_buf.free()
+        b_buf.free()
     kern = time_kernels(prob, args.kernel_reps)
     print(f"[bench] kernel ms {kern}", file=sys.stderr, flush=True)
     # algorithmic bytes per launch (4 B per FP32): sampler writes K*T*C, rollout reads + writes K*T*C and writes K
@@ -234,8 +262,9 @@ def main():
     }
     dom = max((k for k in kern if k != "weights"), key=lambda k: kern[k])
     achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+    roofline = {"kernel": "+".join(KERNEL_NAMES[dom]), "entry_point": "mppi_" + dom, "bound": "hbm",
+                "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dom),
                 "algorithmic_bytes": alg[dom], "avg_ms": round(kern[dom], 5)}
 
     cpu = None

@@ -2,6 +2,8 @@
 // runs whole MPPI iterations (VanillaMPPIController::computeControl, controllers/MPPI/mppi_controller.cu:152-237).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <utility>
 #include <new>
 #include <string>
 #include <vector>
@@ -475,7 +477,27 @@ int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_
   if (!weights_d || !stats_d || !samples_d || !mean_out_d) return fail(MPPI_E_INVALID, "null argument");
   MPPI_CHECK(validate_sizes(K, T));
   if (C < 1 || C > MPPI_MAX_CONTROL) return fail(MPPI_E_INVALID, "control_dim out of range");
-  return launch_weighted_update(weights_d, stats_d, samples_d, K, T, C, mean_out_d, (hipStream_t)stream);
+  // internal workspace of the partial rows, kept per device and grown on demand (a growth allocates, so the first
+  // call at a new maximum size must not be inside a stream capture)
+  static std::mutex mu;
+  static std::vector<std::pair<void*, size_t>> per_dev(64, {nullptr, 0});
+  int dev = 0;
+  MPPI_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= (int)per_dev.size()) return fail(MPPI_E_INVALID, "device index out of range");
+  const size_t need = weighted_update_workspace_bytes(K, T, C);
+  void* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto& slot = per_dev[dev];
+    if (slot.second < need) {
+      if (slot.first) MPPI_HIP_CHECK(hipFree(slot.first));
+      slot = {nullptr, 0};
+      MPPI_CHECK(dev_alloc(&slot.first, need));
+      slot.second = need;
+    }
+    ws = slot.first;
+  }
+  return launch_weighted_update(weights_d, stats_d, samples_d, K, T, C, mean_out_d, ws, (hipStream_t)stream);
 }
 
 // --------------------------------------------------------------------------------------------------- controller

@@ -66,7 +66,7 @@ static Sigma4 colored_sigma(const mppi_sampler_params& sp, int C, int n)
 }
 
 struct ColoredShape {
-  int n, F, Kpad, KP, MT, NS, spw, threads;
+  int n, F, Kpad, KP, MT, MTP, NS, spw, threads;
   size_t lds_floats;
 };
 
@@ -82,8 +82,11 @@ static ColoredShape colored_shape(int T, int C, int mode)
     sh.NS = ns;
     sh.spw = ns / C;
     sh.threads = CTHREADS;
+    // every wavefront owns MPW M-tiles of one N-tile; the basis rows are padded to MPW * MG * 16 (zero rows)
+    const int mg = (CTHREADS / 64) / (ns / 16);
+    sh.MTP = (sh.MT + mg - 1) / mg * mg;
     const size_t tables = 2 * (size_t)sh.n + T + (((size_t)sh.F * C + 3) & ~(size_t)3);
-    sh.lds_floats = tables + (size_t)ns * sh.KP + (size_t)sh.MT * 16 * CKCP;
+    sh.lds_floats = tables + (size_t)ns * sh.KP + (size_t)sh.MTP * 16 * CKCP;
     if (sh.lds_floats * sizeof(float) <= LDS_LIMIT && sh.spw >= 1) break;
   }
   return sh;
@@ -91,7 +94,7 @@ static ColoredShape colored_shape(int T, int C, int mode)
 
 template <int MPW>
 __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_params sp, int K, int T, int C, int n,
-                                                                int F, int MT, int Kpad, int KP, int NS, int spw,
+                                                                int F, int MTP, int Kpad, int KP, int NS, int spw,
                                                            const float* __restrict__ mean, float* __restrict__ out,
                                                            IterRef itr, const float* __restrict__ spectra_in,
                                                            int pstart, Sigma4 sig)
@@ -102,7 +105,7 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   float* dec = st + n;           // [T]
   float* scale = dec + T;        // [F][C]
   float* X = scale + ((F * C + 3) & ~3);  // [NS][KP]   scaled spectra; reused as the output staging [spw][T][C]
-  float* Bc = X + NS * KP;       // [MT*16][CKCP]
+  float* Bc = X + NS * KP;       // [MTP*16][CKCP]
 
   const IterArgs a = load_iter_c(itr);
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -178,7 +181,7 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   for (int kc0 = 0; kc0 < Kpad; kc0 += CKC) {
     __syncthreads();  // spectra staged / previous chunk consumed
     // basis chunk: thread -> (row t, 8 consecutive columns); the phase index advances by t per frequency
-    for (int item = tid; item < MT * 16 * (CKC / 8); item += nthr) {
+    for (int item = tid; item < MTP * 16 * (CKC / 8); item += nthr) {
       const int t = item / (CKC / 8);
       const int cb = (item - t * (CKC / 8)) * 8;
       float* brow = Bc + t * CKCP + cb;
@@ -218,11 +221,8 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
       const float bv = xrow[ks * 4];
 #pragma unroll
       for (int i = 0; i < MPW; ++i) {
-        const int mt = mg + i * MG;
-        if (mt < MT) {
-          const float av = Bc[(mt * 16 + lr) * CKCP + ks * 4 + lk];
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[i], 0, 0, 0);
-        }
+        const float av = Bc[((mg + i * MG) * 16 + lr) * CKCP + ks * 4 + lk];
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[i], 0, 0, 0);
       }
     }
   }
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int t = mt * 16 + lk * 4 + v;
-        if (mt < MT && t < T) {
+        if (t < T) {
           const float eps = acc[i][v] * inv;
           const float mu = mean[t * C + c];
           float u;
@@ -280,7 +280,7 @@ static int launch_colored_t(const mppi_sampler_params& sp, int K, int T, int C, 
   }
   const int grid = (K + sh.spw - 1) / sh.spw;
   colored_mfma_kernel<MPW><<<grid, sh.threads, sh.lds_floats * sizeof(float), s>>>(
-      sp, K, T, C, sh.n, sh.F, sh.MT, sh.Kpad, sh.KP, sh.NS, sh.spw, mean, samples, it, spectra_in, pstart, sig);
+      sp, K, T, C, sh.n, sh.F, sh.MTP, sh.Kpad, sh.KP, sh.NS, sh.spw, mean, samples, it, spectra_in, pstart, sig);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
@@ -294,7 +294,7 @@ int launch_sample_colored(const mppi_sampler_params& sp, int K, int T, int C, co
   if (sh.lds_floats * sizeof(float) > LDS_LIMIT || sh.spw < 1)
     return fail(MPPI_E_SIZE, "colored sampler: horizon too long for the LDS staging");
   const int MG = (CTHREADS / 64) / (sh.NS / 16);
-  switch ((sh.MT + MG - 1) / MG) {
+  switch (sh.MTP / MG) {
 #define MPPI_CMT(M) \
   case M: return launch_colored_t<M>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, sig, s);
     MPPI_CMT(1) MPPI_CMT(2) MPPI_CMT(3) MPPI_CMT(4)

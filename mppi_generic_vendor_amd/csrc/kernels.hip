@@ -266,63 +266,68 @@ int launch_compute_weights(float* costs, int K, float lambda, mppi_weight_stats*
 
 // ================================================================================================ weighted update
 // weightedReductionKernelDeviceNorm (core/mppi_common.cu:746-773): new_u[j] = sum_k w_k u_k[j] / normalizer, j = t*C+c.
-// A workgroup owns 4 consecutive columns; its 256 threads take every 256th sample (16-byte row reads when the rows
-// are 16-byte aligned) and are combined by a fixed-order LDS tree.
+// Two launches over an internal workspace: every workgroup reduces a block of 64 whole rows (contiguous,
+// coalesced reads) into a partial row S_g[T*C]; a column pass adds the partials in fixed order and divides by the
+// normalizer read from the statistics on the device.
+constexpr int WU_ROWS = 64;
 constexpr int WU_THREADS = 256;
+constexpr int WU_COLS = 32;
 
-template <bool VEC4>
-__global__ void __launch_bounds__(WU_THREADS) weighted_update_kernel(const float* __restrict__ w,
-                                                                     const mppi_weight_stats* __restrict__ stats,
-                                                                     const float* __restrict__ samples, int K, int TC,
-                                                                     float* __restrict__ out)
+__global__ void __launch_bounds__(WU_THREADS) update_partial_kernel(const float* __restrict__ w,
+                                                                    const float* __restrict__ samples, int K, int TC,
+                                                                    float* __restrict__ part)
 {
-  __shared__ float4 part[WU_THREADS];
-  const int tid = threadIdx.x;
-  const int j0 = blockIdx.x * 4;
-  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll 4
-  for (int k = tid; k < K; k += WU_THREADS) {
-    const float wk = w[k];
-    const float* row = samples + (size_t)k * TC + j0;
-    float4 u;
-    if (VEC4) {
-      u = *reinterpret_cast<const float4*>(row);
-    } else {
-      u.x = row[0];
-      u.y = (j0 + 1 < TC) ? row[1] : 0.0f;
-      u.z = (j0 + 2 < TC) ? row[2] : 0.0f;
-      u.w = (j0 + 3 < TC) ? row[3] : 0.0f;
-    }
-    acc.x += wk * u.x;
-    acc.y += wk * u.y;
-    acc.z += wk * u.z;
-    acc.w += wk * u.w;
-  }
-  part[tid] = acc;
+  __shared__ float wl[WU_ROWS];
+  const int tid = threadIdx.x, g = blockIdx.x, k0 = g * WU_ROWS;
+  const int nr = min(WU_ROWS, K - k0);
+  if (tid < WU_ROWS) wl[tid] = (tid < nr) ? w[k0 + tid] : 0.0f;
   __syncthreads();
-  for (int s = WU_THREADS / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-      float4 a = part[tid], b = part[tid + s];
-      part[tid] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-    }
-    __syncthreads();
-  }
-  if (tid < 4 && j0 + tid < TC) {
-    const float4 r = part[0];
-    const float v = (tid == 0) ? r.x : (tid == 1) ? r.y : (tid == 2) ? r.z : r.w;
-    out[j0 + tid] = (float)((double)v / stats->normalizer);
+  const float* base = samples + (size_t)k0 * TC;
+  for (int j = tid; j < TC; j += WU_THREADS) {
+    float acc = 0.0f;
+#pragma unroll 8
+    for (int r = 0; r < nr; ++r) acc += wl[r] * base[(size_t)r * TC + j];
+    part[(size_t)g * TC + j] = acc;
   }
 }
 
+__global__ void __launch_bounds__(WU_THREADS) update_columns_kernel(const float* __restrict__ part, int G, int TC,
+                                                                    const mppi_weight_stats* __restrict__ stats,
+                                                                    float* __restrict__ out)
+{
+  constexpr int NG = WU_THREADS / WU_COLS;
+  __shared__ float acc_l[NG][WU_COLS];
+  const int tid = threadIdx.x, col = tid % WU_COLS, grp = tid / WU_COLS;
+  const int j = blockIdx.x * WU_COLS + col;
+  float acc = 0.0f;
+  if (j < TC) {
+#pragma unroll 8
+    for (int q = grp; q < G; q += NG) acc += part[(size_t)q * TC + j];
+  }
+  acc_l[grp][col] = acc;
+  __syncthreads();
+  if (grp == 0 && j < TC) {
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NG; ++q) s += acc_l[q][col];
+    out[j] = (float)((double)s / stats->normalizer);
+  }
+}
+
+size_t weighted_update_workspace_bytes(int K, int T, int C)
+{
+  return (size_t)((K + WU_ROWS - 1) / WU_ROWS) * T * C * sizeof(float);
+}
+
 int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,
-                           float* out, hipStream_t s)
+                           float* out, void* ws, hipStream_t s)
 {
   const int TC = T * C;
-  const int grid = (TC + 3) / 4;
-  if ((TC % 4) == 0 && (((uintptr_t)samples) & 15) == 0)
-    weighted_update_kernel<true><<<grid, WU_THREADS, 0, s>>>(w, stats, samples, K, TC, out);
-  else
-    weighted_update_kernel<false><<<grid, WU_THREADS, 0, s>>>(w, stats, samples, K, TC, out);
+  const int G = (K + WU_ROWS - 1) / WU_ROWS;
+  float* part = static_cast<float*>(ws);
+  update_partial_kernel<<<G, WU_THREADS, 0, s>>>(w, samples, K, TC, part);
+  MPPI_HIP_CHECK(hipGetLastError());
+  update_columns_kernel<<<(TC + WU_COLS - 1) / WU_COLS, WU_THREADS, 0, s>>>(part, G, TC, stats, out);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }

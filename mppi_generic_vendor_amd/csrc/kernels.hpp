@@ -27,7 +27,9 @@ int launch_rollout(const mppi_dynamics_params& dyn, const mppi_cost_params& cost
                    int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
                    float* samples, float* costs, hipStream_t s);
 int launch_compute_weights(float* costs, int K, float lambda, mppi_weight_stats* stats, hipStream_t s);
+// `ws` provides weighted_update_workspace_bytes(K, T, C) bytes of device memory.
+size_t weighted_update_workspace_bytes(int K, int T, int C);
 int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,
-                           float* out, hipStream_t s);
+                           float* out, void* ws, hipStream_t s);
 
 }  // namespace mppi_amd
