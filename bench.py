@@ -27,6 +27,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "MPPI control Hz + rollout-steps/sec; learned vs plain cost at equal time"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# FLOPs of the 9-64-64-4 residual network per sample-step: 2 (9*64 + 64*64 + 64*4)
+NN_FLOP_PER_STEP = 2 * (9 * 64 + 64 * 64 + 64 * 4)
 # kernels behind each timed entry point, as named in rocprofv3 traces
 KERNEL_NAMES = {"sample_colored": ["colored_mfma_kernel"], "sample_gaussian": ["sample_gaussian_kernel"],
                 "rollout": ["rollout_kernel"], "weights": ["weights_kernel"],
@@ -58,7 +61,7 @@ def build_problem(name: str):
     if name == "cartpole":
         return workloads.cartpole(num_rollouts=2048, num_timesteps=100)
     if name == "bicycle_nn":
-        return workloads.bicycle_nn(num_rollouts=16384, num_timesteps=100)
+        return workloads.bicycle_nn(num_rollouts=16384, num_timesteps=100, precision=capi.NN_BF16)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -261,11 +264,21 @@ def main():
         "weighted_update": 4 * K * T * C + 4 * K + 4 * T * C,
     }
     dom = max((k for k in kern if k != "weights"), key=lambda k: kern[k])
-    achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
-    roofline = {"kernel": "+".join(KERNEL_NAMES[dom]), "entry_point": "mppi_" + dom, "bound": "hbm",
-                "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dom),
-                "algorithmic_bytes": alg[dom], "avg_ms": round(kern[dom], 5)}
+    if prob.dyn.model == capi.MODEL_RACER_DUBINS_NN and dom == "rollout":
+        # the network on the matrix cores bounds the rollout: algorithmic FLOPs per launch over its duration
+        flops = NN_FLOP_PER_STEP * K * T
+        achieved = flops / (kern[dom] * 1e-3) / 1e12
+        roofline = {"kernel": "rollout_kernel", "entry_point": "mppi_rollout", "bound": "mfma",
+                    "achieved": round(achieved, 3), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 5),
+                    "traffic": measured_traffic(args.workload, dom), "algorithmic_flops": flops,
+                    "avg_ms": round(kern[dom], 5)}
+    else:
+        achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
+        roofline = {"kernel": "+".join(KERNEL_NAMES[dom]), "entry_point": "mppi_" + dom, "bound": "hbm",
+                    "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dom),
+                    "algorithmic_bytes": alg[dom], "avg_ms": round(kern[dom], 5)}
 
     cpu = None
     if not args.no_cpu:
@@ -282,7 +295,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+bf16" if prob.dyn.model == capi.MODEL_RACER_DUBINS_NN and prob.dyn.nn_precision == capi.NN_BF16
+                 else "f32",
         "data": "synthetic (seeded cost map, Philox noise)",
         "config": {"workload": f"{args.workload}: K={K} samples, T={T}, S={prob.S}, C={C}, dt={prob.dt}, "
                                f"num_iters=1, device-resident graph replay"},

@@ -457,8 +457,6 @@ int mppi_rollout(const mppi_dynamics_params* dyn, const mppi_cost_params* cost, 
   MPPI_CHECK(validate_sampler(*sampler, *dyn));
   if (dyn->model == MPPI_MODEL_RACER_DUBINS_NN && dyn->nn_theta == nullptr)
     return fail(MPPI_E_INVALID, "residual MLP: nn_theta is null");
-  if (dyn->model == MPPI_MODEL_RACER_DUBINS_NN && dyn->nn_precision == MPPI_NN_BF16)
-    return fail(MPPI_E_UNSUPPORTED, "residual MLP: BF16 path not available");
   return launch_rollout(*dyn, *cost, *sampler, K, T, dt, lambda, alpha, x0_d, mean_d, samples_d, costs_d,
                         (hipStream_t)stream);
 }
@@ -530,8 +528,6 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   MPPI_CHECK(validate_sampler(desc->sampler, desc->dynamics));
   if (!(desc->lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
   if (desc->num_iters < 1) return fail(MPPI_E_INVALID, "num_iters must be >= 1");
-  if (desc->dynamics.model == MPPI_MODEL_RACER_DUBINS_NN && desc->dynamics.nn_precision == MPPI_NN_BF16)
-    return fail(MPPI_E_UNSUPPORTED, "residual MLP: BF16 path not available");
   c->S = desc->dynamics.state_dim;
   c->C = desc->dynamics.control_dim;
   c->O = desc->dynamics.output_dim;

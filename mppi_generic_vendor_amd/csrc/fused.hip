@@ -25,7 +25,7 @@ size_t fused_iteration_workspace_bytes(int K, int T, int C)
 }
 
 template <int MODEL, int COST, bool GEN, bool WRITE>
-__global__ void __launch_bounds__(RC_THREADS) rollout_update_kernel(
+__global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kernel(
     mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int K, int T, float dt, float lambda,
     float alpha, const float* __restrict__ x0, const float* __restrict__ mean, float* __restrict__ samples,
     IterRef itr, int pstart, float* __restrict__ costs_out, Partial* __restrict__ parts, float* __restrict__ sums,
@@ -33,6 +33,7 @@ __global__ void __launch_bounds__(RC_THREADS) rollout_update_kernel(
 {
   using M = Model<MODEL>;
   constexpr int C = M::C;
+  constexpr int NT = rollout_threads<MODEL>();
   extern __shared__ float lds[];
   const int TC = T * C;
   const int ROW = TC + 1;
@@ -45,7 +46,7 @@ __global__ void __launch_bounds__(RC_THREADS) rollout_update_kernel(
   const int g = blockIdx.x;
   const int k0 = g * RC_LANES;
   const int nrows = min(RC_LANES, K - k0);
-  for (int j = tid; j < TC; j += RC_THREADS) mu[j] = mean[j];
+  for (int j = tid; j < TC; j += NT) mu[j] = mean[j];
   if (GEN) {
     // u = transform(eps) for the tile's elements; the tile is a contiguous span starting at element k0*T*C, which
     // is a multiple of 4, so Philox block b covers tile elements 4(b - base) .. 4(b - base) + 3.
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(RC_THREADS) rollout_update_kernel(
     const float decay = powf(sp.std_dev_decay, (float)a.iteration);
     const int64_t base = ((int64_t)k0 * TC) >> 2;
     const int n = nrows * TC;
-    for (int e0 = tid * 4; e0 < n; e0 += RC_THREADS * 4) {
+    for (int e0 = tid * 4; e0 < n; e0 += NT * 4) {
       const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(base + (e0 >> 2)));
       int r = e0 / TC;
       int col = e0 - r * TC;
@@ -85,7 +86,7 @@ __global__ void __launch_bounds__(RC_THREADS) rollout_update_kernel(
       }
     }
   } else {
-    tile_load<RC_THREADS>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
+    tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
   }
   __syncthreads();
   float* cost_l;
@@ -116,8 +117,8 @@ __global__ void __launch_bounds__(RC_THREADS) rollout_update_kernel(
     }
   }
   __syncthreads();
-  if (WRITE) tile_store<RC_THREADS>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
-  for (int j = tid; j < TC; j += RC_THREADS) {
+  if (WRITE) tile_store<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
+  for (int j = tid; j < TC; j += NT) {
     float acc = 0.0f;
     for (int r = 0; r < nrows; ++r) acc += wl[r] * tile[r * ROW + j];
     sums[(size_t)g * TC + j] = acc;
@@ -225,9 +226,9 @@ static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& co
   constexpr int C = Model<MODEL>::C, O = Model<MODEL>::O;
   const size_t base = (size_t)RC_LANES * (T * C + 1) + (size_t)((T * C + 3) & ~3) + RC_LANES;
   const size_t limit = LDS_LIMIT / sizeof(float);
-  const int chunk = base < limit ? rollout_chunk<O>(limit - base) : 0;
+  const int chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
   if (chunk == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
-  const size_t lds = sizeof(float) * (base + rollout_scratch_floats<O>(chunk));
+  const size_t lds = sizeof(float) * (base + rollout_scratch_for<MODEL, O>(chunk));
   static bool attr_set = false;
   if (!attr_set) {
     MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE>,
@@ -236,7 +237,7 @@ static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& co
   }
   const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
   const int G = (K + RC_LANES - 1) / RC_LANES;
-  rollout_update_kernel<MODEL, COST, GEN, WRITE><<<G, RC_THREADS, lds, s>>>(
+  rollout_update_kernel<MODEL, COST, GEN, WRITE><<<G, rollout_threads<MODEL>(), lds, s>>>(
       dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, pstart, costs_out, parts, sums, chunk);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
@@ -273,14 +274,18 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
   Partial* parts = static_cast<Partial*>(ws);
   float* sums = reinterpret_cast<float*>(static_cast<char*>(ws) + (size_t)G * sizeof(Partial));
   int st = MPPI_E_UNSUPPORTED;
+  const int model = (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
+                        ? MODEL_RACER_DUBINS_NN_BF16
+                        : dyn.model;
 #define MPPI_FUSED_CASE(MODEL, COST)                                                                            \
-  if (dyn.model == MODEL && cost.cost == COST)                                                                  \
+  if (model == MODEL && cost.cost == COST)                                                                      \
     st = dispatch_flags<MODEL, COST>(generate, write_samples, dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, \
                                      samples, it, costs_out, parts, sums, s);
   MPPI_FUSED_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
   MPPI_FUSED_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
   MPPI_FUSED_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)
   MPPI_FUSED_CASE(MPPI_MODEL_RACER_DUBINS_NN, MPPI_COST_QUADRATIC_MAP)
+  MPPI_FUSED_CASE(MODEL_RACER_DUBINS_NN_BF16, MPPI_COST_QUADRATIC_MAP)
 #undef MPPI_FUSED_CASE
   if (st == MPPI_E_UNSUPPORTED)
     return fail(MPPI_E_UNSUPPORTED, "fused iteration: model/cost combination is not compiled into the kernel library");

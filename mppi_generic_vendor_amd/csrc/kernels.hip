@@ -124,7 +124,7 @@ int launch_sample_gaussian(const mppi_sampler_params& sp, int K, int T, int C, c
 // the sample buffer) in LDS with padded rows, runs the pipelined rollout of rollout_core.hpp, writes the costs and
 // stores the clamped tile back with 16-byte stores.
 template <int MODEL, int COST>
-__global__ void __launch_bounds__(RC_THREADS) rollout_kernel(mppi_dynamics_params dyn, mppi_cost_params cost,
+__global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_kernel(mppi_dynamics_params dyn, mppi_cost_params cost,
                                                              mppi_sampler_params sp, int K, int T, float dt,
                                                              float lambda, float alpha, const float* __restrict__ x0,
                                                              const float* __restrict__ mean,
@@ -132,6 +132,7 @@ __global__ void __launch_bounds__(RC_THREADS) rollout_kernel(mppi_dynamics_param
                                                              int pstart, int chunk)
 {
   constexpr int C = Model<MODEL>::C;
+  constexpr int NT = rollout_threads<MODEL>();
   extern __shared__ float lds[];
   const int TC = T * C;
   const int ROW = TC + 1;
@@ -141,14 +142,14 @@ __global__ void __launch_bounds__(RC_THREADS) rollout_kernel(mppi_dynamics_param
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * RC_LANES;
   const int nrows = min(RC_LANES, K - k0);
-  tile_load<RC_THREADS>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
-  for (int j = tid; j < TC; j += RC_THREADS) mu[j] = mean[j];
+  tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
+  for (int j = tid; j < TC; j += NT) mu[j] = mean[j];
   __syncthreads();
   float* cost_l;
   rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
                             cost_l);
   if (tid < nrows) costs[k0 + tid] = cost_l[tid];
-  tile_store<RC_THREADS>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
+  tile_store<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
 }
 
 template <int MODEL, int COST>
@@ -159,9 +160,9 @@ static int launch_rollout_t(const mppi_dynamics_params& dyn, const mppi_cost_par
   constexpr int C = Model<MODEL>::C, O = Model<MODEL>::O;
   const size_t base = (size_t)RC_LANES * (T * C + 1) + (size_t)((T * C + 3) & ~3);
   const size_t limit = LDS_LIMIT / sizeof(float);
-  const int chunk = base < limit ? rollout_chunk<O>(limit - base) : 0;
+  const int chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
   if (chunk == 0) return fail(MPPI_E_SIZE, "rollout: horizon too long for the LDS control tile");
-  const size_t lds = sizeof(float) * (base + rollout_scratch_floats<O>(chunk));
+  const size_t lds = sizeof(float) * (base + rollout_scratch_for<MODEL, O>(chunk));
   static bool attr_set = false;
   if (!attr_set) {
     MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_kernel<MODEL, COST>,
@@ -170,7 +171,7 @@ static int launch_rollout_t(const mppi_dynamics_params& dyn, const mppi_cost_par
   }
   const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
   const int grid = (K + RC_LANES - 1) / RC_LANES;
-  rollout_kernel<MODEL, COST><<<grid, RC_THREADS, lds, s>>>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean,
+  rollout_kernel<MODEL, COST><<<grid, rollout_threads<MODEL>(), lds, s>>>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean,
                                                             samples, costs, pstart, chunk);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
@@ -180,8 +181,11 @@ int launch_rollout(const mppi_dynamics_params& dyn, const mppi_cost_params& cost
                    int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
                    float* samples, float* costs, hipStream_t s)
 {
+  const int model = (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
+                        ? MODEL_RACER_DUBINS_NN_BF16
+                        : dyn.model;
 #define MPPI_ROLLOUT_CASE(MODEL, COST)                                                                          \
-  if (dyn.model == MODEL && cost.cost == COST)                                                                  \
+  if (model == MODEL && cost.cost == COST)                                                                      \
     return launch_rollout_t<MODEL, COST>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, costs, s);
   MPPI_ROLLOUT_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
   MPPI_ROLLOUT_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_QUADRATIC_MAP)
@@ -189,6 +193,7 @@ int launch_rollout(const mppi_dynamics_params& dyn, const mppi_cost_params& cost
   MPPI_ROLLOUT_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_CARTPOLE_QUADRATIC)
   MPPI_ROLLOUT_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)
   MPPI_ROLLOUT_CASE(MPPI_MODEL_RACER_DUBINS_NN, MPPI_COST_QUADRATIC_MAP)
+  MPPI_ROLLOUT_CASE(MODEL_RACER_DUBINS_NN_BF16, MPPI_COST_QUADRATIC_MAP)
 #undef MPPI_ROLLOUT_CASE
   return fail(MPPI_E_UNSUPPORTED, "rollout: model/cost combination is not compiled into the kernel library");
 }

@@ -14,6 +14,7 @@
 #pragma once
 
 #include "models.hpp"
+#include "nn_bf16.hpp"
 
 namespace mppi_amd {
 
@@ -38,6 +39,141 @@ inline int rollout_chunk(size_t avail)
   return 0;
 }
 
+// Scratch floats a model's rollout needs for `chunk` (the BF16 network also stages its parameters there), and the
+// largest chunk that fits in `avail` floats (0 if none).
+template <int MODEL, int O>
+inline size_t rollout_scratch_for(int chunk)
+{
+  const size_t s = rollout_scratch_floats<O>(chunk);
+  return (MODEL == MODEL_RACER_DUBINS_NN_BF16 && s < (size_t)NN_PARAMS) ? (size_t)NN_PARAMS : s;
+}
+template <int MODEL, int O>
+inline int rollout_chunk_for(size_t avail)
+{
+  for (int ch = 32; ch >= 4; ch >>= 1)
+    if (rollout_scratch_for<MODEL, O>(ch) <= avail) return ch;
+  return 0;
+}
+
+// Workgroup size of the rollout kernels: 16 wavefronts, except for the BF16-network model, whose producers keep
+// their network fragments (about 90 registers) in registers and therefore run 8 wavefronts (2 per SIMD, 256
+// registers each): 4 producer wavefronts of 16 samples and 4 consumers.
+template <int MODEL>
+constexpr int rollout_threads()
+{
+  return MODEL == MODEL_RACER_DUBINS_NN_BF16 ? 512 : RC_THREADS;
+}
+constexpr int NN_THREADS = rollout_threads<MODEL_RACER_DUBINS_NN_BF16>();
+constexpr int NN_PRODUCERS = RC_LANES / 16;
+constexpr int NN_CONSUMERS = NN_THREADS / 64 - NN_PRODUCERS;
+
+// The same pipeline for MODEL_RACER_DUBINS_NN_BF16 (NN_THREADS threads): four producer wavefronts, wavefront w owning
+// samples 16 w .. 16 w + 15 of the tile (lane 16 g + n, every lane group g of a sample carrying its state), each
+// evaluating the network for its 16 samples per step on the matrix cores; the other wavefronts evaluate the costs.
+// `scratch` must hold at least NN_PARAMS floats: the parameters are staged there before the chunk buffers use it.
+template <int COST>
+__device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                                                const mppi_sampler_params& sp, int T, float dt, float lambda,
+                                                float alpha, const float* __restrict__ x0, float* tile, int ROW,
+                                                const float* mu, int pstart, int k0, float* scratch, int chunk,
+                                                float*& cost_l_out)
+{
+  using M = Model<MODEL_RACER_DUBINS_NN_BF16>;
+  using CF = Cost<COST, M::O>;
+  constexpr int S = M::S, C = M::C, O = M::O;
+  static_assert(O == S, "the bicycle's output is its state");
+  float* ybuf = scratch;                                  // [2][chunk][64][O]
+  float* part = ybuf + 2 * chunk * RC_LANES * O;          // [NN_CONSUMERS][64]
+  float* term_l = part + NN_CONSUMERS * RC_LANES;         // [64]
+  float* cost_l = term_l + RC_LANES;                      // [64]
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int nch = (T + chunk - 1) / chunk;
+  const bool producer = wave < NN_PRODUCERS;
+
+  // network parameters: staged in LDS (coalesced), then each producer lane builds its BF16 fragments
+  for (int i = tid; i < NN_PARAMS; i += NN_THREADS) scratch[i] = dyn.nn_theta[i];
+  __syncthreads();
+  NnFrags F;
+  if (producer) nn_load_frags(scratch, lane, F);
+  __syncthreads();
+
+  const int g = lane >> 4;
+  const int r = wave * 16 + (lane & 15);  // the producer lane's sample row in the tile
+  float x[S], un[C];
+  float* row = tile + r * ROW;
+  if (producer) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = x0[i];
+#pragma unroll
+    for (int c = 0; c < C; ++c) un[c] = row[c];
+  }
+  bool deadband = false;
+#pragma unroll
+  for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
+  float acc = 0.0f;
+  const bool pure = (k0 + lane) >= pstart;
+  const float* crow = tile + lane * ROW;
+
+  for (int it = 0; it <= nch; ++it) {
+    if (producer && it < nch) {
+      const int t0 = it * chunk, t1 = min(T, t0 + chunk);
+      float* yb = ybuf + (it & 1) * chunk * RC_LANES * O + r * O;
+      for (int t = t0; t < t1; ++t) {
+        float u[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) u[c] = un[c];
+#pragma unroll
+        for (int c = 0; c < C; ++c) un[c] = row[(t + 1) * C + c];
+        if (deadband) enforce_constraints<C>(dyn, u);
+        else clamp_controls<C>(dyn, u);
+        if (g == 0) {
+#pragma unroll
+          for (int c = 0; c < C; ++c) row[t * C + c] = u[c];
+        }
+        float xd[S], xn[S];
+        M::deriv(dyn, x, u, xd);
+        const f32x4 o = nn_eval_bf16(F, x, u, g);
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+          for (int i = 0; i < NN_OUT; ++i)
+            if (dyn.nn_output_states[i] == s) xd[s] += o[i];
+        M::update(dyn, x, xn, xd, dt);
+        if (g == 0) {
+#pragma unroll
+          for (int i = 0; i < O; ++i) yb[(t - t0) * RC_LANES * O + i] = xn[i];
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i) x[i] = xn[i];
+      }
+      if (it == nch - 1 && g == 0) term_l[r] = CF::terminal(cost, x);
+    } else if (!producer && it > 0) {
+      const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
+      const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * O + lane * O;
+      for (int t = t0 + wave - NN_PRODUCERS; t < t1; t += NN_CONSUMERS) {
+        float yy[O], u[C];
+#pragma unroll
+        for (int i = 0; i < O; ++i) yy[i] = yb[(t - t0) * RC_LANES * O + i];
+#pragma unroll
+        for (int c = 0; c < C; ++c) u[c] = crow[t * C + c];
+        acc += CF::running(cost, yy) + lr_cost<C>(sp, u, mu + t * C, pure, lambda, alpha);
+      }
+    }
+    __syncthreads();
+  }
+  if (!producer) part[(wave - NN_PRODUCERS) * RC_LANES + lane] = acc;
+  __syncthreads();
+  if (wave == 0) {
+    float run = 0.0f;
+#pragma unroll
+    for (int w = 0; w < NN_CONSUMERS; ++w) run += part[w * RC_LANES + lane];
+    cost_l[lane] = run / (float)T + term_l[lane] / (float)T;
+  }
+  __syncthreads();
+  cost_l_out = cost_l;
+}
+
 // Runs the pipelined rollout of the tile; on return (after a barrier) cost_l[r] holds the cost of sample k0 + r for
 // every r < 64 (garbage for r >= nrows). `scratch` provides rollout_scratch_floats<O>(chunk) floats.
 template <int MODEL, int COST>
@@ -47,11 +183,16 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
                                              const float* mu, int pstart, int k0, float* scratch, int chunk,
                                              float*& cost_l_out)
 {
+  if constexpr (MODEL == MODEL_RACER_DUBINS_NN_BF16) {
+    rollout_tile_nn<COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
+                          cost_l_out);
+    return;
+  }
   using M = Model<MODEL>;
   using CF = Cost<COST, M::O>;
   constexpr int S = M::S, C = M::C, O = M::O;
   float* ybuf = scratch;                                  // [2][chunk][64][O]
-  float* part = ybuf + 2 * chunk * RC_LANES * O;          // [3][64]
+  float* part = ybuf + 2 * chunk * RC_LANES * O;          // [15][64]
   float* term_l = part + RC_CONSUMERS * RC_LANES;         // [64]
   float* cost_l = term_l + RC_LANES;                      // [64]
   const int tid = threadIdx.x;

@@ -116,9 +116,25 @@ static void enforce_constraints(const mppi_dynamics_params& p, R* u)
   }
 }
 
-/* FNN forward, utils/nn_helpers/fnn_helper.cu:354-383: acts_{l+1} = tanh(W_l acts_l + b_l), last layer linear. */
+/* Round to the nearest BF16 (ties to even), the conversion the matrix-core path applies to weights and layer
+ * inputs. */
+static float bf16_round(float f)
+{
+  uint32_t b;
+  std::memcpy(&b, &f, 4);
+  if ((b & 0x7f800000u) == 0x7f800000u) return f;
+  b += 0x7fffu + ((b >> 16) & 1u);
+  b &= 0xffff0000u;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+
+/* FNN forward, utils/nn_helpers/fnn_helper.cu:354-383: acts_{l+1} = tanh(W_l acts_l + b_l), last layer linear.
+ * With `bf16`, every weight and every layer input is rounded to BF16 before the products (biases and sums stay in
+ * R): the arithmetic of the BF16 matrix-core network. */
 template <class R>
-static void fnn_forward(int num_layers, const int32_t* layers, const float* theta, const R* in, R* out)
+static void fnn_forward(int num_layers, const int32_t* layers, const float* theta, const R* in, R* out,
+                        bool bf16 = false)
 {
   R a[256], b[256];
   for (int i = 0; i < layers[0]; ++i) a[i] = in[i];
@@ -128,9 +144,11 @@ static void fnn_forward(int num_layers, const int32_t* layers, const float* thet
     const float* W = theta + stride;
     const float* bias = W + nin * nout;
     stride += nin * nout + nout;
+    if (bf16)
+      for (int k = 0; k < nin; ++k) a[k] = (R)bf16_round((float)a[k]);
     for (int j = 0; j < nout; ++j) {
       R s = 0;
-      for (int k = 0; k < nin; ++k) s += (R)W[j * nin + k] * a[k];
+      for (int k = 0; k < nin; ++k) s += (R)(bf16 ? bf16_round(W[j * nin + k]) : W[j * nin + k]) * a[k];
       s += (R)bias[j];
       b[j] = (l + 2 < num_layers) ? std::tanh(s) : s;
     }
@@ -182,7 +200,7 @@ static void compute_state_deriv(const mppi_dynamics_params& p, const R* x, const
         R in[16], out[16];
         for (int i = 0; i < 7; ++i) in[i] = x[i];
         for (int i = 0; i < 2; ++i) in[7 + i] = u[i];
-        fnn_forward<R>(p.nn_num_layers, p.nn_layers, p.nn_theta, in, out);
+        fnn_forward<R>(p.nn_num_layers, p.nn_layers, p.nn_theta, in, out, p.nn_precision == MPPI_NN_BF16);
         for (int i = 0; i < p.nn_layers[p.nn_num_layers - 1]; ++i) xd[p.nn_output_states[i]] += out[i];
       }
       break;
@@ -454,6 +472,14 @@ void oracle_fnn_forward_f64(int num_layers, const int32_t* layers, const float* 
 {
   fnn_forward<double>(num_layers, layers, theta, in, out);
 }
+
+void oracle_fnn_forward_bf16_f64(int num_layers, const int32_t* layers, const float* theta, const double* in,
+                                 double* out)
+{
+  fnn_forward<double>(num_layers, layers, theta, in, out, true);
+}
+
+float oracle_bf16_round(float v) { return bf16_round(v); }
 
 float oracle_map_query_f32(const mppi_cost_params* c, const float* map, float x, float y)
 {

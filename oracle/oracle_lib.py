@@ -34,6 +34,8 @@ _SIGS = {
     "oracle_enforce_constraints_f32": ([ct.POINTER(capi.DynamicsParams), P], None),
     "oracle_fnn_forward_f32": ([_i, P, P, P, P], None),
     "oracle_fnn_forward_f64": ([_i, P, P, P, P], None),
+    "oracle_fnn_forward_bf16_f64": ([_i, P, P, P, P], None),
+    "oracle_bf16_round": ([_f], _f),
     "oracle_map_query_f32": ([ct.POINTER(capi.CostParams), P, _f, _f], _f),
     "oracle_running_cost_f32": ([ct.POINTER(capi.CostParams), P, P], _f),
     "oracle_terminal_cost_f32": ([ct.POINTER(capi.CostParams), P], _f),

@@ -35,6 +35,28 @@ def test_host_step_matches_oracle(engine, oracle):
             np.testing.assert_allclose(xn, xn2[:S], rtol=2e-6, atol=1e-6)
 
 
+def test_host_step_nn_matches_oracle(engine, oracle):
+    """Bicycle + residual network, FP32 and BF16 network arithmetic (the host twin of the matrix-core path)."""
+    from mppi_generic_vendor_amd import capi, workloads
+
+    rng = np.random.default_rng(1)
+    theta = workloads.xavier_mlp([9, 64, 64, 4])
+    for precision in (capi.NN_FP32, capi.NN_BF16):
+        dyn = capi.dynamics_defaults(capi.MODEL_RACER_DUBINS_NN)
+        dyn.nn_precision = precision
+        dyn.nn_theta = theta.ctypes.data
+        for _ in range(20):
+            x = rng.normal(size=7).astype(np.float32)
+            u = rng.uniform(-1, 1, size=2).astype(np.float32)
+            xn = np.zeros(7, np.float32)
+            capi.call("mppi_host_step", ct.byref(dyn), x.ctypes.data, u.ctypes.data, 0.02, xn.ctypes.data, None,
+                      None)
+            xn2, xd2, y2 = (np.zeros(8, np.float32) for _ in range(3))
+            oracle.oracle_step_f32(ct.byref(dyn), x.ctypes.data, u.ctypes.data, 0.02, xn2.ctypes.data,
+                                   xd2.ctypes.data, y2.ctypes.data)
+            np.testing.assert_allclose(xn, xn2[:7], rtol=2e-6, atol=1e-6)
+
+
 @pytest.mark.gpu
 def test_cpp_closed_loop_swing_up(tmp_path, hip, engine):
     exe = PREBUILT if PREBUILT.exists() else build_program("cartpole_closed_loop", tmp_path)

@@ -94,10 +94,18 @@ def _problems():
         "bicycle": lambda: workloads.bicycle_nn(num_rollouts=512, num_timesteps=100, residual=False,
                                                 cost_map=SMALL_MAP),
         "bicycle_nn_fp32": lambda: workloads.bicycle_nn(num_rollouts=512, num_timesteps=100, cost_map=SMALL_MAP),
+        "bicycle_nn_bf16": lambda: workloads.bicycle_nn(num_rollouts=1000, num_timesteps=100, cost_map=SMALL_MAP,
+                                                        precision=capi.NN_BF16),
         "cartpole_ragged": lambda: workloads.cartpole(num_rollouts=100, num_timesteps=7),
         "di_single": lambda: workloads.double_integrator_map(num_rollouts=1, num_timesteps=1, colored=False,
                                                              cost_map=SMALL_MAP),
     }
+
+
+# BF16 network: the oracle evaluates the same BF16-rounded weights and layer inputs in FP32; the GPU's MFMA summation
+# order and its tanh (hardware exp2/rcp, a few 1e-7 absolute) move an activation across a BF16 rounding boundary now
+# and then (one BF16 ulp, 2^-8 relative), hence the looser bar for that case.
+COST_RTOL = {"bicycle_nn_bf16": 1e-3}
 
 
 @pytest.mark.parametrize("name", list(_problems().keys()))
@@ -116,7 +124,12 @@ def test_rollout_matches_oracle(hip, engine, name):
     e_gpu = rel_err(costs_gpu, costs_cpu)
     print(f"[fp64] {name}: gpu-vs-oracle32 {e_gpu:.2e}  gpu-vs-fp64 {rel_err(costs_gpu, costs_64):.2e}  "
           f"oracle32-vs-fp64 {rel_err(costs_cpu, costs_64):.2e}")
-    assert e_gpu < 1e-4
+    if prob.dyn.model == capi.MODEL_RACER_DUBINS_NN and prob.dyn.nn_precision == capi.NN_BF16:
+        full = _problems()[name]()
+        full.dyn.nn_precision = capi.NN_FP32
+        costs_full, _ = ol.rollout_f64(full, samples, mean)
+        print(f"[fp64] {name}: gpu(bf16 network)-vs-fp64(fp32 network) {rel_err(costs_gpu, costs_full):.2e}")
+    assert e_gpu < COST_RTOL.get(name, 1e-4)
     assert int(np.argmin(costs_gpu)) == int(np.argmin(costs_cpu))
 
 
@@ -175,7 +188,7 @@ def _oracle_iteration(prob, samples, mean):
 
 
 @pytest.mark.parametrize("name,graph", [("cartpole", False), ("cartpole", True), ("di_map_colored", True),
-                                        ("bicycle_nn_fp32", False)])
+                                        ("bicycle_nn_fp32", False), ("bicycle_nn_bf16", True)])
 def test_controller_iteration_matches_oracle(hip, engine, name, graph):
     prob = _problems()[name]()
     K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
@@ -202,7 +215,7 @@ def test_controller_iteration_matches_oracle(hip, engine, name, graph):
         err = float(np.max(np.abs(raw - want))) / scale
         print(f"[parity] {name} call {call}: control rel err {err:.2e} baseline {st.baseline:.6g} vs {b:.6g} "
               f"argmin {st.argmin} vs {idx}")
-        assert err < 1e-4
+        assert err < COST_RTOL.get(name, 1e-4)
         assert st.argmin == idx
         assert st.baseline == pytest.approx(b, rel=1e-4)
         # host post-processing: SG smoothing keeps u[0]; the state sequence propagates the returned controls

@@ -219,6 +219,63 @@ def test_fnn_layout_against_numpy(oracle):
         np.testing.assert_allclose(out, a, rtol=1e-12, atol=1e-12)
 
 
+def test_bf16_rounding_known_answers(oracle):
+    """Round to nearest BF16, ties to even: 1 + 2^-8 (halfway, even neighbour 1) -> 1; 1 + 3*2^-8 (halfway, even
+    neighbour 1 + 2^-6) -> 1.015625; pi -> 3.140625; exact BF16 values are fixed points; and agreement with torch's
+    float32 -> bfloat16 conversion on random values."""
+    r = oracle.oracle_bf16_round
+    assert r(1.0 + 2.0 ** -8) == 1.0
+    assert r(1.0 + 3 * 2.0 ** -8) == 1.015625
+    assert r(np.float32(np.pi)) == 3.140625
+    assert r(-2.5) == -2.5 and r(0.0) == 0.0 and r(float("inf")) == float("inf")
+    torch = pytest.importorskip("torch")
+    v = np.random.default_rng(5).normal(scale=3.0, size=2000).astype(np.float32)
+    want = torch.from_numpy(v).to(torch.bfloat16).to(torch.float32).numpy()
+    got = np.array([r(float(x)) for x in v], np.float32)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_fnn_bf16_forward(oracle):
+    """The BF16 network equals the FP64 network when weights and inputs are BF16-exact and the hidden activations
+    are too (all-ones 6-32-32-4 known answer 33 of tests/nn_helpers/fnn_helper_test.cu:403-446: tanh(7) and tanh(33)
+    both round to 1.0); for general weights it equals the FP64 network evaluated on torch-rounded BF16 weights and
+    layer inputs."""
+    L = np.array([6, 32, 32, 4], np.int32)
+    theta = np.ones(1412, dtype=np.float32)
+    out = np.zeros(4)
+    oracle.oracle_fnn_forward_bf16_f64(4, L.ctypes.data, theta.ctypes.data, np.zeros(6).ctypes.data,
+                                       out.ctypes.data)
+    np.testing.assert_array_equal(out, [33, 33, 33, 33])
+    torch = pytest.importorskip("torch")
+
+    def bf(a):
+        return torch.from_numpy(np.asarray(a, np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+
+    layers = [9, 64, 64, 4]
+    theta = workloads.xavier_mlp(layers)
+    L = np.array(layers, np.int32)
+    rng = np.random.default_rng(8)
+    for _ in range(5):
+        x = rng.normal(size=9)
+        a = x
+        off = 0
+        for li, (ni, no) in enumerate(zip(layers[:-1], layers[1:])):
+            W = bf(theta[off:off + ni * no].reshape(no, ni))
+            off += ni * no
+            b = theta[off:off + no].astype(np.float64)
+            off += no
+            a = W @ bf(a) + b
+            if li < 2:
+                a = np.tanh(a)
+        out = np.zeros(4)
+        oracle.oracle_fnn_forward_bf16_f64(4, L.ctypes.data, theta.ctypes.data, x.ctypes.data, out.ctypes.data)
+        np.testing.assert_allclose(out, a, rtol=1e-12, atol=1e-12)
+        # and BF16 rounding is a visible change against the FP32-weight network
+        full = np.zeros(4)
+        oracle.oracle_fnn_forward_f64(4, L.ctypes.data, theta.ctypes.data, x.ctypes.data, full.ctypes.data)
+        assert np.max(np.abs(full - out)) > 1e-6
+
+
 # -------------------------------------------------------------------------------------------------- dynamics
 def deriv(prob_dyn, x, u):
     xd = np.zeros(8)
