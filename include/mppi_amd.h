@@ -108,7 +108,8 @@ typedef struct mppi_dynamics_params {
   int32_t gear_sign;
   /* Residual MLP (FNNHelper layout, utils/nn_helpers/fnn_helper.cu:130-205): per layer l the weights W_l are
    * row-major [out][in] followed by the bias [out]. nn_layers = {in, h1, h2, out}; the output is added to the
-   * derivatives of states nn_output_states[0..out). Device pointer to nn_num_params floats. */
+   * derivatives of states nn_output_states[0..out) (distinct states for MPPI_NN_BF16). Device pointer to
+   * nn_num_params floats. */
   int32_t nn_num_layers;
   int32_t nn_layers[MPPI_MAX_NN_LAYERS];
   int32_t nn_output_states[MPPI_MAX_STATE];

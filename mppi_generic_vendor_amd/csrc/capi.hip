@@ -70,6 +70,11 @@ static int validate_dynamics(const mppi_dynamics_params& d)
         return fail(MPPI_E_INVALID, "residual MLP: nn_output_states out of range");
     if (d.nn_precision != MPPI_NN_FP32 && d.nn_precision != MPPI_NN_BF16)
       return fail(MPPI_E_INVALID, "residual MLP: unknown precision");
+    if (d.nn_precision == MPPI_NN_BF16)
+      for (int i = 0; i < NN_OUT; ++i)
+        for (int j = 0; j < i; ++j)
+          if (d.nn_output_states[i] == d.nn_output_states[j])
+            return fail(MPPI_E_INVALID, "residual MLP: the BF16 path needs distinct nn_output_states");
   }
   return MPPI_OK;
 }

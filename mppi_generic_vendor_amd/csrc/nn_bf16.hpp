@@ -8,8 +8,10 @@
 // 16 g + m. A layer's reduction index may therefore be permuted freely as long as A and B agree: the next layer's
 // K slot (ks, g, j) is taken to be hidden unit 16 (2 ks + j / 4) + 4 g + j % 4, which is exactly accumulator element
 // j % 4 of tile 2 ks + j / 4 in the same lane. Layer outputs feed the next layer without leaving the registers.
-// Biases enter as the accumulator input (FP32). The output layer's 16 rows hold the 4 outputs four times, so every
-// lane group receives all four outputs of its sample.
+// Biases enter as the accumulator input (FP32). The output layer is laid out by state instead of by output: two
+// 16-row tiles whose row 4 g + i of tile q carries the network output added to the derivative of state 4 q + i (zero
+// weights for states that receive none), so every lane group receives, in place, the increments of all states of
+// its sample.
 #pragma once
 
 #include "models.hpp"
@@ -17,6 +19,8 @@
 namespace mppi_amd {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // FNN parameter offsets (utils/nn_helpers/fnn_helper.cu:130-205: W_l row-major [out][in], then b_l).
@@ -42,6 +46,15 @@ __device__ __forceinline__ f32x4 mfma_bf16(const bf16x8& a, const bf16x8& b, con
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+__device__ __forceinline__ bf16x2 cvt_bf16x2(float a, float b) { return __builtin_convertvector(f32x2{a, b}, bf16x2); }
+
+__device__ __forceinline__ bf16x8 cat_bf16(bf16x2 a, bf16x2 b, bf16x2 c, bf16x2 d)
+{
+  const auto ab = __builtin_shufflevector(a, b, 0, 1, 2, 3);
+  const auto cd = __builtin_shufflevector(c, d, 0, 1, 2, 3);
+  return __builtin_shufflevector(ab, cd, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 // B-operand slots of the next layer from two accumulator tiles: tanh, then round to BF16.
 __device__ __forceinline__ bf16x8 act_pack(const f32x4& lo, const f32x4& hi)
 {
@@ -60,12 +73,13 @@ __device__ __forceinline__ int nn_hidden_slot(int ks, int g, int j) { return 16 
 struct NnFrags {
   bf16x8 a1[4];     // layer 1, tile mt: row 16 mt + m, K slot (g, j) = input 8 g + j (zero beyond NN_IN)
   bf16x8 a2[4][2];  // layer 2, tile mt, k-step ks: row 16 mt + m, K slot = hidden nn_hidden_slot(ks, g, j)
-  bf16x8 a3[2];     // layer 3, k-step ks: row m -> output m % 4
-  f32x4 b1[4], b2[4], b3;
+  bf16x8 a3[2][2];  // layer 3, tile q, k-step ks: row m -> the output feeding state 4 q + m % 4
+  f32x4 b1[4], b2[4], b3[2];
 };
 
-// Builds lane `lane`'s fragments from the FP32 parameter vector `th` (LDS or global).
-__device__ __forceinline__ void nn_load_frags(const float* th, int lane, NnFrags& F)
+// Builds lane `lane`'s fragments from the FP32 parameter vector `th` (LDS or global). `out_states` are the distinct
+// states (< 8) that the NN_OUT outputs are added to.
+__device__ __forceinline__ void nn_load_frags(const float* th, const int* out_states, int lane, NnFrags& F)
 {
   const int m = lane & 15, g = lane >> 4;
 #pragma unroll
@@ -87,26 +101,43 @@ __device__ __forceinline__ void nn_load_frags(const float* th, int lane, NnFrags
     }
   }
 #pragma unroll
-  for (int ks = 0; ks < 2; ++ks)
+  for (int q = 0; q < 2; ++q) {
+    // output feeding the state of this lane's row (m % 4) and of its accumulator rows (4 g + i)
+    int o_row = -1;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) F.a3[ks][j] = (__bf16)th[NN_W3 + (m & 3) * NN_H2 + nn_hidden_slot(ks, g, j)];
+    for (int o = 0; o < NN_OUT; ++o)
+      if (out_states[o] == 4 * q + (m & 3)) o_row = o;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) F.b3[i] = th[NN_B3 + i];
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        F.a3[q][ks][j] = (__bf16)(o_row >= 0 ? th[NN_W3 + o_row * NN_H2 + nn_hidden_slot(ks, g, j)] : 0.0f);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float b = 0.0f;
+#pragma unroll
+      for (int o = 0; o < NN_OUT; ++o)
+        if (out_states[o] == 4 * q + i) b = th[NN_B3 + o];
+      F.b3[q][i] = b;
+    }
+  }
 }
 
-// Network output for the lane's sample (every lane group of the sample gets all four outputs).
-// x: the 7 states, u: the 2 controls, g = lane / 16.
-__device__ __forceinline__ f32x4 nn_eval_bf16(const NnFrags& F, const float* x, const float* u, int g)
+// Network increments of the derivatives of states 0..3 (.x) and 4..7 (.y) for the lane's sample (every lane group of
+// the sample gets all of them). x: the 7 states, u: the 2 controls, g = lane / 16.
+struct NnOut {
+  f32x4 lo, hi;
+};
+
+__device__ __forceinline__ NnOut nn_eval_bf16(const NnFrags& F, const float* x, const float* u, int g)
 {
-  float v[8];
   // inputs 0..7 (x0..x6, u0) in lane group 0, input 8 (u1) in slot 0 of group 1, zeros elsewhere
-#pragma unroll
-  for (int j = 0; j < 7; ++j) v[j] = g == 0 ? x[j] : 0.0f;
-  v[0] = g == 1 ? u[1] : v[0];
-  v[7] = g == 0 ? u[0] : 0.0f;
-  bf16x8 bin;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) bin[j] = (__bf16)v[j];
+  const bf16x2 zero = cvt_bf16x2(0.0f, 0.0f);
+  const bf16x2 p0 = cvt_bf16x2(g == 0 ? x[0] : (g == 1 ? u[1] : 0.0f), g == 0 ? x[1] : 0.0f);
+  const bf16x2 p1 = g == 0 ? cvt_bf16x2(x[2], x[3]) : zero;
+  const bf16x2 p2 = g == 0 ? cvt_bf16x2(x[4], x[5]) : zero;
+  const bf16x2 p3 = g == 0 ? cvt_bf16x2(x[6], u[0]) : zero;
+  const bf16x8 bin = cat_bf16(p0, p1, p2, p3);
   f32x4 d1[4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) d1[mt] = mfma_bf16(F.a1[mt], bin, F.b1[mt]);
@@ -117,7 +148,10 @@ __device__ __forceinline__ f32x4 nn_eval_bf16(const NnFrags& F, const float* x, 
   for (int mt = 0; mt < 4; ++mt) d2[mt] = mfma_bf16(F.a2[mt][1], h1b, mfma_bf16(F.a2[mt][0], h1a, F.b2[mt]));
   const bf16x8 h2a = act_pack(d2[0], d2[1]);
   const bf16x8 h2b = act_pack(d2[2], d2[3]);
-  return mfma_bf16(F.a3[1], h2b, mfma_bf16(F.a3[0], h2a, F.b3));
+  NnOut r;
+  r.lo = mfma_bf16(F.a3[0][1], h2b, mfma_bf16(F.a3[0][0], h2a, F.b3[0]));
+  r.hi = mfma_bf16(F.a3[1][1], h2b, mfma_bf16(F.a3[1][0], h2a, F.b3[1]));
+  return r;
 }
 
 }  // namespace mppi_amd

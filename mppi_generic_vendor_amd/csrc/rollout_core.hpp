@@ -95,7 +95,7 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
   for (int i = tid; i < NN_PARAMS; i += NN_THREADS) scratch[i] = dyn.nn_theta[i];
   __syncthreads();
   NnFrags F;
-  if (producer) nn_load_frags(scratch, lane, F);
+  if (producer) nn_load_frags(scratch, dyn.nn_output_states, lane, F);
   __syncthreads();
 
   const int g = lane >> 4;
@@ -133,12 +133,9 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
         }
         float xd[S], xn[S];
         M::deriv(dyn, x, u, xd);
-        const f32x4 o = nn_eval_bf16(F, x, u, g);
+        const NnOut o = nn_eval_bf16(F, x, u, g);
 #pragma unroll
-        for (int s = 0; s < S; ++s)
-#pragma unroll
-          for (int i = 0; i < NN_OUT; ++i)
-            if (dyn.nn_output_states[i] == s) xd[s] += o[i];
+        for (int s = 0; s < S; ++s) xd[s] += s < 4 ? o.lo[s] : o.hi[s - 4];
         M::update(dyn, x, xn, xd, dt);
         if (g == 0) {
 #pragma unroll
