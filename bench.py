@@ -179,12 +179,77 @@ def cpu_baseline(prob, seconds: float, threads: int):
     }
 
 
+def run_batch(args):
+    """256 independent instances (50/50 double integrator / bicycle, K ~ U[512, 8192] in multiples of 64,
+    T ~ U[50, 150]) advanced by one persistent launch + one combine launch per iteration, shape from the autotuner."""
+    from mppi_generic_vendor_amd import hip, workloads
+    from mppi_generic_vendor_amd.batch import MPPIBatch
+
+    probs, m = workloads.batch_mix(256, seed=7)
+    ks = sum(p.num_rollouts * p.num_timesteps for p in probs)
+    b = MPPIBatch(probs, cost_map=m)
+    cfg = b.autotune(reps=3)
+    print(f"[bench] batch256: {sum(p.num_rollouts for p in probs)} samples, {ks} sample-steps per iteration; "
+          f"autotuned threads={cfg.threads} samples/lane={cfg.samples_per_lane} chunk={cfg.chunk} "
+          f"blocks/CU={cfg.blocks_per_cu} ({cfg.iteration_us:.1f} us)", file=sys.stderr, flush=True)
+    x0 = [p.x0 for p in probs]
+    for _ in range(3):
+        b.computeControl(x0, 1)
+    host_ms = []
+    for _ in range(max(5, args.host_calls // 10)):
+        t0 = time.perf_counter()
+        b.computeControl(x0, 1)
+        host_ms.append((time.perf_counter() - t0) * 1e3)
+    b.stepDevice(args.warmup)
+    hip.synchronize()
+    stream = hip.Stream.__new__(hip.Stream)
+    stream.ptr = b.stream()
+    e0, e1 = hip.Event(), hip.Event()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    b.stepDevice(args.steps)
+    e1.record(stream)
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    stream.ptr = None
+    ev_ms = e0.elapsed_ms(e1) / args.steps
+    value = args.steps / wall
+    for i in range(len(probs)):
+        st = b.getWeightStats(i)
+        if not np.isfinite(st.normalizer) or st.normalizer <= 0:
+            raise SystemExit(f"batch instance {i} produced an invalid normalizer")
+    b.close()
+    # algorithmic HBM bytes of one iteration: clamped samples written (4 K T C), costs (4 K), and the per-tile
+    # partial rows (T C floats per 64-sample tile) written and read back by the combine
+    alg = sum(4 * p.num_rollouts * p.num_timesteps * p.C + 4 * p.num_rollouts
+              + 2 * 4 * (p.num_rollouts // 64) * p.num_timesteps * p.C for p in probs)
+    achieved = alg / (ev_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Hz", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded map, states, goals)",
+        "config": {"workload": "batch256: 256 instances (DI / bicycle 50/50), K~U[512,8192]/64, T~U[50,150], "
+                               "one persistent launch + combine per iteration, hipGraph replay",
+                   "launch_shape": {"threads": cfg.threads, "samples_per_lane": cfg.samples_per_lane,
+                                    "chunk": cfg.chunk, "blocks_per_cu": cfg.blocks_per_cu}},
+        "rollout_steps_per_s": round(value * ks, 1),
+        "control_hz_host": round(1e3 / statistics.median(host_ms), 3),
+        "roofline": {"kernel": "batch_rollout_kernel+batch_combine_kernel", "bound": "hbm",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes": alg,
+                     "avg_ms": round(ev_ms, 5)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="di_colored_map", choices=["di_colored_map", "cartpole", "bicycle_nn"])
+    ap.add_argument("--workload", default="di_colored_map",
+                    choices=["di_colored_map", "cartpole", "bicycle_nn", "batch256"])
     ap.add_argument("--host-calls", type=int, default=200)
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -204,6 +269,9 @@ def main():
     capi.load()
     if hip.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.workload == "batch256":
+        run_batch(args)
+        return
     prob = build_problem(args.workload)
     K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
     print(f"[bench] workload {args.workload} K={K} T={T} C={C}", file=sys.stderr, flush=True)

@@ -312,6 +312,68 @@ int mppi_controller_set_seed(mppi_controller* ctrl, uint64_t seed);
 /* The stream all controller work runs on (hipStream_t as void*). */
 int mppi_controller_stream(const mppi_controller* ctrl, void** stream);
 
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * Batch of independent controllers advanced by one persistent launch per iteration.
+ * No single counterpart in the reference: it replaces N VanillaMPPIController objects, each with its own stream
+ * and its own sampling/rollout/weights/update launches per computeControl (controllers/MPPI/mppi_controller.cu:
+ * 152-237), by one work-queue kernel over all instances' 64-sample tiles plus one combine launch, captured in one
+ * hipGraph. Instances may differ in model (double integrator, bicycle, cart-pole), sample count and horizon; each
+ * uses Gaussian sampling with its own seed, its own x0, goal and mean, and gets its own FP64 normalizer.
+ * ---------------------------------------------------------------------------------------------------------------- */
+typedef struct mppi_batch mppi_batch;
+
+typedef struct mppi_instance_desc {
+  mppi_dynamics_params dynamics; /* MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_MODEL_RACER_DUBINS or MPPI_MODEL_CARTPOLE */
+  mppi_cost_params cost;         /* QUADRATIC_MAP (DI, bicycle) or CARTPOLE_QUADRATIC (cart-pole); cost.map NULL:
+                                    the batch's shared map */
+  mppi_sampler_params sampler;   /* MPPI_SAMPLER_GAUSSIAN */
+  int32_t num_rollouts;          /* multiple of 64, at most 65536 */
+  int32_t num_timesteps;
+  float dt;
+  float lambda;
+  float alpha;
+  float slide_control_scale[MPPI_MAX_CONTROL]; /* ControllerParams::slide_control_scale_ */
+  const float* init_control;     /* host [T][C] or NULL for zeros */
+} mppi_instance_desc;
+
+/* Launch shape of the persistent kernel: threads per workgroup (256, 512, 1024: one state-chain wavefront, the rest
+ * evaluate costs and draw the next chunk's samples), samples per lane of the state-chain wavefront (1 or 2: 64 or
+ * 128 samples per tile), horizon chunk (4, 8, 16 steps), and the resulting workgroups per CU. iteration_us is the
+ * measured time of one batched iteration (0 if not measured). */
+typedef struct mppi_batch_config {
+  int32_t threads;
+  int32_t samples_per_lane;
+  int32_t chunk;
+  int32_t blocks_per_cu;
+  float iteration_us;
+} mppi_batch_config;
+
+/* cost_map_host: optional shared map [map_height][map_width] (copied); instances whose cost.map is NULL use it. */
+int mppi_batch_create(const mppi_instance_desc* instances, int32_t num_instances, const float* cost_map_host,
+                      int32_t map_height, int32_t map_width, mppi_batch** out);
+int mppi_batch_destroy(mppi_batch* batch);
+/* Times every valid launch shape on the batch (reps iterations each, from the current means, which are restored)
+ * and keeps the fastest; `best` (may be NULL) receives it. */
+int mppi_batch_autotune(mppi_batch* batch, int32_t reps, mppi_batch_config* best);
+int mppi_batch_set_config(mppi_batch* batch, const mppi_batch_config* config);
+int mppi_batch_get_config(const mppi_batch* batch, mppi_batch_config* config);
+/* One computeControl for every instance: x0_host is [num_instances][MPPI_MAX_STATE]. */
+int mppi_batch_compute_control(mppi_batch* batch, const float* x0_host, int32_t optimization_stride);
+/* num_calls device-resident batched iterations from a captured hipGraph (x0 and means already in device memory,
+ * generator counters advanced on the device). */
+int mppi_batch_step_device(mppi_batch* batch, int32_t num_calls);
+int mppi_batch_get_control_seq(const mppi_batch* batch, int32_t instance, float* out_host);
+int mppi_batch_get_raw_mean(const mppi_batch* batch, int32_t instance, float* out_host);
+int mppi_batch_get_state_seq(const mppi_batch* batch, int32_t instance, float* out_host);
+int mppi_batch_get_weight_stats(const mppi_batch* batch, int32_t instance, mppi_weight_stats* out_host);
+int mppi_batch_get_free_energy(const mppi_batch* batch, int32_t instance, mppi_free_energy* out_host);
+int mppi_batch_set_control_seq(mppi_batch* batch, int32_t instance, const float* u_host);
+int mppi_batch_slide_control_seq(mppi_batch* batch, int32_t steps);
+/* Clamped control samples [K][T][C] and costs [K] of an instance's last iteration (copied to the host). */
+int mppi_batch_get_samples(const mppi_batch* batch, int32_t instance, float* samples_host, float* costs_host);
+int mppi_batch_stream(const mppi_batch* batch, void** stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -1,0 +1,123 @@
+"""Python host wrapper of the C-ABI batch object: many independent MPPI instances advanced by one persistent launch
+per iteration (mppi_batch_* in include/mppi_amd.h)."""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import capi
+from .workloads import Problem
+
+
+class MPPIBatch:
+    def __init__(self, problems: list[Problem], cost_map: np.ndarray | None = None):
+        capi.load()
+        self.problems = problems
+        n = len(problems)
+        descs = (capi.InstanceDesc * n)()
+        self._keep = []
+        for d, p in zip(descs, problems):
+            d.dynamics = p.dyn
+            d.cost = p.cost
+            d.cost.map = None
+            d.sampler = p.sampler
+            d.num_rollouts = p.num_rollouts
+            d.num_timesteps = p.num_timesteps
+            d.dt = p.dt
+            d.lambda_ = p.lambda_
+            d.alpha = p.alpha
+            d.init_control = None
+        mp = None
+        h = w = 0
+        if cost_map is not None:
+            m = np.ascontiguousarray(cost_map, dtype=np.float32)
+            self._keep.append(m)
+            mp = m.ctypes.data
+            h, w = m.shape
+        handle = ct.c_void_p()
+        capi.call("mppi_batch_create", descs, n, mp, h, w, ct.byref(handle))
+        self._h = handle
+        self.n = n
+
+    # ------------------------------------------------------------------------------------------------ control
+    def computeControl(self, x0: np.ndarray, stride: int = 1):
+        x = np.zeros((self.n, capi.MAX_STATE), np.float32)
+        for i, p in enumerate(self.problems):
+            x[i, :p.S] = np.asarray(x0[i], np.float32)[:p.S]
+        capi.call("mppi_batch_compute_control", self._h, x.ctypes.data, stride)
+
+    def stepDevice(self, num_calls: int):
+        capi.call("mppi_batch_step_device", self._h, num_calls)
+
+    def autotune(self, reps: int = 3) -> capi.BatchConfig:
+        c = capi.BatchConfig()
+        capi.call("mppi_batch_autotune", self._h, reps, ct.byref(c))
+        return c
+
+    def setConfig(self, threads: int, samples_per_lane: int, chunk: int):
+        c = capi.BatchConfig(threads, samples_per_lane, chunk, 0, 0.0)
+        capi.call("mppi_batch_set_config", self._h, ct.byref(c))
+
+    def getConfig(self) -> capi.BatchConfig:
+        c = capi.BatchConfig()
+        capi.call("mppi_batch_get_config", self._h, ct.byref(c))
+        return c
+
+    def _seq(self, name: str, i: int, shape):
+        out = np.zeros(shape, np.float32)
+        capi.call(name, self._h, i, out.ctypes.data)
+        return out
+
+    def getControlSeq(self, i: int) -> np.ndarray:
+        p = self.problems[i]
+        return self._seq("mppi_batch_get_control_seq", i, (p.num_timesteps, p.C))
+
+    def getRawMean(self, i: int) -> np.ndarray:
+        p = self.problems[i]
+        return self._seq("mppi_batch_get_raw_mean", i, (p.num_timesteps, p.C))
+
+    def getTargetStateSeq(self, i: int) -> np.ndarray:
+        p = self.problems[i]
+        return self._seq("mppi_batch_get_state_seq", i, (p.num_timesteps, p.S))
+
+    def getWeightStats(self, i: int) -> capi.WeightStats:
+        st = capi.WeightStats()
+        capi.call("mppi_batch_get_weight_stats", self._h, i, ct.byref(st))
+        return st
+
+    def getFreeEnergyStatistics(self, i: int) -> capi.FreeEnergy:
+        fe = capi.FreeEnergy()
+        capi.call("mppi_batch_get_free_energy", self._h, i, ct.byref(fe))
+        return fe
+
+    def setControlSeq(self, i: int, u: np.ndarray):
+        p = self.problems[i]
+        a = np.ascontiguousarray(u, dtype=np.float32).reshape(p.num_timesteps, p.C)
+        capi.call("mppi_batch_set_control_seq", self._h, i, a.ctypes.data)
+
+    def slideControlSequence(self, steps: int):
+        capi.call("mppi_batch_slide_control_seq", self._h, steps)
+
+    def getSamples(self, i: int):
+        p = self.problems[i]
+        u = np.zeros((p.num_rollouts, p.num_timesteps, p.C), np.float32)
+        c = np.zeros(p.num_rollouts, np.float32)
+        capi.call("mppi_batch_get_samples", self._h, i, u.ctypes.data, c.ctypes.data)
+        return u, c
+
+    def stream(self) -> int:
+        s = ct.c_void_p()
+        capi.call("mppi_batch_stream", self._h, ct.byref(s))
+        return s.value
+
+    def close(self):
+        if self._h:
+            capi.call("mppi_batch_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -105,6 +105,19 @@ class ControllerDesc(ct.Structure):
     ]
 
 
+class InstanceDesc(ct.Structure):
+    _fields_ = [
+        ("dynamics", DynamicsParams), ("cost", CostParams), ("sampler", SamplerParams),
+        ("num_rollouts", _i), ("num_timesteps", _i), ("dt", _f), ("lambda_", _f), ("alpha", _f),
+        ("slide_control_scale", _f * MAX_CONTROL), ("init_control", ct.c_void_p),
+    ]
+
+
+class BatchConfig(ct.Structure):
+    _fields_ = [("threads", _i), ("samples_per_lane", _i), ("chunk", _i), ("blocks_per_cu", _i),
+                ("iteration_us", _f)]
+
+
 P = ct.c_void_p
 _SIGS = {
     "mppi_last_error": ([], ct.c_char_p),
@@ -138,6 +151,22 @@ _SIGS = {
     "mppi_controller_device_buffers": ([P, ct.POINTER(P), ct.POINTER(P), ct.POINTER(P)], ct.c_int),
     "mppi_controller_set_seed": ([P, ct.c_uint64], ct.c_int),
     "mppi_controller_stream": ([P, ct.POINTER(P)], ct.c_int),
+    "mppi_batch_create": ([ct.POINTER(InstanceDesc), _i, P, _i, _i, ct.POINTER(P)], ct.c_int),
+    "mppi_batch_destroy": ([P], ct.c_int),
+    "mppi_batch_autotune": ([P, _i, ct.POINTER(BatchConfig)], ct.c_int),
+    "mppi_batch_set_config": ([P, ct.POINTER(BatchConfig)], ct.c_int),
+    "mppi_batch_get_config": ([P, ct.POINTER(BatchConfig)], ct.c_int),
+    "mppi_batch_compute_control": ([P, P, _i], ct.c_int),
+    "mppi_batch_step_device": ([P, _i], ct.c_int),
+    "mppi_batch_get_control_seq": ([P, _i, P], ct.c_int),
+    "mppi_batch_get_raw_mean": ([P, _i, P], ct.c_int),
+    "mppi_batch_get_state_seq": ([P, _i, P], ct.c_int),
+    "mppi_batch_get_weight_stats": ([P, _i, ct.POINTER(WeightStats)], ct.c_int),
+    "mppi_batch_get_free_energy": ([P, _i, ct.POINTER(FreeEnergy)], ct.c_int),
+    "mppi_batch_set_control_seq": ([P, _i, P], ct.c_int),
+    "mppi_batch_slide_control_seq": ([P, _i], ct.c_int),
+    "mppi_batch_get_samples": ([P, _i, P, P], ct.c_int),
+    "mppi_batch_stream": ([P, ct.POINTER(P)], ct.c_int),
 }
 
 _lib = None

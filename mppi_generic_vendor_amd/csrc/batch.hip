@@ -1,0 +1,948 @@
+// Many independent MPPI instances advanced by one persistent launch per iteration.
+//
+// The reference runs one VanillaMPPIController per system, each computeControl issuing its own sampling, rollout,
+// weight and update launches (controllers/MPPI/mppi_controller.cu:152-237). Here all instances' 64- or 128-sample
+// tiles form one work list, ordered longest horizon first, that a persistent kernel drains through an atomic
+// counter. Per tile, one workgroup:
+//   - keeps the instance's mean in LDS and draws the Gaussian samples chunk by chunk (Philox, the same element
+//     numbering as mppi_sample_gaussian, so results equal the single-instance path) into a three-slot LDS ring;
+//   - runs the state chain on wavefront 0 (samples_per_lane samples per lane) one chunk behind the generator while
+//     the other wavefronts evaluate the running + likelihood-ratio costs of the previous chunk and write its clamped
+//     samples to the instance's sample region ([T][R][C] per tile, coalesced);
+//   - reduces the tile against its own baseline into (b_g, argmin_g, eta_g, sum w^2_g) and S_g[t][c].
+// A combine launch (one workgroup per instance) merges the partials with the log-sum-exp rule of the controller path
+// and writes each instance's new mean in place and its FP64 statistics. Both launches are captured in a hipGraph.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "fused.hpp"
+#include "host.hpp"
+#include "kernels.hpp"
+#include "models.hpp"
+
+namespace mppi_amd {
+
+constexpr int BATCH_MAX_K = 65536;
+constexpr int BATCH_MAX_STATE = MPPI_MAX_STATE;
+
+struct BatchInstance {
+  mppi_dynamics_params dyn;
+  mppi_cost_params cost;
+  mppi_sampler_params sp;
+  int32_t K, T, C, pstart;
+  int32_t tiles;      // ceil(K / R) under the current launch shape
+  int32_t part_base;  // index of the first Partial of the instance
+  int64_t sum_base;   // float offset of S_g rows [tiles][T*C]
+  int64_t u_base;     // float offset of the clamped samples [tiles][T][R][C]
+  int32_t mean_base;  // float offset of the mean [T][C]
+  int32_t cost_base;  // float offset of the costs [K]
+  float dt, lambda, alpha;
+  int32_t _pad;
+};
+
+// LDS floats of one workgroup: the work-item slot (4 floats), then per tile the mean, the sample ring
+// [3][chunk][R][C], the outputs [2][chunk][R][O], the consumer partial sums, terminal costs, costs and weights.
+static size_t batch_lds_floats(int TCmax, int Cmax, int Omax, int NT, int R, int ch)
+{
+  const int ncons = NT / 64 - 1;
+  return 4 + (size_t)((TCmax + 3) & ~3) + 3ull * ch * R * Cmax + 2ull * ch * R * Omax + (size_t)ncons * R + 3ull * R;
+}
+
+template <int MODEL, int COST, int NT, int SPL>
+__device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArgs a, int g,
+                                           const float* __restrict__ x0, const float* __restrict__ means,
+                                           float* __restrict__ ubuf, Partial* __restrict__ parts,
+                                           float* __restrict__ sums, float* __restrict__ costs, float* lds, int ch)
+{
+  using M = Model<MODEL>;
+  using CF = Cost<COST, M::O>;
+  constexpr int S = M::S, C = M::C, O = M::O;
+  static_assert(O == S, "outputs are the states for the batched models");
+  constexpr int R = 64 * SPL;
+  constexpr int NW = NT / 64, NCONS = NW - 1;
+  const int T = I.T, TC = T * C, K = I.K;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int k0 = g * R;
+  const int nrows = min(R, K - k0);
+  float* mu = lds;
+  float* ring = mu + ((TC + 3) & ~3);     // [3][ch][R][C]
+  float* ybuf = ring + 3 * ch * R * C;    // [2][ch][R][O]
+  float* part = ybuf + 2 * ch * R * O;    // [NCONS][R]
+  float* term = part + NCONS * R;         // [R]
+  float* cst = term + R;                  // [R]
+  float* wl = cst + R;                    // [R]
+  float* utile = ubuf + I.u_base + (int64_t)g * T * R * C;  // [T][R][C]
+  const mppi_sampler_params& sp = I.sp;
+  for (int j = tid; j < TC; j += NT) mu[j] = means[I.mean_base + j];
+  const float decay = powf(sp.std_dev_decay, (float)a.iteration);
+  const int nch = (T + ch - 1) / ch;
+  __syncthreads();
+
+  // Samples of chunk j into ring slot j % 3, by threads t_off .. t_off + nth - 1. Element (k, t, c) is normal
+  // k*TC + t*C + c of the instance's stream; thread item (r, q) evaluates Philox block q of row r's range.
+  auto gen = [&](int j, int t_off, int nth) {
+    const int t0 = j * ch, t1 = min(T, t0 + ch);
+    const int nb = ((t1 - t0) * C + 3) / 4 + 1;
+    float* slot = ring + (j % 3) * ch * R * C;
+    for (int item = tid - t_off; item < R * nb; item += nth) {
+      const int r = item / nb, q = item - r * nb;
+      const int64_t kk = k0 + r;
+      const int64_t es = kk * TC + (int64_t)t0 * C, ee = kk * TC + (int64_t)t1 * C;
+      const int64_t b = (es >> 2) + q;
+      if (b > ((ee - 1) >> 2)) continue;
+      const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)b);
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const int64_t e = 4 * b + jj;
+        if (e < es || e >= ee) continue;
+        const int loc = (int)(e - kk * TC);
+        const int t = loc / C, c = loc - t * C;
+        const float m = mu[loc];
+        float v;
+        if (kk == 0 || t < a.stride) v = m;
+        else if (kk >= I.pstart) v = decay * sp.std_dev[c] * z.z[jj];
+        else v = m + decay * sp.std_dev[c] * z.z[jj];
+        slot[((t - t0) * R + r) * C + c] = v;
+      }
+    }
+  };
+
+  gen(0, 0, NT);
+  __syncthreads();
+
+  float x[SPL][S];
+  if (wave == 0) {
+#pragma unroll
+    for (int s = 0; s < SPL; ++s)
+#pragma unroll
+      for (int i = 0; i < S; ++i) x[s][i] = x0[i];
+  }
+  bool deadband = false;
+#pragma unroll
+  for (int c = 0; c < C; ++c) deadband |= I.dyn.control_deadband[c] != 0.0f;
+  float acc[SPL];
+  bool pure[SPL];
+#pragma unroll
+  for (int s = 0; s < SPL; ++s) {
+    acc[s] = 0.0f;
+    pure[s] = (k0 + lane + 64 * s) >= I.pstart;
+  }
+
+  for (int it = 0; it <= nch; ++it) {
+    if (wave == 0 && it < nch) {
+      const int t0 = it * ch, n = min(T, t0 + ch) - t0;
+      float* rs = ring + (it % 3) * ch * R * C;
+      float* yb = ybuf + (it & 1) * ch * R * O;
+      float un[SPL][C];
+#pragma unroll
+      for (int s = 0; s < SPL; ++s)
+#pragma unroll
+        for (int c = 0; c < C; ++c) un[s][c] = rs[(lane + 64 * s) * C + c];
+      for (int tt = 0; tt < n; ++tt) {
+        const int tn = min(tt + 1, n - 1);
+#pragma unroll
+        for (int s = 0; s < SPL; ++s) {
+          const int r = lane + 64 * s;
+          float u[C];
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            u[c] = un[s][c];
+            un[s][c] = rs[(tn * R + r) * C + c];
+          }
+          if (deadband) enforce_constraints<C>(I.dyn, u);
+          else clamp_controls<C>(I.dyn, u);
+#pragma unroll
+          for (int c = 0; c < C; ++c) rs[(tt * R + r) * C + c] = u[c];
+          float xd[S], xn[S];
+          M::deriv(I.dyn, x[s], u, xd);
+          M::update(I.dyn, x[s], xn, xd, I.dt);
+#pragma unroll
+          for (int i = 0; i < S; ++i) {
+            yb[(tt * R + r) * O + i] = xn[i];
+            x[s][i] = xn[i];
+          }
+        }
+      }
+      if (it == nch - 1) {
+#pragma unroll
+        for (int s = 0; s < SPL; ++s) term[lane + 64 * s] = CF::terminal(I.cost, x[s]);
+      }
+    } else if (wave > 0) {
+      if (it > 0) {
+        const int t0 = (it - 1) * ch, t1 = min(T, t0 + ch);
+        const float* rs = ring + ((it - 1) % 3) * ch * R * C;
+        const float* yb = ybuf + ((it - 1) & 1) * ch * R * O;
+        for (int t = t0 + wave - 1; t < t1; t += NCONS) {
+#pragma unroll
+          for (int s = 0; s < SPL; ++s) {
+            const int r = lane + 64 * s;
+            float yy[O], u[C];
+#pragma unroll
+            for (int i = 0; i < O; ++i) yy[i] = yb[((t - t0) * R + r) * O + i];
+#pragma unroll
+            for (int c = 0; c < C; ++c) u[c] = rs[((t - t0) * R + r) * C + c];
+            acc[s] += CF::running(I.cost, yy) + lr_cost<C>(sp, u, mu + t * C, pure[s], I.lambda, I.alpha);
+          }
+        }
+        // clamped samples of the chunk to the tile's [T][R][C] region (contiguous, 16-byte stores)
+        const int n4 = (t1 - t0) * R * C / 4;
+        const float4* src = reinterpret_cast<const float4*>(rs);
+        float4* dst = reinterpret_cast<float4*>(utile + (int64_t)t0 * R * C);
+        for (int i = tid - 64; i < n4; i += NT - 64) dst[i] = src[i];
+      }
+      if (it + 1 < nch) gen(it + 1, 64, NT - 64);
+    }
+    __syncthreads();
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int s = 0; s < SPL; ++s) part[(wave - 1) * R + lane + 64 * s] = acc[s];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    float v = INFINITY;
+    int vi = INT_MAX;
+    float cs[SPL];
+#pragma unroll
+    for (int s = 0; s < SPL; ++s) {
+      const int r = lane + 64 * s;
+      float run = 0.0f;
+#pragma unroll 4
+      for (int w = 0; w < NCONS; ++w) run += part[w * R + r];
+      const float cost_r = run / (float)T + term[r] / (float)T;
+      const bool valid = r < nrows;
+      if (valid) costs[I.cost_base + k0 + r] = cost_r;
+      cs[s] = valid ? sanitize_cost(cost_r) : INFINITY;
+      argmin_combine(v, vi, cs[s], valid ? k0 + r : INT_MAX);
+    }
+    wave_argmin(v, vi);
+    double e = 0.0, e2 = 0.0;
+#pragma unroll
+    for (int s = 0; s < SPL; ++s) {
+      const float w = (cs[s] <= FLT_MAX) ? expf(-(1.0f / I.lambda) * (cs[s] - v)) : 0.0f;
+      wl[lane + 64 * s] = w;
+      e += (double)w;
+      e2 += (double)w * (double)w;
+    }
+    e = wave_sum(e);
+    e2 = wave_sum(e2);
+    if (lane == 0) {
+      Partial p;
+      p.baseline = v;
+      p.argmin = vi;
+      p.eta = e;
+      p.s2 = e2;
+      p._pad = 0.0;
+      parts[I.part_base + g] = p;
+    }
+  }
+  __syncthreads();
+  // S_g[t][c] = sum_r w_r u[t][r][c]: wavefront per step, lanes over samples
+  for (int t = wave; t < T; t += NW) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      float v = 0.0f;
+#pragma unroll
+      for (int s = 0; s < SPL; ++s) {
+        const int r = lane + 64 * s;
+        v += wl[r] * utile[((int64_t)t * R + r) * C + c];
+      }
+      v = wave_sum(v);
+      if (lane == 0) sums[I.sum_base + (int64_t)g * TC + t * C + c] = v;
+    }
+  }
+}
+
+template <int NT, int SPL>
+__global__ void __launch_bounds__(NT) batch_rollout_kernel(const BatchInstance* __restrict__ inst,
+                                                           const int2* __restrict__ work, int nwork,
+                                                           int* __restrict__ queue,
+                                                           const IterArgs* __restrict__ iters,
+                                                           const float* __restrict__ x0s,
+                                                           const float* __restrict__ means,
+                                                           float* __restrict__ ubuf, Partial* __restrict__ parts,
+                                                           float* __restrict__ sums, float* __restrict__ costs, int ch)
+{
+  extern __shared__ float lds[];
+  int* s_item = reinterpret_cast<int*>(lds);  // the tile's buffers start 4 floats in
+  for (;;) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(*s_item);
+    if (w >= nwork) break;
+    const int2 wi = work[w];
+    const int i = __builtin_amdgcn_readfirstlane(wi.x);
+    const int g = __builtin_amdgcn_readfirstlane(wi.y);
+    const BatchInstance& I = inst[i];
+    const IterArgs a = iters[i];
+    const float* x0 = x0s + (int64_t)i * BATCH_MAX_STATE;
+    switch (I.dyn.model) {
+      case MPPI_MODEL_DOUBLE_INTEGRATOR:
+        batch_tile<MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP, NT, SPL>(I, a, g, x0, means, ubuf, parts,
+                                                                                   sums, costs, lds + 4, ch);
+        break;
+      case MPPI_MODEL_RACER_DUBINS:
+        batch_tile<MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP, NT, SPL>(I, a, g, x0, means, ubuf, parts, sums,
+                                                                              costs, lds + 4, ch);
+        break;
+      default:
+        batch_tile<MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC, NT, SPL>(I, a, g, x0, means, ubuf, parts,
+                                                                               sums, costs, lds + 4, ch);
+        break;
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup per instance: global baseline (first strict minimum over the tiles' minima), FP64 normalizer and
+// sum of squares, new mean S / eta written in place, statistics; optionally advances the instance's generator
+// counter. Workgroup 0 resets the work-queue counter for the next launch.
+constexpr int BATCH_COMBINE_THREADS = 256;
+constexpr int BATCH_MAX_TILES = BATCH_MAX_K / 64;
+
+__global__ void __launch_bounds__(BATCH_COMBINE_THREADS) batch_combine_kernel(
+    const BatchInstance* __restrict__ inst, const Partial* __restrict__ parts, const float* __restrict__ sums,
+    float* __restrict__ means, mppi_weight_stats* __restrict__ stats, IterArgs* iters, int advance,
+    int* __restrict__ queue)
+{
+  __shared__ float escale[BATCH_MAX_TILES];
+  __shared__ float sv[BATCH_COMBINE_THREADS / 64];
+  __shared__ int si[BATCH_COMBINE_THREADS / 64];
+  __shared__ double se[BATCH_COMBINE_THREADS / 64], se2[BATCH_COMBINE_THREADS / 64];
+  const int i = blockIdx.x;
+  const BatchInstance& I = inst[i];
+  const int G = I.tiles, TC = I.T * I.C;
+  const Partial* P = parts + I.part_base;
+  const float* Sg = sums + I.sum_base;
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  float v = INFINITY;
+  int vi = INT_MAX;
+  for (int q = tid; q < G; q += BATCH_COMBINE_THREADS) argmin_combine(v, vi, P[q].baseline, P[q].argmin);
+  wave_argmin(v, vi);
+  if (lane == 0) {
+    sv[wid] = v;
+    si[wid] = vi;
+  }
+  __syncthreads();
+  float b = sv[0];
+  int bi = si[0];
+  for (int q = 1; q < BATCH_COMBINE_THREADS / 64; ++q) argmin_combine(b, bi, sv[q], si[q]);
+  if (bi == INT_MAX) bi = 0;
+  const double inv = 1.0 / (double)I.lambda;
+  double e = 0.0, e2 = 0.0;
+  for (int q = tid; q < G; q += BATCH_COMBINE_THREADS) {
+    const Partial p = P[q];
+    const double sc = (p.baseline <= FLT_MAX) ? exp(-inv * ((double)p.baseline - (double)b)) : 0.0;
+    escale[q] = (float)sc;
+    e += p.eta * sc;
+    e2 += p.s2 * sc * sc;
+  }
+  e = wave_sum(e);
+  e2 = wave_sum(e2);
+  if (lane == 0) {
+    se[wid] = e;
+    se2[wid] = e2;
+  }
+  __syncthreads();
+  double eta = 0.0, s2 = 0.0;
+  for (int q = 0; q < BATCH_COMBINE_THREADS / 64; ++q) {
+    eta += se[q];
+    s2 += se2[q];
+  }
+  for (int j = tid; j < TC; j += BATCH_COMBINE_THREADS) {
+    float acc = 0.0f;
+    for (int q = 0; q < G; ++q) acc += escale[q] * Sg[(int64_t)q * TC + j];
+    means[I.mean_base + j] = (float)((double)acc / eta);
+  }
+  if (tid == 0) {
+    mppi_weight_stats st;
+    st.baseline = b;
+    st.argmin = bi;
+    st.normalizer = eta;
+    st.sum_w2 = s2;
+    st.lambda = I.lambda;
+    st.num_rollouts = I.K;
+    stats[i] = st;
+    if (advance) iters[i].rng_offset += 1;
+    if (i == 0) *queue = 0;
+  }
+}
+
+template <int NT, int SPL>
+static int launch_batch_t(const BatchInstance* inst, const int2* work, int nwork, int* queue, const IterArgs* iters,
+                          const float* x0s, const float* means, float* ubuf, Partial* parts, float* sums, float* costs,
+                          int ch, size_t lds_bytes, int blocks, hipStream_t s)
+{
+  static std::once_flag once;
+  hipError_t attr = hipSuccess;
+  std::call_once(once, [&] {
+    attr = hipFuncSetAttribute((const void*)batch_rollout_kernel<NT, SPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               LDS_LIMIT);
+  });
+  if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute");
+  batch_rollout_kernel<NT, SPL><<<blocks, NT, lds_bytes, s>>>(inst, work, nwork, queue, iters, x0s, means, ubuf,
+                                                              parts, sums, costs, ch);
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
+template <int NT, int SPL>
+static int occupancy_t(size_t lds_bytes, int* out)
+{
+  const hipError_t e = hipFuncSetAttribute((const void*)batch_rollout_kernel<NT, SPL>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT);
+  if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+  MPPI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(out, (const void*)batch_rollout_kernel<NT, SPL>, NT,
+                                                              lds_bytes));
+  return MPPI_OK;
+}
+
+#define MPPI_BATCH_SHAPES(X) X(256, 1) X(512, 1) X(1024, 1) X(256, 2) X(512, 2) X(1024, 2)
+
+static int batch_occupancy(int NT, int SPL, size_t lds_bytes, int* out)
+{
+#define MPPI_OCC(nt, spl) \
+  if (NT == nt && SPL == spl) return occupancy_t<nt, spl>(lds_bytes, out);
+  MPPI_BATCH_SHAPES(MPPI_OCC)
+#undef MPPI_OCC
+  return fail(MPPI_E_INVALID, "batch: unsupported launch shape");
+}
+
+static int launch_batch(int NT, int SPL, const BatchInstance* inst, const int2* work, int nwork, int* queue,
+                        const IterArgs* iters, const float* x0s, const float* means, float* ubuf, Partial* parts,
+                        float* sums, float* costs, int ch, size_t lds_bytes, int blocks, hipStream_t s)
+{
+#define MPPI_LAUNCH(nt, spl)                                                                                    \
+  if (NT == nt && SPL == spl)                                                                                   \
+    return launch_batch_t<nt, spl>(inst, work, nwork, queue, iters, x0s, means, ubuf, parts, sums, costs, ch,   \
+                                   lds_bytes, blocks, s);
+  MPPI_BATCH_SHAPES(MPPI_LAUNCH)
+#undef MPPI_LAUNCH
+  return fail(MPPI_E_INVALID, "batch: unsupported launch shape");
+}
+
+}  // namespace mppi_amd
+
+using namespace mppi_amd;
+
+// ======================================================================================================= C-ABI
+struct mppi_batch {
+  int n = 0;
+  std::vector<mppi_instance_desc> desc;
+  std::vector<int> K, T, C, S;
+  int TCmax = 0, Cmax = 1, Omax = 1, cus = 0;
+  hipStream_t stream = nullptr;
+  float* map_d = nullptr;
+  BatchInstance* inst_d = nullptr;
+  std::vector<BatchInstance> inst_h;
+  int2* work_d = nullptr;
+  int nwork = 0;
+  int* queue_d = nullptr;
+  IterArgs* iters_d = nullptr;
+  IterArgs* iters_h = nullptr;
+  IterArgs* loop_iters_d = nullptr;
+  float* x0_d = nullptr;
+  float* x0_h = nullptr;
+  float* means_d = nullptr;
+  float* means_backup_d = nullptr;
+  float* nominal_h = nullptr;
+  float* mean_out_h = nullptr;
+  std::vector<int> mean_base;
+  int total_mean = 0;
+  mppi_weight_stats* stats_d = nullptr;
+  mppi_weight_stats* stats_h = nullptr;
+  float* ubuf_d = nullptr;
+  Partial* parts_d = nullptr;
+  float* sums_d = nullptr;
+  float* costs_d = nullptr;
+  std::vector<int> cost_base;
+  mppi_batch_config cfg{};
+  size_t lds_bytes = 0;
+  int blocks = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t loop_graph = nullptr;
+  hipGraphExec_t loop_exec = nullptr;
+  int eager_runs = 0;
+  uint64_t rng_counter = 0;
+  std::vector<std::vector<float>> control, raw_mean, state_seq;
+  std::vector<mppi_weight_stats> stats;
+  std::vector<mppi_free_energy> fe;
+};
+
+static void batch_drop_graphs(mppi_batch* b)
+{
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->loop_exec) (void)hipGraphExecDestroy(b->loop_exec);
+  if (b->loop_graph) (void)hipGraphDestroy(b->loop_graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  b->loop_exec = nullptr;
+  b->loop_graph = nullptr;
+  b->eager_runs = 0;
+}
+
+static int batch_free(mppi_batch* b)
+{
+  if (!b) return MPPI_OK;
+  batch_drop_graphs(b);
+  for (void* p : {(void*)b->map_d, (void*)b->inst_d, (void*)b->work_d, (void*)b->queue_d, (void*)b->iters_d,
+                  (void*)b->loop_iters_d, (void*)b->x0_d, (void*)b->means_d, (void*)b->means_backup_d,
+                  (void*)b->stats_d, (void*)b->ubuf_d, (void*)b->parts_d, (void*)b->sums_d, (void*)b->costs_d})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)b->iters_h, (void*)b->x0_h, (void*)b->nominal_h, (void*)b->mean_out_h, (void*)b->stats_h})
+    if (p) (void)hipHostFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MPPI_OK;
+}
+
+// Instance table, work list (tiles ordered by decreasing horizon, then instance, then tile) and LDS size for the
+// launch shape `c`; uploads them. Returns MPPI_E_SIZE when the shape does not fit the LDS.
+static int batch_apply_config(mppi_batch* b, const mppi_batch_config& c)
+{
+  const int R = 64 * c.samples_per_lane;
+  const size_t lds = sizeof(float) * batch_lds_floats(b->TCmax, b->Cmax, b->Omax, c.threads, R, c.chunk);
+  if (lds > LDS_LIMIT) return fail(MPPI_E_SIZE, "batch: launch shape exceeds the LDS");
+  int occ = 0;
+  MPPI_CHECK(batch_occupancy(c.threads, c.samples_per_lane, lds, &occ));
+  if (occ < 1) return fail(MPPI_E_SIZE, "batch: launch shape does not fit on a CU");
+  int part = 0;
+  int64_t sum = 0, u = 0;
+  std::vector<int2> work;
+  for (int i = 0; i < b->n; ++i) {
+    BatchInstance& I = b->inst_h[i];
+    I.tiles = (I.K + R - 1) / R;
+    I.part_base = part;
+    I.sum_base = sum;
+    I.u_base = u;
+    part += I.tiles;
+    sum += (int64_t)I.tiles * I.T * I.C;
+    u += (int64_t)I.tiles * I.T * R * I.C;
+    for (int g = 0; g < I.tiles; ++g) work.push_back(int2{i, g});
+  }
+  std::stable_sort(work.begin(), work.end(),
+                   [&](const int2& x, const int2& y) { return b->inst_h[x.x].T > b->inst_h[y.x].T; });
+  MPPI_HIP_CHECK(hipMemcpy(b->inst_d, b->inst_h.data(), sizeof(BatchInstance) * b->n, hipMemcpyHostToDevice));
+  MPPI_HIP_CHECK(hipMemcpy(b->work_d, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice));
+  b->nwork = (int)work.size();
+  b->lds_bytes = lds;
+  b->blocks = std::max(1, std::min(b->nwork, occ * b->cus));
+  b->cfg = c;
+  b->cfg.blocks_per_cu = occ;
+  batch_drop_graphs(b);
+  return MPPI_OK;
+}
+
+// One batched iteration: persistent rollout over every tile, then the per-instance combine.
+static int batch_enqueue_iteration(mppi_batch* b, IterArgs* iters, int advance)
+{
+  MPPI_CHECK(launch_batch(b->cfg.threads, b->cfg.samples_per_lane, b->inst_d, b->work_d, b->nwork, b->queue_d, iters,
+                          b->x0_d, b->means_d, b->ubuf_d, b->parts_d, b->sums_d, b->costs_d, b->cfg.chunk,
+                          b->lds_bytes, b->blocks, b->stream));
+  batch_combine_kernel<<<b->n, BATCH_COMBINE_THREADS, 0, b->stream>>>(b->inst_d, b->parts_d, b->sums_d, b->means_d,
+                                                                      b->stats_d, iters, advance, b->queue_d);
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
+static int batch_enqueue_call(mppi_batch* b)
+{
+  const hipStream_t s = b->stream;
+  MPPI_HIP_CHECK(hipMemcpyAsync(b->iters_d, b->iters_h, sizeof(IterArgs) * b->n, hipMemcpyHostToDevice, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(b->x0_d, b->x0_h, sizeof(float) * BATCH_MAX_STATE * b->n, hipMemcpyHostToDevice, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(b->means_d, b->nominal_h, sizeof(float) * b->total_mean, hipMemcpyHostToDevice, s));
+  MPPI_CHECK(batch_enqueue_iteration(b, b->iters_d, 0));
+  MPPI_HIP_CHECK(hipMemcpyAsync(b->mean_out_h, b->means_d, sizeof(float) * b->total_mean, hipMemcpyDeviceToHost, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(b->stats_h, b->stats_d, sizeof(mppi_weight_stats) * b->n, hipMemcpyDeviceToHost, s));
+  return MPPI_OK;
+}
+
+extern "C" {
+
+int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const float* cost_map_host, int32_t map_h,
+                      int32_t map_w, mppi_batch** out)
+{
+  if (!instances || !out || n < 1) return fail(MPPI_E_INVALID, "batch: null argument or no instances");
+  *out = nullptr;
+  mppi_batch* b = new (std::nothrow) mppi_batch();
+  if (!b) return fail(MPPI_E_NOMEM, "batch: out of host memory");
+  auto bail = [&](int st) {
+    batch_free(b);
+    return st;
+  };
+  b->n = n;
+  b->desc.assign(instances, instances + n);
+  if (cost_map_host) {
+    if (map_h < 2 || map_w < 2) return bail(fail(MPPI_E_INVALID, "batch: cost map must be at least 2x2"));
+    const int st = dev_alloc(&b->map_d, sizeof(float) * (size_t)map_h * map_w);
+    if (st != MPPI_OK) return bail(st);
+    const hipError_t e =
+        hipMemcpy(b->map_d, cost_map_host, sizeof(float) * (size_t)map_h * map_w, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy"));
+  }
+  b->inst_h.resize(n);
+  int64_t ubuf = 0, sums = 0, costs = 0;
+  int parts = 0;
+  for (int i = 0; i < n; ++i) {
+    mppi_instance_desc& d = b->desc[i];
+    const int m = d.dynamics.model;
+    if (m != MPPI_MODEL_DOUBLE_INTEGRATOR && m != MPPI_MODEL_RACER_DUBINS && m != MPPI_MODEL_CARTPOLE)
+      return bail(fail(MPPI_E_UNSUPPORTED, "batch: instance " + std::to_string(i) + ": model not batched"));
+    const int want_cost = (m == MPPI_MODEL_CARTPOLE) ? MPPI_COST_CARTPOLE_QUADRATIC : MPPI_COST_QUADRATIC_MAP;
+    if (d.cost.cost != want_cost)
+      return bail(fail(MPPI_E_UNSUPPORTED, "batch: instance " + std::to_string(i) + ": model/cost pair not batched"));
+    if (d.sampler.kind != MPPI_SAMPLER_GAUSSIAN)
+      return bail(fail(MPPI_E_UNSUPPORTED, "batch: instance " + std::to_string(i) + ": Gaussian sampling only"));
+    if (d.cost.cost == MPPI_COST_QUADRATIC_MAP && d.cost.map == nullptr && d.cost.map_coeff != 0.0f) {
+      if (!b->map_d) return bail(fail(MPPI_E_INVALID, "batch: instance " + std::to_string(i) + " needs a map"));
+      d.cost.map = b->map_d;
+      d.cost.map_height = map_h;
+      d.cost.map_width = map_w;
+    }
+    int st = validate_sizes(d.num_rollouts, d.num_timesteps);
+    if (st == MPPI_OK) st = validate_dynamics(d.dynamics);
+    if (st == MPPI_OK) st = validate_cost(d.cost, d.dynamics, true);
+    if (st == MPPI_OK) st = validate_sampler(d.sampler, d.dynamics);
+    if (st != MPPI_OK) return bail(fail(st, "batch: instance " + std::to_string(i) + ": " + mppi_last_error()));
+    if (d.num_rollouts % 64 != 0 || d.num_rollouts > BATCH_MAX_K)
+      return bail(fail(MPPI_E_INVALID, "batch: num_rollouts must be a multiple of 64 and at most 65536"));
+    if (!(d.lambda > 0.0f)) return bail(fail(MPPI_E_INVALID, "batch: lambda must be > 0"));
+    int S, C, O;
+    model_dims(m, &S, &C, &O);
+    const int K = d.num_rollouts, T = d.num_timesteps;
+    b->K.push_back(K);
+    b->T.push_back(T);
+    b->C.push_back(C);
+    b->S.push_back(S);
+    b->TCmax = std::max(b->TCmax, T * C);
+    b->Cmax = std::max(b->Cmax, C);
+    b->Omax = std::max(b->Omax, O);
+    BatchInstance& I = b->inst_h[i];
+    std::memset(&I, 0, sizeof(I));
+    I.dyn = d.dynamics;
+    I.cost = d.cost;
+    I.sp = d.sampler;
+    I.K = K;
+    I.T = T;
+    I.C = C;
+    I.pstart = pure_noise_start(K, d.sampler.pure_noise_trajectories_percentage);
+    I.mean_base = b->total_mean;
+    I.cost_base = (int)costs;
+    I.dt = d.dt;
+    I.lambda = d.lambda;
+    I.alpha = d.alpha;
+    b->mean_base.push_back(b->total_mean);
+    b->cost_base.push_back((int)costs);
+    b->total_mean += T * C;
+    const int t64 = (K + 63) / 64;
+    parts += t64;
+    sums += (int64_t)t64 * T * C;
+    ubuf += (int64_t)((K + 127) / 128) * 128 * T * C;
+    costs += K;
+    b->control.emplace_back((size_t)T * C, 0.0f);
+    if (d.init_control) std::copy(d.init_control, d.init_control + (size_t)T * C, b->control.back().begin());
+    b->raw_mean.emplace_back((size_t)T * C, 0.0f);
+    b->state_seq.emplace_back((size_t)T * S, 0.0f);
+  }
+  b->stats.assign(n, mppi_weight_stats{});
+  b->fe.assign(n, mppi_free_energy{});
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(hip_fail(e, "batch: device setup"));
+  int st = MPPI_OK;
+  auto A = [&](int s2) {
+    if (st == MPPI_OK) st = s2;
+  };
+  A(dev_alloc(&b->inst_d, sizeof(BatchInstance) * n));
+  A(dev_alloc(&b->work_d, sizeof(int2) * (size_t)parts));
+  A(dev_alloc(&b->queue_d, sizeof(int)));
+  A(dev_alloc(&b->iters_d, sizeof(IterArgs) * n));
+  A(dev_alloc(&b->loop_iters_d, sizeof(IterArgs) * n));
+  A(dev_alloc(&b->x0_d, sizeof(float) * BATCH_MAX_STATE * n));
+  A(dev_alloc(&b->means_d, sizeof(float) * b->total_mean));
+  A(dev_alloc(&b->means_backup_d, sizeof(float) * b->total_mean));
+  A(dev_alloc(&b->stats_d, sizeof(mppi_weight_stats) * n));
+  A(dev_alloc(&b->ubuf_d, sizeof(float) * (size_t)ubuf));
+  A(dev_alloc(&b->parts_d, sizeof(Partial) * (size_t)parts));
+  A(dev_alloc(&b->sums_d, sizeof(float) * (size_t)sums));
+  A(dev_alloc(&b->costs_d, sizeof(float) * (size_t)costs));
+  A(host_alloc(&b->iters_h, sizeof(IterArgs) * n));
+  A(host_alloc(&b->x0_h, sizeof(float) * BATCH_MAX_STATE * n));
+  A(host_alloc(&b->nominal_h, sizeof(float) * b->total_mean));
+  A(host_alloc(&b->mean_out_h, sizeof(float) * b->total_mean));
+  A(host_alloc(&b->stats_h, sizeof(mppi_weight_stats) * n));
+  if (st != MPPI_OK) return bail(st);
+  e = hipMemset(b->queue_d, 0, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(b->x0_d, 0, sizeof(float) * BATCH_MAX_STATE * n);
+  if (e == hipSuccess) e = hipMemset(b->means_d, 0, sizeof(float) * b->total_mean);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipMemset"));
+  // default launch shape: 1024 threads, one sample per lane, the largest chunk that fits
+  mppi_batch_config c{1024, 1, 16, 0, 0.0f};
+  st = MPPI_E_SIZE;
+  for (int ch : {16, 8, 4}) {
+    c.chunk = ch;
+    st = batch_apply_config(b, c);
+    if (st == MPPI_OK) break;
+  }
+  if (st != MPPI_OK) return bail(st);
+  *out = b;
+  return MPPI_OK;
+}
+
+int mppi_batch_destroy(mppi_batch* b)
+{
+  if (b && b->stream) (void)hipStreamSynchronize(b->stream);
+  return batch_free(b);
+}
+
+int mppi_batch_set_config(mppi_batch* b, const mppi_batch_config* c)
+{
+  if (!b || !c) return fail(MPPI_E_INVALID, "null argument");
+  if (c->samples_per_lane != 1 && c->samples_per_lane != 2)
+    return fail(MPPI_E_INVALID, "batch: samples_per_lane must be 1 or 2");
+  if (c->threads != 256 && c->threads != 512 && c->threads != 1024)
+    return fail(MPPI_E_INVALID, "batch: threads must be 256, 512 or 1024");
+  if (c->chunk != 4 && c->chunk != 8 && c->chunk != 16) return fail(MPPI_E_INVALID, "batch: chunk must be 4, 8 or 16");
+  MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  return batch_apply_config(b, *c);
+}
+
+int mppi_batch_get_config(const mppi_batch* b, mppi_batch_config* c)
+{
+  if (!b || !c) return fail(MPPI_E_INVALID, "null argument");
+  *c = b->cfg;
+  return MPPI_OK;
+}
+
+int mppi_batch_autotune(mppi_batch* b, int32_t reps, mppi_batch_config* best_out)
+{
+  if (!b) return fail(MPPI_E_INVALID, "null argument");
+  if (reps < 1) return fail(MPPI_E_INVALID, "batch: reps must be >= 1");
+  MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  MPPI_HIP_CHECK(hipMemcpy(b->means_backup_d, b->means_d, sizeof(float) * b->total_mean, hipMemcpyDeviceToDevice));
+  for (int i = 0; i < b->n; ++i) b->iters_h[i] = IterArgs{b->rng_counter, 1, 0};
+  MPPI_HIP_CHECK(hipMemcpy(b->iters_d, b->iters_h, sizeof(IterArgs) * b->n, hipMemcpyHostToDevice));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  MPPI_HIP_CHECK(hipEventCreate(&e0));
+  MPPI_HIP_CHECK(hipEventCreate(&e1));
+  mppi_batch_config best{0, 0, 0, 0, 0.0f};
+  int st = MPPI_OK;
+  for (int nt : {256, 512, 1024})
+    for (int spl : {1, 2})
+      for (int ch : {4, 8, 16}) {
+        mppi_batch_config c{nt, spl, ch, 0, 0.0f};
+        if (batch_apply_config(b, c) != MPPI_OK) continue;
+        float ms = 0.0f;
+        st = batch_enqueue_iteration(b, b->iters_d, 0);  // warm-up
+        if (st == MPPI_OK && hipEventRecord(e0, b->stream) != hipSuccess) st = MPPI_E_HIP;
+        for (int r = 0; r < reps && st == MPPI_OK; ++r) st = batch_enqueue_iteration(b, b->iters_d, 0);
+        if (st == MPPI_OK && hipEventRecord(e1, b->stream) != hipSuccess) st = MPPI_E_HIP;
+        if (st == MPPI_OK && hipEventSynchronize(e1) != hipSuccess) st = MPPI_E_HIP;
+        if (st == MPPI_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = MPPI_E_HIP;
+        if (hipMemcpy(b->means_d, b->means_backup_d, sizeof(float) * b->total_mean, hipMemcpyDeviceToDevice) !=
+            hipSuccess && st == MPPI_OK)
+          st = MPPI_E_HIP;
+        if (st != MPPI_OK) break;
+        c = b->cfg;
+        c.iteration_us = 1000.0f * ms / reps;
+        if (best.threads == 0 || c.iteration_us < best.iteration_us) best = c;
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != MPPI_OK) return st == MPPI_E_HIP ? fail(st, "batch autotune: HIP error") : st;
+  if (best.threads == 0) return fail(MPPI_E_SIZE, "batch autotune: no launch shape fits");
+  const float us = best.iteration_us;
+  MPPI_CHECK(batch_apply_config(b, best));
+  b->cfg.iteration_us = us;
+  if (best_out) *best_out = b->cfg;
+  return MPPI_OK;
+}
+
+int mppi_batch_compute_control(mppi_batch* b, const float* x0_host, int32_t stride)
+{
+  if (!b || !x0_host) return fail(MPPI_E_INVALID, "null argument");
+  for (int i = 0; i < b->n; ++i)
+    if (stride < 0 || stride > b->T[i]) return fail(MPPI_E_INVALID, "optimization_stride out of range");
+  for (int i = 0; i < b->n; ++i) {
+    b->fe[i].previous_baseline = b->stats[i].baseline;
+    for (int j = 0; j < BATCH_MAX_STATE; ++j) b->x0_h[i * BATCH_MAX_STATE + j] = x0_host[i * BATCH_MAX_STATE + j];
+    std::copy(b->control[i].begin(), b->control[i].end(), b->nominal_h + b->mean_base[i]);
+    b->iters_h[i] = IterArgs{b->rng_counter, stride, 0};
+  }
+  b->rng_counter++;
+  if (b->eager_runs > 0) {
+    if (!b->graph_exec) {
+      MPPI_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int st = batch_enqueue_call(b);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (st != MPPI_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      b->graph = g;
+      MPPI_HIP_CHECK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    }
+    MPPI_HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
+  } else {
+    MPPI_CHECK(batch_enqueue_call(b));
+    b->eager_runs++;
+  }
+  MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->n; ++i) {
+    const int T = b->T[i], C = b->C[i];
+    b->stats[i] = b->stats_h[i];
+    free_energy_from_stats(b->stats[i], b->desc[i].lambda, b->K[i], &b->fe[i]);
+    b->raw_mean[i].assign(b->mean_out_h + b->mean_base[i], b->mean_out_h + b->mean_base[i] + (size_t)T * C);
+    b->control[i] = b->raw_mean[i];
+    smooth_control(b->control[i], T, C);
+    propagate(b->desc[i].dynamics, b->x0_h + i * BATCH_MAX_STATE, b->control[i].data(), T, b->desc[i].dt,
+              b->state_seq[i].data());
+    clamp_controls_seq(b->desc[i].dynamics, b->control[i].data(), T);
+  }
+  return MPPI_OK;
+}
+
+int mppi_batch_step_device(mppi_batch* b, int32_t num_calls)
+{
+  if (!b) return fail(MPPI_E_INVALID, "null argument");
+  if (num_calls < 0) return fail(MPPI_E_INVALID, "num_calls must be >= 0");
+  if (num_calls == 0) return MPPI_OK;
+  if (!b->loop_exec) {
+    for (int i = 0; i < b->n; ++i) b->iters_h[i] = IterArgs{b->rng_counter + ((uint64_t)1 << 40), 1, 0};
+    MPPI_HIP_CHECK(hipMemcpyAsync(b->loop_iters_d, b->iters_h, sizeof(IterArgs) * b->n, hipMemcpyHostToDevice,
+                                  b->stream));
+    MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+    MPPI_CHECK(batch_enqueue_iteration(b, b->loop_iters_d, 1));  // eager pass before capture
+    MPPI_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int st = batch_enqueue_iteration(b, b->loop_iters_d, 1);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st != MPPI_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->loop_graph = g;
+    MPPI_HIP_CHECK(hipGraphInstantiate(&b->loop_exec, b->loop_graph, nullptr, nullptr, 0));
+    --num_calls;
+  }
+  for (int i = 0; i < num_calls; ++i) MPPI_HIP_CHECK(hipGraphLaunch(b->loop_exec, b->stream));
+  return MPPI_OK;
+}
+
+static int batch_index(const mppi_batch* b, int32_t i)
+{
+  if (!b) return fail(MPPI_E_INVALID, "null argument");
+  if (i < 0 || i >= b->n) return fail(MPPI_E_INVALID, "batch: instance index out of range");
+  return MPPI_OK;
+}
+
+int mppi_batch_get_control_seq(const mppi_batch* b, int32_t i, float* out)
+{
+  MPPI_CHECK(batch_index(b, i));
+  if (!out) return fail(MPPI_E_INVALID, "null argument");
+  std::copy(b->control[i].begin(), b->control[i].end(), out);
+  return MPPI_OK;
+}
+
+int mppi_batch_get_raw_mean(const mppi_batch* b, int32_t i, float* out)
+{
+  MPPI_CHECK(batch_index(b, i));
+  if (!out) return fail(MPPI_E_INVALID, "null argument");
+  std::copy(b->raw_mean[i].begin(), b->raw_mean[i].end(), out);
+  return MPPI_OK;
+}
+
+int mppi_batch_get_state_seq(const mppi_batch* b, int32_t i, float* out)
+{
+  MPPI_CHECK(batch_index(b, i));
+  if (!out) return fail(MPPI_E_INVALID, "null argument");
+  std::copy(b->state_seq[i].begin(), b->state_seq[i].end(), out);
+  return MPPI_OK;
+}
+
+int mppi_batch_get_weight_stats(const mppi_batch* b, int32_t i, mppi_weight_stats* out)
+{
+  MPPI_CHECK(batch_index(b, i));
+  if (!out) return fail(MPPI_E_INVALID, "null argument");
+  *out = b->stats[i];
+  return MPPI_OK;
+}
+
+int mppi_batch_get_free_energy(const mppi_batch* b, int32_t i, mppi_free_energy* out)
+{
+  MPPI_CHECK(batch_index(b, i));
+  if (!out) return fail(MPPI_E_INVALID, "null argument");
+  *out = b->fe[i];
+  return MPPI_OK;
+}
+
+int mppi_batch_set_control_seq(mppi_batch* b, int32_t i, const float* u)
+{
+  MPPI_CHECK(batch_index(b, i));
+  if (!u) return fail(MPPI_E_INVALID, "null argument");
+  std::copy(u, u + b->control[i].size(), b->control[i].begin());
+  return MPPI_OK;
+}
+
+int mppi_batch_slide_control_seq(mppi_batch* b, int32_t steps)
+{
+  if (!b) return fail(MPPI_E_INVALID, "null argument");
+  if (steps < 0) return fail(MPPI_E_INVALID, "steps must be >= 0");
+  // slideControlSequenceHelper (controllers/controller.cuh:609-621) per instance
+  for (int k = 0; k < b->n; ++k) {
+    const int T = b->T[k], C = b->C[k];
+    const float* zero = b->desc[k].dynamics.zero_control;
+    const float* scale = b->desc[k].slide_control_scale;
+    std::vector<float>& u = b->control[k];
+    for (int i = 0; i < T; ++i) {
+      const int ind = std::min(i + steps, T - 1);
+      for (int j = 0; j < C; ++j) {
+        float v = u[(size_t)ind * C + j];
+        if (i + steps > T - 1) v = (v - zero[j]) * scale[j] + zero[j];
+        u[(size_t)i * C + j] = v;
+      }
+    }
+  }
+  return MPPI_OK;
+}
+
+int mppi_batch_get_samples(const mppi_batch* b, int32_t i, float* samples_host, float* costs_host)
+{
+  MPPI_CHECK(batch_index(b, i));
+  MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  const BatchInstance& I = b->inst_h[i];
+  const int R = 64 * b->cfg.samples_per_lane, K = I.K, T = I.T, C = I.C;
+  if (samples_host) {
+    std::vector<float> tmp((size_t)I.tiles * T * R * C);
+    MPPI_HIP_CHECK(hipMemcpy(tmp.data(), b->ubuf_d + I.u_base, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; ++k) {
+      const int g = k / R, r = k % R;
+      for (int t = 0; t < T; ++t)
+        for (int c = 0; c < C; ++c)
+          samples_host[((size_t)k * T + t) * C + c] = tmp[(((size_t)g * T + t) * R + r) * C + c];
+    }
+  }
+  if (costs_host)
+    MPPI_HIP_CHECK(hipMemcpy(costs_host, b->costs_d + I.cost_base, sizeof(float) * K, hipMemcpyDeviceToHost));
+  return MPPI_OK;
+}
+
+int mppi_batch_stream(const mppi_batch* b, void** stream)
+{
+  if (!b || !stream) return fail(MPPI_E_INVALID, "null argument");
+  *stream = (void*)b->stream;
+  return MPPI_OK;
+}
+
+}  // extern "C"

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "fused.hpp"
+#include "host.hpp"
 #include "kernels.hpp"
 #include "models.hpp"
 
@@ -41,7 +42,7 @@ int pure_noise_start(int K, float pct)
   return std::min(k, K);
 }
 
-static int model_dims(int model, int* S, int* C, int* O)
+int model_dims(int model, int* S, int* C, int* O)
 {
   switch (model) {
     case MPPI_MODEL_CARTPOLE: *S = 4; *C = 1; *O = 4; return MPPI_OK;
@@ -52,7 +53,7 @@ static int model_dims(int model, int* S, int* C, int* O)
   }
 }
 
-static int validate_dynamics(const mppi_dynamics_params& d)
+int validate_dynamics(const mppi_dynamics_params& d)
 {
   int S, C, O;
   MPPI_CHECK(model_dims(d.model, &S, &C, &O));
@@ -79,7 +80,7 @@ static int validate_dynamics(const mppi_dynamics_params& d)
   return MPPI_OK;
 }
 
-static int validate_cost(const mppi_cost_params& c, const mppi_dynamics_params& d, bool need_map_ptr)
+int validate_cost(const mppi_cost_params& c, const mppi_dynamics_params& d, bool need_map_ptr)
 {
   if (c.cost != MPPI_COST_CARTPOLE_QUADRATIC && c.cost != MPPI_COST_QUADRATIC_MAP)
     return fail(MPPI_E_UNSUPPORTED, "unknown cost id " + std::to_string(c.cost));
@@ -99,7 +100,7 @@ static int validate_cost(const mppi_cost_params& c, const mppi_dynamics_params& 
   return MPPI_OK;
 }
 
-static int validate_sampler(const mppi_sampler_params& s, const mppi_dynamics_params& d)
+int validate_sampler(const mppi_sampler_params& s, const mppi_dynamics_params& d)
 {
   if (s.kind != MPPI_SAMPLER_GAUSSIAN && s.kind != MPPI_SAMPLER_COLORED)
     return fail(MPPI_E_UNSUPPORTED, "unknown sampler kind " + std::to_string(s.kind));
@@ -114,7 +115,7 @@ static int validate_sampler(const mppi_sampler_params& s, const mppi_dynamics_pa
   return MPPI_OK;
 }
 
-static int validate_sizes(int K, int T)
+int validate_sizes(int K, int T)
 {
   if (K < 1) return fail(MPPI_E_INVALID, "num_rollouts must be >= 1");
   if (T < 1) return fail(MPPI_E_INVALID, "num_timesteps must be >= 1");
@@ -125,7 +126,7 @@ static int validate_sizes(int K, int T)
 
 // ------------------------------------------------------------------------------------ host post-processing
 // Savitzky-Golay [-3, 12, 17, 12, -3] / 35 with edge replication, u[0] kept (controllers/controller.cuh:573-607).
-static void smooth_control(std::vector<float>& u, int T, int C)
+void smooth_control(std::vector<float>& u, int T, int C)
 {
   const float coef[5] = {-3.0f / 35.0f, 12.0f / 35.0f, 17.0f / 35.0f, 12.0f / 35.0f, -3.0f / 35.0f};
   std::vector<float> buf((size_t)(T + 4) * C);
@@ -161,7 +162,7 @@ static void propagate_states(const mppi_dynamics_params& p, const float* x0, con
 }
 
 // computeOutputTrajectoryHelper (controllers/controller.cuh:664-684) on the host.
-static void propagate(const mppi_dynamics_params& p, const float* x0, const float* u, int T, float dt, float* states)
+void propagate(const mppi_dynamics_params& p, const float* x0, const float* u, int T, float dt, float* states)
 {
   switch (p.model) {
     case MPPI_MODEL_CARTPOLE: propagate_states<MPPI_MODEL_CARTPOLE>(p, x0, u, T, dt, states); break;
@@ -178,21 +179,22 @@ static void clamp_sequence(const mppi_dynamics_params& p, float* u, int T)
   for (int t = 0; t < T; ++t) enforce_constraints<C>(p, u + t * C);
 }
 
-template <class Tp>
-static int dev_alloc(Tp** p, size_t bytes)
+void clamp_controls_seq(const mppi_dynamics_params& p, float* u, int T)
 {
-  hipError_t e = hipMalloc((void**)p, bytes);
-  if (e != hipSuccess) return fail(MPPI_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-  return MPPI_OK;
+  if (p.control_dim == 1) clamp_sequence<1>(p, u, T);
+  else clamp_sequence<2>(p, u, T);
 }
 
-template <class Tp>
-static int host_alloc(Tp** p, size_t bytes)
+void free_energy_from_stats(const mppi_weight_stats& st, double lambda, int K, mppi_free_energy* fe)
 {
-  hipError_t e = hipHostMalloc((void**)p, bytes, hipHostMallocDefault);
-  if (e != hipSuccess) return fail(MPPI_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
-  std::memset(*p, 0, bytes);
-  return MPPI_OK;
+  // core/mppi_common.cu:1101-1117, from the FP64 device sums
+  const double norm = st.normalizer / K;
+  fe->free_energy_mean = (float)(-lambda * std::log(norm) + st.baseline);
+  fe->free_energy_variance = (float)(lambda * (st.sum_w2 / K - norm * norm));
+  const double weird = fe->free_energy_variance / (norm * std::sqrt((double)K));
+  fe->free_energy_modified_variance = (float)(lambda * (weird + 0.5 * weird * weird));
+  fe->normalizer_percent = (float)norm;
+  fe->increase = st.baseline - fe->previous_baseline;
 }
 
 template <int MODEL>
@@ -674,21 +676,12 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
   c->stats = *c->stats_h;
   c->raw_mean.assign(c->mean_out_h, c->mean_out_h + (size_t)T * C);
   c->control = c->raw_mean;
-  // free-energy statistics from the FP64 device sums (core/mppi_common.cu:1101-1117)
-  const double norm = c->stats.normalizer / K;
-  const double lam = d.lambda;
-  c->fe.free_energy_mean = (float)(-lam * std::log(norm) + c->stats.baseline);
-  c->fe.free_energy_variance = (float)(lam * (c->stats.sum_w2 / K - norm * norm));
-  const double weird = c->fe.free_energy_variance / (norm * std::sqrt((double)K));
-  c->fe.free_energy_modified_variance = (float)(lam * (weird + 0.5 * weird * weird));
-  c->fe.normalizer_percent = (float)norm;
-  c->fe.increase = c->stats.baseline - c->fe.previous_baseline;
+  free_energy_from_stats(c->stats, d.lambda, K, &c->fe);
   smooth_control(c->control, T, C);
   mppi_dynamics_params hostdyn = d.dynamics;
   if (!c->nn_host.empty()) hostdyn.nn_theta = c->nn_host.data();
   propagate(hostdyn, c->x0_h, c->control.data(), T, d.dt, c->state_seq.data());
-  if (C == 1) clamp_sequence<1>(d.dynamics, c->control.data(), T);
-  else clamp_sequence<2>(d.dynamics, c->control.data(), T);
+  clamp_controls_seq(d.dynamics, c->control.data(), T);
   return MPPI_OK;
 }
 

@@ -147,3 +147,33 @@ def bicycle_nn(num_rollouts: int = 16384, num_timesteps: int = 100, seed: int = 
     theta = xavier_mlp() if residual else None
     return Problem("bicycle_nn" if residual else "bicycle", dyn, cost, smp, num_rollouts, num_timesteps, 0.02, 1.0,
                    0.0, x0, cost_map=m, nn_theta=theta)
+
+
+def batch_mix(num_instances: int = 256, seed: int = 7, k_range=(512, 8192), t_range=(50, 150),
+              cost_map: np.ndarray | None = None) -> tuple[list[Problem], np.ndarray]:
+    """Independent instances for the batched launch: a random 50/50 mix of double integrators and (plain)
+    bicycles, sample counts ~ U[k_range] rounded to a multiple of 64, horizons ~ U[t_range], and per-instance
+    seeded initial states, goals and noise seeds. All instances share one cost map (returned with the list)."""
+    rng = np.random.default_rng(seed)
+    m = gaussian_blob_map() if cost_map is None else cost_map
+    out = []
+    for i in range(num_instances):
+        K = int(round(rng.uniform(*k_range) / 64.0)) * 64
+        K = int(min(max(K, 64), k_range[1]))
+        T = int(rng.integers(t_range[0], t_range[1] + 1))
+        if rng.random() < 0.5:
+            p = double_integrator_map(num_rollouts=K, num_timesteps=T, colored=False, cost_map=m,
+                                      seed=int(1000 + i))
+            p.x0 = rng.uniform(-2.0, 2.0, 4).astype(np.float32)
+            p.cost.goal[0], p.cost.goal[1] = rng.uniform(-5.0, 5.0, 2)
+        else:
+            p = bicycle_nn(num_rollouts=K, num_timesteps=T, residual=False, cost_map=m, seed=int(1000 + i))
+            x0 = np.zeros(7, np.float32)
+            x0[0] = rng.uniform(2.0, 8.0)
+            x0[1] = rng.uniform(-np.pi, np.pi)
+            x0[2:4] = rng.uniform(-10.0, 10.0, 2)
+            p.x0 = x0
+            p.cost.speed_target = float(rng.uniform(4.0, 10.0))
+        p.name = f"{p.name}#{i}"
+        out.append(p)
+    return out, m

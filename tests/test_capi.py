@@ -40,7 +40,8 @@ def test_library_exports_every_declared_symbol(engine):
 def test_ctypes_layout_matches_c(tmp_path):
     structs = {"mppi_dynamics_params": capi.DynamicsParams, "mppi_cost_params": capi.CostParams,
                "mppi_sampler_params": capi.SamplerParams, "mppi_weight_stats": capi.WeightStats,
-               "mppi_free_energy": capi.FreeEnergy, "mppi_controller_desc": capi.ControllerDesc}
+               "mppi_free_energy": capi.FreeEnergy, "mppi_controller_desc": capi.ControllerDesc,
+               "mppi_instance_desc": capi.InstanceDesc, "mppi_batch_config": capi.BatchConfig}
     fields = []
     for cname, py in structs.items():
         for f in py._fields_:
