@@ -338,8 +338,8 @@ typedef struct mppi_instance_desc {
 } mppi_instance_desc;
 
 /* Launch shape of the persistent kernel: threads per workgroup (256, 512, 1024: one state-chain wavefront, the rest
- * evaluate costs and draw the next chunk's samples), samples per lane of the state-chain wavefront (1 or 2: 64 or
- * 128 samples per tile), horizon chunk (4, 8, 16 steps), and the resulting workgroups per CU. iteration_us is the
+ * evaluate costs and draw the next chunk's samples), samples per lane of the state-chain wavefront (1, 2 or 4: 64,
+ * 128 or 256 samples per tile), horizon chunk (4, 8, 16 steps), and the resulting workgroups per CU. iteration_us is the
  * measured time of one batched iteration (0 if not measured). */
 typedef struct mppi_batch_config {
   int32_t threads;

@@ -1,7 +1,7 @@
 // Many independent MPPI instances advanced by one persistent launch per iteration.
 //
 // The reference runs one VanillaMPPIController per system, each computeControl issuing its own sampling, rollout,
-// weight and update launches (controllers/MPPI/mppi_controller.cu:152-237). Here all instances' 64- or 128-sample
+// weight and update launches (controllers/MPPI/mppi_controller.cu:152-237). Here all instances
 // tiles form one work list, ordered longest horizon first, that a persistent kernel drains through an atomic
 // counter. Per tile, one workgroup:
 //   - keeps the instance's mean in LDS and draws the Gaussian samples chunk by chunk (Philox, the same element
@@ -86,7 +86,10 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
   // k*TC + t*C + c of the instance's stream; thread item (r, q) evaluates Philox block q of row r's range.
   auto gen = [&](int j, int t_off, int nth) {
     const int t0 = j * ch, t1 = min(T, t0 + ch);
-    const int nb = ((t1 - t0) * C + 3) / 4 + 1;
+    // Philox blocks per row: one more when the row's range does not start on a block boundary (k0 * TC is a
+    // multiple of 4 since R is, so only TC and t0 * C decide)
+    const bool aligned = (TC % 4 == 0) && ((t0 * C) % 4 == 0);
+    const int nb = ((t1 - t0) * C + 3) / 4 + (aligned ? 0 : 1);
     float* slot = ring + (j % 3) * ch * R * C;
     for (int item = tid - t_off; item < R * nb; item += nth) {
       const int r = item / nb, q = item - r * nb;
@@ -180,11 +183,10 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
 #pragma unroll
           for (int s = 0; s < SPL; ++s) {
             const int r = lane + 64 * s;
-            float yy[O], u[C];
-#pragma unroll
-            for (int i = 0; i < O; ++i) yy[i] = yb[((t - t0) * R + r) * O + i];
-#pragma unroll
-            for (int c = 0; c < C; ++c) u[c] = rs[((t - t0) * R + r) * C + c];
+            // y is read in place from LDS: the cost's runtime-indexed terms (map position, speed) become LDS
+            // addresses instead of register selects
+            const float* yy = yb + ((t - t0) * R + r) * O;
+            const float* u = rs + ((t - t0) * R + r) * C;
             acc[s] += CF::running(I.cost, yy) + lr_cost<C>(sp, u, mu + t * C, pure[s], I.lambda, I.alpha);
           }
         }
@@ -401,7 +403,8 @@ static int occupancy_t(size_t lds_bytes, int* out)
   return MPPI_OK;
 }
 
-#define MPPI_BATCH_SHAPES(X) X(256, 1) X(512, 1) X(1024, 1) X(256, 2) X(512, 2) X(1024, 2)
+#define MPPI_BATCH_SHAPES(X) \
+  X(256, 1) X(512, 1) X(1024, 1) X(256, 2) X(512, 2) X(1024, 2) X(256, 4) X(512, 4) X(1024, 4)
 
 static int batch_occupancy(int NT, int SPL, size_t lds_bytes, int* out)
 {
@@ -643,7 +646,7 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
     const int t64 = (K + 63) / 64;
     parts += t64;
     sums += (int64_t)t64 * T * C;
-    ubuf += (int64_t)((K + 127) / 128) * 128 * T * C;
+    ubuf += (int64_t)((K + 255) / 256) * 256 * T * C;
     costs += K;
     b->control.emplace_back((size_t)T * C, 0.0f);
     if (d.init_control) std::copy(d.init_control, d.init_control + (size_t)T * C, b->control.back().begin());
@@ -706,8 +709,8 @@ int mppi_batch_destroy(mppi_batch* b)
 int mppi_batch_set_config(mppi_batch* b, const mppi_batch_config* c)
 {
   if (!b || !c) return fail(MPPI_E_INVALID, "null argument");
-  if (c->samples_per_lane != 1 && c->samples_per_lane != 2)
-    return fail(MPPI_E_INVALID, "batch: samples_per_lane must be 1 or 2");
+  if (c->samples_per_lane != 1 && c->samples_per_lane != 2 && c->samples_per_lane != 4)
+    return fail(MPPI_E_INVALID, "batch: samples_per_lane must be 1, 2 or 4");
   if (c->threads != 256 && c->threads != 512 && c->threads != 1024)
     return fail(MPPI_E_INVALID, "batch: threads must be 256, 512 or 1024");
   if (c->chunk != 4 && c->chunk != 8 && c->chunk != 16) return fail(MPPI_E_INVALID, "batch: chunk must be 4, 8 or 16");
@@ -736,7 +739,7 @@ int mppi_batch_autotune(mppi_batch* b, int32_t reps, mppi_batch_config* best_out
   mppi_batch_config best{0, 0, 0, 0, 0.0f};
   int st = MPPI_OK;
   for (int nt : {256, 512, 1024})
-    for (int spl : {1, 2})
+    for (int spl : {1, 2, 4})
       for (int ch : {4, 8, 16}) {
         mppi_batch_config c{nt, spl, ch, 0, 0.0f};
         if (batch_apply_config(b, c) != MPPI_OK) continue;

@@ -148,14 +148,9 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
     } else if (!producer && it > 0) {
       const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
       const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * O + lane * O;
-      for (int t = t0 + wave - NN_PRODUCERS; t < t1; t += NN_CONSUMERS) {
-        float yy[O], u[C];
-#pragma unroll
-        for (int i = 0; i < O; ++i) yy[i] = yb[(t - t0) * RC_LANES * O + i];
-#pragma unroll
-        for (int c = 0; c < C; ++c) u[c] = crow[t * C + c];
-        acc += CF::running(cost, yy) + lr_cost<C>(sp, u, mu + t * C, pure, lambda, alpha);
-      }
+      for (int t = t0 + wave - NN_PRODUCERS; t < t1; t += NN_CONSUMERS)
+        acc += CF::running(cost, yb + (t - t0) * RC_LANES * O) +
+               lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
     }
     __syncthreads();
   }
@@ -246,12 +241,9 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
       const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
       const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * O + lane * O;
       for (int t = t0 + wave - 1; t < t1; t += RC_CONSUMERS) {
-        float yy[O], u[C];
-#pragma unroll
-        for (int i = 0; i < O; ++i) yy[i] = yb[(t - t0) * RC_LANES * O + i];
-#pragma unroll
-        for (int c = 0; c < C; ++c) u[c] = crow[t * C + c];
-        acc += CF::running(cost, yy) + lr_cost<C>(sp, u, mu + t * C, pure, lambda, alpha);
+        // y and u are read in place from LDS (runtime-indexed cost terms become LDS addresses)
+        acc += CF::running(cost, yb + (t - t0) * RC_LANES * O) +
+               lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
       }
     }
     __syncthreads();
