@@ -374,6 +374,37 @@ int mppi_batch_slide_control_seq(mppi_batch* batch, int32_t steps);
 int mppi_batch_get_samples(const mppi_batch* batch, int32_t instance, float* samples_host, float* costs_host);
 int mppi_batch_stream(const mppi_batch* batch, void** stream);
 
+/* ------------------------------------------------------------------------------------------------------------------
+ * Learned proposal network (no counterpart in the reference). Inputs: the state (S values) and the previous control
+ * sequence sampled at the 16 knots t_j = j (T - 1) / 15 (16 C values), each normalized as (v - shift) * scale;
+ * two hidden layers of 128 GELU units; 2 T C outputs: the mean shift in units of the base sampling std (first T C,
+ * [t][c]) and log(sigma_q / sigma_base) (last T C, clamped to [-4, 1] when used). Parameters in the FNN layout
+ * (W[out][in] row-major then bias, per layer). Forward, backward and Adam run on BF16 matrix cores with FP32
+ * accumulation. Loss per sample: mean over j of (r_j - y_j)^2 exp(-2 s_j) / 2 + s_j with the target residual
+ * r = (u* - u_prev) / sigma_base, averaged over the batch.
+ * ---------------------------------------------------------------------------------------------------------------- */
+typedef struct mppi_proposal mppi_proposal;
+
+int mppi_proposal_num_params(int32_t state_dim, int32_t control_dim, int32_t num_timesteps, int32_t* out);
+/* params_host: num_params floats, or NULL for all zeros (the proposal then equals the base distribution). */
+int mppi_proposal_create(int32_t state_dim, int32_t control_dim, int32_t num_timesteps, const float* params_host,
+                         mppi_proposal** out);
+int mppi_proposal_destroy(mppi_proposal* proposal);
+int mppi_proposal_get_params(const mppi_proposal* proposal, float* params_host);
+/* Sets the parameters and resets the optimizer state. */
+int mppi_proposal_set_params(mppi_proposal* proposal, const float* params_host);
+int mppi_proposal_set_input_normalization(mppi_proposal* proposal, const float* shift_host, const float* scale_host);
+/* outputs_d[b][2 T C] for already normalized inputs_d[b][S + 16 C] (device pointers). */
+int mppi_proposal_forward(mppi_proposal* proposal, const float* inputs_d, int32_t batch, float* outputs_d,
+                          void* stream);
+/* Loss (optionally copied to loss_host, which synchronizes) and its gradient grad_d[num_params] for targets_d
+ * [b][T C]. */
+int mppi_proposal_loss_grad(mppi_proposal* proposal, const float* inputs_d, const float* targets_d, int32_t batch,
+                            float* grad_d, double* loss_host, void* stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the batch; loss_host as above or NULL (no synchronization). */
+int mppi_proposal_train_step(mppi_proposal* proposal, const float* inputs_d, const float* targets_d, int32_t batch,
+                             float learning_rate, double* loss_host, void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

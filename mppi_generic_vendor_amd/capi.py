@@ -167,6 +167,15 @@ _SIGS = {
     "mppi_batch_slide_control_seq": ([P, _i], ct.c_int),
     "mppi_batch_get_samples": ([P, _i, P, P], ct.c_int),
     "mppi_batch_stream": ([P, ct.POINTER(P)], ct.c_int),
+    "mppi_proposal_num_params": ([_i, _i, _i, ct.POINTER(_i)], ct.c_int),
+    "mppi_proposal_create": ([_i, _i, _i, P, ct.POINTER(P)], ct.c_int),
+    "mppi_proposal_destroy": ([P], ct.c_int),
+    "mppi_proposal_get_params": ([P, P], ct.c_int),
+    "mppi_proposal_set_params": ([P, P], ct.c_int),
+    "mppi_proposal_set_input_normalization": ([P, P, P], ct.c_int),
+    "mppi_proposal_forward": ([P, P, _i, P, P], ct.c_int),
+    "mppi_proposal_loss_grad": ([P, P, P, _i, P, ct.POINTER(ct.c_double), P], ct.c_int),
+    "mppi_proposal_train_step": ([P, P, P, _i, _f, ct.POINTER(ct.c_double), P], ct.c_int),
 }
 
 _lib = None

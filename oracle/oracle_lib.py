@@ -36,6 +36,13 @@ _SIGS = {
     "oracle_fnn_forward_f64": ([_i, P, P, P, P], None),
     "oracle_fnn_forward_bf16_f64": ([_i, P, P, P, P], None),
     "oracle_bf16_round": ([_f], _f),
+    "oracle_proposal_num_params": ([_i, _i, _i], _i),
+    "oracle_gelu": ([_d], _d),
+    "oracle_dgelu": ([_d], _d),
+    "oracle_proposal_input": ([_i, _i, _i, P, P, P, P, P], None),
+    "oracle_proposal_forward": ([_i, _i, _i, P, _i, P, _i, P], None),
+    "oracle_proposal_loss_grad": ([_i, _i, _i, P, _i, P, P, _i, P], _d),
+    "oracle_adam": ([P, P, P, P, _i, _f, _f, _f, _f, _i], None),
     "oracle_map_query_f32": ([ct.POINTER(capi.CostParams), P, _f, _f], _f),
     "oracle_running_cost_f32": ([ct.POINTER(capi.CostParams), P, P], _f),
     "oracle_terminal_cost_f32": ([ct.POINTER(capi.CostParams), P], _f),
@@ -184,3 +191,22 @@ def enforce(prob, u: np.ndarray) -> np.ndarray:
     for row in v.reshape(-1, v.shape[-1]):
         load().oracle_enforce_constraints_f32(ct.byref(dyn), row.ctypes.data)
     return v
+
+
+# ------------------------------------------------------------------------------------------- proposal network
+def proposal_forward(S, C, T, theta, X, bf16=False):
+    X = np.ascontiguousarray(X, np.float32)
+    B = X.shape[0]
+    Y = np.empty((B, 2 * T * C), np.float64)
+    load().oracle_proposal_forward(S, C, T, ptr(np.ascontiguousarray(theta, np.float32)), B, ptr(X), int(bf16),
+                                   ptr(Y))
+    return Y
+
+
+def proposal_loss_grad(S, C, T, theta, X, R, bf16=False):
+    X = np.ascontiguousarray(X, np.float32)
+    R = np.ascontiguousarray(R, np.float32)
+    th = np.ascontiguousarray(theta, np.float32)
+    g = np.empty(th.size, np.float64)
+    loss = load().oracle_proposal_loss_grad(S, C, T, ptr(th), X.shape[0], ptr(X), ptr(R), int(bf16), ptr(g))
+    return loss, g
