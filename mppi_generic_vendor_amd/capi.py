@@ -84,7 +84,7 @@ class SamplerParams(ct.Structure):
 class WeightStats(ct.Structure):
     _fields_ = [
         ("baseline", _f), ("argmin", _i), ("normalizer", ct.c_double), ("sum_w2", ct.c_double),
-        ("lambda_", _f), ("num_rollouts", _i),
+        ("lambda_", _f), ("num_rollouts", _i), ("weighted_cost", ct.c_double),
     ]
 
 
@@ -176,6 +176,9 @@ _SIGS = {
     "mppi_proposal_forward": ([P, P, _i, P, P], ct.c_int),
     "mppi_proposal_loss_grad": ([P, P, P, _i, P, ct.POINTER(ct.c_double), P], ct.c_int),
     "mppi_proposal_train_step": ([P, P, P, _i, _f, ct.POINTER(ct.c_double), P], ct.c_int),
+    "mppi_controller_set_proposal": ([P, P, _i, _i], ct.c_int),
+    "mppi_controller_get_proposal_stats": ([P, ct.POINTER(_i), ct.POINTER(_i), ct.POINTER(WeightStats),
+                                            ct.POINTER(_f)], ct.c_int),
 }
 
 _lib = None

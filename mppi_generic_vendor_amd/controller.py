@@ -124,3 +124,20 @@ class VanillaMPPIController:
         s = ct.c_void_p()
         capi.call("mppi_controller_stream", self._h, ct.byref(s))
         return s.value
+
+    # ------------------------------------------------------------------------------------------ learned proposal
+    def setProposal(self, net, num_rollouts: int, fallback: bool = True):
+        """Attach a ProposalNet (None detaches): each computeControl then runs one proposal iteration of
+        num_rollouts samples, falling back to the plain iterations when it is rejected."""
+        self._proposal = net
+        capi.call("mppi_controller_set_proposal", self._h, None if net is None else net.handle,
+                  int(num_rollouts), 1 if fallback else 0)
+
+    def getProposalStats(self):
+        n = ct.c_int()
+        f = ct.c_int()
+        st = capi.WeightStats()
+        c0 = ct.c_float()
+        capi.call("mppi_controller_get_proposal_stats", self._h, ct.byref(n), ct.byref(f), ct.byref(st), ct.byref(c0))
+        return n.value, f.value, st, c0.value
+

@@ -222,23 +222,26 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
       argmin_combine(v, vi, cs[s], valid ? k0 + r : INT_MAX);
     }
     wave_argmin(v, vi);
-    double e = 0.0, e2 = 0.0;
+    double e = 0.0, e2 = 0.0, ec = 0.0;
 #pragma unroll
     for (int s = 0; s < SPL; ++s) {
-      const float w = (cs[s] <= FLT_MAX) ? expf(-(1.0f / I.lambda) * (cs[s] - v)) : 0.0f;
+      const bool fin = cs[s] <= FLT_MAX;
+      const float w = fin ? expf(-(1.0f / I.lambda) * (cs[s] - v)) : 0.0f;
       wl[lane + 64 * s] = w;
       e += (double)w;
       e2 += (double)w * (double)w;
+      if (fin) ec += (double)w * (double)cs[s];
     }
     e = wave_sum(e);
     e2 = wave_sum(e2);
+    ec = wave_sum(ec);
     if (lane == 0) {
       Partial p;
       p.baseline = v;
       p.argmin = vi;
       p.eta = e;
       p.s2 = e2;
-      p._pad = 0.0;
+      p.wc = ec;
       parts[I.part_base + g] = p;
     }
   }
@@ -314,7 +317,7 @@ __global__ void __launch_bounds__(BATCH_COMBINE_THREADS) batch_combine_kernel(
   __shared__ float escale[BATCH_MAX_TILES];
   __shared__ float sv[BATCH_COMBINE_THREADS / 64];
   __shared__ int si[BATCH_COMBINE_THREADS / 64];
-  __shared__ double se[BATCH_COMBINE_THREADS / 64], se2[BATCH_COMBINE_THREADS / 64];
+  __shared__ double se[BATCH_COMBINE_THREADS / 64], se2[BATCH_COMBINE_THREADS / 64], sec[BATCH_COMBINE_THREADS / 64];
   const int i = blockIdx.x;
   const BatchInstance& I = inst[i];
   const int G = I.tiles, TC = I.T * I.C;
@@ -335,25 +338,29 @@ __global__ void __launch_bounds__(BATCH_COMBINE_THREADS) batch_combine_kernel(
   for (int q = 1; q < BATCH_COMBINE_THREADS / 64; ++q) argmin_combine(b, bi, sv[q], si[q]);
   if (bi == INT_MAX) bi = 0;
   const double inv = 1.0 / (double)I.lambda;
-  double e = 0.0, e2 = 0.0;
+  double e = 0.0, e2 = 0.0, ec = 0.0;
   for (int q = tid; q < G; q += BATCH_COMBINE_THREADS) {
     const Partial p = P[q];
     const double sc = (p.baseline <= FLT_MAX) ? exp(-inv * ((double)p.baseline - (double)b)) : 0.0;
     escale[q] = (float)sc;
     e += p.eta * sc;
     e2 += p.s2 * sc * sc;
+    ec += p.wc * sc;
   }
   e = wave_sum(e);
   e2 = wave_sum(e2);
+  ec = wave_sum(ec);
   if (lane == 0) {
     se[wid] = e;
     se2[wid] = e2;
+    sec[wid] = ec;
   }
   __syncthreads();
-  double eta = 0.0, s2 = 0.0;
+  double eta = 0.0, s2 = 0.0, wcost = 0.0;
   for (int q = 0; q < BATCH_COMBINE_THREADS / 64; ++q) {
     eta += se[q];
     s2 += se2[q];
+    wcost += sec[q];
   }
   for (int j = tid; j < TC; j += BATCH_COMBINE_THREADS) {
     float acc = 0.0f;
@@ -368,6 +375,7 @@ __global__ void __launch_bounds__(BATCH_COMBINE_THREADS) batch_combine_kernel(
     st.sum_w2 = s2;
     st.lambda = I.lambda;
     st.num_rollouts = I.K;
+    st.weighted_cost = wcost / eta;
     stats[i] = st;
     if (advance) iters[i].rng_offset += 1;
     if (i == 0) *queue = 0;

@@ -10,6 +10,7 @@
 
 #include "fused.hpp"
 #include "host.hpp"
+#include "proposal.hpp"
 #include "kernels.hpp"
 #include "models.hpp"
 
@@ -247,6 +248,20 @@ struct mppi_controller {
   IterArgs* loop_iter_d = nullptr;  // free-running iteration arguments (rng offset advanced on the device)
   hipGraph_t loop_graph = nullptr;
   hipGraphExec_t loop_exec = nullptr;
+  // learned proposal (optional; the network object is owned by the caller)
+  ProposalNet* prop = nullptr;
+  int prop_K = 0;
+  int prop_fallback = 0;
+  float* prop_dmu_d = nullptr;
+  float* prop_sig_d = nullptr;
+  float* sigma_base_d = nullptr;
+  float* cost0_h = nullptr;
+  int prop_runs = 0;
+  hipGraph_t prop_graph = nullptr;
+  hipGraphExec_t prop_exec = nullptr;
+  int32_t n_prop = 0, n_fallback = 0;
+  mppi_weight_stats prop_stats{};
+  float prop_cost0 = 0.0f;
 };
 
 extern "C" {
@@ -513,11 +528,13 @@ static int controller_free(mppi_controller* c)
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->loop_exec) (void)hipGraphExecDestroy(c->loop_exec);
   if (c->loop_graph) (void)hipGraphDestroy(c->loop_graph);
+  if (c->prop_exec) (void)hipGraphExecDestroy(c->prop_exec);
+  if (c->prop_graph) (void)hipGraphDestroy(c->prop_graph);
   void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
-                 c->map_d, c->nn_d, c->loop_iter_d};
+                 c->map_d, c->nn_d, c->loop_iter_d, c->prop_dmu_d, c->prop_sig_d, c->sigma_base_d};
   for (void* p : dev)
     if (p) (void)hipFree(p);
-  void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h};
+  void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h};
   for (void* p : host)
     if (p) (void)hipHostFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -635,6 +652,63 @@ static int controller_enqueue(mppi_controller* c)
   return MPPI_OK;
 }
 
+// The proposal call: network inference on the uploaded state and nominal sequence, then one proposal iteration of
+// prop_K samples (importance-weight corrected costs), the new mean, statistics and the warm start's cost to the host.
+static int controller_enqueue_proposal(mppi_controller* c)
+{
+  const int T = c->T, C = c->C, S = c->S;
+  const hipStream_t s = c->stream;
+  const mppi_controller_desc& d = c->d;
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->iter_d, c->iter_h, sizeof(IterArgs), hipMemcpyHostToDevice, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->x0_d, c->x0_h, sizeof(float) * S, hipMemcpyHostToDevice, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->nominal_h, sizeof(float) * T * C, hipMemcpyHostToDevice, s));
+  MPPI_CHECK(proposal_infer(c->prop, c->x0_d, c->mean_d, c->sigma_base_d, c->prop_dmu_d, c->prop_sig_d, s));
+  MPPI_CHECK(launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->prop_K, T, d.dt, d.lambda, d.alpha, c->x0_d,
+                                    c->mean_d, c->samples_d, true, d.keep_samples != 0,
+                                    IterRef{c->iter_d, {0, 0, 0}}, c->costs_d, c->ws_d, c->stats_d, c->mean_d,
+                                    nullptr, s, c->prop_dmu_d, c->prop_sig_d));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_out_h, c->mean_d, sizeof(float) * T * C, hipMemcpyDeviceToHost, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->stats_h, c->stats_d, sizeof(mppi_weight_stats), hipMemcpyDeviceToHost, s));
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->cost0_h, c->costs_d, sizeof(float), hipMemcpyDeviceToHost, s));
+  return MPPI_OK;
+}
+
+// Runs one enqueue function eagerly on its first use and from a captured hipGraph afterwards (when use_graph).
+static int controller_run(mppi_controller* c, int (*enqueue)(mppi_controller*), hipGraph_t* graph,
+                          hipGraphExec_t* exec, int* runs)
+{
+  if (c->d.use_graph && *runs > 0) {
+    if (!*exec) {
+      MPPI_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int st = enqueue(c);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (st != MPPI_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      *graph = g;
+      MPPI_HIP_CHECK(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+    }
+    MPPI_HIP_CHECK(hipGraphLaunch(*exec, c->stream));
+  } else {
+    MPPI_CHECK(enqueue(c));
+    (*runs)++;
+  }
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return MPPI_OK;
+}
+
+static void controller_set_iters(mppi_controller* c, int stride, int n)
+{
+  for (int it = 0; it < n; ++it) {
+    c->iter_h[it].rng_offset = c->rng_counter++;
+    c->iter_h[it].stride = stride;
+    c->iter_h[it].iteration = it;
+  }
+}
+
 int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, int32_t stride)
 {
   if (!c || !x0_host) return fail(MPPI_E_INVALID, "null argument");
@@ -648,40 +722,79 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
   c->fe.previous_baseline = c->stats.baseline;
   for (int i = 0; i < S; ++i) c->x0_h[i] = x0_host[i];
   std::copy(c->control.begin(), c->control.end(), c->nominal_h);
-  for (int it = 0; it < d.num_iters; ++it) {
-    c->iter_h[it].rng_offset = c->rng_counter++;
-    c->iter_h[it].stride = stride;
-    c->iter_h[it].iteration = it;
+  bool plain = true;
+  int K_used = K;
+  if (c->prop) {
+    // learned proposal: one iteration of prop_K samples; with the fallback enabled it is rejected when its
+    // importance-weighted mean cost exceeds the cost of the warm start (sample 0), and a plain iteration runs
+    controller_set_iters(c, stride, 1);
+    MPPI_CHECK(controller_run(c, controller_enqueue_proposal, &c->prop_graph, &c->prop_exec, &c->prop_runs));
+    c->n_prop++;
+    c->prop_stats = *c->stats_h;
+    c->prop_cost0 = *c->cost0_h;
+    plain = c->prop_fallback && !(c->prop_stats.weighted_cost <= (double)c->prop_cost0);
+    if (plain) c->n_fallback++;
+    K_used = c->prop_K;
   }
-  if (d.use_graph && c->eager_runs > 0) {
-    if (!c->graph_exec) {
-      MPPI_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      const int st = controller_enqueue(c);
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(c->stream, &g);
-      if (st != MPPI_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return st;
-      }
-      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-      c->graph = g;
-      MPPI_HIP_CHECK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-    }
-    MPPI_HIP_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
-  } else {
-    MPPI_CHECK(controller_enqueue(c));
-    c->eager_runs++;
+  if (plain) {
+    controller_set_iters(c, stride, d.num_iters);
+    MPPI_CHECK(controller_run(c, controller_enqueue, &c->graph, &c->graph_exec, &c->eager_runs));
+    K_used = K;
   }
-  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
   c->stats = *c->stats_h;
   c->raw_mean.assign(c->mean_out_h, c->mean_out_h + (size_t)T * C);
   c->control = c->raw_mean;
-  free_energy_from_stats(c->stats, d.lambda, K, &c->fe);
+  free_energy_from_stats(c->stats, d.lambda, K_used, &c->fe);
   smooth_control(c->control, T, C);
   mppi_dynamics_params hostdyn = d.dynamics;
   if (!c->nn_host.empty()) hostdyn.nn_theta = c->nn_host.data();
   propagate(hostdyn, c->x0_h, c->control.data(), T, d.dt, c->state_seq.data());
   clamp_controls_seq(d.dynamics, c->control.data(), T);
+  return MPPI_OK;
+}
+
+int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, int32_t num_rollouts, int32_t fallback)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (c->prop_exec) (void)hipGraphExecDestroy(c->prop_exec);
+  if (c->prop_graph) (void)hipGraphDestroy(c->prop_graph);
+  c->prop_exec = nullptr;
+  c->prop_graph = nullptr;
+  c->prop_runs = 0;
+  if (!proposal) {
+    c->prop = nullptr;
+    return MPPI_OK;
+  }
+  const PropDims pd = proposal_dims(proposal_net_of(proposal));
+  if (pd.S != c->S || pd.C != c->C || pd.T != c->T)
+    return fail(MPPI_E_INVALID, "proposal: state/control/horizon dims do not match the controller");
+  if (c->d.sampler.kind != MPPI_SAMPLER_GAUSSIAN) return fail(MPPI_E_UNSUPPORTED, "proposal: Gaussian sampler only");
+  if (num_rollouts < 1 || num_rollouts > c->K)
+    return fail(MPPI_E_INVALID, "proposal: num_rollouts must be in [1, controller num_rollouts]");
+  const size_t tc = sizeof(float) * c->T * c->C;
+  if (!c->prop_dmu_d) {
+    MPPI_CHECK(dev_alloc(&c->prop_dmu_d, tc));
+    MPPI_CHECK(dev_alloc(&c->prop_sig_d, tc));
+    MPPI_CHECK(dev_alloc(&c->sigma_base_d, sizeof(float) * MPPI_MAX_CONTROL));
+    MPPI_CHECK(host_alloc(&c->cost0_h, sizeof(float)));
+  }
+  MPPI_HIP_CHECK(hipMemcpy(c->sigma_base_d, c->d.sampler.std_dev, sizeof(float) * MPPI_MAX_CONTROL,
+                           hipMemcpyHostToDevice));
+  c->prop = proposal_net_of(proposal);
+  c->prop_K = num_rollouts;
+  c->prop_fallback = fallback != 0;
+  return MPPI_OK;
+}
+
+int mppi_controller_get_proposal_stats(const mppi_controller* c, int32_t* proposal_calls, int32_t* fallbacks,
+                                       mppi_weight_stats* last_proposal_stats, float* last_warm_start_cost)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  if (proposal_calls) *proposal_calls = c->n_prop;
+  if (fallbacks) *fallbacks = c->n_fallback;
+  if (last_proposal_stats) *last_proposal_stats = c->prop_stats;
+  if (last_warm_start_cost) *last_warm_start_cost = c->prop_cost0;
   return MPPI_OK;
 }
 

@@ -24,29 +24,53 @@ size_t fused_iteration_workspace_bytes(int K, int T, int C)
   return G * sizeof(Partial) + G * (size_t)T * C * sizeof(float);
 }
 
-template <int MODEL, int COST, bool GEN, bool WRITE>
+// With PROP the samples come from the learned proposal: u = mu + dmu + sig_q eps (k >= 1; sample 0 and t < stride
+// keep mu; no pure-noise samples) and every cost carries the importance-weight correction (rollout_core.hpp).
+template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP>
 __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kernel(
     mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int K, int T, float dt, float lambda,
     float alpha, const float* __restrict__ x0, const float* __restrict__ mean, float* __restrict__ samples,
     IterRef itr, int pstart, float* __restrict__ costs_out, Partial* __restrict__ parts, float* __restrict__ sums,
-    int chunk)
+    int chunk, const float* __restrict__ prop_dmu, const float* __restrict__ prop_sig)
 {
   using M = Model<MODEL>;
   constexpr int C = M::C;
   constexpr int NT = rollout_threads<MODEL>();
   extern __shared__ float lds[];
   const int TC = T * C;
+  const int TC4 = (TC + 3) & ~3;
   const int ROW = TC + 1;
   float* tile = lds;
   float* mu = tile + RC_LANES * ROW;
-  float* wl = mu + ((TC + 3) & ~3);   // [64] local weights
-  float* scratch = wl + RC_LANES;     // pipelined-rollout buffers
+  float* pl = mu + TC4;                    // PROP: dmu, sig_q, 0.5 / sig_q^2, log(sig_q / sig_p) [4][TC4]
+  float* wl = pl + (PROP ? 4 * TC4 : 0);   // [64] local weights
+  float* scratch = wl + RC_LANES;          // pipelined-rollout buffers
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int g = blockIdx.x;
   const int k0 = g * RC_LANES;
   const int nrows = min(RC_LANES, K - k0);
   for (int j = tid; j < TC; j += NT) mu[j] = mean[j];
+  PropLds pv{};
+  if (PROP) {
+    const float decay_p = powf(sp.std_dev_decay, (float)load_iter_f(itr).iteration);
+    for (int j = tid; j < TC; j += NT) {
+      const float sq = prop_sig[j];
+      const float sp_c = decay_p * sp.std_dev[j % C];
+      pl[j] = prop_dmu[j];
+      pl[TC4 + j] = sq;
+      pl[2 * TC4 + j] = 0.5f / (sq * sq);
+      pl[3 * TC4 + j] = logf(sq / sp_c);
+    }
+    pv.dmu = pl;
+    pv.hq = pl + 2 * TC4;
+    pv.lr = pl + 3 * TC4;
+#pragma unroll
+    for (int c = 0; c < MPPI_MAX_CONTROL; ++c) {
+      const float sp_c = decay_p * sp.std_dev[c];
+      pv.hp[c] = c < C ? 0.5f / (sp_c * sp_c) : 0.0f;
+    }
+  }
   if (GEN) {
     // u = transform(eps) for the tile's elements; the tile is a contiguous span starting at element k0*T*C, which
     // is a multiple of 4, so Philox block b covers tile elements 4(b - base) .. 4(b - base) + 3.
@@ -68,6 +92,7 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
           const float m = mu[col];
           float v;
           if (k == 0 || t < a.stride) v = m;
+          else if (PROP) v = m + pl[col] + pl[TC4 + col] * z.z[j];
           else if (k >= pstart) v = decay * sp.std_dev[c] * z.z[j];
           else v = m + decay * sp.std_dev[c] * z.z[j];
           tile[r * ROW + col] = v;
@@ -91,7 +116,7 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
   __syncthreads();
   float* cost_l;
   rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
-                            cost_l);
+                            cost_l, PROP ? &pv : nullptr);
   if (tid < RC_LANES) {
     const int k = k0 + tid;
     const bool valid = lane < nrows;
@@ -102,17 +127,19 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
     float b = c;
     int bi = valid ? k : INT_MAX;
     wave_argmin(b, bi);
-    const float w = (valid && c <= FLT_MAX) ? expf(-(1.0f / lambda) * (c - b)) : 0.0f;
+    const bool fin = valid && c <= FLT_MAX;
+    const float w = fin ? expf(-(1.0f / lambda) * (c - b)) : 0.0f;
     wl[lane] = w;
     const double e = wave_sum((double)w);
     const double e2 = wave_sum((double)w * (double)w);
+    const double ec = wave_sum(fin ? (double)w * (double)c : 0.0);
     if (lane == 0) {
       Partial p;
       p.baseline = b;
       p.argmin = bi;
       p.eta = e;
       p.s2 = e2;
-      p._pad = 0.0;
+      p.wc = ec;
       parts[g] = p;
     }
   }
@@ -139,7 +166,7 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   __shared__ float escale[MAX_PARTIALS];
   __shared__ float sv[COMBINE_THREADS / 64];
   __shared__ int si[COMBINE_THREADS / 64];
-  __shared__ double se[COMBINE_THREADS / 64], se2[COMBINE_THREADS / 64];
+  __shared__ double se[COMBINE_THREADS / 64], se2[COMBINE_THREADS / 64], sec[COMBINE_THREADS / 64];
   __shared__ float acc_l[COMBINE_THREADS / COMBINE_COLS][COMBINE_COLS];
   constexpr int PER = MAX_PARTIALS / COMBINE_THREADS;
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
@@ -165,7 +192,7 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   int bi = si[0];
   for (int q = 1; q < COMBINE_THREADS / 64; ++q) argmin_combine(b, bi, sv[q], si[q]);
   if (bi == INT_MAX) bi = 0;
-  double e = 0.0, e2 = 0.0;
+  double e = 0.0, e2 = 0.0, ec = 0.0;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int q = tid + i * COMBINE_THREADS;
@@ -174,19 +201,23 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
       escale[q] = (float)sc;
       e += mine[i].eta * sc;
       e2 += mine[i].s2 * sc * sc;
+      ec += mine[i].wc * sc;
     }
   }
   e = wave_sum(e);
   e2 = wave_sum(e2);
+  ec = wave_sum(ec);
   if (lane == 0) {
     se[wid] = e;
     se2[wid] = e2;
+    sec[wid] = ec;
   }
   __syncthreads();
-  double eta = 0.0, s2 = 0.0;
+  double eta = 0.0, s2 = 0.0, wcost = 0.0;
   for (int q = 0; q < COMBINE_THREADS / 64; ++q) {
     eta += se[q];
     s2 += se2[q];
+    wcost += sec[q];
   }
   // columns of this workgroup: thread (grp, col) sums partials g = grp, grp + NG, ...
   constexpr int NG = COMBINE_THREADS / COMBINE_COLS;
@@ -213,32 +244,36 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
     st.sum_w2 = s2;
     st.lambda = lambda;
     st.num_rollouts = K;
+    st.weighted_cost = wcost / eta;
     *stats = st;
     if (advance != nullptr) advance->rng_offset += 1;
   }
 }
 
-template <int MODEL, int COST, bool GEN, bool WRITE>
+template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP>
 static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,
                      int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
-                     float* samples, IterRef it, float* costs_out, Partial* parts, float* sums, hipStream_t s)
+                     float* samples, IterRef it, float* costs_out, Partial* parts, float* sums, const float* prop_dmu,
+                     const float* prop_sig, hipStream_t s)
 {
   constexpr int C = Model<MODEL>::C, O = Model<MODEL>::O;
-  const size_t base = (size_t)RC_LANES * (T * C + 1) + (size_t)((T * C + 3) & ~3) + RC_LANES;
+  const size_t TC4 = (size_t)((T * C + 3) & ~3);
+  const size_t base = (size_t)RC_LANES * (T * C + 1) + TC4 * (PROP ? 5 : 1) + RC_LANES;
   const size_t limit = LDS_LIMIT / sizeof(float);
   const int chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
   if (chunk == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
   const size_t lds = sizeof(float) * (base + rollout_scratch_for<MODEL, O>(chunk));
   static bool attr_set = false;
   if (!attr_set) {
-    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE>,
+    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
     attr_set = true;
   }
-  const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
+  const int pstart = PROP ? K : pure_noise_start(K, sp.pure_noise_trajectories_percentage);
   const int G = (K + RC_LANES - 1) / RC_LANES;
-  rollout_update_kernel<MODEL, COST, GEN, WRITE><<<G, rollout_threads<MODEL>(), lds, s>>>(
-      dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, pstart, costs_out, parts, sums, chunk);
+  rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP><<<G, rollout_threads<MODEL>(), lds, s>>>(
+      dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, pstart, costs_out, parts, sums, chunk, prop_dmu,
+      prop_sig);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
@@ -247,26 +282,34 @@ template <int MODEL, int COST>
 static int dispatch_flags(bool gen, bool write, const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                           const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                           const float* x0, const float* mean, float* samples, IterRef it, float* costs_out,
-                          Partial* parts, float* sums, hipStream_t s)
+                          Partial* parts, float* sums, const float* prop_dmu, const float* prop_sig, hipStream_t s)
 {
+  if (prop_dmu != nullptr) {
+    if (!gen) return fail(MPPI_E_INVALID, "fused iteration: the learned proposal draws Gaussian samples");
+    if (write)
+      return launch_ru<MODEL, COST, true, true, true>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                                      costs_out, parts, sums, prop_dmu, prop_sig, s);
+    return launch_ru<MODEL, COST, true, false, true>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                                     costs_out, parts, sums, prop_dmu, prop_sig, s);
+  }
   if (gen && write)
-    return launch_ru<MODEL, COST, true, true>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                              costs_out, parts, sums, s);
+    return launch_ru<MODEL, COST, true, true, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                                     costs_out, parts, sums, nullptr, nullptr, s);
   if (gen)
-    return launch_ru<MODEL, COST, true, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                               costs_out, parts, sums, s);
+    return launch_ru<MODEL, COST, true, false, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                                      costs_out, parts, sums, nullptr, nullptr, s);
   if (write)
-    return launch_ru<MODEL, COST, false, true>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                               costs_out, parts, sums, s);
-  return launch_ru<MODEL, COST, false, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                              costs_out, parts, sums, s);
+    return launch_ru<MODEL, COST, false, true, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                                      costs_out, parts, sums, nullptr, nullptr, s);
+  return launch_ru<MODEL, COST, false, false, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
+                                                     costs_out, parts, sums, nullptr, nullptr, s);
 }
 
 int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                            const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                            const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
-                           IterArgs* advance, hipStream_t s)
+                           IterArgs* advance, hipStream_t s, const float* prop_dmu, const float* prop_sig)
 {
   const int C = dyn.control_dim;
   const int G = (K + RC_LANES - 1) / RC_LANES;
@@ -280,7 +323,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
 #define MPPI_FUSED_CASE(MODEL, COST)                                                                            \
   if (model == MODEL && cost.cost == COST)                                                                      \
     st = dispatch_flags<MODEL, COST>(generate, write_samples, dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, \
-                                     samples, it, costs_out, parts, sums, s);
+                                     samples, it, costs_out, parts, sums, prop_dmu, prop_sig, s);
   MPPI_FUSED_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
   MPPI_FUSED_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
   MPPI_FUSED_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)

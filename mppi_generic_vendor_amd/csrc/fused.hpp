@@ -11,7 +11,7 @@ struct Partial {
   int argmin;
   double eta;
   double s2;
-  double _pad;
+  double wc;  // sum_k w_k cost_k with the workgroup's weights (finite costs)
 };
 
 size_t fused_iteration_workspace_bytes(int K, int T, int C);
@@ -24,6 +24,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                            const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
-                           IterArgs* advance, hipStream_t s);
+                           IterArgs* advance, hipStream_t s, const float* prop_dmu = nullptr,
+                           const float* prop_sig = nullptr);
 
 }  // namespace mppi_amd

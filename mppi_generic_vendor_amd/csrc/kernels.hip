@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(1024) weights_kernel(float* __restrict__ costs
 {
   __shared__ float sv[16];
   __shared__ int si[16];
-  __shared__ double se[16], se2[16];
+  __shared__ double se[16], se2[16], sec[16];
   __shared__ float sbase;
   __shared__ int sidx;
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
@@ -230,26 +230,31 @@ __global__ void __launch_bounds__(1024) weights_kernel(float* __restrict__ costs
   __syncthreads();
   const float b = sbase;
   const float inv = 1.0f / lambda;
-  double e = 0.0, e2 = 0.0;
+  double e = 0.0, e2 = 0.0, ec = 0.0;
   for (int i = tid; i < K; i += blockDim.x) {
     const float c = costs[i];
-    const float w = (c <= FLT_MAX) ? expf(-inv * (c - b)) : 0.0f;
+    const bool fin = c <= FLT_MAX;
+    const float w = fin ? expf(-inv * (c - b)) : 0.0f;
     costs[i] = w;
     e += (double)w;
     e2 += (double)w * (double)w;
+    if (fin) ec += (double)w * (double)c;
   }
   e = wave_sum(e);
   e2 = wave_sum(e2);
+  ec = wave_sum(ec);
   if (lane == 0) {
     se[wid] = e;
     se2[wid] = e2;
+    sec[wid] = ec;
   }
   __syncthreads();
   if (tid == 0) {
-    double te = 0.0, te2 = 0.0;
+    double te = 0.0, te2 = 0.0, tec = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
       te += se[w];
       te2 += se2[w];
+      tec += sec[w];
     }
     mppi_weight_stats st;
     st.baseline = b;
@@ -258,6 +263,7 @@ __global__ void __launch_bounds__(1024) weights_kernel(float* __restrict__ costs
     st.sum_w2 = te2;
     st.lambda = lambda;
     st.num_rollouts = K;
+    st.weighted_cost = tec / te;
     *stats = st;
   }
 }

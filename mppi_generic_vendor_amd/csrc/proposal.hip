@@ -328,6 +328,9 @@ struct mppi_proposal {
   ProposalNet net;
 };
 
+ProposalNet* mppi_amd::proposal_net_of(mppi_proposal* q) { return &q->net; }
+PropDims mppi_amd::proposal_dims(const ProposalNet* p) { return p->d; }
+
 static void proposal_free(mppi_proposal* q)
 {
   if (!q) return;

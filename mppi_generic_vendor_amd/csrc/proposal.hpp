@@ -21,6 +21,8 @@ PropDims prop_dims(int S, int C, int T);
 // Inference for one state: writes the absolute mean shift dmu[T][C] = sigma_base[c] * y and the proposal std
 // sig[T][C] = sigma_base[c] * exp(clamp(s, PROP_S_LO, PROP_S_HI)). x0 [S], nominal [T][C] are device arrays.
 struct ProposalNet;
+ProposalNet* proposal_net_of(mppi_proposal* q);
+PropDims proposal_dims(const ProposalNet* p);
 int proposal_infer(ProposalNet* p, const float* x0, const float* nominal, const float* sigma_base, float* dmu,
                    float* sig, hipStream_t s);
 

@@ -67,6 +67,30 @@ constexpr int NN_THREADS = rollout_threads<MODEL_RACER_DUBINS_NN_BF16>();
 constexpr int NN_PRODUCERS = RC_LANES / 16;
 constexpr int NN_CONSUMERS = NN_THREADS / 64 - NN_PRODUCERS;
 
+// Importance-weight correction of a proposal sample (learned proposal q = N(mu + dmu, sig_q^2) instead of the base
+// p = N(mu, sig_p^2)): lambda (log q(u) - log p(u)) summed over the controls of one step, added to the sample's cost
+// so that exp(-cost / lambda) carries the density ratio p / q. The LDS arrays hold dmu, 0.5 / sig_q^2 and
+// log(sig_q / sig_p) per (t, c); hp is 0.5 / sig_p^2 per control.
+struct PropLds {
+  const float* dmu;
+  const float* hq;
+  const float* lr;
+  float hp[MPPI_MAX_CONTROL];
+};
+
+template <int C>
+__device__ __forceinline__ float is_term(const PropLds& p, const float* u, const float* mu, int j)
+{
+  float s = 0.0f;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const float d0 = u[c] - mu[c];
+    const float d1 = d0 - p.dmu[j + c];
+    s += d0 * d0 * p.hp[c] - d1 * d1 * p.hq[j + c] - p.lr[j + c];
+  }
+  return s;
+}
+
 // The same pipeline for MODEL_RACER_DUBINS_NN_BF16 (NN_THREADS threads): four producer wavefronts, wavefront w owning
 // samples 16 w .. 16 w + 15 of the tile (lane 16 g + n, every lane group g of a sample carrying its state), each
 // evaluating the network for its 16 samples per step on the matrix cores; the other wavefronts evaluate the costs.
@@ -76,7 +100,7 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
                                                 const mppi_sampler_params& sp, int T, float dt, float lambda,
                                                 float alpha, const float* __restrict__ x0, float* tile, int ROW,
                                                 const float* mu, int pstart, int k0, float* scratch, int chunk,
-                                                float*& cost_l_out)
+                                                float*& cost_l_out, const PropLds* prop)
 {
   using M = Model<MODEL_RACER_DUBINS_NN_BF16>;
   using CF = Cost<COST, M::O>;
@@ -148,9 +172,12 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
     } else if (!producer && it > 0) {
       const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
       const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * O + lane * O;
-      for (int t = t0 + wave - NN_PRODUCERS; t < t1; t += NN_CONSUMERS)
+      for (int t = t0 + wave - NN_PRODUCERS; t < t1; t += NN_CONSUMERS) {
         acc += CF::running(cost, yb + (t - t0) * RC_LANES * O) +
                lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
+        // the correction is not normalized by the horizon: scaled by T here, divided by T below
+        if (prop) acc += (float)T * lambda * is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
+      }
     }
     __syncthreads();
   }
@@ -173,11 +200,11 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
                                              const mppi_sampler_params& sp, int T, float dt, float lambda,
                                              float alpha, const float* __restrict__ x0, float* tile, int ROW,
                                              const float* mu, int pstart, int k0, float* scratch, int chunk,
-                                             float*& cost_l_out)
+                                             float*& cost_l_out, const PropLds* prop = nullptr)
 {
   if constexpr (MODEL == MODEL_RACER_DUBINS_NN_BF16) {
     rollout_tile_nn<COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
-                          cost_l_out);
+                          cost_l_out, prop);
     return;
   }
   using M = Model<MODEL>;
@@ -244,6 +271,8 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
         // y and u are read in place from LDS (runtime-indexed cost terms become LDS addresses)
         acc += CF::running(cost, yb + (t - t0) * RC_LANES * O) +
                lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
+        // the correction is not normalized by the horizon: scaled by T here, divided by T below
+        if (prop) acc += (float)T * lambda * is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
       }
     }
     __syncthreads();

@@ -43,6 +43,7 @@ _SIGS = {
     "oracle_proposal_forward": ([_i, _i, _i, P, _i, P, _i, P], None),
     "oracle_proposal_loss_grad": ([_i, _i, _i, P, _i, P, P, _i, P], _d),
     "oracle_adam": ([P, P, P, P, _i, _f, _f, _f, _f, _i], None),
+    "oracle_is_cost": ([P, P, P, P, P, _i, _i, _i, _d, P], None),
     "oracle_map_query_f32": ([ct.POINTER(capi.CostParams), P, _f, _f], _f),
     "oracle_running_cost_f32": ([ct.POINTER(capi.CostParams), P, P], _f),
     "oracle_terminal_cost_f32": ([ct.POINTER(capi.CostParams), P], _f),
@@ -92,8 +93,10 @@ def normals(seed: int, offset: int, n: int) -> np.ndarray:
 
 def gaussian_transform(sp, K, T, C, mean, eps, stride, iteration) -> np.ndarray:
     out = np.empty((K, T, C), dtype=np.float32)
-    load().oracle_gaussian_transform(ct.byref(sp), K, T, C, ptr(np.ascontiguousarray(mean, np.float32)),
-                                     ptr(np.ascontiguousarray(eps, np.float32)), stride, iteration, ptr(out))
+    _a0 = np.ascontiguousarray(mean, np.float32)
+    _a1 = np.ascontiguousarray(eps, np.float32)
+    load().oracle_gaussian_transform(ct.byref(sp), K, T, C, ptr(_a0),
+                                     ptr(_a1), stride, iteration, ptr(out))
     return out
 
 
@@ -108,8 +111,10 @@ def colored_scales(sp, T, C):
 def colored_samples(sp, K, T, C, mean, spectra, stride, iteration):
     eps = np.empty((K, T, C), dtype=np.float32)
     out = np.empty((K, T, C), dtype=np.float32)
-    load().oracle_colored_samples(ct.byref(sp), K, T, C, ptr(np.ascontiguousarray(mean, np.float32)),
-                                  ptr(np.ascontiguousarray(spectra, np.float32)), stride, iteration, ptr(eps),
+    _a0 = np.ascontiguousarray(mean, np.float32)
+    _a1 = np.ascontiguousarray(spectra, np.float32)
+    load().oracle_colored_samples(ct.byref(sp), K, T, C, ptr(_a0),
+                                  ptr(_a1), stride, iteration, ptr(eps),
                                   ptr(out))
     return eps, out
 
@@ -122,9 +127,11 @@ def rollout_f32(prob, samples: np.ndarray, mean: np.ndarray, nthreads: int = 1, 
     dyn = _host_dyn(prob)
     m = prob.cost_map
     x0 = prob.x0 if x0 is None else x0
+    _a0 = np.ascontiguousarray(x0, np.float32)
+    _a1 = np.ascontiguousarray(mean, np.float32)
     load().oracle_rollout_f32(ct.byref(dyn), ct.byref(prob.cost), ptr(m), ct.byref(prob.sampler), K, T, prob.dt,
-                              prob.lambda_, prob.alpha, ptr(np.ascontiguousarray(x0, np.float32)),
-                              ptr(np.ascontiguousarray(mean, np.float32)), ptr(s), ptr(costs), nthreads)
+                              prob.lambda_, prob.alpha, ptr(_a0),
+                              ptr(_a1), ptr(s), ptr(costs), nthreads)
     return costs, s
 
 
@@ -134,9 +141,11 @@ def rollout_f64(prob, samples: np.ndarray, mean: np.ndarray, x0=None):
     costs = np.empty(K, dtype=np.float64)
     dyn = _host_dyn(prob)
     x0 = prob.x0 if x0 is None else x0
+    _a0 = np.ascontiguousarray(x0, np.float32)
+    _a1 = np.ascontiguousarray(mean, np.float32)
     load().oracle_rollout_f64(ct.byref(dyn), ct.byref(prob.cost), ptr(prob.cost_map), ct.byref(prob.sampler), K, T,
-                              prob.dt, prob.lambda_, prob.alpha, ptr(np.ascontiguousarray(x0, np.float32)),
-                              ptr(np.ascontiguousarray(mean, np.float32)), ptr(s), ptr(costs))
+                              prob.dt, prob.lambda_, prob.alpha, ptr(_a0),
+                              ptr(_a1), ptr(s), ptr(costs))
     return costs, s
 
 
@@ -164,8 +173,10 @@ def weights(costs: np.ndarray, lam: float):
 def weighted_update(w: np.ndarray, eta: float, samples: np.ndarray) -> np.ndarray:
     K, T, C = samples.shape
     out = np.empty((T, C), dtype=np.float64)
-    load().oracle_weighted_update(ptr(np.ascontiguousarray(w, np.float64)), eta,
-                                  ptr(np.ascontiguousarray(samples, np.float32)), K, T, C, ptr(out))
+    _a0 = np.ascontiguousarray(w, np.float64)
+    _a1 = np.ascontiguousarray(samples, np.float32)
+    load().oracle_weighted_update(ptr(_a0), eta,
+                                  ptr(_a1), K, T, C, ptr(out))
     return out
 
 
@@ -180,8 +191,10 @@ def state_trajectory(prob, x0, u) -> np.ndarray:
     T, C = u.shape
     out = np.empty((T, prob.S), dtype=np.float32)
     dyn = _host_dyn(prob)
-    load().oracle_state_trajectory(ct.byref(dyn), ptr(np.ascontiguousarray(x0, np.float32)),
-                                   ptr(np.ascontiguousarray(u, np.float32)), T, prob.dt, ptr(out))
+    _a0 = np.ascontiguousarray(x0, np.float32)
+    _a1 = np.ascontiguousarray(u, np.float32)
+    load().oracle_state_trajectory(ct.byref(dyn), ptr(_a0),
+                                   ptr(_a1), T, prob.dt, ptr(out))
     return out
 
 
@@ -198,7 +211,8 @@ def proposal_forward(S, C, T, theta, X, bf16=False):
     X = np.ascontiguousarray(X, np.float32)
     B = X.shape[0]
     Y = np.empty((B, 2 * T * C), np.float64)
-    load().oracle_proposal_forward(S, C, T, ptr(np.ascontiguousarray(theta, np.float32)), B, ptr(X), int(bf16),
+    _a0 = np.ascontiguousarray(theta, np.float32)
+    load().oracle_proposal_forward(S, C, T, ptr(_a0), B, ptr(X), int(bf16),
                                    ptr(Y))
     return Y
 
@@ -210,3 +224,17 @@ def proposal_loss_grad(S, C, T, theta, X, R, bf16=False):
     g = np.empty(th.size, np.float64)
     loss = load().oracle_proposal_loss_grad(S, C, T, ptr(th), X.shape[0], ptr(X), ptr(R), int(bf16), ptr(g))
     return loss, g
+
+
+def is_cost(u, mean, dmu, sig, sig_p, lam):
+    u = np.ascontiguousarray(u, np.float32)
+    K, T, C = u.shape
+    out = np.empty(K, np.float64)
+    _a0 = np.ascontiguousarray(mean, np.float32)
+    _a1 = np.ascontiguousarray(dmu, np.float32)
+    _a2 = np.ascontiguousarray(sig, np.float32)
+    _a3 = np.ascontiguousarray(sig_p, np.float32)
+    load().oracle_is_cost(ptr(u), ptr(_a0),
+                          ptr(_a1), ptr(_a2),
+                          ptr(_a3), K, T, C, lam, ptr(out))
+    return out

@@ -206,6 +206,25 @@ double oracle_proposal_loss_grad(int S, int C, int T, const float* theta, int B,
   return loss;
 }
 
+/* Importance-weight correction per sample: lambda sum_{t,c} [ (u - mu)^2 / (2 sp_c^2) - (u - mu - dmu)^2 / (2 sq^2)
+ * - log(sq / sp_c) ] = lambda (log q(u) - log p(u)) for q = N(mu + dmu, sq^2), p = N(mu, sp^2). */
+void oracle_is_cost(const float* u, const float* mean, const float* dmu, const float* sig, const float* sig_p, int K,
+                    int T, int C, double lambda, double* out)
+{
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int t = 0; t < T; ++t)
+      for (int c = 0; c < C; ++c) {
+        const int j = t * C + c;
+        const double d0 = (double)u[((size_t)k * T + t) * C + c] - mean[j];
+        const double d1 = d0 - dmu[j];
+        const double sp = sig_p[c], sq = sig[j];
+        s += d0 * d0 / (2.0 * sp * sp) - d1 * d1 / (2.0 * sq * sq) - std::log(sq / sp);
+      }
+    out[k] = lambda * s;
+  }
+}
+
 /* Adam (Kingma & Ba 2015) with bias correction; `step` counts from 1. */
 void oracle_adam(float* theta, float* m, float* v, const float* grad, int P, float lr, float b1, float b2, float eps,
                  int step)

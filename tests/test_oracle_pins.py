@@ -243,7 +243,8 @@ def test_fnn_bf16_forward(oracle):
     L = np.array([6, 32, 32, 4], np.int32)
     theta = np.ones(1412, dtype=np.float32)
     out = np.zeros(4)
-    oracle.oracle_fnn_forward_bf16_f64(4, L.ctypes.data, theta.ctypes.data, np.zeros(6).ctypes.data,
+    zin = np.zeros(6)
+    oracle.oracle_fnn_forward_bf16_f64(4, L.ctypes.data, theta.ctypes.data, zin.ctypes.data,
                                        out.ctypes.data)
     np.testing.assert_array_equal(out, [33, 33, 33, 33])
     torch = pytest.importorskip("torch")
@@ -279,8 +280,9 @@ def test_fnn_bf16_forward(oracle):
 # -------------------------------------------------------------------------------------------------- dynamics
 def deriv(prob_dyn, x, u):
     xd = np.zeros(8)
-    ol.load().oracle_state_deriv_f64(ct.byref(prob_dyn), np.asarray(x, np.float64).ctypes.data,
-                                     np.asarray(u, np.float64).ctypes.data, xd.ctypes.data)
+    xa = np.asarray(x, np.float64)
+    ua = np.asarray(u, np.float64)
+    ol.load().oracle_state_deriv_f64(ct.byref(prob_dyn), xa.ctypes.data, ua.ctypes.data, xd.ctypes.data)
     return xd
 
 
