@@ -173,7 +173,8 @@ typedef struct mppi_weight_stats {
   double sum_w2;     /* sum_k w_k^2, FP64 (free-energy variance) */
   float lambda;
   int32_t num_rollouts;
-  double weighted_cost; /* sum_k w_k cost_k / normalizer (finite costs), FP64 */
+  double weighted_cost; /* sum_k w_k cost_k / normalizer over finite costs, FP64; with a learned proposal the costs
+                           averaged are those without the importance-weight correction */
 } mppi_weight_stats;
 
 /* Free-energy statistics (core/mppi_common.cu:1101-1117, controllers/controller.cuh:22-30). */
@@ -409,9 +410,10 @@ int mppi_proposal_train_step(mppi_proposal* proposal, const float* inputs_d, con
 /* Learned-proposal mode of a controller (NULL proposal: plain MPPI again). Each computeControl then runs the network
  * on (x0, nominal sequence) and one iteration of num_rollouts samples u = mu + dmu + sigma_q eps (sample 0 = mu, no
  * pure-noise samples), whose costs carry the importance-weight correction lambda (log q(u) - log p(u)) against the
- * base sampler p = N(mu, std_dev^2). With `fallback`, a proposal iteration whose importance-weighted mean cost
- * exceeds the warm start's cost (sample 0) is rejected and the plain iterations (num_rollouts of the controller,
- * num_iters) run instead. The proposal must stay alive while attached. */
+ * base sampler p = N(mu, std_dev^2) (sample 0, the warm start, carries none). With `fallback`, a proposal iteration
+ * whose weighted mean cost (weighted_cost: the uncorrected costs under the corrected weights) is not below the
+ * warm start's cost is rejected and the plain iterations (num_rollouts of the controller, num_iters) run instead. The
+ * proposal must stay alive while attached. */
 int mppi_controller_set_proposal(mppi_controller* ctrl, mppi_proposal* proposal, int32_t num_rollouts,
                                  int32_t fallback);
 int mppi_controller_get_proposal_stats(const mppi_controller* ctrl, int32_t* proposal_calls, int32_t* fallbacks,

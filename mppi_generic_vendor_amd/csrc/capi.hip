@@ -725,14 +725,14 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
   bool plain = true;
   int K_used = K;
   if (c->prop) {
-    // learned proposal: one iteration of prop_K samples; with the fallback enabled it is rejected when its
-    // importance-weighted mean cost exceeds the cost of the warm start (sample 0), and a plain iteration runs
+    // learned proposal: one iteration of prop_K samples; with the fallback enabled it is rejected unless its
+    // weighted mean cost is below the cost of the warm start (sample 0), and the plain iterations run instead
     controller_set_iters(c, stride, 1);
     MPPI_CHECK(controller_run(c, controller_enqueue_proposal, &c->prop_graph, &c->prop_exec, &c->prop_runs));
     c->n_prop++;
     c->prop_stats = *c->stats_h;
     c->prop_cost0 = *c->cost0_h;
-    plain = c->prop_fallback && !(c->prop_stats.weighted_cost <= (double)c->prop_cost0);
+    plain = c->prop_fallback && !(c->prop_stats.weighted_cost < (double)c->prop_cost0);
     if (plain) c->n_fallback++;
     K_used = c->prop_K;
   }

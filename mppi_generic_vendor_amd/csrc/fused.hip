@@ -115,8 +115,9 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
   }
   __syncthreads();
   float* cost_l;
+  float* raw_l;
   rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
-                            cost_l, PROP ? &pv : nullptr);
+                            cost_l, raw_l, PROP ? &pv : nullptr);
   if (tid < RC_LANES) {
     const int k = k0 + tid;
     const bool valid = lane < nrows;
@@ -132,7 +133,9 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
     wl[lane] = w;
     const double e = wave_sum((double)w);
     const double e2 = wave_sum((double)w * (double)w);
-    const double ec = wave_sum(fin ? (double)w * (double)c : 0.0);
+    // weighted mean of the uncorrected costs (the proposal's achieved cost)
+    const float raw = raw_l[lane];
+    const double ec = wave_sum((fin && raw <= FLT_MAX) ? (double)w * (double)raw : 0.0);
     if (lane == 0) {
       Partial p;
       p.baseline = b;

@@ -146,8 +146,9 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_kernel(mppi_
   for (int j = tid; j < TC; j += NT) mu[j] = mean[j];
   __syncthreads();
   float* cost_l;
+  float* raw_l;
   rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
-                            cost_l);
+                            cost_l, raw_l);
   if (tid < nrows) costs[k0 + tid] = cost_l[tid];
   tile_store<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
 }

@@ -22,12 +22,12 @@ constexpr int RC_LANES = 64;      // samples per workgroup (lanes of the produce
 constexpr int RC_THREADS = 1024;  // producer + 15 consumer wavefronts (all of them stage the tile)
 constexpr int RC_CONSUMERS = RC_THREADS / 64 - 1;
 
-// LDS floats the pipelined rollout needs besides the control tile: the two output chunks, the consumer partial sums,
-// the terminal costs and the per-sample costs.
+// LDS floats the pipelined rollout needs besides the control tile: the two output chunks, the consumer partial sums
+// (running costs and importance-weight corrections), the terminal costs, the costs and the uncorrected costs.
 template <int O>
 inline size_t rollout_scratch_floats(int chunk)
 {
-  return 2 * (size_t)chunk * RC_LANES * O + (size_t)RC_CONSUMERS * RC_LANES + 2 * RC_LANES;
+  return 2 * (size_t)chunk * RC_LANES * O + 2 * (size_t)RC_CONSUMERS * RC_LANES + 3 * RC_LANES;
 }
 
 // Largest chunk (a power of two, 4..32 steps) whose buffers fit in `avail` floats; 0 if none.
@@ -100,7 +100,7 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
                                                 const mppi_sampler_params& sp, int T, float dt, float lambda,
                                                 float alpha, const float* __restrict__ x0, float* tile, int ROW,
                                                 const float* mu, int pstart, int k0, float* scratch, int chunk,
-                                                float*& cost_l_out, const PropLds* prop)
+                                                float*& cost_l_out, float*& raw_l_out, const PropLds* prop)
 {
   using M = Model<MODEL_RACER_DUBINS_NN_BF16>;
   using CF = Cost<COST, M::O>;
@@ -108,8 +108,10 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
   static_assert(O == S, "the bicycle's output is its state");
   float* ybuf = scratch;                                  // [2][chunk][64][O]
   float* part = ybuf + 2 * chunk * RC_LANES * O;          // [NN_CONSUMERS][64]
-  float* term_l = part + NN_CONSUMERS * RC_LANES;         // [64]
+  float* part_is = part + NN_CONSUMERS * RC_LANES;        // [NN_CONSUMERS][64]
+  float* term_l = part_is + NN_CONSUMERS * RC_LANES;      // [64]
   float* cost_l = term_l + RC_LANES;                      // [64]
+  float* raw_l = cost_l + RC_LANES;                       // [64]
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int nch = (T + chunk - 1) / chunk;
@@ -135,7 +137,7 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
   bool deadband = false;
 #pragma unroll
   for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
-  float acc = 0.0f;
+  float acc = 0.0f, acc_is = 0.0f;
   const bool pure = (k0 + lane) >= pstart;
   const float* crow = tile + lane * ROW;
 
@@ -175,36 +177,46 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
       for (int t = t0 + wave - NN_PRODUCERS; t < t1; t += NN_CONSUMERS) {
         acc += CF::running(cost, yb + (t - t0) * RC_LANES * O) +
                lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
-        // the correction is not normalized by the horizon: scaled by T here, divided by T below
-        if (prop) acc += (float)T * lambda * is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
+        if (prop) acc_is += is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
       }
     }
     __syncthreads();
   }
-  if (!producer) part[(wave - NN_PRODUCERS) * RC_LANES + lane] = acc;
+  if (!producer) {
+    part[(wave - NN_PRODUCERS) * RC_LANES + lane] = acc;
+    part_is[(wave - NN_PRODUCERS) * RC_LANES + lane] = acc_is;
+  }
   __syncthreads();
   if (wave == 0) {
-    float run = 0.0f;
+    float run = 0.0f, isv = 0.0f;
 #pragma unroll
-    for (int w = 0; w < NN_CONSUMERS; ++w) run += part[w * RC_LANES + lane];
-    cost_l[lane] = run / (float)T + term_l[lane] / (float)T;
+    for (int w = 0; w < NN_CONSUMERS; ++w) {
+      run += part[w * RC_LANES + lane];
+      isv += part_is[w * RC_LANES + lane];
+    }
+    const float raw = run / (float)T + term_l[lane] / (float)T;
+    raw_l[lane] = raw;
+    // the warm start (sample 0) is not a draw from the proposal and carries no density ratio
+    cost_l[lane] = (prop && k0 + lane != 0) ? raw + lambda * isv : raw;
   }
   __syncthreads();
   cost_l_out = cost_l;
+  raw_l_out = raw_l;
 }
 
 // Runs the pipelined rollout of the tile; on return (after a barrier) cost_l[r] holds the cost of sample k0 + r for
-// every r < 64 (garbage for r >= nrows). `scratch` provides rollout_scratch_floats<O>(chunk) floats.
+// every r < 64 (garbage for r >= nrows) and raw_l[r] the same cost without the importance-weight correction (equal
+// to it without a proposal). `scratch` provides rollout_scratch_floats<O>(chunk) floats.
 template <int MODEL, int COST>
 __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                                              const mppi_sampler_params& sp, int T, float dt, float lambda,
                                              float alpha, const float* __restrict__ x0, float* tile, int ROW,
                                              const float* mu, int pstart, int k0, float* scratch, int chunk,
-                                             float*& cost_l_out, const PropLds* prop = nullptr)
+                                             float*& cost_l_out, float*& raw_l_out, const PropLds* prop = nullptr)
 {
   if constexpr (MODEL == MODEL_RACER_DUBINS_NN_BF16) {
     rollout_tile_nn<COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
-                          cost_l_out, prop);
+                          cost_l_out, raw_l_out, prop);
     return;
   }
   using M = Model<MODEL>;
@@ -212,8 +224,10 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
   constexpr int S = M::S, C = M::C, O = M::O;
   float* ybuf = scratch;                                  // [2][chunk][64][O]
   float* part = ybuf + 2 * chunk * RC_LANES * O;          // [15][64]
-  float* term_l = part + RC_CONSUMERS * RC_LANES;         // [64]
+  float* part_is = part + RC_CONSUMERS * RC_LANES;        // [15][64]
+  float* term_l = part_is + RC_CONSUMERS * RC_LANES;      // [64]
   float* cost_l = term_l + RC_LANES;                      // [64]
+  float* raw_l = cost_l + RC_LANES;                       // [64]
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int nch = (T + chunk - 1) / chunk;
@@ -233,7 +247,7 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
 #pragma unroll
   for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
   // consumer state
-  float acc = 0.0f;
+  float acc = 0.0f, acc_is = 0.0f;
   const bool pure = (k0 + lane) >= pstart;
   const float* crow = tile + lane * ROW;
 
@@ -271,22 +285,31 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
         // y and u are read in place from LDS (runtime-indexed cost terms become LDS addresses)
         acc += CF::running(cost, yb + (t - t0) * RC_LANES * O) +
                lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
-        // the correction is not normalized by the horizon: scaled by T here, divided by T below
-        if (prop) acc += (float)T * lambda * is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
+        if (prop) acc_is += is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
       }
     }
     __syncthreads();
   }
-  if (wave > 0) part[(wave - 1) * RC_LANES + lane] = acc;
+  if (wave > 0) {
+    part[(wave - 1) * RC_LANES + lane] = acc;
+    part_is[(wave - 1) * RC_LANES + lane] = acc_is;
+  }
   __syncthreads();
   if (wave == 0) {
-    float run = 0.0f;
+    float run = 0.0f, isv = 0.0f;
 #pragma unroll
-    for (int w = 0; w < RC_CONSUMERS; ++w) run += part[w * RC_LANES + lane];
-    cost_l[lane] = run / (float)T + term_l[lane] / (float)T;
+    for (int w = 0; w < RC_CONSUMERS; ++w) {
+      run += part[w * RC_LANES + lane];
+      isv += part_is[w * RC_LANES + lane];
+    }
+    const float raw = run / (float)T + term_l[lane] / (float)T;
+    raw_l[lane] = raw;
+    // the warm start (sample 0) is not a draw from the proposal and carries no density ratio
+    cost_l[lane] = (prop && k0 + lane != 0) ? raw + lambda * isv : raw;
   }
   __syncthreads();
   cost_l_out = cost_l;
+  raw_l_out = raw_l;
 }
 
 }  // namespace mppi_amd

@@ -206,7 +206,7 @@ double oracle_proposal_loss_grad(int S, int C, int T, const float* theta, int B,
   return loss;
 }
 
-/* Importance-weight correction per sample: lambda sum_{t,c} [ (u - mu)^2 / (2 sp_c^2) - (u - mu - dmu)^2 / (2 sq^2)
+/* Importance-weight correction per sample (the caller leaves it off the warm-start sample 0): lambda sum_{t,c} [ (u - mu)^2 / (2 sp_c^2) - (u - mu - dmu)^2 / (2 sq^2)
  * - log(sq / sp_c) ] = lambda (log q(u) - log p(u)) for q = N(mu + dmu, sq^2), p = N(mu, sp^2). */
 void oracle_is_cost(const float* u, const float* mean, const float* dmu, const float* sig, const float* sig_p, int K,
                     int T, int C, double lambda, double* out)

@@ -89,3 +89,84 @@ def test_training_lowers_the_loss(hip, engine):
     print(f"[proposal] loss {l0:.4f} -> {l1:.4f} ({hist[0]:.4f} .. {hist[-1]:.4f})")
     assert l1 < 0.5 * l0
     net.close()
+
+
+def _proposal_problem():
+    from mppi_generic_vendor_amd import capi, workloads
+
+    small = workloads.gaussian_blob_map(256, 256, blobs=16, seed=7)
+    return workloads.bicycle_nn(num_rollouts=1024, num_timesteps=100, cost_map=small, precision=capi.NN_BF16)
+
+
+def test_proposal_iteration_matches_oracle(hip, engine):
+    """One learned-proposal iteration: network on (x0, nominal), samples mu + dmu + sig_q eps from the same Philox
+    stream, costs with the importance-weight correction, weights and update -- against the oracle chain."""
+    from gpu_util import gpu_normals, rel_err
+    from mppi_generic_vendor_amd.controller import VanillaMPPIController
+    from mppi_generic_vendor_amd.proposal import network_inputs
+
+    banner("proposal controller iteration vs oracle")
+    prob = _proposal_problem()
+    S, C, T = 7, 2, 100
+    TC = T * C
+    th = _random_params(S, C, T, 11)
+    net = ProposalNet(S, C, T, th)
+    ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=False)
+    Kp = 512
+    ctrl.setProposal(net, Kp, fallback=False)
+    nominal = np.zeros((T, C), np.float32)
+    ctrl.computeControl(prob.x0, 1)
+    raw = ctrl.getRawMean()
+    ncalls, nfall, st, c0 = ctrl.getProposalStats()
+    assert ncalls == 1 and nfall == 0
+    X = network_inputs(prob.x0[None, :], nominal[None])
+    y = ol.proposal_forward(S, C, T, th, X, bf16=True)[0]
+    sb = np.array([prob.sampler.std_dev[c] for c in range(C)], np.float32)
+    dmu = (y[:TC].reshape(T, C) * sb).astype(np.float32)
+    sig = (np.exp(np.clip(y[TC:], -4.0, 1.0)).reshape(T, C) * sb).astype(np.float32)
+    eps = gpu_normals(prob.sampler.seed, 0, Kp * TC).reshape(Kp, T, C)
+    u = (nominal + dmu + sig * eps).astype(np.float32)
+    u[0] = nominal
+    u[:, 0] = nominal[0]
+    costs, clamped = ol.rollout_f32(prob, u, nominal, nthreads=8)
+    isc = ol.is_cost(clamped, nominal, dmu, sig, sb, prob.lambda_)
+    isc[0] = 0.0  # the warm start is not a draw from the proposal
+    total = (costs + isc).astype(np.float32)
+    w, eta, b, idx, s2 = ol.weights(total, prob.lambda_)
+    want = ol.weighted_update(w, eta, clamped)
+    wc = float(np.sum(w * costs) / eta)
+    scale = max(1e-3, float(np.max(np.abs(want))))
+    err = float(np.max(np.abs(raw - want))) / scale
+    print(f"[proposal] iteration: mean rel err {err:.2e}, argmin {st.argmin} vs {idx}, baseline {st.baseline:.5f} "
+          f"vs {b:.5f}, weighted cost {st.weighted_cost:.5f} vs {wc:.5f}, warm start {c0:.5f} vs {total[0]:.5f}")
+    assert st.argmin == idx
+    assert st.baseline == pytest.approx(b, rel=1e-3)
+    assert st.weighted_cost == pytest.approx(wc, rel=1e-3)
+    assert c0 == pytest.approx(float(total[0]), rel=1e-3)
+    assert err < 1e-3
+    ctrl.close()
+    net.close()
+
+
+def test_proposal_fallback_runs_plain_iteration(hip, engine):
+    """A proposal whose samples are far off the warm start leaves all weight on the warm start (its importance
+    corrections are large), so its weighted cost is not below the warm start's: it is rejected and the plain
+    iteration with the controller's full sample count runs instead."""
+    from mppi_generic_vendor_amd.controller import VanillaMPPIController
+
+    banner("proposal fallback")
+    prob = _proposal_problem()
+    S, C, T = 7, 2, 100
+    th = init_params(S, C, T, 2)
+    th[-2 * T * C:-T * C] = 4.0  # mean shift of 4 sigma on every step and control
+    net = ProposalNet(S, C, T, th)
+    ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+    ctrl.setProposal(net, 256, fallback=True)
+    for _ in range(3):
+        ctrl.computeControl(prob.x0, 1)
+    ncalls, nfall, st, c0 = ctrl.getProposalStats()
+    print(f"[proposal] fallback: {nfall} of {ncalls}, proposal weighted cost {st.weighted_cost:.3f} vs warm {c0:.3f}")
+    assert ncalls == 3 and nfall == 3
+    assert ctrl.getWeightStats().num_rollouts == prob.num_rollouts
+    ctrl.close()
+    net.close()
