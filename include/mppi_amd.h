@@ -210,6 +210,8 @@ int mppi_pure_noise_start(int32_t num_rollouts, float pure_noise_percentage, int
 int mppi_host_step(const mppi_dynamics_params* p, const float* x, const float* u, float dt, float* x_next,
                    float* x_dot, float* y);
 int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u);
+/* Cost::computeRunningCost on the host (the kernels' cost code; c->map must point to host memory here). */
+int mppi_host_running_cost(const mppi_cost_params* c, const float* y, float* out);
 
 /* ------------------------------------------------------------------------------------------------------------------
  * Kernel-level entry points (device pointers, caller stream).
@@ -380,7 +382,7 @@ int mppi_batch_stream(const mppi_batch* batch, void** stream);
  * Learned proposal network (no counterpart in the reference). Inputs: the state (S values) and the previous control
  * sequence sampled at the 16 knots t_j = j (T - 1) / 15 (16 C values), each normalized as (v - shift) * scale;
  * two hidden layers of 128 GELU units; 2 T C outputs: the mean shift in units of the base sampling std (first T C,
- * [t][c]) and log(sigma_q / sigma_base) (last T C, clamped to [-4, 1] when used). Parameters in the FNN layout
+ * [t][c]) and log(sigma_q / sigma_base) (last T C, clamped to [-1.5, 1] when used). Parameters in the FNN layout
  * (W[out][in] row-major then bias, per layer). Forward, backward and Adam run on BF16 matrix cores with FP32
  * accumulation. Loss per sample: mean over j of (r_j - y_j)^2 exp(-2 s_j) / 2 + s_j with the target residual
  * r = (u* - u_prev) / sigma_base, averaged over the batch.
@@ -399,6 +401,12 @@ int mppi_proposal_set_input_normalization(mppi_proposal* proposal, const float* 
 /* outputs_d[b][2 T C] for already normalized inputs_d[b][S + 16 C] (device pointers). */
 int mppi_proposal_forward(mppi_proposal* proposal, const float* inputs_d, int32_t batch, float* outputs_d,
                           void* stream);
+/* Proposal of one state as the controller uses it: the input from x0_d[S] and the nominal sequence nominal_d[T][C]
+ * (normalized with the stored shift/scale), the network, then dmu_d[T][C] = sigma_base * mean shift and
+ * sig_d[T][C] = sigma_base * exp(clamp(log-std, -1.5, 1)); sigma_base_d[C]. fused = 1: one single-workgroup kernel
+ * (the controller's path); 0: the input kernel, the batched forward launches and a head kernel. */
+int mppi_proposal_infer(mppi_proposal* proposal, const float* x0_d, const float* nominal_d,
+                        const float* sigma_base_d, float* dmu_d, float* sig_d, int32_t fused, void* stream);
 /* Loss (optionally copied to loss_host, which synchronizes) and its gradient grad_d[num_params] for targets_d
  * [b][T C]. */
 int mppi_proposal_loss_grad(mppi_proposal* proposal, const float* inputs_d, const float* targets_d, int32_t batch,
@@ -408,16 +416,21 @@ int mppi_proposal_train_step(mppi_proposal* proposal, const float* inputs_d, con
                              float learning_rate, double* loss_host, void* stream);
 
 /* Learned-proposal mode of a controller (NULL proposal: plain MPPI again). Each computeControl then runs the network
- * on (x0, nominal sequence) and one iteration of num_rollouts samples u = mu + dmu + sigma_q eps (sample 0 = mu, no
- * pure-noise samples), whose costs carry the importance-weight correction lambda (log q(u) - log p(u)) against the
- * base sampler p = N(mu, std_dev^2) (sample 0, the warm start, carries none). With `fallback`, a proposal iteration
- * whose weighted mean cost (weighted_cost: the uncorrected costs under the corrected weights) is not below the
- * warm start's cost is rejected and the plain iterations (num_rollouts of the controller, num_iters) run instead. The
- * proposal must stay alive while attached. */
+ * on (x0, nominal sequence) and one iteration of num_rollouts samples u = mu + dmu + sigma_q eps (no pure-noise
+ * samples), whose costs carry the mean-shift importance correction lambda sum ((u - mu)^2 - (u - mu - dmu)^2) /
+ * (2 std_dev^2) (log-density ratio of the shifted to the base Gaussian at the base std; the proposal's spread sets
+ * the exploration width without a density ratio). Sample 0 is the warm start mu itself: its (uncorrected) cost is evaluated but it takes no
+ * part in the weights. With fallback_tolerance >= 0, a proposal iteration whose weighted mean cost (weighted_cost:
+ * the uncorrected costs under the corrected weights) exceeds the warm start's cost c0 by more than
+ * fallback_tolerance * |c0| is rejected and the plain iterations (num_rollouts of the controller, num_iters) run
+ * instead; a negative tolerance disables the fallback. The proposal must stay alive while attached. */
 int mppi_controller_set_proposal(mppi_controller* ctrl, mppi_proposal* proposal, int32_t num_rollouts,
-                                 int32_t fallback);
+                                 float fallback_tolerance);
 int mppi_controller_get_proposal_stats(const mppi_controller* ctrl, int32_t* proposal_calls, int32_t* fallbacks,
                                        mppi_weight_stats* last_proposal_stats, float* last_warm_start_cost);
+/* Stream time of the last computeControl from its first upload to its last download (HIP events), i.e. the
+ * control latency without the host post-processing (smoothing, state propagation). */
+int mppi_controller_get_last_call_ms(const mppi_controller* ctrl, float* ms);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -130,6 +130,7 @@ _SIGS = {
     "mppi_pure_noise_start": ([_i, _f, ct.POINTER(_i)], ct.c_int),
     "mppi_host_step": ([ct.POINTER(DynamicsParams), P, P, _f, P, P, P], ct.c_int),
     "mppi_host_enforce_constraints": ([ct.POINTER(DynamicsParams), P], ct.c_int),
+    "mppi_host_running_cost": ([ct.POINTER(CostParams), P, ct.POINTER(_f)], ct.c_int),
     "mppi_generate_normals": ([ct.c_uint64, ct.c_uint64, P, ct.c_int64, P], ct.c_int),
     "mppi_sample_gaussian": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),
     "mppi_sample_colored": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),
@@ -174,11 +175,13 @@ _SIGS = {
     "mppi_proposal_set_params": ([P, P], ct.c_int),
     "mppi_proposal_set_input_normalization": ([P, P, P], ct.c_int),
     "mppi_proposal_forward": ([P, P, _i, P, P], ct.c_int),
+    "mppi_proposal_infer": ([P, P, P, P, P, P, _i, P], ct.c_int),
     "mppi_proposal_loss_grad": ([P, P, P, _i, P, ct.POINTER(ct.c_double), P], ct.c_int),
     "mppi_proposal_train_step": ([P, P, P, _i, _f, ct.POINTER(ct.c_double), P], ct.c_int),
-    "mppi_controller_set_proposal": ([P, P, _i, _i], ct.c_int),
+    "mppi_controller_set_proposal": ([P, P, _i, _f], ct.c_int),
     "mppi_controller_get_proposal_stats": ([P, ct.POINTER(_i), ct.POINTER(_i), ct.POINTER(WeightStats),
                                             ct.POINTER(_f)], ct.c_int),
+    "mppi_controller_get_last_call_ms": ([P, ct.POINTER(_f)], ct.c_int),
 }
 
 _lib = None

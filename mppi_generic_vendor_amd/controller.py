@@ -126,12 +126,14 @@ class VanillaMPPIController:
         return s.value
 
     # ------------------------------------------------------------------------------------------ learned proposal
-    def setProposal(self, net, num_rollouts: int, fallback: bool = True):
+    def setProposal(self, net, num_rollouts: int, fallback_tolerance: float | None = 0.0):
         """Attach a ProposalNet (None detaches): each computeControl then runs one proposal iteration of
-        num_rollouts samples, falling back to the plain iterations when it is rejected."""
+        num_rollouts samples, falling back to the plain iterations when its weighted cost exceeds the warm start's
+        by more than fallback_tolerance (relative; None disables the fallback)."""
         self._proposal = net
-        capi.call("mppi_controller_set_proposal", self._h, None if net is None else net.handle,
-                  int(num_rollouts), 1 if fallback else 0)
+        tol = -1.0 if fallback_tolerance is None else float(fallback_tolerance)
+        capi.call("mppi_controller_set_proposal", self._h, None if net is None else net.handle, int(num_rollouts),
+                  tol)
 
     def getProposalStats(self):
         n = ct.c_int()
@@ -141,3 +143,8 @@ class VanillaMPPIController:
         capi.call("mppi_controller_get_proposal_stats", self._h, ct.byref(n), ct.byref(f), ct.byref(st), ct.byref(c0))
         return n.value, f.value, st, c0.value
 
+    def lastCallMs(self) -> float:
+        """Stream time of the last computeControl without the host post-processing."""
+        v = ct.c_float()
+        capi.call("mppi_controller_get_last_call_ms", self._h, ct.byref(v))
+        return v.value

@@ -251,7 +251,7 @@ struct mppi_controller {
   // learned proposal (optional; the network object is owned by the caller)
   ProposalNet* prop = nullptr;
   int prop_K = 0;
-  int prop_fallback = 0;
+  float prop_tolerance = -1.0f;  // < 0: no fallback
   float* prop_dmu_d = nullptr;
   float* prop_sig_d = nullptr;
   float* sigma_base_d = nullptr;
@@ -262,6 +262,9 @@ struct mppi_controller {
   int32_t n_prop = 0, n_fallback = 0;
   mppi_weight_stats prop_stats{};
   float prop_cost0 = 0.0f;
+  // stream time of the last computeControl from the first upload to the last download
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_gpu_ms = 0.0f;
 };
 
 extern "C" {
@@ -419,6 +422,24 @@ int mppi_host_step(const mppi_dynamics_params* p, const float* x, const float* u
   return MPPI_OK;
 }
 
+int mppi_host_running_cost(const mppi_cost_params* c, const float* y, float* out)
+{
+  if (!c || !y || !out) return fail(MPPI_E_INVALID, "null argument");
+  if (c->cost == MPPI_COST_QUADRATIC_MAP) {
+    if (c->map == nullptr && c->map_coeff != 0.0f) return fail(MPPI_E_INVALID, "running cost: map is null");
+    switch (c->output_dim) {
+      case 4: *out = Cost<MPPI_COST_QUADRATIC_MAP, 4>::running(*c, y); return MPPI_OK;
+      case 7: *out = Cost<MPPI_COST_QUADRATIC_MAP, 7>::running(*c, y); return MPPI_OK;
+      default: return fail(MPPI_E_UNSUPPORTED, "running cost: output_dim");
+    }
+  }
+  if (c->cost == MPPI_COST_CARTPOLE_QUADRATIC && c->output_dim == 4) {
+    *out = Cost<MPPI_COST_CARTPOLE_QUADRATIC, 4>::running(*c, y);
+    return MPPI_OK;
+  }
+  return fail(MPPI_E_UNSUPPORTED, "running cost: unknown cost id");
+}
+
 int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u)
 {
   if (!p || !u) return fail(MPPI_E_INVALID, "null argument");
@@ -530,6 +551,8 @@ static int controller_free(mppi_controller* c)
   if (c->loop_graph) (void)hipGraphDestroy(c->loop_graph);
   if (c->prop_exec) (void)hipGraphExecDestroy(c->prop_exec);
   if (c->prop_graph) (void)hipGraphDestroy(c->prop_graph);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
                  c->map_d, c->nn_d, c->loop_iter_d, c->prop_dmu_d, c->prop_sig_d, c->sigma_base_d};
   for (void* p : dev)
@@ -722,17 +745,25 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
   c->fe.previous_baseline = c->stats.baseline;
   for (int i = 0; i < S; ++i) c->x0_h[i] = x0_host[i];
   std::copy(c->control.begin(), c->control.end(), c->nominal_h);
+  if (!c->ev0) {
+    MPPI_HIP_CHECK(hipEventCreate(&c->ev0));
+    MPPI_HIP_CHECK(hipEventCreate(&c->ev1));
+  }
+  MPPI_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   bool plain = true;
   int K_used = K;
   if (c->prop) {
-    // learned proposal: one iteration of prop_K samples; with the fallback enabled it is rejected unless its
-    // weighted mean cost is below the cost of the warm start (sample 0), and the plain iterations run instead
+    // learned proposal: one iteration of prop_K samples; with the fallback enabled it is rejected when its
+    // weighted mean cost exceeds the cost of the warm start (sample 0) by more than the tolerance, and the plain
+    // iterations run instead
     controller_set_iters(c, stride, 1);
     MPPI_CHECK(controller_run(c, controller_enqueue_proposal, &c->prop_graph, &c->prop_exec, &c->prop_runs));
     c->n_prop++;
     c->prop_stats = *c->stats_h;
     c->prop_cost0 = *c->cost0_h;
-    plain = c->prop_fallback && !(c->prop_stats.weighted_cost < (double)c->prop_cost0);
+    const double c0 = c->prop_cost0;
+    plain = c->prop_tolerance >= 0.0f &&
+            !(c->prop_stats.weighted_cost <= c0 + (double)c->prop_tolerance * std::fabs(c0));
     if (plain) c->n_fallback++;
     K_used = c->prop_K;
   }
@@ -741,6 +772,9 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
     MPPI_CHECK(controller_run(c, controller_enqueue, &c->graph, &c->graph_exec, &c->eager_runs));
     K_used = K;
   }
+  MPPI_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  MPPI_HIP_CHECK(hipEventSynchronize(c->ev1));
+  MPPI_HIP_CHECK(hipEventElapsedTime(&c->last_gpu_ms, c->ev0, c->ev1));
   c->stats = *c->stats_h;
   c->raw_mean.assign(c->mean_out_h, c->mean_out_h + (size_t)T * C);
   c->control = c->raw_mean;
@@ -753,7 +787,8 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
   return MPPI_OK;
 }
 
-int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, int32_t num_rollouts, int32_t fallback)
+int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, int32_t num_rollouts,
+                                 float fallback_tolerance)
 {
   if (!c) return fail(MPPI_E_INVALID, "null argument");
   MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -770,8 +805,8 @@ int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, in
   if (pd.S != c->S || pd.C != c->C || pd.T != c->T)
     return fail(MPPI_E_INVALID, "proposal: state/control/horizon dims do not match the controller");
   if (c->d.sampler.kind != MPPI_SAMPLER_GAUSSIAN) return fail(MPPI_E_UNSUPPORTED, "proposal: Gaussian sampler only");
-  if (num_rollouts < 1 || num_rollouts > c->K)
-    return fail(MPPI_E_INVALID, "proposal: num_rollouts must be in [1, controller num_rollouts]");
+  if (num_rollouts < 2 || num_rollouts > c->K)
+    return fail(MPPI_E_INVALID, "proposal: num_rollouts must be in [2, controller num_rollouts]");
   const size_t tc = sizeof(float) * c->T * c->C;
   if (!c->prop_dmu_d) {
     MPPI_CHECK(dev_alloc(&c->prop_dmu_d, tc));
@@ -783,7 +818,14 @@ int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, in
                            hipMemcpyHostToDevice));
   c->prop = proposal_net_of(proposal);
   c->prop_K = num_rollouts;
-  c->prop_fallback = fallback != 0;
+  c->prop_tolerance = fallback_tolerance;
+  return MPPI_OK;
+}
+
+int mppi_controller_get_last_call_ms(const mppi_controller* c, float* ms)
+{
+  if (!c || !ms) return fail(MPPI_E_INVALID, "null argument");
+  *ms = c->last_gpu_ms;
   return MPPI_OK;
 }
 

@@ -25,7 +25,7 @@ size_t fused_iteration_workspace_bytes(int K, int T, int C)
 }
 
 // With PROP the samples come from the learned proposal: u = mu + dmu + sig_q eps (k >= 1; sample 0 and t < stride
-// keep mu; no pure-noise samples) and every cost carries the importance-weight correction (rollout_core.hpp).
+// keep mu; no pure-noise samples) and every cost carries the mean-shift correction (rollout_core.hpp).
 template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP>
 __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kernel(
     mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int K, int T, float dt, float lambda,
@@ -42,8 +42,8 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
   const int ROW = TC + 1;
   float* tile = lds;
   float* mu = tile + RC_LANES * ROW;
-  float* pl = mu + TC4;                    // PROP: dmu, sig_q, 0.5 / sig_q^2, log(sig_q / sig_p) [4][TC4]
-  float* wl = pl + (PROP ? 4 * TC4 : 0);   // [64] local weights
+  float* pl = mu + TC4;                    // PROP: dmu, sig_q [2][TC4]
+  float* wl = pl + (PROP ? 2 * TC4 : 0);   // [64] local weights
   float* scratch = wl + RC_LANES;          // pipelined-rollout buffers
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -55,16 +55,10 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
   if (PROP) {
     const float decay_p = powf(sp.std_dev_decay, (float)load_iter_f(itr).iteration);
     for (int j = tid; j < TC; j += NT) {
-      const float sq = prop_sig[j];
-      const float sp_c = decay_p * sp.std_dev[j % C];
       pl[j] = prop_dmu[j];
-      pl[TC4 + j] = sq;
-      pl[2 * TC4 + j] = 0.5f / (sq * sq);
-      pl[3 * TC4 + j] = logf(sq / sp_c);
+      pl[TC4 + j] = prop_sig[j];
     }
     pv.dmu = pl;
-    pv.hq = pl + 2 * TC4;
-    pv.lr = pl + 3 * TC4;
 #pragma unroll
     for (int c = 0; c < MPPI_MAX_CONTROL; ++c) {
       const float sp_c = decay_p * sp.std_dev[c];
@@ -120,9 +114,11 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
                             cost_l, raw_l, PROP ? &pv : nullptr);
   if (tid < RC_LANES) {
     const int k = k0 + tid;
-    const bool valid = lane < nrows;
     const float my_cost = cost_l[lane];
-    if (valid && costs_out != nullptr) costs_out[k] = my_cost;
+    if (lane < nrows && costs_out != nullptr) costs_out[k] = my_cost;
+    // with a proposal, sample 0 (the warm start) is only a probe for the fallback test: it is not a draw from the
+    // proposal and takes no part in the weights
+    const bool valid = lane < nrows && !(PROP && k == 0);
     // local weights against the workgroup's own baseline
     const float c = valid ? sanitize_cost(my_cost) : INFINITY;
     float b = c;
@@ -261,7 +257,7 @@ static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& co
 {
   constexpr int C = Model<MODEL>::C, O = Model<MODEL>::O;
   const size_t TC4 = (size_t)((T * C + 3) & ~3);
-  const size_t base = (size_t)RC_LANES * (T * C + 1) + TC4 * (PROP ? 5 : 1) + RC_LANES;
+  const size_t base = (size_t)RC_LANES * (T * C + 1) + TC4 * (PROP ? 3 : 1) + RC_LANES;
   const size_t limit = LDS_LIMIT / sizeof(float);
   const int chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
   if (chunk == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");

@@ -247,7 +247,7 @@ struct Model<MPPI_MODEL_RACER_DUBINS_NN> : Bicycle {
 // Bilinear cost-map query with the reference's CPU semantics (utils/texture_helpers/two_d_texture_helper.cu:151-245
 // and the world->texture transform of texture_helper.cu:94-134 with identity rotation): clamp address mode,
 // texel centres at +0.5, x0 = min(floor(q), W - 2).
-__device__ __forceinline__ float map_query(const mppi_cost_params& c, float px, float py)
+MPPI_HD float map_query(const mppi_cost_params& c, float px, float py)
 {
   const int W = c.map_width, H = c.map_height;
   // texel coordinate (p - origin) / resolution - 0.5 (the reference's normalise-by-extent and scale-back cancel);
@@ -285,13 +285,13 @@ struct Cost;
 // cost_functions/cartpole/cartpole_quadratic_cost.cu:23-55
 template <int O>
 struct Cost<MPPI_COST_CARTPOLE_QUADRATIC, O> {
-  __device__ __forceinline__ static bool has_map(const mppi_cost_params&) { return false; }
-  __device__ __forceinline__ static float running_nomap(const mppi_cost_params& c, const float* y)
+  MPPI_HD static bool has_map(const mppi_cost_params&) { return false; }
+  MPPI_HD static float running_nomap(const mppi_cost_params& c, const float* y)
   {
     return quad_state<O>(c, y);
   }
-  __device__ __forceinline__ static float running(const mppi_cost_params& c, const float* y) { return quad_state<O>(c, y); }
-  __device__ __forceinline__ static float terminal(const mppi_cost_params& c, const float* y)
+  MPPI_HD static float running(const mppi_cost_params& c, const float* y) { return quad_state<O>(c, y); }
+  MPPI_HD static float terminal(const mppi_cost_params& c, const float* y)
   {
     return quad_state<O>(c, y) * c.terminal_cost_coeff;
   }
@@ -301,12 +301,12 @@ struct Cost<MPPI_COST_CARTPOLE_QUADRATIC, O> {
 // (cost_functions/autorally/ar_standard_cost.cu:284-297); terminal = terminal_cost_coeff * quadratic term.
 template <int O>
 struct Cost<MPPI_COST_QUADRATIC_MAP, O> {
-  __device__ __forceinline__ static bool has_map(const mppi_cost_params& c)
+  MPPI_HD static bool has_map(const mppi_cost_params& c)
   {
     return c.map != nullptr && c.map_coeff != 0.0f;
   }
   // every term but the cost map (the map term is then added by a separate gather pass)
-  __device__ __forceinline__ static float running_nomap(const mppi_cost_params& c, const float* y)
+  MPPI_HD static float running_nomap(const mppi_cost_params& c, const float* y)
   {
     float s = quad_state<O>(c, y);
     if (c.speed_coeff != 0.0f) {
@@ -315,7 +315,7 @@ struct Cost<MPPI_COST_QUADRATIC_MAP, O> {
     }
     return s;
   }
-  __device__ __forceinline__ static float running(const mppi_cost_params& c, const float* y)
+  MPPI_HD static float running(const mppi_cost_params& c, const float* y)
   {
     float s = quad_state<O>(c, y);
     if (c.map != nullptr && c.map_coeff != 0.0f) s += c.map_coeff * map_query(c, y[c.map_x_index], y[c.map_y_index]);
@@ -325,7 +325,7 @@ struct Cost<MPPI_COST_QUADRATIC_MAP, O> {
     }
     return s;
   }
-  __device__ __forceinline__ static float terminal(const mppi_cost_params& c, const float* y)
+  MPPI_HD static float terminal(const mppi_cost_params& c, const float* y)
   {
     return (c.terminal_cost_coeff == 0.0f) ? 0.0f : quad_state<O>(c, y) * c.terminal_cost_coeff;
   }

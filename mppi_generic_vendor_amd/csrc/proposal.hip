@@ -214,6 +214,94 @@ __global__ void proposal_head_kernel(const float* __restrict__ y, int TC, int C,
   }
 }
 
+// Inference for one state in one workgroup (16 wavefronts): input construction, the three layers on the matrix cores
+// with the weights as the A operand (16-row tiles) and the input vector broadcast over the 16 B columns, and the
+// head. The operands are rounded to BF16 and accumulated in FP32 over 32-wide k-steps in order, the arithmetic of
+// gemm_bf16_kernel, so the result equals the batched forward pass.
+constexpr int PI_THREADS = 1024;
+
+template <int K>
+__device__ __forceinline__ void infer_layer(const float* __restrict__ W, const float* __restrict__ bias, int N,
+                                            int Kreal, const float* vin, float* vout, bool gelu)
+{
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, l16 = lane & 15;
+  constexpr int KS = (K + 31) / 32;
+  pbf16x8 bfr[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 32 * ks + 8 * g + j;
+      bfr[ks][j] = (__bf16)(k < Kreal ? vin[k] : 0.0f);
+    }
+  for (int mt = wave; mt * 16 < N; mt += PI_THREADS / 64) {
+    const int row = 16 * mt + l16;
+    // every weight of the tile is requested before the first product
+    float wv[KS][8];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = 32 * ks + 8 * g + j;
+        wv[ks][j] = (row < N && k < Kreal) ? W[(int64_t)row * Kreal + k] : 0.0f;
+      }
+    pf32x4 acc = pf32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      pbf16x8 a;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = (__bf16)wv[ks][j];
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[ks], acc, 0, 0, 0);
+    }
+    if (l16 == 0) {  // column 0 of the result
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * mt + 4 * g + i;
+        if (r < N) {
+          const float z = acc[i] + bias[r];
+          vout[r] = gelu ? gelu_f(z) : z;
+        }
+      }
+    }
+  }
+}
+
+template <int DIN>
+__global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
+    int S, int C, int T, const float* __restrict__ th, int64_t w1, int64_t b1, int64_t w2, int64_t b2, int64_t w3,
+    int64_t b3, const float* __restrict__ x0, const float* __restrict__ nominal, const float* __restrict__ shift,
+    const float* __restrict__ scale, const float* __restrict__ sigma_base, float* __restrict__ dmu,
+    float* __restrict__ sig)
+{
+  __shared__ float xin[DIN];
+  __shared__ float h1[PROP_H], h2[PROP_H];
+  extern __shared__ float yout[];  // [2 T C]
+  const int din = S + PROP_KNOTS * C, TC = T * C;
+  for (int o = threadIdx.x; o < DIN; o += PI_THREADS) {
+    float v = 0.0f;
+    if (o < S) {
+      v = (x0[o] - shift[o]) * scale[o];
+    } else if (o < din) {
+      const int j = (o - S) / C, c = (o - S) - j * C;
+      v = (nominal[(j * (T - 1) / 15) * C + c] - shift[o]) * scale[o];
+    }
+    xin[o] = v;
+  }
+  __syncthreads();
+  infer_layer<DIN>(th + w1, th + b1, PROP_H, din, xin, h1, true);
+  __syncthreads();
+  infer_layer<PROP_H>(th + w2, th + b2, PROP_H, PROP_H, h1, h2, true);
+  __syncthreads();
+  infer_layer<PROP_H>(th + w3, th + b3, 2 * TC, PROP_H, h2, yout, false);
+  __syncthreads();
+  for (int j = threadIdx.x; j < TC; j += PI_THREADS) {
+    const float sb = sigma_base[j % C];
+    dmu[j] = sb * yout[j];
+    sig[j] = sb * expf(fminf(fmaxf(yout[TC + j], PROP_S_LO), PROP_S_HI));
+  }
+}
+
 struct ProposalNet {
   PropDims d{};
   float* theta = nullptr;
@@ -310,6 +398,24 @@ static int prop_loss_grad(ProposalNet* p, const float* X, const float* R, int B,
 
 int proposal_infer(ProposalNet* p, const float* x0, const float* nominal, const float* sigma_base, float* dmu,
                    float* sig, hipStream_t s)
+{
+  const PropDims& d = p->d;
+  const size_t lds = sizeof(float) * (size_t)d.dout;
+#define MPPI_INFER(DIN)                                                                                          \
+  proposal_infer_kernel<DIN><<<1, PI_THREADS, lds, s>>>(d.S, d.C, d.T, p->theta, d.w1, d.b1, d.w2, d.b2, d.w3,   \
+                                                        d.b3, x0, nominal, p->shift, p->scale, sigma_base, dmu, sig)
+  if (d.din <= 32) MPPI_INFER(32);
+  else if (d.din <= 64) MPPI_INFER(64);
+  else MPPI_INFER(96);
+#undef MPPI_INFER
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
+// The same inference through the batched kernels (input kernel, three GEMM launches, head), kept for checking the
+// fused kernel against them.
+int proposal_infer_unfused(ProposalNet* p, const float* x0, const float* nominal, const float* sigma_base,
+                           float* dmu, float* sig, hipStream_t s)
 {
   const PropDims& d = p->d;
   proposal_input_kernel<<<1, 128, 0, s>>>(d.S, d.C, d.T, x0, nominal, p->shift, p->scale, p->xin);
@@ -445,6 +551,15 @@ int mppi_proposal_forward(mppi_proposal* q, const float* inputs_d, int32_t B, fl
   MPPI_CHECK(prop_forward(p, inputs_d, B, s));
   MPPI_HIP_CHECK(hipMemcpyAsync(outputs_d, p->y, sizeof(float) * (size_t)B * p->d.dout, hipMemcpyDeviceToDevice, s));
   return MPPI_OK;
+}
+
+int mppi_proposal_infer(mppi_proposal* q, const float* x0_d, const float* nominal_d, const float* sigma_base_d,
+                        float* dmu_d, float* sig_d, int32_t fused, void* stream)
+{
+  if (!q || !x0_d || !nominal_d || !sigma_base_d || !dmu_d || !sig_d) return fail(MPPI_E_INVALID, "null argument");
+  const hipStream_t s = (hipStream_t)stream;
+  return fused ? proposal_infer(&q->net, x0_d, nominal_d, sigma_base_d, dmu_d, sig_d, s)
+               : proposal_infer_unfused(&q->net, x0_d, nominal_d, sigma_base_d, dmu_d, sig_d, s);
 }
 
 int mppi_proposal_loss_grad(mppi_proposal* q, const float* inputs_d, const float* targets_d, int32_t B, float* grad_d,

@@ -67,14 +67,14 @@ constexpr int NN_THREADS = rollout_threads<MODEL_RACER_DUBINS_NN_BF16>();
 constexpr int NN_PRODUCERS = RC_LANES / 16;
 constexpr int NN_CONSUMERS = NN_THREADS / 64 - NN_PRODUCERS;
 
-// Importance-weight correction of a proposal sample (learned proposal q = N(mu + dmu, sig_q^2) instead of the base
-// p = N(mu, sig_p^2)): lambda (log q(u) - log p(u)) summed over the controls of one step, added to the sample's cost
-// so that exp(-cost / lambda) carries the density ratio p / q. The LDS arrays hold dmu, 0.5 / sig_q^2 and
-// log(sig_q / sig_p) per (t, c); hp is 0.5 / sig_p^2 per control.
+// Importance-weight correction of a proposal sample drawn around mu + dmu instead of the base mean mu: the
+// mean-shift term lambda (log N(u; mu + dmu, sig_p^2) - log N(u; mu, sig_p^2)) summed over the controls of one step,
+// lambda sum_c ((u - mu)^2 - (u - mu - dmu)^2) / (2 sig_p^2), linear in u (the control-cost term of MPPI's
+// derivation for a shifted sampling mean). The proposal's own spread sig_q sets the exploration width and, like the
+// base sampler's std_dev_decay, carries no density ratio (a variance ratio would weight the proposal's outliers by
+// exp(chi^2 / 2)). `dmu` (LDS) holds the shift per (t, c), hp[c] = 1 / (2 sig_p^2).
 struct PropLds {
   const float* dmu;
-  const float* hq;
-  const float* lr;
   float hp[MPPI_MAX_CONTROL];
 };
 
@@ -86,7 +86,7 @@ __device__ __forceinline__ float is_term(const PropLds& p, const float* u, const
   for (int c = 0; c < C; ++c) {
     const float d0 = u[c] - mu[c];
     const float d1 = d0 - p.dmu[j + c];
-    s += d0 * d0 * p.hp[c] - d1 * d1 * p.hq[j + c] - p.lr[j + c];
+    s += (d0 * d0 - d1 * d1) * p.hp[c];
   }
   return s;
 }

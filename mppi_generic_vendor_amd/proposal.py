@@ -85,6 +85,16 @@ class ProposalNet:
         capi.call("mppi_proposal_forward", self._h, xb.ptr, X.shape[0], yb.ptr, None)
         return yb.to_numpy((X.shape[0], self.dout))
 
+    def infer(self, x0: np.ndarray, nominal: np.ndarray, sigma_base: np.ndarray, fused: bool = True):
+        """(dmu [T][C], sigma_q [T][C]) for one state, as the controller computes them."""
+        xb = DeviceBuffer.from_numpy(np.ascontiguousarray(x0, np.float32))
+        nb = DeviceBuffer.from_numpy(np.ascontiguousarray(nominal, np.float32))
+        sb = DeviceBuffer.from_numpy(np.ascontiguousarray(sigma_base, np.float32))
+        db = DeviceBuffer(self.T * self.C * 4)
+        gb = DeviceBuffer(self.T * self.C * 4)
+        capi.call("mppi_proposal_infer", self._h, xb.ptr, nb.ptr, sb.ptr, db.ptr, gb.ptr, 1 if fused else 0, None)
+        return db.to_numpy((self.T, self.C)), gb.to_numpy((self.T, self.C))
+
     def loss_grad(self, X: np.ndarray, R: np.ndarray):
         xb = DeviceBuffer.from_numpy(np.ascontiguousarray(X, np.float32))
         rb = DeviceBuffer.from_numpy(np.ascontiguousarray(R, np.float32))

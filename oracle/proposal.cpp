@@ -23,7 +23,7 @@
 namespace {
 
 constexpr int H = 128;
-constexpr double S_LO = -4.0, S_HI = 1.0;
+constexpr double S_LO = -1.5, S_HI = 1.0;
 
 float bf16_round_p(float f)
 {
@@ -206,11 +206,13 @@ double oracle_proposal_loss_grad(int S, int C, int T, const float* theta, int B,
   return loss;
 }
 
-/* Importance-weight correction per sample (the caller leaves it off the warm-start sample 0): lambda sum_{t,c} [ (u - mu)^2 / (2 sp_c^2) - (u - mu - dmu)^2 / (2 sq^2)
- * - log(sq / sp_c) ] = lambda (log q(u) - log p(u)) for q = N(mu + dmu, sq^2), p = N(mu, sp^2). */
+/* Importance-weight correction per sample (the caller leaves it off the warm-start sample 0): the mean-shift term
+ * lambda sum_{t,c} ((u - mu)^2 - (u - mu - dmu)^2) / (2 sp_c^2) = lambda (log N(u; mu + dmu, sp^2) - log N(u; mu, sp^2)).
+ * `sig` (the proposal's spread) does not enter. */
 void oracle_is_cost(const float* u, const float* mean, const float* dmu, const float* sig, const float* sig_p, int K,
                     int T, int C, double lambda, double* out)
 {
+  (void)sig;
   for (int k = 0; k < K; ++k) {
     double s = 0.0;
     for (int t = 0; t < T; ++t)
@@ -218,8 +220,8 @@ void oracle_is_cost(const float* u, const float* mean, const float* dmu, const f
         const int j = t * C + c;
         const double d0 = (double)u[((size_t)k * T + t) * C + c] - mean[j];
         const double d1 = d0 - dmu[j];
-        const double sp = sig_p[c], sq = sig[j];
-        s += d0 * d0 / (2.0 * sp * sp) - d1 * d1 / (2.0 * sq * sq) - std::log(sq / sp);
+        const double sp = sig_p[c];
+        s += (d0 * d0 - d1 * d1) / (2.0 * sp * sp);
       }
     out[k] = lambda * s;
   }

@@ -64,3 +64,19 @@ def test_cpp_closed_loop_swing_up(tmp_path, hip, engine):
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stdout + res.stderr
+
+
+def test_host_running_cost_matches_oracle(engine, oracle):
+    from mppi_generic_vendor_amd import capi, workloads
+
+    m = workloads.gaussian_blob_map(128, 96, blobs=8, seed=3)
+    prob = workloads.bicycle_nn(num_rollouts=64, num_timesteps=10, residual=False, cost_map=m)
+    cost = capi.CostParams.from_buffer_copy(prob.cost)
+    cost.map = m.ctypes.data
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        y = rng.normal(scale=4.0, size=7).astype(np.float32)
+        got = ct.c_float()
+        capi.call("mppi_host_running_cost", ct.byref(cost), y.ctypes.data, ct.byref(got))
+        want = oracle.oracle_running_cost_f32(ct.byref(cost), m.ctypes.data, y.ctypes.data)
+        assert got.value == pytest.approx(want, rel=2e-6, abs=1e-6)

@@ -113,7 +113,7 @@ def test_proposal_iteration_matches_oracle(hip, engine):
     net = ProposalNet(S, C, T, th)
     ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=False)
     Kp = 512
-    ctrl.setProposal(net, Kp, fallback=False)
+    ctrl.setProposal(net, Kp, fallback_tolerance=None)
     nominal = np.zeros((T, C), np.float32)
     ctrl.computeControl(prob.x0, 1)
     raw = ctrl.getRawMean()
@@ -123,45 +123,45 @@ def test_proposal_iteration_matches_oracle(hip, engine):
     y = ol.proposal_forward(S, C, T, th, X, bf16=True)[0]
     sb = np.array([prob.sampler.std_dev[c] for c in range(C)], np.float32)
     dmu = (y[:TC].reshape(T, C) * sb).astype(np.float32)
-    sig = (np.exp(np.clip(y[TC:], -4.0, 1.0)).reshape(T, C) * sb).astype(np.float32)
+    sig = (np.exp(np.clip(y[TC:], -1.5, 1.0)).reshape(T, C) * sb).astype(np.float32)
     eps = gpu_normals(prob.sampler.seed, 0, Kp * TC).reshape(Kp, T, C)
     u = (nominal + dmu + sig * eps).astype(np.float32)
     u[0] = nominal
     u[:, 0] = nominal[0]
     costs, clamped = ol.rollout_f32(prob, u, nominal, nthreads=8)
     isc = ol.is_cost(clamped, nominal, dmu, sig, sb, prob.lambda_)
-    isc[0] = 0.0  # the warm start is not a draw from the proposal
     total = (costs + isc).astype(np.float32)
-    w, eta, b, idx, s2 = ol.weights(total, prob.lambda_)
-    want = ol.weighted_update(w, eta, clamped)
-    wc = float(np.sum(w * costs) / eta)
+    # sample 0 (the warm start) is a probe: its uncorrected cost is reported, the weights use samples 1..K-1
+    w, eta, b, idx, s2 = ol.weights(total[1:], prob.lambda_)
+    idx += 1
+    want = ol.weighted_update(w, eta, clamped[1:])
+    wc = float(np.sum(w * costs[1:]) / eta)
     scale = max(1e-3, float(np.max(np.abs(want))))
     err = float(np.max(np.abs(raw - want))) / scale
     print(f"[proposal] iteration: mean rel err {err:.2e}, argmin {st.argmin} vs {idx}, baseline {st.baseline:.5f} "
-          f"vs {b:.5f}, weighted cost {st.weighted_cost:.5f} vs {wc:.5f}, warm start {c0:.5f} vs {total[0]:.5f}")
+          f"vs {b:.5f}, weighted cost {st.weighted_cost:.5f} vs {wc:.5f}, warm start {c0:.5f} vs {costs[0]:.5f}")
     assert st.argmin == idx
     assert st.baseline == pytest.approx(b, rel=1e-3)
     assert st.weighted_cost == pytest.approx(wc, rel=1e-3)
-    assert c0 == pytest.approx(float(total[0]), rel=1e-3)
+    assert c0 == pytest.approx(float(costs[0]), rel=1e-3)
     assert err < 1e-3
     ctrl.close()
     net.close()
 
 
 def test_proposal_fallback_runs_plain_iteration(hip, engine):
-    """A proposal whose samples are far off the warm start leaves all weight on the warm start (its importance
-    corrections are large), so its weighted cost is not below the warm start's: it is rejected and the plain
-    iteration with the controller's full sample count runs instead."""
+    """A proposal that brakes fully and steers hard on every step has a weighted cost above the warm start's: it is
+    rejected and the plain iteration with the controller's full sample count runs instead."""
     from mppi_generic_vendor_amd.controller import VanillaMPPIController
 
     banner("proposal fallback")
     prob = _proposal_problem()
     S, C, T = 7, 2, 100
     th = init_params(S, C, T, 2)
-    th[-2 * T * C:-T * C] = 4.0  # mean shift of 4 sigma on every step and control
+    th[-2 * T * C:-T * C] = np.tile([-4.0, 4.0], T)  # full brake and full steering on every step
     net = ProposalNet(S, C, T, th)
     ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True)
-    ctrl.setProposal(net, 256, fallback=True)
+    ctrl.setProposal(net, 256, fallback_tolerance=0.0)
     for _ in range(3):
         ctrl.computeControl(prob.x0, 1)
     ncalls, nfall, st, c0 = ctrl.getProposalStats()
@@ -169,4 +169,31 @@ def test_proposal_fallback_runs_plain_iteration(hip, engine):
     assert ncalls == 3 and nfall == 3
     assert ctrl.getWeightStats().num_rollouts == prob.num_rollouts
     ctrl.close()
+    net.close()
+
+
+def test_fused_inference_equals_batched_forward(hip, engine):
+    """The controller's single-workgroup inference kernel reproduces the batched forward pass (same BF16 rounding
+    and k-order of the accumulation) and the oracle's BF16 forward with the head applied."""
+    from mppi_generic_vendor_amd.proposal import network_inputs
+
+    banner("proposal fused inference")
+    S, C, T = 7, 2, 100
+    th = _random_params(S, C, T, 21)
+    net = ProposalNet(S, C, T, th)
+    rng = np.random.default_rng(22)
+    shift = rng.normal(size=S + 16 * C).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, S + 16 * C).astype(np.float32)
+    net.set_normalization(shift, scale)
+    x0 = rng.normal(size=S).astype(np.float32)
+    nominal = rng.normal(scale=0.3, size=(T, C)).astype(np.float32)
+    sb = np.array([0.5, 0.3], np.float32)
+    d1, s1 = net.infer(x0, nominal, sb, fused=True)
+    d0, s0 = net.infer(x0, nominal, sb, fused=False)
+    np.testing.assert_allclose(d1, d0, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(s1, s0, rtol=1e-6, atol=1e-7)
+    X = ((network_inputs(x0[None], nominal[None]) - shift) * scale).astype(np.float32)
+    y = ol.proposal_forward(S, C, T, th, X, bf16=True)[0]
+    np.testing.assert_allclose(d1, y[:T * C].reshape(T, C) * sb, rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(s1, np.exp(np.clip(y[T * C:], -1.5, 1)).reshape(T, C) * sb, rtol=1e-4, atol=1e-6)
     net.close()

@@ -100,15 +100,20 @@ def test_input_downsampling_knots(oracle):
 
 
 def test_importance_correction_closed_forms(oracle):
-    """q = p gives zero correction for any u; one step by hand: u = 1, mu = 0, dmu = 0.5, sq = 0.5, sp = 1,
-    lambda = 2: 2 (1/2 - 0.25 / 0.5 - log 0.5) = 2 log 2."""
+    """No shift gives zero correction for any u and spread; one step by hand: u = 1, mu = 0, dmu = 0.5, sp = 1,
+    lambda = 2: 2 (1 - 0.25) / 2 = 0.75; and the term is linear in u: (2 d0 dmu - dmu^2) / (2 sp^2)."""
     rng = np.random.default_rng(2)
     K, T, C = 5, 7, 2
     u = rng.normal(size=(K, T, C)).astype(np.float32)
     mean = rng.normal(size=(T, C)).astype(np.float32)
     sp = np.array([0.7, 1.3], np.float32)
-    z = ol.is_cost(u, mean, np.zeros((T, C)), np.broadcast_to(sp, (T, C)), sp, 1.5)
+    z = ol.is_cost(u, mean, np.zeros((T, C)), rng.uniform(0.1, 2.0, (T, C)), sp, 1.5)
     np.testing.assert_allclose(z, 0.0, atol=1e-12)
     v = ol.is_cost(np.ones((1, 1, 1)), np.zeros((1, 1)), np.full((1, 1), 0.5), np.full((1, 1), 0.5),
                    np.ones(1, np.float32), 2.0)
-    assert v[0] == pytest.approx(2.0 * math.log(2.0), abs=1e-12)
+    assert v[0] == pytest.approx(0.75, abs=1e-12)
+    dmu = rng.normal(size=(T, C)).astype(np.float32)
+    got = ol.is_cost(u, mean, dmu, np.ones((T, C)), sp, 1.0)
+    d0 = u.astype(np.float64) - mean
+    want = ((2 * d0 * dmu - dmu.astype(np.float64) ** 2) / (2 * sp.astype(np.float64) ** 2)).sum(axis=(1, 2))
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
