@@ -254,6 +254,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--learned-episodes", type=int, default=4,
+                    help="closed-loop episodes of the learned-proposal comparison (0: skip)")
     ap.add_argument("--calibration-mib", type=int, default=256,
                     help="device-to-device copy of this size, a plain copy kernel of known traffic for counter scaling")
     args = ap.parse_args()
@@ -351,6 +353,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(prob, args.cpu_seconds, 1)
+    learned_cmp = learned_comparison(args.learned_episodes) if args.workload == "di_colored_map" else None
 
     line = {
         "metric": METRIC,
@@ -373,8 +376,35 @@ def main():
         "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_vs_plain": learned_cmp,
     }
     print(json.dumps(line), flush=True)
+
+
+def learned_comparison(episodes: int):
+    """Learned vs plain MPPI on the bicycle + residual-network task with the committed trained proposal: closed-loop
+    executed-trajectory cost from held-out starts and stream time per control step of plain MPPI (8192 samples,
+    3 iterations), the learned proposal (2048 samples, 1 iteration, fallback above 0.5 %), and plain MPPI with the
+    proposal's samples and one iteration (about the learned step's time)."""
+    from mppi_generic_vendor_amd import learned, train_proposal, workloads
+
+    if episodes <= 0 or not train_proposal.DEFAULT_OUT.exists():
+        return None
+    net, summary = train_proposal.load()
+    res = learned.evaluate(net, episodes=episodes, steps=100, seed=99, proposal_rollouts=2048,
+                           cost_map=workloads.gaussian_blob_map())
+    net.close()
+    for v in res.values():
+        v.pop("costs", None)
+    p, l, s = res["plain"], res["learned"], res["plain_small"]
+    print(f"[bench] learned vs plain: cost {l['cost']:.4f} vs {p['cost']:.4f} (same samples, one iteration: "
+          f"{s['cost']:.4f}), {l['hz']:.0f} vs {p['hz']:.0f} control Hz", file=sys.stderr, flush=True)
+    return {"task": "bicycle + BF16 residual MLP, K=8192 T=100, closed loop from held-out starts",
+            "episodes": episodes, "trained_on_pairs": summary.get("pairs"),
+            "plain": p, "learned": l, "plain_same_samples": s,
+            "cost_ratio_learned_to_plain": round(l["cost"] / p["cost"], 6),
+            "rate_ratio_learned_to_plain": round(l["hz"] / p["hz"], 4),
+            "cost_gain_over_same_samples": round((s["cost"] - l["cost"]) / s["cost"], 6)}
 
 
 if __name__ == "__main__":
