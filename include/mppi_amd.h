@@ -382,7 +382,7 @@ int mppi_batch_stream(const mppi_batch* batch, void** stream);
  * Learned proposal network (no counterpart in the reference). Inputs: the state (S values) and the previous control
  * sequence sampled at the 16 knots t_j = j (T - 1) / 15 (16 C values), each normalized as (v - shift) * scale;
  * two hidden layers of 128 GELU units; 2 T C outputs: the mean shift in units of the base sampling std (first T C,
- * [t][c]) and log(sigma_q / sigma_base) (last T C, clamped to [-1.5, 1] when used). Parameters in the FNN layout
+ * [t][c]) and log(sigma_q / sigma_base) (last T C, clamped to [log 0.5, 1] when used). Parameters in the FNN layout
  * (W[out][in] row-major then bias, per layer). Forward, backward and Adam run on BF16 matrix cores with FP32
  * accumulation. Loss per sample: mean over j of (r_j - y_j)^2 exp(-2 s_j) / 2 + s_j with the target residual
  * r = (u* - u_prev) / sigma_base, averaged over the batch.
@@ -403,7 +403,7 @@ int mppi_proposal_forward(mppi_proposal* proposal, const float* inputs_d, int32_
                           void* stream);
 /* Proposal of one state as the controller uses it: the input from x0_d[S] and the nominal sequence nominal_d[T][C]
  * (normalized with the stored shift/scale), the network, then dmu_d[T][C] = sigma_base * mean shift and
- * sig_d[T][C] = sigma_base * exp(clamp(log-std, -1.5, 1)); sigma_base_d[C]. fused = 1: one single-workgroup kernel
+ * sig_d[T][C] = sigma_base * exp(clamp(log-std, log 0.5, 1)); sigma_base_d[C]. fused = 1: one single-workgroup kernel
  * (the controller's path); 0: the input kernel, the batched forward launches and a head kernel. */
 int mppi_proposal_infer(mppi_proposal* proposal, const float* x0_d, const float* nominal_d,
                         const float* sigma_base_d, float* dmu_d, float* sig_d, int32_t fused, void* stream);

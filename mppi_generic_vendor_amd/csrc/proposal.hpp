@@ -9,7 +9,7 @@ namespace mppi_amd {
 
 constexpr int PROP_H = 128;
 constexpr int PROP_KNOTS = 16;
-constexpr float PROP_S_LO = -1.5f, PROP_S_HI = 1.0f;  // sigma_q in [0.22, 2.7] sigma_base
+constexpr float PROP_S_LO = -0.693147f, PROP_S_HI = 1.0f;  // sigma_q in [0.5, 2.7] sigma_base
 
 struct PropDims {
   int S, C, T, din, dout;

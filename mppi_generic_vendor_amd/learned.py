@@ -195,3 +195,46 @@ def evaluate(net: ProposalNet | None, episodes: int = 10, steps: int = 100, seed
         if log:
             log(f"[learned] {name}: cost {entry['cost']:.4f}, {entry['ms_per_call']:.4f} ms per control step")
     return res
+
+
+def sequence_cost(prob: workloads.Problem, plant: Plant, x0: np.ndarray, u: np.ndarray) -> float:
+    """Cost of one control sequence from x0 through the model: mean running cost of the visited states."""
+    x = x0.copy()
+    total = 0.0
+    for t in range(u.shape[0]):
+        x = plant.step(x, u[t])
+        total += plant.cost_of(x)
+    return total / u.shape[0]
+
+
+def cold_start_quality(net: ProposalNet | None, starts: int = 32, seed: int = 11, plain_rollouts: int = 8192,
+                       plain_iters: int = 3, proposal_rollouts: int = 2048, cost_map: np.ndarray | None = None,
+                       log=None) -> dict:
+    """One control step from a cold start (zero warm start) per state: the cost of the returned sequence for plain
+    MPPI at its converged setting, the learned proposal (one iteration, no fallback) and plain MPPI with the
+    proposal's samples and one iteration. Isolates the optimizer's quality from closed-loop divergence."""
+    m = workloads.gaussian_blob_map() if cost_map is None else cost_map
+    rng = np.random.default_rng(seed)
+    xs = [random_start(rng) for _ in range(starts)]
+    res = {}
+    modes = [("plain", plain_rollouts, plain_iters, False), ("plain_small", proposal_rollouts, 1, False)]
+    if net is not None:
+        modes.insert(1, ("learned", plain_rollouts, plain_iters, True))
+    for name, K, iters, learned_mode in modes:
+        prob = task(K, cost_map=m, seed=4243)
+        plant = Plant(prob)
+        ctrl = VanillaMPPIController(prob, num_iters=iters, use_graph=True)
+        if learned_mode:
+            ctrl.setProposal(net, proposal_rollouts, fallback_tolerance=None)
+        T, C = prob.num_timesteps, prob.C
+        js = []
+        for x in xs:
+            ctrl.updateImportanceSampler(np.zeros((T, C), np.float32))
+            ctrl.computeControl(x, 1)
+            js.append(sequence_cost(prob, plant, x, ctrl.getControlSeq()))
+        ctrl.close()
+        res[name] = {"cost": float(np.mean(js)), "costs": js}
+        if log:
+            log(f"[learned] cold start {name}: sequence cost {res[name]['cost']:.4f}")
+    return res
+

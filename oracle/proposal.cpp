@@ -23,7 +23,7 @@
 namespace {
 
 constexpr int H = 128;
-constexpr double S_LO = -1.5, S_HI = 1.0;
+constexpr double S_LO = -0.693147, S_HI = 1.0;
 
 float bf16_round_p(float f)
 {

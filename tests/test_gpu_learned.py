@@ -3,9 +3,10 @@
 The network shipped in mppi_generic_vendor_amd/data/proposal_bicycle_nn.npz was trained by the project's kernels
 (python -m mppi_generic_vendor_amd.train_proposal: 200k pairs from closed-loop plain MPPI with 8192 samples and 3
 iterations). Closed-loop episodes from held-out starts compare the executed-trajectory cost of (a) plain MPPI at that
-converged setting, (b) the learned proposal with 2048 samples and one iteration (fallback to (a) above a 0.5 % cost
-tolerance), (c) plain MPPI with the same 2048 samples and one iteration -- the method without the network. A second
-test runs the data-generation and training pipeline on a small set and checks the held-out loss falls.
+converged setting, (b) the learned proposal with 2048 samples and one iteration, (c) plain MPPI with the same 2048
+samples and one iteration -- the method without the network -- on cold starts (the cost of the returned sequence)
+and in closed loop. A second test runs the data-generation and training pipeline on a small set and checks the
+held-out loss falls.
 """
 import numpy as np
 import pytest
@@ -16,19 +17,25 @@ pytestmark = pytest.mark.gpu
 
 
 def test_trained_proposal_beats_plain_at_equal_samples(hip, engine):
+    """Cold starts (zero warm start) from 48 seeded states: the sequence one learned iteration returns costs less
+    than the one a plain iteration with the same 2048 samples returns, and lies within 1 % of plain MPPI at 8192
+    samples and 3 iterations. Then closed loop from 4 held-out starts: within 1 % of plain at its converged
+    setting."""
     from mppi_generic_vendor_amd import learned, train_proposal, workloads
 
     banner("learned proposal gain (shipped network)")
     net, summary = train_proposal.load()
     m = workloads.gaussian_blob_map()
-    res = learned.evaluate(net, episodes=4, steps=100, seed=5, proposal_rollouts=2048, cost_map=m, log=print)
-    plain, lrn, small = res["plain"]["cost"], res["learned"]["cost"], res["plain_small"]["cost"]
-    print(f"[learned] cost: plain {plain:.4f}, learned {lrn:.4f} (fallback {res['learned']['fallback_fraction']:.2f})"
-          f", plain with the same samples {small:.4f}; gain {(small - lrn) / small * 100:.3f} %, "
-          f"gap to plain {(lrn - plain) / plain * 100:.3f} %; rate {res['learned']['hz']:.0f} vs "
-          f"{res['plain']['hz']:.0f} Hz")
+    q = learned.cold_start_quality(net, starts=48, seed=11, proposal_rollouts=2048, cost_map=m, log=print)
+    plain, lrn, small = q["plain"]["cost"], q["learned"]["cost"], q["plain_small"]["cost"]
+    print(f"[learned] cold-start sequence cost: plain {plain:.4f}, learned {lrn:.4f}, same samples {small:.4f}: "
+          f"gain {(small - lrn) / small * 100:.3f} %, gap to plain {(lrn - plain) / plain * 100:.3f} %")
     assert lrn < small
     assert lrn <= plain * 1.01
+    res = learned.evaluate(net, episodes=4, steps=100, seed=7, proposal_rollouts=2048, cost_map=m, log=print)
+    print(f"[learned] closed loop: plain {res['plain']['cost']:.4f}, learned {res['learned']['cost']:.4f}, same "
+          f"samples {res['plain_small']['cost']:.4f}; rate {res['learned']['hz']:.0f} vs {res['plain']['hz']:.0f} Hz")
+    assert res["learned"]["cost"] <= res["plain"]["cost"] * 1.01
     net.close()
 
 

@@ -123,7 +123,7 @@ def test_proposal_iteration_matches_oracle(hip, engine):
     y = ol.proposal_forward(S, C, T, th, X, bf16=True)[0]
     sb = np.array([prob.sampler.std_dev[c] for c in range(C)], np.float32)
     dmu = (y[:TC].reshape(T, C) * sb).astype(np.float32)
-    sig = (np.exp(np.clip(y[TC:], -1.5, 1.0)).reshape(T, C) * sb).astype(np.float32)
+    sig = (np.exp(np.clip(y[TC:], -0.693147, 1.0)).reshape(T, C) * sb).astype(np.float32)
     eps = gpu_normals(prob.sampler.seed, 0, Kp * TC).reshape(Kp, T, C)
     u = (nominal + dmu + sig * eps).astype(np.float32)
     u[0] = nominal
@@ -195,5 +195,6 @@ def test_fused_inference_equals_batched_forward(hip, engine):
     X = ((network_inputs(x0[None], nominal[None]) - shift) * scale).astype(np.float32)
     y = ol.proposal_forward(S, C, T, th, X, bf16=True)[0]
     np.testing.assert_allclose(d1, y[:T * C].reshape(T, C) * sb, rtol=1e-4, atol=1e-6)
-    np.testing.assert_allclose(s1, np.exp(np.clip(y[T * C:], -1.5, 1)).reshape(T, C) * sb, rtol=1e-4, atol=1e-6)
+    want_sig = np.exp(np.clip(y[T * C:], -0.693147, 1)).reshape(T, C) * sb
+    np.testing.assert_allclose(s1, want_sig, rtol=1e-4, atol=1e-6)
     net.close()
