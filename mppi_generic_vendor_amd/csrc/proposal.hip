@@ -219,47 +219,73 @@ __global__ void proposal_head_kernel(const float* __restrict__ y, int TC, int C,
 // head. The operands are rounded to BF16 and accumulated in FP32 over 32-wide k-steps in order, the arithmetic of
 // gemm_bf16_kernel, so the result equals the batched forward pass.
 constexpr int PI_THREADS = 1024;
+constexpr int PI_WAVES = PI_THREADS / 64;
 
-template <int K>
-__device__ __forceinline__ void infer_layer(const float* __restrict__ W, const float* __restrict__ bias, int N,
-                                            int Kreal, const float* vin, float* vout, bool gelu)
+// Weights of the 16-row tile `mt` of an N x Kreal layer for this lane (row 16 mt + lane % 16, K slots of the lane
+// group), loaded as FP32 (16-byte loads when the row stride allows) and rounded to BF16 A-operand fragments.
+template <int KS>
+__device__ __forceinline__ void infer_load_tile(const float* __restrict__ W, int N, int Kreal, int mt,
+                                                pbf16x8 (&a)[KS])
 {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int g = lane >> 4, l16 = lane & 15;
-  constexpr int KS = (K + 31) / 32;
-  pbf16x8 bfr[KS];
+  const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
+  const int row = 16 * mt + l16;
+  if ((Kreal & 3) == 0) {
 #pragma unroll
-  for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = 32 * ks + 8 * g + j;
-      bfr[ks][j] = (__bf16)(k < Kreal ? vin[k] : 0.0f);
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = 32 * ks + 8 * g;
+      float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0;
+      if (row < N && k < Kreal) {
+        const float4* p4 = reinterpret_cast<const float4*>(W + (int64_t)row * Kreal + k);
+        v0 = p4[0];
+        v1 = p4[1];
+      }
+      a[ks][0] = (__bf16)v0.x;
+      a[ks][1] = (__bf16)v0.y;
+      a[ks][2] = (__bf16)v0.z;
+      a[ks][3] = (__bf16)v0.w;
+      a[ks][4] = (__bf16)v1.x;
+      a[ks][5] = (__bf16)v1.y;
+      a[ks][6] = (__bf16)v1.z;
+      a[ks][7] = (__bf16)v1.w;
     }
-  for (int mt = wave; mt * 16 < N; mt += PI_THREADS / 64) {
-    const int row = 16 * mt + l16;
-    // every weight of the tile is requested before the first product
-    float wv[KS][8];
+  } else {
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int k = 32 * ks + 8 * g + j;
-        wv[ks][j] = (row < N && k < Kreal) ? W[(int64_t)row * Kreal + k] : 0.0f;
+        a[ks][j] = (__bf16)((row < N && k < Kreal) ? W[(int64_t)row * Kreal + k] : 0.0f);
       }
+  }
+}
+
+// One layer for the input vector vin (LDS, broadcast over the 16 B columns): vout[r] = act(W[r] . vin + bias[r]) for
+// the tiles whose A fragments `a` this wave holds (tile mt_of(i) for i < nt).
+template <int KS, int NT>
+__device__ __forceinline__ void infer_apply(const pbf16x8 (&a)[NT][KS], const int (&mt)[NT], const float* bias,
+                                            int N, int Kreal, const float* vin, float* vout, bool gelu)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
+  pbf16x8 b[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 32 * ks + 8 * g + j;
+      b[ks][j] = (__bf16)(k < Kreal ? vin[k] : 0.0f);
+    }
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    if (mt[i] * 16 >= N) continue;
     pf32x4 acc = pf32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      pbf16x8 a;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) a[j] = (__bf16)wv[ks][j];
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[ks], acc, 0, 0, 0);
-    }
+    for (int ks = 0; ks < KS; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][ks], b[ks], acc, 0, 0, 0);
     if (l16 == 0) {  // column 0 of the result
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = 16 * mt + 4 * g + i;
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const int r = 16 * mt[i] + 4 * g + r4;
         if (r < N) {
-          const float z = acc[i] + bias[r];
+          const float z = acc[r4] + bias[r];
           vout[r] = gelu ? gelu_f(z) : z;
         }
       }
@@ -277,7 +303,17 @@ __global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
   __shared__ float xin[DIN];
   __shared__ float h1[PROP_H], h2[PROP_H];
   extern __shared__ float yout[];  // [2 T C]
-  const int din = S + PROP_KNOTS * C, TC = T * C;
+  constexpr int KS1 = DIN / 32, KSH = PROP_H / 32;
+  constexpr int NT3 = 2;  // output tiles per wave: 2 T C <= 16 * PI_WAVES * NT3
+  const int wave = threadIdx.x >> 6;
+  const int din = S + PROP_KNOTS * C, TC = T * C, dout = 2 * TC;
+  // every wave requests all the weights it will multiply before anything else (they do not depend on the input)
+  const int mt1[1] = {wave}, mt2[1] = {wave}, mt3[NT3] = {wave, wave + PI_WAVES};
+  pbf16x8 a1[1][KS1], a2[1][KSH], a3[NT3][KSH];
+  infer_load_tile<KS1>(th + w1, wave < PROP_H / 16 ? PROP_H : 0, din, wave, a1[0]);
+  infer_load_tile<KSH>(th + w2, wave < PROP_H / 16 ? PROP_H : 0, PROP_H, wave, a2[0]);
+#pragma unroll
+  for (int i = 0; i < NT3; ++i) infer_load_tile<KSH>(th + w3, dout, PROP_H, mt3[i], a3[i]);
   for (int o = threadIdx.x; o < DIN; o += PI_THREADS) {
     float v = 0.0f;
     if (o < S) {
@@ -289,11 +325,11 @@ __global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
     xin[o] = v;
   }
   __syncthreads();
-  infer_layer<DIN>(th + w1, th + b1, PROP_H, din, xin, h1, true);
+  if (wave < PROP_H / 16) infer_apply<KS1, 1>(a1, mt1, th + b1, PROP_H, din, xin, h1, true);
   __syncthreads();
-  infer_layer<PROP_H>(th + w2, th + b2, PROP_H, PROP_H, h1, h2, true);
+  if (wave < PROP_H / 16) infer_apply<KSH, 1>(a2, mt2, th + b2, PROP_H, PROP_H, h1, h2, true);
   __syncthreads();
-  infer_layer<PROP_H>(th + w3, th + b3, 2 * TC, PROP_H, h2, yout, false);
+  infer_apply<KSH, NT3>(a3, mt3, th + b3, dout, PROP_H, h2, yout, false);
   __syncthreads();
   for (int j = threadIdx.x; j < TC; j += PI_THREADS) {
     const float sb = sigma_base[j % C];
@@ -401,6 +437,7 @@ int proposal_infer(ProposalNet* p, const float* x0, const float* nominal, const 
 {
   const PropDims& d = p->d;
   const size_t lds = sizeof(float) * (size_t)d.dout;
+  if (d.dout > 16 * PI_WAVES * 2) return fail(MPPI_E_SIZE, "proposal: output too wide for the inference kernel");
 #define MPPI_INFER(DIN)                                                                                          \
   proposal_infer_kernel<DIN><<<1, PI_THREADS, lds, s>>>(d.S, d.C, d.T, p->theta, d.w1, d.b1, d.w2, d.b2, d.w3,   \
                                                         d.b3, x0, nominal, p->shift, p->scale, sigma_base, dmu, sig)
