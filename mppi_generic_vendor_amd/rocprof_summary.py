@@ -1,6 +1,6 @@
 """Summarise rocprofv3 databases (rocpd SQLite): kernel-trace statistics and HBM traffic per dispatch.
 
-    python -m mppi_generic_vendor_amd.rocprof_summary TRACE_DB FETCH_DB WRITE_DB [--calibration-mib 256]
+    python -m mppi_generic_vendor_amd.rocprof_summary TRACE_DB [FETCH_DB WRITE_DB] [--calibration-mib 256]
                                                      [--json OUT.json]
 
 TRACE_DB comes from `rocprofv3 --kernel-trace --stats`, FETCH_DB from `rocprofv3 --pmc FETCH_SIZE --kernel-trace`,
@@ -76,11 +76,16 @@ def render(s) -> str:
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("trace_db")
-    ap.add_argument("fetch_db")
-    ap.add_argument("write_db")
+    ap.add_argument("fetch_db", nargs="?")
+    ap.add_argument("write_db", nargs="?")
     ap.add_argument("--calibration-mib", type=int, default=256)
     ap.add_argument("--json", default=None, help="also write the traffic table as JSON")
     a = ap.parse_args()
+    if a.fetch_db is None or a.write_db is None:
+        print("kernel-trace statistics (average over all dispatches of the run):")
+        for name, calls, avg, pct in kernel_stats(a.trace_db):
+            print(f"  {avg:10.3f} us  calls {calls:6d}  {pct:5.1f}%  {name}")
+        return
     s = summarise(a.trace_db, a.fetch_db, a.write_db, a.calibration_mib)
     print(render(s))
     if a.json:
