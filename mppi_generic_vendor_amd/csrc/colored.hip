@@ -11,11 +11,15 @@
 //     B[t][2f]   =  w_f (cos(2 pi f t/n) - decay^t cos(2 pi f stride/n))
 //     B[t][2f+1] = -w_f (sin(2 pi f t/n) - decay^t sin(2 pi f stride/n)),   w_f = 1 at f = 0 and f = n/2, else 2
 // (the imaginary parts the C2R transform ignores -- DC and Nyquist -- and those configureFrequencyNoise zeroes
-// (:26-33) have B = 0), evaluated as an FP32 MFMA product (v_mfma_f32_16x16x4f32): every workgroup of 16 wavefronts
-// owns up to 64 series, stages their scaled spectra in LDS, builds the basis in K-chunks of 32 columns; wavefront w
-// accumulates the 16 x 16 output tiles (M-tile mg + i*MG, N-tile nt) in registers. The epilogue divides by
-// sigma_c * n, applies the Gaussian transform and writes the workgroup's contiguous span of the [K][T][C] sample
-// buffer with 16-byte stores.
+// (:26-33) have B = 0). The basis is built once per launch (colored_basis_kernel) as FP16 high and low parts; every
+// workgroup of 16 wavefronts owns up to 64 series and stages their scaled spectra in LDS as FP16 high and low parts;
+// wavefront w accumulates its 16 x 16 output tiles (M-tile mg + i*MG, N-tile nt) with v_mfma_f32_16x16x32_f16 as
+// Bh Yh + Bh Yl + Bl Yh. The epilogue divides by sigma_c * n, applies the Gaussian transform and writes the
+// workgroup's contiguous span of the [K][T][C] sample buffer with 16-byte stores.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -27,9 +31,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ IterArgs load_iter_c(const IterRef& r) { return r.ptr ? *r.ptr : r.val; }
 
-constexpr int CKC = 32;        // basis K-chunk (columns)
+constexpr int CKC = 32;        // k-step of the FP16 product (basis columns)
 constexpr int CTHREADS = 1024; // 16 wavefronts
-constexpr int CKCP = CKC + 1;  // padded LDS row of the basis chunk
 
 struct Sigma4 {
   float v[MPPI_MAX_CONTROL];
@@ -65,10 +68,16 @@ static Sigma4 colored_sigma(const mppi_sampler_params& sp, int C, int n)
   return s;
 }
 
+// Launch shape: NS series (NS / C samples) per workgroup of 16 wavefronts, MT = ceil(T / 16) row tiles, Kpad = 2F
+// rounded up to the 32-wide k-step; wavefront w owns row tile w % MT against all NS / 16 series tiles over the k-group
+// w / MT of S = min(16 / MT, Kpad / 32) k-groups (the k-groups are summed in LDS in a fixed order).
 struct ColoredShape {
-  int n, F, Kpad, KP, MT, MTP, NS, spw, threads;
-  size_t lds_floats;
+  int n, F, Kpad, KPH, MT, NS, spw, S;
+  size_t lds_bytes;
 };
+
+// launch constants: the scale table [F][C] (F C <= 257 * 4 = 1028) at 0, two per-component factors at CST_INV, CST_SD
+constexpr int CST_INV = 1032, CST_SD = 1036, CONST_FLOATS = 1040;
 
 static ColoredShape colored_shape(int T, int C, int mode)
 {
@@ -76,36 +85,159 @@ static ColoredShape colored_shape(int T, int C, int mode)
   sh.n = (mode == MPPI_FFT_LEN_T) ? T : 2 * T;
   sh.F = sh.n / 2 + 1;
   sh.Kpad = (2 * sh.F + CKC - 1) / CKC * CKC;
-  sh.KP = sh.Kpad + 1;
+  sh.KPH = sh.Kpad + 8;  // LDS row of a series' spectrum in halves: 16-byte aligned, consecutive rows shift banks
   sh.MT = (T + 15) / 16;
+  sh.S = sh.MT > 0 ? std::max(1, std::min(16 / sh.MT, sh.Kpad / CKC)) : 1;
   for (int ns = 64; ns >= 16; ns /= 2) {
     sh.NS = ns;
     sh.spw = ns / C;
-    sh.threads = CTHREADS;
-    // every wavefront owns MPW M-tiles of one N-tile; the basis rows are padded to MPW * MG * 16 (zero rows)
-    const int mg = (CTHREADS / 64) / (ns / 16);
-    sh.MTP = (sh.MT + mg - 1) / mg * mg;
-    const size_t tables = 2 * (size_t)sh.n + T + (((size_t)sh.F * C + 3) & ~(size_t)3);
-    sh.lds_floats = tables + (size_t)ns * sh.KP + (size_t)sh.MTP * 16 * CKCP;
-    if (sh.lds_floats * sizeof(float) <= LDS_LIMIT && sh.spw >= 1) break;
+    const size_t tables = (((size_t)sh.F * C + 3) & ~(size_t)3) + (((size_t)T * C + 3) & ~(size_t)3) + 8;
+    // high and low spectra; the output staging [spw][T][C] reuses the same space (KPH > T)
+    const size_t xh = 2 * (size_t)ns * sh.KPH * sizeof(_Float16);
+    sh.lds_bytes = tables * sizeof(float) + xh;
+    if (sh.lds_bytes <= LDS_LIMIT && sh.spw >= 1) break;
   }
   return sh;
 }
 
-template <int MPW>
-__global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_params sp, int K, int T, int C, int n,
-                                                                int F, int MTP, int Kpad, int KP, int NS, int spw,
-                                                           const float* __restrict__ mean, float* __restrict__ out,
-                                                           IterRef itr, const float* __restrict__ spectra_in,
-                                                           int pstart, Sigma4 sig)
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+// v = hi + lo to about 2^-22 relative: hi is v rounded to FP16, lo the rounded remainder. v is pinned to one FP32
+// register first (an empty asm with a register operand, no memory access): otherwise the product that computes v is
+// fused into one of the conversions (v_fma_mix rounds the exact product to FP16 once) and not into the other, and at
+// FP16 ties the two roundings differ by one unit, which lo no longer compensates.
+__device__ __forceinline__ void split_f16(float v, _Float16& hi, _Float16& lo)
 {
+  asm volatile("" : "+v"(v));
+  hi = (_Float16)v;
+  lo = (_Float16)(v - (float)hi);
+}
+
+// Power-of-two scales of the FP16 operands (exact): the spectra |Y| <= 6-ish and the basis |B| <= 4 are multiplied by
+// 2^8 and 2^12, so that the low part (about 2^-11 of the value) stays a normal FP16 number down to |Y| ~ 5e-4 and
+// |B| ~ 3e-5, and the largest products stay far inside the FP32 range. Fed spectra must stay below 255 in magnitude.
+constexpr float Y_SCALE = 256.0f, B_SCALE = 4096.0f;
+
+// Per-launch constants, written by colored_prep_kernel:
+//   cst[f * C + c]        Y_SCALE * s_c(f) / smax_c (power-law scale; low frequencies take the first admissible one)
+//   cst[CST_INV + c]      smax_c / (sigma_c n Y_SCALE B_SCALE)      (the epilogue's factor)
+//   cst[CST_SD + c]       std_dev_decay^iteration * std_dev[c]
+// and the basis times B_SCALE, FP16 high and low parts: row t < MT * 16, column j < Kpad (2f: real part, 2f + 1:
+// imaginary part); rows >= T and columns >= 2F are zero. The phases f t / n are reduced in integers.
+__global__ void __launch_bounds__(256) colored_prep_kernel(mppi_sampler_params sp, int n, int F, int T, int C,
+                                                           int rows, int Kpad, IterRef itr, Sigma4 sig,
+                                                           float* __restrict__ cst, _Float16* __restrict__ bhi,
+                                                           _Float16* __restrict__ blo)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const IterArgs a = load_iter_c(itr);
+  const float cutoff = fmaxf(sp.fmin, 1.0f / n);
+  int j0 = 0;
+  while (j0 < F && (j0 / (1.0f * n)) < cutoff) ++j0;
+  if (i < F * C) {
+    const int fi = i / C, c = i - fi * C;
+    const int src = (fi < j0 && j0 < F) ? j0 : fi;
+    // f^(-beta/2) is monotone in f: the largest scale is at the lowest or the highest admissible frequency
+    const int lo = (j0 < F) ? j0 : 0;
+    const float e = -sp.exponents[c] / 2.0f;
+    const float smax = fmaxf(powf(lo / (1.0f * n), e), powf((F - 1) / (1.0f * n), e));
+    cst[i] = powf(src / (1.0f * n), e) / smax * Y_SCALE;
+    if (fi == 0) {
+      cst[CST_INV + c] = smax / (sig.v[c] * n) / (Y_SCALE * B_SCALE);
+      cst[CST_SD + c] = powf(sp.std_dev_decay, (float)a.iteration) * sp.std_dev[c];
+    }
+  }
+  const int Fp = Kpad >> 1;
+  if (i >= rows * Fp) return;
+  const int t = i / Fp, f = i - t * Fp;
+  float re = 0.0f, im = 0.0f;
+  if (t < T && f < F) {
+    const int stride = min(max(a.stride, 0), n - 1);
+    const float dt = (sp.offset_decay_rate == 0.0f) ? 0.0f : powf(sp.offset_decay_rate, (float)t);
+    const int mt = (int)(((long long)f * t) % n), ms = (int)(((long long)f * stride) % n);
+    float s_t, c_t, s_s, c_s;
+    sincospif((float)(2 * mt) / (float)n, &s_t, &c_t);
+    sincospif((float)(2 * ms) / (float)n, &s_s, &c_s);
+    const bool edge = (f == 0) || (2 * f == n);
+    const float w = edge ? 1.0f : 2.0f;
+    re = w * (c_t - dt * c_s);
+    const bool zero_im = edge || ((F & 1) && f == F - 1);
+    im = zero_im ? 0.0f : -w * (s_t - dt * s_s);
+  }
+  _Float16 h0, l0, h1, l1;
+  split_f16(re * B_SCALE, h0, l0);
+  split_f16(im * B_SCALE, h1, l1);
+  const half2v h = {h0, h1}, l = {l0, l1};
+  *reinterpret_cast<half2v*>(bhi + (size_t)t * Kpad + 2 * f) = h;
+  *reinterpret_cast<half2v*>(blo + (size_t)t * Kpad + 2 * f) = l;
+}
+
+// Stages the 4-block of normals z starting at element e0 (even; -2 for the first block when the span does not start
+// on a 4-boundary) of the workgroup's span of cnt spectrum values: TF is even, so (z0, z1) and (z2, z3) are each one
+// (Re, Im) pair of one series, scaled and split into FP16 high and low parts.
+__device__ __forceinline__ void stage_spectra(const float* z, int e0, int cnt, int TF, int C, int KPH, float invTF,
+                                              float invC, const float* scale, _Float16* Xh, _Float16* Xl)
+{
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const int e = e0 + 2 * p;
+    if (e >= 0 && e < cnt) {
+      // e / TF and ns % C by a float reciprocal: exact, the quotients' fractional parts are >= 1 / (2 TF) from 1
+      const int ns = (int)(((float)e + 0.5f) * invTF);
+      const int jj = e - ns * TF;
+      const int c = ns - (int)(((float)ns + 0.5f) * invC) * C;
+      const float s = scale[(jj >> 1) * C + c];
+      _Float16 h0, l0, h1, l1;
+      split_f16(z[2 * p] * s, h0, l0);
+      split_f16(z[2 * p + 1] * s, h1, l1);
+      const half2v h = {h0, h1}, l = {l0, l1};
+      *reinterpret_cast<half2v*>(Xh + ns * KPH + jj) = h;
+      *reinterpret_cast<half2v*>(Xl + ns * KPH + jj) = l;
+    }
+  }
+}
+
+constexpr int KPRE = 4;  // k-steps whose basis fragments are held in registers from the kernel's start
+
+// One 32-wide k-step at column kc0: acc[q] += Bh Yh + Bh Yl + Bl Yh for the series tiles q < NTT (Y from LDS).
+template <int NTT>
+__device__ __forceinline__ void product_step(const half8& ah, const half8& al, const _Float16* xh, const _Float16* xl,
+                                             int KPH, int kc0, floatx4* acc)
+{
+#pragma unroll
+  for (int q = 0; q < NTT; ++q) {
+    const half8 bh = *reinterpret_cast<const half8*>(xh + q * 16 * KPH + kc0);
+    const half8 bl = *reinterpret_cast<const half8*>(xl + q * 16 * KPH + kc0);
+    acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[q], 0, 0, 0);
+    acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[q], 0, 0, 0);
+    acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[q], 0, 0, 0);
+  }
+}
+
+// Per workgroup: the scaled spectra of its series (every scale divided by the component's largest, so the values stay
+// far inside the FP16 range; the factor returns in the epilogue), staged in LDS as FP16 high and low parts; then
+// OUT = B Y on v_mfma_f32_16x16x32_f16 as Bh Yh + Bh Yl + Bl Yh (the dropped Bl Yl term is about 2^-22 relative),
+// accumulated in FP32. Each basis fragment is read from the launch's basis once per workgroup (one k-step ahead)
+// and applied to all NTT series tiles.
+template <int NTT>
+__global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_params sp, int K, int T, int C, int F,
+                                                                int Kpad, int KPH, int MT, int S, int spw,
+                                                                const float* __restrict__ mean,
+                                                                float* __restrict__ out, IterRef itr,
+                                                                const float* __restrict__ spectra_in, int pstart,
+                                                                const float* __restrict__ cst,
+                                                                const _Float16* __restrict__ bhi,
+                                                                const _Float16* __restrict__ blo)
+{
+  constexpr int NS = NTT * 16;
   extern __shared__ float lds[];
-  float* ct = lds;               // [n]
-  float* st = ct + n;            // [n]
-  float* dec = st + n;           // [T]
-  float* scale = dec + T;        // [F][C]
-  float* X = scale + ((F * C + 3) & ~3);  // [NS][KP]   scaled spectra; reused as the output staging [spw][T][C]
-  float* Bc = X + NS * KP;       // [MTP*16][CKCP]
+  float* scale = lds;                                                 // [F][C]
+  float* mean_l = scale + ((F * C + 3) & ~3);                         // [T][C]
+  float* fac = mean_l + ((T * C + 3) & ~3);                           // [8] cst[CST_INV ..]: inv[4], sd[4]
+  _Float16* Xh = reinterpret_cast<_Float16*>(fac + 8);                // [NS][KPH] high parts
+  _Float16* Xl = Xh + NS * KPH;                                       // [NS][KPH] low parts
+  float* stage = fac + 8;                                             // [spw][T][C], after the product
 
   const IterArgs a = load_iter_c(itr);
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -113,174 +245,197 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   const int nsamp = min(spw, K - k0);
   const int nser = nsamp * C;
   const int TF = 2 * F;
-  const int stride = min(max(a.stride, 0), n - 1);
+  const int TC = T * C;
 
-  // tables: cos/sin of 2 pi m / n, decay^t, power-law scales (low frequencies take the first admissible one)
-  for (int m = tid; m < n; m += nthr) {
-    float s, c;
-    sincospif((float)(2 * m) / (float)n, &s, &c);
-    ct[m] = c;
-    st[m] = s;
+  // wavefront w: row tile m = w % MT against all series tiles over the k-steps [kb, ke) of k-group w / MT; lane
+  // (lr, lk) supplies row lr and k-columns 8 lk .. 8 lk + 7 of each 32-wide k-step. The basis fragments of the
+  // first KPRE k-steps are requested here, so that their latency passes while the spectra are generated.
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int m = wave % MT, sg = wave / MT;
+  const bool active = sg < S;
+  const int ks = Kpad / CKC;
+  const int kb = active ? (sg * ks) / S : 0, ke = active ? ((sg + 1) * ks) / S : 0;
+  const _Float16* gh = bhi + (size_t)(m * 16 + lr) * Kpad + 8 * lk;
+  const _Float16* gl = blo + (size_t)(m * 16 + lr) * Kpad + 8 * lk;
+  // the tables first (the LDS stores below wait for them, and the memory counter retires in order), then the basis
+  // fragments; every load is unconditional, at a clamped (in-bounds) index
+  const int FC = F * C;
+  const float sv0 = cst[min(tid, FC - 1)], sv1 = cst[min(tid + CTHREADS, FC - 1)];
+  const float mv = mean[min(tid, TC - 1)];
+  const float fv = cst[CST_INV + min(tid, 7)];
+  __builtin_amdgcn_sched_barrier(0);  // keeps the table loads ahead of the basis loads
+  half8 ph[KPRE], pl[KPRE];
+#pragma unroll
+  for (int i = 0; i < KPRE; ++i) {
+    const int k = min(kb + i, ks - 1);
+    ph[i] = *reinterpret_cast<const half8*>(gh + k * CKC);
+    pl[i] = *reinterpret_cast<const half8*>(gl + k * CKC);
   }
-  for (int t = tid; t < T; t += nthr) dec[t] = (sp.offset_decay_rate == 0.0f) ? 0.0f : powf(sp.offset_decay_rate, (float)t);
-  const float cutoff = fmaxf(sp.fmin, 1.0f / n);
-  int j0 = 0;
-  while (j0 < F && (j0 / (1.0f * n)) < cutoff) ++j0;
-  for (int i = tid; i < F * C; i += nthr) {
-    const int fi = i / C, c = i - fi * C;
-    const int src = (fi < j0 && j0 < F) ? j0 : fi;
-    scale[i] = powf(src / (1.0f * n), -sp.exponents[c] / 2.0f);
-  }
-  __syncthreads();
-  // scaled spectra of series s0 .. s0 + nser - 1 (a contiguous span of the [K][C][F][2] normal stream)
+  floatx4 acc[NTT];
+#pragma unroll
+  for (int q = 0; q < NTT; ++q) acc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  // unconditional stores (threads past the end write the last entry's own value again), so that no load is sunk
+  // into a branch behind the basis loads
+  // the first 4-block of normals is generated while the loads are in flight
   const int64_t base = (int64_t)k0 * C * TF;
   const int64_t cnt = (int64_t)nser * TF;
-  for (int64_t gb = (base >> 2) + tid; gb * 4 < base + cnt; gb += nthr) {
-    float z[4];
-    if (spectra_in != nullptr) {
+  const int64_t gb0 = (base >> 2) + tid;
+  Normal4 nz0;
+  if (spectra_in == nullptr) nz0 = normals4(sp.seed, a.rng_offset, (uint64_t)gb0);
+  scale[min(tid, FC - 1)] = sv0;
+  scale[min(tid + CTHREADS, FC - 1)] = sv1;
+  mean_l[min(tid, TC - 1)] = mv;
+  fac[min(tid, 7)] = fv;
+  __syncthreads();
+  // scaled spectra of series s0 .. s0 + nser - 1 (a contiguous span of the [K][C][F][2] normal stream). TF is even
+  // and so is base, so every (2j, 2j + 1) pair of a 4-block is one (Re, Im) pair of one series.
+  const float invTF = 1.0f / TF, invC = 1.0f / C;
+  // one loop per source, so that the generated one holds no memory load (a load in the other path would make every
+  // iteration wait for the basis fragments in flight)
+  if (spectra_in != nullptr) {
+    for (int64_t gb = (base >> 2) + tid; gb * 4 < base + cnt; gb += nthr) {
+      float z[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int64_t g = gb * 4 + j;
         z[j] = (g >= base && g < base + cnt) ? spectra_in[g] : 0.0f;
       }
-    } else {
+      stage_spectra(z, (int)(gb * 4 - base), (int)cnt, TF, C, KPH, invTF, invC, scale, Xh, Xl);
+    }
+  } else {
+    if (gb0 * 4 < base + cnt) stage_spectra(nz0.z, (int)(gb0 * 4 - base), (int)cnt, TF, C, KPH, invTF, invC, scale, Xh, Xl);
+    for (int64_t gb = gb0 + nthr; gb * 4 < base + cnt; gb += nthr) {
       const Normal4 nz = normals4(sp.seed, a.rng_offset, (uint64_t)gb);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) z[j] = nz.z[j];
-    }
-    const int e0 = (int)(gb * 4 - base);  // may be negative for the first block
-    int ns_ = (e0 >= 0) ? e0 / TF : -1;
-    int jj = e0 - ns_ * TF;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = e0 + j;
-      if (e >= 0 && e < cnt) X[ns_ * KP + jj] = z[j] * scale[(jj >> 1) * C + (ns_ % C)];
-      if (++jj == TF) {
-        jj = 0;
-        ++ns_;
-      }
+      stage_spectra(nz.z, (int)(gb * 4 - base), (int)cnt, TF, C, KPH, invTF, invC, scale, Xh, Xl);
     }
   }
-  // zero padding: columns [2F, KP) of the valid series, whole rows of the unused series
-  const int padc = KP - TF;
-  for (int i = tid; i < nser * padc; i += nthr) {
-    const int r = i / padc;
-    X[r * KP + TF + (i - r * padc)] = 0.0f;
-  }
-  for (int i = tid; i < (NS - nser) * KP; i += nthr) X[nser * KP + i] = 0.0f;
-
-  const int wave = tid >> 6, lane = tid & 63;
-  const int lr = lane & 15, lk = lane >> 4;
-  const int NTT = NS / 16;               // N-tiles (16 series each)
-  const int MG = (CTHREADS / 64) / NTT;  // wavefronts sharing one N-tile
-  const int nt = wave % NTT, mg = wave / NTT;
-  floatx4 acc[MPW];
-#pragma unroll
-  for (int i = 0; i < MPW; ++i) acc[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-
-  for (int kc0 = 0; kc0 < Kpad; kc0 += CKC) {
-    __syncthreads();  // spectra staged / previous chunk consumed
-    // basis chunk: thread -> (row t, 8 consecutive columns); the phase index advances by t per frequency
-    for (int item = tid; item < MTP * 16 * (CKC / 8); item += nthr) {
-      const int t = item / (CKC / 8);
-      const int cb = (item - t * (CKC / 8)) * 8;
-      float* brow = Bc + t * CKCP + cb;
-      if (t >= T) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) brow[q] = 0.0f;
-        continue;
-      }
-      const float dt_ = dec[t];
-      const int jbeg = kc0 + cb;  // even
-      int f = jbeg >> 1;
-      int mt_ = (int)(((long long)f * t) % n);
-      int ms_ = (int)(((long long)f * stride) % n);
-#pragma unroll
-      for (int q = 0; q < 8; q += 2) {
-        float re = 0.0f, im = 0.0f;
-        if (f < F) {
-          const bool edge = (f == 0) || (2 * f == n);
-          const float w = edge ? 1.0f : 2.0f;
-          re = w * (ct[mt_] - dt_ * ct[ms_]);
-          const bool zero_im = edge || ((F & 1) && f == F - 1);
-          im = zero_im ? 0.0f : -w * (st[mt_] - dt_ * st[ms_]);
-        }
-        brow[q] = re;
-        brow[q + 1] = im;
-        ++f;
-        mt_ += t;
-        if (mt_ >= n) mt_ -= n;
-        ms_ += stride;
-        if (ms_ >= n) ms_ -= n;
-      }
+  // zero padding: columns [2F, Kpad) of the valid series, whole rows of the unused series (pairs of halves)
+  {
+    const int padp = (Kpad - TF) >> 1;
+    const half2v zz = {(_Float16)0.0f, (_Float16)0.0f};
+    for (int i = tid; i < nser * padp; i += nthr) {
+      const int r = i / padp, col = TF + 2 * (i - r * padp);
+      *reinterpret_cast<half2v*>(Xh + r * KPH + col) = zz;
+      *reinterpret_cast<half2v*>(Xl + r * KPH + col) = zz;
     }
-    __syncthreads();
-    const float* xrow = X + (nt * 16 + lr) * KP + kc0 + lk;
-#pragma unroll
-    for (int ks = 0; ks < CKC / 4; ++ks) {
-      const float bv = xrow[ks * 4];
-#pragma unroll
-      for (int i = 0; i < MPW; ++i) {
-        const float av = Bc[((mg + i * MG) * 16 + lr) * CKCP + ks * 4 + lk];
-        acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[i], 0, 0, 0);
-      }
-    }
-  }
-  __syncthreads();  // X is reused as the output staging below
-
-  // epilogue: lane holds OUT[t = 16 mt + 4 lk + v][series 16 nt + lr] for mt = mg + i * MG
-  const float decay_it = powf(sp.std_dev_decay, (float)a.iteration);
-  const int TC = T * C;
-  const int sidx = nt * 16 + lr;
-  if (sidx < nser) {
-    const int c = sidx % C, r = sidx / C, k = k0 + r;
-    const float inv = 1.0f / (sig.v[c] * n);
-    const float sd = decay_it * sp.std_dev[c];
-#pragma unroll
-    for (int i = 0; i < MPW; ++i) {
-      const int mt = mg + i * MG;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int t = mt * 16 + lk * 4 + v;
-        if (t < T) {
-          const float eps = acc[i][v] * inv;
-          const float mu = mean[t * C + c];
-          float u;
-          if (k == 0 || t < a.stride) u = mu;
-          else if (k >= pstart) u = sd * eps;
-          else u = mu + sd * eps;
-          X[r * TC + t * C + c] = u;
-        }
-      }
+    for (int i = tid; i < ((NS - nser) * KPH) >> 1; i += nthr) {
+      *reinterpret_cast<half2v*>(Xh + nser * KPH + 2 * i) = zz;
+      *reinterpret_cast<half2v*>(Xl + nser * KPH + 2 * i) = zz;
     }
   }
   __syncthreads();
+
+  // product: the k-steps beyond the prefetched ones stream their basis fragments from memory
+  if (active) {
+    const _Float16* xh = Xh + lr * KPH + 8 * lk;
+    const _Float16* xl = Xl + lr * KPH + 8 * lk;
+#pragma unroll
+    for (int i = 0; i < KPRE; ++i)
+      if (kb + i < ke) product_step<NTT>(ph[i], pl[i], xh, xl, KPH, (kb + i) * CKC, acc);
+    for (int k = kb + KPRE; k < ke; ++k)
+      product_step<NTT>(*reinterpret_cast<const half8*>(gh + k * CKC), *reinterpret_cast<const half8*>(gl + k * CKC),
+                        xh, xl, KPH, k * CKC, acc);
+  }
+  __syncthreads();  // the spectra space is reused as the output staging below
+
+  // k-groups summed into the staging buffer in order 0, 1, ..., S - 1; the last one applies the Gaussian transform.
+  // Lane holds OUT[t = 16 m + 4 lk + v][series 16 q + lr] in acc[q][v].
+  for (int g = 0; g < S; ++g) {
+    if (active && sg == g) {
+#pragma unroll
+      for (int q = 0; q < NTT; ++q) {
+        const int sidx = q * 16 + lr;
+        if (sidx < nser) {
+          const int r = sidx / C, c = sidx - r * C, k = k0 + r;
+          const float inv = fac[c], sd = fac[4 + c];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int t = m * 16 + lk * 4 + v;
+            if (t < T) {
+              float* p = stage + r * TC + t * C + c;
+              const float sum = (g > 0) ? *p + acc[q][v] : acc[q][v];
+              if (g == S - 1) {
+                const float eps = sum * inv;
+                const float mu = mean_l[t * C + c];
+                float u;
+                if (k == 0 || t < a.stride) u = mu;
+                else if (k >= pstart) u = sd * eps;
+                else u = mu + sd * eps;
+                *p = u;
+              } else {
+                *p = sum;
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
   float* dst = out + (size_t)k0 * TC;
   const int total = nsamp * TC;
   if ((((uintptr_t)dst) & 15) == 0) {
     const int n4 = total >> 2;
     for (int i = tid; i < n4; i += nthr)
-      reinterpret_cast<float4*>(dst)[i] = make_float4(X[4 * i], X[4 * i + 1], X[4 * i + 2], X[4 * i + 3]);
-    for (int e = (n4 << 2) + tid; e < total; e += nthr) dst[e] = X[e];
+      reinterpret_cast<float4*>(dst)[i] =
+          make_float4(stage[4 * i], stage[4 * i + 1], stage[4 * i + 2], stage[4 * i + 3]);
+    for (int e = (n4 << 2) + tid; e < total; e += nthr) dst[e] = stage[e];
   } else {
-    for (int e = tid; e < total; e += nthr) dst[e] = X[e];
+    for (int e = tid; e < total; e += nthr) dst[e] = stage[e];
   }
 }
 
-template <int MPW>
+// Workspace of the launch constants and the basis: one per (device, stream), sized for the largest accepted shape
+// (T <= 256: 256 rows; n <= 512: 544 columns), allocated on the first launch on that stream and kept until the
+// process ends. Launches on one stream are ordered, so each reads the constants it wrote; a graph captured on a
+// stream keeps a valid pointer. The first launch on a stream must not be inside a graph capture (the controllers
+// run their work once eagerly before capturing it).
+constexpr size_t COLORED_BASIS_MAX = (size_t)256 * 544;
+
+struct ColoredWorkspace {
+  float* cst;
+  _Float16* bhi;
+  _Float16* blo;
+};
+
+static int colored_workspace(hipStream_t s, ColoredWorkspace* w)
+{
+  int dev = 0;
+  MPPI_HIP_CHECK(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, void*> ws;
+  std::lock_guard<std::mutex> lock(mu);
+  void*& p = ws[{dev, s}];
+  if (!p) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MPPI_HIP_CHECK(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(MPPI_E_INVALID, "colored sampler: the first launch on a stream cannot be graph-captured");
+    MPPI_HIP_CHECK(hipMalloc(&p, CONST_FLOATS * sizeof(float) + 2 * COLORED_BASIS_MAX * sizeof(_Float16)));
+  }
+  w->cst = static_cast<float*>(p);
+  w->bhi = reinterpret_cast<_Float16*>(w->cst + CONST_FLOATS);
+  w->blo = w->bhi + COLORED_BASIS_MAX;
+  return MPPI_OK;
+}
+
+template <int NTT>
 static int launch_colored_t(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
-                            IterRef it, const float* spectra_in, int pstart, const ColoredShape& sh, Sigma4 sig,
-                            hipStream_t s)
+                            IterRef it, const float* spectra_in, int pstart, const ColoredShape& sh,
+                            const ColoredWorkspace& w, hipStream_t s)
 {
   static bool attr_set = false;
   if (!attr_set) {
-    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)colored_mfma_kernel<MPW>,
+    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)colored_mfma_kernel<NTT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
     attr_set = true;
   }
   const int grid = (K + sh.spw - 1) / sh.spw;
-  colored_mfma_kernel<MPW><<<grid, sh.threads, sh.lds_floats * sizeof(float), s>>>(
-      sp, K, T, C, sh.n, sh.F, sh.MTP, sh.Kpad, sh.KP, sh.NS, sh.spw, mean, samples, it, spectra_in, pstart, sig);
+  colored_mfma_kernel<NTT><<<grid, CTHREADS, sh.lds_bytes, s>>>(sp, K, T, C, sh.F, sh.Kpad, sh.KPH, sh.MT, sh.S,
+                                                               sh.spw, mean, samples, it, spectra_in, pstart, w.cst,
+                                                               w.bhi, w.blo);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
@@ -290,16 +445,23 @@ int launch_sample_colored(const mppi_sampler_params& sp, int K, int T, int C, co
 {
   const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
   const ColoredShape sh = colored_shape(T, C, sp.fft_len_mode);
-  const Sigma4 sig = colored_sigma(sp, C, sh.n);
-  if (sh.lds_floats * sizeof(float) > LDS_LIMIT || sh.spw < 1)
+  if (sh.MT > 16) return fail(MPPI_E_SIZE, "colored sampler: horizon exceeds 256");
+  if (sh.lds_bytes > LDS_LIMIT || sh.spw < 1)
     return fail(MPPI_E_SIZE, "colored sampler: horizon too long for the LDS staging");
-  const int MG = (CTHREADS / 64) / (sh.NS / 16);
-  switch (sh.MTP / MG) {
-#define MPPI_CMT(M) \
-  case M: return launch_colored_t<M>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, sig, s);
-    MPPI_CMT(1) MPPI_CMT(2) MPPI_CMT(3) MPPI_CMT(4)
-#undef MPPI_CMT
-    default: return fail(MPPI_E_SIZE, "colored sampler: horizon exceeds 256");
+  const int rows = sh.MT * 16;
+  if ((size_t)rows * sh.Kpad > COLORED_BASIS_MAX || sh.F * C > CST_INV)
+    return fail(MPPI_E_SIZE, "colored sampler: basis exceeds its workspace");
+  const Sigma4 sig = colored_sigma(sp, C, sh.n);
+  ColoredWorkspace w;
+  MPPI_CHECK(colored_workspace(s, &w));
+  const int items = std::max(rows * (sh.Kpad >> 1), sh.F * C);
+  colored_prep_kernel<<<(items + 255) / 256, 256, 0, s>>>(sp, sh.n, sh.F, T, C, rows, sh.Kpad, it, sig, w.cst, w.bhi,
+                                                          w.blo);
+  MPPI_HIP_CHECK(hipGetLastError());
+  switch (sh.NS / 16) {
+    case 4: return launch_colored_t<4>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, w, s);
+    case 2: return launch_colored_t<2>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, w, s);
+    default: return launch_colored_t<1>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, w, s);
   }
 }
 

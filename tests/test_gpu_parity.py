@@ -72,7 +72,10 @@ def test_colored_sampler_matches_oracle(hip, engine, T, mode, stride, decay):
     fed = gpu_sample_colored(sp, K, T, C, mean, stride, 1, spectra=spectra)
     assert np.array_equal(gen, fed)
     eps, want = ol.colored_samples(sp, K, T, C, mean, spectra, stride, 1)
-    np.testing.assert_allclose(gen, want, rtol=1e-4, atol=1e-4)
+    # The FP16 high/low split keeps about 22 bits: errors stay near 1e-6. A split whose two parts round the same
+    # value differently (at an FP16 tie, about 1 value in 10^4) leaves one spectral term off by one FP16 unit, about
+    # 1e-4 here, which this bound catches.
+    np.testing.assert_allclose(gen, want, rtol=1e-5, atol=1e-5)
     assert np.array_equal(gen[0], mean)
 
 
