@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -70,14 +71,31 @@ static Sigma4 colored_sigma(const mppi_sampler_params& sp, int C, int n)
 
 // Launch shape: NS series (NS / C samples) per workgroup of 16 wavefronts, MT = ceil(T / 16) row tiles, Kpad = 2F
 // rounded up to the 32-wide k-step; wavefront w owns row tile w % MT against all NS / 16 series tiles over the k-group
-// w / MT of S = min(16 / MT, Kpad / 32) k-groups (the k-groups are summed in LDS in a fixed order).
+// w / MT of S = min(16 / MT, Kpad / 32) k-groups (groups 1 .. S - 1 pass their sums to group 0 through LDS).
 struct ColoredShape {
-  int n, F, Kpad, KPH, MT, NS, spw, S;
+  int n, F, Kpad, KPH, MT, NS, spw, S, TCS;
   size_t lds_bytes;
 };
 
-// launch constants: the scale table [F][C] (F C <= 257 * 4 = 1028) at 0, two per-component factors at CST_INV, CST_SD
-constexpr int CST_INV = 1032, CST_SD = 1036, CONST_FLOATS = 1040;
+// Largest number of a wavefront's 64 lanes on one LDS bank (32 or 64 banks) when the lanes write the staging buffer
+// with row stride TCS: lane (lr, lk) writes series 16 q + lr (sample r, component c) at t = 4 lk (+ a constant).
+static int staging_conflict(int TCS, int C, int NTT)
+{
+  int worst = 0;
+  for (int banks = 32; banks <= 64; banks *= 2)
+    for (int q = 0; q < NTT; ++q) {
+      int cnt[64] = {0};
+      for (int lane = 0; lane < 64; ++lane) {
+        const int sidx = 16 * q + (lane & 15), r = sidx / C, c = sidx % C;
+        const int addr = r * TCS + 4 * (lane >> 4) * C + c;
+        worst = std::max(worst, ++cnt[addr % banks]);
+      }
+    }
+  return worst;
+}
+
+// launch constants, one contiguous table (F C <= 257 * 4 = 1028, T C <= 1024): see colored_prep_kernel
+constexpr int CONST_FLOATS = 1028 + 8 + 1024;
 
 static ColoredShape colored_shape(int T, int C, int mode)
 {
@@ -91,11 +109,23 @@ static ColoredShape colored_shape(int T, int C, int mode)
   for (int ns = 64; ns >= 16; ns /= 2) {
     sh.NS = ns;
     sh.spw = ns / C;
-    const size_t tables = (((size_t)sh.F * C + 3) & ~(size_t)3) + (((size_t)T * C + 3) & ~(size_t)3) + 8;
+    const size_t tables = ((((size_t)sh.F * C + 3) & ~(size_t)3) + 8 + (size_t)T * C + 3) & ~(size_t)3;
     // high and low spectra; the output staging [spw][T][C] reuses the same space (KPH > T)
     const size_t xh = 2 * (size_t)ns * sh.KPH * sizeof(_Float16);
-    sh.lds_bytes = tables * sizeof(float) + xh;
+    const size_t part = (size_t)(sh.S - 1) * sh.MT * (ns / 16) * 64 * 16;  // k-group partial sums
+    sh.lds_bytes = tables * sizeof(float) + xh + part;
     if (sh.lds_bytes <= LDS_LIMIT && sh.spw >= 1) break;
+  }
+  // staging row stride: T C padded (within the spectra space it reuses) to spread the stores over the banks
+  const int TC = T * C;
+  sh.TCS = TC;
+  int best = staging_conflict(TC, C, sh.NS / 16);
+  for (int pad = 1; pad < 32 && (size_t)sh.spw * (TC + pad) <= (size_t)sh.NS * sh.KPH; ++pad) {
+    const int cf = staging_conflict(TC + pad, C, sh.NS / 16);
+    if (cf < best) {
+      best = cf;
+      sh.TCS = TC + pad;
+    }
   }
   return sh;
 }
@@ -119,18 +149,22 @@ __device__ __forceinline__ void split_f16(float v, _Float16& hi, _Float16& lo)
 // |B| ~ 3e-5, and the largest products stay far inside the FP32 range. Fed spectra must stay below 255 in magnitude.
 constexpr float Y_SCALE = 256.0f, B_SCALE = 4096.0f;
 
-// Per-launch constants, written by colored_prep_kernel:
+// Per-launch constants, written by colored_prep_kernel as one table that the main kernel copies to LDS in one pass
+// (FCp = F C rounded up to 4):
 //   cst[f * C + c]        Y_SCALE * s_c(f) / smax_c (power-law scale; low frequencies take the first admissible one)
-//   cst[CST_INV + c]      smax_c / (sigma_c n Y_SCALE B_SCALE)      (the epilogue's factor)
-//   cst[CST_SD + c]       std_dev_decay^iteration * std_dev[c]
+//   cst[FCp + c]          smax_c / (sigma_c n Y_SCALE B_SCALE)      (the epilogue's factor)
+//   cst[FCp + 4 + c]      std_dev_decay^iteration * std_dev[c]
+//   cst[FCp + 8 + j]      the mean sequence, j < T C
 // and the basis times B_SCALE, FP16 high and low parts: row t < MT * 16, column j < Kpad (2f: real part, 2f + 1:
 // imaginary part); rows >= T and columns >= 2F are zero. The phases f t / n are reduced in integers.
 __global__ void __launch_bounds__(256) colored_prep_kernel(mppi_sampler_params sp, int n, int F, int T, int C,
                                                            int rows, int Kpad, IterRef itr, Sigma4 sig,
-                                                           float* __restrict__ cst, _Float16* __restrict__ bhi,
-                                                           _Float16* __restrict__ blo)
+                                                           const float* __restrict__ mean, float* __restrict__ cst,
+                                                           _Float16* __restrict__ bhi, _Float16* __restrict__ blo)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
+  const int FCp = (F * C + 3) & ~3;
+  if (i < T * C) cst[FCp + 8 + i] = mean[i];
   const IterArgs a = load_iter_c(itr);
   const float cutoff = fmaxf(sp.fmin, 1.0f / n);
   int j0 = 0;
@@ -144,8 +178,8 @@ __global__ void __launch_bounds__(256) colored_prep_kernel(mppi_sampler_params s
     const float smax = fmaxf(powf(lo / (1.0f * n), e), powf((F - 1) / (1.0f * n), e));
     cst[i] = powf(src / (1.0f * n), e) / smax * Y_SCALE;
     if (fi == 0) {
-      cst[CST_INV + c] = smax / (sig.v[c] * n) / (Y_SCALE * B_SCALE);
-      cst[CST_SD + c] = powf(sp.std_dev_decay, (float)a.iteration) * sp.std_dev[c];
+      cst[FCp + c] = smax / (sig.v[c] * n) / (Y_SCALE * B_SCALE);
+      cst[FCp + 4 + c] = powf(sp.std_dev_decay, (float)a.iteration) * sp.std_dev[c];
     }
   }
   const int Fp = Kpad >> 1;
@@ -222,8 +256,7 @@ __device__ __forceinline__ void product_step(const half8& ah, const half8& al, c
 // and applied to all NTT series tiles.
 template <int NTT>
 __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_params sp, int K, int T, int C, int F,
-                                                                int Kpad, int KPH, int MT, int S, int spw,
-                                                                const float* __restrict__ mean,
+                                                                int Kpad, int KPH, int MT, int S, int spw, int TCS,
                                                                 float* __restrict__ out, IterRef itr,
                                                                 const float* __restrict__ spectra_in, int pstart,
                                                                 const float* __restrict__ cst,
@@ -232,12 +265,13 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
 {
   constexpr int NS = NTT * 16;
   extern __shared__ float lds[];
-  float* scale = lds;                                                 // [F][C]
-  float* mean_l = scale + ((F * C + 3) & ~3);                         // [T][C]
-  float* fac = mean_l + ((T * C + 3) & ~3);                           // [8] cst[CST_INV ..]: inv[4], sd[4]
-  _Float16* Xh = reinterpret_cast<_Float16*>(fac + 8);                // [NS][KPH] high parts
+  const int FCp = (F * C + 3) & ~3, NTAB = FCp + 8 + T * C;
+  float* scale = lds;                                                 // [F][C]   the constant table's copy
+  float* fac = lds + FCp;                                             // [8]      inv[4], sd[4]
+  float* mean_l = lds + FCp + 8;                                      // [T][C]
+  _Float16* Xh = reinterpret_cast<_Float16*>(lds + ((NTAB + 3) & ~3));  // [NS][KPH] high parts
   _Float16* Xl = Xh + NS * KPH;                                       // [NS][KPH] low parts
-  float* stage = fac + 8;                                             // [spw][T][C], after the product
+  float* stage = lds + ((NTAB + 3) & ~3);                             // [spw][TCS >= T C], after the product
 
   const IterArgs a = load_iter_c(itr);
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -258,12 +292,9 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   const int kb = active ? (sg * ks) / S : 0, ke = active ? ((sg + 1) * ks) / S : 0;
   const _Float16* gh = bhi + (size_t)(m * 16 + lr) * Kpad + 8 * lk;
   const _Float16* gl = blo + (size_t)(m * 16 + lr) * Kpad + 8 * lk;
-  // the tables first (the LDS stores below wait for them, and the memory counter retires in order), then the basis
-  // fragments; every load is unconditional, at a clamped (in-bounds) index
-  const int FC = F * C;
-  const float sv0 = cst[min(tid, FC - 1)], sv1 = cst[min(tid + CTHREADS, FC - 1)];
-  const float mv = mean[min(tid, TC - 1)];
-  const float fv = cst[CST_INV + min(tid, 7)];
+  // the constant table first, in one pass (the memory counter retires in order, so the waits of these stores must
+  // not cover the basis loads), then the basis fragments
+  for (int i = tid; i < NTAB; i += nthr) lds[i] = cst[i];
   __builtin_amdgcn_sched_barrier(0);  // keeps the table loads ahead of the basis loads
   half8 ph[KPRE], pl[KPRE];
 #pragma unroll
@@ -276,18 +307,12 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
 #pragma unroll
   for (int q = 0; q < NTT; ++q) acc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  // unconditional stores (threads past the end write the last entry's own value again), so that no load is sunk
-  // into a branch behind the basis loads
-  // the first 4-block of normals is generated while the loads are in flight
+  // the first 4-block of normals is generated while the basis loads are in flight
   const int64_t base = (int64_t)k0 * C * TF;
   const int64_t cnt = (int64_t)nser * TF;
   const int64_t gb0 = (base >> 2) + tid;
   Normal4 nz0;
   if (spectra_in == nullptr) nz0 = normals4(sp.seed, a.rng_offset, (uint64_t)gb0);
-  scale[min(tid, FC - 1)] = sv0;
-  scale[min(tid + CTHREADS, FC - 1)] = sv1;
-  mean_l[min(tid, TC - 1)] = mv;
-  fac[min(tid, 7)] = fv;
   __syncthreads();
   // scaled spectra of series s0 .. s0 + nser - 1 (a contiguous span of the [K][C][F][2] normal stream). TF is even
   // and so is base, so every (2j, 2j + 1) pair of a 4-block is one (Re, Im) pair of one series.
@@ -340,50 +365,58 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   }
   __syncthreads();  // the spectra space is reused as the output staging below
 
-  // k-groups summed into the staging buffer in order 0, 1, ..., S - 1; the last one applies the Gaussian transform.
-  // Lane holds OUT[t = 16 m + 4 lk + v][series 16 q + lr] in acc[q][v].
-  for (int g = 0; g < S; ++g) {
-    if (active && sg == g) {
+  // k-groups 1 .. S - 1 hand their partial sums to group 0 through LDS (lane-major 16-byte slots), which adds them in
+  // order and applies the Gaussian transform. Lane holds OUT[t = 16 m + 4 lk + v][series 16 q + lr] in acc[q][v].
+  floatx4* part = reinterpret_cast<floatx4*>(Xl + NS * KPH);  // [S - 1][MT][NTT][64]
+  if (S > 1) {
+    if (active && sg > 0) {
 #pragma unroll
-      for (int q = 0; q < NTT; ++q) {
-        const int sidx = q * 16 + lr;
-        if (sidx < nser) {
-          const int r = sidx / C, c = sidx - r * C, k = k0 + r;
-          const float inv = fac[c], sd = fac[4 + c];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int t = m * 16 + lk * 4 + v;
-            if (t < T) {
-              float* p = stage + r * TC + t * C + c;
-              const float sum = (g > 0) ? *p + acc[q][v] : acc[q][v];
-              if (g == S - 1) {
-                const float eps = sum * inv;
-                const float mu = mean_l[t * C + c];
-                float u;
-                if (k == 0 || t < a.stride) u = mu;
-                else if (k >= pstart) u = sd * eps;
-                else u = mu + sd * eps;
-                *p = u;
-              } else {
-                *p = sum;
-              }
-            }
-          }
-        }
-      }
+      for (int q = 0; q < NTT; ++q) part[(((sg - 1) * MT + m) * NTT + q) * 64 + lane] = acc[q];
     }
     __syncthreads();
   }
+  if (active && sg == 0) {
+    for (int g = 1; g < S; ++g) {
+#pragma unroll
+      for (int q = 0; q < NTT; ++q) acc[q] += part[(((g - 1) * MT + m) * NTT + q) * 64 + lane];
+    }
+    const int cdiv = 65536 / C + 1;  // (x * cdiv) >> 16 == x / C for 0 <= x < 64
+#pragma unroll
+    for (int q = 0; q < NTT; ++q) {
+      const int sidx = q * 16 + lr;
+      const int r = (sidx * cdiv) >> 16, c = sidx - r * C;
+      const float inv = fac[c], sd = fac[4 + c];
+      const bool valid = sidx < nser;
+      float* srow = stage + r * TCS + c;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int t = m * 16 + lk * 4 + v;
+        if (valid && t < T) srow[t * C] = sd * (acc[q][v] * inv);
+      }
+    }
+  }
+  __syncthreads();
+  // the Gaussian transform on the way out, by every wavefront: element e of the span belongs to sample r = e / TC at
+  // rem = e % TC = t C + c, and the staging buffer holds sd_c eps at r TCS + rem
+  const float invTC = 1.0f / TC;  // e / TC by the reciprocal is exact: e < 64 TC, the fraction is >= 1 / (2 TC) from 1
+  const int sc = a.stride * C;
+  auto transform = [&](int e) -> float {
+    const int r = (int)(((float)e + 0.5f) * invTC);
+    const int rem = e - r * TC, k = k0 + r;
+    const float mu = mean_l[rem], v = stage[r * TCS + rem];
+    const float u = (k >= pstart) ? v : mu + v;
+    return (k == 0 || rem < sc) ? mu : u;
+  };
   float* dst = out + (size_t)k0 * TC;
   const int total = nsamp * TC;
   if ((((uintptr_t)dst) & 15) == 0) {
     const int n4 = total >> 2;
     for (int i = tid; i < n4; i += nthr)
       reinterpret_cast<float4*>(dst)[i] =
-          make_float4(stage[4 * i], stage[4 * i + 1], stage[4 * i + 2], stage[4 * i + 3]);
-    for (int e = (n4 << 2) + tid; e < total; e += nthr) dst[e] = stage[e];
+          make_float4(transform(4 * i), transform(4 * i + 1), transform(4 * i + 2), transform(4 * i + 3));
+    for (int e = (n4 << 2) + tid; e < total; e += nthr) dst[e] = transform(e);
   } else {
-    for (int e = tid; e < total; e += nthr) dst[e] = stage[e];
+    for (int e = tid; e < total; e += nthr) dst[e] = transform(e);
   }
 }
 
@@ -434,29 +467,40 @@ static int launch_colored_t(const mppi_sampler_params& sp, int K, int T, int C, 
   }
   const int grid = (K + sh.spw - 1) / sh.spw;
   colored_mfma_kernel<NTT><<<grid, CTHREADS, sh.lds_bytes, s>>>(sp, K, T, C, sh.F, sh.Kpad, sh.KPH, sh.MT, sh.S,
-                                                               sh.spw, mean, samples, it, spectra_in, pstart, w.cst,
+                                                               sh.spw, sh.TCS, samples, it, spectra_in, pstart, w.cst,
                                                                w.bhi, w.blo);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
+}
+
+// colored_shape per (T, C, transform length mode), computed once: the staging-stride search is not free
+static ColoredShape colored_shape_cached(int T, int C, int mode)
+{
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, ColoredShape> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({T, C, mode});
+  if (it == cache.end()) it = cache.emplace(std::make_tuple(T, C, mode), colored_shape(T, C, mode)).first;
+  return it->second;
 }
 
 int launch_sample_colored(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
                           IterRef it, const float* spectra_in, hipStream_t s)
 {
   const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
-  const ColoredShape sh = colored_shape(T, C, sp.fft_len_mode);
+  const ColoredShape sh = colored_shape_cached(T, C, sp.fft_len_mode);
   if (sh.MT > 16) return fail(MPPI_E_SIZE, "colored sampler: horizon exceeds 256");
   if (sh.lds_bytes > LDS_LIMIT || sh.spw < 1)
     return fail(MPPI_E_SIZE, "colored sampler: horizon too long for the LDS staging");
   const int rows = sh.MT * 16;
-  if ((size_t)rows * sh.Kpad > COLORED_BASIS_MAX || sh.F * C > CST_INV)
+  if ((size_t)rows * sh.Kpad > COLORED_BASIS_MAX || sh.F * C > 1028 || T * C > 1024)
     return fail(MPPI_E_SIZE, "colored sampler: basis exceeds its workspace");
   const Sigma4 sig = colored_sigma(sp, C, sh.n);
   ColoredWorkspace w;
   MPPI_CHECK(colored_workspace(s, &w));
-  const int items = std::max(rows * (sh.Kpad >> 1), sh.F * C);
-  colored_prep_kernel<<<(items + 255) / 256, 256, 0, s>>>(sp, sh.n, sh.F, T, C, rows, sh.Kpad, it, sig, w.cst, w.bhi,
-                                                          w.blo);
+  const int items = std::max(rows * (sh.Kpad >> 1), std::max(sh.F * C, T * C));
+  colored_prep_kernel<<<(items + 255) / 256, 256, 0, s>>>(sp, sh.n, sh.F, T, C, rows, sh.Kpad, it, sig, mean, w.cst,
+                                                          w.bhi, w.blo);
   MPPI_HIP_CHECK(hipGetLastError());
   switch (sh.NS / 16) {
     case 4: return launch_colored_t<4>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, w, s);
