@@ -50,7 +50,14 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
   const int g = blockIdx.x;
   const int k0 = g * RC_LANES;
   const int nrows = min(RC_LANES, K - k0);
-  for (int j = tid; j < TC; j += NT) mu[j] = mean[j];
+  // the mean and x0 are requested first (T C <= 1024 <= 2 NT): their loads then retire with the tile's, whose waits
+  // come later; x0 is staged in the local-weight slots, which are free until the rollout has read it
+  const float mv0 = mean[min(tid, TC - 1)], mv1 = mean[min(tid + NT, TC - 1)];
+  const float xv = x0[min(tid, M::S - 1)];
+  float* x0_l = wl;
+  bool deadband = false;
+#pragma unroll
+  for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
   PropLds pv{};
   if (PROP) {
     const float decay_p = powf(sp.std_dev_decay, (float)load_iter_f(itr).iteration);
@@ -68,6 +75,9 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
   if (GEN) {
     // u = transform(eps) for the tile's elements; the tile is a contiguous span starting at element k0*T*C, which
     // is a multiple of 4, so Philox block b covers tile elements 4(b - base) .. 4(b - base) + 3.
+    if (tid < TC) mu[tid] = mv0;
+    if (tid + NT < TC) mu[tid + NT] = mv1;
+    if (tid < M::S) x0_l[tid] = xv;
     __syncthreads();
     const IterArgs a = load_iter_f(itr);
     const float decay = powf(sp.std_dev_decay, (float)a.iteration);
@@ -89,7 +99,7 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
           else if (PROP) v = m + pl[col] + pl[TC4 + col] * z.z[j];
           else if (k >= pstart) v = decay * sp.std_dev[c] * z.z[j];
           else v = m + decay * sp.std_dev[c] * z.z[j];
-          tile[r * ROW + col] = v;
+          tile[r * ROW + col] = constrain_control(dyn, c, v, deadband);
         }
         ++col;
         if (++c == C) {
@@ -105,12 +115,16 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
       }
     }
   } else {
-    tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
+    if (deadband) tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlConstrain<C>{&dyn});
+    else tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlClamp<C>(dyn));
+    if (tid < TC) mu[tid] = mv0;
+    if (tid + NT < TC) mu[tid + NT] = mv1;
+    if (tid < M::S) x0_l[tid] = xv;
   }
   __syncthreads();
   float* cost_l;
   float* raw_l;
-  rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
+  rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0_l, tile, ROW, mu, pstart, k0, scratch, chunk,
                             cost_l, raw_l, PROP ? &pv : nullptr);
   if (tid < RC_LANES) {
     const int k = k0 + tid;

@@ -142,8 +142,15 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_kernel(mppi_
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * RC_LANES;
   const int nrows = min(RC_LANES, K - k0);
-  tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
-  for (int j = tid; j < TC; j += NT) mu[j] = mean[j];
+  // the mean is requested first (T C <= 1024 <= 2 NT): its loads then retire with the tile's
+  const float mv0 = mean[min(tid, TC - 1)], mv1 = mean[min(tid + NT, TC - 1)];
+  bool deadband = false;
+#pragma unroll
+  for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
+  if (deadband) tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlConstrain<C>{&dyn});
+  else tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlClamp<C>(dyn));
+  if (tid < TC) mu[tid] = mv0;
+  if (tid + NT < TC) mu[tid + NT] = mv1;
   __syncthreads();
   float* cost_l;
   float* raw_l;

@@ -57,6 +57,50 @@ MPPI_HD void enforce_constraints(const mppi_dynamics_params& p, float* u)
   }
 }
 
+// enforce_constraints for control i alone (constraints act on each control separately); `deadband` selects the
+// full form (deadband + clamp), as enforce_constraints, or the clamp alone, as clamp_controls, with the same
+// arithmetic, so that whole sample tiles can be constrained before the rollout instead of step by step.
+MPPI_HD inline float constrain_control(const mppi_dynamics_params& p, int i, float v, bool deadband)
+{
+  if (deadband) {
+    if (fabsf(v) < p.control_deadband[i]) v = p.zero_control[i];
+    else v += p.control_deadband[i] * -(v >= 0.0f ? 1.0f : -1.0f);
+  }
+  return fminf(fmaxf(p.control_min[i], v), p.control_max[i]);
+}
+
+// Element transforms for staging a sample tile (f(col, v), col = t C + c): the clamp alone, with the bounds held in
+// registers, and the full constraint (deadband + clamp). The kernels pick one for the whole tile.
+template <int C>
+struct ControlClamp {
+  float lo[C], hi[C];
+  __device__ explicit ControlClamp(const mppi_dynamics_params& p)
+  {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      lo[c] = p.control_min[c];
+      hi[c] = p.control_max[c];
+    }
+  }
+  __device__ float operator()(int col, float v) const
+  {
+    const int c = col % C;
+    float l = lo[0], h = hi[0];
+#pragma unroll
+    for (int k = 1; k < C; ++k) {
+      l = (c == k) ? lo[k] : l;
+      h = (c == k) ? hi[k] : h;
+    }
+    return fminf(fmaxf(l, v), h);
+  }
+};
+
+template <int C>
+struct ControlConstrain {
+  const mppi_dynamics_params* p;
+  __device__ float operator()(int col, float v) const { return constrain_control(*p, col % C, v, true); }
+};
+
 // enforce_constraints without a deadband (all control_deadband == 0): the clamp alone.
 template <int C>
 MPPI_HD void clamp_controls(const mppi_dynamics_params& p, float* u)

@@ -2,10 +2,12 @@
 // u_t from the LDS control row, enforce constraints, write the clamped u_t back, step the dynamics, accumulate the
 // running cost of the post-step output plus the likelihood-ratio cost; add the terminal cost; divide by T.
 //
-// The work is split across the four wavefronts of the workgroup so that the serial chain only carries the dynamics:
-//   producer  (wavefront 0, one sample per lane) -- the state chain: u_t (prefetched one step ahead), constraints,
-//             write-back of the clamped u_t, dynamics step, and the output y_t stored to a double-buffered LDS chunk
-//             ybuf[t - t0][lane][O];
+// Constraints act on each control alone, so the kernels apply them to the whole tile while staging it in LDS
+// (constrain_control, the same arithmetic): the tile holds the constrained controls on entry, which are also the
+// values written back. The work is then split across the wavefronts of the workgroup so that the serial chain only
+// carries the dynamics:
+//   producer  (wavefront 0, one sample per lane) -- the state chain: u_t (read one group of steps ahead), dynamics
+//             step, and the output y_t stored to a double-buffered LDS chunk ybuf[t - t0][lane][O | 1];
 //   consumers (wavefronts 1..15) -- the running + likelihood-ratio cost of every (sample, step) of the previous
 //             chunk: thread (w, lane) takes sample `lane` and the steps t0 + w, t0 + w + 15, ...; the cost-map
 //             gathers of these independent items overlap in flight.
@@ -23,11 +25,12 @@ constexpr int RC_THREADS = 1024;  // producer + 15 consumer wavefronts (all of t
 constexpr int RC_CONSUMERS = RC_THREADS / 64 - 1;
 
 // LDS floats the pipelined rollout needs besides the control tile: the two output chunks, the consumer partial sums
-// (running costs and importance-weight corrections), the terminal costs, the costs and the uncorrected costs.
+// (running costs and importance-weight corrections), the terminal costs, the costs and the uncorrected costs. The
+// output chunks are [step][lane][O | 1]: an odd lane stride keeps a wavefront's accesses off shared LDS banks.
 template <int O>
 inline size_t rollout_scratch_floats(int chunk)
 {
-  return 2 * (size_t)chunk * RC_LANES * O + 2 * (size_t)RC_CONSUMERS * RC_LANES + 3 * RC_LANES;
+  return 2 * (size_t)chunk * RC_LANES * (O | 1) + 2 * (size_t)RC_CONSUMERS * RC_LANES + 3 * RC_LANES;
 }
 
 // Largest chunk (a power of two, 4..32 steps) whose buffers fit in `avail` floats; 0 if none.
@@ -91,6 +94,51 @@ __device__ __forceinline__ float is_term(const PropLds& p, const float* u, const
   return s;
 }
 
+// The producer's steps t0 .. t1 - 1 of one chunk: un holds the (already constrained) controls of the next PG steps,
+// read one group ahead so that the LDS latency passes while the current group is stepped; each step advances the
+// dynamics and stores the output to the chunk buffer yb (lane's row of step t0 at yb). Full groups run without
+// per-step guards; a partial group only occurs at the end of the horizon.
+template <int MODEL, int PG>
+__device__ __forceinline__ void produce_chunk(const mppi_dynamics_params& dyn, float dt, const float* row, float* yb,
+                                              int t0, int t1, float* x, float* y, float (&un)[PG][Model<MODEL>::C])
+{
+  using M = Model<MODEL>;
+  constexpr int S = M::S, C = M::C, O = M::O;
+  auto step = [&](const float* u, int t) {
+    float xd[S], xn[S];
+    M::deriv(dyn, x, u, xd);
+    M::update(dyn, x, xn, xd, dt);
+#pragma unroll
+    for (int i = 0; i < O; ++i) {
+      y[i] = xn[i];
+      yb[(t - t0) * RC_LANES * (O | 1) + i] = xn[i];
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = xn[i];
+  };
+  int tg = t0;
+  for (; tg + PG <= t1; tg += PG) {
+    float ug[PG][C];
+#pragma unroll
+    for (int j = 0; j < PG; ++j)
+#pragma unroll
+      for (int c = 0; c < C; ++c) ug[j][c] = un[j][c];
+    // the next group's controls; past the horizon these read the row's padding and the following LDS words
+    // (inside the tile or the buffers behind it), never used
+#pragma unroll
+    for (int j = 0; j < PG; ++j)
+#pragma unroll
+      for (int c = 0; c < C; ++c) un[j][c] = row[(tg + PG + j) * C + c];
+#pragma unroll
+    for (int j = 0; j < PG; ++j) step(ug[j], tg + j);
+  }
+  if (tg < t1) {
+#pragma unroll
+    for (int j = 0; j < PG; ++j)
+      if (tg + j < t1) step(un[j], tg + j);
+  }
+}
+
 // The same pipeline for MODEL_RACER_DUBINS_NN_BF16 (NN_THREADS threads): four producer wavefronts, wavefront w owning
 // samples 16 w .. 16 w + 15 of the tile (lane 16 g + n, every lane group g of a sample carrying its state), each
 // evaluating the network for its 16 samples per step on the matrix cores; the other wavefronts evaluate the costs.
@@ -107,7 +155,7 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
   constexpr int S = M::S, C = M::C, O = M::O;
   static_assert(O == S, "the bicycle's output is its state");
   float* ybuf = scratch;                                  // [2][chunk][64][O]
-  float* part = ybuf + 2 * chunk * RC_LANES * O;          // [NN_CONSUMERS][64]
+  float* part = ybuf + 2 * chunk * RC_LANES * (O | 1);          // [NN_CONSUMERS][64]
   float* part_is = part + NN_CONSUMERS * RC_LANES;        // [NN_CONSUMERS][64]
   float* term_l = part_is + NN_CONSUMERS * RC_LANES;      // [64]
   float* cost_l = term_l + RC_LANES;                      // [64]
@@ -134,9 +182,6 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
 #pragma unroll
     for (int c = 0; c < C; ++c) un[c] = row[c];
   }
-  bool deadband = false;
-#pragma unroll
-  for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
   float acc = 0.0f, acc_is = 0.0f;
   const bool pure = (k0 + lane) >= pstart;
   const float* crow = tile + lane * ROW;
@@ -144,19 +189,13 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
   for (int it = 0; it <= nch; ++it) {
     if (producer && it < nch) {
       const int t0 = it * chunk, t1 = min(T, t0 + chunk);
-      float* yb = ybuf + (it & 1) * chunk * RC_LANES * O + r * O;
+      float* yb = ybuf + (it & 1) * chunk * RC_LANES * (O | 1) + r * (O | 1);
       for (int t = t0; t < t1; ++t) {
         float u[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) u[c] = un[c];
 #pragma unroll
         for (int c = 0; c < C; ++c) un[c] = row[(t + 1) * C + c];
-        if (deadband) enforce_constraints<C>(dyn, u);
-        else clamp_controls<C>(dyn, u);
-        if (g == 0) {
-#pragma unroll
-          for (int c = 0; c < C; ++c) row[t * C + c] = u[c];
-        }
         float xd[S], xn[S];
         M::deriv(dyn, x, u, xd);
         const NnOut o = nn_eval_bf16(F, x, u, g);
@@ -165,7 +204,7 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
         M::update(dyn, x, xn, xd, dt);
         if (g == 0) {
 #pragma unroll
-          for (int i = 0; i < O; ++i) yb[(t - t0) * RC_LANES * O + i] = xn[i];
+          for (int i = 0; i < O; ++i) yb[(t - t0) * RC_LANES * (O | 1) + i] = xn[i];
         }
 #pragma unroll
         for (int i = 0; i < S; ++i) x[i] = xn[i];
@@ -173,9 +212,9 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
       if (it == nch - 1 && g == 0) term_l[r] = CF::terminal(cost, x);
     } else if (!producer && it > 0) {
       const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
-      const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * O + lane * O;
+      const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * (O | 1) + lane * (O | 1);
       for (int t = t0 + wave - NN_PRODUCERS; t < t1; t += NN_CONSUMERS) {
-        acc += CF::running(cost, yb + (t - t0) * RC_LANES * O) +
+        acc += CF::running(cost, yb + (t - t0) * RC_LANES * (O | 1)) +
                lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
         if (prop) acc_is += is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
       }
@@ -223,7 +262,7 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
   using CF = Cost<COST, M::O>;
   constexpr int S = M::S, C = M::C, O = M::O;
   float* ybuf = scratch;                                  // [2][chunk][64][O]
-  float* part = ybuf + 2 * chunk * RC_LANES * O;          // [15][64]
+  float* part = ybuf + 2 * chunk * RC_LANES * (O | 1);          // [15][64]
   float* part_is = part + RC_CONSUMERS * RC_LANES;        // [15][64]
   float* term_l = part_is + RC_CONSUMERS * RC_LANES;      // [64]
   float* cost_l = term_l + RC_LANES;                      // [64]
@@ -232,8 +271,9 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
   const int wave = tid >> 6, lane = tid & 63;
   const int nch = (T + chunk - 1) / chunk;
 
-  // producer state
-  float x[S], y[O], un[C];
+  // producer state; un holds the controls of the next group of PG steps (chunks are multiples of PG)
+  constexpr int PG = 4;
+  float x[S], y[O], un[PG][C];
   float* row = tile + lane * ROW;
   if (wave == 0) {
 #pragma unroll
@@ -241,11 +281,10 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
 #pragma unroll
     for (int i = 0; i < O; ++i) y[i] = x[i];
 #pragma unroll
-    for (int c = 0; c < C; ++c) un[c] = row[c];
-  }
-  bool deadband = false;
+    for (int j = 0; j < PG; ++j)
 #pragma unroll
-  for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
+      for (int c = 0; c < C; ++c) un[j][c] = row[j * C + c];
+  }
   // consumer state
   float acc = 0.0f, acc_is = 0.0f;
   const bool pure = (k0 + lane) >= pstart;
@@ -254,36 +293,15 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
   for (int it = 0; it <= nch; ++it) {
     if (wave == 0 && it < nch) {
       const int t0 = it * chunk, t1 = min(T, t0 + chunk);
-      float* yb = ybuf + (it & 1) * chunk * RC_LANES * O + lane * O;
-      for (int t = t0; t < t1; ++t) {
-        float u[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) u[c] = un[c];
-        // prefetch u_{t+1}; at t = T-1 this reads the row's padding / the next LDS words, never used
-#pragma unroll
-        for (int c = 0; c < C; ++c) un[c] = row[(t + 1) * C + c];
-        if (deadband) enforce_constraints<C>(dyn, u);
-        else clamp_controls<C>(dyn, u);
-#pragma unroll
-        for (int c = 0; c < C; ++c) row[t * C + c] = u[c];
-        float xd[S], xn[S];
-        M::deriv(dyn, x, u, xd);
-        M::update(dyn, x, xn, xd, dt);
-#pragma unroll
-        for (int i = 0; i < O; ++i) {
-          y[i] = xn[i];
-          yb[(t - t0) * RC_LANES * O + i] = xn[i];
-        }
-#pragma unroll
-        for (int i = 0; i < S; ++i) x[i] = xn[i];
-      }
+      float* yb = ybuf + (it & 1) * chunk * RC_LANES * (O | 1) + lane * (O | 1);
+      produce_chunk<MODEL, PG>(dyn, dt, row, yb, t0, t1, x, y, un);
       if (it == nch - 1) term_l[lane] = CF::terminal(cost, y);
     } else if (wave > 0 && it > 0) {
       const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
-      const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * O + lane * O;
+      const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * (O | 1) + lane * (O | 1);
       for (int t = t0 + wave - 1; t < t1; t += RC_CONSUMERS) {
         // y and u are read in place from LDS (runtime-indexed cost terms become LDS addresses)
-        acc += CF::running(cost, yb + (t - t0) * RC_LANES * O) +
+        acc += CF::running(cost, yb + (t - t0) * RC_LANES * (O | 1)) +
                lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
         if (prop) acc_is += is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
       }

@@ -9,9 +9,14 @@ namespace mppi_amd {
 
 constexpr int TILE_BATCH = 16;
 
-template <int NT>
+struct TileIdentity {
+  __device__ float operator()(int, float v) const { return v; }
+};
+
+// f(col, v) is applied to every element on its way into LDS (col = t C + c within the row).
+template <int NT, typename F = TileIdentity>
 __device__ __forceinline__ void tile_load(const float* __restrict__ src, float* __restrict__ tile, int nrows, int TC,
-                                          int ROW, int tid)
+                                          int ROW, int tid, F f = F())
 {
   const int n = nrows * TC;
   if ((((uintptr_t)src) & 15) == 0) {
@@ -32,7 +37,7 @@ __device__ __forceinline__ void tile_load(const float* __restrict__ src, float* 
           const float vv[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            tile[r * ROW + col] = vv[q];
+            tile[r * ROW + col] = f(col, vv[q]);
             if (++col == TC) {
               col = 0;
               ++r;
@@ -43,12 +48,12 @@ __device__ __forceinline__ void tile_load(const float* __restrict__ src, float* 
     }
     for (int e = (n4 << 2) + tid; e < n; e += NT) {
       const int r = e / TC;
-      tile[r * ROW + (e - r * TC)] = src[e];
+      tile[r * ROW + (e - r * TC)] = f(e - r * TC, src[e]);
     }
   } else {
     for (int e = tid; e < n; e += NT) {
       const int r = e / TC;
-      tile[r * ROW + (e - r * TC)] = src[e];
+      tile[r * ROW + (e - r * TC)] = f(e - r * TC, src[e]);
     }
   }
 }
