@@ -94,9 +94,6 @@ static int staging_conflict(int TCS, int C, int NTT)
   return worst;
 }
 
-// launch constants, one contiguous table (F C <= 257 * 4 = 1028, T C <= 1024): see colored_prep_kernel
-constexpr int CONST_FLOATS = 1028 + 8 + 1024;
-
 static ColoredShape colored_shape(int T, int C, int mode)
 {
   ColoredShape sh;
@@ -109,7 +106,9 @@ static ColoredShape colored_shape(int T, int C, int mode)
   for (int ns = 64; ns >= 16; ns /= 2) {
     sh.NS = ns;
     sh.spw = ns / C;
-    const size_t tables = ((((size_t)sh.F * C + 3) & ~(size_t)3) + 8 + (size_t)T * C + 3) & ~(size_t)3;
+    // scale table, factors, mean, decay powers, x[stride] per series, the stride's basis row
+    const size_t tables = ((((size_t)sh.F * C + 3) & ~(size_t)3) + 8 + (size_t)T * C + T + 64 + sh.Kpad + 3) &
+                          ~(size_t)3;
     // high and low spectra; the output staging [spw][T][C] reuses the same space (KPH > T)
     const size_t xh = 2 * (size_t)ns * sh.KPH * sizeof(_Float16);
     const size_t part = (size_t)(sh.S - 1) * sh.MT * (ns / 16) * 64 * 16;  // k-group partial sums
@@ -144,61 +143,40 @@ __device__ __forceinline__ void split_f16(float v, _Float16& hi, _Float16& lo)
   lo = (_Float16)(v - (float)hi);
 }
 
-// Power-of-two scales of the FP16 operands (exact): the spectra |Y| <= 6-ish and the basis |B| <= 4 are multiplied by
+// Power-of-two scales of the FP16 operands (exact): the spectra |Y| <= 6-ish and the basis |B| <= 2 are multiplied by
 // 2^8 and 2^12, so that the low part (about 2^-11 of the value) stays a normal FP16 number down to |Y| ~ 5e-4 and
 // |B| ~ 3e-5, and the largest products stay far inside the FP32 range. Fed spectra must stay below 255 in magnitude.
 constexpr float Y_SCALE = 256.0f, B_SCALE = 4096.0f;
 
-// Per-launch constants, written by colored_prep_kernel as one table that the main kernel copies to LDS in one pass
-// (FCp = F C rounded up to 4):
-//   cst[f * C + c]        Y_SCALE * s_c(f) / smax_c (power-law scale; low frequencies take the first admissible one)
-//   cst[FCp + c]          smax_c / (sigma_c n Y_SCALE B_SCALE)      (the epilogue's factor)
-//   cst[FCp + 4 + c]      std_dev_decay^iteration * std_dev[c]
-//   cst[FCp + 8 + j]      the mean sequence, j < T C
-// and the basis times B_SCALE, FP16 high and low parts: row t < MT * 16, column j < Kpad (2f: real part, 2f + 1:
-// imaginary part); rows >= T and columns >= 2F are zero. The phases f t / n are reduced in integers.
-__global__ void __launch_bounds__(256) colored_prep_kernel(mppi_sampler_params sp, int n, int F, int T, int C,
-                                                           int rows, int Kpad, IterRef itr, Sigma4 sig,
-                                                           const float* __restrict__ mean, float* __restrict__ cst,
-                                                           _Float16* __restrict__ bhi, _Float16* __restrict__ blo)
+// Column pair f of the inverse real transform's row t: w_f cos(2 pi f t / n) and -w_f sin(2 pi f t / n), the phase
+// f t reduced modulo n in integers; zero where the transform ignores the term (f >= F, the imaginary parts at DC and
+// Nyquist and the last bin that configureFrequencyNoise zeroes when F is odd).
+__device__ __forceinline__ void basis_pair(int n, int F, int t, int f, float& re, float& im)
 {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int FCp = (F * C + 3) & ~3;
-  if (i < T * C) cst[FCp + 8 + i] = mean[i];
-  const IterArgs a = load_iter_c(itr);
-  const float cutoff = fmaxf(sp.fmin, 1.0f / n);
-  int j0 = 0;
-  while (j0 < F && (j0 / (1.0f * n)) < cutoff) ++j0;
-  if (i < F * C) {
-    const int fi = i / C, c = i - fi * C;
-    const int src = (fi < j0 && j0 < F) ? j0 : fi;
-    // f^(-beta/2) is monotone in f: the largest scale is at the lowest or the highest admissible frequency
-    const int lo = (j0 < F) ? j0 : 0;
-    const float e = -sp.exponents[c] / 2.0f;
-    const float smax = fmaxf(powf(lo / (1.0f * n), e), powf((F - 1) / (1.0f * n), e));
-    cst[i] = powf(src / (1.0f * n), e) / smax * Y_SCALE;
-    if (fi == 0) {
-      cst[FCp + c] = smax / (sig.v[c] * n) / (Y_SCALE * B_SCALE);
-      cst[FCp + 4 + c] = powf(sp.std_dev_decay, (float)a.iteration) * sp.std_dev[c];
-    }
-  }
+  re = 0.0f;
+  im = 0.0f;
+  if (f >= F) return;
+  const int m = (int)(((long long)f * t) % n);
+  float s, c;
+  sincospif((float)(2 * m) / (float)n, &s, &c);
+  const bool edge = (f == 0) || (2 * f == n);
+  const float w = edge ? 1.0f : 2.0f;
+  re = w * c;
+  const bool zero_im = edge || ((F & 1) && f == F - 1);
+  im = zero_im ? 0.0f : -w * s;
+}
+
+// The transform's basis for rows t < T (rows T .. rows - 1 zero) times B_SCALE, FP16 high and low parts, columns
+// j < Kpad (2f: real part, 2f + 1: imaginary part). It depends on n and T only and is built once per shape.
+__global__ void __launch_bounds__(256) colored_basis_kernel(int n, int F, int T, int rows, int Kpad,
+                                                            _Float16* __restrict__ bhi, _Float16* __restrict__ blo)
+{
   const int Fp = Kpad >> 1;
+  const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= rows * Fp) return;
   const int t = i / Fp, f = i - t * Fp;
   float re = 0.0f, im = 0.0f;
-  if (t < T && f < F) {
-    const int stride = min(max(a.stride, 0), n - 1);
-    const float dt = (sp.offset_decay_rate == 0.0f) ? 0.0f : powf(sp.offset_decay_rate, (float)t);
-    const int mt = (int)(((long long)f * t) % n), ms = (int)(((long long)f * stride) % n);
-    float s_t, c_t, s_s, c_s;
-    sincospif((float)(2 * mt) / (float)n, &s_t, &c_t);
-    sincospif((float)(2 * ms) / (float)n, &s_s, &c_s);
-    const bool edge = (f == 0) || (2 * f == n);
-    const float w = edge ? 1.0f : 2.0f;
-    re = w * (c_t - dt * c_s);
-    const bool zero_im = edge || ((F & 1) && f == F - 1);
-    im = zero_im ? 0.0f : -w * (s_t - dt * s_s);
-  }
+  if (t < T) basis_pair(n, F, t, f, re, im);
   _Float16 h0, l0, h1, l1;
   split_f16(re * B_SCALE, h0, l0);
   split_f16(im * B_SCALE, h1, l1);
@@ -251,27 +229,35 @@ __device__ __forceinline__ void product_step(const half8& ah, const half8& al, c
 
 // Per workgroup: the scaled spectra of its series (every scale divided by the component's largest, so the values stay
 // far inside the FP16 range; the factor returns in the epilogue), staged in LDS as FP16 high and low parts; then
-// OUT = B Y on v_mfma_f32_16x16x32_f16 as Bh Yh + Bh Yl + Bl Yh (the dropped Bl Yl term is about 2^-22 relative),
-// accumulated in FP32. Each basis fragment is read from the launch's basis once per workgroup (one k-step ahead)
-// and applied to all NTT series tiles.
+// X = B Y on v_mfma_f32_16x16x32_f16 as Bh Yh + Bh Yl + Bl Yh (the dropped Bl Yl term is about 2^-22 relative),
+// accumulated in FP32, with B the stride-free basis. Each basis fragment is read once per workgroup (the first KPRE
+// k-steps' fragments before the spectra are generated) and applied to all NTT series tiles. The epilogue forms
+// x[t] - decay^t x[stride]: for stride < T, x[stride] is the product's own row (at t = stride the difference then
+// carries no rounding of its own beyond one FMA); for stride >= T it is a dot product of the staged spectra with
+// that row of the basis, done in FP32.
 template <int NTT>
-__global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_params sp, int K, int T, int C, int F,
-                                                                int Kpad, int KPH, int MT, int S, int spw, int TCS,
+__global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_params sp, int K, int T, int C, int n,
+                                                                int F, int Kpad, int KPH, int MT, int S, int spw,
+                                                                int TCS, const float* __restrict__ mean,
                                                                 float* __restrict__ out, IterRef itr,
                                                                 const float* __restrict__ spectra_in, int pstart,
-                                                                const float* __restrict__ cst,
-                                                                const _Float16* __restrict__ bhi,
+                                                                Sigma4 sig, const _Float16* __restrict__ bhi,
                                                                 const _Float16* __restrict__ blo)
 {
   constexpr int NS = NTT * 16;
   extern __shared__ float lds[];
-  const int FCp = (F * C + 3) & ~3, NTAB = FCp + 8 + T * C;
-  float* scale = lds;                                                 // [F][C]   the constant table's copy
-  float* fac = lds + FCp;                                             // [8]      inv[4], sd[4]
-  float* mean_l = lds + FCp + 8;                                      // [T][C]
-  _Float16* Xh = reinterpret_cast<_Float16*>(lds + ((NTAB + 3) & ~3));  // [NS][KPH] high parts
-  _Float16* Xl = Xh + NS * KPH;                                       // [NS][KPH] low parts
-  float* stage = lds + ((NTAB + 3) & ~3);                             // [spw][TCS >= T C], after the product
+  const int TC = T * C;
+  const int FCp = (F * C + 3) & ~3;
+  float* scale = lds;                  // [F][C]  Y_SCALE s_c(f) / smax_c
+  float* fac = scale + FCp;            // [8]     inv[4] = smax_c / (sigma_c n Y_SCALE B_SCALE), sd[4]
+  float* mean_l = fac + 8;             // [T][C]
+  float* dpow = mean_l + TC;           // [T]     decay^t (0 for a zero decay rate)
+  float* xs = dpow + T;                // [64]    x[stride] per series, in the product's units
+  float* bs = xs + 64;                 // [Kpad]  the basis row of the stride, used when stride >= T
+  const int NTAB = (FCp + 8 + TC + T + 64 + Kpad + 3) & ~3;
+  _Float16* Xh = reinterpret_cast<_Float16*>(lds + NTAB);  // [NS][KPH] high parts
+  _Float16* Xl = Xh + NS * KPH;                            // [NS][KPH] low parts
+  float* stage = lds + NTAB;                               // [spw][TCS], after the product
 
   const IterArgs a = load_iter_c(itr);
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -279,11 +265,11 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   const int nsamp = min(spw, K - k0);
   const int nser = nsamp * C;
   const int TF = 2 * F;
-  const int TC = T * C;
+  const int stride = min(max(a.stride, 0), n - 1);
+  const bool decay = sp.offset_decay_rate != 0.0f;
 
   // wavefront w: row tile m = w % MT against all series tiles over the k-steps [kb, ke) of k-group w / MT; lane
-  // (lr, lk) supplies row lr and k-columns 8 lk .. 8 lk + 7 of each 32-wide k-step. The basis fragments of the
-  // first KPRE k-steps are requested here, so that their latency passes while the spectra are generated.
+  // (lr, lk) supplies row lr and k-columns 8 lk .. 8 lk + 7 of each 32-wide k-step
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int lr = lane & 15, lk = lane >> 4;
   const int m = wave % MT, sg = wave / MT;
@@ -292,10 +278,11 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   const int kb = active ? (sg * ks) / S : 0, ke = active ? ((sg + 1) * ks) / S : 0;
   const _Float16* gh = bhi + (size_t)(m * 16 + lr) * Kpad + 8 * lk;
   const _Float16* gl = blo + (size_t)(m * 16 + lr) * Kpad + 8 * lk;
-  // the constant table first, in one pass (the memory counter retires in order, so the waits of these stores must
-  // not cover the basis loads), then the basis fragments
-  for (int i = tid; i < NTAB; i += nthr) lds[i] = cst[i];
-  __builtin_amdgcn_sched_barrier(0);  // keeps the table loads ahead of the basis loads
+  // the mean first (T C <= 1024 threads; the memory counter retires in order, so its wait below does not cover the
+  // basis loads), then the basis fragments of the first KPRE k-steps, whose latency passes while the tables and the
+  // spectra are computed
+  const float mv = mean[min(tid, TC - 1)];
+  __builtin_amdgcn_sched_barrier(0);
   half8 ph[KPRE], pl[KPRE];
 #pragma unroll
   for (int i = 0; i < KPRE; ++i) {
@@ -306,6 +293,39 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   floatx4 acc[NTT];
 #pragma unroll
   for (int q = 0; q < NTT; ++q) acc[q] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  // tables: power-law scales (low frequencies take the first admissible one; f^(-beta/2) is monotone in f, so the
+  // largest scale is at the lowest or the highest admissible frequency), epilogue factors, decay powers
+  {
+    const float cutoff = fmaxf(sp.fmin, 1.0f / n);
+    int j0 = 0;
+    while (j0 < F && (j0 / (1.0f * n)) < cutoff) ++j0;
+    const int lo = (j0 < F) ? j0 : 0;
+    for (int i = tid; i < F * C; i += nthr) {
+      const int fi = i / C, c = i - fi * C;
+      const int src = (fi < j0 && j0 < F) ? j0 : fi;
+      const float e = -sp.exponents[c] / 2.0f;
+      const float smax = fmaxf(powf(lo / (1.0f * n), e), powf((F - 1) / (1.0f * n), e));
+      scale[i] = powf(src / (1.0f * n), e) / smax * Y_SCALE;
+      if (fi == 0) {
+        fac[c] = smax / (sig.v[c] * n) / (Y_SCALE * B_SCALE);
+        fac[4 + c] = powf(sp.std_dev_decay, (float)a.iteration) * sp.std_dev[c];
+      }
+    }
+    for (int t = tid; t < T; t += nthr) dpow[t] = decay ? powf(sp.offset_decay_rate, (float)t) : 0.0f;
+    if (decay && stride >= T) {
+      for (int f = tid; 2 * f < Kpad; f += nthr) {
+        float re, im;
+        basis_pair(n, F, stride, f, re, im);
+        bs[2 * f] = re * B_SCALE;
+        bs[2 * f + 1] = im * B_SCALE;
+      }
+    }
+    if (!decay && tid < 64) xs[tid] = 0.0f;
+    // unconditional store (threads past the end write the last entry's own value again), so that the load is not
+    // sunk behind the basis loads
+    mean_l[min(tid, TC - 1)] = mv;
+  }
 
   // the first 4-block of normals is generated while the basis loads are in flight
   const int64_t base = (int64_t)k0 * C * TF;
@@ -352,6 +372,19 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   }
   __syncthreads();
 
+  // x[stride] for stride >= T: 16 lanes per series, each over the columns j = part (mod 16), summed in a fixed order
+  if (decay && stride >= T) {
+    for (int sidx = tid >> 4; sidx < NS; sidx += nthr >> 4) {
+      const int part = tid & 15;
+      float v = 0.0f;
+      for (int j = part; j < Kpad; j += 16)
+        v += bs[j] * ((float)Xh[sidx * KPH + j] + (float)Xl[sidx * KPH + j]);
+#pragma unroll
+      for (int o = 8; o >= 1; o >>= 1) v += __shfl_xor(v, o, 16);
+      if (part == 0) xs[sidx] = v;
+    }
+  }
+
   // product: the k-steps beyond the prefetched ones stream their basis fragments from memory
   if (active) {
     const _Float16* xh = Xh + lr * KPH + 8 * lk;
@@ -366,7 +399,7 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   __syncthreads();  // the spectra space is reused as the output staging below
 
   // k-groups 1 .. S - 1 hand their partial sums to group 0 through LDS (lane-major 16-byte slots), which adds them in
-  // order and applies the Gaussian transform. Lane holds OUT[t = 16 m + 4 lk + v][series 16 q + lr] in acc[q][v].
+  // order. Lane holds X[t = 16 m + 4 lk + v][series 16 q + lr] in acc[q][v].
   floatx4* part = reinterpret_cast<floatx4*>(Xl + NS * KPH);  // [S - 1][MT][NTT][64]
   if (S > 1) {
     if (active && sg > 0) {
@@ -380,18 +413,28 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
 #pragma unroll
       for (int q = 0; q < NTT; ++q) acc[q] += part[(((g - 1) * MT + m) * NTT + q) * 64 + lane];
     }
+  }
+  // x[stride] for stride < T: the row's values, from the wavefront and lanes that hold it
+  if (decay && stride < T) {
+    if (active && sg == 0 && m == (stride >> 4) && lk == ((stride & 15) >> 2)) {
+#pragma unroll
+      for (int q = 0; q < NTT; ++q) xs[q * 16 + lr] = acc[q][stride & 3];
+    }
+    __syncthreads();
+  }
+  if (active && sg == 0) {
     const int cdiv = 65536 / C + 1;  // (x * cdiv) >> 16 == x / C for 0 <= x < 64
 #pragma unroll
     for (int q = 0; q < NTT; ++q) {
       const int sidx = q * 16 + lr;
       const int r = (sidx * cdiv) >> 16, c = sidx - r * C;
-      const float inv = fac[c], sd = fac[4 + c];
+      const float inv = fac[c], sd = fac[4 + c], xv = xs[sidx];
       const bool valid = sidx < nser;
       float* srow = stage + r * TCS + c;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int t = m * 16 + lk * 4 + v;
-        if (valid && t < T) srow[t * C] = sd * (acc[q][v] * inv);
+        if (valid && t < T) srow[t * C] = sd * ((acc[q][v] - dpow[t] * xv) * inv);
       }
     }
   }
@@ -420,44 +463,45 @@ __global__ void __launch_bounds__(CTHREADS) colored_mfma_kernel(mppi_sampler_par
   }
 }
 
-// Workspace of the launch constants and the basis: one per (device, stream), sized for the largest accepted shape
-// (T <= 256: 256 rows; n <= 512: 544 columns), allocated on the first launch on that stream and kept until the
-// process ends. Launches on one stream are ordered, so each reads the constants it wrote; a graph captured on a
-// stream keeps a valid pointer. The first launch on a stream must not be inside a graph capture (the controllers
-// run their work once eagerly before capturing it).
-constexpr size_t COLORED_BASIS_MAX = (size_t)256 * 544;
-
-struct ColoredWorkspace {
-  float* cst;
-  _Float16* bhi;
-  _Float16* blo;
+// The basis of one shape (n, T): built once per device on first use and kept until the process ends, read-only
+// afterwards (so any stream may use it, and a graph captured with it keeps a valid pointer). The first use of a
+// shape on a device must not be inside a graph capture (the controllers run their work once eagerly first).
+struct ColoredBasis {
+  _Float16* hi;
+  _Float16* lo;
 };
 
-static int colored_workspace(hipStream_t s, ColoredWorkspace* w)
+static int colored_basis(int n, int T, int F, int rows, int Kpad, hipStream_t s, ColoredBasis* out)
 {
   int dev = 0;
   MPPI_HIP_CHECK(hipGetDevice(&dev));
   static std::mutex mu;
-  static std::map<std::pair<int, hipStream_t>, void*> ws;
+  static std::map<std::tuple<int, int, int>, ColoredBasis> cache;
   std::lock_guard<std::mutex> lock(mu);
-  void*& p = ws[{dev, s}];
-  if (!p) {
+  auto it = cache.find({dev, n, T});
+  if (it == cache.end()) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     MPPI_HIP_CHECK(hipStreamIsCapturing(s, &cs));
     if (cs != hipStreamCaptureStatusNone)
-      return fail(MPPI_E_INVALID, "colored sampler: the first launch on a stream cannot be graph-captured");
-    MPPI_HIP_CHECK(hipMalloc(&p, CONST_FLOATS * sizeof(float) + 2 * COLORED_BASIS_MAX * sizeof(_Float16)));
+      return fail(MPPI_E_INVALID, "colored sampler: the first launch of a horizon cannot be graph-captured");
+    const size_t nb = (size_t)rows * Kpad;
+    void* p = nullptr;
+    MPPI_HIP_CHECK(hipMalloc(&p, 2 * nb * sizeof(_Float16)));
+    ColoredBasis b{static_cast<_Float16*>(p), static_cast<_Float16*>(p) + nb};
+    const int items = rows * (Kpad >> 1);
+    colored_basis_kernel<<<(items + 255) / 256, 256, 0, s>>>(n, F, T, rows, Kpad, b.hi, b.lo);
+    MPPI_HIP_CHECK(hipGetLastError());
+    MPPI_HIP_CHECK(hipStreamSynchronize(s));  // complete before any stream reads it
+    it = cache.emplace(std::make_tuple(dev, n, T), b).first;
   }
-  w->cst = static_cast<float*>(p);
-  w->bhi = reinterpret_cast<_Float16*>(w->cst + CONST_FLOATS);
-  w->blo = w->bhi + COLORED_BASIS_MAX;
+  *out = it->second;
   return MPPI_OK;
 }
 
 template <int NTT>
 static int launch_colored_t(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
-                            IterRef it, const float* spectra_in, int pstart, const ColoredShape& sh,
-                            const ColoredWorkspace& w, hipStream_t s)
+                            IterRef it, const float* spectra_in, int pstart, const ColoredShape& sh, Sigma4 sig,
+                            const ColoredBasis& b, hipStream_t s)
 {
   static bool attr_set = false;
   if (!attr_set) {
@@ -466,9 +510,9 @@ static int launch_colored_t(const mppi_sampler_params& sp, int K, int T, int C, 
     attr_set = true;
   }
   const int grid = (K + sh.spw - 1) / sh.spw;
-  colored_mfma_kernel<NTT><<<grid, CTHREADS, sh.lds_bytes, s>>>(sp, K, T, C, sh.F, sh.Kpad, sh.KPH, sh.MT, sh.S,
-                                                               sh.spw, sh.TCS, samples, it, spectra_in, pstart, w.cst,
-                                                               w.bhi, w.blo);
+  colored_mfma_kernel<NTT><<<grid, CTHREADS, sh.lds_bytes, s>>>(sp, K, T, C, sh.n, sh.F, sh.Kpad, sh.KPH, sh.MT,
+                                                               sh.S, sh.spw, sh.TCS, mean, samples, it, spectra_in,
+                                                               pstart, sig, b.hi, b.lo);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
@@ -492,20 +536,13 @@ int launch_sample_colored(const mppi_sampler_params& sp, int K, int T, int C, co
   if (sh.MT > 16) return fail(MPPI_E_SIZE, "colored sampler: horizon exceeds 256");
   if (sh.lds_bytes > LDS_LIMIT || sh.spw < 1)
     return fail(MPPI_E_SIZE, "colored sampler: horizon too long for the LDS staging");
-  const int rows = sh.MT * 16;
-  if ((size_t)rows * sh.Kpad > COLORED_BASIS_MAX || sh.F * C > 1028 || T * C > 1024)
-    return fail(MPPI_E_SIZE, "colored sampler: basis exceeds its workspace");
   const Sigma4 sig = colored_sigma(sp, C, sh.n);
-  ColoredWorkspace w;
-  MPPI_CHECK(colored_workspace(s, &w));
-  const int items = std::max(rows * (sh.Kpad >> 1), std::max(sh.F * C, T * C));
-  colored_prep_kernel<<<(items + 255) / 256, 256, 0, s>>>(sp, sh.n, sh.F, T, C, rows, sh.Kpad, it, sig, mean, w.cst,
-                                                          w.bhi, w.blo);
-  MPPI_HIP_CHECK(hipGetLastError());
+  ColoredBasis b;
+  MPPI_CHECK(colored_basis(sh.n, T, sh.F, sh.MT * 16, sh.Kpad, s, &b));
   switch (sh.NS / 16) {
-    case 4: return launch_colored_t<4>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, w, s);
-    case 2: return launch_colored_t<2>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, w, s);
-    default: return launch_colored_t<1>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, w, s);
+    case 4: return launch_colored_t<4>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, sig, b, s);
+    case 2: return launch_colored_t<2>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, sig, b, s);
+    default: return launch_colored_t<1>(sp, K, T, C, mean, samples, it, spectra_in, pstart, sh, sig, b, s);
   }
 }
 
