@@ -5,17 +5,16 @@
 // computes, with scales s_c(f) = f^(-beta_c/2) (low frequencies clamped to the cutoff, :300-326), the unnormalised
 // inverse real FFT x = C2R(s_c X_s) of length n = 2T (or T), and
 //     eps[k][t][c] = (x[t] - x[stride] * decay^t) / (sigma_c * n),      t < T                      (:39-56)
-// which it then feeds through the Gaussian transform. Here the whole map from the scaled spectrum to eps is one
-// linear operator per workgroup:
-//     OUT[t][s] = sum_j B[t][j] * Y[j][s],   Y[2f][s] = s_c(f) Re X_s[f],  Y[2f+1][s] = s_c(f) Im X_s[f]
-//     B[t][2f]   =  w_f (cos(2 pi f t/n) - decay^t cos(2 pi f stride/n))
-//     B[t][2f+1] = -w_f (sin(2 pi f t/n) - decay^t sin(2 pi f stride/n)),   w_f = 1 at f = 0 and f = n/2, else 2
+// which it then feeds through the Gaussian transform. Here the inverse transform of a workgroup's series is one
+// matrix product
+//     X[t][s] = sum_j B[t][j] * Y[j][s],   Y[2f][s] = s_c(f) Re X_s[f],  Y[2f+1][s] = s_c(f) Im X_s[f]
+//     B[t][2f] = w_f cos(2 pi f t/n),  B[t][2f+1] = -w_f sin(2 pi f t/n),   w_f = 1 at f = 0 and f = n/2, else 2
 // (the imaginary parts the C2R transform ignores -- DC and Nyquist -- and those configureFrequencyNoise zeroes
-// (:26-33) have B = 0). The basis is built once per launch (colored_basis_kernel) as FP16 high and low parts; every
-// workgroup of 16 wavefronts owns up to 64 series and stages their scaled spectra in LDS as FP16 high and low parts;
-// wavefront w accumulates its 16 x 16 output tiles (M-tile mg + i*MG, N-tile nt) with v_mfma_f32_16x16x32_f16 as
-// Bh Yh + Bh Yl + Bl Yh. The epilogue divides by sigma_c * n, applies the Gaussian transform and writes the
-// workgroup's contiguous span of the [K][T][C] sample buffer with 16-byte stores.
+// (:26-33) have B = 0). B depends on n and T only: it is built once per shape (colored_basis_kernel) as FP16 high and
+// low parts. Every workgroup of 16 wavefronts owns up to 64 series, stages their scaled spectra in LDS as FP16 high and
+// low parts and forms the product on v_mfma_f32_16x16x32_f16 as Bh Yh + Bh Yl + Bl Yh; the epilogue subtracts
+// decay^t x[stride], divides by sigma_c * n, applies the Gaussian transform and writes the workgroup's contiguous
+// span of the [K][T][C] sample buffer with 16-byte stores.
 #include <algorithm>
 #include <map>
 #include <mutex>
