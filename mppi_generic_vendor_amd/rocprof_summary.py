@@ -14,12 +14,20 @@ from __future__ import annotations
 import argparse
 import json
 import sqlite3
+import subprocess
 from collections import defaultdict
 
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
 def short_name(n: str) -> str:
+    """Kernel name without return type, namespace and parameter list; mangled names (some traces keep them) are
+    demangled with c++filt first when it is available."""
+    if n.startswith("_Z"):
+        try:
+            n = subprocess.run(["c++filt", "-p", n], capture_output=True, text=True, check=True).stdout.strip() or n
+        except (OSError, subprocess.CalledProcessError):
+            pass
     return n.split("(")[0].replace("void ", "").replace("mppi_amd::", "")
 
 
