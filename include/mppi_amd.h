@@ -235,7 +235,9 @@ int mppi_sample_gaussian(const mppi_sampler_params* sampler, int32_t num_rollout
  * (sampling_distributions/colored_noise/colored_noise.cu:286-392): complex Gaussian spectra scaled by f^(-beta/2),
  * inverse real DFT of length n (2T or T), first T outputs, offset-decay subtraction x[t] - x[stride]*decay^t,
  * division by sigma_c * n, then the Gaussian transform above. If spectra_in_d is non-NULL it holds the unscaled
- * complex spectra [K][C][n/2+1][2] used instead of the generator. */
+ * complex spectra [K][C][n/2+1][2] used instead of the generator (values of magnitude below 255: they are staged in
+ * FP16 parts). The transform's basis is built on the first call for a given (n, T) on a device and kept: that first
+ * call synchronises the stream and must not be made inside a graph capture (later calls may be captured). */
 int mppi_sample_colored(const mppi_sampler_params* sampler, int32_t num_rollouts, int32_t num_timesteps,
                         const float* mean_d, float* samples_d, int32_t optimization_stride, int32_t iteration_num,
                         uint64_t rng_offset, const float* spectra_in_d, void* stream);
