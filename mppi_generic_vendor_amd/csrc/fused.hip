@@ -7,6 +7,7 @@
 //                             FP64 normalizer, new mean S / eta.
 // The result equals fullGPUcomputeWeights + weightedReductionKernelDeviceNorm (core/mppi_common.cu:746-773,
 // 1067-1089) on the same costs: sum_k exp(-(c_k - b)/lambda) u_k = sum_g exp(-(b_g - b)/lambda) S_g.
+#include <atomic>
 #include <climits>
 
 #include "fused.hpp"
@@ -18,16 +19,30 @@ namespace mppi_amd {
 
 __device__ __forceinline__ IterArgs load_iter_f(const IterRef& r) { return r.ptr ? *r.ptr : r.val; }
 
+// Samples per workgroup and threads per workgroup of the fused kernel; SPLIT selects the small-grid BF16-network
+// rollout (rollout_tile_nn_split).
+template <bool SPLIT>
+constexpr int ru_rows()
+{
+  return SPLIT ? NS_ROWS : RC_LANES;
+}
+template <int MODEL, bool SPLIT>
+constexpr int ru_threads()
+{
+  return SPLIT ? NS_THREADS : rollout_threads<MODEL>();
+}
+
+// sized for the smaller workgroup tile, which gives the most partials
 size_t fused_iteration_workspace_bytes(int K, int T, int C)
 {
-  const size_t G = (size_t)(K + RC_LANES - 1) / RC_LANES;
+  const size_t G = (size_t)(K + NS_ROWS - 1) / NS_ROWS;
   return G * sizeof(Partial) + G * (size_t)T * C * sizeof(float);
 }
 
 // With PROP the samples come from the learned proposal: u = mu + dmu + sig_q eps (k >= 1; sample 0 and t < stride
 // keep mu; no pure-noise samples) and every cost carries the mean-shift correction (rollout_core.hpp).
-template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP>
-__global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kernel(
+template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP, bool SPLIT>
+__global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_kernel(
     mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int K, int T, float dt, float lambda,
     float alpha, const float* __restrict__ x0, const float* __restrict__ mean, float* __restrict__ samples,
     IterRef itr, int pstart, float* __restrict__ costs_out, Partial* __restrict__ parts, float* __restrict__ sums,
@@ -35,24 +50,29 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
 {
   using M = Model<MODEL>;
   constexpr int C = M::C;
-  constexpr int NT = rollout_threads<MODEL>();
+  constexpr int NT = ru_threads<MODEL, SPLIT>();
+  constexpr int ROWS = ru_rows<SPLIT>();
+  constexpr int MV = 1024 / NT;  // mean elements per thread: T C <= MPPI_MAX_TIMESTEPS MPPI_MAX_CONTROL = 1024
+  static_assert(MV * NT == 1024 && MPPI_MAX_TIMESTEPS * MPPI_MAX_CONTROL <= 1024, "mean staging");
   extern __shared__ float lds[];
   const int TC = T * C;
   const int TC4 = (TC + 3) & ~3;
   const int ROW = TC + 1;
   float* tile = lds;
-  float* mu = tile + RC_LANES * ROW;
+  float* mu = tile + ROWS * ROW;
   float* pl = mu + TC4;                    // PROP: dmu, sig_q [2][TC4]
   float* wl = pl + (PROP ? 2 * TC4 : 0);   // [64] local weights
   float* scratch = wl + RC_LANES;          // pipelined-rollout buffers
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int g = blockIdx.x;
-  const int k0 = g * RC_LANES;
-  const int nrows = min(RC_LANES, K - k0);
-  // the mean and x0 are requested first (T C <= 1024 <= 2 NT): their loads then retire with the tile's, whose waits
-  // come later; x0 is staged in the local-weight slots, which are free until the rollout has read it
-  const float mv0 = mean[min(tid, TC - 1)], mv1 = mean[min(tid + NT, TC - 1)];
+  const int k0 = g * ROWS;
+  const int nrows = min(ROWS, K - k0);
+  // the mean and x0 are requested first: their loads then retire with the tile's, whose waits come later; x0 is
+  // staged in the local-weight slots, which are free until the rollout has read it
+  float mv[MV];
+#pragma unroll
+  for (int i = 0; i < MV; ++i) mv[i] = mean[min(tid + i * NT, TC - 1)];
   const float xv = x0[min(tid, M::S - 1)];
   float* x0_l = wl;
   bool deadband = false;
@@ -75,8 +95,9 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
   if (GEN) {
     // u = transform(eps) for the tile's elements; the tile is a contiguous span starting at element k0*T*C, which
     // is a multiple of 4, so Philox block b covers tile elements 4(b - base) .. 4(b - base) + 3.
-    if (tid < TC) mu[tid] = mv0;
-    if (tid + NT < TC) mu[tid + NT] = mv1;
+#pragma unroll
+    for (int i = 0; i < MV; ++i)
+      if (tid + i * NT < TC) mu[tid + i * NT] = mv[i];
     if (tid < M::S) x0_l[tid] = xv;
     __syncthreads();
     const IterArgs a = load_iter_f(itr);
@@ -117,15 +138,20 @@ __global__ void __launch_bounds__(rollout_threads<MODEL>()) rollout_update_kerne
   } else {
     if (deadband) tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlConstrain<C>{&dyn});
     else tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlClamp<C>(dyn));
-    if (tid < TC) mu[tid] = mv0;
-    if (tid + NT < TC) mu[tid + NT] = mv1;
+#pragma unroll
+    for (int i = 0; i < MV; ++i)
+      if (tid + i * NT < TC) mu[tid + i * NT] = mv[i];
     if (tid < M::S) x0_l[tid] = xv;
   }
   __syncthreads();
   float* cost_l;
   float* raw_l;
-  rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0_l, tile, ROW, mu, pstart, k0, scratch, chunk,
-                            cost_l, raw_l, PROP ? &pv : nullptr);
+  if constexpr (SPLIT)
+    rollout_tile_nn_split<COST>(dyn, cost, sp, T, dt, lambda, alpha, x0_l, tile, ROW, mu, pstart, k0, scratch, cost_l,
+                                raw_l, PROP ? &pv : nullptr);
+  else
+    rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0_l, tile, ROW, mu, pstart, k0, scratch, chunk,
+                              cost_l, raw_l, PROP ? &pv : nullptr);
   if (tid < RC_LANES) {
     const int k = k0 + tid;
     const float my_cost = cost_l[lane];
@@ -263,7 +289,19 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   }
 }
 
-template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP>
+// Compute units of the current device (cached per device; 0 if the query fails).
+static int device_cu_count()
+{
+  static std::atomic<int> cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  int n = cus[dev].load(std::memory_order_relaxed);
+  if (n == 0 && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+    cus[dev].store(n, std::memory_order_relaxed);
+  return n;
+}
+
+template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP, bool SPLIT>
 static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,
                      int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
                      float* samples, IterRef it, float* costs_out, Partial* parts, float* sums, const float* prop_dmu,
@@ -271,51 +309,51 @@ static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& co
 {
   constexpr int C = Model<MODEL>::C, O = Model<MODEL>::O;
   const size_t TC4 = (size_t)((T * C + 3) & ~3);
-  const size_t base = (size_t)RC_LANES * (T * C + 1) + TC4 * (PROP ? 3 : 1) + RC_LANES;
+  constexpr int ROWS = ru_rows<SPLIT>();
+  const size_t base = (size_t)ROWS * (T * C + 1) + TC4 * (PROP ? 3 : 1) + RC_LANES;
   const size_t limit = LDS_LIMIT / sizeof(float);
-  const int chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
-  if (chunk == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
-  const size_t lds = sizeof(float) * (base + rollout_scratch_for<MODEL, O>(chunk));
+  int chunk = 0;
+  size_t lds = 0;
+  if constexpr (SPLIT) {
+    if (base + nn_split_scratch_floats<O>() > limit)
+      return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
+    lds = sizeof(float) * (base + nn_split_scratch_floats<O>());
+  } else {
+    chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
+    if (chunk == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
+    lds = sizeof(float) * (base + rollout_scratch_for<MODEL, O>(chunk));
+  }
   static bool attr_set = false;
   if (!attr_set) {
-    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP>,
+    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
     attr_set = true;
   }
   const int pstart = PROP ? K : pure_noise_start(K, sp.pure_noise_trajectories_percentage);
-  const int G = (K + RC_LANES - 1) / RC_LANES;
-  rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP><<<G, rollout_threads<MODEL>(), lds, s>>>(
+  const int G = (K + ROWS - 1) / ROWS;
+  rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT><<<G, ru_threads<MODEL, SPLIT>(), lds, s>>>(
       dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, pstart, costs_out, parts, sums, chunk, prop_dmu,
       prop_sig);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
 
-template <int MODEL, int COST>
+template <int MODEL, int COST, bool SPLIT>
 static int dispatch_flags(bool gen, bool write, const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                           const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                           const float* x0, const float* mean, float* samples, IterRef it, float* costs_out,
                           Partial* parts, float* sums, const float* prop_dmu, const float* prop_sig, hipStream_t s)
 {
+#define MPPI_LAUNCH_RU(GEN, WRITE, PROP)                                                                         \
+  launch_ru<MODEL, COST, GEN, WRITE, PROP, SPLIT>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, \
+                                                  costs_out, parts, sums, prop_dmu, prop_sig, s)
   if (prop_dmu != nullptr) {
     if (!gen) return fail(MPPI_E_INVALID, "fused iteration: the learned proposal draws Gaussian samples");
-    if (write)
-      return launch_ru<MODEL, COST, true, true, true>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                                      costs_out, parts, sums, prop_dmu, prop_sig, s);
-    return launch_ru<MODEL, COST, true, false, true>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                                     costs_out, parts, sums, prop_dmu, prop_sig, s);
+    return write ? MPPI_LAUNCH_RU(true, true, true) : MPPI_LAUNCH_RU(true, false, true);
   }
-  if (gen && write)
-    return launch_ru<MODEL, COST, true, true, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                                     costs_out, parts, sums, nullptr, nullptr, s);
-  if (gen)
-    return launch_ru<MODEL, COST, true, false, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                                      costs_out, parts, sums, nullptr, nullptr, s);
-  if (write)
-    return launch_ru<MODEL, COST, false, true, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                                      costs_out, parts, sums, nullptr, nullptr, s);
-  return launch_ru<MODEL, COST, false, false, false>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it,
-                                                     costs_out, parts, sums, nullptr, nullptr, s);
+  if (gen) return write ? MPPI_LAUNCH_RU(true, true, false) : MPPI_LAUNCH_RU(true, false, false);
+  return write ? MPPI_LAUNCH_RU(false, true, false) : MPPI_LAUNCH_RU(false, false, false);
+#undef MPPI_LAUNCH_RU
 }
 
 int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
@@ -325,18 +363,27 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            IterArgs* advance, hipStream_t s, const float* prop_dmu, const float* prop_sig)
 {
   const int C = dyn.control_dim;
-  const int G = (K + RC_LANES - 1) / RC_LANES;
+  const int model = (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
+                        ? MODEL_RACER_DUBINS_NN_BF16
+                        : dyn.model;
+  // the BF16-network rollout is bound by the latency of its per-step chain: while every CU can take one 16-sample
+  // workgroup (one wavefront per SIMD), splitting each step's network over four wavefronts finishes sooner; beyond
+  // that, the 64-sample pipeline does the same work with a quarter of the replicated dynamics
+  const bool split = model == MODEL_RACER_DUBINS_NN_BF16 && (K + NS_ROWS - 1) / NS_ROWS <= device_cu_count();
+  const int rows = split ? NS_ROWS : RC_LANES;
+  const int G = (K + rows - 1) / rows;
   if (G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "fused iteration: too many samples");
   Partial* parts = static_cast<Partial*>(ws);
   float* sums = reinterpret_cast<float*>(static_cast<char*>(ws) + (size_t)G * sizeof(Partial));
   int st = MPPI_E_UNSUPPORTED;
-  const int model = (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
-                        ? MODEL_RACER_DUBINS_NN_BF16
-                        : dyn.model;
-#define MPPI_FUSED_CASE(MODEL, COST)                                                                            \
-  if (model == MODEL && cost.cost == COST)                                                                      \
-    st = dispatch_flags<MODEL, COST>(generate, write_samples, dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, \
-                                     samples, it, costs_out, parts, sums, prop_dmu, prop_sig, s);
+#define MPPI_FUSED_CASE(MODEL, COST)                                                                      \
+  if (model == MODEL && cost.cost == COST)                                                                \
+    st = split ? dispatch_flags<MODEL, COST, MODEL == MODEL_RACER_DUBINS_NN_BF16>(                        \
+                     generate, write_samples, dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples,  \
+                     it, costs_out, parts, sums, prop_dmu, prop_sig, s)                                   \
+               : dispatch_flags<MODEL, COST, false>(generate, write_samples, dyn, cost, sp, K, T, dt,     \
+                                                    lambda, alpha, x0, mean, samples, it, costs_out,      \
+                                                    parts, sums, prop_dmu, prop_sig, s);
   MPPI_FUSED_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
   MPPI_FUSED_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
   MPPI_FUSED_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)

@@ -10,10 +10,16 @@
 namespace mppi_amd {
 
 // utils/angle_utils.cuh:23-29 (float version)
+// fmodf(a + pi, 2 pi) takes no remainder loop for |a + pi| < 4 pi: there it is a + pi itself (below 2 pi) or
+// a + pi -+ 2 pi, which is exact (Sterbenz) -- the value fmodf returns up to the sign of a zero, which the final
+// select maps to pi either way.
 MPPI_HD float normalize_angle(float a)
 {
   const float pi = 3.14159265358979323846f;
-  const float r = fmodf(a + pi, 6.28318530717958647692f);
+  const float two_pi = 6.28318530717958647692f;
+  const float s = a + pi;
+  const float m = fabsf(s);
+  const float r = m < two_pi ? s : (m < 2.0f * two_pi ? s - copysignf(two_pi, s) : fmodf(s, two_pi));
   return (r <= 0.0f) ? r + pi : r - pi;
 }
 

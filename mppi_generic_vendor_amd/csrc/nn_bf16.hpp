@@ -19,6 +19,7 @@
 namespace mppi_amd {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -55,17 +56,22 @@ __device__ __forceinline__ bf16x8 cat_bf16(bf16x2 a, bf16x2 b, bf16x2 c, bf16x2 
   return __builtin_shufflevector(ab, cd, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
-// B-operand slots of the next layer from two accumulator tiles: tanh, then round to BF16.
-__device__ __forceinline__ bf16x8 act_pack(const f32x4& lo, const f32x4& hi)
+// One accumulator tile's activations: tanh, then round to BF16.
+__device__ __forceinline__ bf16x4 act4(const f32x4& d)
 {
-  bf16x8 r;
+  bf16x4 r;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    r[i] = (__bf16)tanh_fast(lo[i]);
-    r[4 + i] = (__bf16)tanh_fast(hi[i]);
-  }
+  for (int i = 0; i < 4; ++i) r[i] = (__bf16)tanh_fast(d[i]);
   return r;
 }
+
+__device__ __forceinline__ bf16x8 cat_bf16(bf16x4 a, bf16x4 b)
+{
+  return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// B-operand slots of the next layer from two accumulator tiles.
+__device__ __forceinline__ bf16x8 act_pack(const f32x4& lo, const f32x4& hi) { return cat_bf16(act4(lo), act4(hi)); }
 
 // Hidden unit carried by K slot (ks, g, j) of layers 2 and 3.
 __device__ __forceinline__ int nn_hidden_slot(int ks, int g, int j) { return 16 * (2 * ks + j / 4) + 4 * g + (j & 3); }
@@ -77,29 +83,34 @@ struct NnFrags {
   f32x4 b1[4], b2[4], b3[2];
 };
 
-// Builds lane `lane`'s fragments from the FP32 parameter vector `th` (LDS or global). `out_states` are the distinct
-// states (< 8) that the NN_OUT outputs are added to.
-__device__ __forceinline__ void nn_load_frags(const float* th, const int* out_states, int lane, NnFrags& F)
+// Lane `lane`'s fragments of hidden tile mt (units 16 mt .. 16 mt + 15) of layers 1 and 2, from the FP32 parameter
+// vector `th` (LDS or global).
+__device__ __forceinline__ void nn_load_hidden_tile(const float* th, int lane, int mt, bf16x8& a1, bf16x8 (&a2)[2],
+                                                    f32x4& b1, f32x4& b2)
 {
   const int m = lane & 15, g = lane >> 4;
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int f = 8 * g + j;
-      F.a1[mt][j] = (__bf16)(f < NN_IN ? th[NN_W1 + (16 * mt + m) * NN_IN + f] : 0.0f);
-    }
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        F.a2[mt][ks][j] = (__bf16)th[NN_W2 + (16 * mt + m) * NN_H1 + nn_hidden_slot(ks, g, j)];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      F.b1[mt][i] = th[NN_B1 + 16 * mt + 4 * g + i];
-      F.b2[mt][i] = th[NN_B2 + 16 * mt + 4 * g + i];
-    }
+  for (int j = 0; j < 8; ++j) {
+    const int f = 8 * g + j;
+    a1[j] = (__bf16)(f < NN_IN ? th[NN_W1 + (16 * mt + m) * NN_IN + f] : 0.0f);
   }
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a2[ks][j] = (__bf16)th[NN_W2 + (16 * mt + m) * NN_H1 + nn_hidden_slot(ks, g, j)];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    b1[i] = th[NN_B1 + 16 * mt + 4 * g + i];
+    b2[i] = th[NN_B2 + 16 * mt + 4 * g + i];
+  }
+}
+
+// Lane `lane`'s fragments of the output layer. `out_states` are the distinct states (< 8) that the NN_OUT outputs
+// are added to.
+__device__ __forceinline__ void nn_load_out_tiles(const float* th, const int* out_states, int lane,
+                                                  bf16x8 (&a3)[2][2], f32x4 (&b3)[2])
+{
+  const int m = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     // output feeding the state of this lane's row (m % 4) and of its accumulator rows (4 g + i)
@@ -111,33 +122,54 @@ __device__ __forceinline__ void nn_load_frags(const float* th, const int* out_st
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        F.a3[q][ks][j] = (__bf16)(o_row >= 0 ? th[NN_W3 + o_row * NN_H2 + nn_hidden_slot(ks, g, j)] : 0.0f);
+        a3[q][ks][j] = (__bf16)(o_row >= 0 ? th[NN_W3 + o_row * NN_H2 + nn_hidden_slot(ks, g, j)] : 0.0f);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       float b = 0.0f;
 #pragma unroll
       for (int o = 0; o < NN_OUT; ++o)
         if (out_states[o] == 4 * q + i) b = th[NN_B3 + o];
-      F.b3[q][i] = b;
+      b3[q][i] = b;
     }
   }
 }
 
+// Builds lane `lane`'s fragments of the whole network.
+__device__ __forceinline__ void nn_load_frags(const float* th, const int* out_states, int lane, NnFrags& F)
+{
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) nn_load_hidden_tile(th, lane, mt, F.a1[mt], F.a2[mt], F.b1[mt], F.b2[mt]);
+  nn_load_out_tiles(th, out_states, lane, F.a3, F.b3);
+}
+
 // Network increments of the derivatives of states 0..3 (.x) and 4..7 (.y) for the lane's sample (every lane group of
-// the sample gets all of them). x: the 7 states, u: the 2 controls, g = lane / 16.
+// the sample gets all of them).
 struct NnOut {
   f32x4 lo, hi;
 };
 
+// Layer-1 B operand: inputs 0..7 (x0..x6, u0) in the K slots of lane group 0, input 8 (u1) in slot 0 of group 1.
+// The other groups repeat group 0's values: their layer-1 weights are zero, so (for finite inputs) they add exact
+// zeros. x: the 7 states, u: the 2 controls, g = lane / 16.
+__device__ __forceinline__ bf16x8 nn_input(const float* x, const float* u, int g)
+{
+  return cat_bf16(cvt_bf16x2(g == 1 ? u[1] : x[0], x[1]), cvt_bf16x2(x[2], x[3]), cvt_bf16x2(x[4], x[5]),
+                  cvt_bf16x2(x[6], u[0]));
+}
+
+// Output layer from the two layer-2 B operands.
+__device__ __forceinline__ NnOut nn_output(const bf16x8 (&a3)[2][2], const f32x4 (&b3)[2], const bf16x8& h2a,
+                                           const bf16x8& h2b)
+{
+  NnOut r;
+  r.lo = mfma_bf16(a3[0][1], h2b, mfma_bf16(a3[0][0], h2a, b3[0]));
+  r.hi = mfma_bf16(a3[1][1], h2b, mfma_bf16(a3[1][0], h2a, b3[1]));
+  return r;
+}
+
 __device__ __forceinline__ NnOut nn_eval_bf16(const NnFrags& F, const float* x, const float* u, int g)
 {
-  // inputs 0..7 (x0..x6, u0) in lane group 0, input 8 (u1) in slot 0 of group 1, zeros elsewhere
-  const bf16x2 zero = cvt_bf16x2(0.0f, 0.0f);
-  const bf16x2 p0 = cvt_bf16x2(g == 0 ? x[0] : (g == 1 ? u[1] : 0.0f), g == 0 ? x[1] : 0.0f);
-  const bf16x2 p1 = g == 0 ? cvt_bf16x2(x[2], x[3]) : zero;
-  const bf16x2 p2 = g == 0 ? cvt_bf16x2(x[4], x[5]) : zero;
-  const bf16x2 p3 = g == 0 ? cvt_bf16x2(x[6], u[0]) : zero;
-  const bf16x8 bin = cat_bf16(p0, p1, p2, p3);
+  const bf16x8 bin = nn_input(x, u, g);
   f32x4 d1[4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) d1[mt] = mfma_bf16(F.a1[mt], bin, F.b1[mt]);
@@ -148,10 +180,7 @@ __device__ __forceinline__ NnOut nn_eval_bf16(const NnFrags& F, const float* x, 
   for (int mt = 0; mt < 4; ++mt) d2[mt] = mfma_bf16(F.a2[mt][1], h1b, mfma_bf16(F.a2[mt][0], h1a, F.b2[mt]));
   const bf16x8 h2a = act_pack(d2[0], d2[1]);
   const bf16x8 h2b = act_pack(d2[2], d2[3]);
-  NnOut r;
-  r.lo = mfma_bf16(F.a3[0][1], h2b, mfma_bf16(F.a3[0][0], h2a, F.b3[0]));
-  r.hi = mfma_bf16(F.a3[1][1], h2b, mfma_bf16(F.a3[1][0], h2a, F.b3[1]));
-  return r;
+  return nn_output(F.a3, F.b3, h2a, h2b);
 }
 
 }  // namespace mppi_amd

@@ -243,6 +243,141 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
   raw_l_out = raw_l;
 }
 
+// Small-grid variant for MODEL_RACER_DUBINS_NN_BF16: a workgroup of NS_THREADS threads rolls out NS_ROWS = 16
+// samples, so that a small K spreads over four times as many CUs. Its four wavefronts share each step's network:
+// wavefront q evaluates hidden tile q (units 16 q .. 16 q + 15) of layers 1 and 2 for all 16 samples and hands its
+// BF16 activations to the others through LDS (one barrier per layer); every wavefront then evaluates the output
+// layer and steps the dynamics of the 16 samples itself (lane 16 g + n carries sample n). The MFMA operands and
+// rounding points are those of nn_eval_bf16, so the states equal the 64-sample pipeline's bit for bit. Costs: wavefront
+// 0 keeps the outputs of the last NS_BLOCK steps in LDS; after each block every lane evaluates the running and
+// likelihood-ratio cost of one (sample, step) of it -- lane 16 g + n of wavefront q takes step 4 q + g of sample n.
+// `scratch` must hold nn_split_scratch_floats<O>() floats.
+constexpr int NS_ROWS = 16;
+constexpr int NS_THREADS = 256;
+constexpr int NS_BLOCK = 16;
+static_assert(NS_BLOCK == NS_THREADS / NS_ROWS, "one (sample, step) of a block per lane");
+
+template <int O>
+constexpr size_t nn_split_scratch_floats()
+{
+  // activation exchange (2 layers x 4 tiles x 64 lanes x 4 BF16), output history, partial sums, terminal costs,
+  // costs and uncorrected costs (64 each: the caller reads a full wavefront)
+  constexpr size_t s = 2 * 4 * 64 * 2 + (size_t)NS_BLOCK * NS_ROWS * (O | 1) + 2 * 4 * NS_ROWS + 3 * 64;
+  return s < (size_t)NN_PARAMS ? (size_t)NN_PARAMS : s;
+}
+
+template <int COST>
+__device__ __forceinline__ void rollout_tile_nn_split(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                                                      const mppi_sampler_params& sp, int T, float dt, float lambda,
+                                                      float alpha, const float* __restrict__ x0, float* tile, int ROW,
+                                                      const float* mu, int pstart, int k0, float* scratch,
+                                                      float*& cost_l_out, float*& raw_l_out, const PropLds* prop)
+{
+  using M = Model<MODEL_RACER_DUBINS_NN_BF16>;
+  using CF = Cost<COST, M::O>;
+  constexpr int S = M::S, C = M::C, O = M::O;
+  static_assert(O == S, "the bicycle's output is its state");
+  bf16x4* xh1 = reinterpret_cast<bf16x4*>(scratch);        // [4 tiles][64 lanes]
+  bf16x4* xh2 = xh1 + 4 * 64;                                // [4 tiles][64 lanes]
+  float* hist = scratch + 2 * 4 * 64 * 2;                    // [NS_BLOCK][NS_ROWS][O | 1]
+  float* part = hist + NS_BLOCK * NS_ROWS * (O | 1);         // [4][NS_ROWS]
+  float* part_is = part + 4 * NS_ROWS;                       // [4][NS_ROWS]
+  float* term_l = part_is + 4 * NS_ROWS;                     // [64]
+  float* cost_l = term_l + 64;                               // [64]
+  float* raw_l = cost_l + 64;                                // [64]
+  const int tid = threadIdx.x;
+  const int q = tid >> 6, lane = tid & 63;
+  const int g = lane >> 4, n = lane & 15;
+
+  // network parameters: staged in LDS (coalesced), then each lane builds its wavefront's hidden tile and the output
+  // layer; the staging area is reused once every lane has its fragments
+  for (int i = tid; i < NN_PARAMS; i += NS_THREADS) scratch[i] = dyn.nn_theta[i];
+  __syncthreads();
+  bf16x8 a1, a2[2], a3[2][2];
+  f32x4 b1, b2, b3[2];
+  nn_load_hidden_tile(scratch, lane, q, a1, a2, b1, b2);
+  nn_load_out_tiles(scratch, dyn.nn_output_states, lane, a3, b3);
+  __syncthreads();
+
+  float x[S], un[C];
+  const float* row = tile + n * ROW;
+#pragma unroll
+  for (int i = 0; i < S; ++i) x[i] = x0[i];
+#pragma unroll
+  for (int c = 0; c < C; ++c) un[c] = row[c];
+  const bool pure = (k0 + n) >= pstart;
+  float acc = 0.0f, acc_is = 0.0f;
+
+  for (int t = 0; t < T; ++t) {
+    float u[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) u[c] = un[c];
+    // past the horizon this reads the row's padding word and the next row (or the mean behind the tile), unused
+#pragma unroll
+    for (int c = 0; c < C; ++c) un[c] = row[(t + 1) * C + c];
+    xh1[q * 64 + lane] = act4(mfma_bf16(a1, nn_input(x, u, g), b1));
+    // the bicycle's own derivatives are computed while the activations are in flight (held here, before the
+    // barrier, rather than sunk to their use)
+    float xd[S], xn[S];
+    M::deriv(dyn, x, u, xd);
+#pragma unroll
+    for (int s = 0; s < S; ++s) asm volatile("" : "+v"(xd[s]));
+    __syncthreads();
+    const bf16x8 h1a = cat_bf16(xh1[lane], xh1[64 + lane]);
+    const bf16x8 h1b = cat_bf16(xh1[128 + lane], xh1[192 + lane]);
+    asm volatile("" ::"v"(h1a), "v"(h1b));  // both reads in flight before the first wait
+    xh2[q * 64 + lane] = act4(mfma_bf16(a2[1], h1b, mfma_bf16(a2[0], h1a, b2)));
+    __syncthreads();
+    const NnOut o = nn_output(a3, b3, cat_bf16(xh2[lane], xh2[64 + lane]), cat_bf16(xh2[128 + lane], xh2[192 + lane]));
+#pragma unroll
+    for (int s = 0; s < S; ++s) xd[s] += s < 4 ? o.lo[s] : o.hi[s - 4];
+    M::update(dyn, x, xn, xd, dt);
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = xn[i];
+    if (q == 0 && g == 0) {
+#pragma unroll
+      for (int i = 0; i < O; ++i) hist[((t % NS_BLOCK) * NS_ROWS + n) * (O | 1) + i] = xn[i];
+    }
+    if (t % NS_BLOCK == NS_BLOCK - 1 || t == T - 1) {
+      // the block's history is complete; it is next written after the following step's barriers
+      __syncthreads();
+      const int tb = t - t % NS_BLOCK;
+      const int ts = tb + 4 * q + g;
+      if (ts <= t) {
+        acc += CF::running(cost, hist + ((ts - tb) * NS_ROWS + n) * (O | 1)) +
+               lr_cost<C>(sp, row + ts * C, mu + ts * C, pure, lambda, alpha);
+        if (prop) acc_is += is_term<C>(*prop, row + ts * C, mu + ts * C, ts * C);
+      }
+    }
+  }
+  if (q == 0 && g == 0) term_l[n] = CF::terminal(cost, x);
+  // per-sample sums: the four lane groups of a wavefront, then the four wavefronts, in a fixed order
+  acc += __shfl_xor(acc, 16, 64);
+  acc += __shfl_xor(acc, 32, 64);
+  acc_is += __shfl_xor(acc_is, 16, 64);
+  acc_is += __shfl_xor(acc_is, 32, 64);
+  if (g == 0) {
+    part[q * NS_ROWS + n] = acc;
+    part_is[q * NS_ROWS + n] = acc_is;
+  }
+  __syncthreads();
+  if (tid < NS_ROWS) {
+    float run = 0.0f, isv = 0.0f;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      run += part[w * NS_ROWS + tid];
+      isv += part_is[w * NS_ROWS + tid];
+    }
+    const float raw = run / (float)T + term_l[tid] / (float)T;
+    raw_l[tid] = raw;
+    // the warm start (sample 0) is not a draw from the proposal and carries no density ratio
+    cost_l[tid] = (prop && k0 + tid != 0) ? raw + lambda * isv : raw;
+  }
+  __syncthreads();
+  cost_l_out = cost_l;
+  raw_l_out = raw_l;
+}
+
 // Runs the pipelined rollout of the tile; on return (after a barrier) cost_l[r] holds the cost of sample k0 + r for
 // every r < 64 (garbage for r >= nrows) and raw_l[r] the same cost without the importance-weight correction (equal
 // to it without a proposal). `scratch` provides rollout_scratch_floats<O>(chunk) floats.
