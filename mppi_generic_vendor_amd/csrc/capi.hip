@@ -233,11 +233,18 @@ struct mppi_controller {
   IterArgs* iter_d = nullptr;
   float* map_d = nullptr;
   float* nn_d = nullptr;
+  // per-call exchange buffers: mapped fine-grained host memory (the *_hd members are their device-side addresses),
+  // moved by copy_words launches on the controller's stream
   IterArgs* iter_h = nullptr;
   float* x0_h = nullptr;
   float* nominal_h = nullptr;
   float* mean_out_h = nullptr;
   mppi_weight_stats* stats_h = nullptr;
+  IterArgs* iter_hd = nullptr;
+  float* x0_hd = nullptr;
+  float* nominal_hd = nullptr;
+  float* mean_out_hd = nullptr;
+  mppi_weight_stats* stats_hd = nullptr;
   std::vector<float> control, raw_mean, state_seq, nn_host;
   mppi_weight_stats stats{};
   mppi_free_energy fe{};
@@ -256,6 +263,7 @@ struct mppi_controller {
   float* prop_sig_d = nullptr;
   float* sigma_base_d = nullptr;
   float* cost0_h = nullptr;
+  float* cost0_hd = nullptr;
   int prop_runs = 0;
   hipGraph_t prop_graph = nullptr;
   hipGraphExec_t prop_exec = nullptr;
@@ -589,11 +597,11 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   MPPI_CHECK(dev_alloc(&c->ws_d, ws));
   MPPI_CHECK(dev_alloc(&c->iter_d, sizeof(IterArgs) * desc->num_iters));
   MPPI_CHECK(dev_alloc(&c->loop_iter_d, sizeof(IterArgs)));
-  MPPI_CHECK(host_alloc(&c->iter_h, sizeof(IterArgs) * desc->num_iters));
-  MPPI_CHECK(host_alloc(&c->x0_h, sizeof(float) * MPPI_MAX_STATE));
-  MPPI_CHECK(host_alloc(&c->nominal_h, sizeof(float) * T * C));
-  MPPI_CHECK(host_alloc(&c->mean_out_h, sizeof(float) * T * C));
-  MPPI_CHECK(host_alloc(&c->stats_h, sizeof(mppi_weight_stats)));
+  MPPI_CHECK(host_alloc_mapped(&c->iter_h, &c->iter_hd, sizeof(IterArgs) * desc->num_iters));
+  MPPI_CHECK(host_alloc_mapped(&c->x0_h, &c->x0_hd, sizeof(float) * MPPI_MAX_STATE));
+  MPPI_CHECK(host_alloc_mapped(&c->nominal_h, &c->nominal_hd, sizeof(float) * T * C));
+  MPPI_CHECK(host_alloc_mapped(&c->mean_out_h, &c->mean_out_hd, sizeof(float) * T * C));
+  MPPI_CHECK(host_alloc_mapped(&c->stats_h, &c->stats_hd, sizeof(mppi_weight_stats)));
   // plugin-owned device copies of the cost map and the MLP parameters
   if (desc->cost_map_host != nullptr) {
     const size_t n = (size_t)desc->cost.map_width * desc->cost.map_height;
@@ -666,13 +674,13 @@ static int controller_enqueue(mppi_controller* c)
   const int K = c->K, T = c->T, C = c->C, S = c->S;
   const hipStream_t s = c->stream;
   const mppi_controller_desc& d = c->d;
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->iter_d, c->iter_h, sizeof(IterArgs) * d.num_iters, hipMemcpyHostToDevice, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->x0_d, c->x0_h, sizeof(float) * S, hipMemcpyHostToDevice, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->nominal_h, sizeof(float) * T * C, hipMemcpyHostToDevice, s));
+  const WordSpan in[] = {{c->iter_hd, c->iter_d, words_of<IterArgs>(d.num_iters)},
+                         {c->x0_hd, c->x0_d, S},
+                         {c->nominal_hd, c->mean_d, T * C}};
+  MPPI_CHECK(launch_copy_words(in, 3, s));
   for (int it = 0; it < d.num_iters; ++it) MPPI_CHECK(enqueue_iteration(c, IterRef{c->iter_d + it, {0, 0, 0}}, nullptr));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_out_h, c->mean_d, sizeof(float) * T * C, hipMemcpyDeviceToHost, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->stats_h, c->stats_d, sizeof(mppi_weight_stats), hipMemcpyDeviceToHost, s));
-  return MPPI_OK;
+  const WordSpan out[] = {{c->mean_d, c->mean_out_hd, T * C}, {c->stats_d, c->stats_hd, words_of<mppi_weight_stats>(1)}};
+  return launch_copy_words(out, 2, s);
 }
 
 // The proposal call: network inference on the uploaded state and nominal sequence, then one proposal iteration of
@@ -682,17 +690,19 @@ static int controller_enqueue_proposal(mppi_controller* c)
   const int T = c->T, C = c->C, S = c->S;
   const hipStream_t s = c->stream;
   const mppi_controller_desc& d = c->d;
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->iter_d, c->iter_h, sizeof(IterArgs), hipMemcpyHostToDevice, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->x0_d, c->x0_h, sizeof(float) * S, hipMemcpyHostToDevice, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, c->nominal_h, sizeof(float) * T * C, hipMemcpyHostToDevice, s));
+  const WordSpan in[] = {{c->iter_hd, c->iter_d, words_of<IterArgs>(1)},
+                         {c->x0_hd, c->x0_d, S},
+                         {c->nominal_hd, c->mean_d, T * C}};
+  MPPI_CHECK(launch_copy_words(in, 3, s));
   MPPI_CHECK(proposal_infer(c->prop, c->x0_d, c->mean_d, c->sigma_base_d, c->prop_dmu_d, c->prop_sig_d, s));
   MPPI_CHECK(launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->prop_K, T, d.dt, d.lambda, d.alpha, c->x0_d,
                                     c->mean_d, c->samples_d, true, d.keep_samples != 0,
                                     IterRef{c->iter_d, {0, 0, 0}}, c->costs_d, c->ws_d, c->stats_d, c->mean_d,
                                     nullptr, s, c->prop_dmu_d, c->prop_sig_d));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_out_h, c->mean_d, sizeof(float) * T * C, hipMemcpyDeviceToHost, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->stats_h, c->stats_d, sizeof(mppi_weight_stats), hipMemcpyDeviceToHost, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->cost0_h, c->costs_d, sizeof(float), hipMemcpyDeviceToHost, s));
+  const WordSpan out[] = {{c->mean_d, c->mean_out_hd, T * C},
+                          {c->stats_d, c->stats_hd, words_of<mppi_weight_stats>(1)},
+                          {c->costs_d, c->cost0_hd, 1}};
+  MPPI_CHECK(launch_copy_words(out, 3, s));
   return MPPI_OK;
 }
 
@@ -812,7 +822,7 @@ int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, in
     MPPI_CHECK(dev_alloc(&c->prop_dmu_d, tc));
     MPPI_CHECK(dev_alloc(&c->prop_sig_d, tc));
     MPPI_CHECK(dev_alloc(&c->sigma_base_d, sizeof(float) * MPPI_MAX_CONTROL));
-    MPPI_CHECK(host_alloc(&c->cost0_h, sizeof(float)));
+    MPPI_CHECK(host_alloc_mapped(&c->cost0_h, &c->cost0_hd, sizeof(float)));
   }
   MPPI_HIP_CHECK(hipMemcpy(c->sigma_base_d, c->d.sampler.std_dev, sizeof(float) * MPPI_MAX_CONTROL,
                            hipMemcpyHostToDevice));

@@ -42,4 +42,18 @@ int host_alloc(Tp** p, size_t bytes)
   return MPPI_OK;
 }
 
+// Pinned host memory that kernels read and write directly: mapped, fine-grained (uncached on the device side, so a
+// kernel's stores are in host memory once it has completed and the host's stores are seen by the next launch).
+// *dev receives the device-side address of the same bytes.
+template <class Tp>
+int host_alloc_mapped(Tp** p, Tp** dev, size_t bytes)
+{
+  hipError_t e = hipHostMalloc((void**)p, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+  if (e != hipSuccess) return fail(MPPI_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  std::memset(*p, 0, bytes);
+  e = hipHostGetDevicePointer((void**)dev, *p, 0);
+  if (e != hipSuccess) return fail(MPPI_E_HIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+  return MPPI_OK;
+}
+
 }  // namespace mppi_amd

@@ -32,4 +32,22 @@ size_t weighted_update_workspace_bytes(int K, int T, int C);
 int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,
                            float* out, void* ws, hipStream_t s);
 
+// The small per-call exchanges of a control call in one launch: up to four spans of 32-bit words, each between
+// device memory and mapped fine-grained host memory (device-side alias), read and written by the kernel itself.
+struct WordSpan {
+  const void* src;
+  void* dst;
+  int words;
+};
+constexpr int MAX_WORD_SPANS = 4;
+int launch_copy_words(const WordSpan* spans, int count, hipStream_t s);
+
+// 32-bit words of n objects of type Tp.
+template <class Tp>
+constexpr int words_of(int n)
+{
+  static_assert(sizeof(Tp) % 4 == 0, "copied in 32-bit words");
+  return n * (int)(sizeof(Tp) / 4);
+}
+
 }  // namespace mppi_amd
