@@ -405,8 +405,8 @@ int mppi_proposal_forward(mppi_proposal* proposal, const float* inputs_d, int32_
                           void* stream);
 /* Proposal of one state as the controller uses it: the input from x0_d[S] and the nominal sequence nominal_d[T][C]
  * (normalized with the stored shift/scale), the network, then dmu_d[T][C] = sigma_base * mean shift and
- * sig_d[T][C] = sigma_base * exp(clamp(log-std, log 0.5, 1)); sigma_base_d[C]. fused = 1: one single-workgroup kernel
- * (the controller's path); 0: the input kernel, the batched forward launches and a head kernel. */
+ * sig_d[T][C] = sigma_base * exp(clamp(log-std, log 0.5, 1)); sigma_base_d[C]. fused = 1: one kernel, a workgroup
+ * per 16 outputs (the controller's path); 0: the input kernel, the batched forward launches and a head kernel. */
 int mppi_proposal_infer(mppi_proposal* proposal, const float* x0_d, const float* nominal_d,
                         const float* sigma_base_d, float* dmu_d, float* sig_d, int32_t fused, void* stream);
 /* Loss (optionally copied to loss_host, which synchronizes) and its gradient grad_d[num_params] for targets_d

@@ -214,12 +214,15 @@ __global__ void proposal_head_kernel(const float* __restrict__ y, int TC, int C,
   }
 }
 
-// Inference for one state in one workgroup (16 wavefronts): input construction, the three layers on the matrix cores
-// with the weights as the A operand (16-row tiles) and the input vector broadcast over the 16 B columns, and the
-// head. The operands are rounded to BF16 and accumulated in FP32 over 32-wide k-steps in order, the arithmetic of
-// gemm_bf16_kernel, so the result equals the batched forward pass.
-constexpr int PI_THREADS = 1024;
-constexpr int PI_WAVES = PI_THREADS / 64;
+// Inference for one state: workgroup b of a grid of ceil(2 T C / 16) evaluates the two hidden layers (wavefront w
+// holds hidden tile w) and output tile b (wavefront 0), then the head of those 16 outputs. The hidden layers are
+// repeated per workgroup so that no workgroup waits for another; what bounds the kernel is the weight traffic one
+// CU can keep in flight, and this split brings it from all three layers down to the hidden layers plus one output
+// tile per CU. The operands are rounded to BF16 and accumulated in FP32 over 32-wide k-steps in order, the arithmetic
+// of gemm_bf16_kernel, so the result equals the batched forward pass.
+constexpr int PI_WAVES = PROP_H / 16;
+constexpr int PI_THREADS = 64 * PI_WAVES;
+constexpr int PI_MAX_TILES = 64;  // output tiles (grid size): 2 T C <= 1024
 
 // Weights of the 16-row tile `mt` of an N x Kreal layer for this lane (row 16 mt + lane % 16, K slots of the lane
 // group), loaded as FP32 (16-byte loads when the row stride allows) and rounded to BF16 A-operand fragments.
@@ -259,11 +262,11 @@ __device__ __forceinline__ void infer_load_tile(const float* __restrict__ W, int
   }
 }
 
-// One layer for the input vector vin (LDS, broadcast over the 16 B columns): vout[r] = act(W[r] . vin + bias[r]) for
-// the tiles whose A fragments `a` this wave holds (tile mt_of(i) for i < nt).
+// One layer for the input vector vin (LDS, broadcast over the 16 B columns): vout[r - row0] = act(W[r] . vin + bias[r])
+// for the tiles whose A fragments `a` this wave holds (tile mt[i] for i < NT).
 template <int KS, int NT>
 __device__ __forceinline__ void infer_apply(const pbf16x8 (&a)[NT][KS], const int (&mt)[NT], const float* bias,
-                                            int N, int Kreal, const float* vin, float* vout, bool gelu)
+                                            int N, int Kreal, const float* vin, float* vout, bool gelu, int row0 = 0)
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
   pbf16x8 b[KS];
@@ -286,7 +289,7 @@ __device__ __forceinline__ void infer_apply(const pbf16x8 (&a)[NT][KS], const in
         const int r = 16 * mt[i] + 4 * g + r4;
         if (r < N) {
           const float z = acc[r4] + bias[r];
-          vout[r] = gelu ? gelu_f(z) : z;
+          vout[r - row0] = gelu ? gelu_f(z) : z;
         }
       }
     }
@@ -302,18 +305,17 @@ __global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
 {
   __shared__ float xin[DIN];
   __shared__ float h1[PROP_H], h2[PROP_H];
-  extern __shared__ float yout[];  // [2 T C]
+  __shared__ float yout[16];
   constexpr int KS1 = DIN / 32, KSH = PROP_H / 32;
-  constexpr int NT3 = 2;  // output tiles per wave: 2 T C <= 16 * PI_WAVES * NT3
   const int wave = threadIdx.x >> 6;
   const int din = S + PROP_KNOTS * C, TC = T * C, dout = 2 * TC;
+  const int tile = blockIdx.x;
   // every wave requests all the weights it will multiply before anything else (they do not depend on the input)
-  const int mt1[1] = {wave}, mt2[1] = {wave}, mt3[NT3] = {wave, wave + PI_WAVES};
-  pbf16x8 a1[1][KS1], a2[1][KSH], a3[NT3][KSH];
-  infer_load_tile<KS1>(th + w1, wave < PROP_H / 16 ? PROP_H : 0, din, wave, a1[0]);
-  infer_load_tile<KSH>(th + w2, wave < PROP_H / 16 ? PROP_H : 0, PROP_H, wave, a2[0]);
-#pragma unroll
-  for (int i = 0; i < NT3; ++i) infer_load_tile<KSH>(th + w3, dout, PROP_H, mt3[i], a3[i]);
+  const int mth[1] = {wave}, mt3[1] = {tile};
+  pbf16x8 a1[1][KS1], a2[1][KSH], a3[1][KSH];
+  infer_load_tile<KS1>(th + w1, PROP_H, din, wave, a1[0]);
+  infer_load_tile<KSH>(th + w2, PROP_H, PROP_H, wave, a2[0]);
+  if (wave == 0) infer_load_tile<KSH>(th + w3, dout, PROP_H, tile, a3[0]);
   for (int o = threadIdx.x; o < DIN; o += PI_THREADS) {
     float v = 0.0f;
     if (o < S) {
@@ -325,16 +327,23 @@ __global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
     xin[o] = v;
   }
   __syncthreads();
-  if (wave < PROP_H / 16) infer_apply<KS1, 1>(a1, mt1, th + b1, PROP_H, din, xin, h1, true);
+  infer_apply<KS1, 1>(a1, mth, th + b1, PROP_H, din, xin, h1, true);
   __syncthreads();
-  if (wave < PROP_H / 16) infer_apply<KSH, 1>(a2, mt2, th + b2, PROP_H, PROP_H, h1, h2, true);
+  infer_apply<KSH, 1>(a2, mth, th + b2, PROP_H, PROP_H, h1, h2, true);
   __syncthreads();
-  infer_apply<KSH, NT3>(a3, mt3, th + b3, dout, PROP_H, h2, yout, false);
+  if (wave == 0) {
+    // output rows 16 tile .. 16 tile + 15, stored at yout[row - 16 tile]
+    infer_apply<KSH, 1>(a3, mt3, th + b3, dout, PROP_H, h2, yout, false, 16 * tile);
+  }
   __syncthreads();
-  for (int j = threadIdx.x; j < TC; j += PI_THREADS) {
-    const float sb = sigma_base[j % C];
-    dmu[j] = sb * yout[j];
-    sig[j] = sb * expf(fminf(fmaxf(yout[TC + j], PROP_S_LO), PROP_S_HI));
+  if (threadIdx.x < 16) {
+    const int r = 16 * tile + threadIdx.x;
+    if (r < TC) {
+      dmu[r] = sigma_base[r % C] * yout[threadIdx.x];
+    } else if (r < dout) {
+      const int j = r - TC;
+      sig[j] = sigma_base[j % C] * expf(fminf(fmaxf(yout[threadIdx.x], PROP_S_LO), PROP_S_HI));
+    }
   }
 }
 
@@ -436,11 +445,11 @@ int proposal_infer(ProposalNet* p, const float* x0, const float* nominal, const 
                    float* sig, hipStream_t s)
 {
   const PropDims& d = p->d;
-  const size_t lds = sizeof(float) * (size_t)d.dout;
-  if (d.dout > 16 * PI_WAVES * 2) return fail(MPPI_E_SIZE, "proposal: output too wide for the inference kernel");
-#define MPPI_INFER(DIN)                                                                                          \
-  proposal_infer_kernel<DIN><<<1, PI_THREADS, lds, s>>>(d.S, d.C, d.T, p->theta, d.w1, d.b1, d.w2, d.b2, d.w3,   \
-                                                        d.b3, x0, nominal, p->shift, p->scale, sigma_base, dmu, sig)
+  const int tiles = (d.dout + 15) / 16;
+  if (tiles > PI_MAX_TILES) return fail(MPPI_E_SIZE, "proposal: output too wide for the inference kernel");
+#define MPPI_INFER(DIN)                                                                                              \
+  proposal_infer_kernel<DIN><<<tiles, PI_THREADS, 0, s>>>(d.S, d.C, d.T, p->theta, d.w1, d.b1, d.w2, d.b2, d.w3, d.b3, \
+                                                          x0, nominal, p->shift, p->scale, sigma_base, dmu, sig)
   if (d.din <= 32) MPPI_INFER(32);
   else if (d.din <= 64) MPPI_INFER(64);
   else MPPI_INFER(96);

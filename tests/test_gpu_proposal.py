@@ -173,7 +173,7 @@ def test_proposal_fallback_runs_plain_iteration(hip, engine):
 
 
 def test_fused_inference_equals_batched_forward(hip, engine):
-    """The controller's single-workgroup inference kernel reproduces the batched forward pass (same BF16 rounding
+    """The controller's fused inference kernel reproduces the batched forward pass (same BF16 rounding
     and k-order of the accumulation) and the oracle's BF16 forward with the head applied."""
     from mppi_generic_vendor_amd.proposal import network_inputs
 
