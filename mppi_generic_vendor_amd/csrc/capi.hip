@@ -655,7 +655,8 @@ int mppi_controller_destroy(mppi_controller* ctrl)
 
 // One MPPI iteration on the controller's stream: colored sampling (if configured) into the sample buffer, then the
 // fused rollout + weights + update (Gaussian samples drawn inside the rollout kernel).
-static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance)
+static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
+                             const ResultCopies* copies = nullptr)
 {
   const mppi_controller_desc& d = c->d;
   const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
@@ -665,7 +666,7 @@ static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance)
   // control_means_d_ (sampling_distributions/gaussian/gaussian.cu:434-467); the next iteration samples around it
   return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->K, c->T, d.dt, d.lambda, d.alpha, c->x0_d,
                                 c->mean_d, c->samples_d, !colored, d.keep_samples != 0, ref, c->costs_d, c->ws_d,
-                                c->stats_d, c->mean_d, advance, c->stream);
+                                c->stats_d, c->mean_d, advance, c->stream, nullptr, nullptr, copies);
 }
 
 // The work of one computeControl call, enqueued on the controller's stream (captured once when use_graph is set).
@@ -678,9 +679,15 @@ static int controller_enqueue(mppi_controller* c)
                          {c->x0_hd, c->x0_d, S},
                          {c->nominal_hd, c->mean_d, T * C}};
   MPPI_CHECK(launch_copy_words(in, 3, s));
-  for (int it = 0; it < d.num_iters; ++it) MPPI_CHECK(enqueue_iteration(c, IterRef{c->iter_d + it, {0, 0, 0}}, nullptr));
-  const WordSpan out[] = {{c->mean_d, c->mean_out_hd, T * C}, {c->stats_d, c->stats_hd, words_of<mppi_weight_stats>(1)}};
-  return launch_copy_words(out, 2, s);
+  // the last combine also writes the new mean and the statistics to the host buffers
+  ResultCopies out;
+  out.mean = c->mean_out_hd;
+  out.stats = c->stats_hd;
+  for (int it = 0; it < d.num_iters; ++it) {
+    const IterRef ref{c->iter_d + it, {0, 0, 0}};
+    MPPI_CHECK(enqueue_iteration(c, ref, nullptr, it + 1 == d.num_iters ? &out : nullptr));
+  }
+  return MPPI_OK;
 }
 
 // The proposal call: network inference on the uploaded state and nominal sequence, then one proposal iteration of
@@ -695,15 +702,15 @@ static int controller_enqueue_proposal(mppi_controller* c)
                          {c->nominal_hd, c->mean_d, T * C}};
   MPPI_CHECK(launch_copy_words(in, 3, s));
   MPPI_CHECK(proposal_infer(c->prop, c->x0_d, c->mean_d, c->sigma_base_d, c->prop_dmu_d, c->prop_sig_d, s));
-  MPPI_CHECK(launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->prop_K, T, d.dt, d.lambda, d.alpha, c->x0_d,
-                                    c->mean_d, c->samples_d, true, d.keep_samples != 0,
-                                    IterRef{c->iter_d, {0, 0, 0}}, c->costs_d, c->ws_d, c->stats_d, c->mean_d,
-                                    nullptr, s, c->prop_dmu_d, c->prop_sig_d));
-  const WordSpan out[] = {{c->mean_d, c->mean_out_hd, T * C},
-                          {c->stats_d, c->stats_hd, words_of<mppi_weight_stats>(1)},
-                          {c->costs_d, c->cost0_hd, 1}};
-  MPPI_CHECK(launch_copy_words(out, 3, s));
-  return MPPI_OK;
+  ResultCopies out;
+  out.mean = c->mean_out_hd;
+  out.stats = c->stats_hd;
+  out.cost0_src = c->costs_d;
+  out.cost0 = c->cost0_hd;
+  return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->prop_K, T, d.dt, d.lambda, d.alpha, c->x0_d,
+                                c->mean_d, c->samples_d, true, d.keep_samples != 0, IterRef{c->iter_d, {0, 0, 0}},
+                                c->costs_d, c->ws_d, c->stats_d, c->mean_d, nullptr, s, c->prop_dmu_d, c->prop_sig_d,
+                                &out);
 }
 
 // Runs one enqueue function eagerly on its first use and from a captured hipGraph afterwards (when use_graph).

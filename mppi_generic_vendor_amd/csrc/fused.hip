@@ -200,7 +200,8 @@ constexpr int MAX_PARTIALS = 4096;
 __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial* __restrict__ parts,
                                                                   const float* __restrict__ sums, int G, int K, int TC,
                                                                   float lambda, mppi_weight_stats* __restrict__ stats,
-                                                                  float* __restrict__ mean_out, IterArgs* advance)
+                                                                  float* __restrict__ mean_out, IterArgs* advance,
+                                                                  ResultCopies copies)
 {
   __shared__ float escale[MAX_PARTIALS];
   __shared__ float sv[COMBINE_THREADS / 64];
@@ -273,7 +274,9 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
     float s = 0.0f;
 #pragma unroll
     for (int q = 0; q < NG; ++q) s += acc_l[q][col];
-    mean_out[j] = (float)((double)s / eta);
+    const float m = (float)((double)s / eta);
+    mean_out[j] = m;
+    if (copies.mean != nullptr) copies.mean[j] = m;
   }
   if (blockIdx.x == 0 && tid == 0) {
     mppi_weight_stats st;
@@ -285,6 +288,8 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
     st.num_rollouts = K;
     st.weighted_cost = wcost / eta;
     *stats = st;
+    if (copies.stats != nullptr) *copies.stats = st;
+    if (copies.cost0 != nullptr) *copies.cost0 = *copies.cost0_src;
     if (advance != nullptr) advance->rng_offset += 1;
   }
 }
@@ -360,7 +365,8 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                            const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
-                           IterArgs* advance, hipStream_t s, const float* prop_dmu, const float* prop_sig)
+                           IterArgs* advance, hipStream_t s, const float* prop_dmu, const float* prop_sig,
+                           const ResultCopies* copies)
 {
   const int C = dyn.control_dim;
   const int model = (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
@@ -394,7 +400,8 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
     return fail(MPPI_E_UNSUPPORTED, "fused iteration: model/cost combination is not compiled into the kernel library");
   if (st != MPPI_OK) return st;
   const int grid = (T * C + COMBINE_COLS - 1) / COMBINE_COLS;
-  combine_kernel<<<grid, COMBINE_THREADS, 0, s>>>(parts, sums, G, K, T * C, lambda, stats, mean_out, advance);
+  combine_kernel<<<grid, COMBINE_THREADS, 0, s>>>(parts, sums, G, K, T * C, lambda, stats, mean_out, advance,
+                                                  copies != nullptr ? *copies : ResultCopies{});
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }

@@ -16,15 +16,25 @@ struct Partial {
 
 size_t fused_iteration_workspace_bytes(int K, int T, int C);
 
+// Optional copies of an iteration's results that the combine kernel also writes, typically to mapped host memory
+// (device-side addresses): the new mean [T C], the statistics, and cost0_src[0] to cost0. Null members are skipped.
+struct ResultCopies {
+  float* mean = nullptr;
+  mppi_weight_stats* stats = nullptr;
+  const float* cost0_src = nullptr;
+  float* cost0 = nullptr;
+};
+
 // One MPPI iteration in two launches. With `generate` the Gaussian control samples are drawn inside the rollout
 // kernel (rng offset, stride and iteration from `it`); otherwise they are read from `samples` (e.g. written by the
 // colored sampler). `write_samples` stores the clamped samples back to `samples`; `costs_out` (optional) receives
-// the per-sample costs. `advance` (optional) gets its rng offset incremented by the combine kernel.
+// the per-sample costs. `advance` (optional) gets its rng offset incremented by the combine kernel; `copies`
+// (optional) names further destinations of the results.
 int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                            const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                            const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
                            IterArgs* advance, hipStream_t s, const float* prop_dmu = nullptr,
-                           const float* prop_sig = nullptr);
+                           const float* prop_sig = nullptr, const ResultCopies* copies = nullptr);
 
 }  // namespace mppi_amd
