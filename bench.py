@@ -256,6 +256,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--learned-episodes", type=int, default=4,
                     help="closed-loop episodes of the learned-proposal comparison (0: skip)")
+    ap.add_argument("--learned-eager", action="store_true",
+                    help="run the learned-proposal comparison with eager launches instead of graph replay "
+                         "(for kernel-trace profiles of that path)")
     ap.add_argument("--calibration-mib", type=int, default=256,
                     help="device-to-device copy of this size, a plain copy kernel of known traffic for counter scaling")
     args = ap.parse_args()
@@ -353,7 +356,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(prob, args.cpu_seconds, 1)
-    learned_cmp = learned_comparison(args.learned_episodes) if args.workload == "di_colored_map" else None
+    learned_cmp = (learned_comparison(args.learned_episodes, not args.learned_eager)
+                   if args.workload == "di_colored_map" else None)
 
     line = {
         "metric": METRIC,
@@ -381,7 +385,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def learned_comparison(episodes: int):
+def learned_comparison(episodes: int, use_graph: bool = True):
     """Learned vs plain MPPI on the bicycle + residual-network task with the committed trained proposal: closed-loop
     executed-trajectory cost from held-out starts and stream time per control step of plain MPPI (8192 samples,
     3 iterations), the learned proposal (2048 samples, 1 iteration, fallback above 0.5 %), and plain MPPI with the
@@ -392,7 +396,7 @@ def learned_comparison(episodes: int):
         return None
     net, summary = train_proposal.load()
     res = learned.evaluate(net, episodes=episodes, steps=100, seed=99, proposal_rollouts=2048,
-                           cost_map=workloads.gaussian_blob_map())
+                           cost_map=workloads.gaussian_blob_map(), use_graph=use_graph)
     net.close()
     for v in res.values():
         v.pop("costs", None)

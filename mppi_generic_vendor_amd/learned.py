@@ -168,7 +168,8 @@ def closed_loop(prob: workloads.Problem, ctrl: VanillaMPPIController, starts: li
 
 def evaluate(net: ProposalNet | None, episodes: int = 10, steps: int = 100, seed: int = 99,
              plain_rollouts: int = 8192, plain_iters: int = 3, proposal_rollouts: int = 2048,
-             fallback_tolerance: float = 0.005, cost_map: np.ndarray | None = None, log=None) -> dict:
+             fallback_tolerance: float = 0.005, cost_map: np.ndarray | None = None, log=None,
+             use_graph: bool = True) -> dict:
     m = workloads.gaussian_blob_map() if cost_map is None else cost_map
     rng = np.random.default_rng(seed)
     starts = [random_start(rng) for _ in range(episodes)]
@@ -178,7 +179,7 @@ def evaluate(net: ProposalNet | None, episodes: int = 10, steps: int = 100, seed
         modes.insert(1, ("learned", plain_rollouts, plain_iters, True))
     for name, K, iters, learned in modes:
         prob = task(K, cost_map=m, seed=4242)
-        ctrl = VanillaMPPIController(prob, num_iters=iters, use_graph=True)
+        ctrl = VanillaMPPIController(prob, num_iters=iters, use_graph=use_graph)
         if learned:
             ctrl.setProposal(net, proposal_rollouts, fallback_tolerance=fallback_tolerance)
         ctrl.computeControl(starts[0], 1)  # warm-up (graph capture)
