@@ -309,7 +309,8 @@ int mppi_controller_set_control_seq(mppi_controller* ctrl, const float* u_host);
 /* slideControlSequence(steps) (controllers/controller.cuh:609-621). */
 int mppi_controller_slide_control_seq(mppi_controller* ctrl, int32_t steps);
 /* Device pointers to the controller's buffers for inspection: samples [K][T][C] (clamped samples of the last
- * iteration when keep_samples is set; the colored sampler always writes its pre-clamp samples there), costs [K] of
+ * iteration when keep_samples is set; otherwise unspecified: colored samples are drawn inside the rollout kernel
+ * where its workgroup shape allows, else written there pre-clamp by the colored sampler), costs [K] of
  * the last iteration, and the [T][C] mean (after computeControl: the updated mean, which the next iteration samples
  * around, as the reference's control_means_d_). */
 int mppi_controller_device_buffers(const mppi_controller* ctrl, float** samples_d, float** costs_d, float** mean_d);

@@ -653,20 +653,22 @@ int mppi_controller_destroy(mppi_controller* ctrl)
   return controller_free(ctrl);
 }
 
-// One MPPI iteration on the controller's stream: colored sampling (if configured) into the sample buffer, then the
-// fused rollout + weights + update (Gaussian samples drawn inside the rollout kernel).
+// One MPPI iteration on the controller's stream: the fused sampling + rollout + weights + update (Gaussian samples
+// drawn inside the rollout kernel; colored samples too where the kernel's workgroup shape allows, else by the colored
+// sampler into the sample buffer first).
 static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
                              const ResultCopies* copies = nullptr)
 {
   const mppi_controller_desc& d = c->d;
   const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
-  if (colored) MPPI_CHECK(launch_sample_colored(d.sampler, c->K, c->T, c->C, c->mean_d, c->samples_d, ref, nullptr,
-                                                c->stream));
+  ColoredPlan plan;
+  if (colored) MPPI_CHECK(colored_plan(d.sampler, c->K, c->T, c->C, c->stream, &plan));
   // the combine kernel writes the new mean over the sampling mean, as updateDistributionParamsFromDevice writes
   // control_means_d_ (sampling_distributions/gaussian/gaussian.cu:434-467); the next iteration samples around it
   return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->K, c->T, d.dt, d.lambda, d.alpha, c->x0_d,
                                 c->mean_d, c->samples_d, !colored, d.keep_samples != 0, ref, c->costs_d, c->ws_d,
-                                c->stats_d, c->mean_d, advance, c->stream, nullptr, nullptr, copies);
+                                c->stats_d, c->mean_d, advance, c->stream, nullptr, nullptr, copies,
+                                colored ? &plan : nullptr);
 }
 
 // The work of one computeControl call, enqueued on the controller's stream (captured once when use_graph is set).

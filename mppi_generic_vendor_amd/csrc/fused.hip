@@ -7,9 +7,11 @@
 //                             FP64 normalizer, new mean S / eta.
 // The result equals fullGPUcomputeWeights + weightedReductionKernelDeviceNorm (core/mppi_common.cu:746-773,
 // 1067-1089) on the same costs: sum_k exp(-(c_k - b)/lambda) u_k = sum_g exp(-(b_g - b)/lambda) S_g.
+#include <algorithm>
 #include <atomic>
 #include <climits>
 
+#include "colored_core.hpp"
 #include "fused.hpp"
 #include "models.hpp"
 #include "rollout_core.hpp"
@@ -40,13 +42,15 @@ size_t fused_iteration_workspace_bytes(int K, int T, int C)
 }
 
 // With PROP the samples come from the learned proposal: u = mu + dmu + sig_q eps (k >= 1; sample 0 and t < stride
-// keep mu; no pure-noise samples) and every cost carries the mean-shift correction (rollout_core.hpp).
-template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP, bool SPLIT>
+// keep mu; no pure-noise samples) and every cost carries the mean-shift correction (rollout_core.hpp). With CNTT > 0
+// the workgroup's spw samples are colored noise generated in place (colored_stage with NTT = CNTT: the same series,
+// arithmetic and values as colored_mfma_kernel's workgroup `blockIdx.x`) and written into the control tile.
+template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP, bool SPLIT, int CNTT>
 __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_kernel(
     mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int K, int T, float dt, float lambda,
     float alpha, const float* __restrict__ x0, const float* __restrict__ mean, float* __restrict__ samples,
     IterRef itr, int pstart, float* __restrict__ costs_out, Partial* __restrict__ parts, float* __restrict__ sums,
-    int chunk, const float* __restrict__ prop_dmu, const float* __restrict__ prop_sig)
+    int chunk, const float* __restrict__ prop_dmu, const float* __restrict__ prop_sig, ColoredPlan cpl)
 {
   using M = Model<MODEL>;
   constexpr int C = M::C;
@@ -54,25 +58,29 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
   constexpr int ROWS = ru_rows<SPLIT>();
   constexpr int MV = 1024 / NT;  // mean elements per thread: T C <= MPPI_MAX_TIMESTEPS MPPI_MAX_CONTROL = 1024
   static_assert(MV * NT == 1024 && MPPI_MAX_TIMESTEPS * MPPI_MAX_CONTROL <= 1024, "mean staging");
+  static_assert(CNTT == 0 || (NT == CTHREADS && !GEN && !PROP && !SPLIT), "in-place colored samples");
   extern __shared__ float lds[];
   const int TC = T * C;
   const int TC4 = (TC + 3) & ~3;
   const int ROW = TC + 1;
+  const int rows = CNTT > 0 ? cpl.sh.spw : ROWS;  // samples per workgroup
   float* tile = lds;
-  float* mu = tile + ROWS * ROW;
+  float* mu = tile + ((rows * ROW + 3) & ~3);
   float* pl = mu + TC4;                    // PROP: dmu, sig_q [2][TC4]
   float* wl = pl + (PROP ? 2 * TC4 : 0);   // [64] local weights
   float* scratch = wl + RC_LANES;          // pipelined-rollout buffers
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int g = blockIdx.x;
-  const int k0 = g * ROWS;
-  const int nrows = min(ROWS, K - k0);
+  const int k0 = g * rows;
+  const int nrows = min(rows, K - k0);
   // the mean and x0 are requested first: their loads then retire with the tile's, whose waits come later; x0 is
   // staged in the local-weight slots, which are free until the rollout has read it
   float mv[MV];
+  if constexpr (CNTT == 0) {
 #pragma unroll
-  for (int i = 0; i < MV; ++i) mv[i] = mean[min(tid + i * NT, TC - 1)];
+    for (int i = 0; i < MV; ++i) mv[i] = mean[min(tid + i * NT, TC - 1)];
+  }
   const float xv = x0[min(tid, M::S - 1)];
   float* x0_l = wl;
   bool deadband = false;
@@ -135,6 +143,23 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
         }
       }
     }
+  } else if constexpr (CNTT > 0) {
+    // colored samples of this workgroup's span, generated in the scratch area (the rollout reuses it afterwards),
+    // constrained into the tile; the mean comes with them
+    const ColoredTile ct = colored_stage<CNTT>(sp, K, T, C, cpl.sh.n, cpl.sh.F, cpl.sh.Kpad, cpl.sh.KPH, cpl.sh.MT,
+                                               cpl.sh.S, cpl.sh.spw, cpl.sh.TCS, mean, itr, nullptr, cpl.sig, cpl.bhi,
+                                               cpl.blo, g, scratch);
+    const int n = nrows * TC;
+    auto fill = [&](auto f) {
+      for (int e = tid; e < n; e += NT) {
+        const int r = colored_row(ct, e), rem = e - r * TC;
+        tile[r * ROW + rem] = f(rem, colored_value(ct, r, rem, pstart));
+      }
+    };
+    if (deadband) fill(ControlConstrain<C>{&dyn});
+    else fill(ControlClamp<C>(dyn));
+    for (int j = tid; j < TC; j += NT) mu[j] = ct.mean_l[j];
+    if (tid < M::S) x0_l[tid] = xv;
   } else {
     if (deadband) tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlConstrain<C>{&dyn});
     else tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlClamp<C>(dyn));
@@ -151,7 +176,7 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
                                 raw_l, PROP ? &pv : nullptr);
   else
     rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0_l, tile, ROW, mu, pstart, k0, scratch, chunk,
-                              cost_l, raw_l, PROP ? &pv : nullptr);
+                              cost_l, raw_l, PROP ? &pv : nullptr, rows);
   if (tid < RC_LANES) {
     const int k = k0 + tid;
     const float my_cost = cost_l[lane];
@@ -306,58 +331,99 @@ static int device_cu_count()
   return n;
 }
 
-template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP, bool SPLIT>
-static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,
-                     int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
-                     float* samples, IterRef it, float* costs_out, Partial* parts, float* sums, const float* prop_dmu,
-                     const float* prop_sig, hipStream_t s)
+// LDS floats of the fused kernel for `rows` samples per workgroup: the control tile (rounded to 16 bytes), the mean
+// (and the proposal's shift and spread), the local weights, then the rollout scratch, which for in-place colored
+// samples also holds the colored stage (colored_bytes). 0 when it does not fit; *chunk_out gets the rollout chunk.
+template <int MODEL, bool PROP, bool SPLIT>
+static size_t ru_lds_floats(int T, int rows, size_t colored_bytes, int* chunk_out)
 {
   constexpr int C = Model<MODEL>::C, O = Model<MODEL>::O;
   const size_t TC4 = (size_t)((T * C + 3) & ~3);
-  constexpr int ROWS = ru_rows<SPLIT>();
-  const size_t base = (size_t)ROWS * (T * C + 1) + TC4 * (PROP ? 3 : 1) + RC_LANES;
+  const size_t tile_f = ((size_t)rows * (T * C + 1) + 3) & ~(size_t)3;
+  const size_t base = tile_f + TC4 * (PROP ? 3 : 1) + RC_LANES;
   const size_t limit = LDS_LIMIT / sizeof(float);
-  int chunk = 0;
-  size_t lds = 0;
+  *chunk_out = 0;
+  if (base >= limit) return 0;
   if constexpr (SPLIT) {
-    if (base + nn_split_scratch_floats<O>() > limit)
-      return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
-    lds = sizeof(float) * (base + nn_split_scratch_floats<O>());
+    const size_t total = base + nn_split_scratch_floats<O>();
+    return total <= limit ? total : 0;
   } else {
-    chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
-    if (chunk == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
-    lds = sizeof(float) * (base + rollout_scratch_for<MODEL, O>(chunk));
+    const int chunk = rollout_chunk_for<MODEL, O>(limit - base);
+    if (chunk == 0) return 0;
+    const size_t scratch = std::max(rollout_scratch_for<MODEL, O>(chunk), (colored_bytes + 15) / 16 * 4);
+    if (base + scratch > limit) return 0;
+    *chunk_out = chunk;
+    return base + scratch;
   }
+}
+
+// Whether the fused kernel can generate the colored samples in place: 1024-thread workgroups (every model but the
+// BF16 network), the 64-series colored shape with at least 16 samples per workgroup (the partial workspace is sized
+// for 16), and the LDS for tile + colored stage.
+static bool colored_inline_fits(int model, int T, const ColoredPlan& pl)
+{
+  if (pl.sh.NS != 64 || pl.sh.spw < NS_ROWS) return false;
+  int chunk = 0;
+  const int rows = pl.sh.spw;
+  const size_t cb = pl.sh.lds_bytes;
+  switch (model) {
+    case MPPI_MODEL_CARTPOLE: return ru_lds_floats<MPPI_MODEL_CARTPOLE, false, false>(T, rows, cb, &chunk) > 0;
+    case MPPI_MODEL_DOUBLE_INTEGRATOR:
+      return ru_lds_floats<MPPI_MODEL_DOUBLE_INTEGRATOR, false, false>(T, rows, cb, &chunk) > 0;
+    case MPPI_MODEL_RACER_DUBINS: return ru_lds_floats<MPPI_MODEL_RACER_DUBINS, false, false>(T, rows, cb, &chunk) > 0;
+    case MPPI_MODEL_RACER_DUBINS_NN:
+      return ru_lds_floats<MPPI_MODEL_RACER_DUBINS_NN, false, false>(T, rows, cb, &chunk) > 0;
+    default: return false;
+  }
+}
+
+template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP, bool SPLIT, int CNTT>
+static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,
+                     int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
+                     float* samples, IterRef it, float* costs_out, Partial* parts, float* sums, const float* prop_dmu,
+                     const float* prop_sig, const ColoredPlan* cpl, hipStream_t s)
+{
+  const int rows = CNTT > 0 ? cpl->sh.spw : ru_rows<SPLIT>();
+  int chunk = 0;
+  const size_t lf = ru_lds_floats<MODEL, PROP, SPLIT>(T, rows, CNTT > 0 ? cpl->sh.lds_bytes : 0, &chunk);
+  if (lf == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
   static bool attr_set = false;
   if (!attr_set) {
-    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT>,
+    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT, CNTT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
     attr_set = true;
   }
   const int pstart = PROP ? K : pure_noise_start(K, sp.pure_noise_trajectories_percentage);
-  const int G = (K + ROWS - 1) / ROWS;
-  rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT><<<G, ru_threads<MODEL, SPLIT>(), lds, s>>>(
-      dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, pstart, costs_out, parts, sums, chunk, prop_dmu,
-      prop_sig);
+  const int G = (K + rows - 1) / rows;
+  rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT, CNTT>
+      <<<G, ru_threads<MODEL, SPLIT>(), sizeof(float) * lf, s>>>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean,
+                                                                 samples, it, pstart, costs_out, parts, sums, chunk,
+                                                                 prop_dmu, prop_sig,
+                                                                 CNTT > 0 ? *cpl : ColoredPlan{});
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
 
-template <int MODEL, int COST, bool SPLIT>
+template <int MODEL, int COST, bool SPLIT, int CNTT>
 static int dispatch_flags(bool gen, bool write, const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                           const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                           const float* x0, const float* mean, float* samples, IterRef it, float* costs_out,
-                          Partial* parts, float* sums, const float* prop_dmu, const float* prop_sig, hipStream_t s)
+                          Partial* parts, float* sums, const float* prop_dmu, const float* prop_sig,
+                          const ColoredPlan* cpl, hipStream_t s)
 {
-#define MPPI_LAUNCH_RU(GEN, WRITE, PROP)                                                                         \
-  launch_ru<MODEL, COST, GEN, WRITE, PROP, SPLIT>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, \
-                                                  costs_out, parts, sums, prop_dmu, prop_sig, s)
-  if (prop_dmu != nullptr) {
-    if (!gen) return fail(MPPI_E_INVALID, "fused iteration: the learned proposal draws Gaussian samples");
-    return write ? MPPI_LAUNCH_RU(true, true, true) : MPPI_LAUNCH_RU(true, false, true);
+#define MPPI_LAUNCH_RU(GEN, WRITE, PROP)                                                                               \
+  launch_ru<MODEL, COST, GEN, WRITE, PROP, SPLIT, CNTT>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, \
+                                                        costs_out, parts, sums, prop_dmu, prop_sig, cpl, s)
+  if constexpr (CNTT > 0) {
+    return write ? MPPI_LAUNCH_RU(false, true, false) : MPPI_LAUNCH_RU(false, false, false);
+  } else {
+    if (prop_dmu != nullptr) {
+      if (!gen) return fail(MPPI_E_INVALID, "fused iteration: the learned proposal draws Gaussian samples");
+      return write ? MPPI_LAUNCH_RU(true, true, true) : MPPI_LAUNCH_RU(true, false, true);
+    }
+    if (gen) return write ? MPPI_LAUNCH_RU(true, true, false) : MPPI_LAUNCH_RU(true, false, false);
+    return write ? MPPI_LAUNCH_RU(false, true, false) : MPPI_LAUNCH_RU(false, false, false);
   }
-  if (gen) return write ? MPPI_LAUNCH_RU(true, true, false) : MPPI_LAUNCH_RU(true, false, false);
-  return write ? MPPI_LAUNCH_RU(false, true, false) : MPPI_LAUNCH_RU(false, false, false);
 #undef MPPI_LAUNCH_RU
 }
 
@@ -366,36 +432,48 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
                            IterArgs* advance, hipStream_t s, const float* prop_dmu, const float* prop_sig,
-                           const ResultCopies* copies)
+                           const ResultCopies* copies, const ColoredPlan* colored)
 {
   const int C = dyn.control_dim;
   const int model = (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
                         ? MODEL_RACER_DUBINS_NN_BF16
                         : dyn.model;
+  // colored samples: generated inside the rollout kernel where its workgroup shape allows, else by the colored
+  // sampler into `samples` first
+  bool inline_colored = false;
+  if (colored != nullptr) {
+    if (generate || prop_dmu != nullptr)
+      return fail(MPPI_E_INVALID, "fused iteration: colored samples exclude generated and proposal samples");
+    inline_colored = colored_inline_fits(model, T, *colored);
+    if (!inline_colored) MPPI_CHECK(launch_sample_colored(sp, K, T, C, mean, samples, it, nullptr, s));
+  }
   // the BF16-network rollout is bound by the latency of its per-step chain: while every CU can take one 16-sample
   // workgroup (one wavefront per SIMD), splitting each step's network over four wavefronts finishes sooner; beyond
   // that, the 64-sample pipeline does the same work with a quarter of the replicated dynamics
   const bool split = model == MODEL_RACER_DUBINS_NN_BF16 && (K + NS_ROWS - 1) / NS_ROWS <= device_cu_count();
-  const int rows = split ? NS_ROWS : RC_LANES;
+  const int rows = inline_colored ? colored->sh.spw : (split ? NS_ROWS : RC_LANES);
   const int G = (K + rows - 1) / rows;
   if (G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "fused iteration: too many samples");
   Partial* parts = static_cast<Partial*>(ws);
   float* sums = reinterpret_cast<float*>(static_cast<char*>(ws) + (size_t)G * sizeof(Partial));
   int st = MPPI_E_UNSUPPORTED;
-#define MPPI_FUSED_CASE(MODEL, COST)                                                                      \
-  if (model == MODEL && cost.cost == COST)                                                                \
-    st = split ? dispatch_flags<MODEL, COST, MODEL == MODEL_RACER_DUBINS_NN_BF16>(                        \
-                     generate, write_samples, dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples,  \
-                     it, costs_out, parts, sums, prop_dmu, prop_sig, s)                                   \
-               : dispatch_flags<MODEL, COST, false>(generate, write_samples, dyn, cost, sp, K, T, dt,     \
-                                                    lambda, alpha, x0, mean, samples, it, costs_out,      \
-                                                    parts, sums, prop_dmu, prop_sig, s);
+#define MPPI_FUSED_ARGS                                                                                         \
+  generate, write_samples, dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, costs_out, parts, sums, \
+      prop_dmu, prop_sig, colored, s
+#define MPPI_FUSED_CASE(MODEL, COST)                                                                   \
+  if (model == MODEL && cost.cost == COST) {                                                           \
+    constexpr bool NN16 = MODEL == MODEL_RACER_DUBINS_NN_BF16;                                         \
+    if (inline_colored) st = dispatch_flags<MODEL, COST, false, NN16 ? 0 : 4>(MPPI_FUSED_ARGS);        \
+    else if (split) st = dispatch_flags<MODEL, COST, NN16, 0>(MPPI_FUSED_ARGS);                        \
+    else st = dispatch_flags<MODEL, COST, false, 0>(MPPI_FUSED_ARGS);                                  \
+  }
   MPPI_FUSED_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
   MPPI_FUSED_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
   MPPI_FUSED_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)
   MPPI_FUSED_CASE(MPPI_MODEL_RACER_DUBINS_NN, MPPI_COST_QUADRATIC_MAP)
   MPPI_FUSED_CASE(MODEL_RACER_DUBINS_NN_BF16, MPPI_COST_QUADRATIC_MAP)
 #undef MPPI_FUSED_CASE
+#undef MPPI_FUSED_ARGS
   if (st == MPPI_E_UNSUPPORTED)
     return fail(MPPI_E_UNSUPPORTED, "fused iteration: model/cost combination is not compiled into the kernel library");
   if (st != MPPI_OK) return st;

@@ -1,6 +1,7 @@
 // Fused iteration (sampling + rollout + local weights + partial update, then a log-sum-exp combine).
 #pragma once
 
+#include "colored_core.hpp"
 #include "kernels.hpp"
 
 namespace mppi_amd {
@@ -29,12 +30,15 @@ struct ResultCopies {
 // kernel (rng offset, stride and iteration from `it`); otherwise they are read from `samples` (e.g. written by the
 // colored sampler). `write_samples` stores the clamped samples back to `samples`; `costs_out` (optional) receives
 // the per-sample costs. `advance` (optional) gets its rng offset incremented by the combine kernel; `copies`
-// (optional) names further destinations of the results.
+// (optional) names further destinations of the results. With `colored` (generate false) the samples are colored
+// noise of that plan: drawn inside the rollout kernel where its workgroup shape allows (the same values as the colored
+// sampler's), else by the colored sampler into `samples` before it.
 int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                            const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                            const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
                            IterArgs* advance, hipStream_t s, const float* prop_dmu = nullptr,
-                           const float* prop_sig = nullptr, const ResultCopies* copies = nullptr);
+                           const float* prop_sig = nullptr, const ResultCopies* copies = nullptr,
+                           const ColoredPlan* colored = nullptr);
 
 }  // namespace mppi_amd

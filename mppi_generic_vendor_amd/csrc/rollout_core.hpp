@@ -380,13 +380,15 @@ __device__ __forceinline__ void rollout_tile_nn_split(const mppi_dynamics_params
 
 // Runs the pipelined rollout of the tile; on return (after a barrier) cost_l[r] holds the cost of sample k0 + r for
 // every r < 64 (garbage for r >= nrows) and raw_l[r] the same cost without the importance-weight correction (equal
-// to it without a proposal). `scratch` provides rollout_scratch_floats<O>(chunk) floats.
+// to it without a proposal). `scratch` provides rollout_scratch_floats<O>(chunk) floats. A tile of fewer than 64
+// rows (`rows`) has its last row repeated for the lanes beyond it.
 template <int MODEL, int COST>
 __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                                              const mppi_sampler_params& sp, int T, float dt, float lambda,
                                              float alpha, const float* __restrict__ x0, float* tile, int ROW,
                                              const float* mu, int pstart, int k0, float* scratch, int chunk,
-                                             float*& cost_l_out, float*& raw_l_out, const PropLds* prop = nullptr)
+                                             float*& cost_l_out, float*& raw_l_out, const PropLds* prop = nullptr,
+                                             int rows = RC_LANES)
 {
   if constexpr (MODEL == MODEL_RACER_DUBINS_NN_BF16) {
     rollout_tile_nn<COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
@@ -409,7 +411,8 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
   // producer state; un holds the controls of the next group of PG steps (chunks are multiples of PG)
   constexpr int PG = 4;
   float x[S], y[O], un[PG][C];
-  float* row = tile + lane * ROW;
+  const int lrow = min(lane, rows - 1);
+  float* row = tile + lrow * ROW;
   if (wave == 0) {
 #pragma unroll
     for (int i = 0; i < S; ++i) x[i] = x0[i];
@@ -423,7 +426,7 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
   // consumer state
   float acc = 0.0f, acc_is = 0.0f;
   const bool pure = (k0 + lane) >= pstart;
-  const float* crow = tile + lane * ROW;
+  const float* crow = tile + lrow * ROW;
 
   for (int it = 0; it <= nch; ++it) {
     if (wave == 0 && it < nch) {
