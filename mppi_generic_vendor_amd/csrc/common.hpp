@@ -92,17 +92,56 @@ __device__ __forceinline__ Normal4 normals4(uint64_t seed, uint64_t offset, uint
 }
 
 // --------------------------------------------------------------------------------------------- wave reductions
-__device__ __forceinline__ float wave_sum(float v)
+// Butterfly exchanges over a full wavefront: stage 0..3 pair lanes inside each row of 16 by DPP (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror), stage 4 pairs rows r and r ^ 1 and stage 5 the two halves by
+// the gfx950 row-swap permutes (v_permlane16_swap / v_permlane32_swap of a register with itself: result 0 holds row
+// 2 j in rows 2 j, 2 j + 1 and result 1 row 2 j + 1; likewise for halves). After stages 0 .. s every group of 2^(s+1)
+// lanes has exchanged with every other, and both lanes of a pair see the same two values, so a commutative combine
+// leaves the same result in every lane. Every lane of the wavefront must be active.
+template <int STAGE>
+__device__ __forceinline__ uint32_t lane_partner(uint32_t v)
 {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  if constexpr (STAGE == 0) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+  else if constexpr (STAGE == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+  else if constexpr (STAGE == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);
+  else if constexpr (STAGE == 3) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);
+  else if constexpr (STAGE == 4) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return ((threadIdx.x >> 4) & 1) ? r[0] : r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32) ? r[0] : r[1];
+  }
 }
 
-__device__ __forceinline__ double wave_sum(double v)
+template <int STAGE>
+__device__ __forceinline__ float partner(float v)
 {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return __uint_as_float(lane_partner<STAGE>(__float_as_uint(v)));
+}
+template <int STAGE>
+__device__ __forceinline__ int partner(int v)
+{
+  return (int)lane_partner<STAGE>((uint32_t)v);
+}
+template <int STAGE>
+__device__ __forceinline__ double partner(double v)
+{
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint64_t lo = lane_partner<STAGE>((uint32_t)b), hi = lane_partner<STAGE>((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(lo | (hi << 32)));
+}
+
+// Sum over the 64 lanes, the same value in every lane.
+template <class Tp>
+__device__ __forceinline__ Tp wave_sum(Tp v)
+{
+  v += partner<0>(v);
+  v += partner<1>(v);
+  v += partner<2>(v);
+  v += partner<3>(v);
+  v += partner<4>(v);
+  v += partner<5>(v);
   return v;
 }
 
@@ -115,14 +154,23 @@ __device__ __forceinline__ void argmin_combine(float& v, int& i, float v2, int i
   }
 }
 
+template <int STAGE>
+__device__ __forceinline__ void argmin_stage(float& v, int& i)
+{
+  const float v2 = partner<STAGE>(v);
+  const int i2 = partner<STAGE>(i);
+  argmin_combine(v, i, v2, i2);
+}
+
+// Minimum (first strict one) over the 64 lanes, the same in every lane.
 __device__ __forceinline__ void wave_argmin(float& v, int& i)
 {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const float v2 = __shfl_xor(v, o, 64);
-    const int i2 = __shfl_xor(i, o, 64);
-    argmin_combine(v, i, v2, i2);
-  }
+  argmin_stage<0>(v, i);
+  argmin_stage<1>(v, i);
+  argmin_stage<2>(v, i);
+  argmin_stage<3>(v, i);
+  argmin_stage<4>(v, i);
+  argmin_stage<5>(v, i);
 }
 
 // A cost that is NaN or +inf carries no weight and never becomes the baseline.

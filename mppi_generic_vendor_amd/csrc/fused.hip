@@ -150,10 +150,23 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
                                                cpl.sh.S, cpl.sh.spw, cpl.sh.TCS, mean, itr, nullptr, cpl.sig, cpl.bhi,
                                                cpl.blo, g, scratch);
     const int n = nrows * TC;
+    // FU elements per thread and pass: their LDS reads are all issued before the first write
+    constexpr int FU = 4;
     auto fill = [&](auto f) {
-      for (int e = tid; e < n; e += NT) {
-        const int r = colored_row(ct, e), rem = e - r * TC;
-        tile[r * ROW + rem] = f(rem, colored_value(ct, r, rem, pstart));
+      for (int e0 = tid; e0 < n; e0 += FU * NT) {
+        float v[FU];
+        int at[FU], col[FU];
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+          const int e = min(e0 + u * NT, n - 1);
+          const int r = colored_row(ct, e);
+          col[u] = e - r * TC;
+          at[u] = r * ROW + col[u];
+          v[u] = colored_value(ct, r, col[u], pstart);
+        }
+#pragma unroll
+        for (int u = 0; u < FU; ++u)
+          if (e0 + u * NT < n) tile[at[u]] = f(col[u], v[u]);
       }
     };
     if (deadband) fill(ControlConstrain<C>{&dyn});
@@ -211,6 +224,8 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
   if (WRITE) tile_store<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
   for (int j = tid; j < TC; j += NT) {
     float acc = 0.0f;
+    // in row order; unrolled so that the LDS reads of 8 rows are in flight together
+#pragma unroll 8
     for (int r = 0; r < nrows; ++r) acc += wl[r] * tile[r * ROW + j];
     sums[(size_t)g * TC + j] = acc;
   }

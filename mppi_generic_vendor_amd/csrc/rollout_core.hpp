@@ -352,10 +352,10 @@ __device__ __forceinline__ void rollout_tile_nn_split(const mppi_dynamics_params
   }
   if (q == 0 && g == 0) term_l[n] = CF::terminal(cost, x);
   // per-sample sums: the four lane groups of a wavefront, then the four wavefronts, in a fixed order
-  acc += __shfl_xor(acc, 16, 64);
-  acc += __shfl_xor(acc, 32, 64);
-  acc_is += __shfl_xor(acc_is, 16, 64);
-  acc_is += __shfl_xor(acc_is, 32, 64);
+  acc += partner<4>(acc);
+  acc += partner<5>(acc);
+  acc_is += partner<4>(acc_is);
+  acc_is += partner<5>(acc_is);
   if (g == 0) {
     part[q * NS_ROWS + n] = acc;
     part_is[q * NS_ROWS + n] = acc_is;
