@@ -233,7 +233,7 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
 
 // Combination of G partials: workgroup q owns COMBINE_COLS columns; every workgroup recomputes the global baseline
 // and the scales e_g = exp(-(b_g - b)/lambda) (G <= MAX_PARTIALS values), workgroup 0 also writes the statistics.
-constexpr int COMBINE_COLS = 32;
+constexpr int COMBINE_COLS = 8;
 constexpr int COMBINE_THREADS = 256;
 constexpr int MAX_PARTIALS = 4096;
 
