@@ -354,14 +354,18 @@ __device__ __forceinline__ ColoredTile colored_stage(const mppi_sampler_params& 
 // Sample row of element e of the workgroup's span.
 __device__ __forceinline__ int colored_row(const ColoredTile& ct, int e) { return (int)(((float)e + 0.5f) * ct.invTC); }
 
-// The Gaussian transform of element rem = t C + c of sample row r: sample 0 and t < stride keep the mean, pure-noise
-// samples (k >= pstart) are the noise alone.
+// The Gaussian transform of sample k's element with mean mu and staged noise v: sample 0 and t < stride (`head`) keep
+// the mean, pure-noise samples (k >= pstart) are the noise alone.
+__device__ __forceinline__ float colored_transform(float mu, float v, int k, bool head, int pstart)
+{
+  const float u = (k >= pstart) ? v : mu + v;
+  return (k == 0 || head) ? mu : u;
+}
+
+// The same for element rem = t C + c of sample row r of the tile.
 __device__ __forceinline__ float colored_value(const ColoredTile& ct, int r, int rem, int pstart)
 {
-  const int k = ct.k0 + r;
-  const float mu = ct.mean_l[rem], v = ct.stage[r * ct.TCS + rem];
-  const float u = (k >= pstart) ? v : mu + v;
-  return (k == 0 || rem < ct.sc) ? mu : u;
+  return colored_transform(ct.mean_l[rem], ct.stage[r * ct.TCS + rem], ct.k0 + r, rem < ct.sc, pstart);
 }
 
 }  // namespace mppi_amd

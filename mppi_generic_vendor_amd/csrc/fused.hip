@@ -149,29 +149,21 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
     const ColoredTile ct = colored_stage<CNTT>(sp, K, T, C, cpl.sh.n, cpl.sh.F, cpl.sh.Kpad, cpl.sh.KPH, cpl.sh.MT,
                                                cpl.sh.S, cpl.sh.spw, cpl.sh.TCS, mean, itr, nullptr, cpl.sig, cpl.bhi,
                                                cpl.blo, g, scratch);
-    const int n = nrows * TC;
-    // FU elements per thread and pass: their LDS reads are all issued before the first write
-    constexpr int FU = 4;
-    auto fill = [&](auto f) {
-      for (int e0 = tid; e0 < n; e0 += FU * NT) {
-        float v[FU];
-        int at[FU], col[FU];
-#pragma unroll
-        for (int u = 0; u < FU; ++u) {
-          const int e = min(e0 + u * NT, n - 1);
-          const int r = colored_row(ct, e);
-          col[u] = e - r * TC;
-          at[u] = r * ROW + col[u];
-          v[u] = colored_value(ct, r, col[u], pstart);
-        }
-#pragma unroll
-        for (int u = 0; u < FU; ++u)
-          if (e0 + u * NT < n) tile[at[u]] = f(col[u], v[u]);
-      }
-    };
-    if (deadband) fill(ControlConstrain<C>{&dyn});
-    else fill(ControlClamp<C>(dyn));
-    for (int j = tid; j < TC; j += NT) mu[j] = ct.mean_l[j];
+    // thread (row group, column): column col of rows r0, r0 + RG, ... (RG = NT / TC row groups), so that the mean,
+    // the component and the stride test are per thread
+    const int RG = NT / TC;
+    const int r0 = tid / TC, col = tid - r0 * TC;
+    if (r0 < RG) {
+      const float mu_c = ct.mean_l[col];
+      const bool head = col < ct.sc;
+      auto fill = [&](auto f) {
+        for (int r = r0; r < nrows; r += RG)
+          tile[r * ROW + col] = f(col, colored_transform(mu_c, ct.stage[r * ct.TCS + col], k0 + r, head, pstart));
+      };
+      if (deadband) fill(ControlConstrain<C>{&dyn});
+      else fill(ControlClamp<C>(dyn));
+      if (r0 == 0) mu[col] = mu_c;
+    }
     if (tid < M::S) x0_l[tid] = xv;
   } else {
     if (deadband) tile_load<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid, ControlConstrain<C>{&dyn});
