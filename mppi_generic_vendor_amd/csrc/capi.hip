@@ -699,11 +699,10 @@ static int controller_enqueue_proposal(mppi_controller* c)
   const int T = c->T, C = c->C, S = c->S;
   const hipStream_t s = c->stream;
   const mppi_controller_desc& d = c->d;
-  const WordSpan in[] = {{c->iter_hd, c->iter_d, words_of<IterArgs>(1)},
-                         {c->x0_hd, c->x0_d, S},
-                         {c->nominal_hd, c->mean_d, T * C}};
-  MPPI_CHECK(launch_copy_words(in, 3, s));
-  MPPI_CHECK(proposal_infer(c->prop, c->x0_d, c->mean_d, c->sigma_base_d, c->prop_dmu_d, c->prop_sig_d, s));
+  // the inference reads x0 and the nominal sequence from the host buffers and copies them (and the iteration
+  // arguments) on for the rollout
+  const InferStaging stg{c->x0_d, c->mean_d, c->iter_hd, c->iter_d};
+  MPPI_CHECK(proposal_infer(c->prop, c->x0_hd, c->nominal_hd, c->sigma_base_d, c->prop_dmu_d, c->prop_sig_d, s, &stg));
   ResultCopies out;
   out.mean = c->mean_out_hd;
   out.stats = c->stats_hd;

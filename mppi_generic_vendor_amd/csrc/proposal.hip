@@ -185,6 +185,28 @@ __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ th, float
   th[i] = th[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// BF16 copies of the three weight matrices for the controller's inference, rows padded with zeros to din8 (a multiple
+// of 8: 16-byte rows) for W1: W1 [PROP_H][din8], W2 [PROP_H][PROP_H], W3 [dout][PROP_H], each element the round to
+// nearest even of the FP32 parameter (the operand rounding of every BF16 product here).
+__global__ void __launch_bounds__(256) mirror_bf16_kernel(const float* __restrict__ th, PropDims d, int din8,
+                                                          __bf16* __restrict__ wb)
+{
+  const int n1 = PROP_H * din8, n2 = PROP_H * PROP_H, n3 = d.dout * PROP_H;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  float v = 0.0f;
+  if (i < n1) {
+    const int r = i / din8, c = i - r * din8;
+    v = c < d.din ? th[d.w1 + (size_t)r * d.din + c] : 0.0f;
+  } else if (i < n1 + n2) {
+    v = th[d.w2 + (i - n1)];
+  } else if (i < n1 + n2 + n3) {
+    v = th[d.w3 + (i - n1 - n2)];
+  } else {
+    return;
+  }
+  wb[i] = (__bf16)v;
+}
+
 // Network input from the state and the nominal sequence (16 knots t_j = j (T - 1) / 15), normalized.
 __global__ void proposal_input_kernel(int S, int C, int T, const float* __restrict__ x0,
                                       const float* __restrict__ nominal, const float* __restrict__ shift,
@@ -262,6 +284,23 @@ __device__ __forceinline__ void infer_load_tile(const float* __restrict__ W, int
   }
 }
 
+// The same fragments from a BF16 copy with row stride ld (a multiple of 8, zero beyond the layer's width): one 16-byte
+// load per k-step.
+template <int KS>
+__device__ __forceinline__ void infer_load_tile_b(const __bf16* __restrict__ W, int N, int ld, int mt,
+                                                  pbf16x8 (&a)[KS])
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
+  const int row = 16 * mt + l16;
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const int k = 32 * ks + 8 * g;
+    pbf16x8 v = {};
+    if (row < N && k < ld) v = *reinterpret_cast<const pbf16x8*>(W + (int64_t)row * ld + k);
+    a[ks] = v;
+  }
+}
+
 // One layer for the input vector vin (LDS, broadcast over the 16 B columns): vout[r - row0] = act(W[r] . vin + bias[r])
 // for the tiles whose A fragments `a` this wave holds (tile mt[i] for i < NT).
 template <int KS, int NT>
@@ -301,7 +340,7 @@ __global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
     int S, int C, int T, const float* __restrict__ th, int64_t w1, int64_t b1, int64_t w2, int64_t b2, int64_t w3,
     int64_t b3, const float* __restrict__ x0, const float* __restrict__ nominal, const float* __restrict__ shift,
     const float* __restrict__ scale, const float* __restrict__ sigma_base, float* __restrict__ dmu,
-    float* __restrict__ sig)
+    float* __restrict__ sig, const __bf16* __restrict__ wb, int din8, InferStaging stg)
 {
   __shared__ float xin[DIN];
   __shared__ float h1[PROP_H], h2[PROP_H];
@@ -313,9 +352,13 @@ __global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
   // every wave requests all the weights it will multiply before anything else (they do not depend on the input)
   const int mth[1] = {wave}, mt3[1] = {tile};
   pbf16x8 a1[1][KS1], a2[1][KSH], a3[1][KSH];
-  infer_load_tile<KS1>(th + w1, PROP_H, din, wave, a1[0]);
-  infer_load_tile<KSH>(th + w2, PROP_H, PROP_H, wave, a2[0]);
-  if (wave == 0) infer_load_tile<KSH>(th + w3, dout, PROP_H, tile, a3[0]);
+  // weights from the BF16 copy (the same operand values as rounding the FP32 parameters, half the bytes)
+  const __bf16* w1b = wb;
+  const __bf16* w2b = w1b + PROP_H * din8;
+  const __bf16* w3b = w2b + PROP_H * PROP_H;
+  infer_load_tile_b<KS1>(w1b, PROP_H, din8, wave, a1[0]);
+  infer_load_tile_b<KSH>(w2b, PROP_H, PROP_H, wave, a2[0]);
+  if (wave == 0) infer_load_tile_b<KSH>(w3b, dout, PROP_H, tile, a3[0]);
   for (int o = threadIdx.x; o < DIN; o += PI_THREADS) {
     float v = 0.0f;
     if (o < S) {
@@ -325,6 +368,11 @@ __global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
       v = (nominal[(j * (T - 1) / 15) * C + c] - shift[o]) * scale[o];
     }
     xin[o] = v;
+  }
+  if (stg.nominal_dst != nullptr && tile == 0) {
+    for (int j = threadIdx.x; j < TC; j += PI_THREADS) stg.nominal_dst[j] = nominal[j];
+    if (threadIdx.x < S) stg.x0_dst[threadIdx.x] = x0[threadIdx.x];
+    if (threadIdx.x == 0) *stg.iter_dst = *stg.iter_src;
   }
   __syncthreads();
   infer_apply<KS1, 1>(a1, mth, th + b1, PROP_H, din, xin, h1, true);
@@ -350,6 +398,8 @@ __global__ void __launch_bounds__(PI_THREADS) proposal_infer_kernel(
 struct ProposalNet {
   PropDims d{};
   float* theta = nullptr;
+  __bf16* wb = nullptr;  // BF16 weight copy for the inference (mirror_bf16_kernel), rewritten with theta
+  int din8 = 0;
   float* adam_m = nullptr;
   float* adam_v = nullptr;
   float* grad = nullptr;
@@ -442,14 +492,16 @@ static int prop_loss_grad(ProposalNet* p, const float* X, const float* R, int B,
 }
 
 int proposal_infer(ProposalNet* p, const float* x0, const float* nominal, const float* sigma_base, float* dmu,
-                   float* sig, hipStream_t s)
+                   float* sig, hipStream_t s, const InferStaging* staging)
 {
+  const InferStaging stg = staging != nullptr ? *staging : InferStaging{nullptr, nullptr, nullptr, nullptr};
   const PropDims& d = p->d;
   const int tiles = (d.dout + 15) / 16;
   if (tiles > PI_MAX_TILES) return fail(MPPI_E_SIZE, "proposal: output too wide for the inference kernel");
 #define MPPI_INFER(DIN)                                                                                              \
   proposal_infer_kernel<DIN><<<tiles, PI_THREADS, 0, s>>>(d.S, d.C, d.T, p->theta, d.w1, d.b1, d.w2, d.b2, d.w3, d.b3, \
-                                                          x0, nominal, p->shift, p->scale, sigma_base, dmu, sig)
+                                                          x0, nominal, p->shift, p->scale, sigma_base, dmu, sig,  \
+                                                          p->wb, p->din8, stg)
   if (d.din <= 32) MPPI_INFER(32);
   else if (d.din <= 64) MPPI_INFER(64);
   else MPPI_INFER(96);
@@ -483,11 +535,21 @@ struct mppi_proposal {
 ProposalNet* mppi_amd::proposal_net_of(mppi_proposal* q) { return &q->net; }
 PropDims mppi_amd::proposal_dims(const ProposalNet* p) { return p->d; }
 
+// Rewrites the BF16 weight copy from theta on stream s.
+static int prop_mirror(ProposalNet* p, hipStream_t s)
+{
+  const int n = PROP_H * p->din8 + PROP_H * PROP_H + p->d.dout * PROP_H;
+  mirror_bf16_kernel<<<(n + 255) / 256, 256, 0, s>>>(p->theta, p->d, p->din8, p->wb);
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
 static void proposal_free(mppi_proposal* q)
 {
   if (!q) return;
   ProposalNet* p = &q->net;
   prop_free_acts(p);
+  if (p->wb) (void)hipFree(p->wb);
   for (float* x : {p->theta, p->adam_m, p->adam_v, p->grad, p->shift, p->scale})
     if (x) (void)hipFree(x);
   if (p->loss) (void)hipFree(p->loss);
@@ -522,6 +584,9 @@ int mppi_proposal_create(int32_t S, int32_t C, int32_t T, const float* params_ho
     if (st == MPPI_OK) st = s2;
   };
   A(dev_alloc(&p->theta, sizeof(float) * P));
+  p->din8 = (p->d.din + 7) & ~7;
+  A(dev_alloc(&p->wb, sizeof(__bf16) * ((size_t)PROP_H * p->din8 + (size_t)PROP_H * PROP_H +
+                                        (size_t)p->d.dout * PROP_H)));
   A(dev_alloc(&p->adam_m, sizeof(float) * P));
   A(dev_alloc(&p->adam_v, sizeof(float) * P));
   A(dev_alloc(&p->grad, sizeof(float) * P));
@@ -545,6 +610,8 @@ int mppi_proposal_create(int32_t S, int32_t C, int32_t T, const float* params_ho
     std::vector<float> ones(p->d.din, 1.0f);
     e = hipMemcpy(p->scale, ones.data(), sizeof(float) * p->d.din, hipMemcpyHostToDevice);
   }
+  if (e == hipSuccess && prop_mirror(p, nullptr) != MPPI_OK) e = hipErrorLaunchFailure;
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     proposal_free(q);
     return hip_fail(e, "proposal: init");
@@ -573,6 +640,8 @@ int mppi_proposal_set_params(mppi_proposal* q, const float* host)
   if (!q || !host) return fail(MPPI_E_INVALID, "null argument");
   MPPI_HIP_CHECK(hipDeviceSynchronize());
   MPPI_HIP_CHECK(hipMemcpy(q->net.theta, host, sizeof(float) * q->net.d.total, hipMemcpyHostToDevice));
+  MPPI_CHECK(prop_mirror(&q->net, nullptr));
+  MPPI_HIP_CHECK(hipDeviceSynchronize());
   MPPI_HIP_CHECK(hipMemset(q->net.adam_m, 0, sizeof(float) * q->net.d.total));
   MPPI_HIP_CHECK(hipMemset(q->net.adam_v, 0, sizeof(float) * q->net.d.total));
   q->net.step = 0;
@@ -641,6 +710,7 @@ int mppi_proposal_train_step(mppi_proposal* q, const float* inputs_d, const floa
   const int P = (int)p->d.total;
   adam_kernel<<<(P + 255) / 256, 256, 0, s>>>(p->theta, p->adam_m, p->adam_v, p->grad, P, lr, b1, b2, 1e-8f, c1, c2);
   MPPI_HIP_CHECK(hipGetLastError());
+  MPPI_CHECK(prop_mirror(p, s));
   if (loss_host) {
     MPPI_HIP_CHECK(hipMemcpyAsync(p->loss_h, p->loss, sizeof(double), hipMemcpyDeviceToHost, s));
     MPPI_HIP_CHECK(hipStreamSynchronize(s));
