@@ -241,20 +241,33 @@ def test_controller_iteration_matches_oracle(hip, engine, name, graph):
     ctrl.close()
 
 
-@pytest.mark.parametrize("deadband", [False, True])
-def test_controller_colored_samples_equal_sampler(hip, engine, deadband):
-    """The controller draws colored samples inside its rollout kernel (32 samples per workgroup; K = 1000 leaves a
-    ragged last one); stored back with keep_samples they equal the colored sampler's samples, constrained, bit for
-    bit -- with and without a control deadband."""
+@pytest.mark.parametrize("model,deadband", [("di", False), ("di", True), ("cartpole", False)])
+def test_controller_colored_samples_equal_sampler(hip, engine, model, deadband):
+    """The controller draws colored samples inside its rollout kernel (32 samples per workgroup for the double
+    integrator's two controls, 64 for the cart-pole's one with a length-T transform; K = 1000 leaves a ragged last
+    workgroup); stored back with keep_samples they equal the colored sampler's samples, constrained, bit for bit --
+    with and without a control deadband."""
     from mppi_generic_vendor_amd.hip import D2H, _load
 
-    prob = workloads.double_integrator_map(num_rollouts=1000, num_timesteps=100, cost_map=SMALL_MAP)
+    if model == "di":
+        prob = workloads.double_integrator_map(num_rollouts=1000, num_timesteps=100, cost_map=SMALL_MAP)
+    else:
+        prob = workloads.cartpole(num_rollouts=1000, num_timesteps=100)
+        smp = capi.sampler_defaults(capi.SAMPLER_COLORED, 1)
+        smp.exponents[0] = 1.5
+        smp.std_dev[0] = 1.0
+        smp.control_cost_coeff[0] = 0.1
+        smp.fft_len_mode = capi.FFT_LEN_T
+        smp.offset_decay_rate = 0.0
+        smp.seed = 11
+        prob.sampler = smp
+        prob.dyn.control_min[0], prob.dyn.control_max[0] = -2.0, 2.0
     if deadband:
         for c in range(prob.C):
             prob.dyn.control_deadband[c] = 0.05
             prob.dyn.zero_control[c] = 0.01
     K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
-    banner(f"controller colored samples K={K} deadband={deadband}")
+    banner(f"controller colored samples {model} K={K} deadband={deadband}")
     ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=False, keep_samples=True)
     nominal = np.random.default_rng(5).normal(scale=0.2, size=(T, C)).astype(np.float32)
     ctrl.updateImportanceSampler(nominal)
@@ -263,7 +276,7 @@ def test_controller_colored_samples_equal_sampler(hip, engine, deadband):
     got = np.empty((K, T, C), np.float32)
     assert _load().hipMemcpy(got.ctypes.data, ptr, got.nbytes, D2H) == 0
     want = ol.enforce(prob, gpu_sample_colored(prob.sampler, K, T, C, nominal, 1, 0, rng_offset=0))
-    print(f"[parity] colored in-kernel vs sampler: max |diff| {float(np.max(np.abs(got - want))):.3g}")
+    print(f"[parity] colored in-kernel vs sampler ({model}): max |diff| {float(np.max(np.abs(got - want))):.3g}")
     assert np.array_equal(got, want)
     ctrl.close()
 
