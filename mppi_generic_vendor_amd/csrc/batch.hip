@@ -480,7 +480,11 @@ struct mppi_batch {
   hipGraphExec_t loop_exec = nullptr;
   int eager_runs = 0;
   uint64_t rng_counter = 0;
-  std::vector<std::vector<float>> control, raw_mean, state_seq;
+  std::vector<std::vector<float>> control, raw_mean;
+  // per instance, propagated on the first request after a call from that call's x0 and smoothed controls
+  mutable std::vector<std::vector<float>> state_seq;
+  std::vector<std::vector<float>> state_x0, state_u;
+  mutable std::vector<char> state_stale;
   std::vector<mppi_weight_stats> stats;
   std::vector<mppi_free_energy> fe;
 };
@@ -660,7 +664,10 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
     if (d.init_control) std::copy(d.init_control, d.init_control + (size_t)T * C, b->control.back().begin());
     b->raw_mean.emplace_back((size_t)T * C, 0.0f);
     b->state_seq.emplace_back((size_t)T * S, 0.0f);
+    b->state_x0.emplace_back();
+    b->state_u.emplace_back();
   }
+  b->state_stale.assign(n, 0);
   b->stats.assign(n, mppi_weight_stats{});
   b->fe.assign(n, mppi_free_energy{});
   int dev = 0;
@@ -816,8 +823,9 @@ int mppi_batch_compute_control(mppi_batch* b, const float* x0_host, int32_t stri
     b->raw_mean[i].assign(b->mean_out_h + b->mean_base[i], b->mean_out_h + b->mean_base[i] + (size_t)T * C);
     b->control[i] = b->raw_mean[i];
     smooth_control(b->control[i], T, C);
-    propagate(b->desc[i].dynamics, b->x0_h + i * BATCH_MAX_STATE, b->control[i].data(), T, b->desc[i].dt,
-              b->state_seq[i].data());
+    b->state_x0[i].assign(b->x0_h + i * BATCH_MAX_STATE, b->x0_h + (i + 1) * BATCH_MAX_STATE);
+    b->state_u[i] = b->control[i];
+    b->state_stale[i] = 1;
     clamp_controls_seq(b->desc[i].dynamics, b->control[i].data(), T);
   }
   return MPPI_OK;
@@ -878,6 +886,11 @@ int mppi_batch_get_state_seq(const mppi_batch* b, int32_t i, float* out)
 {
   MPPI_CHECK(batch_index(b, i));
   if (!out) return fail(MPPI_E_INVALID, "null argument");
+  if (b->state_stale[i]) {
+    propagate(b->desc[i].dynamics, b->state_x0[i].data(), b->state_u[i].data(), b->T[i], b->desc[i].dt,
+              b->state_seq[i].data());
+    b->state_stale[i] = 0;
+  }
   std::copy(b->state_seq[i].begin(), b->state_seq[i].end(), out);
   return MPPI_OK;
 }

@@ -245,7 +245,12 @@ struct mppi_controller {
   float* nominal_hd = nullptr;
   float* mean_out_hd = nullptr;
   mppi_weight_stats* stats_hd = nullptr;
-  std::vector<float> control, raw_mean, state_seq, nn_host;
+  std::vector<float> control, raw_mean, nn_host;
+  // getTargetStateSeq: propagated on the first request after a call, from that call's x0 and smoothed (not yet
+  // constrained) controls
+  mutable std::vector<float> state_seq;
+  std::vector<float> state_x0, state_u;
+  mutable bool state_stale = false;
   mppi_weight_stats stats{};
   mppi_free_energy fe{};
   uint64_t rng_counter = 0;
@@ -714,7 +719,8 @@ static int controller_enqueue_proposal(mppi_controller* c)
                                 &out);
 }
 
-// Runs one enqueue function eagerly on its first use and from a captured hipGraph afterwards (when use_graph).
+// Runs one enqueue function eagerly on its first use and from a captured hipGraph afterwards (when use_graph), then
+// records ev1 on the stream (the end of the call's GPU work) and waits for it.
 static int controller_run(mppi_controller* c, int (*enqueue)(mppi_controller*), hipGraph_t* graph,
                           hipGraphExec_t* exec, int* runs)
 {
@@ -737,7 +743,8 @@ static int controller_run(mppi_controller* c, int (*enqueue)(mppi_controller*), 
     MPPI_CHECK(enqueue(c));
     (*runs)++;
   }
-  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  MPPI_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  MPPI_HIP_CHECK(hipEventSynchronize(c->ev1));
   return MPPI_OK;
 }
 
@@ -790,17 +797,17 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
     MPPI_CHECK(controller_run(c, controller_enqueue, &c->graph, &c->graph_exec, &c->eager_runs));
     K_used = K;
   }
-  MPPI_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
-  MPPI_HIP_CHECK(hipEventSynchronize(c->ev1));
+  // stream time of the call: from ev0 (before the first launch) to ev1 (recorded right after the last one)
   MPPI_HIP_CHECK(hipEventElapsedTime(&c->last_gpu_ms, c->ev0, c->ev1));
   c->stats = *c->stats_h;
   c->raw_mean.assign(c->mean_out_h, c->mean_out_h + (size_t)T * C);
   c->control = c->raw_mean;
   free_energy_from_stats(c->stats, d.lambda, K_used, &c->fe);
   smooth_control(c->control, T, C);
-  mppi_dynamics_params hostdyn = d.dynamics;
-  if (!c->nn_host.empty()) hostdyn.nn_theta = c->nn_host.data();
-  propagate(hostdyn, c->x0_h, c->control.data(), T, d.dt, c->state_seq.data());
+  // the state trajectory of the smoothed controls (computeStateTrajectory) is propagated when first requested
+  c->state_x0.assign(c->x0_h, c->x0_h + S);
+  c->state_u = c->control;
+  c->state_stale = true;
   clamp_controls_seq(d.dynamics, c->control.data(), T);
   return MPPI_OK;
 }
@@ -915,6 +922,12 @@ int mppi_controller_get_raw_mean(const mppi_controller* c, float* out)
 int mppi_controller_get_state_seq(const mppi_controller* c, float* out)
 {
   if (!c || !out) return fail(MPPI_E_INVALID, "null argument");
+  if (c->state_stale) {
+    mppi_dynamics_params hostdyn = c->d.dynamics;
+    if (!c->nn_host.empty()) hostdyn.nn_theta = c->nn_host.data();
+    propagate(hostdyn, c->state_x0.data(), c->state_u.data(), c->T, c->d.dt, c->state_seq.data());
+    c->state_stale = false;
+  }
   std::copy(c->state_seq.begin(), c->state_seq.end(), out);
   return MPPI_OK;
 }

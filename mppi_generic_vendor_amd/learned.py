@@ -15,6 +15,7 @@ from __future__ import annotations
 import ctypes as ct
 import dataclasses
 import statistics
+import time
 import threading
 
 import numpy as np
@@ -147,8 +148,10 @@ def train(ds: Dataset, sigma: np.ndarray, epochs: int = 30, batch: int = 512, lr
     return net
 
 
-def closed_loop(prob: workloads.Problem, ctrl: VanillaMPPIController, starts: list[np.ndarray], steps: int):
-    """Executed-trajectory cost per episode and the stream time of every control step."""
+def closed_loop(prob: workloads.Problem, ctrl: VanillaMPPIController, starts: list[np.ndarray], steps: int,
+                wall: list | None = None):
+    """Executed-trajectory cost per episode and the stream time of every control step (appended to `wall`: the
+    host wall time of every computeControl call)."""
     plant = Plant(prob)
     T, C = prob.num_timesteps, prob.C
     costs, ms = [], []
@@ -157,7 +160,10 @@ def closed_loop(prob: workloads.Problem, ctrl: VanillaMPPIController, starts: li
         ctrl.updateImportanceSampler(np.zeros((T, C), np.float32))
         total = 0.0
         for _ in range(steps):
+            t0 = time.perf_counter()
             ctrl.computeControl(x, 1)
+            if wall is not None:
+                wall.append((time.perf_counter() - t0) * 1e3)
             ms.append(ctrl.lastCallMs())
             x = plant.step(x, ctrl.getControlSeq()[0])
             total += plant.cost_of(x)
@@ -184,9 +190,11 @@ def evaluate(net: ProposalNet | None, episodes: int = 10, steps: int = 100, seed
             ctrl.setProposal(net, proposal_rollouts, fallback_tolerance=fallback_tolerance)
         ctrl.computeControl(starts[0], 1)  # warm-up (graph capture)
         ctrl.computeControl(starts[0], 1)
-        c, ms = closed_loop(prob, ctrl, starts, steps)
+        wall = []
+        c, ms = closed_loop(prob, ctrl, starts, steps, wall)
         entry = {"cost": float(c.mean()), "costs": c.tolist(), "ms_per_call": float(np.mean(ms)),
                  "ms_median": float(statistics.median(ms)), "hz": 1e3 / float(np.mean(ms)),
+                 "ms_host_median": float(statistics.median(wall)), "hz_host": 1e3 / float(statistics.median(wall)),
                  "samples": proposal_rollouts if learned else K, "iterations": 1 if learned else iters}
         if learned:
             n, f, _, _ = ctrl.getProposalStats()
