@@ -300,7 +300,8 @@ int mppi_controller_step_device(mppi_controller* ctrl, int32_t num_calls);
 int mppi_controller_get_control_seq(const mppi_controller* ctrl, float* out_host);
 /* Raw (pre-smoothing) weighted mean of the last iteration [T][C]. */
 int mppi_controller_get_raw_mean(const mppi_controller* ctrl, float* out_host);
-/* getTargetStateSeq(): [T][S] propagated state trajectory of the returned controls. */
+/* getTargetStateSeq(): [T][S] state trajectory of the last call's smoothed controls from its x0 (computeStateTrajectory,
+ * controllers/MPPI/mppi_controller.cu:222), propagated on the host on the first request after each call. */
 int mppi_controller_get_state_seq(const mppi_controller* ctrl, float* out_host);
 int mppi_controller_get_weight_stats(const mppi_controller* ctrl, mppi_weight_stats* out_host);
 int mppi_controller_get_free_energy(const mppi_controller* ctrl, mppi_free_energy* out_host);
