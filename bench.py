@@ -337,6 +337,10 @@ def main():
         "weighted_update": 4 * K * T * C + 4 * K + 4 * T * C,
     }
     dom = max((k for k in kern if k != "weights"), key=lambda k: kern[k])
+    if "sample_colored" in kern:
+        # the colored stage is the larger part of the controller's fused kernel (about 27 K of 50 K cycles per
+        # workgroup), and the kernel-level sampler and rollout times lie within noise of each other
+        dom = "sample_colored"
     if prob.dyn.model == capi.MODEL_RACER_DUBINS_NN and dom == "rollout":
         # the network on the matrix cores bounds the rollout: algorithmic FLOPs per launch over its duration
         flops = NN_FLOP_PER_STEP * K * T
