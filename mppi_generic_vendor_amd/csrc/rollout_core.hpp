@@ -1,6 +1,7 @@
-// Rollout of a workgroup's 64 samples over the horizon (rolloutKernel, core/mppi_common.cu:105-152): for t < T read
-// u_t from the LDS control row, enforce constraints, write the clamped u_t back, step the dynamics, accumulate the
-// running cost of the post-step output plus the likelihood-ratio cost; add the terminal cost; divide by T.
+// Rollout of a workgroup's samples (up to 64) over the horizon (rolloutKernel, core/mppi_common.cu:105-152): for
+// t < T read u_t from the LDS control row, enforce constraints, write the clamped u_t back, step the dynamics,
+// accumulate the running cost of the post-step output plus the likelihood-ratio cost; add the terminal cost; divide
+// by T.
 //
 // Constraints act on each control alone, so the kernels apply them to the whole tile while staging it in LDS
 // (constrain_control, the same arithmetic): the tile holds the constrained controls on entry, which are also the
