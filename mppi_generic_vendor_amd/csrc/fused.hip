@@ -52,7 +52,7 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
     mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int K, int T, float dt, float lambda,
     float alpha, const float* __restrict__ x0, const float* __restrict__ mean, float* __restrict__ samples,
     IterRef itr, int pstart, float* __restrict__ costs_out, Partial* __restrict__ parts, float* __restrict__ sums,
-    int chunk, const float* __restrict__ prop_dmu, const float* __restrict__ prop_sig, ColoredPlan cpl)
+    int chunk, const float* __restrict__ prop_dmu, const float* __restrict__ prop_sig, ColoredPlan cpl, int rows_arg)
 {
   using M = Model<MODEL>;
   constexpr int C = M::C;
@@ -65,7 +65,8 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
   const int TC = T * C;
   const int TC4 = (TC + 3) & ~3;
   const int ROW = TC + 1;
-  const int rows = CNTT > 0 ? cpl.sh.spw : ROWS;  // samples per workgroup
+  // samples per workgroup: the colored shape's, the split variant's 16, else the launcher's choice (16, 32 or 64)
+  const int rows = CNTT > 0 ? cpl.sh.spw : (SPLIT ? ROWS : rows_arg);
   float* tile = lds;
   float* mu = tile + ((rows * ROW + 3) & ~3);
   float* pl = mu + TC4;                    // PROP: dmu, sig_q [2][TC4]
@@ -390,9 +391,9 @@ template <int MODEL, int COST, bool GEN, bool WRITE, bool PROP, bool SPLIT, int 
 static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,
                      int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
                      float* samples, IterRef it, float* costs_out, Partial* parts, float* sums, const float* prop_dmu,
-                     const float* prop_sig, const ColoredPlan* cpl, hipStream_t s)
+                     const float* prop_sig, const ColoredPlan* cpl, int rows_sel, hipStream_t s)
 {
-  const int rows = CNTT > 0 ? cpl->sh.spw : ru_rows<SPLIT>();
+  const int rows = CNTT > 0 ? cpl->sh.spw : (SPLIT ? ru_rows<SPLIT>() : rows_sel);
   int chunk = 0;
   const size_t lf = ru_lds_floats<MODEL, PROP, SPLIT>(T, rows, CNTT > 0 ? cpl->sh.lds_bytes : 0, &chunk);
   if (lf == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
@@ -408,7 +409,7 @@ static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& co
       <<<G, ru_threads<MODEL, SPLIT>(), sizeof(float) * lf, s>>>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean,
                                                                  samples, it, pstart, costs_out, parts, sums, chunk,
                                                                  prop_dmu, prop_sig,
-                                                                 CNTT > 0 ? *cpl : ColoredPlan{});
+                                                                 CNTT > 0 ? *cpl : ColoredPlan{}, rows);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
@@ -418,11 +419,11 @@ static int dispatch_flags(bool gen, bool write, const mppi_dynamics_params& dyn,
                           const mppi_sampler_params& sp, int K, int T, float dt, float lambda, float alpha,
                           const float* x0, const float* mean, float* samples, IterRef it, float* costs_out,
                           Partial* parts, float* sums, const float* prop_dmu, const float* prop_sig,
-                          const ColoredPlan* cpl, hipStream_t s)
+                          const ColoredPlan* cpl, int rows, hipStream_t s)
 {
 #define MPPI_LAUNCH_RU(GEN, WRITE, PROP)                                                                               \
   launch_ru<MODEL, COST, GEN, WRITE, PROP, SPLIT, CNTT>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, \
-                                                        costs_out, parts, sums, prop_dmu, prop_sig, cpl, s)
+                                                        costs_out, parts, sums, prop_dmu, prop_sig, cpl, rows, s)
   if constexpr (CNTT > 0) {
     return write ? MPPI_LAUNCH_RU(false, true, false) : MPPI_LAUNCH_RU(false, false, false);
   } else {
@@ -460,7 +461,18 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
   // workgroup (one wavefront per SIMD), splitting each step's network over four wavefronts finishes sooner; beyond
   // that, the 64-sample pipeline does the same work with a quarter of the replicated dynamics
   const bool split = model == MODEL_RACER_DUBINS_NN_BF16 && (K + NS_ROWS - 1) / NS_ROWS <= device_cu_count();
-  const int rows = inline_colored ? colored->sh.spw : (split ? NS_ROWS : RC_LANES);
+  // samples per workgroup: the colored shape's or the split variant's; otherwise (1024-thread workgroups) the smallest
+  // of 64, 32, 16 with at most one workgroup per CU -- the state chain takes as long for 16 samples as for 64, while
+  // drawing the tile and reducing it scale with the samples
+  int rows = RC_LANES;
+  if (inline_colored) {
+    rows = colored->sh.spw;
+  } else if (split) {
+    rows = NS_ROWS;
+  } else if (model != MODEL_RACER_DUBINS_NN_BF16) {
+    const int cus = device_cu_count();
+    while (rows > NS_ROWS && (K + rows / 2 - 1) / (rows / 2) <= cus) rows /= 2;
+  }
   const int G = (K + rows - 1) / rows;
   if (G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "fused iteration: too many samples");
   Partial* parts = static_cast<Partial*>(ws);
@@ -468,7 +480,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
   int st = MPPI_E_UNSUPPORTED;
 #define MPPI_FUSED_ARGS                                                                                         \
   generate, write_samples, dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean, samples, it, costs_out, parts, sums, \
-      prop_dmu, prop_sig, colored, s
+      prop_dmu, prop_sig, colored, rows, s
 #define MPPI_FUSED_CASE(MODEL, COST)                                                                   \
   if (model == MODEL && cost.cost == COST) {                                                           \
     constexpr bool NN16 = MODEL == MODEL_RACER_DUBINS_NN_BF16;                                         \
