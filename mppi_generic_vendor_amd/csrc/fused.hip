@@ -228,8 +228,11 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
 
 // Combination of G partials: workgroup q owns COMBINE_COLS columns; every workgroup recomputes the global baseline
 // and the scales e_g = exp(-(b_g - b)/lambda) (G <= MAX_PARTIALS values), workgroup 0 also writes the statistics.
+// Each thread's first COMBINE_PREFETCH column values are requested before the partials, so that their latency
+// overlaps the baseline and normalizer reductions.
 constexpr int COMBINE_COLS = 8;
 constexpr int COMBINE_THREADS = 256;
+constexpr int COMBINE_PREFETCH = 8;
 constexpr int MAX_PARTIALS = 4096;
 
 __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial* __restrict__ parts,
@@ -246,6 +249,16 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   constexpr int PER = MAX_PARTIALS / COMBINE_THREADS;
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const double inv = 1.0 / (double)lambda;
+  // columns of this workgroup: thread (grp, col) sums partials g = grp, grp + NG, ...
+  constexpr int NG = COMBINE_THREADS / COMBINE_COLS;
+  const int col = tid % COMBINE_COLS, grp = tid / COMBINE_COLS;
+  const int j = blockIdx.x * COMBINE_COLS + col;
+  float pre[COMBINE_PREFETCH];
+#pragma unroll
+  for (int i = 0; i < COMBINE_PREFETCH; ++i) {
+    const int q = grp + i * NG;
+    pre[i] = (j < TC && q < G) ? sums[(size_t)q * TC + j] : 0.0f;
+  }
   Partial mine[PER];
   float v = INFINITY;
   int idx = INT_MAX;
@@ -294,14 +307,15 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
     s2 += se2[q];
     wcost += sec[q];
   }
-  // columns of this workgroup: thread (grp, col) sums partials g = grp, grp + NG, ...
-  constexpr int NG = COMBINE_THREADS / COMBINE_COLS;
-  const int col = tid % COMBINE_COLS, grp = tid / COMBINE_COLS;
-  const int j = blockIdx.x * COMBINE_COLS + col;
   float acc = 0.0f;
   if (j < TC) {
+#pragma unroll
+    for (int i = 0; i < COMBINE_PREFETCH; ++i) {
+      const int q = grp + i * NG;
+      if (q < G) acc += escale[q] * pre[i];
+    }
 #pragma unroll 8
-    for (int q = grp; q < G; q += NG) acc += escale[q] * sums[(size_t)q * TC + j];
+    for (int q = grp + COMBINE_PREFETCH * NG; q < G; q += NG) acc += escale[q] * sums[(size_t)q * TC + j];
   }
   acc_l[grp][col] = acc;
   __syncthreads();
