@@ -106,6 +106,10 @@ def _problems():
         "bicycle_nn_bf16_wide": lambda: workloads.bicycle_nn(num_rollouts=16384, num_timesteps=100,
                                                              cost_map=SMALL_MAP, precision=capi.NN_BF16),
         "cartpole_ragged": lambda: workloads.cartpole(num_rollouts=100, num_timesteps=7),
+        # controller workgroup sizes of the Gaussian iteration on a 256-CU MI355X: 16 samples at K = 2048 and 100
+        # (ragged), 32 at K = 8192, 64 at K = 16500 (ragged last workgroup of 52)
+        "cartpole_k8192": lambda: workloads.cartpole(num_rollouts=8192, num_timesteps=100),
+        "cartpole_k16500": lambda: workloads.cartpole(num_rollouts=16500, num_timesteps=100),
         "di_single": lambda: workloads.double_integrator_map(num_rollouts=1, num_timesteps=1, colored=False,
                                                              cost_map=SMALL_MAP),
     }
@@ -117,8 +121,11 @@ def _problems():
 COST_RTOL = {"bicycle_nn_bf16": 1e-3, "bicycle_nn_bf16_short": 1e-3, "bicycle_nn_bf16_wide": 1e-3}
 
 
-# (the two extra BF16-network sizes only select controller launch shapes; the rollout kernel is the same)
-@pytest.mark.parametrize("name", [n for n in _problems() if n not in ("bicycle_nn_bf16_short", "bicycle_nn_bf16_wide")])
+# the extra sizes only select controller launch shapes; the rollout kernel is the same
+CONTROLLER_SHAPES = ("bicycle_nn_bf16_short", "bicycle_nn_bf16_wide", "cartpole_k8192", "cartpole_k16500")
+
+
+@pytest.mark.parametrize("name", [n for n in _problems() if n not in CONTROLLER_SHAPES])
 def test_rollout_matches_oracle(hip, engine, name):
     prob = _problems()[name]()
     if name == "cartpole":
@@ -199,7 +206,9 @@ def _oracle_iteration(prob, samples, mean):
 
 @pytest.mark.parametrize("name,graph", [("cartpole", False), ("cartpole", True), ("di_map_colored", True),
                                         ("bicycle_nn_fp32", False), ("bicycle_nn_bf16", True),
-                                        ("bicycle_nn_bf16_short", False), ("bicycle_nn_bf16_wide", False)])
+                                        ("bicycle_nn_bf16_short", False), ("bicycle_nn_bf16_wide", False),
+                                        ("cartpole_ragged", True), ("cartpole_k8192", False),
+                                        ("cartpole_k16500", True)])
 def test_controller_iteration_matches_oracle(hip, engine, name, graph):
     prob = _problems()[name]()
     K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
