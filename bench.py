@@ -52,6 +52,24 @@ def measured_traffic(workload: str, entry: str):
     return total
 
 
+def measured_utilisation(workload: str, prefixes):
+    """VALU and MFMA utilisation of this workload's kernels whose names start with one of `prefixes`, from the
+    committed rocprofv3 counter summary (derived counters VALUBusy: VALU issue time as a percentage of the CUs'
+    capacity; VALUUtilization: active lanes per VALU instruction; MfmaUtil: matrix-core busy cycles as a percentage
+    of SIMD cycles), or None when there is none."""
+    path = ROOT / "profiles" / f"utilisation_{workload}.json"
+    if not path.exists():
+        return None
+    tab = json.loads(path.read_text())
+    out = {k: {"valu_busy_pct": round(v["VALUBusy"], 1), "valu_lanes_pct": round(v["VALUUtilization"], 1),
+               "mfma_busy_pct": round(v["MfmaUtil"], 1)}
+           for k, v in sorted(tab.items()) if any(k.startswith(p) for p in prefixes)}
+    if not out:
+        return None
+    out["source"] = f"profiles/utilisation_{workload}.json (rocprofv3 VALUBusy, VALUUtilization, MfmaUtil)"
+    return out
+
+
 def build_problem(name: str):
     from mppi_generic_vendor_amd import capi, workloads
 
@@ -238,6 +256,8 @@ def run_batch(args):
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes": alg,
                      "avg_ms": round(ev_ms, 5)},
+        "utilisation": measured_utilisation(
+            "batch256", [f"batch_rollout_kernel<{cfg.threads}, {cfg.samples_per_lane}>"]),
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -383,6 +403,8 @@ def main():
         "control_hz_host": round(control_hz_host, 3),
         "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
         "roofline": roofline,
+        # the controller iteration's fused kernel and the kernel behind the roofline entry point
+        "utilisation": measured_utilisation(args.workload, ["rollout_update_kernel<"] + KERNEL_NAMES[dom]),
         "cpu_baseline": cpu,
         "learned_vs_plain": learned_cmp,
     }

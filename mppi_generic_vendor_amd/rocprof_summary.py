@@ -2,12 +2,16 @@
 
     python -m mppi_generic_vendor_amd.rocprof_summary TRACE_DB [FETCH_DB WRITE_DB] [--calibration-mib 256]
                                                      [--json OUT.json]
+    python -m mppi_generic_vendor_amd.rocprof_summary --counters NAME[,NAME...] PMC_DB [--json OUT.json]
 
 TRACE_DB comes from `rocprofv3 --kernel-trace --stats`, FETCH_DB from `rocprofv3 --pmc FETCH_SIZE --kernel-trace`,
 WRITE_DB from `rocprofv3 --pmc WRITE_SIZE --kernel-trace`, all of a run that also performs one device-to-device copy
 of `calibration-mib` MiB (bench.py does). The copy reads and writes exactly that many bytes, so the counter value it
 produces divided by its size is the factor that converts the counter into HBM bytes; every kernel's median counter
 value is divided by that factor.
+
+With --counters, PMC_DB comes from `rocprofv3 --pmc NAME ... --kernel-trace` and the table lists, per kernel, the median
+over its dispatches of each named counter (raw or derived, e.g. VALUBusy, VALUUtilization, MfmaUtil).
 """
 from __future__ import annotations
 
@@ -68,6 +72,16 @@ def summarise(trace_db: str, fetch_db: str, write_db: str, calibration_mib: int)
             "traffic": traffic}
 
 
+def counter_table(path: str, names):
+    """{kernel: {counter: (median over dispatches, dispatches)}} for the named counters."""
+    out = defaultdict(dict)
+    for c in names:
+        for k, vals in counter_values(path, c).items():
+            v = sorted(vals)
+            out[k][c] = (v[len(v) // 2], len(v))
+    return dict(out)
+
+
 def render(s) -> str:
     out = ["kernel-trace statistics (average over all dispatches of the run):"]
     for name, calls, avg, pct in s["kernels"]:
@@ -83,12 +97,25 @@ def render(s) -> str:
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--counters", default=None, help="comma-separated counter names: print their per-kernel medians")
     ap.add_argument("trace_db")
     ap.add_argument("fetch_db", nargs="?")
     ap.add_argument("write_db", nargs="?")
     ap.add_argument("--calibration-mib", type=int, default=256)
     ap.add_argument("--json", default=None, help="also write the traffic table as JSON")
     a = ap.parse_args()
+    if a.counters:
+        names = [c for c in a.counters.split(",") if c]
+        table = counter_table(a.trace_db, names)
+        print("per-kernel median over dispatches: " + ", ".join(names))
+        for k in sorted(table):
+            cells = "  ".join(f"{c} {table[k][c][0]:10.3f}" if c in table[k] else f"{c} {'-':>10}" for c in names)
+            n = max(v[1] for v in table[k].values())
+            print(f"  {cells}  ({n} dispatches)  {k}")
+        if a.json:
+            with open(a.json, "w") as fh:
+                json.dump({k: {c: v[0] for c, v in t.items()} for k, t in table.items()}, fh, indent=1)
+        return
     if a.fetch_db is None or a.write_db is None:
         print("kernel-trace statistics (average over all dispatches of the run):")
         for name, calls, avg, pct in kernel_stats(a.trace_db):
