@@ -1,10 +1,11 @@
 // Fused MPPI iteration for the controller path:
-//   rollout_update_kernel  -- per workgroup of samples (64; 16 for the small-grid BF16-network variant; the colored
-//                             sampler's 64 / C series for in-place colored samples): the control tile is drawn into
-//                             LDS (Gaussian from Philox, colored by the sampler's workgroup body) or loaded from the
-//                             sample buffer, the samples are rolled out over the horizon, then the workgroup reduces
-//                             its clamped tile against a local baseline into (b_g, argmin_g, eta_g, sum w^2_g,
-//                             S_g[T*C]).
+//   rollout_update_kernel  -- per workgroup of samples (16, 32 or 64: the smallest with at most one workgroup per
+//                             CU, else 64; 16 for the small-grid BF16-network variant, 64 for its large grids; the
+//                             colored sampler's 64 / C series for in-place colored samples): the control tile is
+//                             drawn into LDS (Gaussian from Philox, colored by the sampler's workgroup body) or loaded
+//                             from the sample buffer, the samples are rolled out over the horizon, then the workgroup
+//                             reduces its clamped tile against a local baseline into (b_g, argmin_g, eta_g,
+//                             sum w^2_g, S_g[T*C]).
 //   combine_kernel         -- log-sum-exp combination of the partials: global baseline (first strict minimum),
 //                             FP64 normalizer, new mean S / eta.
 // The result equals fullGPUcomputeWeights + weightedReductionKernelDeviceNorm (core/mppi_common.cu:746-773,
