@@ -110,6 +110,12 @@ def _problems():
         # (ragged), 32 at K = 8192, 64 at K = 16500 (ragged last workgroup of 52)
         "cartpole_k8192": lambda: workloads.cartpole(num_rollouts=8192, num_timesteps=100),
         "cartpole_k16500": lambda: workloads.cartpole(num_rollouts=16500, num_timesteps=100),
+        # the longest horizon (MPPI_MAX_TIMESTEPS = 256): 64-sample workgroups whose control tile and rollout buffers
+        # take 40640 of the 40960 LDS floats, and the BF16 network's 16-sample split variant
+        "di_t256": lambda: workloads.double_integrator_map(num_rollouts=16500, num_timesteps=256, colored=False,
+                                                           cost_map=SMALL_MAP),
+        "bicycle_nn_bf16_t256": lambda: workloads.bicycle_nn(num_rollouts=512, num_timesteps=256, cost_map=SMALL_MAP,
+                                                             precision=capi.NN_BF16),
         "di_single": lambda: workloads.double_integrator_map(num_rollouts=1, num_timesteps=1, colored=False,
                                                              cost_map=SMALL_MAP),
     }
@@ -118,11 +124,13 @@ def _problems():
 # BF16 network: the oracle evaluates the same BF16-rounded weights and layer inputs in FP32; the GPU's MFMA summation
 # order and its tanh (hardware exp2/rcp, a few 1e-7 absolute) move an activation across a BF16 rounding boundary now
 # and then (one BF16 ulp, 2^-8 relative), hence the looser bar for that case.
-COST_RTOL = {"bicycle_nn_bf16": 1e-3, "bicycle_nn_bf16_short": 1e-3, "bicycle_nn_bf16_wide": 1e-3}
+COST_RTOL = {"bicycle_nn_bf16": 1e-3, "bicycle_nn_bf16_short": 1e-3, "bicycle_nn_bf16_wide": 1e-3,
+             "bicycle_nn_bf16_t256": 1e-3}
 
 
 # the extra sizes only select controller launch shapes; the rollout kernel is the same
-CONTROLLER_SHAPES = ("bicycle_nn_bf16_short", "bicycle_nn_bf16_wide", "cartpole_k8192", "cartpole_k16500")
+CONTROLLER_SHAPES = ("bicycle_nn_bf16_short", "bicycle_nn_bf16_wide", "cartpole_k8192", "cartpole_k16500", "di_t256",
+                     "bicycle_nn_bf16_t256")
 
 
 @pytest.mark.parametrize("name", [n for n in _problems() if n not in CONTROLLER_SHAPES])
@@ -208,7 +216,8 @@ def _oracle_iteration(prob, samples, mean):
                                         ("bicycle_nn_fp32", False), ("bicycle_nn_bf16", True),
                                         ("bicycle_nn_bf16_short", False), ("bicycle_nn_bf16_wide", False),
                                         ("cartpole_ragged", True), ("cartpole_k8192", False),
-                                        ("cartpole_k16500", True)])
+                                        ("cartpole_k16500", True), ("di_t256", False),
+                                        ("bicycle_nn_bf16_t256", True)])
 def test_controller_iteration_matches_oracle(hip, engine, name, graph):
     prob = _problems()[name]()
     K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
