@@ -260,6 +260,8 @@ struct mppi_controller {
   IterArgs* loop_iter_d = nullptr;  // free-running iteration arguments (rng offset advanced on the device)
   hipGraph_t loop_graph = nullptr;
   hipGraphExec_t loop_exec = nullptr;
+  hipGraph_t loop_group_graph = nullptr;  // LOOP_GROUP consecutive loop calls in one graph
+  hipGraphExec_t loop_group_exec = nullptr;
   // learned proposal (optional; the network object is owned by the caller)
   ProposalNet* prop = nullptr;
   int prop_K = 0;
@@ -562,6 +564,8 @@ static int controller_free(mppi_controller* c)
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->loop_exec) (void)hipGraphExecDestroy(c->loop_exec);
   if (c->loop_graph) (void)hipGraphDestroy(c->loop_graph);
+  if (c->loop_group_exec) (void)hipGraphExecDestroy(c->loop_group_exec);
+  if (c->loop_group_graph) (void)hipGraphDestroy(c->loop_group_graph);
   if (c->prop_exec) (void)hipGraphExecDestroy(c->prop_exec);
   if (c->prop_graph) (void)hipGraphDestroy(c->prop_graph);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -877,6 +881,27 @@ static int controller_enqueue_loop(mppi_controller* c)
   return MPPI_OK;
 }
 
+// Device-resident calls replay in graphs of LOOP_GROUP calls (the same kernels in the same order as LOOP_GROUP
+// single-call replays), which leaves fewer graph-launch boundaries between the calls' kernels.
+constexpr int LOOP_GROUP = 8;
+
+static int capture_loop(mppi_controller* c, int calls, hipGraph_t* graph, hipGraphExec_t* exec)
+{
+  MPPI_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int st = MPPI_OK;
+  for (int i = 0; i < calls && st == MPPI_OK; ++i) st = controller_enqueue_loop(c);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (st != MPPI_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  *graph = g;
+  MPPI_HIP_CHECK(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+  return MPPI_OK;
+}
+
 int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
 {
   if (!c) return fail(MPPI_E_INVALID, "null argument");
@@ -888,19 +913,12 @@ int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
     MPPI_HIP_CHECK(hipMemcpyAsync(c->loop_iter_d, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
     MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
     MPPI_CHECK(controller_enqueue_loop(c));  // eager pass (sets kernel attributes before capture)
-    MPPI_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int st = controller_enqueue_loop(c);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (st != MPPI_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return st;
-    }
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    c->loop_graph = g;
-    MPPI_HIP_CHECK(hipGraphInstantiate(&c->loop_exec, c->loop_graph, nullptr, nullptr, 0));
+    MPPI_CHECK(capture_loop(c, 1, &c->loop_graph, &c->loop_exec));
+    MPPI_CHECK(capture_loop(c, LOOP_GROUP, &c->loop_group_graph, &c->loop_group_exec));
     --num_calls;
   }
+  for (; num_calls >= LOOP_GROUP; num_calls -= LOOP_GROUP)
+    MPPI_HIP_CHECK(hipGraphLaunch(c->loop_group_exec, c->stream));
   for (int i = 0; i < num_calls; ++i) MPPI_HIP_CHECK(hipGraphLaunch(c->loop_exec, c->stream));
   return MPPI_OK;
 }

@@ -299,6 +299,34 @@ def test_controller_colored_samples_equal_sampler(hip, engine, model, deadband):
     ctrl.close()
 
 
+def test_device_loop_grouped_replays_equal_single_replays(hip, engine):
+    """stepDevice replays graphs of eight calls where it can: eleven calls as one eager pass, one group of eight and
+    two single replays leave the same device mean, bit for bit, as eleven one-call steps."""
+    from mppi_generic_vendor_amd import hip as hipmod
+    banner("device loop: grouped vs single replays, 11 calls")
+    prob = workloads.cartpole(num_rollouts=2048, num_timesteps=100)
+    T, C = prob.num_timesteps, prob.C
+    means = []
+    for grouped in (True, False):
+        ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+        ctrl.computeControl(prob.x0, 1)
+        first = ctrl.getRawMean()
+        if grouped:
+            ctrl.stepDevice(11)
+        else:
+            for _ in range(11):
+                ctrl.stepDevice(1)
+        hipmod.synchronize()
+        _, _, mean_d = ctrl.device_buffers()
+        m = np.empty((T, C), np.float32)
+        hipmod.check(hipmod._load().hipMemcpy(m.ctypes.data, ct.c_void_p(mean_d), m.nbytes, hipmod.D2H), "hipMemcpy")
+        assert not np.array_equal(m, first)
+        means.append(m)
+        ctrl.close()
+    print(f"[loop] grouped vs single: max |diff| {float(np.max(np.abs(means[0] - means[1]))):.3g}")
+    assert np.array_equal(means[0], means[1])
+
+
 def test_graph_replay_equals_eager_over_many_calls(hip, engine):
     banner("graph replay vs eager, 6 calls")
     prob = workloads.cartpole(num_rollouts=1024, num_timesteps=60)
