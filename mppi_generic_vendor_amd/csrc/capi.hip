@@ -917,7 +917,7 @@ int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
     MPPI_CHECK(capture_loop(c, LOOP_GROUP, &c->loop_group_graph, &c->loop_group_exec));
     --num_calls;
   }
-  for (; num_calls >= LOOP_GROUP; num_calls -= LOOP_GROUP)
+  for (; c->loop_group_exec != nullptr && num_calls >= LOOP_GROUP; num_calls -= LOOP_GROUP)
     MPPI_HIP_CHECK(hipGraphLaunch(c->loop_group_exec, c->stream));
   for (int i = 0; i < num_calls; ++i) MPPI_HIP_CHECK(hipGraphLaunch(c->loop_exec, c->stream));
   return MPPI_OK;
