@@ -292,8 +292,8 @@ int mppi_controller_destroy(mppi_controller* ctrl);
  * (controllers/MPPI/mppi_controller.cu:224-227). */
 int mppi_controller_compute_control(mppi_controller* ctrl, const float* x0_host, int32_t optimization_stride);
 /* Enqueue `num_calls` device-resident repetitions of the GPU work of one computeControl call (num_iters x sample,
- * rollout, weights, update) on the controller's stream, replayed from captured hipGraphs of eight calls (single-call
- * graphs for the remainder; the same kernels in the same order): x0 and the nominal sequence are the ones already in
+ * rollout, weights, update) on the controller's stream, replayed from captured hipGraphs of 32 and 8 calls
+ * (single-call graphs for the remainder; the same kernels in the same order): x0 and the nominal sequence are the ones already in
  * device memory, each call warm-starts from the previous mean, and the generator counter advances on the device. No
  * host transfer and no synchronisation; for tight loops and throughput timing. */
 int mppi_controller_step_device(mppi_controller* ctrl, int32_t num_calls);

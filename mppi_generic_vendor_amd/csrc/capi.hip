@@ -262,6 +262,8 @@ struct mppi_controller {
   hipGraphExec_t loop_exec = nullptr;
   hipGraph_t loop_group_graph = nullptr;  // LOOP_GROUP consecutive loop calls in one graph
   hipGraphExec_t loop_group_exec = nullptr;
+  hipGraph_t loop_big_graph = nullptr;  // LOOP_BIG consecutive loop calls
+  hipGraphExec_t loop_big_exec = nullptr;
   // learned proposal (optional; the network object is owned by the caller)
   ProposalNet* prop = nullptr;
   int prop_K = 0;
@@ -566,6 +568,8 @@ static int controller_free(mppi_controller* c)
   if (c->loop_graph) (void)hipGraphDestroy(c->loop_graph);
   if (c->loop_group_exec) (void)hipGraphExecDestroy(c->loop_group_exec);
   if (c->loop_group_graph) (void)hipGraphDestroy(c->loop_group_graph);
+  if (c->loop_big_exec) (void)hipGraphExecDestroy(c->loop_big_exec);
+  if (c->loop_big_graph) (void)hipGraphDestroy(c->loop_big_graph);
   if (c->prop_exec) (void)hipGraphExecDestroy(c->prop_exec);
   if (c->prop_graph) (void)hipGraphDestroy(c->prop_graph);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -881,9 +885,10 @@ static int controller_enqueue_loop(mppi_controller* c)
   return MPPI_OK;
 }
 
-// Device-resident calls replay in graphs of LOOP_GROUP calls (the same kernels in the same order as LOOP_GROUP
-// single-call replays), which leaves fewer graph-launch boundaries between the calls' kernels.
+// Device-resident calls replay in graphs of LOOP_BIG, then LOOP_GROUP calls (the same kernels in the same order as
+// that many single-call replays), which leaves fewer graph-launch boundaries between the calls' kernels.
 constexpr int LOOP_GROUP = 8;
+constexpr int LOOP_BIG = 32;
 
 static int capture_loop(mppi_controller* c, int calls, hipGraph_t* graph, hipGraphExec_t* exec)
 {
@@ -915,8 +920,11 @@ int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
     MPPI_CHECK(controller_enqueue_loop(c));  // eager pass (sets kernel attributes before capture)
     MPPI_CHECK(capture_loop(c, 1, &c->loop_graph, &c->loop_exec));
     MPPI_CHECK(capture_loop(c, LOOP_GROUP, &c->loop_group_graph, &c->loop_group_exec));
+    MPPI_CHECK(capture_loop(c, LOOP_BIG, &c->loop_big_graph, &c->loop_big_exec));
     --num_calls;
   }
+  for (; c->loop_big_exec != nullptr && num_calls >= LOOP_BIG; num_calls -= LOOP_BIG)
+    MPPI_HIP_CHECK(hipGraphLaunch(c->loop_big_exec, c->stream));
   for (; c->loop_group_exec != nullptr && num_calls >= LOOP_GROUP; num_calls -= LOOP_GROUP)
     MPPI_HIP_CHECK(hipGraphLaunch(c->loop_group_exec, c->stream));
   for (int i = 0; i < num_calls; ++i) MPPI_HIP_CHECK(hipGraphLaunch(c->loop_exec, c->stream));

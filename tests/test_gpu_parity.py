@@ -300,10 +300,10 @@ def test_controller_colored_samples_equal_sampler(hip, engine, model, deadband):
 
 
 def test_device_loop_grouped_replays_equal_single_replays(hip, engine):
-    """stepDevice replays graphs of eight calls where it can: eleven calls as one eager pass, one group of eight and
-    two single replays leave the same device mean, bit for bit, as eleven one-call steps."""
+    """stepDevice replays graphs of 32 and 8 calls where it can: 43 calls as one eager pass, one group of 32, one of
+    eight and two single replays leave the same device mean, bit for bit, as 43 one-call steps."""
     from mppi_generic_vendor_amd import hip as hipmod
-    banner("device loop: grouped vs single replays, 11 calls")
+    banner("device loop: grouped vs single replays, 43 calls")
     prob = workloads.cartpole(num_rollouts=2048, num_timesteps=100)
     T, C = prob.num_timesteps, prob.C
     means = []
@@ -312,9 +312,9 @@ def test_device_loop_grouped_replays_equal_single_replays(hip, engine):
         ctrl.computeControl(prob.x0, 1)
         first = ctrl.getRawMean()
         if grouped:
-            ctrl.stepDevice(11)
+            ctrl.stepDevice(43)
         else:
-            for _ in range(11):
+            for _ in range(43):
                 ctrl.stepDevice(1)
         hipmod.synchronize()
         _, _, mean_d = ctrl.device_buffers()
