@@ -87,16 +87,18 @@ def cartpole(num_rollouts: int = 2048, num_timesteps: int = 100, seed: int = 42)
 
 def double_integrator_map(num_rollouts: int = 8192, num_timesteps: int = 100, seed: int = 42,
                           colored: bool = True, fft_len_mode: int = capi.FFT_LEN_2T,
-                          cost_map: np.ndarray | None = None) -> Problem:
+                          cost_map: np.ndarray | None = None, terminal_cost_coeff: float = 0.0) -> Problem:
     """2-D double integrator with power-law noise (exponents 1, 2) and a 1024^2 blob cost map at 0.05 m/cell
-    centred on the origin; quadratic goal term to (2, 2)."""
+    centred on the origin; quadratic goal term to (2, 2) plus the bilinear map lookup. The terminal cost is 0, as
+    QuadraticCost::terminalCost returns (cost_functions/quadratic_cost/quadratic_cost.cu:51-54), unless
+    terminal_cost_coeff asks for terminal_cost_coeff x the quadratic term."""
     dyn = capi.dynamics_defaults(capi.MODEL_DOUBLE_INTEGRATOR)
     cost = capi.cost_defaults(capi.COST_QUADRATIC_MAP, dyn)
     for i, q in enumerate((2.0, 2.0, 0.5, 0.5)):
         cost.coeff[i] = q
     cost.goal[0] = 2.0
     cost.goal[1] = 2.0
-    cost.terminal_cost_coeff = 10.0
+    cost.terminal_cost_coeff = terminal_cost_coeff
     m = gaussian_blob_map() if cost_map is None else cost_map
     cost.map_height, cost.map_width = m.shape
     cost.map_resolution = 0.05

@@ -316,11 +316,115 @@ def test_bicycle_known_answers():
     assert xd[0] == pytest.approx(1.3 * 0.5 - 2.5 * 0.2 - 3.7 * 2.0 + 4.9)
 
 
+# Hand-evaluated steps of the kinematic bicycle (dynamics/racer_dubins/racer_dubins.cu:5-33 derivative, :44-59
+# updateState) with the RacerDubinsParams defaults (c_t 1.3, c_b 2.5, c_v 3.7, c_0 4.9, steering_constant 0.6,
+# steer_command_angle_scale 5, steer_angle_scale -9.1, max_steer_angle 0.5, max_steer_rate 5, brake_delay_constant
+# 6.6, max_brake_rate 0.9 / 0.33, wheel_base 0.3) and throttle range [-1, 1]. The numbers were worked out by hand
+# (a few lines of arithmetic each) and are typed in as constants:
+#  case A (braking, dt 0.2): brake rate (1 - 0.99) 6.6 = 0.066 -> brake 0.99 + 0.0132 = 1.0032, clamped to
+#    -control_min[0] = 1; v' = 2.5 0.99 (-1) - 3.7 3 + 4.9 = -8.675 (no throttle while braking); yaw' = (3 / 0.3)
+#    tan(-0.45 / -9.1) = 0.49490897 -> yaw 3.1 + 0.09898 = 3.19898 wraps to 3.19898 - 2 pi = -3.08420351;
+#    steer' = clamp((-1 5 + 0.45) 0.6) = -2.73 -> steer -0.45 - 0.546 = -0.996 clamped to -0.5; the steer rate state
+#    copies steer' = -2.73; x' = 3 cos(3.1) = -2.99740545, y' = 3 sin(3.1) = 0.124741987.
+#  case B (throttle, dt 0.2): brake' = clamp(-0.05 6.6 = -0.33) -> brake 0.05 - 0.066 = -0.016 clamped to 0;
+#    v' = 1.3 0.5 - 2.5 0.05 - 3.7 2 + 4.9 = -1.975; yaw -3.1 - 0.0659879 = -3.16599 wraps to 3.11719744;
+#    steer' = (5 - 0.45) 0.6 = 2.73 -> 0.45 + 0.546 = 0.996 clamped to 0.5.
+BICYCLE_STEPS = [
+    # x0, u, dt, x_dot, x_next
+    ([3.0, 3.1, 1.0, -2.0, -0.45, 0.99, 0.0], [-1.0, -1.0], 0.2,
+     [-8.675, 0.49490897, -2.99740545, 0.124741987, -2.73, 0.066, 0.0],
+     [1.265, -3.08420351, 0.40051891, -1.9750516, -0.5, 1.0, -2.73]),
+    ([2.0, -3.1, 0.5, 0.5, 0.45, 0.05, 0.0], [0.5, 1.0], 0.2,
+     [-1.975, -0.329939313, -1.9982703, -0.0831613249, 2.73, -0.33, 0.0],
+     [1.605, 3.11719744, 0.10034594, 0.483367735, 0.5, 0.0, 2.73]),
+]
+
+
+@pytest.mark.parametrize("x,u,dt,xd_want,xn_want", BICYCLE_STEPS)
+def test_bicycle_update_state_known_answers(oracle, engine, x, u, dt, xd_want, xn_want):
+    """Yaw wrap in both directions, steer-angle clamp at both limits, brake-state clamp at 0 and at -control_min[0],
+    the steer-rate state copying the steer derivative, throttle vs brake branches: the oracle's step and the engine's
+    host step (the device code's functions compiled for the host) against the hand-evaluated values above."""
+    dyn = capi.dynamics_defaults(capi.MODEL_RACER_DUBINS)
+    xa = np.array(x, np.float32)
+    ua = np.array(u, np.float32)
+    for fn in ("oracle", "engine"):
+        xn = np.zeros(7, np.float32)
+        xd = np.zeros(7, np.float32)
+        y = np.zeros(7, np.float32)
+        if fn == "oracle":
+            oracle.oracle_step_f32(ct.byref(dyn), xa.ctypes.data, ua.ctypes.data, dt, xn.ctypes.data, xd.ctypes.data,
+                                   y.ctypes.data)
+        else:
+            capi.call("mppi_host_step", ct.byref(dyn), xa.ctypes.data, ua.ctypes.data, ct.c_float(dt), xn.ctypes.data,
+                      xd.ctypes.data, y.ctypes.data)
+        np.testing.assert_allclose(xd, xd_want, rtol=2e-6, atol=2e-6, err_msg=fn)
+        np.testing.assert_allclose(xn, xn_want, rtol=2e-6, atol=2e-6, err_msg=fn)
+        np.testing.assert_array_equal(y, xn)
+
+
+def test_speed_term_closed_form(oracle):
+    """Speed term speed_coeff (y[speed_index] - target)^2 (cost_functions/autorally/ar_standard_cost.cu:284-297, L2
+    branch), alone and added to the quadratic term: y = (5.5, 0, ...), target 8, coeff 2 -> 2 (2.5)^2 = 12.5; the
+    index selects the component (index 3 reads 1.0 -> 2 49 = 98); coefficient 0 removes the term."""
+    dyn = capi.dynamics_defaults(capi.MODEL_RACER_DUBINS)
+    cost = capi.cost_defaults(capi.COST_QUADRATIC_MAP, dyn)
+    for i in range(capi.MAX_OUTPUT):
+        cost.coeff[i] = 0.0
+    cost.map_coeff = 0.0
+    cost.speed_coeff, cost.speed_target, cost.speed_index = 2.0, 8.0, 0
+    y = np.array([5.5, 0.3, 0.0, 0.0, 0.0, 0.0, 0.0], np.float32)
+    y1 = y.copy()
+    y1[3] = 1.0
+    for lib in ("oracle", "engine"):
+        def run(c, v):
+            if lib == "oracle":
+                return oracle.oracle_running_cost_f32(ct.byref(c), None, v.ctypes.data)
+            out = ct.c_float()
+            capi.call("mppi_host_running_cost", ct.byref(c), v.ctypes.data, ct.byref(out))
+            return out.value
+        assert run(cost, y) == pytest.approx(12.5, rel=1e-7)
+        cost.speed_index = 3
+        assert run(cost, y1) == pytest.approx(98.0, rel=1e-7)
+        cost.speed_index = 0
+        cost.coeff[1], cost.goal[1] = 4.0, 0.1  # + 4 (0.3 - 0.1)^2 = 0.16
+        assert run(cost, y) == pytest.approx(12.66, rel=1e-6)
+        cost.coeff[1], cost.goal[1] = 0.0, 0.0
+        cost.speed_coeff = 0.0
+        assert run(cost, y) == 0.0
+        cost.speed_coeff = 2.0
+
+
+def test_deadband_known_answers(oracle, engine):
+    """enforceConstraints (dynamics/dynamics.cu:107-126): |u| < d -> zero_control, else u - d sign(u) with sign(0) = 1
+    (utils/math_utils.h:744-747), then the clamp. d = 0.1, zero_control 0.02, range [-1, 1]:
+    0.05 -> 0.02, -0.05 -> 0.02, 0.3 -> 0.2, -0.3 -> -0.2, 0.1 (not below d) -> 0.0, 5 -> 4.9 -> 1, -5 -> -1;
+    with d = 0: u unchanged (0 stays 0) apart from the clamp."""
+    dyn = capi.dynamics_defaults(capi.MODEL_CARTPOLE)
+    dyn.control_min[0], dyn.control_max[0] = -1.0, 1.0
+    dyn.control_deadband[0] = 0.1
+    dyn.zero_control[0] = 0.02
+    cases = [(0.05, 0.02), (-0.05, 0.02), (0.3, 0.2), (-0.3, -0.2), (0.1, 0.0), (5.0, 1.0), (-5.0, -1.0)]
+    for lib in ("oracle", "engine"):
+        for u_in, want in cases:
+            u = np.array([u_in], np.float32)
+            if lib == "oracle":
+                oracle.oracle_enforce_constraints_f32(ct.byref(dyn), u.ctypes.data)
+            else:
+                capi.call("mppi_host_enforce_constraints", ct.byref(dyn), u.ctypes.data)
+            assert u[0] == pytest.approx(want, abs=1e-7), (lib, u_in)
+    dyn.control_deadband[0] = 0.0
+    for u_in in (0.0, 0.7, -0.3):
+        u = np.array([u_in], np.float32)
+        oracle.oracle_enforce_constraints_f32(ct.byref(dyn), u.ctypes.data)
+        assert u[0] == np.float32(u_in)
+
+
 def test_double_integrator_rollout_closed_form():
     """Euler DI: v_t = v0 + dt * cumsum(u), p_t = p0 + dt * cumsum(v) -- a prefix-sum formulation of the step loop;
     cost = (sum_t l(y_{t+1}) + LR + phi(y_T)) / T with the clamped controls written back."""
     prob = workloads.double_integrator_map(num_rollouts=3, num_timesteps=40, colored=False,
-                                           cost_map=np.zeros((8, 8), np.float32))
+                                           cost_map=np.zeros((8, 8), np.float32), terminal_cost_coeff=10.0)
     prob.cost.map_coeff = 0.0
     prob.sampler.control_cost_coeff[0] = 0.3
     prob.sampler.control_cost_coeff[1] = 0.7

@@ -36,6 +36,32 @@ def _net(S, C, T, seed):
     return theta
 
 
+def test_forward_matches_numpy(oracle):
+    """The FP64 forward against an independent numpy formulation: X W1^T + b1 -> GELU (scipy erf) -> W2^T + b2 ->
+    GELU -> W3^T + b3, with W_l read as [out][in] row-major blocks followed by b_l (the FNN layout). Non-square
+    layers make a transposed weight block fail, and the distinct biases catch a dropped or misplaced bias."""
+    from scipy import special
+
+    S, C, T, B = 7, 2, 11, 6
+    theta = _net(S, C, T, 5)
+    din, dout = S + 16 * C, 2 * T * C
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(B, din)).astype(np.float32)
+    th = theta.astype(np.float64)
+    off = 0
+    h = X.astype(np.float64)
+    for li, (ni, no) in enumerate(((din, 128), (128, 128), (128, dout))):
+        W = th[off:off + ni * no].reshape(no, ni)
+        off += ni * no
+        b = th[off:off + no]
+        off += no
+        h = h @ W.T + b
+        if li < 2:
+            h = 0.5 * h * (1.0 + special.erf(h / math.sqrt(2.0)))
+    got = ol.proposal_forward(S, C, T, theta, X)
+    np.testing.assert_allclose(got, h, rtol=1e-12, atol=1e-12)
+
+
 def test_backward_matches_finite_differences(oracle):
     S, C, T, B = 3, 1, 5, 4
     theta = _net(S, C, T, 0)
