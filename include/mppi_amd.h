@@ -293,9 +293,13 @@ int mppi_controller_destroy(mppi_controller* ctrl);
 int mppi_controller_compute_control(mppi_controller* ctrl, const float* x0_host, int32_t optimization_stride);
 /* Enqueue `num_calls` device-resident repetitions of the GPU work of one computeControl call (num_iters x sample,
  * rollout, weights, update) on the controller's stream, replayed from captured hipGraphs of 32 and 8 calls
- * (single-call graphs for the remainder; the same kernels in the same order): x0 and the nominal sequence are the ones already in
- * device memory, each call warm-starts from the previous mean, and the generator counter advances on the device. No
- * host transfer and no synchronisation; for tight loops and throughput timing. */
+ * (single-call graphs for the remainder; the same kernels in the same order). x0 is the one the last computeControl
+ * uploaded; each call warm-starts from the previous call's raw mean (no smoothing, no slide); stride is 1; iteration
+ * it of a call samples with std_dev * std_dev_decay^it; call n, iteration it draws with generator offset
+ * base + n num_iters + it, advanced on the device (base: a counter range computeControl does not reach).
+ * Returns MPPI_E_INVALID until computeControl has run once. The first call after creation or set_seed uploads the
+ * loop's arguments, synchronises the stream and captures the graphs (do not call it inside a stream capture); later
+ * calls only enqueue graph launches (no host transfer, no synchronisation). For tight loops and throughput timing. */
 int mppi_controller_step_device(mppi_controller* ctrl, int32_t num_calls);
 /* getControlSeq(): [T][C] smoothed control sequence. */
 int mppi_controller_get_control_seq(const mppi_controller* ctrl, float* out_host);

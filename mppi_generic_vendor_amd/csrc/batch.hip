@@ -387,13 +387,7 @@ static int launch_batch_t(const BatchInstance* inst, const int2* work, int nwork
                           const float* x0s, const float* means, float* ubuf, Partial* parts, float* sums, float* costs,
                           int ch, size_t lds_bytes, int blocks, hipStream_t s)
 {
-  static std::once_flag once;
-  hipError_t attr = hipSuccess;
-  std::call_once(once, [&] {
-    attr = hipFuncSetAttribute((const void*)batch_rollout_kernel<NT, SPL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               LDS_LIMIT);
-  });
-  if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute");
+  MPPI_CHECK(allow_max_dynamic_lds((const void*)batch_rollout_kernel<NT, SPL>));
   batch_rollout_kernel<NT, SPL><<<blocks, NT, lds_bytes, s>>>(inst, work, nwork, queue, iters, x0s, means, ubuf,
                                                               parts, sums, costs, ch);
   MPPI_HIP_CHECK(hipGetLastError());
@@ -403,9 +397,7 @@ static int launch_batch_t(const BatchInstance* inst, const int2* work, int nwork
 template <int NT, int SPL>
 static int occupancy_t(size_t lds_bytes, int* out)
 {
-  const hipError_t e = hipFuncSetAttribute((const void*)batch_rollout_kernel<NT, SPL>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT);
-  if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+  MPPI_CHECK(allow_max_dynamic_lds((const void*)batch_rollout_kernel<NT, SPL>));
   MPPI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(out, (const void*)batch_rollout_kernel<NT, SPL>, NT,
                                                               lds_bytes));
   return MPPI_OK;

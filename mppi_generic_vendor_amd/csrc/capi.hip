@@ -5,6 +5,7 @@
 #include <mutex>
 #include <utility>
 #include <new>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,21 @@ int hip_fail(hipError_t e, const char* what)
 {
   set_last_error(std::string(what) + ": " + hipGetErrorString(e));
   return MPPI_E_HIP;
+}
+
+// The attribute belongs to the kernel on the device that is current when it is set, so it is recorded per
+// (device, kernel); the lock serialises controllers created from several host threads.
+int allow_max_dynamic_lds(const void* fn)
+{
+  static std::mutex mu;
+  static std::set<std::pair<int, const void*>> done;
+  int dev = 0;
+  MPPI_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.count({dev, fn}) != 0) return MPPI_OK;
+  MPPI_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
+  done.insert({dev, fn});
+  return MPPI_OK;
 }
 
 int pure_noise_start(int K, float pct)
@@ -257,7 +273,9 @@ struct mppi_controller {
   int eager_runs = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  IterArgs* loop_iter_d = nullptr;  // free-running iteration arguments (rng offset advanced on the device)
+  bool computed = false;            // computeControl has run once (the device loop starts from its x0 and mean)
+  IterArgs* loop_iter_d = nullptr;  // free-running iteration arguments, one per iteration of a call (rng offsets
+                                    // advanced on the device)
   hipGraph_t loop_graph = nullptr;
   hipGraphExec_t loop_exec = nullptr;
   hipGraph_t loop_group_graph = nullptr;  // LOOP_GROUP consecutive loop calls in one graph
@@ -559,19 +577,26 @@ int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_
 }
 
 // --------------------------------------------------------------------------------------------------- controller
+// Destroys every captured graph of the controller (the next use captures again); the caller has synchronised the
+// stream, so no replay of them is still queued.
+static void drop_graphs(mppi_controller* c)
+{
+  hipGraphExec_t* execs[] = {&c->graph_exec, &c->loop_exec, &c->loop_group_exec, &c->loop_big_exec, &c->prop_exec};
+  hipGraph_t* graphs[] = {&c->graph, &c->loop_graph, &c->loop_group_graph, &c->loop_big_graph, &c->prop_graph};
+  for (hipGraphExec_t* e : execs) {
+    if (*e) (void)hipGraphExecDestroy(*e);
+    *e = nullptr;
+  }
+  for (hipGraph_t* g : graphs) {
+    if (*g) (void)hipGraphDestroy(*g);
+    *g = nullptr;
+  }
+}
+
 static int controller_free(mppi_controller* c)
 {
   if (!c) return MPPI_OK;
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  if (c->loop_exec) (void)hipGraphExecDestroy(c->loop_exec);
-  if (c->loop_graph) (void)hipGraphDestroy(c->loop_graph);
-  if (c->loop_group_exec) (void)hipGraphExecDestroy(c->loop_group_exec);
-  if (c->loop_group_graph) (void)hipGraphDestroy(c->loop_group_graph);
-  if (c->loop_big_exec) (void)hipGraphExecDestroy(c->loop_big_exec);
-  if (c->loop_big_graph) (void)hipGraphDestroy(c->loop_big_graph);
-  if (c->prop_exec) (void)hipGraphExecDestroy(c->prop_exec);
-  if (c->prop_graph) (void)hipGraphDestroy(c->prop_graph);
+  drop_graphs(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
@@ -609,7 +634,7 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   const size_t ws = fused_iteration_workspace_bytes(K, T, C);
   MPPI_CHECK(dev_alloc(&c->ws_d, ws));
   MPPI_CHECK(dev_alloc(&c->iter_d, sizeof(IterArgs) * desc->num_iters));
-  MPPI_CHECK(dev_alloc(&c->loop_iter_d, sizeof(IterArgs)));
+  MPPI_CHECK(dev_alloc(&c->loop_iter_d, sizeof(IterArgs) * desc->num_iters));
   MPPI_CHECK(host_alloc_mapped(&c->iter_h, &c->iter_hd, sizeof(IterArgs) * desc->num_iters));
   MPPI_CHECK(host_alloc_mapped(&c->x0_h, &c->x0_hd, sizeof(float) * MPPI_MAX_STATE));
   MPPI_CHECK(host_alloc_mapped(&c->nominal_h, &c->nominal_hd, sizeof(float) * T * C));
@@ -670,7 +695,7 @@ int mppi_controller_destroy(mppi_controller* ctrl)
 // drawn inside the rollout kernel; colored samples too where the kernel's workgroup shape allows, else by the colored
 // sampler into the sample buffer first).
 static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
-                             const ResultCopies* copies = nullptr)
+                             const ResultCopies* copies = nullptr, int advance_step = 1)
 {
   const mppi_controller_desc& d = c->d;
   const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
@@ -681,7 +706,7 @@ static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
   return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->K, c->T, d.dt, d.lambda, d.alpha, c->x0_d,
                                 c->mean_d, c->samples_d, !colored, d.keep_samples != 0, ref, c->costs_d, c->ws_d,
                                 c->stats_d, c->mean_d, advance, c->stream, nullptr, nullptr, copies,
-                                colored ? &plan : nullptr);
+                                colored ? &plan : nullptr, advance_step);
 }
 
 // The work of one computeControl call, enqueued on the controller's stream (captured once when use_graph is set).
@@ -817,6 +842,7 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
   c->state_u = c->control;
   c->state_stale = true;
   clamp_controls_seq(d.dynamics, c->control.data(), T);
+  c->computed = true;
   return MPPI_OK;
 }
 
@@ -875,13 +901,14 @@ int mppi_controller_get_proposal_stats(const mppi_controller* c, int32_t* propos
 
 // One device-resident iteration sequence: the same kernels as computeControl, reading x0 and the mean already in
 // device memory, warm-starting the next call from the new mean, with the rng offset advanced on the device.
+// Iteration `it` of every call reads slot `it` of loop_iter_d (its own iteration index, so std_dev_decay^it applies as
+// in computeControl) and advances that slot's rng offset by num_iters, so that call n, iteration it draws with offset
+// base + n num_iters + it, the numbering of consecutive computeControl calls.
 static int controller_enqueue_loop(mppi_controller* c)
 {
-  const int K = c->K, T = c->T, C = c->C;
-  const hipStream_t s = c->stream;
-  const mppi_controller_desc& d = c->d;
-  for (int it = 0; it < d.num_iters; ++it)
-    MPPI_CHECK(enqueue_iteration(c, IterRef{c->loop_iter_d, {0, 0, 0}}, c->loop_iter_d));
+  const int n = c->d.num_iters;
+  for (int it = 0; it < n; ++it)
+    MPPI_CHECK(enqueue_iteration(c, IterRef{c->loop_iter_d + it, {0, 0, 0}}, c->loop_iter_d + it, nullptr, n));
   return MPPI_OK;
 }
 
@@ -911,11 +938,16 @@ int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
 {
   if (!c) return fail(MPPI_E_INVALID, "null argument");
   if (num_calls < 0) return fail(MPPI_E_INVALID, "num_calls must be >= 0");
+  if (!c->computed)
+    return fail(MPPI_E_INVALID, "step_device: call computeControl once first (the loop starts from its x0 and mean)");
   if (num_calls == 0) return MPPI_OK;
   if (!c->loop_exec) {
-    // arguments of the free-running loop: x0 and the nominal sequence as last uploaded, stride 1, iteration 0
-    IterArgs a{c->rng_counter + ((uint64_t)1 << 40), 1, 0};
-    MPPI_HIP_CHECK(hipMemcpyAsync(c->loop_iter_d, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+    // arguments of the free-running loop: x0 as last uploaded, the mean left by the last call, stride 1, iteration
+    // it in slot it, rng offsets from a range computeControl does not reach
+    const int n = c->d.num_iters;
+    std::vector<IterArgs> a((size_t)n);
+    for (int it = 0; it < n; ++it) a[it] = IterArgs{c->rng_counter + ((uint64_t)1 << 40) + (uint64_t)it, 1, it};
+    MPPI_HIP_CHECK(hipMemcpyAsync(c->loop_iter_d, a.data(), sizeof(IterArgs) * n, hipMemcpyHostToDevice, c->stream));
     MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
     MPPI_CHECK(controller_enqueue_loop(c));  // eager pass (sets kernel attributes before capture)
     MPPI_CHECK(capture_loop(c, 1, &c->loop_graph, &c->loop_exec));
@@ -1009,17 +1041,12 @@ int mppi_controller_device_buffers(const mppi_controller* c, float** samples_d, 
 int mppi_controller_set_seed(mppi_controller* c, uint64_t seed)
 {
   if (!c) return fail(MPPI_E_INVALID, "null argument");
+  // the seed is a by-value kernel argument of every captured graph (the control call's, the proposal call's and the
+  // device loop's): wait for queued replays, then drop all of them so that each is captured again with the new seed
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   c->d.sampler.seed = seed;
   c->rng_counter = 0;
-  // the seed is a kernel argument of the captured graph: drop the graph so the next call captures it again
-  if (c->graph_exec) {
-    (void)hipGraphExecDestroy(c->graph_exec);
-    c->graph_exec = nullptr;
-  }
-  if (c->graph) {
-    (void)hipGraphDestroy(c->graph);
-    c->graph = nullptr;
-  }
   return MPPI_OK;
 }
 

@@ -200,12 +200,7 @@ template <int NTT>
 static int launch_colored_t(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
                             IterRef it, const float* spectra_in, const ColoredPlan& pl, hipStream_t s)
 {
-  static bool attr_set = false;
-  if (!attr_set) {
-    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)colored_mfma_kernel<NTT>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
-    attr_set = true;
-  }
+  MPPI_CHECK(allow_max_dynamic_lds((const void*)colored_mfma_kernel<NTT>));
   const ColoredShape& sh = pl.sh;
   const int grid = (K + sh.spw - 1) / sh.spw;
   colored_mfma_kernel<NTT><<<grid, CTHREADS, sh.lds_bytes, s>>>(sp, K, T, C, sh.n, sh.F, sh.Kpad, sh.KPH, sh.MT,

@@ -22,6 +22,8 @@ namespace mppi_amd {
 void set_last_error(const std::string& msg);
 int fail(int status, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+// Raises kernel `fn`'s dynamic-LDS limit to LDS_LIMIT on the current device, once per (device, kernel); thread-safe.
+int allow_max_dynamic_lds(const void* fn);
 
 #define MPPI_HIP_CHECK(call)                              \
   do {                                                    \

@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
                                                                   const float* __restrict__ sums, int G, int K, int TC,
                                                                   float lambda, mppi_weight_stats* __restrict__ stats,
                                                                   float* __restrict__ mean_out, IterArgs* advance,
-                                                                  ResultCopies copies)
+                                                                  int advance_step, ResultCopies copies)
 {
   __shared__ float escale[MAX_PARTIALS];
   __shared__ float sv[COMBINE_THREADS / 64];
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
     *stats = st;
     if (copies.stats != nullptr) *copies.stats = st;
     if (copies.cost0 != nullptr) *copies.cost0 = *copies.cost0_src;
-    if (advance != nullptr) advance->rng_offset += 1;
+    if (advance != nullptr) advance->rng_offset += (uint64_t)advance_step;
   }
 }
 
@@ -412,12 +412,7 @@ static int launch_ru(const mppi_dynamics_params& dyn, const mppi_cost_params& co
   int chunk = 0;
   const size_t lf = ru_lds_floats<MODEL, PROP, SPLIT>(T, rows, CNTT > 0 ? cpl->sh.lds_bytes : 0, &chunk);
   if (lf == 0) return fail(MPPI_E_SIZE, "fused rollout: horizon too long for the LDS control tile");
-  static bool attr_set = false;
-  if (!attr_set) {
-    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT, CNTT>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
-    attr_set = true;
-  }
+  MPPI_CHECK(allow_max_dynamic_lds((const void*)rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT, CNTT>));
   const int pstart = PROP ? K : pure_noise_start(K, sp.pure_noise_trajectories_percentage);
   const int G = (K + rows - 1) / rows;
   rollout_update_kernel<MODEL, COST, GEN, WRITE, PROP, SPLIT, CNTT>
@@ -457,7 +452,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
                            IterArgs* advance, hipStream_t s, const float* prop_dmu, const float* prop_sig,
-                           const ResultCopies* copies, const ColoredPlan* colored)
+                           const ResultCopies* copies, const ColoredPlan* colored, int advance_step)
 {
   const int C = dyn.control_dim;
   const int model = (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
@@ -515,7 +510,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
   if (st != MPPI_OK) return st;
   const int grid = (T * C + COMBINE_COLS - 1) / COMBINE_COLS;
   combine_kernel<<<grid, COMBINE_THREADS, 0, s>>>(parts, sums, G, K, T * C, lambda, stats, mean_out, advance,
-                                                  copies != nullptr ? *copies : ResultCopies{});
+                                                  advance_step, copies != nullptr ? *copies : ResultCopies{});
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }

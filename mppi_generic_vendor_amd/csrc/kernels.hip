@@ -171,12 +171,7 @@ static int launch_rollout_t(const mppi_dynamics_params& dyn, const mppi_cost_par
   const int chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
   if (chunk == 0) return fail(MPPI_E_SIZE, "rollout: horizon too long for the LDS control tile");
   const size_t lds = sizeof(float) * (base + rollout_scratch_for<MODEL, O>(chunk));
-  static bool attr_set = false;
-  if (!attr_set) {
-    MPPI_HIP_CHECK(hipFuncSetAttribute((const void*)rollout_kernel<MODEL, COST>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, LDS_LIMIT));
-    attr_set = true;
-  }
+  MPPI_CHECK(allow_max_dynamic_lds((const void*)rollout_kernel<MODEL, COST>));
   const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
   const int grid = (K + RC_LANES - 1) / RC_LANES;
   rollout_kernel<MODEL, COST><<<grid, rollout_threads<MODEL>(), lds, s>>>(dyn, cost, sp, K, T, dt, lambda, alpha, x0, mean,

@@ -1,0 +1,155 @@
+"""Controller API behaviour on the GPU: the device-resident loop's iteration arguments, its start condition, and the
+generator reseed (setSeedCUDARandomNumberGen, controllers/controller.cu:203-209) across every captured graph."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_util import banner, gpu_normals, gpu_sample_gaussian
+from mppi_generic_vendor_amd import capi, workloads
+from mppi_generic_vendor_amd.controller import VanillaMPPIController
+
+pytestmark = pytest.mark.gpu
+
+LOOP_BASE = 1 << 40  # generator offsets of the device loop start at computeControl's counter + 2^40
+
+
+def _device_mean(ctrl):
+    from mppi_generic_vendor_amd import hip as hipmod
+
+    hipmod.synchronize()
+    _, _, mean_d = ctrl.device_buffers()
+    m = np.empty((ctrl.T, ctrl.C), np.float32)
+    hipmod.check(hipmod._load().hipMemcpy(m.ctypes.data, ct.c_void_p(mean_d), m.nbytes, hipmod.D2H), "hipMemcpy")
+    return m
+
+
+def _oracle_chain(prob, x0, nominal, offsets, stride=1):
+    """Iterations it = 0, 1, ... with generator offsets[it], each sampling around the previous iteration's mean with
+    std_dev * std_dev_decay^it."""
+    K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
+    mean = nominal.astype(np.float32)
+    for it, off in enumerate(offsets):
+        eps = gpu_normals(prob.sampler.seed, off, K * T * C)
+        u = gpu_sample_gaussian(prob.sampler, K, T, C, mean, stride, it, eps=eps)
+        costs, clamped = ol.rollout_f32(prob, u, mean, nthreads=8, x0=x0)
+        w, eta, *_ = ol.weights(costs, prob.lambda_)
+        mean = ol.weighted_update(w, eta, clamped).astype(np.float32)
+    return mean
+
+
+def test_step_device_needs_a_compute_control_call(hip, engine):
+    banner("step_device before computeControl")
+    prob = workloads.cartpole(num_rollouts=256, num_timesteps=20)
+    ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+    with pytest.raises(RuntimeError):
+        ctrl.stepDevice(1)
+    ctrl.computeControl(prob.x0, 1)
+    ctrl.stepDevice(2)
+    assert np.all(np.isfinite(_device_mean(ctrl)))
+    ctrl.close()
+
+
+def test_device_loop_iterations_follow_decay_and_offsets(hip, engine):
+    """num_iters = 2 with std_dev_decay 0.8: one device-loop call after a computeControl runs iteration 0 with offset
+    base and std_dev, iteration 1 with offset base + 1 and std_dev 0.8, around the raw mean the call left; the second
+    loop call continues with offsets base + 2, base + 3."""
+    banner("device loop num_iters=2 decay=0.8 vs oracle")
+    prob = workloads.cartpole(num_rollouts=1024, num_timesteps=50)
+    prob.sampler.std_dev_decay = 0.8
+    ctrl = VanillaMPPIController(prob, num_iters=2, use_graph=True)
+    ctrl.computeControl(prob.x0, 1)
+    start = ctrl.getRawMean()
+    np.testing.assert_array_equal(_device_mean(ctrl), start)
+    ctrl.stepDevice(1)
+    got1 = _device_mean(ctrl)
+    base = LOOP_BASE + 2  # two generator offsets used by the computeControl call
+    want1 = _oracle_chain(prob, prob.x0, start, [base, base + 1])
+    ctrl.stepDevice(1)
+    got2 = _device_mean(ctrl)
+    want2 = _oracle_chain(prob, prob.x0, got1, [base + 2, base + 3])
+    for got, want in ((got1, want1), (got2, want2)):
+        err = float(np.max(np.abs(got - want))) / max(1e-3, float(np.max(np.abs(want))))
+        print(f"[loop] num_iters=2: mean rel err {err:.2e}")
+        assert err < 1e-4
+    ctrl.close()
+
+
+def _fresh_result(prob, seed, x, calls):
+    prob.sampler.seed = seed
+    c = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+    for _ in range(calls):
+        c.updateImportanceSampler(np.zeros((prob.num_timesteps, prob.C), np.float32))
+        c.computeControl(x, 1)
+    raw = c.getRawMean()
+    c.stepDevice(3)
+    dev = _device_mean(c)
+    c.close()
+    return raw, dev
+
+
+def test_set_seed_resets_every_graph(hip, engine):
+    """After computeControl calls (the second one captured), device-loop replays and a reseed, the controller's next
+    call and loop equal a fresh controller built with the new seed (same nominal, same x0), and differ from the run
+    that was not reseeded."""
+    banner("set_seed after captured calls and device loop")
+    prob = workloads.cartpole(num_rollouts=2048, num_timesteps=60)
+    x = prob.x0
+    T, C = prob.num_timesteps, prob.C
+    zeros = np.zeros((T, C), np.float32)
+    runs = {}
+    for reseed in (True, False):
+        prob.sampler.seed = 42
+        a = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+        for _ in range(3):
+            a.computeControl(x, 1)
+        a.stepDevice(5)
+        if reseed:
+            a.setSeedCUDARandomNumberGen(777)
+        a.updateImportanceSampler(zeros)
+        a.computeControl(x, 1)
+        raw = a.getRawMean()
+        a.stepDevice(3)
+        runs[reseed] = (raw, _device_mean(a))
+        a.close()
+    fresh_raw, fresh_dev = _fresh_result(prob, 777, x, 1)
+    np.testing.assert_array_equal(runs[True][0], fresh_raw)
+    np.testing.assert_array_equal(runs[True][1], fresh_dev)
+    assert not np.array_equal(runs[False][0], fresh_raw)
+    assert not np.array_equal(runs[False][1], fresh_dev)
+
+
+def test_set_seed_resets_the_proposal_graph(hip, engine):
+    """The learned-proposal call is replayed from its own graph: after a reseed it draws from the new seed's stream
+    from offset 0, like a fresh controller with that seed."""
+    from mppi_generic_vendor_amd.proposal import ProposalNet, init_params
+
+    banner("set_seed with a learned proposal attached")
+    small = workloads.gaussian_blob_map(256, 256, blobs=16, seed=7)
+    prob = workloads.bicycle_nn(num_rollouts=1024, num_timesteps=50, cost_map=small, residual=False)
+    S, C, T = prob.S, prob.C, prob.num_timesteps
+    net = ProposalNet(S, C, T, init_params(S, C, T, 3))
+    zeros = np.zeros((T, C), np.float32)
+
+    def run(seed0, reseed):
+        prob.sampler.seed = seed0
+        c = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+        c.setProposal(net, 512, fallback_tolerance=None)
+        if reseed is not None:
+            for _ in range(3):
+                c.computeControl(prob.x0, 1)
+            c.setSeedCUDARandomNumberGen(reseed)
+        c.updateImportanceSampler(zeros)
+        c.computeControl(prob.x0, 1)
+        out = c.getRawMean()
+        c.close()
+        return out
+
+    fresh = run(777, None)
+    reseeded = run(42, 777)
+    kept = run(42, 42)
+    np.testing.assert_array_equal(reseeded, fresh)
+    assert np.array_equal(kept, run(42, None))  # reseeding with the same seed restarts its stream
+    assert not np.array_equal(fresh, kept)
+    net.close()
