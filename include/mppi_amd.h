@@ -260,6 +260,17 @@ int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_
                          int32_t num_rollouts, int32_t num_timesteps, int32_t control_dim, float* mean_out_d,
                          void* stream);
 
+/* mppi_weighted_update that allocates nothing. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses that forbid hipMalloc). Its partial rows
+ * (ceil(K / 64) * T * C floats) go to a file-scope device array of MPPI_NOALLOC_SCRATCH_BYTES on each GPU, held until
+ * the process ends, instead of the workspace mppi_weighted_update grows with hipMalloc; larger sizes return
+ * MPPI_E_SIZE. All calls on one GPU share that array: calls that overlap on one GPU (different streams) race. The
+ * result equals mppi_weighted_update's bit for bit (the same kernels). */
+#define MPPI_NOALLOC_SCRATCH_BYTES (2u << 20)
+int mppi_weighted_update_noalloc(const float* weights_d, const mppi_weight_stats* stats_d, const float* samples_d,
+                                 int32_t num_rollouts, int32_t num_timesteps, int32_t control_dim, float* mean_out_d,
+                                 void* stream);
+
 /* ------------------------------------------------------------------------------------------------------------------
  * Controller object (controllers/MPPI/mppi_controller.cu:152-237, controllers/controller.cuh:70-1058).
  * ---------------------------------------------------------------------------------------------------------------- */

@@ -138,6 +138,7 @@ _SIGS = {
                       _f, P, P, P, P, P], ct.c_int),
     "mppi_compute_weights": ([P, _i, _f, P, P], ct.c_int),
     "mppi_weighted_update": ([P, P, P, _i, _i, _i, P, P], ct.c_int),
+    "mppi_weighted_update_noalloc": ([P, P, P, _i, _i, _i, P, P], ct.c_int),
     "mppi_controller_create": ([ct.POINTER(ControllerDesc), ct.POINTER(P)], ct.c_int),
     "mppi_controller_destroy": ([P], ct.c_int),
     "mppi_controller_compute_control": ([P, P, _i], ct.c_int),
