@@ -521,10 +521,21 @@ void oracle_rollout_f32(const mppi_dynamics_params* dyn, const mppi_cost_params*
 
 void oracle_rollout_f64(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,
                         const mppi_sampler_params* smp, int K, int T, double dt, double lambda, double alpha,
-                        const float* x0, const float* mean, float* samples, double* costs)
+                        const float* x0, const float* mean, float* samples, double* costs, int nthreads)
 {
   /* FP64 run of the same loop; the clamped controls written back are rounded to float. */
-  rollout<double>(*dyn, *cst, map, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, 0, K);
+  if (nthreads <= 1) {
+    rollout<double>(*dyn, *cst, map, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, 0, K);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int chunk = (K + nthreads - 1) / nthreads;
+  for (int w = 0; w < nthreads; ++w) {
+    const int b = w * chunk, e = std::min(K, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=]() { rollout<double>(*dyn, *cst, map, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, b, e); });
+  }
+  for (auto& t : th) t.join();
 }
 
 /* --------------------------------------------------------------------------------------------- weights/update */
@@ -620,6 +631,20 @@ void oracle_slide(float* u, int T, int C, int steps, const float* zero_control, 
       if (i + steps > T - 1) v = (v - zero_control[c]) * scale[c] + zero_control[c];
       u[i * C + c] = v;
     }
+  }
+}
+
+/* The same propagation in FP64 (the controls clamped in FP64), for error reports. */
+void oracle_state_trajectory_f64(const mppi_dynamics_params* p, const float* x0, const float* u, int T, double dt,
+                                 double* states)
+{
+  const int S = p->state_dim, C = p->control_dim;
+  for (int i = 0; i < S; ++i) states[i] = x0[i];
+  double xd[MPPI_MAX_STATE], y[MPPI_MAX_OUTPUT], uc[MPPI_MAX_CONTROL];
+  for (int t = 0; t + 1 < T; ++t) {
+    for (int c = 0; c < C; ++c) uc[c] = u[t * C + c];
+    enforce_constraints<double>(*p, uc);
+    step<double>(*p, states + t * S, states + (t + 1) * S, xd, uc, y, dt);
   }
 }
 

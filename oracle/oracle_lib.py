@@ -51,7 +51,7 @@ _SIGS = {
     "oracle_rollout_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
                             ct.POINTER(capi.SamplerParams), _i, _i, _f, _f, _f, P, P, P, P, _i], None),
     "oracle_rollout_f64": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
-                            ct.POINTER(capi.SamplerParams), _i, _i, _d, _d, _d, P, P, P, P], None),
+                            ct.POINTER(capi.SamplerParams), _i, _i, _d, _d, _d, P, P, P, P, _i], None),
     "oracle_best_index": ([P, _i], _i),
     "oracle_weights": ([P, _i, _d, P, ct.POINTER(_f), ct.POINTER(_i), ct.POINTER(_d)], _d),
     "oracle_weights_f32": ([P, _i, _f, P], None),
@@ -60,6 +60,7 @@ _SIGS = {
     "oracle_smooth": ([P, _i, _i], None),
     "oracle_slide": ([P, _i, _i, _i, P, P], None),
     "oracle_state_trajectory": ([ct.POINTER(capi.DynamicsParams), P, P, _i, _f, P], None),
+    "oracle_state_trajectory_f64": ([ct.POINTER(capi.DynamicsParams), P, P, _i, _d, P], None),
 }
 
 _lib = None
@@ -135,7 +136,7 @@ def rollout_f32(prob, samples: np.ndarray, mean: np.ndarray, nthreads: int = 1, 
     return costs, s
 
 
-def rollout_f64(prob, samples: np.ndarray, mean: np.ndarray, x0=None):
+def rollout_f64(prob, samples: np.ndarray, mean: np.ndarray, x0=None, nthreads: int = 8):
     K, T = samples.shape[0], samples.shape[1]
     s = np.ascontiguousarray(samples, dtype=np.float32).copy()
     costs = np.empty(K, dtype=np.float64)
@@ -145,7 +146,7 @@ def rollout_f64(prob, samples: np.ndarray, mean: np.ndarray, x0=None):
     _a1 = np.ascontiguousarray(mean, np.float32)
     load().oracle_rollout_f64(ct.byref(dyn), ct.byref(prob.cost), ptr(prob.cost_map), ct.byref(prob.sampler), K, T,
                               prob.dt, prob.lambda_, prob.alpha, ptr(_a0),
-                              ptr(_a1), ptr(s), ptr(costs))
+                              ptr(_a1), ptr(s), ptr(costs), nthreads)
     return costs, s
 
 
@@ -196,6 +197,24 @@ def state_trajectory(prob, x0, u) -> np.ndarray:
     load().oracle_state_trajectory(ct.byref(dyn), ptr(_a0),
                                    ptr(_a1), T, prob.dt, ptr(out))
     return out
+
+
+def state_trajectory_f64(prob, x0, u) -> np.ndarray:
+    T, C = u.shape
+    out = np.empty((T, prob.S), dtype=np.float64)
+    dyn = _host_dyn(prob)
+    _a0 = np.ascontiguousarray(x0, np.float32)
+    _a1 = np.ascontiguousarray(u, np.float32)
+    load().oracle_state_trajectory_f64(ct.byref(dyn), ptr(_a0), ptr(_a1), T, float(np.float32(prob.dt)), ptr(out))
+    return out
+
+
+def free_energy(w: np.ndarray, baseline: float, lam: float):
+    """(mean, variance, modified variance) of core/mppi_common.cu:1101-1117 in FP64 from the weights."""
+    m, v, mod = _d(), _d(), _d()
+    _a0 = np.ascontiguousarray(w, np.float64)
+    load().oracle_free_energy(ptr(_a0), _a0.size, baseline, lam, ct.byref(m), ct.byref(v), ct.byref(mod))
+    return m.value, v.value, mod.value
 
 
 def enforce(prob, u: np.ndarray) -> np.ndarray:
