@@ -1,14 +1,23 @@
 #!/usr/bin/env python3
 """MPPI benchmark on one MI355X.
 
-Workload (default): 2-D double integrator with power-law (colored) noise and a 1024x1024 cost map, K = 8192 samples,
-T = 100, C = 2 -- one step is the full GPU work of one computeControl call with num_iters = 1 (colored sampling,
-rollout with cost-map lookups, weights, weighted update). The timed region replays device-resident steps (x0, the
-nominal sequence and the map already in HBM) from a captured hipGraph, bracketed by device synchronisations.
+Headline workload (default, `--workload batch256`): 256 independent MPPI instances (double integrator / kinematic
+bicycle 50/50, K ~ U[512, 8192] in multiples of 64, T ~ U[50, 150], the 1024 x 1024 blob cost map), advanced by one
+persistent rollout launch + one combine launch per control call, launch shape from the autotuner. One step is one
+host-inclusive computeControl(x, 1) of the whole batch: the 256 states uploaded, the iteration, the 256 new control
+sequences downloaded and smoothed on the host. `value` = steps / wall time of exactly --steps such calls (Hz);
+rollout_steps_per_s = value * sum_i K_i T_i.
 
-Also reported: the host-inclusive control rate (median wall time of computeControl(x, 1), which adds the 16 B x0
-upload, the 800 B control download, SG smoothing and the host state propagation), per-kernel HIP-event durations
-with the roofline of the dominant kernel, and the CPU restatement of the reference rollout loop timed on this host.
+The same line carries, from the same run:
+  - the device-resident rate (graph replays with the inputs already in HBM, no host transfer): device_loop_hz;
+  - per-kernel durations from HIP events on the launching stream, and the roofline of the dominant kernel;
+  - the CPU restatement of the reference's rollout loop (+ weights + update) on the same noise, timed on this host on
+    a bounded sample, on one core and on all cores;
+  - `configs`: the single-controller configurations (cart-pole K=2048; double integrator + colored noise + map
+    K=8192; bicycle + BF16 residual network K=16384), each with its host-inclusive and device-resident control rate,
+    its fused kernel's roofline and combine-kernel bandwidth, and its CPU baseline;
+  - `learned_vs_plain`: the learned-proposal comparison on the bicycle + network task.
+`--workload cartpole|di_colored_map|bicycle_nn` makes that controller configuration the headline instead.
 """
 from __future__ import annotations
 
@@ -30,33 +39,48 @@ HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 # FLOPs of the 9-64-64-4 residual network per sample-step: 2 (9*64 + 64*64 + 64*4)
 NN_FLOP_PER_STEP = 2 * (9 * 64 + 64 * 64 + 64 * 4)
-# kernels behind each timed entry point, as named in rocprofv3 traces
-KERNEL_NAMES = {"sample_colored": ["colored_mfma_kernel"], "sample_gaussian": ["sample_gaussian_kernel"],
-                "rollout": ["rollout_kernel"], "weights": ["weights_kernel"],
-                "weighted_update": ["update_partial_kernel", "update_columns_kernel"]}
+PARTIAL_BYTES = 32  # sizeof(Partial): baseline, argmin, three FP64 sums
 
 
-def measured_traffic(workload: str, entry: str):
-    """HBM bytes per launch of `entry` from the committed rocprofv3 counter summary of this workload (FETCH_SIZE and
-    WRITE_SIZE scaled by a plain copy of known size), or None when there is none."""
+def host_threads() -> int:
+    """Host cores this process may use: the scheduler's affinity set, capped by OMP_NUM_THREADS when set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def fused_alg_bytes(K: int, T: int, C: int) -> int:
+    """Algorithmic HBM bytes of one fused iteration over K samples: the Gaussian/colored sampler's write (4 K T C),
+    the rollout's read and write of the samples and its costs (8 K T C + 4 K) and the weighted update's read of the
+    samples and weights and write of the mean (4 K T C + 4 K + 4 T C) -- the three per-unit figures of the kernels
+    the fused kernel replaces."""
+    return 16 * K * T * C + 8 * K + 4 * T * C
+
+
+def rollout_alg_bytes(K: int, T: int, C: int) -> int:
+    return 8 * K * T * C + 4 * K
+
+
+def measured_traffic(workload: str, kernel_prefix: str):
+    """HBM bytes per dispatch of the kernel whose name starts with `kernel_prefix`, from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known size), or None."""
     path = ROOT / "profiles" / f"mppi_iteration_traffic_{workload}.json"
     if not path.exists():
         return None
     tab = json.loads(path.read_text())["traffic"]
-    total = 0
-    for prefix in KERNEL_NAMES[entry]:
-        hits = [v for k, v in tab.items() if k.startswith(prefix + "<") or k == prefix]
-        if len(hits) != 1:
-            return None
-        total += hits[0]["total_bytes"]
-    return total
+    hits = [v for k, v in tab.items() if k.startswith(kernel_prefix)]
+    return hits[0]["total_bytes"] if len(hits) == 1 else None
 
 
 def measured_utilisation(workload: str, prefixes):
-    """VALU and MFMA utilisation of this workload's kernels whose names start with one of `prefixes`, from the
-    committed rocprofv3 counter summary (derived counters VALUBusy: VALU issue time as a percentage of the CUs'
-    capacity; VALUUtilization: active lanes per VALU instruction; MfmaUtil: matrix-core busy cycles as a percentage
-    of SIMD cycles), or None when there is none."""
+    """VALU and MFMA utilisation of this workload's kernels (committed rocprofv3 derived counters VALUBusy,
+    VALUUtilization, MfmaUtil), or None."""
     path = ROOT / "profiles" / f"utilisation_{workload}.json"
     if not path.exists():
         return None
@@ -83,66 +107,6 @@ def build_problem(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def time_kernels(prob, reps: int):
-    """Average duration of each kernel of one iteration, HIP events on the stream the kernel is launched on."""
-    import ctypes as ct
-
-    from mppi_generic_vendor_amd import capi
-    from mppi_generic_vendor_amd.hip import DeviceBuffer, Event, Stream
-
-    K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
-    s = Stream()
-    dyn = capi.DynamicsParams.from_buffer_copy(prob.dyn)
-    cost = capi.CostParams.from_buffer_copy(prob.cost)
-    keep = []
-    if prob.cost_map is not None:
-        mb = DeviceBuffer.from_numpy(prob.cost_map)
-        keep.append(mb)
-        cost.map = mb.ptr
-    if prob.nn_theta is not None:
-        tb = DeviceBuffer.from_numpy(prob.nn_theta)
-        keep.append(tb)
-        dyn.nn_theta = tb.ptr
-    x0 = DeviceBuffer.from_numpy(prob.x0)
-    mean = DeviceBuffer.zeros((T, C))
-    samples = DeviceBuffer(K * T * C * 4)
-    costs = DeviceBuffer(K * 4)
-    stats = DeviceBuffer(64)
-    out = DeviceBuffer(T * C * 4)
-    sampler_name = "sample_colored" if prob.sampler.kind == capi.SAMPLER_COLORED else "sample_gaussian"
-
-    def sample(i):
-        capi.call("mppi_" + sampler_name, ct.byref(prob.sampler), K, T, mean.ptr, samples.ptr, 1, 0,
-                  ct.c_uint64(i), None, s.ptr)
-
-    def rollout(i):
-        capi.call("mppi_rollout", ct.byref(dyn), ct.byref(cost), ct.byref(prob.sampler), K, T, prob.dt, prob.lambda_,
-                  prob.alpha, x0.ptr, mean.ptr, samples.ptr, costs.ptr, s.ptr)
-
-    def weights(i):
-        capi.call("mppi_compute_weights", costs.ptr, K, prob.lambda_, stats.ptr, s.ptr)
-
-    def update(i):
-        capi.call("mppi_weighted_update", costs.ptr, stats.ptr, samples.ptr, K, T, C, out.ptr, s.ptr)
-
-    res = {}
-    sample(0)
-    for name, fn in ((sampler_name, sample), ("rollout", rollout), ("weights", weights), ("weighted_update", update)):
-        # back-to-back launches between two events on the launching stream (the rollout's clamping is idempotent,
-        # so repeated rollouts of one sample buffer do identical work)
-        for i in range(3):
-            fn(i)
-        s.synchronize()
-        e0, e1 = Event(), Event()
-        e0.record(s)
-        for i in range(reps):
-            fn(i)
-        e1.record(s)
-        e1.synchronize()
-        res[name] = e0.elapsed_ms(e1) / reps
-    return res
-
-
 def samples_for(prob, k: int, mean: np.ndarray) -> np.ndarray:
     """The first k control samples of the GPU sampler's buffer for this problem (same seed, rng offset 0)."""
     import ctypes as ct
@@ -159,264 +123,215 @@ def samples_for(prob, k: int, mean: np.ndarray) -> np.ndarray:
     return out.to_numpy((k, T, C))
 
 
-def cpu_baseline(prob, seconds: float, threads: int):
-    """The CPU restatement of the reference rollout loop (+ weights and update) on a bounded sample of the same
-    workload and noise, timed on this host; scaled to iterations/s of the full K."""
+def cpu_baseline(items, total_steps: int, seconds: float, threads: int, what: str):
+    """The oracle's FP32 rollout loop (the reference's launchCPURolloutKernel restated) + weights + weighted update
+    over `items` = [(problem, samples, mean)], repeated until `seconds` have passed (at least once), on `threads`
+    host threads; the measured seconds per sample-step give the rate of a full iteration of `total_steps`
+    sample-steps."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_lib as ol
 
-    K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
-    mean = np.zeros((T, C), np.float32)
-    k_probe = 256
-    u = samples_for(prob, k_probe, mean)
-    t0 = time.perf_counter()
-    costs, clamped = ol.rollout_f32(prob, u, mean, nthreads=threads)
-    w, eta, *_ = ol.weights(costs, prob.lambda_)
-    ol.weighted_update(w, eta, clamped)
-    dt_probe = max(time.perf_counter() - t0, 1e-6)
-    k_sample = int(min(K, max(k_probe, k_probe * seconds / dt_probe / 3)))
-    u = samples_for(prob, k_sample, mean)
+    steps = sum(u.shape[0] * u.shape[1] for _, u, _ in items)
     reps = 0
     t0 = time.perf_counter()
     while True:
-        costs, clamped = ol.rollout_f32(prob, u, mean, nthreads=threads)
-        w, eta, *_ = ol.weights(costs, prob.lambda_)
-        ol.weighted_update(w, eta, clamped)
+        for prob, u, mean in items:
+            costs, clamped = ol.rollout_f32(prob, u, mean, nthreads=threads)
+            w, eta, *_ = ol.weights(costs, prob.lambda_)
+            ol.weighted_update(w, eta, clamped)
         reps += 1
         el = time.perf_counter() - t0
-        if el > seconds * 0.5 or reps >= 50:
+        if el >= seconds or reps >= 1000:
             break
-    per_iter_full = el / reps * (K / k_sample)
-    return {
-        "value": 1.0 / per_iter_full,
-        "unit": "Hz",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{k_sample} of {K} samples x {T} steps, {reps} reps, rollout+weights+update, scaled to K={K}",
-        "rollout_steps_per_s": k_sample * T * reps / el,
-    }
+    per_step = el / (reps * steps)
+    return {"value": round(1.0 / (per_step * total_steps), 4), "unit": "Hz", "cores": threads, "kind": "port",
+            "sample": f"{what}, {reps} pass(es) of {steps} sample-steps in {el:.1f} s, scaled to the full "
+                      f"{total_steps} sample-steps",
+            "rollout_steps_per_s": round(1.0 / per_step, 1)}
 
 
+def cpu_baselines(items, total_steps: int, seconds: float, what: str):
+    """Both modes of the reference's CPU loop: all host cores (the object's main fields) and one core."""
+    n = host_threads()
+    allc = cpu_baseline(items, total_steps, seconds * 0.4, n, what)
+    one = cpu_baseline(items, total_steps, seconds * 0.6, 1, what)
+    allc["single_core"] = {k: one[k] for k in ("value", "rollout_steps_per_s", "sample")}
+    return allc
+
+
+# ------------------------------------------------------------------------------------------------ batch (C4)
 def run_batch(args):
-    """256 independent instances (50/50 double integrator / bicycle, K ~ U[512, 8192] in multiples of 64,
-    T ~ U[50, 150]) advanced by one persistent launch + one combine launch per iteration, shape from the autotuner."""
     from mppi_generic_vendor_amd import hip, workloads
     from mppi_generic_vendor_amd.batch import MPPIBatch
 
     probs, m = workloads.batch_mix(256, seed=7)
-    ks = sum(p.num_rollouts * p.num_timesteps for p in probs)
+    KT = sum(p.num_rollouts * p.num_timesteps for p in probs)
     b = MPPIBatch(probs, cost_map=m)
     cfg = b.autotune(reps=3)
-    print(f"[bench] batch256: {sum(p.num_rollouts for p in probs)} samples, {ks} sample-steps per iteration; "
-          f"autotuned threads={cfg.threads} samples/lane={cfg.samples_per_lane} chunk={cfg.chunk} "
-          f"blocks/CU={cfg.blocks_per_cu} ({cfg.iteration_us:.1f} us)", file=sys.stderr, flush=True)
+    log(f"batch256: {sum(p.num_rollouts for p in probs)} samples, {KT} sample-steps per call; autotuned "
+        f"threads={cfg.threads} samples/lane={cfg.samples_per_lane} chunk={cfg.chunk} blocks/CU={cfg.blocks_per_cu}")
     x0 = [p.x0 for p in probs]
-    for _ in range(3):
+    for _ in range(args.warmup):
         b.computeControl(x0, 1)
-    host_ms = []
-    for _ in range(max(5, args.host_calls // 10)):
-        t0 = time.perf_counter()
-        b.computeControl(x0, 1)
-        host_ms.append((time.perf_counter() - t0) * 1e3)
-    b.stepDevice(args.warmup)
+    # timed region of `value`: exactly --steps host-inclusive control calls
     hip.synchronize()
-    stream = hip.Stream.__new__(hip.Stream)
-    stream.ptr = b.stream()
-    e0, e1 = hip.Event(), hip.Event()
+    per = []
     t0 = time.perf_counter()
-    e0.record(stream)
-    b.stepDevice(args.steps)
-    e1.record(stream)
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        b.computeControl(x0, 1)
+        per.append(time.perf_counter() - t1)
     hip.synchronize()
     wall = time.perf_counter() - t0
-    stream.ptr = None
-    ev_ms = e0.elapsed_ms(e1) / args.steps
     value = args.steps / wall
+    log(f"host-inclusive batched computeControl: {wall * 1e3 / args.steps:.4f} ms/call")
     for i in range(len(probs)):
         st = b.getWeightStats(i)
         if not np.isfinite(st.normalizer) or st.normalizer <= 0:
             raise SystemExit(f"batch instance {i} produced an invalid normalizer")
+    # device-resident iterations (inputs in HBM, graph replays)
+    b.stepDevice(args.warmup)
+    hip.synchronize()
+    n_dev = max(args.steps, 20)
+    t0 = time.perf_counter()
+    b.stepDevice(n_dev)
+    hip.synchronize()
+    dev_wall = time.perf_counter() - t0
+    roll_ms, comb_ms = b.timeKernels(args.kernel_reps)
+    log(f"device loop {dev_wall * 1e3 / n_dev:.4f} ms/call; kernels: rollout {roll_ms:.4f} ms, combine "
+        f"{comb_ms:.4f} ms")
+    spl = cfg.samples_per_lane
+    tiles = [-(-p.num_rollouts // (64 * spl)) for p in probs]
+    alg = sum(fused_alg_bytes(p.num_rollouts, p.num_timesteps, p.C) for p in probs)
+    alg_roll = sum(rollout_alg_bytes(p.num_rollouts, p.num_timesteps, p.C) for p in probs)
+    comb_bytes = sum(g * (4 * p.num_timesteps * p.C + PARTIAL_BYTES) + 4 * p.num_timesteps * p.C + 40
+                     for g, p in zip(tiles, probs))
+    kname = f"batch_rollout_kernel<{cfg.threads}, {spl}>"
+    achieved = alg / (roll_ms * 1e-3) / 1e9
+    roofline = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("batch256", kname),
+                "algorithmic_bytes": alg, "avg_ms": round(roll_ms, 5),
+                "rollout_only": {"algorithmic_bytes": alg_roll,
+                                 "frac": round(alg_roll / (roll_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}}
+    cpu = None
+    if not args.no_cpu:
+        sample = probs[::16]
+        items = [(p, samples_for(p, p.num_rollouts, np.zeros((p.num_timesteps, p.C), np.float32)),
+                  np.zeros((p.num_timesteps, p.C), np.float32)) for p in sample]
+        cpu = cpu_baselines(items, KT, args.cpu_seconds,
+                            f"{len(sample)} of the 256 instances (every 16th, full K and T)")
     b.close()
-    # algorithmic HBM bytes of one iteration: clamped samples written (4 K T C), costs (4 K), and the per-tile
-    # partial rows (T C floats per 64-sample tile) written and read back by the combine
-    alg = sum(4 * p.num_rollouts * p.num_timesteps * p.C + 4 * p.num_rollouts
-              + 2 * 4 * (p.num_rollouts // 64) * p.num_timesteps * p.C for p in probs)
-    achieved = alg / (ev_ms * 1e-3) / 1e9
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Hz", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 5), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded map, states, goals)",
-        "config": {"workload": "batch256: 256 instances (DI / bicycle 50/50), K~U[512,8192]/64, T~U[50,150], "
-                               "one persistent launch + combine per iteration, hipGraph replay",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded map, states, goals; Philox noise)",
+        "config": {"workload": "batch256: 256 MPPI instances (double integrator / bicycle 50/50), "
+                               "K~U[512,8192]/64, T~U[50,150], 1024x1024 cost map; one step = one host-inclusive "
+                               "batched computeControl (x0 upload, persistent rollout + combine, control download "
+                               "and smoothing)",
+                   "instances": len(probs), "sample_steps_per_call": KT,
                    "launch_shape": {"threads": cfg.threads, "samples_per_lane": cfg.samples_per_lane,
                                     "chunk": cfg.chunk, "blocks_per_cu": cfg.blocks_per_cu}},
-        "rollout_steps_per_s": round(value * ks, 1),
-        "control_hz_host": round(1e3 / statistics.median(host_ms), 3),
-        "roofline": {"kernel": "batch_rollout_kernel+batch_combine_kernel", "bound": "hbm",
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes": alg,
-                     "avg_ms": round(ev_ms, 5)},
-        "utilisation": measured_utilisation(
-            "batch256", [f"batch_rollout_kernel<{cfg.threads}, {cfg.samples_per_lane}>"]),
-        "cpu_baseline": None,
+        "rollout_steps_per_s": round(value * KT, 1),
+        "instance_controls_per_s": round(value * len(probs), 1),
+        "control_hz_host_median": round(1.0 / statistics.median(per), 3),
+        "device_loop_hz": round(n_dev / dev_wall, 3),
+        "device_loop_rollout_steps_per_s": round(n_dev / dev_wall * KT, 1),
+        "kernels_ms": {kname: round(roll_ms, 5), "batch_combine_kernel": round(comb_ms, 5)},
+        "reduction_kernels": {"batch_combine_kernel": {
+            "algorithmic_bytes": comb_bytes, "avg_ms": round(comb_ms, 5),
+            "achieved_gbs": round(comb_bytes / (comb_ms * 1e-3) / 1e9, 2)}},
+        "roofline": roofline,
+        "utilisation": measured_utilisation("batch256", [kname]),
+        "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="di_colored_map",
-                    choices=["di_colored_map", "cartpole", "bicycle_nn", "batch256"])
-    ap.add_argument("--host-calls", type=int, default=200)
-    ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--learned-episodes", type=int, default=4,
-                    help="closed-loop episodes of the learned-proposal comparison (0: skip)")
-    ap.add_argument("--learned-eager", action="store_true",
-                    help="run the learned-proposal comparison with eager launches instead of graph replay "
-                         "(for kernel-trace profiles of that path)")
-    ap.add_argument("--calibration-mib", type=int, default=256,
-                    help="device-to-device copy of this size, a plain copy kernel of known traffic for counter scaling")
-    args = ap.parse_args()
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: there is no multi-GPU path (pass --gpus 1)",
-              file=sys.stderr)
-        sys.exit(2)
-
+# ------------------------------------------------------------------------------------------ one controller
+def run_controller(name: str, args, steps: int, warmup: int, cpu_seconds: float):
+    """Host-inclusive computeControl rate over exactly `steps` calls, the device-resident rate, the fused kernel's
+    and the combine kernel's durations and rooflines, and the CPU baseline, for one controller configuration."""
     from mppi_generic_vendor_amd import capi, hip
     from mppi_generic_vendor_amd.controller import VanillaMPPIController
 
-    capi.load()
-    if hip.device_count() < 1:
-        raise SystemExit("no HIP device visible")
-    if args.workload == "batch256":
-        run_batch(args)
-        return
-    prob = build_problem(args.workload)
+    prob = build_problem(name)
     K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
-    print(f"[bench] workload {args.workload} K={K} T={T} C={C}", file=sys.stderr, flush=True)
-
     ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True)
     x = prob.x0.copy()
-    # host-inclusive control rate: computeControl(x, 1) from the x0 upload to the smoothed host control
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         ctrl.computeControl(x, 1)
-    host_ms = []
-    for _ in range(args.host_calls):
-        t0 = time.perf_counter()
-        ctrl.computeControl(x, 1)
-        host_ms.append((time.perf_counter() - t0) * 1e3)
-    control_hz_host = 1e3 / statistics.median(host_ms)
-    print(f"[bench] host-inclusive computeControl median {statistics.median(host_ms):.4f} ms", file=sys.stderr,
-          flush=True)
-
-    # device-resident steps (the timed region of `value`)
-    ctrl.stepDevice(args.warmup)
     hip.synchronize()
-    stream = hip.Stream.__new__(hip.Stream)
-    stream.ptr = ctrl.stream()
-    e0, e1 = hip.Event(), hip.Event()
-    hip.synchronize()
+    per = []
     t0 = time.perf_counter()
-    e0.record(stream)
-    ctrl.stepDevice(args.steps)
-    e1.record(stream)
-    hip.synchronize()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        ctrl.computeControl(x, 1)
+        per.append(time.perf_counter() - t1)
     wall = time.perf_counter() - t0
-    stream.ptr = None
-    ev_ms = e0.elapsed_ms(e1)
-    value = args.steps / wall
-    ms_per_step = wall * 1e3 / args.steps
-    print(f"[bench] device-resident {ms_per_step:.4f} ms/step (events {ev_ms / args.steps:.4f})", file=sys.stderr,
-          flush=True)
     st = ctrl.getWeightStats()
     if not np.isfinite(st.normalizer) or st.normalizer <= 0:
-        raise SystemExit("benchmark iteration produced an invalid normalizer")
+        raise SystemExit(f"{name}: invalid normalizer")
+    ctrl.stepDevice(warmup)
+    hip.synchronize()
+    n_dev = max(steps, 200)
+    t0 = time.perf_counter()
+    ctrl.stepDevice(n_dev)
+    hip.synchronize()
+    dev_wall = time.perf_counter() - t0
+    roll_ms, comb_ms, G = ctrl.timeKernels(args.kernel_reps)
     ctrl.close()
-
-    if args.calibration_mib > 0:
-        # a plain copy kernel of known size (reads and writes calibration_mib MiB), for scaling HBM counters
-        nb = args.calibration_mib << 20
-        a_buf, b_buf = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
-        hip.check(hip._load().hipMemcpy(b_buf.ptr, a_buf.ptr, nb, hip.D2D), "calibration copy")
-        hip.synchronize()
-        a_buf.free()
-        b_buf.free()
-    kern = time_kernels(prob, args.kernel_reps)
-    print(f"[bench] kernel ms {kern}", file=sys.stderr, flush=True)
-    # algorithmic bytes per launch (4 B per FP32): sampler writes K*T*C, rollout reads + writes K*T*C and writes K
-    alg = {
-        "sample_colored": 4 * K * T * C, "sample_gaussian": 4 * K * T * C,
-        "rollout": 8 * K * T * C + 4 * K,
-        "weights": 8 * K + 16,
-        "weighted_update": 4 * K * T * C + 4 * K + 4 * T * C,
-    }
-    dom = max((k for k in kern if k != "weights"), key=lambda k: kern[k])
-    if "sample_colored" in kern:
-        # the colored stage is the larger part of the controller's fused kernel (about 27 K of 50 K cycles per
-        # workgroup), and the kernel-level sampler and rollout times lie within noise of each other
-        dom = "sample_colored"
-    if prob.dyn.model == capi.MODEL_RACER_DUBINS_NN and dom == "rollout":
-        # the network on the matrix cores bounds the rollout: algorithmic FLOPs per launch over its duration
+    log(f"{name}: host-inclusive {wall * 1e3 / steps:.4f} ms/call, device loop {dev_wall * 1e3 / n_dev:.4f} ms/call, "
+        f"fused kernel {roll_ms:.4f} ms, combine {comb_ms:.4f} ms over {G} partials")
+    kname = "rollout_update_kernel"
+    alg = fused_alg_bytes(K, T, C)
+    bf16 = prob.dyn.model == capi.MODEL_RACER_DUBINS_NN and prob.dyn.nn_precision == capi.NN_BF16
+    if bf16:
         flops = NN_FLOP_PER_STEP * K * T
-        achieved = flops / (kern[dom] * 1e-3) / 1e12
-        roofline = {"kernel": "rollout_kernel", "entry_point": "mppi_rollout", "bound": "mfma",
-                    "achieved": round(achieved, 3), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 5),
-                    "traffic": measured_traffic(args.workload, dom), "algorithmic_flops": flops,
-                    "avg_ms": round(kern[dom], 5)}
+        ach = flops / (roll_ms * 1e-3) / 1e12
+        roofline = {"kernel": kname, "bound": "mfma", "achieved": round(ach, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(ach / MFMA_BF16_PEAK_TFLOPS, 5),
+                    "traffic": measured_traffic(name, kname), "algorithmic_flops": flops, "avg_ms": round(roll_ms, 5)}
     else:
-        achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
-        roofline = {"kernel": "+".join(KERNEL_NAMES[dom]), "entry_point": "mppi_" + dom, "bound": "hbm",
-                    "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dom),
-                    "algorithmic_bytes": alg[dom], "avg_ms": round(kern[dom], 5)}
-
+        ach = alg / (roll_ms * 1e-3) / 1e9
+        roofline = {"kernel": kname, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": measured_traffic(name, kname),
+                    "algorithmic_bytes": alg, "avg_ms": round(roll_ms, 5),
+                    "rollout_only": {"algorithmic_bytes": rollout_alg_bytes(K, T, C),
+                                     "frac": round(rollout_alg_bytes(K, T, C) / (roll_ms * 1e-3) / 1e9
+                                                   / HBM_PEAK_GBS, 5)}}
+    comb_bytes = G * (4 * T * C + PARTIAL_BYTES) + 4 * T * C + 40
     cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(prob, args.cpu_seconds, 1)
-    learned_cmp = (learned_comparison(args.learned_episodes, not args.learned_eager)
-                   if args.workload == "di_colored_map" else None)
-
-    line = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "Hz",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+bf16" if prob.dyn.model == capi.MODEL_RACER_DUBINS_NN and prob.dyn.nn_precision == capi.NN_BF16
-                 else "f32",
-        "data": "synthetic (seeded cost map, Philox noise)",
-        "config": {"workload": f"{args.workload}: K={K} samples, T={T}, S={prob.S}, C={C}, dt={prob.dt}, "
-                               f"num_iters=1, device-resident graph replay"},
-        "rollout_steps_per_s": round(value * K * T, 1),
-        "control_hz_host": round(control_hz_host, 3),
-        "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
+    if cpu_seconds > 0:
+        mean = np.zeros((T, C), np.float32)
+        k_s = min(K, 2048)
+        cpu = cpu_baselines([(prob, samples_for(prob, k_s, mean), mean)], K * T, cpu_seconds,
+                            f"{k_s} of {K} samples x {T} steps")
+    return {
+        "workload": f"{name}: K={K}, T={T}, S={prob.S}, C={C}, dt={prob.dt}, num_iters=1",
+        "control_hz_host": round(steps / wall, 3),
+        "control_hz_host_median": round(1.0 / statistics.median(per), 3),
+        "rollout_steps_per_s": round(steps / wall * K * T, 1),
+        "device_loop_hz": round(n_dev / dev_wall, 3),
+        "device_loop_rollout_steps_per_s": round(n_dev / dev_wall * K * T, 1),
+        "kernels_ms": {kname: round(roll_ms, 5), "combine_kernel": round(comb_ms, 5)},
         "roofline": roofline,
-        # the controller iteration's fused kernel and the kernel behind the roofline entry point
-        "utilisation": measured_utilisation(args.workload, ["rollout_update_kernel<"] + KERNEL_NAMES[dom]),
+        "reduction_kernels": {"combine_kernel": {"partials": G, "algorithmic_bytes": comb_bytes,
+                                                 "avg_ms": round(comb_ms, 5),
+                                                 "achieved_gbs": round(comb_bytes / (comb_ms * 1e-3) / 1e9, 2)}},
+        "utilisation": measured_utilisation(name, [kname]),
         "cpu_baseline": cpu,
-        "learned_vs_plain": learned_cmp,
+        "dtype": "f32+bf16" if bf16 else "f32",
+        "_wall": wall,
     }
-    print(json.dumps(line), flush=True)
 
 
 def learned_comparison(episodes: int, use_graph: bool = True):
     """Learned vs plain MPPI on the bicycle + residual-network task with the committed trained proposal: closed-loop
-    executed-trajectory cost from held-out starts, and the stream time (from the event before the call's first
-    launch to the one recorded after its last) and host wall time per control step, of plain MPPI (8192 samples,
-    3 iterations), the learned proposal (2048 samples, 1 iteration, fallback above 0.5 %), and plain MPPI with the
-    proposal's samples and one iteration (about the learned step's time)."""
+    executed-trajectory cost from held-out starts, and the stream time and host wall time per control step, of
+    plain MPPI (8192 samples, 3 iterations), the learned proposal (2048 samples, 1 iteration, fallback above 0.5 %),
+    and plain MPPI with the proposal's samples and one iteration."""
     from mppi_generic_vendor_amd import learned, train_proposal, workloads
 
     if episodes <= 0 or not train_proposal.DEFAULT_OUT.exists():
@@ -428,9 +343,8 @@ def learned_comparison(episodes: int, use_graph: bool = True):
     for v in res.values():
         v.pop("costs", None)
     p, l, s = res["plain"], res["learned"], res["plain_small"]
-    print(f"[bench] learned vs plain: cost {l['cost']:.4f} vs {p['cost']:.4f} (same samples, one iteration: "
-          f"{s['cost']:.4f}), {l['hz']:.0f} vs {p['hz']:.0f} control Hz (stream time), {l['hz_host']:.0f} vs "
-          f"{p['hz_host']:.0f} (host wall time)", file=sys.stderr, flush=True)
+    log(f"learned vs plain: cost {l['cost']:.4f} vs {p['cost']:.4f} (same samples, one iteration: {s['cost']:.4f}), "
+        f"{l['hz_host']:.0f} vs {p['hz_host']:.0f} control Hz (host wall time)")
     return {"task": "bicycle + BF16 residual MLP, K=8192 T=100, closed loop from held-out starts",
             "episodes": episodes, "trained_on_pairs": summary.get("pairs"),
             "plain": p, "learned": l, "plain_same_samples": s,
@@ -438,6 +352,77 @@ def learned_comparison(episodes: int, use_graph: bool = True):
             "rate_ratio_learned_to_plain": round(l["hz"] / p["hz"], 4),
             "rate_ratio_host_learned_to_plain": round(l["hz_host"] / p["hz_host"], 4),
             "cost_gain_over_same_samples": round((s["cost"] - l["cost"]) / s["cost"], 6)}
+
+
+CONTROLLER_WORKLOADS = ("cartpole", "di_colored_map", "bicycle_nn")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="batch256", choices=["batch256", *CONTROLLER_WORKLOADS])
+    ap.add_argument("--host-calls", type=int, default=200,
+                    help="timed computeControl calls of each configuration in `configs`")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time of the headline workload")
+    ap.add_argument("--config-cpu-seconds", type=float, default=3.0,
+                    help="CPU-baseline time of each configuration in `configs`")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
+    ap.add_argument("--learned-episodes", type=int, default=4,
+                    help="closed-loop episodes of the learned-proposal comparison (0: skip)")
+    ap.add_argument("--learned-eager", action="store_true",
+                    help="run the learned-proposal comparison with eager launches instead of graph replay")
+    ap.add_argument("--calibration-mib", type=int, default=256,
+                    help="device-to-device copy of this size, a plain copy kernel of known traffic for counter scaling")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: every GPU box of this setup has one GPU and the engine has no "
+              "multi-GPU path (pass --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from mppi_generic_vendor_amd import capi, hip
+
+    capi.load()
+    if hip.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    if args.calibration_mib > 0:
+        # a plain copy kernel of known size (reads and writes calibration_mib MiB), for scaling HBM counters
+        nb = args.calibration_mib << 20
+        a_buf, b_buf = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
+        hip.check(hip._load().hipMemcpy(b_buf.ptr, a_buf.ptr, nb, hip.D2D), "calibration copy")
+        hip.synchronize()
+        a_buf.free()
+        b_buf.free()
+
+    if args.workload == "batch256":
+        line = run_batch(args)
+    else:
+        r = run_controller(args.workload, args, args.steps, args.warmup, 0 if args.no_cpu else args.cpu_seconds)
+        prob = build_problem(args.workload)
+        KT = prob.num_rollouts * prob.num_timesteps
+        line = {
+            "metric": METRIC, "value": r["control_hz_host"], "unit": "Hz", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(r["_wall"] * 1e3 / args.steps, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": r["dtype"],
+            "data": "synthetic (seeded cost map, Philox noise)",
+            "config": {"workload": r["workload"] + "; one step = one host-inclusive computeControl(x, 1)"},
+            "sample_steps_per_call": KT,
+        }
+        line.update({k: v for k, v in r.items() if k not in ("workload", "control_hz_host", "_wall", "dtype")})
+    if not args.no_configs:
+        line["configs"] = {}
+        for name in CONTROLLER_WORKLOADS:
+            if name == args.workload:
+                continue
+            r = run_controller(name, args, args.host_calls, 20, 0 if args.no_cpu else args.config_cpu_seconds)
+            r.pop("_wall")
+            line["configs"][name] = r
+    line["learned_vs_plain"] = learned_comparison(args.learned_episodes, not args.learned_eager)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

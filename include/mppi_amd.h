@@ -397,6 +397,10 @@ int mppi_batch_slide_control_seq(mppi_batch* batch, int32_t steps);
 /* Clamped control samples [K][T][C] and costs [K] of an instance's last iteration (copied to the host). */
 int mppi_batch_get_samples(const mppi_batch* batch, int32_t instance, float* samples_host, float* costs_host);
 int mppi_batch_stream(const mppi_batch* batch, void** stream);
+/* Benchmark utility: `reps` device-loop iterations launched eagerly with HIP events around the persistent rollout
+ * launch and the combine launch on the batch's stream; average durations of each. Needs a computeControl call first
+ * (as mppi_batch_step_device, which also returns MPPI_E_INVALID before it). */
+int mppi_batch_time_kernels(mppi_batch* batch, int32_t reps, float* rollout_ms, float* combine_ms);
 
 /* ------------------------------------------------------------------------------------------------------------------
  * Learned proposal network (no counterpart in the reference). Inputs: the state (S values) and the previous control
@@ -451,6 +455,13 @@ int mppi_controller_get_proposal_stats(const mppi_controller* ctrl, int32_t* pro
 /* Stream time of the last computeControl from its first upload to its last download (HIP events), i.e. the
  * control latency without the host post-processing (smoothing, state propagation). */
 int mppi_controller_get_last_call_ms(const mppi_controller* ctrl, float* ms);
+/* Benchmark utility (no counterpart in the reference): `reps` iterations of the device loop (as
+ * mppi_controller_step_device, iteration 0 of a call) launched eagerly with HIP events on the controller's stream
+ * before the rollout kernel, between it and the combine kernel, and after the combine; the average durations of the
+ * two launches are returned, and the number of per-workgroup partials the combine reduces (num_partials may be
+ * NULL). Advances the device loop's state like step_device. Needs a computeControl call first. */
+int mppi_controller_time_kernels(mppi_controller* ctrl, int32_t reps, float* rollout_ms, float* combine_ms,
+                                 int32_t* num_partials);
 
 #ifdef __cplusplus
 } /* extern "C" */

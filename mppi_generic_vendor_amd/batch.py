@@ -111,6 +111,13 @@ class MPPIBatch:
         capi.call("mppi_batch_stream", self._h, ct.byref(s))
         return s.value
 
+    def timeKernels(self, reps: int = 20):
+        """Average (persistent rollout kernel, combine kernel) durations in ms over `reps` device-loop iterations,
+        from HIP events on the batch's stream."""
+        a, b = ct.c_float(), ct.c_float()
+        capi.call("mppi_batch_time_kernels", self._h, int(reps), ct.byref(a), ct.byref(b))
+        return a.value, b.value
+
     def close(self):
         if self._h:
             capi.call("mppi_batch_destroy", self._h)

@@ -183,6 +183,8 @@ _SIGS = {
     "mppi_controller_get_proposal_stats": ([P, ct.POINTER(_i), ct.POINTER(_i), ct.POINTER(WeightStats),
                                             ct.POINTER(_f)], ct.c_int),
     "mppi_controller_get_last_call_ms": ([P, ct.POINTER(_f)], ct.c_int),
+    "mppi_controller_time_kernels": ([P, _i, ct.POINTER(_f), ct.POINTER(_f), ct.POINTER(_i)], ct.c_int),
+    "mppi_batch_time_kernels": ([P, _i, ct.POINTER(_f), ct.POINTER(_f)], ct.c_int),
 }
 
 _lib = None

@@ -143,6 +143,14 @@ class VanillaMPPIController:
         capi.call("mppi_controller_get_proposal_stats", self._h, ct.byref(n), ct.byref(f), ct.byref(st), ct.byref(c0))
         return n.value, f.value, st, c0.value
 
+    def timeKernels(self, reps: int = 50):
+        """Average (rollout kernel, combine kernel) durations in ms over `reps` device-loop iterations, from HIP events
+        on the controller's stream (advances the device loop like stepDevice), and the number of per-workgroup
+        partials the combine reduces."""
+        a, b, g = ct.c_float(), ct.c_float(), ct.c_int32()
+        capi.call("mppi_controller_time_kernels", self._h, int(reps), ct.byref(a), ct.byref(b), ct.byref(g))
+        return a.value, b.value, g.value
+
     def lastCallMs(self) -> float:
         """Stream time of the last computeControl without the host post-processing."""
         v = ct.c_float()

@@ -471,6 +471,8 @@ struct mppi_batch {
   hipGraph_t loop_graph = nullptr;
   hipGraphExec_t loop_exec = nullptr;
   int eager_runs = 0;
+  bool computed = false;    // computeControl has run once (the device loop starts from its x0 and means)
+  bool loop_ready = false;  // loop_iters_d holds the device loop's arguments
   uint64_t rng_counter = 0;
   std::vector<std::vector<float>> control, raw_mean;
   // per instance, propagated on the first request after a call from that call's x0 and smoothed controls
@@ -547,11 +549,13 @@ static int batch_apply_config(mppi_batch* b, const mppi_batch_config& c)
 }
 
 // One batched iteration: persistent rollout over every tile, then the per-instance combine.
-static int batch_enqueue_iteration(mppi_batch* b, IterArgs* iters, int advance)
+// `between` (optional) is recorded between the persistent rollout launch and the combine launch.
+static int batch_enqueue_iteration(mppi_batch* b, IterArgs* iters, int advance, hipEvent_t between = nullptr)
 {
   MPPI_CHECK(launch_batch(b->cfg.threads, b->cfg.samples_per_lane, b->inst_d, b->work_d, b->nwork, b->queue_d, iters,
                           b->x0_d, b->means_d, b->ubuf_d, b->parts_d, b->sums_d, b->costs_d, b->cfg.chunk,
                           b->lds_bytes, b->blocks, b->stream));
+  if (between != nullptr) MPPI_HIP_CHECK(hipEventRecord(between, b->stream));
   batch_combine_kernel<<<b->n, BATCH_COMBINE_THREADS, 0, b->stream>>>(b->inst_d, b->parts_d, b->sums_d, b->means_d,
                                                                       b->stats_d, iters, advance, b->queue_d);
   MPPI_HIP_CHECK(hipGetLastError());
@@ -820,6 +824,19 @@ int mppi_batch_compute_control(mppi_batch* b, const float* x0_host, int32_t stri
     b->state_stale[i] = 1;
     clamp_controls_seq(b->desc[i].dynamics, b->control[i].data(), T);
   }
+  b->computed = true;
+  return MPPI_OK;
+}
+
+// Arguments of the device loop (once): stride 1, iteration 0, rng offsets from a range computeControl does not reach.
+static int batch_loop_args_init(mppi_batch* b)
+{
+  if (b->loop_ready) return MPPI_OK;
+  for (int i = 0; i < b->n; ++i) b->iters_h[i] = IterArgs{b->rng_counter + ((uint64_t)1 << 40), 1, 0};
+  MPPI_HIP_CHECK(hipMemcpyAsync(b->loop_iters_d, b->iters_h, sizeof(IterArgs) * b->n, hipMemcpyHostToDevice,
+                                b->stream));
+  MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->loop_ready = true;
   return MPPI_OK;
 }
 
@@ -827,12 +844,11 @@ int mppi_batch_step_device(mppi_batch* b, int32_t num_calls)
 {
   if (!b) return fail(MPPI_E_INVALID, "null argument");
   if (num_calls < 0) return fail(MPPI_E_INVALID, "num_calls must be >= 0");
+  if (!b->computed)
+    return fail(MPPI_E_INVALID, "batch step_device: call computeControl once first (the loop starts from its state)");
   if (num_calls == 0) return MPPI_OK;
   if (!b->loop_exec) {
-    for (int i = 0; i < b->n; ++i) b->iters_h[i] = IterArgs{b->rng_counter + ((uint64_t)1 << 40), 1, 0};
-    MPPI_HIP_CHECK(hipMemcpyAsync(b->loop_iters_d, b->iters_h, sizeof(IterArgs) * b->n, hipMemcpyHostToDevice,
-                                  b->stream));
-    MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+    MPPI_CHECK(batch_loop_args_init(b));
     MPPI_CHECK(batch_enqueue_iteration(b, b->loop_iters_d, 1));  // eager pass before capture
     MPPI_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int st = batch_enqueue_iteration(b, b->loop_iters_d, 1);
@@ -848,6 +864,41 @@ int mppi_batch_step_device(mppi_batch* b, int32_t num_calls)
     --num_calls;
   }
   for (int i = 0; i < num_calls; ++i) MPPI_HIP_CHECK(hipGraphLaunch(b->loop_exec, b->stream));
+  return MPPI_OK;
+}
+
+int mppi_batch_time_kernels(mppi_batch* b, int32_t reps, float* rollout_ms, float* combine_ms)
+{
+  if (!b || !rollout_ms || !combine_ms || reps < 1) return fail(MPPI_E_INVALID, "bad argument");
+  if (!b->computed) return fail(MPPI_E_INVALID, "batch time_kernels: call computeControl once first");
+  MPPI_CHECK(batch_loop_args_init(b));
+  std::vector<hipEvent_t> ev((size_t)(2 * reps + 1), nullptr);
+  int st = MPPI_OK;
+  for (auto& e : ev)
+    if (st == MPPI_OK && hipEventCreate(&e) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventCreate");
+  if (st == MPPI_OK && hipEventRecord(ev[0], b->stream) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventRecord");
+  // device-loop iterations, launch after launch: rollout between ev[2r] and ev[2r+1], combine to ev[2r+2]
+  for (int r = 0; r < reps && st == MPPI_OK; ++r) {
+    st = batch_enqueue_iteration(b, b->loop_iters_d, 1, ev[2 * r + 1]);
+    if (st == MPPI_OK && hipEventRecord(ev[2 * r + 2], b->stream) != hipSuccess)
+      st = fail(MPPI_E_HIP, "hipEventRecord");
+  }
+  float roll = 0.0f, comb = 0.0f;
+  if (st == MPPI_OK && hipEventSynchronize(ev.back()) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventSynchronize");
+  for (int r = 0; r < reps && st == MPPI_OK; ++r) {
+    float x = 0.0f, y = 0.0f;
+    if (hipEventElapsedTime(&x, ev[2 * r], ev[2 * r + 1]) != hipSuccess ||
+        hipEventElapsedTime(&y, ev[2 * r + 1], ev[2 * r + 2]) != hipSuccess)
+      st = fail(MPPI_E_HIP, "hipEventElapsedTime");
+    roll += x;
+    comb += y;
+  }
+  (void)hipStreamSynchronize(b->stream);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st != MPPI_OK) return st;
+  *rollout_ms = roll / reps;
+  *combine_ms = comb / reps;
   return MPPI_OK;
 }
 

@@ -274,6 +274,7 @@ struct mppi_controller {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool computed = false;            // computeControl has run once (the device loop starts from its x0 and mean)
+  bool loop_ready = false;          // loop_iter_d holds the device loop's arguments
   IterArgs* loop_iter_d = nullptr;  // free-running iteration arguments, one per iteration of a call (rng offsets
                                     // advanced on the device)
   hipGraph_t loop_graph = nullptr;
@@ -695,7 +696,8 @@ int mppi_controller_destroy(mppi_controller* ctrl)
 // drawn inside the rollout kernel; colored samples too where the kernel's workgroup shape allows, else by the colored
 // sampler into the sample buffer first).
 static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
-                             const ResultCopies* copies = nullptr, int advance_step = 1)
+                             const ResultCopies* copies = nullptr, int advance_step = 1,
+                             hipEvent_t between = nullptr, int* partials_out = nullptr)
 {
   const mppi_controller_desc& d = c->d;
   const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
@@ -706,7 +708,7 @@ static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
   return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->K, c->T, d.dt, d.lambda, d.alpha, c->x0_d,
                                 c->mean_d, c->samples_d, !colored, d.keep_samples != 0, ref, c->costs_d, c->ws_d,
                                 c->stats_d, c->mean_d, advance, c->stream, nullptr, nullptr, copies,
-                                colored ? &plan : nullptr, advance_step);
+                                colored ? &plan : nullptr, advance_step, between, partials_out);
 }
 
 // The work of one computeControl call, enqueued on the controller's stream (captured once when use_graph is set).
@@ -934,6 +936,20 @@ static int capture_loop(mppi_controller* c, int calls, hipGraph_t* graph, hipGra
   return MPPI_OK;
 }
 
+// Arguments of the free-running loop (once after creation or a reseed): x0 as last uploaded, the mean left by the
+// last call, stride 1, iteration it in slot it, rng offsets from a range computeControl does not reach.
+static int loop_args_init(mppi_controller* c)
+{
+  if (c->loop_ready) return MPPI_OK;
+  const int n = c->d.num_iters;
+  std::vector<IterArgs> a((size_t)n);
+  for (int it = 0; it < n; ++it) a[it] = IterArgs{c->rng_counter + ((uint64_t)1 << 40) + (uint64_t)it, 1, it};
+  MPPI_HIP_CHECK(hipMemcpyAsync(c->loop_iter_d, a.data(), sizeof(IterArgs) * n, hipMemcpyHostToDevice, c->stream));
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->loop_ready = true;
+  return MPPI_OK;
+}
+
 int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
 {
   if (!c) return fail(MPPI_E_INVALID, "null argument");
@@ -942,13 +958,7 @@ int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
     return fail(MPPI_E_INVALID, "step_device: call computeControl once first (the loop starts from its x0 and mean)");
   if (num_calls == 0) return MPPI_OK;
   if (!c->loop_exec) {
-    // arguments of the free-running loop: x0 as last uploaded, the mean left by the last call, stride 1, iteration
-    // it in slot it, rng offsets from a range computeControl does not reach
-    const int n = c->d.num_iters;
-    std::vector<IterArgs> a((size_t)n);
-    for (int it = 0; it < n; ++it) a[it] = IterArgs{c->rng_counter + ((uint64_t)1 << 40) + (uint64_t)it, 1, it};
-    MPPI_HIP_CHECK(hipMemcpyAsync(c->loop_iter_d, a.data(), sizeof(IterArgs) * n, hipMemcpyHostToDevice, c->stream));
-    MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+    MPPI_CHECK(loop_args_init(c));
     MPPI_CHECK(controller_enqueue_loop(c));  // eager pass (sets kernel attributes before capture)
     MPPI_CHECK(capture_loop(c, 1, &c->loop_graph, &c->loop_exec));
     MPPI_CHECK(capture_loop(c, LOOP_GROUP, &c->loop_group_graph, &c->loop_group_exec));
@@ -960,6 +970,45 @@ int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
   for (; c->loop_group_exec != nullptr && num_calls >= LOOP_GROUP; num_calls -= LOOP_GROUP)
     MPPI_HIP_CHECK(hipGraphLaunch(c->loop_group_exec, c->stream));
   for (int i = 0; i < num_calls; ++i) MPPI_HIP_CHECK(hipGraphLaunch(c->loop_exec, c->stream));
+  return MPPI_OK;
+}
+
+int mppi_controller_time_kernels(mppi_controller* c, int32_t reps, float* rollout_ms, float* combine_ms,
+                                 int32_t* num_partials)
+{
+  if (!c || !rollout_ms || !combine_ms || reps < 1) return fail(MPPI_E_INVALID, "bad argument");
+  if (!c->computed) return fail(MPPI_E_INVALID, "time_kernels: call computeControl once first");
+  MPPI_CHECK(loop_args_init(c));
+  std::vector<hipEvent_t> ev((size_t)(2 * reps + 1), nullptr);
+  int st = MPPI_OK;
+  for (auto& e : ev)
+    if (st == MPPI_OK && hipEventCreate(&e) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventCreate");
+  if (st == MPPI_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventRecord");
+  // iteration 0 of the device loop, launch after launch: rollout between ev[2r] and ev[2r+1], combine to ev[2r+2]
+  for (int r = 0; r < reps && st == MPPI_OK; ++r) {
+    int G = 0;
+    st = enqueue_iteration(c, IterRef{c->loop_iter_d, {0, 0, 0}}, c->loop_iter_d, nullptr, c->d.num_iters,
+                           ev[2 * r + 1], &G);
+    if (num_partials != nullptr) *num_partials = G;
+    if (st == MPPI_OK && hipEventRecord(ev[2 * r + 2], c->stream) != hipSuccess)
+      st = fail(MPPI_E_HIP, "hipEventRecord");
+  }
+  float roll = 0.0f, comb = 0.0f;
+  if (st == MPPI_OK && hipEventSynchronize(ev.back()) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventSynchronize");
+  for (int r = 0; r < reps && st == MPPI_OK; ++r) {
+    float a = 0.0f, b = 0.0f;
+    if (hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]) != hipSuccess ||
+        hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]) != hipSuccess)
+      st = fail(MPPI_E_HIP, "hipEventElapsedTime");
+    roll += a;
+    comb += b;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st != MPPI_OK) return st;
+  *rollout_ms = roll / reps;
+  *combine_ms = comb / reps;
   return MPPI_OK;
 }
 
@@ -1047,6 +1096,7 @@ int mppi_controller_set_seed(mppi_controller* c, uint64_t seed)
   drop_graphs(c);
   c->d.sampler.seed = seed;
   c->rng_counter = 0;
+  c->loop_ready = false;
   return MPPI_OK;
 }
 

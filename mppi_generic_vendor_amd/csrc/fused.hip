@@ -452,7 +452,8 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            const float* x0, const float* mean, float* samples, bool generate, bool write_samples,
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
                            IterArgs* advance, hipStream_t s, const float* prop_dmu, const float* prop_sig,
-                           const ResultCopies* copies, const ColoredPlan* colored, int advance_step)
+                           const ResultCopies* copies, const ColoredPlan* colored, int advance_step,
+                           hipEvent_t between, int* partials_out)
 {
   const int C = dyn.control_dim;
   const int model = (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
@@ -485,6 +486,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
   }
   const int G = (K + rows - 1) / rows;
   if (G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "fused iteration: too many samples");
+  if (partials_out != nullptr) *partials_out = G;
   Partial* parts = static_cast<Partial*>(ws);
   float* sums = reinterpret_cast<float*>(static_cast<char*>(ws) + (size_t)G * sizeof(Partial));
   int st = MPPI_E_UNSUPPORTED;
@@ -508,6 +510,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
   if (st == MPPI_E_UNSUPPORTED)
     return fail(MPPI_E_UNSUPPORTED, "fused iteration: model/cost combination is not compiled into the kernel library");
   if (st != MPPI_OK) return st;
+  if (between != nullptr) MPPI_HIP_CHECK(hipEventRecord(between, s));
   const int grid = (T * C + COMBINE_COLS - 1) / COMBINE_COLS;
   combine_kernel<<<grid, COMBINE_THREADS, 0, s>>>(parts, sums, G, K, T * C, lambda, stats, mean_out, advance,
                                                   advance_step, copies != nullptr ? *copies : ResultCopies{});

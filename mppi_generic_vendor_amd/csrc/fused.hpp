@@ -30,7 +30,8 @@ struct ResultCopies {
 // kernel (rng offset, stride and iteration from `it`); otherwise they are read from `samples` (e.g. written by the
 // colored sampler). `write_samples` stores the clamped samples back to `samples`; `costs_out` (optional) receives
 // the per-sample costs. `advance` (optional) gets its rng offset incremented by `advance_step` in the combine kernel;
-// `copies`
+// `between` (optional) is recorded on the stream between the rollout and the combine launches; `partials_out`
+// (optional) receives the number of per-workgroup partials the combine reduces; `copies`
 // (optional) names further destinations of the results. With `colored` (generate false) the samples are colored
 // noise of that plan: drawn inside the rollout kernel where its workgroup shape allows (the same values as the colored
 // sampler's), else by the colored sampler into `samples` before it.
@@ -40,6 +41,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            IterRef it, float* costs_out, void* ws, mppi_weight_stats* stats, float* mean_out,
                            IterArgs* advance, hipStream_t s, const float* prop_dmu = nullptr,
                            const float* prop_sig = nullptr, const ResultCopies* copies = nullptr,
-                           const ColoredPlan* colored = nullptr, int advance_step = 1);
+                           const ColoredPlan* colored = nullptr, int advance_step = 1,
+                           hipEvent_t between = nullptr, int* partials_out = nullptr);
 
 }  // namespace mppi_amd
