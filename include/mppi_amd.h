@@ -338,6 +338,55 @@ int mppi_controller_stream(const mppi_controller* ctrl, void** stream);
 
 
 /* ------------------------------------------------------------------------------------------------------------------
+ * User-written device plugins. The reference takes any CRTP Dynamics / Cost class with __device__ methods
+ * (dynamics/dynamics.cuh:366-432: computeStateDeriv, updateState, stateToOutput, enforceConstraints;
+ * cost_functions/cost.cu:39-53: computeRunningCost, terminalCost) and instantiates its rollout kernel for them. Device
+ * code cannot cross a C ABI, so a user's hipcc translation unit instantiates the engine's kernel templates for its
+ * own classes (include/mppi_amd/device_plugin.hpp) and hands the controller a table of callbacks: the controller
+ * keeps its sampling inputs, graphs, combine kernel, smoothing and getters, and calls the plugin to enqueue the
+ * rollout of each iteration.
+ * ---------------------------------------------------------------------------------------------------------------- */
+typedef struct mppi_plugin_iteration {
+  const float* x0_d;        /* [S] initial state */
+  const float* mean_d;      /* [T][C] mean the samples are drawn around */
+  float* samples_d;         /* [K][T][C]: input when generate == 0, output (clamped samples) when write_samples */
+  float* costs_d;           /* [K] costs out (may be NULL) */
+  const void* iter_d;       /* device iteration arguments {uint64_t rng_offset; int32_t stride; int32_t iteration} */
+  void* partials_d;         /* room for max_partials per-workgroup partials (32 bytes each) */
+  float* sums_d;            /* room for max_partials rows of T*C weighted control sums */
+  int32_t max_partials;
+  int32_t num_rollouts;
+  int32_t num_timesteps;
+  int32_t generate;         /* 1: draw the Gaussian samples in the kernel; 0: read them from samples_d */
+  int32_t write_samples;    /* 1: store the clamped samples to samples_d */
+  float dt;
+  float lambda;
+  float alpha;
+  mppi_sampler_params sampler;
+  void* stream;             /* hipStream_t */
+} mppi_plugin_iteration;
+
+typedef struct mppi_plugin {
+  int32_t state_dim;
+  int32_t control_dim;
+  int32_t output_dim;
+  int32_t _pad;
+  void* user; /* passed back to every callback; must outlive the controller */
+  /* Enqueue one iteration's rollout kernel on it->stream (no synchronisation; it may run inside a graph capture,
+   * except on the first call for a device); *num_partials receives the number of partials written (<= max_partials). */
+  int (*enqueue_iteration)(void* user, const mppi_plugin_iteration* it, int32_t* num_partials);
+  /* Dynamics::step on the host for the state trajectory: x_next from x and a constrained control u. */
+  int (*host_step)(void* user, const float* x, const float* u, float dt, float* x_next);
+  /* Dynamics::enforceConstraints on the host, in place on one control vector. */
+  int (*host_enforce_constraints)(void* user, float* u);
+} mppi_plugin;
+
+/* A controller whose rollouts run the plugin's kernels. desc->dynamics supplies only control_dim, the control limits
+ * and zero_control (slide); desc->cost is unused; desc->sampler must be Gaussian (drawn in the plugin's kernel) or
+ * colored (drawn by the library's sampler into the sample buffer first). Learned proposals are not supported. */
+int mppi_controller_create_plugin(const mppi_controller_desc* desc, const mppi_plugin* plugin, mppi_controller** out);
+
+/* ------------------------------------------------------------------------------------------------------------------
  * Batch of independent controllers advanced by one persistent launch per iteration.
  * No single counterpart in the reference: it replaces N VanillaMPPIController objects, each with its own stream
  * and its own sampling/rollout/weights/update launches per computeControl (controllers/MPPI/mppi_controller.cu:

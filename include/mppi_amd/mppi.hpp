@@ -21,6 +21,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../mppi_amd.h"
@@ -580,8 +581,19 @@ struct ControllerParams {
   bool use_graph_ = true;
 };
 
+// User-written device plugins (include/mppi_amd/device_plugin.hpp) mark themselves with DEVICE_PLUGIN; a controller
+// over them is created through PluginControllerFactory, which that header defines (it needs hipcc).
+template <class T, class = void>
+struct is_device_plugin : std::false_type {};
+template <class T>
+struct is_device_plugin<T, std::void_t<decltype(T::DEVICE_PLUGIN)>> : std::bool_constant<T::DEVICE_PLUGIN> {};
+template <class DYN, class COST>
+struct PluginControllerFactory;
+
+// (The default sampler is GaussianDistribution<DYN_T::DYN_PARAMS_T>, i.e. GaussianDistributionC of the model's control
+// dimension, which also covers plugin models.)
 template <class DYN_T, class COST_T, class FB_T, int MAX_TIMESTEPS, int NUM_ROLLOUTS,
-          class SAMPLING_T = sampling_distributions::GaussianDistribution<typename DYN_T::DYN_PARAMS_T>,
+          class SAMPLING_T = sampling_distributions::GaussianDistributionC<DYN_T::CONTROL_DIM>,
           class PARAMS_T = ControllerParams<DYN_T::STATE_DIM, DYN_T::CONTROL_DIM, MAX_TIMESTEPS>>
 class VanillaMPPIController {
  public:
@@ -625,7 +637,7 @@ class VanillaMPPIController {
     check(mppi_controller_set_control_seq(ctrl_, control_.data()), "updateImportanceSampler");
     check(mppi_controller_compute_control(ctrl_, state.data(), optimization_stride), "computeControl");
     check(mppi_controller_get_control_seq(ctrl_, control_.data()), "getControlSeq");
-    check(mppi_controller_get_state_seq(ctrl_, state_.data()), "getTargetStateSeq");
+    state_fresh_ = false;  // the state trajectory is propagated when getTargetStateSeq asks for it
     mppi_weight_stats st;
     check(mppi_controller_get_weight_stats(ctrl_, &st), "weights");
     baseline_ = st.baseline;
@@ -641,7 +653,16 @@ class VanillaMPPIController {
   }
 
   control_trajectory getControlSeq() const { return control_; }
-  state_trajectory getTargetStateSeq() const { return state_; }
+  // computeStateTrajectory of the last call's controls (controllers/MPPI/mppi_controller.cu:222), propagated on the
+  // host on the first request after each call
+  state_trajectory getTargetStateSeq() const
+  {
+    if (!state_fresh_ && ctrl_) {
+      check(mppi_controller_get_state_seq(ctrl_, state_.data()), "getTargetStateSeq");
+      state_fresh_ = true;
+    }
+    return state_;
+  }
   float getBaselineCost(int = 0) const { return baseline_; }
   float getNormalizerCost(int = 0) const { return normalizer_; }
   MPPIFreeEnergyStatistics getFreeEnergyStatistics() const { return fe_; }
@@ -686,8 +707,6 @@ class VanillaMPPIController {
     if (T < 1 || T > MAX_TIMESTEPS) throw std::runtime_error("num_timesteps out of range");
     mppi_controller_desc d;
     std::memset(&d, 0, sizeof(d));
-    d.dynamics = model_->desc();
-    d.cost = cost_->desc(*model_);
     d.sampler = sampler_->desc();
     d.sampler.seed = params_.seed_;
     d.num_rollouts = NUM_ROLLOUTS;
@@ -699,14 +718,22 @@ class VanillaMPPIController {
     d.use_graph = params_.use_graph_ ? 1 : 0;
     for (int c = 0; c < CONTROL_DIM; ++c) d.slide_control_scale[c] = params_.slide_control_scale_[c];
     d.init_control = control_.data();  // [T][C] row-major == the first T columns of the col-major C x MAX_T matrix
-    d.cost_map_host = cost_->costMapHost();
-    d.nn_theta_host = model_->hostParameters();
-    check(mppi_controller_create(&d, &ctrl_), "mppi_controller_create");
+    if constexpr (is_device_plugin<DYN_T>::value) {
+      ctrl_ = PluginControllerFactory<DYN_T, COST_T>::create(d, model_, cost_, plugin_keep_);
+    } else {
+      d.dynamics = model_->desc();
+      d.cost = cost_->desc(*model_);
+      d.cost_map_host = cost_->costMapHost();
+      d.nn_theta_host = model_->hostParameters();
+      check(mppi_controller_create(&d, &ctrl_), "mppi_controller_create");
+    }
   }
   void release()
   {
     if (ctrl_) mppi_controller_destroy(ctrl_);
     ctrl_ = nullptr;
+    plugin_keep_.reset();
+    state_fresh_ = false;
   }
 
   DYN_T* model_;
@@ -715,8 +742,10 @@ class VanillaMPPIController {
   SAMPLING_T* sampler_;
   PARAMS_T params_;
   mppi_controller* ctrl_ = nullptr;
+  std::shared_ptr<void> plugin_keep_;  // the plugin callback table's state (plugin models only)
   control_trajectory control_;
-  state_trajectory state_;
+  mutable state_trajectory state_;
+  mutable bool state_fresh_ = false;
   float baseline_ = 0.0f, normalizer_ = 0.0f;
   MPPIFreeEnergyStatistics fe_;
 };

@@ -141,6 +141,7 @@ _SIGS = {
     "mppi_weighted_update_noalloc": ([P, P, P, _i, _i, _i, P, P], ct.c_int),
     "mppi_controller_create": ([ct.POINTER(ControllerDesc), ct.POINTER(P)], ct.c_int),
     "mppi_controller_destroy": ([P], ct.c_int),
+    "mppi_controller_create_plugin": ([ct.POINTER(ControllerDesc), P, ct.POINTER(P)], ct.c_int),
     "mppi_controller_compute_control": ([P, P, _i], ct.c_int),
     "mppi_controller_step_device": ([P, _i], ct.c_int),
     "mppi_controller_get_control_seq": ([P, P], ct.c_int),

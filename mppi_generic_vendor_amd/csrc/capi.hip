@@ -238,6 +238,8 @@ using namespace mppi_amd;
 // ======================================================================================================= C-ABI
 struct mppi_controller {
   mppi_controller_desc d{};
+  bool has_plugin = false;  // rollouts run a user plugin's kernels (mppi_controller_create_plugin)
+  mppi_plugin plug{};
   int S = 0, C = 0, O = 0, K = 0, T = 0;
   hipStream_t stream = nullptr;
   float* x0_d = nullptr;
@@ -578,6 +580,25 @@ int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_
 }
 
 // --------------------------------------------------------------------------------------------------- controller
+// Host post-processing of a plugin controller: the plugin's own constraint and step (Dynamics::enforceConstraints and
+// Dynamics::step on the host) in place of the built-in models'.
+static void plugin_clamp_sequence(const mppi_controller* c, float* u)
+{
+  for (int t = 0; t < c->T; ++t) (void)c->plug.host_enforce_constraints(c->plug.user, u + (size_t)t * c->C);
+}
+
+static void plugin_propagate(const mppi_controller* c, const float* x0, const float* u, float* states)
+{
+  const int S = c->S, C = c->C;
+  for (int i = 0; i < S; ++i) states[i] = x0[i];
+  std::vector<float> uc((size_t)C);
+  for (int t = 0; t + 1 < c->T; ++t) {
+    for (int j = 0; j < C; ++j) uc[j] = u[(size_t)t * C + j];
+    (void)c->plug.host_enforce_constraints(c->plug.user, uc.data());
+    (void)c->plug.host_step(c->plug.user, states + (size_t)t * S, uc.data(), c->d.dt, states + (size_t)(t + 1) * S);
+  }
+}
+
 // Destroys every captured graph of the controller (the next use captures again); the caller has synchronised the
 // stream, so no replay of them is still queued.
 static void drop_graphs(mppi_controller* c)
@@ -618,13 +639,28 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   c->K = desc->num_rollouts;
   c->T = desc->num_timesteps;
   MPPI_CHECK(validate_sizes(c->K, c->T));
-  MPPI_CHECK(validate_dynamics(desc->dynamics));
-  MPPI_CHECK(validate_sampler(desc->sampler, desc->dynamics));
+  if (c->has_plugin) {
+    const mppi_plugin& p = c->plug;
+    if (p.state_dim < 1 || p.state_dim > MPPI_MAX_STATE || p.control_dim < 1 || p.control_dim > MPPI_MAX_CONTROL ||
+        p.output_dim < 1)
+      return fail(MPPI_E_INVALID, "plugin: dimensions out of range");
+    if (!p.enqueue_iteration || !p.host_step || !p.host_enforce_constraints)
+      return fail(MPPI_E_INVALID, "plugin: missing callback");
+    c->d.dynamics.model = 0;
+    c->d.dynamics.state_dim = p.state_dim;
+    c->d.dynamics.control_dim = p.control_dim;
+    c->d.dynamics.output_dim = p.output_dim;
+    c->d.dynamics.nn_theta = nullptr;
+    c->d.cost.map = nullptr;
+  } else {
+    MPPI_CHECK(validate_dynamics(desc->dynamics));
+  }
+  MPPI_CHECK(validate_sampler(desc->sampler, c->d.dynamics));
   if (!(desc->lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
   if (desc->num_iters < 1) return fail(MPPI_E_INVALID, "num_iters must be >= 1");
-  c->S = desc->dynamics.state_dim;
-  c->C = desc->dynamics.control_dim;
-  c->O = desc->dynamics.output_dim;
+  c->S = c->d.dynamics.state_dim;
+  c->C = c->d.dynamics.control_dim;
+  c->O = c->d.dynamics.output_dim;
   const int K = c->K, T = c->T, C = c->C, S = c->S;
   MPPI_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   MPPI_CHECK(dev_alloc(&c->x0_d, sizeof(float) * MPPI_MAX_STATE));
@@ -642,15 +678,15 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   MPPI_CHECK(host_alloc_mapped(&c->mean_out_h, &c->mean_out_hd, sizeof(float) * T * C));
   MPPI_CHECK(host_alloc_mapped(&c->stats_h, &c->stats_hd, sizeof(mppi_weight_stats)));
   // plugin-owned device copies of the cost map and the MLP parameters
-  if (desc->cost_map_host != nullptr) {
+  if (!c->has_plugin && desc->cost_map_host != nullptr) {
     const size_t n = (size_t)desc->cost.map_width * desc->cost.map_height;
     if (n < 4) return fail(MPPI_E_INVALID, "cost map: width/height must be >= 2");
     MPPI_CHECK(dev_alloc(&c->map_d, sizeof(float) * n));
     MPPI_HIP_CHECK(hipMemcpy(c->map_d, desc->cost_map_host, sizeof(float) * n, hipMemcpyHostToDevice));
     c->d.cost.map = c->map_d;
   }
-  MPPI_CHECK(validate_cost(c->d.cost, c->d.dynamics, true));
-  if (desc->dynamics.model == MPPI_MODEL_RACER_DUBINS_NN) {
+  if (!c->has_plugin) MPPI_CHECK(validate_cost(c->d.cost, c->d.dynamics, true));
+  if (!c->has_plugin && desc->dynamics.model == MPPI_MODEL_RACER_DUBINS_NN) {
     if (desc->nn_theta_host == nullptr) return fail(MPPI_E_INVALID, "residual MLP: nn_theta_host is required");
     c->nn_host.assign(desc->nn_theta_host, desc->nn_theta_host + desc->dynamics.nn_num_params);
     MPPI_CHECK(dev_alloc(&c->nn_d, sizeof(float) * c->nn_host.size()));
@@ -669,12 +705,16 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   return MPPI_OK;
 }
 
-int mppi_controller_create(const mppi_controller_desc* desc, mppi_controller** out)
+static int controller_create(const mppi_controller_desc* desc, const mppi_plugin* plugin, mppi_controller** out)
 {
   if (!desc || !out) return fail(MPPI_E_INVALID, "null argument");
   *out = nullptr;
   mppi_controller* c = new (std::nothrow) mppi_controller();
   if (!c) return fail(MPPI_E_NOMEM, "host allocation failed");
+  if (plugin != nullptr) {
+    c->has_plugin = true;
+    c->plug = *plugin;
+  }
   const int st = controller_setup(c, desc);
   if (st != MPPI_OK) {
     const std::string msg = g_last_error;
@@ -686,10 +726,61 @@ int mppi_controller_create(const mppi_controller_desc* desc, mppi_controller** o
   return MPPI_OK;
 }
 
+int mppi_controller_create(const mppi_controller_desc* desc, mppi_controller** out)
+{
+  return controller_create(desc, nullptr, out);
+}
+
+int mppi_controller_create_plugin(const mppi_controller_desc* desc, const mppi_plugin* plugin, mppi_controller** out)
+{
+  if (!plugin) return fail(MPPI_E_INVALID, "null plugin");
+  return controller_create(desc, plugin, out);
+}
+
 int mppi_controller_destroy(mppi_controller* ctrl)
 {
   if (ctrl && ctrl->stream) (void)hipStreamSynchronize(ctrl->stream);
   return controller_free(ctrl);
+}
+
+// One iteration of a plugin controller: colored samples (if any) drawn by the library's sampler into the sample
+// buffer, the plugin's fused rollout kernel writing per-workgroup partials, then the library's combine. The partial
+// rows live after room for ceil(K / 16) partials in the workspace (the plugin picks its own workgroup size).
+static int enqueue_plugin_iteration(mppi_controller* c, IterRef ref, IterArgs* advance, const ResultCopies* copies,
+                                    int advance_step, hipEvent_t between, int* partials_out)
+{
+  const mppi_controller_desc& d = c->d;
+  const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
+  if (ref.ptr == nullptr) return fail(MPPI_E_INVALID, "plugin iteration: device iteration arguments required");
+  if (colored) MPPI_CHECK(launch_sample_colored(d.sampler, c->K, c->T, c->C, c->mean_d, c->samples_d, ref, nullptr,
+                                                c->stream));
+  const int gmax = (c->K + 15) / 16;
+  mppi_plugin_iteration it{};
+  it.x0_d = c->x0_d;
+  it.mean_d = c->mean_d;
+  it.samples_d = c->samples_d;
+  it.costs_d = c->costs_d;
+  it.iter_d = ref.ptr;
+  it.partials_d = c->ws_d;
+  it.sums_d = reinterpret_cast<float*>(static_cast<char*>(c->ws_d) + (size_t)gmax * sizeof(Partial));
+  it.max_partials = gmax;
+  it.num_rollouts = c->K;
+  it.num_timesteps = c->T;
+  it.generate = colored ? 0 : 1;
+  it.write_samples = d.keep_samples != 0 ? 1 : 0;
+  it.dt = d.dt;
+  it.lambda = d.lambda;
+  it.alpha = d.alpha;
+  it.sampler = d.sampler;
+  it.stream = (void*)c->stream;
+  int32_t G = 0;
+  const int st = c->plug.enqueue_iteration(c->plug.user, &it, &G);
+  if (st != MPPI_OK) return fail(st, "plugin enqueue_iteration failed with status " + std::to_string(st));
+  if (G < 1 || G > gmax) return fail(MPPI_E_INVALID, "plugin enqueue_iteration: partial count out of range");
+  if (partials_out != nullptr) *partials_out = G;
+  if (between != nullptr) MPPI_HIP_CHECK(hipEventRecord(between, c->stream));
+  return launch_combine(static_cast<const Partial*>(c->ws_d), it.sums_d, G, c->K, c->T * c->C, d.lambda, c->stats_d,
+                        c->mean_d, advance, advance_step, copies, c->stream);
 }
 
 // One MPPI iteration on the controller's stream: the fused sampling + rollout + weights + update (Gaussian samples
@@ -701,6 +792,7 @@ static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
 {
   const mppi_controller_desc& d = c->d;
   const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
+  if (c->has_plugin) return enqueue_plugin_iteration(c, ref, advance, copies, advance_step, between, partials_out);
   ColoredPlan plan;
   if (colored) MPPI_CHECK(colored_plan(d.sampler, c->K, c->T, c->C, c->stream, &plan));
   // the combine kernel writes the new mean over the sampling mean, as updateDistributionParamsFromDevice writes
@@ -843,7 +935,8 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
   c->state_x0.assign(c->x0_h, c->x0_h + S);
   c->state_u = c->control;
   c->state_stale = true;
-  clamp_controls_seq(d.dynamics, c->control.data(), T);
+  if (c->has_plugin) plugin_clamp_sequence(c, c->control.data());
+  else clamp_controls_seq(d.dynamics, c->control.data(), T);
   c->computed = true;
   return MPPI_OK;
 }
@@ -862,6 +955,7 @@ int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, in
     c->prop = nullptr;
     return MPPI_OK;
   }
+  if (c->has_plugin) return fail(MPPI_E_UNSUPPORTED, "proposal: not available with a dynamics plugin");
   const PropDims pd = proposal_dims(proposal_net_of(proposal));
   if (pd.S != c->S || pd.C != c->C || pd.T != c->T)
     return fail(MPPI_E_INVALID, "proposal: state/control/horizon dims do not match the controller");
@@ -1030,9 +1124,13 @@ int mppi_controller_get_state_seq(const mppi_controller* c, float* out)
 {
   if (!c || !out) return fail(MPPI_E_INVALID, "null argument");
   if (c->state_stale) {
-    mppi_dynamics_params hostdyn = c->d.dynamics;
-    if (!c->nn_host.empty()) hostdyn.nn_theta = c->nn_host.data();
-    propagate(hostdyn, c->state_x0.data(), c->state_u.data(), c->T, c->d.dt, c->state_seq.data());
+    if (c->has_plugin) {
+      plugin_propagate(c, c->state_x0.data(), c->state_u.data(), c->state_seq.data());
+    } else {
+      mppi_dynamics_params hostdyn = c->d.dynamics;
+      if (!c->nn_host.empty()) hostdyn.nn_theta = c->nn_host.data();
+      propagate(hostdyn, c->state_x0.data(), c->state_u.data(), c->T, c->d.dt, c->state_seq.data());
+    }
     c->state_stale = false;
   }
   std::copy(c->state_seq.begin(), c->state_seq.end(), out);

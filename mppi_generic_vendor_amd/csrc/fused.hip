@@ -16,13 +16,14 @@
 
 #include "colored_core.hpp"
 #include "fused.hpp"
+#include "iteration_core.hpp"
 #include "models.hpp"
 #include "rollout_core.hpp"
 #include "tile.hpp"
 
 namespace mppi_amd {
 
-__device__ __forceinline__ IterArgs load_iter_f(const IterRef& r) { return r.ptr ? *r.ptr : r.val; }
+__device__ __forceinline__ IterArgs load_iter_f(const IterRef& r) { return load_iter_args(r); }
 
 // Samples per workgroup and threads per workgroup of the fused kernel; SPLIT selects the small-grid BF16-network
 // rollout (rollout_tile_nn_split).
@@ -112,41 +113,8 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
       if (tid + i * NT < TC) mu[tid + i * NT] = mv[i];
     if (tid < M::S) x0_l[tid] = xv;
     __syncthreads();
-    const IterArgs a = load_iter_f(itr);
-    const float decay = powf(sp.std_dev_decay, (float)a.iteration);
-    const int64_t base = ((int64_t)k0 * TC) >> 2;
-    const int n = nrows * TC;
-    for (int e0 = tid * 4; e0 < n; e0 += NT * 4) {
-      const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(base + (e0 >> 2)));
-      int r = e0 / TC;
-      int col = e0 - r * TC;
-      int t = col / C;
-      int c = col - t * C;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (e0 + j < n) {
-          const int k = k0 + r;
-          const float m = mu[col];
-          float v;
-          if (k == 0 || t < a.stride) v = m;
-          else if (PROP) v = m + pl[col] + pl[TC4 + col] * z.z[j];
-          else if (k >= pstart) v = decay * sp.std_dev[c] * z.z[j];
-          else v = m + decay * sp.std_dev[c] * z.z[j];
-          tile[r * ROW + col] = constrain_control(dyn, c, v, deadband);
-        }
-        ++col;
-        if (++c == C) {
-          c = 0;
-          ++t;
-        }
-        if (col == TC) {
-          col = 0;
-          t = 0;
-          c = 0;
-          ++r;
-        }
-      }
-    }
+    gaussian_tile_fill<C, NT, PROP>(sp, load_iter_f(itr), k0, nrows, TC, ROW, pstart, mu, pl, TC4, tile,
+                                    [&](int col, float v) { return constrain_control(dyn, col % C, v, deadband); });
   } else if constexpr (CNTT > 0) {
     // colored samples of this workgroup's span, generated in the scratch area (the rollout reuses it afterwards),
     // constrained into the tile; the mean comes with them
@@ -186,45 +154,10 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
   else
     rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0_l, tile, ROW, mu, pstart, k0, scratch, chunk,
                               cost_l, raw_l, PROP ? &pv : nullptr, rows);
-  if (tid < RC_LANES) {
-    const int k = k0 + tid;
-    const float my_cost = cost_l[lane];
-    if (lane < nrows && costs_out != nullptr) costs_out[k] = my_cost;
-    // with a proposal, sample 0 (the warm start) is only a probe for the fallback test: it is not a draw from the
-    // proposal and takes no part in the weights
-    const bool valid = lane < nrows && !(PROP && k == 0);
-    // local weights against the workgroup's own baseline
-    const float c = valid ? sanitize_cost(my_cost) : INFINITY;
-    float b = c;
-    int bi = valid ? k : INT_MAX;
-    wave_argmin(b, bi);
-    const bool fin = valid && c <= FLT_MAX;
-    const float w = fin ? expf(-(1.0f / lambda) * (c - b)) : 0.0f;
-    wl[lane] = w;
-    const double e = wave_sum((double)w);
-    const double e2 = wave_sum((double)w * (double)w);
-    // weighted mean of the uncorrected costs (the proposal's achieved cost)
-    const float raw = raw_l[lane];
-    const double ec = wave_sum((fin && raw <= FLT_MAX) ? (double)w * (double)raw : 0.0);
-    if (lane == 0) {
-      Partial p;
-      p.baseline = b;
-      p.argmin = bi;
-      p.eta = e;
-      p.s2 = e2;
-      p.wc = ec;
-      parts[g] = p;
-    }
-  }
-  __syncthreads();
-  if (WRITE) tile_store<NT>(samples + (size_t)k0 * TC, tile, nrows, TC, ROW, tid);
-  for (int j = tid; j < TC; j += NT) {
-    float acc = 0.0f;
-    // in row order; unrolled so that the LDS reads of 8 rows are in flight together
-#pragma unroll 8
-    for (int r = 0; r < nrows; ++r) acc += wl[r] * tile[r * ROW + j];
-    sums[(size_t)g * TC + j] = acc;
-  }
+  // with a proposal, sample 0 (the warm start) is only a probe for the fallback test: it is not a draw from the
+  // proposal and takes no part in the weights
+  tile_partials<NT, WRITE, PROP>(cost_l, raw_l, wl, tile, ROW, TC, k0, nrows, g, lambda, costs_out, parts, sums,
+                                 samples);
 }
 
 // Combination of G partials: workgroup q owns COMBINE_COLS columns; every workgroup recomputes the global baseline
@@ -234,7 +167,6 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
 constexpr int COMBINE_COLS = 8;
 constexpr int COMBINE_THREADS = 256;
 constexpr int COMBINE_PREFETCH = 8;
-constexpr int MAX_PARTIALS = 4096;
 
 __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial* __restrict__ parts,
                                                                   const float* __restrict__ sums, int G, int K, int TC,
@@ -481,8 +413,7 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
   } else if (split) {
     rows = NS_ROWS;
   } else if (model != MODEL_RACER_DUBINS_NN_BF16) {
-    const int cus = device_cu_count();
-    while (rows > NS_ROWS && (K + rows / 2 - 1) / (rows / 2) <= cus) rows /= 2;
+    rows = fused_rows_for(K, device_cu_count());
   }
   const int G = (K + rows - 1) / rows;
   if (G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "fused iteration: too many samples");
@@ -511,8 +442,16 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
     return fail(MPPI_E_UNSUPPORTED, "fused iteration: model/cost combination is not compiled into the kernel library");
   if (st != MPPI_OK) return st;
   if (between != nullptr) MPPI_HIP_CHECK(hipEventRecord(between, s));
-  const int grid = (T * C + COMBINE_COLS - 1) / COMBINE_COLS;
-  combine_kernel<<<grid, COMBINE_THREADS, 0, s>>>(parts, sums, G, K, T * C, lambda, stats, mean_out, advance,
+  return launch_combine(parts, sums, G, K, T * C, lambda, stats, mean_out, advance, advance_step, copies, s);
+}
+
+int launch_combine(const Partial* parts, const float* sums, int G, int K, int TC, float lambda,
+                   mppi_weight_stats* stats, float* mean_out, IterArgs* advance, int advance_step,
+                   const ResultCopies* copies, hipStream_t s)
+{
+  if (G < 1 || G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "combine: partial count out of range");
+  const int grid = (TC + COMBINE_COLS - 1) / COMBINE_COLS;
+  combine_kernel<<<grid, COMBINE_THREADS, 0, s>>>(parts, sums, G, K, TC, lambda, stats, mean_out, advance,
                                                   advance_step, copies != nullptr ? *copies : ResultCopies{});
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;

@@ -44,4 +44,12 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
                            const ColoredPlan* colored = nullptr, int advance_step = 1,
                            hipEvent_t between = nullptr, int* partials_out = nullptr);
 
+// The combine launch alone, over G per-workgroup partials parts[G] and weighted sums sums[G][TC] produced by any
+// rollout kernel (the built-in fused kernel or a user plugin's): global first-strict baseline, FP64 normalizer, new
+// mean sums / eta into mean_out, statistics, and the optional advance / copies as above. G <= MAX_PARTIALS.
+constexpr int MAX_PARTIALS = 4096;
+int launch_combine(const Partial* parts, const float* sums, int G, int K, int TC, float lambda,
+                   mppi_weight_stats* stats, float* mean_out, IterArgs* advance, int advance_step,
+                   const ResultCopies* copies, hipStream_t s);
+
 }  // namespace mppi_amd

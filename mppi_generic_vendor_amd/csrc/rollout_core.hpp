@@ -95,25 +95,47 @@ __device__ __forceinline__ float is_term(const PropLds& p, const float* u, const
   return s;
 }
 
+// The generic pipeline below is written against two traits types:
+//   M  -- S, C, O; deriv(dp, x, u, x_dot); update(dp, x, x_next, x_dot, dt); output(dp, x_next, y) (the reference's
+//         computeStateDeriv / updateState / stateToOutput, dynamics/dynamics.cuh:369-432, over a parameter block dp);
+//   CF -- running(cp, y, u, t) and terminal(cp, y) (computeRunningCost / terminalCost, cost_functions/cost.cu:39-53).
+// BuiltinModel / BuiltinCost adapt the engine's own plugins (models.hpp); user plugins are adapted in
+// plugin_kernels.hpp.
+template <int MODEL>
+struct BuiltinModel : Model<MODEL> {
+  static_assert(Model<MODEL>::O == Model<MODEL>::S, "the built-in models output their state");
+  MPPI_HD static void output(const mppi_dynamics_params&, const float* x, float* y)
+  {
+#pragma unroll
+    for (int i = 0; i < Model<MODEL>::O; ++i) y[i] = x[i];
+  }
+};
+
+template <int COST, int O>
+struct BuiltinCost {
+  MPPI_HD static float running(const mppi_cost_params& c, const float* y, const float*, int)
+  {
+    return Cost<COST, O>::running(c, y);
+  }
+  MPPI_HD static float terminal(const mppi_cost_params& c, const float* y) { return Cost<COST, O>::terminal(c, y); }
+};
+
 // The producer's steps t0 .. t1 - 1 of one chunk: un holds the (already constrained) controls of the next PG steps,
 // read one group ahead so that the LDS latency passes while the current group is stepped; each step advances the
 // dynamics and stores the output to the chunk buffer yb (lane's row of step t0 at yb). Full groups run without
 // per-step guards; a partial group only occurs at the end of the horizon.
-template <int MODEL, int PG>
-__device__ __forceinline__ void produce_chunk(const mppi_dynamics_params& dyn, float dt, const float* row, float* yb,
-                                              int t0, int t1, float* x, float* y, float (&un)[PG][Model<MODEL>::C])
+template <class M, int PG, class DP>
+__device__ __forceinline__ void produce_chunk(const DP& dyn, float dt, const float* row, float* yb, int t0, int t1,
+                                              float* x, float* y, float (&un)[PG][M::C])
 {
-  using M = Model<MODEL>;
   constexpr int S = M::S, C = M::C, O = M::O;
   auto step = [&](const float* u, int t) {
     float xd[S], xn[S];
     M::deriv(dyn, x, u, xd);
     M::update(dyn, x, xn, xd, dt);
+    M::output(dyn, xn, y);
 #pragma unroll
-    for (int i = 0; i < O; ++i) {
-      y[i] = xn[i];
-      yb[(t - t0) * RC_LANES * (O | 1) + i] = xn[i];
-    }
+    for (int i = 0; i < O; ++i) yb[(t - t0) * RC_LANES * (O | 1) + i] = y[i];
 #pragma unroll
     for (int i = 0; i < S; ++i) x[i] = xn[i];
   };
@@ -379,25 +401,18 @@ __device__ __forceinline__ void rollout_tile_nn_split(const mppi_dynamics_params
   raw_l_out = raw_l;
 }
 
-// Runs the pipelined rollout of the tile; on return (after a barrier) cost_l[r] holds the cost of sample k0 + r for
-// every r < 64 (garbage for r >= nrows) and raw_l[r] the same cost without the importance-weight correction (equal
-// to it without a proposal). `scratch` provides rollout_scratch_floats<O>(chunk) floats. A tile of fewer than 64
-// rows (`rows`) has its last row repeated for the lanes beyond it.
-template <int MODEL, int COST>
-__device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
-                                             const mppi_sampler_params& sp, int T, float dt, float lambda,
-                                             float alpha, const float* __restrict__ x0, float* tile, int ROW,
-                                             const float* mu, int pstart, int k0, float* scratch, int chunk,
-                                             float*& cost_l_out, float*& raw_l_out, const PropLds* prop = nullptr,
-                                             int rows = RC_LANES)
+// Runs the pipelined rollout of the tile for the traits types M and CF (above); on return (after a barrier)
+// cost_l[r] holds the cost of sample k0 + r for every r < 64 (garbage for r >= nrows) and raw_l[r] the same cost
+// without the importance-weight correction (equal to it without a proposal). `scratch` provides
+// rollout_scratch_floats<O>(chunk) floats. A tile of fewer than 64 rows (`rows`) has its last row repeated for the
+// lanes beyond it.
+template <class M, class CF, class DP, class CP>
+__device__ __forceinline__ void rollout_tile_t(const DP& dyn, const CP& cost, const mppi_sampler_params& sp, int T,
+                                               float dt, float lambda, float alpha, const float* __restrict__ x0,
+                                               float* tile, int ROW, const float* mu, int pstart, int k0,
+                                               float* scratch, int chunk, float*& cost_l_out, float*& raw_l_out,
+                                               const PropLds* prop = nullptr, int rows = RC_LANES)
 {
-  if constexpr (MODEL == MODEL_RACER_DUBINS_NN_BF16) {
-    rollout_tile_nn<COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
-                          cost_l_out, raw_l_out, prop);
-    return;
-  }
-  using M = Model<MODEL>;
-  using CF = Cost<COST, M::O>;
   constexpr int S = M::S, C = M::C, O = M::O;
   float* ybuf = scratch;                                  // [2][chunk][64][O]
   float* part = ybuf + 2 * chunk * RC_LANES * (O | 1);          // [15][64]
@@ -417,8 +432,7 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
   if (wave == 0) {
 #pragma unroll
     for (int i = 0; i < S; ++i) x[i] = x0[i];
-#pragma unroll
-    for (int i = 0; i < O; ++i) y[i] = x[i];
+    M::output(dyn, x, y);
 #pragma unroll
     for (int j = 0; j < PG; ++j)
 #pragma unroll
@@ -433,14 +447,14 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
     if (wave == 0 && it < nch) {
       const int t0 = it * chunk, t1 = min(T, t0 + chunk);
       float* yb = ybuf + (it & 1) * chunk * RC_LANES * (O | 1) + lane * (O | 1);
-      produce_chunk<MODEL, PG>(dyn, dt, row, yb, t0, t1, x, y, un);
+      produce_chunk<M, PG>(dyn, dt, row, yb, t0, t1, x, y, un);
       if (it == nch - 1) term_l[lane] = CF::terminal(cost, y);
     } else if (wave > 0 && it > 0) {
       const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
       const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * (O | 1) + lane * (O | 1);
       for (int t = t0 + wave - 1; t < t1; t += RC_CONSUMERS) {
         // y and u are read in place from LDS (runtime-indexed cost terms become LDS addresses)
-        acc += CF::running(cost, yb + (t - t0) * RC_LANES * (O | 1)) +
+        acc += CF::running(cost, yb + (t - t0) * RC_LANES * (O | 1), crow + t * C, t) +
                lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
         if (prop) acc_is += is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
       }
@@ -467,6 +481,25 @@ __device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, co
   __syncthreads();
   cost_l_out = cost_l;
   raw_l_out = raw_l;
+}
+
+// The pipelined rollout of a built-in (model, cost) pair; the BF16-network model has its own pipeline.
+template <int MODEL, int COST>
+__device__ __forceinline__ void rollout_tile(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                                             const mppi_sampler_params& sp, int T, float dt, float lambda,
+                                             float alpha, const float* __restrict__ x0, float* tile, int ROW,
+                                             const float* mu, int pstart, int k0, float* scratch, int chunk,
+                                             float*& cost_l_out, float*& raw_l_out, const PropLds* prop = nullptr,
+                                             int rows = RC_LANES)
+{
+  if constexpr (MODEL == MODEL_RACER_DUBINS_NN_BF16) {
+    rollout_tile_nn<COST>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0, scratch, chunk,
+                          cost_l_out, raw_l_out, prop);
+  } else {
+    using M = BuiltinModel<MODEL>;
+    rollout_tile_t<M, BuiltinCost<COST, M::O>>(dyn, cost, sp, T, dt, lambda, alpha, x0, tile, ROW, mu, pstart, k0,
+                                               scratch, chunk, cost_l_out, raw_l_out, prop, rows);
+  }
 }
 
 }  // namespace mppi_amd

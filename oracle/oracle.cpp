@@ -32,6 +32,7 @@
  *   - Box-Muller              Box & Muller 1958
  */
 #include "../include/mppi_amd.h"
+#include "rollout_loop.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -97,24 +98,6 @@ static inline R normalize_angle(R a)
   return res - pi;
 }
 
-template <class R>
-static inline R sgn(R v)
-{
-  return v >= (R)0 ? (R)1 : (R)-1;
-}
-
-template <class R>
-static void enforce_constraints(const mppi_dynamics_params& p, R* u)
-{
-  for (int i = 0; i < p.control_dim; ++i) {
-    if (std::fabs(u[i]) < (R)p.control_deadband[i]) {
-      u[i] = (R)p.zero_control[i];
-    } else {
-      u[i] += (R)p.control_deadband[i] * -sgn(u[i]);
-    }
-    u[i] = std::min(std::max((R)p.control_min[i], u[i]), (R)p.control_max[i]);
-  }
-}
 
 /* Round to the nearest BF16 (ties to even), the conversion the matrix-core path applies to weights and layer
  * inputs. */
@@ -287,51 +270,19 @@ static R terminal_cost(const mppi_cost_params& c, const R* y)
   return quad_state<R>(c, y) * (R)c.terminal_cost_coeff;
 }
 
-static int pure_noise_start(int K, float pct)
-{
-  const float thr = (1.0f - pct) * (float)K;
-  int k = 0;
-  while (k < K && !((float)k >= thr)) ++k;
-  return k;
-}
-
-/* Likelihood-ratio control cost, gaussian.cu:650-675, mean = the mean the samples were drawn around. */
+/* The built-in models and costs as the rollout loop's system (rollout_loop.hpp). */
 template <class R>
-static R lr_cost(const mppi_sampler_params& s, int C, const R* u, const float* mean_t, bool pure, R lambda, R alpha)
-{
-  R cost = 0;
-  for (int i = 0; i < C; ++i) {
-    const R m = pure ? (R)0 : (R)mean_t[i];
-    const R sd = s.std_dev[i];
-    cost += (R)s.control_cost_coeff[i] * m * (m - (R)2 * u[i]) / (sd * sd);
-  }
-  return (R)0.5 * lambda * ((R)1 - alpha) * cost;
-}
-
-template <class R>
-static void rollout(const mppi_dynamics_params& dyn, const mppi_cost_params& cst, const float* map,
-                    const mppi_sampler_params& smp, int K, int T, R dt, R lambda, R alpha, const float* x0,
-                    const float* mean, float* samples, R* costs, int k_begin, int k_end)
-{
-  const int S = dyn.state_dim, C = dyn.control_dim;
-  const int pstart = pure_noise_start(K, smp.pure_noise_trajectories_percentage);
-  for (int k = k_begin; k < k_end; ++k) {
-    R x[MPPI_MAX_STATE] = {0}, xn[MPPI_MAX_STATE] = {0}, xd[MPPI_MAX_STATE] = {0}, y[MPPI_MAX_OUTPUT] = {0};
-    R u[MPPI_MAX_CONTROL];
-    for (int i = 0; i < S; ++i) x[i] = x0[i];
-    R run = 0;
-    for (int t = 0; t < T; ++t) {
-      float* ut = samples + ((size_t)k * T + t) * C;
-      for (int i = 0; i < C; ++i) u[i] = ut[i];
-      enforce_constraints<R>(dyn, u);
-      for (int i = 0; i < C; ++i) ut[i] = (float)u[i];
-      step<R>(dyn, x, xn, xd, u, y, dt);
-      run += running_cost<R>(cst, map, y) + lr_cost<R>(smp, C, u, mean + (size_t)t * C, k >= pstart, lambda, alpha);
-      for (int i = 0; i < S; ++i) x[i] = xn[i];
-    }
-    costs[k] = run / (R)T + terminal_cost<R>(cst, y) / (R)T;
-  }
-}
+struct BuiltinSystem {
+  const mppi_dynamics_params& dyn;
+  const mppi_cost_params& cst;
+  const float* map;
+  int state_dim() const { return dyn.state_dim; }
+  int control_dim() const { return dyn.control_dim; }
+  void enforce(R* u) const { enforce_constraints<R>(dyn, u); }
+  void step(const R* x, R* xn, R* xd, const R* u, R* y, R dt) const { oracle::step<R>(dyn, x, xn, xd, u, y, dt); }
+  R running(const R* y, const R*, int) const { return running_cost<R>(cst, map, y); }
+  R terminal(const R* y) const { return terminal_cost<R>(cst, y); }
+};
 
 } /* namespace oracle */
 
@@ -505,18 +456,8 @@ void oracle_rollout_f32(const mppi_dynamics_params* dyn, const mppi_cost_params*
                         const mppi_sampler_params* smp, int K, int T, float dt, float lambda, float alpha,
                         const float* x0, const float* mean, float* samples, float* costs, int nthreads)
 {
-  if (nthreads <= 1) {
-    rollout<float>(*dyn, *cst, map, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, 0, K);
-    return;
-  }
-  std::vector<std::thread> th;
-  const int chunk = (K + nthreads - 1) / nthreads;
-  for (int w = 0; w < nthreads; ++w) {
-    const int b = w * chunk, e = std::min(K, b + chunk);
-    if (b >= e) break;
-    th.emplace_back([=]() { rollout<float>(*dyn, *cst, map, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, b, e); });
-  }
-  for (auto& t : th) t.join();
+  const BuiltinSystem<float> sys{*dyn, *cst, map};
+  rollout_loop_threads<float>(sys, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, nthreads);
 }
 
 void oracle_rollout_f64(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,
@@ -524,18 +465,8 @@ void oracle_rollout_f64(const mppi_dynamics_params* dyn, const mppi_cost_params*
                         const float* x0, const float* mean, float* samples, double* costs, int nthreads)
 {
   /* FP64 run of the same loop; the clamped controls written back are rounded to float. */
-  if (nthreads <= 1) {
-    rollout<double>(*dyn, *cst, map, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, 0, K);
-    return;
-  }
-  std::vector<std::thread> th;
-  const int chunk = (K + nthreads - 1) / nthreads;
-  for (int w = 0; w < nthreads; ++w) {
-    const int b = w * chunk, e = std::min(K, b + chunk);
-    if (b >= e) break;
-    th.emplace_back([=]() { rollout<double>(*dyn, *cst, map, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, b, e); });
-  }
-  for (auto& t : th) t.join();
+  const BuiltinSystem<double> sys{*dyn, *cst, map};
+  rollout_loop_threads<double>(sys, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, nthreads);
 }
 
 /* --------------------------------------------------------------------------------------------- weights/update */
