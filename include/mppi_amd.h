@@ -331,6 +331,14 @@ int mppi_controller_slide_control_seq(mppi_controller* ctrl, int32_t steps);
  * the last iteration, and the [T][C] mean (after computeControl: the updated mean, which the next iteration samples
  * around, as the reference's control_means_d_). */
 int mppi_controller_device_buffers(const mppi_controller* ctrl, float** samples_d, float** costs_d, float** mean_d);
+/* Parameter hot-swap (the reference's setParams / paramsToDevice on the plugins and the controller,
+ * controllers/controller.cuh:842-876, core/base_plant.hpp:420-456): replaces the dynamics, cost and sampler parameter
+ * blocks, dt, lambda, alpha, slide_control_scale and keep_samples of a controller in place. cost_map_host (optional)
+ * uploads a new cost map into the controller's copy (reallocated when its size changes); nn_theta_host (optional)
+ * new network parameters of the same shape. The number of samples, horizon, iteration count, model, cost and sampler
+ * kind must stay the same (MPPI_E_INVALID otherwise: create a new controller). Synchronises the controller's stream
+ * and drops its captured graphs (the next calls capture them again); a new seed restarts the generator counter. */
+int mppi_controller_set_params(mppi_controller* ctrl, const mppi_controller_desc* desc);
 /* Reseed the generator and reset its counter (controllers/controller.cu:203-209). */
 int mppi_controller_set_seed(mppi_controller* ctrl, uint64_t seed);
 /* The stream all controller work runs on (hipStream_t as void*). */

@@ -352,6 +352,7 @@ class CartpoleQuadraticCost {
     return c;
   }
   const float* costMapHost() const { return nullptr; }
+  unsigned long long costMapVersion() const { return 0; }
 
  private:
   CartpoleQuadraticCostParams params_;
@@ -394,7 +395,10 @@ class QuadraticMapCost {
     map_ = map;
     width_ = width;
     height_ = height;
+    ++map_version_;
   }
+  // changes with every setCostMap: the controller uploads the new map on its next computeControl
+  unsigned long long costMapVersion() const { return map_version_; }
   void GPUSetup() {}
   void freeCudaMem() {}
   void bindToStream(void*) {}
@@ -428,6 +432,7 @@ class QuadraticMapCost {
   COST_PARAMS_T params_;
   std::vector<float> map_;
   int width_ = 0, height_ = 0;
+  unsigned long long map_version_ = 0;
 };
 
 // ==================================================================================================== samplers
@@ -597,11 +602,20 @@ template <class DYN_T, class COST_T, class FB_T, int MAX_TIMESTEPS, int NUM_ROLL
           class PARAMS_T = ControllerParams<DYN_T::STATE_DIM, DYN_T::CONTROL_DIM, MAX_TIMESTEPS>>
 class VanillaMPPIController {
  public:
-  static const int STATE_DIM = DYN_T::STATE_DIM, CONTROL_DIM = DYN_T::CONTROL_DIM;
+  static const int STATE_DIM = DYN_T::STATE_DIM, CONTROL_DIM = DYN_T::CONTROL_DIM, OUTPUT_DIM = DYN_T::OUTPUT_DIM;
   using state_array = typename DYN_T::state_array;
   using control_array = typename DYN_T::control_array;
+  using output_array = Mat<float, OUTPUT_DIM, 1>;
   using control_trajectory = Mat<float, CONTROL_DIM, MAX_TIMESTEPS>;
   using state_trajectory = Mat<float, STATE_DIM, MAX_TIMESTEPS>;
+  using output_trajectory = Mat<float, OUTPUT_DIM, MAX_TIMESTEPS>;
+  // the template arguments, as the reference's plant reads them (controllers/controller.cuh:70-110)
+  using TEMPLATED_DYNAMICS = DYN_T;
+  using TEMPLATED_COSTS = COST_T;
+  using TEMPLATED_FEEDBACK = FB_T;
+  using TEMPLATED_SAMPLING = SAMPLING_T;
+  using TEMPLATED_SAMPLING_PARAMS = typename SAMPLING_T::SAMPLING_PARAMS_T;
+  using TEMPLATED_PARAMS = PARAMS_T;
 
   // controllers/MPPI/mppi_controller.cuh:36-41
   VanillaMPPIController(DYN_T* model, COST_T* cost, FB_T* fb_controller, SAMPLING_T* sampler, float dt, int max_iter,
@@ -682,11 +696,72 @@ class VanillaMPPIController {
     }
   }
   PARAMS_T getParams() const { return params_; }
-  void setParams(const PARAMS_T& p)
+  // New controller parameters; dt, lambda, alpha, the slide scale and the seed are swapped into the running controller
+  // on the next computeControl, other changes (horizon, iteration count) create a new one.
+  void setParams(const PARAMS_T& p) { params_ = p; }
+  void setSamplingParams(const TEMPLATED_SAMPLING_PARAMS& p) { sampler_->setParams(p); }
+
+  // --------------------------------------------------------------------- what the plant uses between solutions
+  // interpolateControls (controllers/controller.cuh:383-394): linear between the two neighbouring steps
+  control_array interpolateControls(double rel_time, const control_trajectory& c_traj) const
   {
-    params_ = p;
-    release();
+    const double dt = params_.dt_;
+    const int last = params_.num_timesteps_ - 1;
+    const int lo = std::min(std::max((int)(rel_time / dt), 0), last), hi = std::min(lo + 1, last);
+    const double a = (rel_time - lo * dt) / dt;
+    control_array u;
+    for (int c = 0; c < CONTROL_DIM; ++c) u[c] = (float)((1.0 - a) * c_traj(c, lo) + a * c_traj(c, hi));
+    return u;
   }
+  // interpolateState (controllers/controller.cuh:396-403): linear, per state
+  state_array interpolateState(const state_trajectory& s_traj, double rel_time) const
+  {
+    const double dt = params_.dt_;
+    const int last = params_.num_timesteps_ - 1;
+    const int lo = std::min(std::max((int)(rel_time / dt), 0), last), hi = std::min(lo + 1, last);
+    const double a = (rel_time - lo * dt) / dt;
+    state_array x;
+    for (int i = 0; i < STATE_DIM; ++i) x[i] = (float)((1.0 - a) * s_traj(i, lo) + a * s_traj(i, hi));
+    return x;
+  }
+  // getCurrentControl (controllers/controller.cuh:347-363): the interpolated feed-forward control, constrained; the
+  // feedback term is zero (NoFeedback)
+  template <class FB_STATE>
+  control_array getCurrentControl(const state_array& /*state*/, double rel_time,
+                                  const state_array& /*target_nominal_state*/, const control_trajectory& c_traj,
+                                  FB_STATE& /*fb_state*/) const
+  {
+    control_array u = interpolateControls(rel_time, c_traj);
+    enforce(u);
+    return u;
+  }
+  // only used by RMPPI in the reference (controllers/controller.cuh:327-329)
+  void updateImportanceSamplingControl(const state_array&, int) {}
+  void resetControls() { control_ = params_.init_control_traj_; }
+  bool getFeedbackEnabled() const { return false; }
+  void computeFeedback(const state_array&) {}
+  void computeFeedbackPropagatedStateSeq() {}
+  // outputs of the state trajectory (stateToOutput per step; the built-in models output their state)
+  output_trajectory getTargetOutputSeq() const
+  {
+    const state_trajectory s = getTargetStateSeq();
+    output_trajectory o;
+    for (int t = 0; t < params_.num_timesteps_; ++t) {
+      if constexpr (is_device_plugin<DYN_T>::value) {
+        model_->stateToOutput(s.data() + (size_t)t * STATE_DIM, o.data() + (size_t)t * OUTPUT_DIM);
+      } else {
+        for (int i = 0; i < OUTPUT_DIM; ++i) o(i, t) = s(i, t);
+      }
+    }
+    return o;
+  }
+
+  // the plugins, non-owning (controllers/controller.cuh:804-807); parameter changes made through them are picked up
+  // by the next computeControl
+  DYN_T* model_;
+  COST_T* cost_;
+  FB_T* fb_;
+  SAMPLING_T* sampler_;
   void setSeedCUDARandomNumberGen(unsigned long long seed)
   {
     params_.seed_ = seed;
@@ -696,21 +771,51 @@ class VanillaMPPIController {
   float getDt() const { return params_.dt_; }
   float getLambda() const { return params_.lambda_; }
   float getAlpha() const { return params_.alpha_; }
-  // plugin parameters changed: the device copies are rebuilt on the next computeControl
+  // rebuild the device side from scratch on the next computeControl
   void resetPlugins() { release(); }
 
  private:
-  void ensure()
+  void enforce(control_array& u) const
   {
-    if (ctrl_) return;
-    const int T = params_.num_timesteps_;
-    if (T < 1 || T > MAX_TIMESTEPS) throw std::runtime_error("num_timesteps out of range");
+    if constexpr (is_device_plugin<DYN_T>::value) {
+      model_->enforceConstraints(u.data());
+    } else {
+      state_array x = state_array::Zero();
+      model_->enforceConstraints(x, u);
+    }
+  }
+
+  // Everything the device side depends on, as bytes: the parameter blocks (for plugin models the objects themselves,
+  // which are passed to the kernels by value), the controller parameters and the cost map's version.
+  std::string snapshot(const mppi_controller_desc& d) const
+  {
+    std::string s;
+    auto add = [&s](const void* p, size_t n) { s.append(static_cast<const char*>(p), n); };
+    add(&d.dynamics, sizeof(d.dynamics));
+    add(&d.cost, sizeof(d.cost));
+    add(&d.sampler, sizeof(d.sampler));
+    add(&d.num_rollouts, sizeof(int32_t) * 2);
+    add(&d.dt, sizeof(float) * 3);
+    add(&d.num_iters, sizeof(int32_t) * 3);
+    add(d.slide_control_scale, sizeof(d.slide_control_scale));
+    if constexpr (is_device_plugin<DYN_T>::value) {
+      add(static_cast<const void*>(model_), sizeof(DYN_T));
+      add(static_cast<const void*>(cost_), sizeof(COST_T));
+    } else {
+      const unsigned long long v = cost_->costMapVersion();
+      add(&v, sizeof(v));
+    }
+    return s;
+  }
+
+  mppi_controller_desc make_desc() const
+  {
     mppi_controller_desc d;
     std::memset(&d, 0, sizeof(d));
     d.sampler = sampler_->desc();
     d.sampler.seed = params_.seed_;
     d.num_rollouts = NUM_ROLLOUTS;
-    d.num_timesteps = T;
+    d.num_timesteps = params_.num_timesteps_;
     d.dt = params_.dt_;
     d.lambda = params_.lambda_;
     d.alpha = params_.alpha_;
@@ -719,14 +824,44 @@ class VanillaMPPIController {
     for (int c = 0; c < CONTROL_DIM; ++c) d.slide_control_scale[c] = params_.slide_control_scale_[c];
     d.init_control = control_.data();  // [T][C] row-major == the first T columns of the col-major C x MAX_T matrix
     if constexpr (is_device_plugin<DYN_T>::value) {
-      ctrl_ = PluginControllerFactory<DYN_T, COST_T>::create(d, model_, cost_, plugin_keep_);
+      d.dynamics = model_->desc();
     } else {
       d.dynamics = model_->desc();
       d.cost = cost_->desc(*model_);
+    }
+    return d;
+  }
+
+  // Creates the device side on first use; afterwards swaps changed parameters into it (mppi_controller_set_params),
+  // or creates a new one when the change needs new buffers.
+  void ensure()
+  {
+    const int T = params_.num_timesteps_;
+    if (T < 1 || T > MAX_TIMESTEPS) throw std::runtime_error("num_timesteps out of range");
+    mppi_controller_desc d = make_desc();
+    std::string snap = snapshot(d);
+    if (ctrl_) {
+      if (snap == snapshot_) return;
+      if constexpr (!is_device_plugin<DYN_T>::value) {
+        if (cost_->costMapVersion() != map_version_) d.cost_map_host = cost_->costMapHost();
+        d.nn_theta_host = model_->hostParameters();
+      }
+      if (mppi_controller_set_params(ctrl_, &d) == MPPI_OK) {
+        snapshot_ = std::move(snap);
+        if constexpr (!is_device_plugin<DYN_T>::value) map_version_ = cost_->costMapVersion();
+        return;
+      }
+      release();
+    }
+    if constexpr (is_device_plugin<DYN_T>::value) {
+      ctrl_ = PluginControllerFactory<DYN_T, COST_T>::create(d, model_, cost_, plugin_keep_);
+    } else {
       d.cost_map_host = cost_->costMapHost();
       d.nn_theta_host = model_->hostParameters();
       check(mppi_controller_create(&d, &ctrl_), "mppi_controller_create");
+      map_version_ = cost_->costMapVersion();
     }
+    snapshot_ = std::move(snap);
   }
   void release()
   {
@@ -736,12 +871,10 @@ class VanillaMPPIController {
     state_fresh_ = false;
   }
 
-  DYN_T* model_;
-  COST_T* cost_;
-  FB_T* fb_;
-  SAMPLING_T* sampler_;
   PARAMS_T params_;
   mppi_controller* ctrl_ = nullptr;
+  std::string snapshot_;              // the parameters the device side was last given
+  unsigned long long map_version_ = 0;
   std::shared_ptr<void> plugin_keep_;  // the plugin callback table's state (plugin models only)
   control_trajectory control_;
   mutable state_trajectory state_;

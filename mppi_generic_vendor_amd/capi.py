@@ -153,6 +153,7 @@ _SIGS = {
     "mppi_controller_slide_control_seq": ([P, _i], ct.c_int),
     "mppi_controller_device_buffers": ([P, ct.POINTER(P), ct.POINTER(P), ct.POINTER(P)], ct.c_int),
     "mppi_controller_set_seed": ([P, ct.c_uint64], ct.c_int),
+    "mppi_controller_set_params": ([P, ct.POINTER(ControllerDesc)], ct.c_int),
     "mppi_controller_stream": ([P, ct.POINTER(P)], ct.c_int),
     "mppi_batch_create": ([ct.POINTER(InstanceDesc), _i, P, _i, _i, ct.POINTER(P)], ct.c_int),
     "mppi_batch_destroy": ([P], ct.c_int),

@@ -47,6 +47,8 @@ class VanillaMPPIController:
             th = np.ascontiguousarray(problem.nn_theta, dtype=np.float32)
             self._keep.append(th)
             d.nn_theta_host = th.ctypes.data
+        self._num_iters, self._use_graph, self._keep_samples = num_iters, d.use_graph, d.keep_samples
+        self._slide = [d.slide_control_scale[i] for i in range(capi.MAX_CONTROL)]
         h = ct.c_void_p()
         capi.call("mppi_controller_create", ct.byref(d), ct.byref(h))
         self._h = h
@@ -111,6 +113,36 @@ class VanillaMPPIController:
 
     def slideControlSequence(self, steps: int):
         capi.call("mppi_controller_slide_control_seq", self._h, int(steps))
+
+    def setParams(self, problem: Problem, lambda_: float | None = None, alpha: float | None = None,
+                  dt: float | None = None):
+        """Swap new parameter blocks (dynamics, cost, sampler, dt, lambda, alpha; the problem's cost map and network
+        are uploaded again) into the running controller (mppi_controller_set_params): same sizes and kinds only."""
+        d = capi.ControllerDesc()
+        d.dynamics = problem.dyn
+        d.cost = problem.cost
+        d.sampler = problem.sampler
+        d.num_rollouts = self.K
+        d.num_timesteps = self.T
+        d.dt = problem.dt if dt is None else dt
+        d.lambda_ = problem.lambda_ if lambda_ is None else lambda_
+        d.alpha = problem.alpha if alpha is None else alpha
+        d.num_iters = self._num_iters
+        d.use_graph = self._use_graph
+        d.keep_samples = self._keep_samples
+        for i in range(capi.MAX_CONTROL):
+            d.slide_control_scale[i] = self._slide[i]
+        keep = []
+        if problem.cost_map is not None:
+            m = np.ascontiguousarray(problem.cost_map, dtype=np.float32)
+            keep.append(m)
+            d.cost_map_host = m.ctypes.data
+        if problem.nn_theta is not None:
+            th = np.ascontiguousarray(problem.nn_theta, dtype=np.float32)
+            keep.append(th)
+            d.nn_theta_host = th.ctypes.data
+        capi.call("mppi_controller_set_params", self._h, ct.byref(d))
+        self.problem = problem
 
     def setSeedCUDARandomNumberGen(self, seed: int):
         capi.call("mppi_controller_set_seed", self._h, ct.c_uint64(seed))

@@ -1198,6 +1198,74 @@ int mppi_controller_set_seed(mppi_controller* c, uint64_t seed)
   return MPPI_OK;
 }
 
+int mppi_controller_set_params(mppi_controller* c, const mppi_controller_desc* desc)
+{
+  if (!c || !desc) return fail(MPPI_E_INVALID, "null argument");
+  const mppi_controller_desc& o = c->d;
+  if (desc->num_rollouts != o.num_rollouts || desc->num_timesteps != o.num_timesteps ||
+      desc->num_iters != o.num_iters || desc->sampler.kind != o.sampler.kind ||
+      (!c->has_plugin && (desc->dynamics.model != o.dynamics.model || desc->cost.cost != o.cost.cost)))
+    return fail(MPPI_E_INVALID, "set_params: sizes, iteration count, model, cost or sampler kind changed "
+                                "(create a new controller)");
+  if (!(desc->lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
+  mppi_controller_desc n = *desc;
+  if (c->has_plugin) {
+    if (desc->dynamics.control_dim != c->C) return fail(MPPI_E_INVALID, "set_params: control_dim changed");
+    n.dynamics.model = 0;
+    n.dynamics.state_dim = c->S;
+    n.dynamics.output_dim = c->O;
+    n.dynamics.nn_theta = nullptr;
+    n.cost.map = nullptr;
+  } else {
+    MPPI_CHECK(validate_dynamics(n.dynamics));
+  }
+  MPPI_CHECK(validate_sampler(n.sampler, n.dynamics));
+  // queued replays read the old parameters: wait for them, then update the device-side copies
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (!c->has_plugin) {
+    if (desc->cost_map_host != nullptr) {
+      const size_t nm = (size_t)n.cost.map_width * n.cost.map_height;
+      if (nm < 4) return fail(MPPI_E_INVALID, "cost map: width/height must be >= 2");
+      const size_t old = (size_t)o.cost.map_width * o.cost.map_height;
+      if (c->map_d == nullptr || nm != old) {
+        if (c->map_d) MPPI_HIP_CHECK(hipFree(c->map_d));
+        c->map_d = nullptr;
+        MPPI_CHECK(dev_alloc(&c->map_d, sizeof(float) * nm));
+      }
+      MPPI_HIP_CHECK(hipMemcpy(c->map_d, desc->cost_map_host, sizeof(float) * nm, hipMemcpyHostToDevice));
+    }
+    // the controller's map copy stays in use unless the caller passes a device map of its own
+    if (c->map_d != nullptr && (desc->cost_map_host != nullptr || n.cost.map == nullptr)) n.cost.map = c->map_d;
+    MPPI_CHECK(validate_cost(n.cost, n.dynamics, true));
+    if (n.dynamics.model == MPPI_MODEL_RACER_DUBINS_NN) {
+      if (desc->nn_theta_host != nullptr) {
+        if (n.dynamics.nn_num_params != (int)c->nn_host.size())
+          return fail(MPPI_E_INVALID, "set_params: residual MLP parameter count changed");
+        c->nn_host.assign(desc->nn_theta_host, desc->nn_theta_host + c->nn_host.size());
+        MPPI_HIP_CHECK(hipMemcpy(c->nn_d, c->nn_host.data(), sizeof(float) * c->nn_host.size(),
+                                 hipMemcpyHostToDevice));
+      }
+      n.dynamics.nn_theta = c->nn_d;
+    }
+  }
+  n.init_control = nullptr;
+  n.cost_map_host = nullptr;
+  n.nn_theta_host = nullptr;
+  const bool reseed = n.sampler.seed != o.sampler.seed;
+  // every captured graph holds the old parameter blocks as kernel arguments
+  drop_graphs(c);
+  c->d = n;
+  if (reseed) {
+    c->rng_counter = 0;
+    c->loop_ready = false;
+  }
+  if (c->prop != nullptr) {
+    MPPI_HIP_CHECK(hipMemcpy(c->sigma_base_d, c->d.sampler.std_dev, sizeof(float) * MPPI_MAX_CONTROL,
+                             hipMemcpyHostToDevice));
+  }
+  return MPPI_OK;
+}
+
 int mppi_controller_stream(const mppi_controller* c, void** stream)
 {
   if (!c || !stream) return fail(MPPI_E_INVALID, "null argument");

@@ -9,7 +9,8 @@ PKG = ROOT / "mppi_generic_vendor_amd"
 def build_program(name: str, out_dir: Path) -> Path:
     src = ROOT / "tests" / "cpp" / f"{name}.cpp"
     exe = out_dir / name
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-pthread", "-I", str(ROOT / "include"), str(src), "-o",
+           str(exe),
            "-L", str(PKG), "-lmppi_amd", f"-Wl,-rpath,{PKG}"]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     return exe

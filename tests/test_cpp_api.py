@@ -12,8 +12,8 @@ PREBUILT = ROOT / "tests" / "cpp" / "_build" / "cartpole_closed_loop"
 
 
 def test_cpp_api_compiles_with_gcc(tmp_path, engine):
-    exe = build_program("cartpole_closed_loop", tmp_path)
-    assert exe.exists()
+    for name in ("cartpole_closed_loop", "plant_loop"):
+        assert build_program(name, tmp_path).exists()
 
 
 def test_host_step_matches_oracle(engine, oracle):
@@ -61,6 +61,21 @@ def test_host_step_nn_matches_oracle(engine, oracle):
 def test_cpp_closed_loop_swing_up(tmp_path, hip, engine):
     exe = PREBUILT if PREBUILT.exists() else build_program("cartpole_closed_loop", tmp_path)
     print("[gpu] C++ cartpole closed loop", flush=True)
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0, res.stdout + res.stderr
+
+
+PLANT = ROOT / "tests" / "cpp" / "_build" / "plant_loop"
+
+
+@pytest.mark.gpu
+def test_cpp_plant_loop(tmp_path, hip, engine):
+    """BasePlant (include/mppi_amd/plant.hpp) around the cart-pole controller: interpolated controls between
+    solutions, swing-up, a cost-parameter swap mid-run, the threaded runControlLoop with a dynamics-parameter swap
+    queued from the robot thread (tests/cpp/plant_loop.cpp prints each check)."""
+    exe = PLANT if PLANT.exists() else build_program("plant_loop", tmp_path)
+    print("[gpu] C++ plant loop", flush=True)
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stdout + res.stderr

@@ -153,3 +153,39 @@ def test_set_seed_resets_the_proposal_graph(hip, engine):
     assert np.array_equal(kept, run(42, None))  # reseeding with the same seed restarts its stream
     assert not np.array_equal(fresh, kept)
     net.close()
+
+
+def test_set_params_swaps_parameters_in_place(hip, engine):
+    """mppi_controller_set_params: after captured calls, new cost / dynamics / sampler blocks, lambda and a new cost
+    map are swapped in; the next call equals a fresh controller built with them (same nominal, same generator
+    counter position restarted by the new seed), and a model change is refused."""
+    banner("set_params hot swap")
+    small = workloads.gaussian_blob_map(256, 256, blobs=16, seed=7)
+    prob = workloads.double_integrator_map(num_rollouts=2048, num_timesteps=60, colored=False, cost_map=small)
+    T, C = prob.num_timesteps, prob.C
+    zeros = np.zeros((T, C), np.float32)
+    a = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+    for _ in range(3):
+        a.computeControl(prob.x0, 1)
+    new = workloads.double_integrator_map(num_rollouts=2048, num_timesteps=60, colored=False,
+                                          cost_map=workloads.gaussian_blob_map(256, 256, blobs=9, seed=3))
+    new.cost.goal[0], new.cost.goal[1] = -1.0, 0.5
+    new.cost.terminal_cost_coeff = 3.0
+    new.dyn.control_min[0], new.dyn.control_max[0] = -0.7, 0.7
+    new.sampler.std_dev[1] = 0.4
+    new.sampler.seed = 99
+    new.lambda_ = 0.5
+    a.setParams(new)
+    a.updateImportanceSampler(zeros)
+    a.computeControl(prob.x0, 1)
+    got = a.getRawMean()
+    got_st = a.getWeightStats()
+    b = VanillaMPPIController(new, num_iters=1, use_graph=True)
+    b.computeControl(prob.x0, 1)
+    np.testing.assert_array_equal(got, b.getRawMean())
+    assert got_st.baseline == b.getWeightStats().baseline and got_st.lambda_ == 0.5
+    bad = workloads.cartpole(num_rollouts=2048, num_timesteps=60)
+    with pytest.raises(RuntimeError):
+        a.setParams(bad)
+    a.close()
+    b.close()
