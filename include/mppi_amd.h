@@ -339,6 +339,26 @@ int mppi_controller_device_buffers(const mppi_controller* ctrl, float** samples_
  * kind must stay the same (MPPI_E_INVALID otherwise: create a new controller). Synchronises the controller's stream
  * and drops its captured graphs (the next calls capture them again); a new seed restarts the generator counter. */
 int mppi_controller_set_params(mppi_controller* ctrl, const mppi_controller_desc* desc);
+/* Sampled-trajectory export (copySampledControlFromDevice / copyTopControlFromDevice and
+ * calculateSampledStateTrajectories, controllers/controller.cu:57-181,287-354, controllers/MPPI/mppi_controller.cuh:
+ * 54-59, with the visualisation rollouts of core/mppi_common.cu:371-691). Enabling it (num_random + num_top > 0) makes
+ * every computeControl keep the constrained samples of its last iteration (stored from the rollout kernel's LDS tile)
+ * and that iteration's sampling mean; 0 and 0 turn it off. 1 + num_random + num_top <= min(K, 4096). Drops the
+ * captured graphs. Not available with a learned proposal or a dynamics plugin. */
+#define MPPI_MAX_SAMPLED_TRAJECTORIES 4096
+int mppi_controller_set_sampled_trajectories(mppi_controller* ctrl, int32_t num_random, int32_t num_top);
+/* After a computeControl: count = 1 + num_random + num_top trajectories, re-rolled on the GPU by one gather kernel
+ * (one lane per trajectory) from the call's x0: slot 0 is sample 0 (the nominal sequence), then num_random samples
+ * drawn without replacement from samples 1 .. 0.98 K - 1 (a generator seeded from the sampler seed and the number of
+ * exports so far), then the num_top samples of largest weight (lowest cost; lower index first on ties) by decreasing
+ * weight. Per trajectory: indices (sample index), controls [T][C] (as rolled out, constrained), outputs [T][O] (the
+ * post-step output y_{t+1} at index t), step_costs [T] ((running cost of y_{t+1} + likelihood-ratio cost of u_t) / T;
+ * every step, where the reference's visualisation drops step 0), terminal_costs (terminal cost of y_T / T; the step
+ * costs and the terminal cost add up to the sample's rollout cost) and weights (w_k / normalizer of the last
+ * iteration). Any output pointer may be NULL; arrays are sized for count trajectories. Synchronises. */
+int mppi_controller_get_sampled_trajectories(mppi_controller* ctrl, int32_t* count, int32_t* indices, float* controls,
+                                             float* outputs, float* step_costs, float* terminal_costs,
+                                             float* weights);
 /* Reseed the generator and reset its counter (controllers/controller.cu:203-209). */
 int mppi_controller_set_seed(mppi_controller* ctrl, uint64_t seed);
 /* The stream all controller work runs on (hipStream_t as void*). */

@@ -154,6 +154,8 @@ _SIGS = {
     "mppi_controller_device_buffers": ([P, ct.POINTER(P), ct.POINTER(P), ct.POINTER(P)], ct.c_int),
     "mppi_controller_set_seed": ([P, ct.c_uint64], ct.c_int),
     "mppi_controller_set_params": ([P, ct.POINTER(ControllerDesc)], ct.c_int),
+    "mppi_controller_set_sampled_trajectories": ([P, _i, _i], ct.c_int),
+    "mppi_controller_get_sampled_trajectories": ([P, ct.POINTER(_i), P, P, P, P, P, P], ct.c_int),
     "mppi_controller_stream": ([P, ct.POINTER(P)], ct.c_int),
     "mppi_batch_create": ([ct.POINTER(InstanceDesc), _i, P, _i, _i, ct.POINTER(P)], ct.c_int),
     "mppi_batch_destroy": ([P], ct.c_int),

@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <numeric>
+#include <random>
 #include <utility>
 #include <new>
 #include <set>
@@ -300,6 +302,12 @@ struct mppi_controller {
   int32_t n_prop = 0, n_fallback = 0;
   mppi_weight_stats prop_stats{};
   float prop_cost0 = 0.0f;
+  // sampled-trajectory export: the last iteration's constrained samples are kept and its sampling mean saved
+  int vis_random = 0, vis_top = 0;
+  uint64_t vis_draws = 0;
+  float* vis_mean_d = nullptr;
+  int* vis_idx_d = nullptr;
+  float* vis_buf_d = nullptr;  // controls [n][T][C], outputs [n][T][O], step costs [n][T], terminal [n]
   // stream time of the last computeControl from the first upload to the last download
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_gpu_ms = 0.0f;
@@ -580,6 +588,9 @@ int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_
 }
 
 // --------------------------------------------------------------------------------------------------- controller
+// The sample buffer receives the constrained samples of every iteration with keep_samples or the trajectory export.
+static bool keeps_samples(const mppi_controller* c) { return c->d.keep_samples != 0 || c->vis_random + c->vis_top > 0; }
+
 // Host post-processing of a plugin controller: the plugin's own constraint and step (Dynamics::enforceConstraints and
 // Dynamics::step on the host) in place of the built-in models'.
 static void plugin_clamp_sequence(const mppi_controller* c, float* u)
@@ -622,7 +633,8 @@ static int controller_free(mppi_controller* c)
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
-                 c->map_d, c->nn_d, c->loop_iter_d, c->prop_dmu_d, c->prop_sig_d, c->sigma_base_d};
+                 c->map_d, c->nn_d, c->loop_iter_d, c->prop_dmu_d, c->prop_sig_d, c->sigma_base_d,
+                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h};
@@ -767,7 +779,7 @@ static int enqueue_plugin_iteration(mppi_controller* c, IterRef ref, IterArgs* a
   it.num_rollouts = c->K;
   it.num_timesteps = c->T;
   it.generate = colored ? 0 : 1;
-  it.write_samples = d.keep_samples != 0 ? 1 : 0;
+  it.write_samples = keeps_samples(c) ? 1 : 0;
   it.dt = d.dt;
   it.lambda = d.lambda;
   it.alpha = d.alpha;
@@ -798,7 +810,7 @@ static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
   // the combine kernel writes the new mean over the sampling mean, as updateDistributionParamsFromDevice writes
   // control_means_d_ (sampling_distributions/gaussian/gaussian.cu:434-467); the next iteration samples around it
   return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->K, c->T, d.dt, d.lambda, d.alpha, c->x0_d,
-                                c->mean_d, c->samples_d, !colored, d.keep_samples != 0, ref, c->costs_d, c->ws_d,
+                                c->mean_d, c->samples_d, !colored, keeps_samples(c), ref, c->costs_d, c->ws_d,
                                 c->stats_d, c->mean_d, advance, c->stream, nullptr, nullptr, copies,
                                 colored ? &plan : nullptr, advance_step, between, partials_out);
 }
@@ -809,15 +821,22 @@ static int controller_enqueue(mppi_controller* c)
   const int K = c->K, T = c->T, C = c->C, S = c->S;
   const hipStream_t s = c->stream;
   const mppi_controller_desc& d = c->d;
+  const bool vis = c->vis_random + c->vis_top > 0;
   const WordSpan in[] = {{c->iter_hd, c->iter_d, words_of<IterArgs>(d.num_iters)},
                          {c->x0_hd, c->x0_d, S},
-                         {c->nominal_hd, c->mean_d, T * C}};
-  MPPI_CHECK(launch_copy_words(in, 3, s));
+                         {c->nominal_hd, c->mean_d, T * C},
+                         {c->nominal_hd, c->vis_mean_d, T * C}};
+  // with the trajectory export and one iteration, the nominal is also the last iteration's sampling mean
+  MPPI_CHECK(launch_copy_words(in, vis && d.num_iters == 1 ? 4 : 3, s));
   // the last combine also writes the new mean and the statistics to the host buffers
   ResultCopies out;
   out.mean = c->mean_out_hd;
   out.stats = c->stats_hd;
   for (int it = 0; it < d.num_iters; ++it) {
+    if (vis && d.num_iters > 1 && it + 1 == d.num_iters) {
+      const WordSpan m[] = {{c->mean_d, c->vis_mean_d, T * C}};
+      MPPI_CHECK(launch_copy_words(m, 1, s));
+    }
     const IterRef ref{c->iter_d + it, {0, 0, 0}};
     MPPI_CHECK(enqueue_iteration(c, ref, nullptr, it + 1 == d.num_iters ? &out : nullptr));
   }
@@ -956,6 +975,8 @@ int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, in
     return MPPI_OK;
   }
   if (c->has_plugin) return fail(MPPI_E_UNSUPPORTED, "proposal: not available with a dynamics plugin");
+  if (c->vis_random + c->vis_top > 0)
+    return fail(MPPI_E_UNSUPPORTED, "proposal: turn the sampled-trajectory export off first");
   const PropDims pd = proposal_dims(proposal_net_of(proposal));
   if (pd.S != c->S || pd.C != c->C || pd.T != c->T)
     return fail(MPPI_E_INVALID, "proposal: state/control/horizon dims do not match the controller");
@@ -1263,6 +1284,99 @@ int mppi_controller_set_params(mppi_controller* c, const mppi_controller_desc* d
     MPPI_HIP_CHECK(hipMemcpy(c->sigma_base_d, c->d.sampler.std_dev, sizeof(float) * MPPI_MAX_CONTROL,
                              hipMemcpyHostToDevice));
   }
+  return MPPI_OK;
+}
+
+int mppi_controller_set_sampled_trajectories(mppi_controller* c, int32_t num_random, int32_t num_top)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  if (num_random < 0 || num_top < 0) return fail(MPPI_E_INVALID, "sampled trajectories: negative count");
+  const int n = 1 + num_random + num_top;
+  const int pool = (int)(0.98 * c->K) - 1;  // random picks come from samples 1 .. 0.98 K - 1
+  if (n > std::min(c->K, MPPI_MAX_SAMPLED_TRAJECTORIES) || num_random > std::max(pool, 0) || num_top > c->K)
+    return fail(MPPI_E_SIZE, "sampled trajectories: too many for the sample count");
+  if (c->prop != nullptr || c->has_plugin)
+    return fail(MPPI_E_UNSUPPORTED, "sampled trajectories: not available with a learned proposal or a plugin");
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  // the control call's graph changes (samples kept, mean saved): capture it again
+  drop_graphs(c);
+  for (void* p : {(void*)c->vis_mean_d, (void*)c->vis_idx_d, (void*)c->vis_buf_d})
+    if (p) MPPI_HIP_CHECK(hipFree(p));
+  c->vis_mean_d = nullptr;
+  c->vis_idx_d = nullptr;
+  c->vis_buf_d = nullptr;
+  c->vis_random = c->vis_top = 0;
+  if (num_random + num_top == 0) return MPPI_OK;
+  const int T = c->T;
+  MPPI_CHECK(dev_alloc(&c->vis_mean_d, sizeof(float) * T * c->C));
+  MPPI_CHECK(dev_alloc(&c->vis_idx_d, sizeof(int) * n));
+  MPPI_CHECK(dev_alloc(&c->vis_buf_d, sizeof(float) * (size_t)n * (T * c->C + T * c->O + T + 1)));
+  c->vis_random = num_random;
+  c->vis_top = num_top;
+  return MPPI_OK;
+}
+
+int mppi_controller_get_sampled_trajectories(mppi_controller* c, int32_t* count, int32_t* indices, float* controls,
+                                             float* outputs, float* step_costs, float* terminal_costs,
+                                             float* weights)
+{
+  if (!c || !count) return fail(MPPI_E_INVALID, "null argument");
+  if (c->vis_random + c->vis_top == 0)
+    return fail(MPPI_E_INVALID, "sampled trajectories: enable them with mppi_controller_set_sampled_trajectories");
+  if (!c->computed) return fail(MPPI_E_INVALID, "sampled trajectories: call computeControl first");
+  const int K = c->K, T = c->T, C = c->C, O = c->O;
+  const int n = 1 + c->vis_random + c->vis_top;
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  std::vector<float> cost((size_t)K);
+  MPPI_HIP_CHECK(hipMemcpy(cost.data(), c->costs_d, sizeof(float) * K, hipMemcpyDeviceToHost));
+  for (auto& v : cost)
+    if (!(v <= FLT_MAX)) v = INFINITY;
+  std::vector<int> sel;
+  sel.reserve((size_t)n);
+  sel.push_back(0);  // sample 0 is the nominal sequence itself
+  // random samples without replacement from 1 .. 0.98 K - 1 (the last ones are pure noise), partial Fisher-Yates
+  {
+    std::mt19937_64 gen(c->d.sampler.seed * 0x9E3779B97F4A7C15ull + 0x5851F42D4C957F2Dull * ++c->vis_draws);
+    std::vector<int> pool((size_t)((int)(0.98 * K) - 1));
+    std::iota(pool.begin(), pool.end(), 1);
+    for (int i = 0; i < c->vis_random; ++i) {
+      std::uniform_int_distribution<size_t> pick((size_t)i, pool.size() - 1);
+      std::swap(pool[(size_t)i], pool[pick(gen)]);
+      sel.push_back(pool[(size_t)i]);
+    }
+  }
+  // the num_top samples of largest weight (lowest cost, the lower index first on ties), by decreasing weight
+  {
+    std::vector<int> order((size_t)K);
+    std::iota(order.begin(), order.end(), 0);
+    std::partial_sort(order.begin(), order.begin() + c->vis_top, order.end(), [&](int a, int b) {
+      return cost[(size_t)a] < cost[(size_t)b] || (cost[(size_t)a] == cost[(size_t)b] && a < b);
+    });
+    for (int i = 0; i < c->vis_top; ++i) sel.push_back(order[(size_t)i]);
+  }
+  MPPI_HIP_CHECK(hipMemcpy(c->vis_idx_d, sel.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  const mppi_controller_desc& d = c->d;
+  float* ctl = c->vis_buf_d;
+  float* out = ctl + (size_t)n * T * C;
+  float* stp = out + (size_t)n * T * O;
+  float* term = stp + (size_t)n * T;
+  MPPI_CHECK(launch_vis_rollouts(d.dynamics, d.cost, d.sampler, K, T, d.dt, d.lambda, d.alpha, c->x0_d, c->vis_mean_d,
+                                 c->samples_d, c->vis_idx_d, n, ctl, out, stp, term, c->stream));
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (controls) MPPI_HIP_CHECK(hipMemcpy(controls, ctl, sizeof(float) * n * T * C, hipMemcpyDeviceToHost));
+  if (outputs) MPPI_HIP_CHECK(hipMemcpy(outputs, out, sizeof(float) * n * T * O, hipMemcpyDeviceToHost));
+  if (step_costs) MPPI_HIP_CHECK(hipMemcpy(step_costs, stp, sizeof(float) * n * T, hipMemcpyDeviceToHost));
+  if (terminal_costs) MPPI_HIP_CHECK(hipMemcpy(terminal_costs, term, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (indices) std::copy(sel.begin(), sel.end(), indices);
+  if (weights) {
+    // normalized weights w_k / eta of the last iteration (core/mppi_common.cu:994-1002 over its costs)
+    const double b = c->stats.baseline, eta = c->stats.normalizer, inv = 1.0 / (double)d.lambda;
+    for (int i = 0; i < n; ++i) {
+      const double ck = cost[(size_t)sel[(size_t)i]];
+      weights[i] = (float)((ck <= FLT_MAX) ? std::exp(-inv * (ck - b)) / eta : 0.0);
+    }
+  }
+  *count = n;
   return MPPI_OK;
 }
 

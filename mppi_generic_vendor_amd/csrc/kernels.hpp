@@ -32,6 +32,13 @@ size_t weighted_update_workspace_bytes(int K, int T, int C);
 int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,
                            float* out, void* ws, hipStream_t s);
 
+// Re-rolls of n selected samples idx[n] of a [K][T][C] buffer of constrained samples (one lane each), recording
+// controls [n][T][C], post-step outputs [n][T][O], per-step costs [n][T] and terminal costs [n], each / T.
+int launch_vis_rollouts(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,
+                        int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
+                        const float* samples, const int* idx, int n, float* controls, float* outputs,
+                        float* step_costs, float* terminal, hipStream_t s);
+
 // The small per-call exchanges of a control call in one launch: up to four spans of 32-bit words, each between
 // device memory and mapped fine-grained host memory (device-side alias), read and written by the kernel itself.
 struct WordSpan {

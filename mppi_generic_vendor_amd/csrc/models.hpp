@@ -203,16 +203,17 @@ constexpr int NN_PARAMS = (NN_IN + 1) * NN_H1 + (NN_H1 + 1) * NN_H2 + (NN_H2 + 1
 
 MPPI_HD float tanh_acc(float v) { return tanhf(v); }
 
-// Host evaluation of the BF16 network (nn_precision == MPPI_NN_BF16): weights and layer inputs rounded to BF16 (ties
-// to even), biases and sums in FP32 -- the arithmetic of the matrix-core path, for the host-side state propagation.
-inline float bf16_round_host(float f)
+// Per-lane evaluation of the BF16 network (nn_precision == MPPI_NN_BF16): weights and layer inputs rounded to BF16
+// (ties to even), biases and sums in FP32 -- the arithmetic of the matrix-core path, for the host-side state
+// propagation and the per-trajectory re-rolls of the sampled-trajectory export.
+MPPI_HD float bf16_round_host(float f)
 {
   uint32_t b;
-  memcpy(&b, &f, 4);
+  __builtin_memcpy(&b, &f, 4);
   if ((b & 0x7f800000u) == 0x7f800000u) return f;
   b += 0x7fffu + ((b >> 16) & 1u);
   b &= 0xffff0000u;
-  memcpy(&f, &b, 4);
+  __builtin_memcpy(&f, &b, 4);
   return f;
 }
 
@@ -249,7 +250,7 @@ MPPI_HD void nn_forward_f32(const float* __restrict__ th, const float* in, float
   }
 }
 
-inline void nn_forward_bf16_host(const float* th, const float* in, float* out)
+MPPI_HD void nn_forward_bf16_host(const float* th, const float* in, float* out)
 {
   float a[NN_H1], b[NN_H1];
   const int layers[4] = {NN_IN, NN_H1, NN_H2, NN_OUT};
@@ -265,12 +266,29 @@ inline void nn_forward_bf16_host(const float* th, const float* in, float* out)
       float s = 0.0f;
       for (int k = 0; k < nin; ++k) s += bf16_round_host(W[j * nin + k]) * a[k];
       s += bias[j];
-      b[j] = l < 2 ? tanhf(s) : s;
+      b[j] = l < 2 ? tanh_acc(s) : s;
     }
     for (int j = 0; j < nout; ++j) a[j] = b[j];
   }
   for (int j = 0; j < NN_OUT; ++j) out[j] = a[j];
 }
+
+// The BF16-network bicycle evaluated one sample per lane (the export's re-rolls; the rollout kernels evaluate it on
+// the matrix cores).
+struct BicycleNNBf16Lane : Bicycle {
+  MPPI_HD static void deriv(const mppi_dynamics_params& p, const float* x, const float* u, float* xd)
+  {
+    Bicycle::deriv(p, x, u, xd);
+    float in[NN_IN], out[NN_OUT];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) in[i] = x[i];
+    in[7] = u[0];
+    in[8] = u[1];
+    nn_forward_bf16_host(p.nn_theta, in, out);
+#pragma unroll
+    for (int i = 0; i < NN_OUT; ++i) xd[p.nn_output_states[i]] += out[i];
+  }
+};
 
 template <>
 struct Model<MPPI_MODEL_RACER_DUBINS_NN> : Bicycle {

@@ -278,11 +278,39 @@ struct BuiltinSystem {
   const float* map;
   int state_dim() const { return dyn.state_dim; }
   int control_dim() const { return dyn.control_dim; }
+  int output_dim() const { return dyn.output_dim; }
   void enforce(R* u) const { enforce_constraints<R>(dyn, u); }
   void step(const R* x, R* xn, R* xd, const R* u, R* y, R dt) const { oracle::step<R>(dyn, x, xn, xd, u, y, dt); }
   R running(const R* y, const R*, int) const { return running_cost<R>(cst, map, y); }
   R terminal(const R* y) const { return terminal_cost<R>(cst, y); }
 };
+
+/* The same loop recording, per sample, the post-step outputs y_{t+1} [K][T][O], the per-step costs
+ * (running + likelihood-ratio) / T [K][T] and the terminal cost / T [K]. */
+template <class R>
+struct OutputTrace {
+  int T;
+  R* outputs;
+  R* step_costs;
+  R* terminal_costs;
+  void step(int k, int t, const R* y, int O, R c) const
+  {
+    for (int i = 0; i < O; ++i) outputs[((size_t)k * T + t) * O + i] = y[i];
+    step_costs[(size_t)k * T + t] = c;
+  }
+  void terminal(int k, R c) const { terminal_costs[k] = c; }
+};
+
+template <class R>
+static void rollout_trace(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,
+                          const mppi_sampler_params* smp, int K, int T, R dt, R lambda, R alpha, const float* x0,
+                          const float* mean, float* samples, R* costs, R* outputs, R* step_costs, R* terminal)
+{
+  const BuiltinSystem<R> sys{*dyn, *cst, map};
+  const OutputTrace<R> tr{T, outputs, step_costs, terminal};
+  rollout_loop<R>(sys, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, 0, K, tr);
+}
+
 
 } /* namespace oracle */
 
@@ -467,6 +495,24 @@ void oracle_rollout_f64(const mppi_dynamics_params* dyn, const mppi_cost_params*
   /* FP64 run of the same loop; the clamped controls written back are rounded to float. */
   const BuiltinSystem<double> sys{*dyn, *cst, map};
   rollout_loop_threads<double>(sys, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, nthreads);
+}
+
+void oracle_rollout_trace_f32(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,
+                              const mppi_sampler_params* smp, int K, int T, float dt, float lambda, float alpha,
+                              const float* x0, const float* mean, float* samples, float* costs, float* outputs,
+                              float* step_costs, float* terminal)
+{
+  rollout_trace<float>(dyn, cst, map, smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, outputs, step_costs,
+                       terminal);
+}
+
+void oracle_rollout_trace_f64(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,
+                              const mppi_sampler_params* smp, int K, int T, double dt, double lambda, double alpha,
+                              const float* x0, const float* mean, float* samples, double* costs, double* outputs,
+                              double* step_costs, double* terminal)
+{
+  rollout_trace<double>(dyn, cst, map, smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, outputs, step_costs,
+                        terminal);
 }
 
 /* --------------------------------------------------------------------------------------------- weights/update */

@@ -52,6 +52,10 @@ _SIGS = {
                             ct.POINTER(capi.SamplerParams), _i, _i, _f, _f, _f, P, P, P, P, _i], None),
     "oracle_rollout_f64": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
                             ct.POINTER(capi.SamplerParams), _i, _i, _d, _d, _d, P, P, P, P, _i], None),
+    "oracle_rollout_trace_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
+                                  ct.POINTER(capi.SamplerParams), _i, _i, _f, _f, _f, P, P, P, P, P, P, P], None),
+    "oracle_rollout_trace_f64": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
+                                  ct.POINTER(capi.SamplerParams), _i, _i, _d, _d, _d, P, P, P, P, P, P, P], None),
     "oracle_best_index": ([P, _i], _i),
     "oracle_weights": ([P, _i, _d, P, ct.POINTER(_f), ct.POINTER(_i), ct.POINTER(_d)], _d),
     "oracle_weights_f32": ([P, _i, _f, P], None),
@@ -148,6 +152,25 @@ def rollout_f64(prob, samples: np.ndarray, mean: np.ndarray, x0=None, nthreads: 
                               prob.dt, prob.lambda_, prob.alpha, ptr(_a0),
                               ptr(_a1), ptr(s), ptr(costs), nthreads)
     return costs, s
+
+
+def rollout_trace(prob, samples: np.ndarray, mean: np.ndarray, x0=None, f64: bool = False):
+    """(costs[K], outputs[K][T][O], step costs[K][T], terminal costs[K]) of the oracle loop, FP32 or FP64: outputs are
+    the post-step y_{t+1}, step costs (running + likelihood-ratio) / T, terminal / T."""
+    K, T = samples.shape[0], samples.shape[1]
+    O = prob.dyn.output_dim
+    dt = np.float64 if f64 else np.float32
+    s = np.ascontiguousarray(samples, dtype=np.float32).copy()
+    costs, outs = np.empty(K, dt), np.empty((K, T, O), dt)
+    steps, term = np.empty((K, T), dt), np.empty(K, dt)
+    dyn = _host_dyn(prob)
+    x0 = prob.x0 if x0 is None else x0
+    _a0 = np.ascontiguousarray(x0, np.float32)
+    _a1 = np.ascontiguousarray(mean, np.float32)
+    fn = load().oracle_rollout_trace_f64 if f64 else load().oracle_rollout_trace_f32
+    fn(ct.byref(dyn), ct.byref(prob.cost), ptr(prob.cost_map), ct.byref(prob.sampler), K, T, prob.dt, prob.lambda_,
+       prob.alpha, ptr(_a0), ptr(_a1), ptr(s), ptr(costs), ptr(outs), ptr(steps), ptr(term))
+    return costs, outs, steps, term
 
 
 _keep = []

@@ -12,7 +12,7 @@
  * restatement of that model (the oracle side of a user-plugin parity test).
  *
  * Sys provides, for the arithmetic type R:
- *   int state_dim() const, int control_dim() const
+ *   int state_dim() const, int control_dim() const, int output_dim() const
  *   void enforce(R* u) const                                   -- Dynamics::enforceConstraints
  *   void step(const R* x, R* xn, R* xd, const R* u, R* y, R dt) const   -- Dynamics::step
  *   R running(const R* y, const R* u, int t) const             -- Cost::computeRunningCost
@@ -74,9 +74,18 @@ inline R lr_cost(const mppi_sampler_params& s, int C, const R* u, const float* m
 
 constexpr int LOOP_MAX_DIM = 64;
 
-template <class R, class Sys>
+/* Records nothing (the plain loop). */
+struct NoTrace {
+  template <class R>
+  void step(int, int, const R*, int, R) const {}
+  template <class R>
+  void terminal(int, R) const {}
+};
+
+template <class R, class Sys, class Trace = NoTrace>
 void rollout_loop(const Sys& sys, const mppi_sampler_params& smp, int K, int T, R dt, R lambda, R alpha,
-                  const float* x0, const float* mean, float* samples, R* costs, int k_begin, int k_end)
+                  const float* x0, const float* mean, float* samples, R* costs, int k_begin, int k_end,
+                  const Trace& trace = Trace())
 {
   const int S = sys.state_dim(), C = sys.control_dim();
   const int pstart = pure_noise_start(K, smp.pure_noise_trajectories_percentage);
@@ -91,9 +100,12 @@ void rollout_loop(const Sys& sys, const mppi_sampler_params& smp, int K, int T, 
       sys.enforce(u);
       for (int i = 0; i < C; ++i) ut[i] = (float)u[i];
       sys.step(x, xn, xd, u, y, dt);
-      run += sys.running(y, u, t) + lr_cost<R>(smp, C, u, mean + (size_t)t * C, k >= pstart, lambda, alpha);
+      const R c = sys.running(y, u, t) + lr_cost<R>(smp, C, u, mean + (size_t)t * C, k >= pstart, lambda, alpha);
+      trace.step(k, t, y, sys.output_dim(), c / (R)T);
+      run += c;
       for (int i = 0; i < S; ++i) x[i] = xn[i];
     }
+    trace.terminal(k, sys.terminal(y) / (R)T);
     costs[k] = run / (R)T + sys.terminal(y) / (R)T;
   }
 }

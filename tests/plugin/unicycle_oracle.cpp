@@ -30,6 +30,7 @@ struct UnicycleSystem {
   mppi_dynamics_params lim;
   int state_dim() const { return 3; }
   int control_dim() const { return 2; }
+  int output_dim() const { return 4; }
   void enforce(R* u) const { oracle::enforce_constraints<R>(lim, u); }
   void step(const R* x, R* xn, R* xd, const R* u, R* y, R dt) const
   {
