@@ -144,6 +144,13 @@ typedef struct mppi_cost_params {
   float speed_coeff;
   float speed_target;
   int32_t speed_index;
+  /* Goal trajectory (QuadraticCostTrajectoryParams, cost_functions/quadratic_cost/quadratic_cost.cuh:12-63), for
+   * MPPI_COST_QUADRATIC_MAP: goal_traj [goal_traj_len][output_dim] (device pointer for the kernels, host pointer for
+   * mppi_host_running_cost); the running cost of step t uses row min(goal_time + t, goal_traj_len - 1) in place of
+   * goal (getIndex, :48-57). goal stays the terminal term's goal. goal_traj == NULL: goal at every step. */
+  int32_t goal_traj_len;
+  const float* goal_traj;
+  int32_t goal_time;
   int32_t _pad;
 } mppi_cost_params;
 
@@ -163,6 +170,11 @@ typedef struct mppi_sampler_params {
   int32_t _pad;
   /* Counter-based RNG (Philox4x32-10): normals are a pure function of (seed, counter, element index). */
   uint64_t seed;
+  /* Per-step standard deviations (GaussianTimeVaryingStdDevParams with time_specific_std_dev,
+   * sampling_distributions/gaussian/gaussian.cuh:66-80, gaussian.cu:38-43,499-502,659-667): device pointer
+   * [num_timesteps][control_dim] used in place of std_dev at step t, in the samples (times std_dev_decay^iteration)
+   * and in the likelihood-ratio cost. NULL: std_dev at every step. */
+  const float* std_dev_t;
 } mppi_sampler_params;
 
 /* Per-instance result of the weight transform (device-resident, 32 bytes). */
@@ -210,8 +222,10 @@ int mppi_pure_noise_start(int32_t num_rollouts, float pure_noise_percentage, int
 int mppi_host_step(const mppi_dynamics_params* p, const float* x, const float* u, float dt, float* x_next,
                    float* x_dot, float* y);
 int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u);
-/* Cost::computeRunningCost on the host (the kernels' cost code; c->map must point to host memory here). */
+/* Cost::computeRunningCost on the host (the kernels' cost code; c->map and c->goal_traj must point to host memory
+ * here), at step 0 and at step t (the step index picks the goal_traj row). */
 int mppi_host_running_cost(const mppi_cost_params* c, const float* y, float* out);
+int mppi_host_running_cost_at(const mppi_cost_params* c, const float* y, int32_t t, float* out);
 
 /* ------------------------------------------------------------------------------------------------------------------
  * Kernel-level entry points (device pointers, caller stream).
@@ -293,6 +307,10 @@ typedef struct mppi_controller_desc {
   const float* cost_map_host; /* optional host [map_height][map_width]; copied into controller-owned memory and
                                  used in place of cost.map */
   const float* nn_theta_host; /* optional host FNN parameters, copied like cost_map_host */
+  const float* std_dev_t_host; /* optional host [T][C] per-step standard deviations, copied and used in place of
+                                  sampler.std_dev_t */
+  const float* goal_traj_host; /* optional host [cost.goal_traj_len][O] goal trajectory, copied and used in place of
+                                  cost.goal_traj */
 } mppi_controller_desc;
 
 int mppi_controller_create(const mppi_controller_desc* desc, mppi_controller** out);
@@ -436,6 +454,10 @@ typedef struct mppi_instance_desc {
   float alpha;
   float slide_control_scale[MPPI_MAX_CONTROL]; /* ControllerParams::slide_control_scale_ */
   const float* init_control;     /* host [T][C] or NULL for zeros */
+  const float* std_dev_t_host;   /* optional host [T][C] per-step standard deviations (copied; replaces
+                                    sampler.std_dev_t) */
+  const float* goal_traj_host;   /* optional host [cost.goal_traj_len][O] goal trajectory (copied; replaces
+                                    cost.goal_traj) */
 } mppi_instance_desc;
 
 /* Launch shape of the persistent kernel: threads per workgroup (256, 512, 1024: one state-chain wavefront, the rest

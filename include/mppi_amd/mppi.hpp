@@ -353,6 +353,8 @@ class CartpoleQuadraticCost {
   }
   const float* costMapHost() const { return nullptr; }
   unsigned long long costMapVersion() const { return 0; }
+  const float* goalTrajectoryHost() const { return nullptr; }
+  int goalTrajectoryLength() const { return 0; }
 
  private:
   CartpoleQuadraticCostParams params_;
@@ -427,6 +429,8 @@ class QuadraticMapCost {
     return c;
   }
   const float* costMapHost() const { return map_.empty() ? nullptr : map_.data(); }
+  const float* goalTrajectoryHost() const { return nullptr; }
+  int goalTrajectoryLength() const { return 0; }
 
  private:
   COST_PARAMS_T params_;
@@ -434,6 +438,85 @@ class QuadraticMapCost {
   int width_ = 0, height_ = 0;
   unsigned long long map_version_ = 0;
 };
+
+// QuadraticCostTrajectoryParams (cost_functions/quadratic_cost/quadratic_cost.cuh:12-63): a goal per step for the
+// first SIM_TIME_HORIZON steps from current_time on; step t of a rollout is charged against row
+// getIndex(t) = min(current_time + t, SIM_TIME_HORIZON - 1).
+template <class DYN_T, int SIM_TIME_HORIZON = 1>
+struct QuadraticCostTrajectoryParams {
+  static const int TIME_HORIZON = SIM_TIME_HORIZON;
+  float s_goal[DYN_T::OUTPUT_DIM * SIM_TIME_HORIZON] = {0};
+  float s_coeffs[DYN_T::OUTPUT_DIM];
+  int current_time = 0;
+  QuadraticCostTrajectoryParams()
+  {
+    for (int i = 0; i < DYN_T::OUTPUT_DIM; ++i) s_coeffs[i] = 1.0f;
+  }
+  int getIndex(int t) const
+  {
+    int index = current_time + t;
+    if (index >= TIME_HORIZON) index = TIME_HORIZON - 1;
+    return index * DYN_T::OUTPUT_DIM;
+  }
+  Mat<float, DYN_T::OUTPUT_DIM, 1> getDesiredState(int t) const
+  {
+    Mat<float, DYN_T::OUTPUT_DIM, 1> g;
+    for (int i = 0; i < DYN_T::OUTPUT_DIM; ++i) g[i] = s_goal[getIndex(t) + i];
+    return g;
+  }
+  void setCurrentTime(int new_time) { current_time = new_time; }
+};
+
+// QuadraticCostTrajectory / QuadraticCost (quadratic_cost.cuh:111-127): the quadratic goal term alone, terminal cost
+// 0 (quadratic_cost.cu:28-32). Runs as the built-in quadratic cost with the map and speed terms off and, for a
+// horizon above 1, the goal table uploaded with the parameters.
+template <class DYN_T, int SIM_TIME_HORIZON>
+class QuadraticCostTrajectory {
+ public:
+  using COST_PARAMS_T = QuadraticCostTrajectoryParams<DYN_T, SIM_TIME_HORIZON>;
+  explicit QuadraticCostTrajectory(void* /*stream*/ = nullptr) {}
+  void setParams(const COST_PARAMS_T& p) { params_ = p; }
+  COST_PARAMS_T getParams() const { return params_; }
+  void GPUSetup() {}
+  void freeCudaMem() {}
+  void bindToStream(void*) {}
+  // computeStateCost (quadratic_cost.cu:10-26): sum_i s_coeffs[i] (y_i - goal_i(t))^2
+  float computeStateCost(const Mat<float, DYN_T::OUTPUT_DIM, 1>& y, int t = 0) const
+  {
+    float c = 0.0f;
+    for (int i = 0; i < DYN_T::OUTPUT_DIM; ++i) {
+      const float e = y[i] - params_.s_goal[params_.getIndex(t) + i];
+      c += e * e * params_.s_coeffs[i];
+    }
+    return c;
+  }
+  template <class DYN>
+  mppi_cost_params desc(const DYN& dyn) const
+  {
+    const mppi_dynamics_params d = dyn.desc();
+    mppi_cost_params c;
+    check(mppi_cost_defaults(MPPI_COST_QUADRATIC_MAP, &d, &c), "mppi_cost_defaults");
+    for (int i = 0; i < DYN_T::OUTPUT_DIM; ++i) {
+      c.coeff[i] = params_.s_coeffs[i];
+      c.goal[i] = params_.s_goal[params_.getIndex(0) + i];
+    }
+    c.terminal_cost_coeff = 0.0f;
+    c.map_coeff = 0.0f;
+    c.goal_traj_len = goalTrajectoryLength();
+    c.goal_time = SIM_TIME_HORIZON > 1 ? params_.current_time : 0;
+    return c;
+  }
+  const float* costMapHost() const { return nullptr; }
+  unsigned long long costMapVersion() const { return 0; }
+  const float* goalTrajectoryHost() const { return SIM_TIME_HORIZON > 1 ? params_.s_goal : nullptr; }
+  int goalTrajectoryLength() const { return SIM_TIME_HORIZON > 1 ? SIM_TIME_HORIZON : 0; }
+
+ private:
+  COST_PARAMS_T params_;
+};
+
+template <class DYN_T>
+using QuadraticCost = QuadraticCostTrajectory<DYN_T, 1>;
 
 // ==================================================================================================== samplers
 namespace sampling_distributions {
@@ -448,12 +531,26 @@ struct GaussianParams {
   float std_dev_decay = 1.0f;
   dim3 rewrite_controls_block_dim = dim3(32, 16, 1);  // accepted, unused: the engine picks its own launch shapes
   int sum_strides = 32;                               // accepted, unused
+  bool time_specific_std_dev = false;                 // set by GaussianTimeVaryingStdDevParams
   GaussianParams()
   {
     for (int i = 0; i < C_DIM; ++i) {
       std_dev[i] = 1.0f;
       control_cost_coeff[i] = 0.0f;
     }
+  }
+};
+
+// GaussianTimeVaryingStdDevParams (sampling_distributions/gaussian/gaussian.cuh:66-80): std_dev holds a row of
+// C_DIM standard deviations per step for the first MAX_TIMESTEPS steps (time_specific_std_dev set); the controller
+// uses the first num_timesteps rows, in sampling and in the likelihood-ratio cost.
+template <int C_DIM, int MAX_TIMESTEPS = 1>
+struct GaussianTimeVaryingStdDevParams : GaussianParams<C_DIM> {
+  float std_dev[C_DIM * MAX_TIMESTEPS];
+  GaussianTimeVaryingStdDevParams()
+  {
+    this->time_specific_std_dev = true;
+    for (int i = 0; i < C_DIM * MAX_TIMESTEPS; ++i) std_dev[i] = 1.0f;
   }
 };
 
@@ -494,6 +591,10 @@ class DistributionBase {
     s.std_dev_decay = params_.std_dev_decay;
     return s;
   }
+  // the per-step standard deviations [T][C] (row-major, the layout of the reference's time-varying std_dev), or
+  // nullptr: params_.std_dev of the final params type, which for the time-varying params is the [MAX_T][C] table
+  const float* stdDevTHost() const { return params_.time_specific_std_dev ? params_.std_dev : nullptr; }
+  int stdDevTRows() const { return (int)(sizeof(params_.std_dev) / sizeof(float)) / CONTROL_DIM; }
   virtual ~DistributionBase() = default;
 
  protected:
@@ -509,6 +610,16 @@ template <int C_DIM>
 class GaussianDistributionC : public DistributionBase<GaussianParams<C_DIM>, MPPI_SAMPLER_GAUSSIAN> {
  public:
   using DistributionBase<GaussianParams<C_DIM>, MPPI_SAMPLER_GAUSSIAN>::DistributionBase;
+};
+
+// GaussianDistribution over GaussianTimeVaryingStdDevParams (gaussian.cuh:120 with that params template).
+template <int C_DIM, int MAX_TIMESTEPS>
+class GaussianTimeVaryingDistributionC
+    : public DistributionBase<GaussianTimeVaryingStdDevParams<C_DIM, MAX_TIMESTEPS>, MPPI_SAMPLER_GAUSSIAN> {
+ public:
+  static const int MAX_T = MAX_TIMESTEPS;
+  using DistributionBase<GaussianTimeVaryingStdDevParams<C_DIM, MAX_TIMESTEPS>,
+                         MPPI_SAMPLER_GAUSSIAN>::DistributionBase;
 };
 
 template <int C_DIM>
@@ -804,7 +915,9 @@ class VanillaMPPIController {
     } else {
       const unsigned long long v = cost_->costMapVersion();
       add(&v, sizeof(v));
+      if (d.goal_traj_host) add(d.goal_traj_host, sizeof(float) * d.cost.goal_traj_len * DYN_T::OUTPUT_DIM);
     }
+    if (d.std_dev_t_host) add(d.std_dev_t_host, sizeof(float) * d.num_timesteps * CONTROL_DIM);
     return s;
   }
 
@@ -823,11 +936,13 @@ class VanillaMPPIController {
     d.use_graph = params_.use_graph_ ? 1 : 0;
     for (int c = 0; c < CONTROL_DIM; ++c) d.slide_control_scale[c] = params_.slide_control_scale_[c];
     d.init_control = control_.data();  // [T][C] row-major == the first T columns of the col-major C x MAX_T matrix
+    d.std_dev_t_host = sampler_->stdDevTHost();
     if constexpr (is_device_plugin<DYN_T>::value) {
       d.dynamics = model_->desc();
     } else {
       d.dynamics = model_->desc();
       d.cost = cost_->desc(*model_);
+      d.goal_traj_host = cost_->goalTrajectoryHost();
     }
     return d;
   }
@@ -838,6 +953,8 @@ class VanillaMPPIController {
   {
     const int T = params_.num_timesteps_;
     if (T < 1 || T > MAX_TIMESTEPS) throw std::runtime_error("num_timesteps out of range");
+    if (sampler_->stdDevTHost() && sampler_->stdDevTRows() < T)
+      throw std::runtime_error("per-step std_dev table holds fewer rows than num_timesteps");
     mppi_controller_desc d = make_desc();
     std::string snap = snapshot(d);
     if (ctrl_) {

@@ -28,6 +28,15 @@ class MPPIBatch:
             d.lambda_ = p.lambda_
             d.alpha = p.alpha
             d.init_control = None
+            if p.std_dev_t is not None:
+                a = np.ascontiguousarray(p.std_dev_t, dtype=np.float32).reshape(p.num_timesteps, p.C)
+                self._keep.append(a)
+                d.std_dev_t_host = a.ctypes.data
+            if p.goal_traj is not None:
+                g = np.ascontiguousarray(p.goal_traj, dtype=np.float32).reshape(-1, p.dyn.output_dim)
+                self._keep.append(g)
+                d.goal_traj_host = g.ctypes.data
+                d.cost.goal_traj_len = g.shape[0]
         mp = None
         h = w = 0
         if cost_map is not None:

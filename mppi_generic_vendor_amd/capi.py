@@ -67,7 +67,8 @@ class CostParams(ct.Structure):
         ("coeff", _f * MAX_OUTPUT), ("goal", _f * MAX_OUTPUT), ("terminal_cost_coeff", _f),
         ("map", ct.c_void_p), ("map_width", _i), ("map_height", _i), ("map_origin_x", _f), ("map_origin_y", _f),
         ("map_resolution", _f), ("map_coeff", _f), ("map_x_index", _i), ("map_y_index", _i),
-        ("speed_coeff", _f), ("speed_target", _f), ("speed_index", _i), ("_pad", _i),
+        ("speed_coeff", _f), ("speed_target", _f), ("speed_index", _i),
+        ("goal_traj_len", _i), ("goal_traj", ct.c_void_p), ("goal_time", _i), ("_pad", _i),
     ]
 
 
@@ -77,7 +78,7 @@ class SamplerParams(ct.Structure):
         ("std_dev", _f * MAX_CONTROL), ("control_cost_coeff", _f * MAX_CONTROL),
         ("pure_noise_trajectories_percentage", _f), ("std_dev_decay", _f),
         ("exponents", _f * MAX_CONTROL), ("offset_decay_rate", _f), ("fmin", _f),
-        ("fft_len_mode", _i), ("_pad", _i), ("seed", ct.c_uint64),
+        ("fft_len_mode", _i), ("_pad", _i), ("seed", ct.c_uint64), ("std_dev_t", ct.c_void_p),
     ]
 
 
@@ -102,6 +103,7 @@ class ControllerDesc(ct.Structure):
         ("num_iters", _i), ("use_graph", _i), ("keep_samples", _i),
         ("slide_control_scale", _f * MAX_CONTROL),
         ("init_control", ct.c_void_p), ("cost_map_host", ct.c_void_p), ("nn_theta_host", ct.c_void_p),
+        ("std_dev_t_host", ct.c_void_p), ("goal_traj_host", ct.c_void_p),
     ]
 
 
@@ -110,6 +112,7 @@ class InstanceDesc(ct.Structure):
         ("dynamics", DynamicsParams), ("cost", CostParams), ("sampler", SamplerParams),
         ("num_rollouts", _i), ("num_timesteps", _i), ("dt", _f), ("lambda_", _f), ("alpha", _f),
         ("slide_control_scale", _f * MAX_CONTROL), ("init_control", ct.c_void_p),
+        ("std_dev_t_host", ct.c_void_p), ("goal_traj_host", ct.c_void_p),
     ]
 
 
@@ -131,6 +134,7 @@ _SIGS = {
     "mppi_host_step": ([ct.POINTER(DynamicsParams), P, P, _f, P, P, P], ct.c_int),
     "mppi_host_enforce_constraints": ([ct.POINTER(DynamicsParams), P], ct.c_int),
     "mppi_host_running_cost": ([ct.POINTER(CostParams), P, ct.POINTER(_f)], ct.c_int),
+    "mppi_host_running_cost_at": ([ct.POINTER(CostParams), P, _i, ct.POINTER(_f)], ct.c_int),
     "mppi_generate_normals": ([ct.c_uint64, ct.c_uint64, P, ct.c_int64, P], ct.c_int),
     "mppi_sample_gaussian": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),
     "mppi_sample_colored": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),

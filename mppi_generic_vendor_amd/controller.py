@@ -10,6 +10,31 @@ from . import capi
 from .workloads import Problem
 
 
+def _host_arrays(d, problem: Problem) -> list:
+    """Point the descriptor's host-copy fields at the problem's host arrays (cost map, network parameters, per-step
+    standard deviations, goal trajectory); returns the arrays, which must stay alive across the C call."""
+    keep = []
+
+    def host(name, a, shape=None):
+        arr = np.ascontiguousarray(a, dtype=np.float32)
+        if shape is not None:
+            arr = arr.reshape(shape)
+        keep.append(arr)
+        setattr(d, name, arr.ctypes.data)
+        return arr
+
+    if problem.cost_map is not None:
+        host("cost_map_host", problem.cost_map)
+    if problem.nn_theta is not None:
+        host("nn_theta_host", problem.nn_theta)
+    if problem.std_dev_t is not None:
+        host("std_dev_t_host", problem.std_dev_t, (problem.num_timesteps, problem.C))
+    if problem.goal_traj is not None:
+        g = host("goal_traj_host", problem.goal_traj, (-1, problem.dyn.output_dim))
+        d.cost.goal_traj_len = g.shape[0]
+    return keep
+
+
 class VanillaMPPIController:
     def __init__(self, problem: Problem, num_iters: int = 1, use_graph: bool = True,
                  init_control: np.ndarray | None = None, slide_control_scale=None, keep_samples: bool = False):
@@ -34,19 +59,11 @@ class VanillaMPPIController:
         if slide_control_scale is not None:
             for i, v in enumerate(slide_control_scale):
                 d.slide_control_scale[i] = v
-        self._keep = []
+        self._keep = _host_arrays(d, problem)
         if init_control is not None:
             ic = np.ascontiguousarray(init_control, dtype=np.float32).reshape(self.T, self.C)
             self._keep.append(ic)
             d.init_control = ic.ctypes.data
-        if problem.cost_map is not None:
-            m = np.ascontiguousarray(problem.cost_map, dtype=np.float32)
-            self._keep.append(m)
-            d.cost_map_host = m.ctypes.data
-        if problem.nn_theta is not None:
-            th = np.ascontiguousarray(problem.nn_theta, dtype=np.float32)
-            self._keep.append(th)
-            d.nn_theta_host = th.ctypes.data
         self._num_iters, self._use_graph, self._keep_samples = num_iters, d.use_graph, d.keep_samples
         self._slide = [d.slide_control_scale[i] for i in range(capi.MAX_CONTROL)]
         h = ct.c_void_p()
@@ -132,15 +149,7 @@ class VanillaMPPIController:
         d.keep_samples = self._keep_samples
         for i in range(capi.MAX_CONTROL):
             d.slide_control_scale[i] = self._slide[i]
-        keep = []
-        if problem.cost_map is not None:
-            m = np.ascontiguousarray(problem.cost_map, dtype=np.float32)
-            keep.append(m)
-            d.cost_map_host = m.ctypes.data
-        if problem.nn_theta is not None:
-            th = np.ascontiguousarray(problem.nn_theta, dtype=np.float32)
-            keep.append(th)
-            d.nn_theta_host = th.ctypes.data
+        keep = _host_arrays(d, problem)  # noqa: F841 (alive across the call)
         capi.call("mppi_controller_set_params", self._h, ct.byref(d))
         self.problem = problem
 

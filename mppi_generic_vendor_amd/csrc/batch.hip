@@ -107,8 +107,8 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
         const float m = mu[loc];
         float v;
         if (kk == 0 || t < a.stride) v = m;
-        else if (kk >= I.pstart) v = decay * sp.std_dev[c] * z.z[jj];
-        else v = m + decay * sp.std_dev[c] * z.z[jj];
+        else if (kk >= I.pstart) v = decay * step_std_dev(sp, t, c, C) * z.z[jj];
+        else v = m + decay * step_std_dev(sp, t, c, C) * z.z[jj];
         slot[((t - t0) * R + r) * C + c] = v;
       }
     }
@@ -187,7 +187,7 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
             // addresses instead of register selects
             const float* yy = yb + ((t - t0) * R + r) * O;
             const float* u = rs + ((t - t0) * R + r) * C;
-            acc[s] += CF::running(I.cost, yy) + lr_cost<C>(sp, u, mu + t * C, pure[s], I.lambda, I.alpha);
+            acc[s] += CF::running(I.cost, yy, t) + lr_cost<C>(sp, u, mu + t * C, pure[s], I.lambda, I.alpha, t);
           }
         }
         // clamped samples of the chunk to the tile's [T][R][C] region (contiguous, 16-byte stores)
@@ -440,6 +440,7 @@ struct mppi_batch {
   int TCmax = 0, Cmax = 1, Omax = 1, cus = 0;
   hipStream_t stream = nullptr;
   float* map_d = nullptr;
+  float* tables_d = nullptr;  // the instances' per-step standard deviations and goal trajectories, back to back
   BatchInstance* inst_d = nullptr;
   std::vector<BatchInstance> inst_h;
   int2* work_d = nullptr;
@@ -500,7 +501,7 @@ static int batch_free(mppi_batch* b)
 {
   if (!b) return MPPI_OK;
   batch_drop_graphs(b);
-  for (void* p : {(void*)b->map_d, (void*)b->inst_d, (void*)b->work_d, (void*)b->queue_d, (void*)b->iters_d,
+  for (void* p : {(void*)b->map_d, (void*)b->tables_d, (void*)b->inst_d, (void*)b->work_d, (void*)b->queue_d, (void*)b->iters_d,
                   (void*)b->loop_iters_d, (void*)b->x0_d, (void*)b->means_d, (void*)b->means_backup_d,
                   (void*)b->stats_d, (void*)b->ubuf_d, (void*)b->parts_d, (void*)b->sums_d, (void*)b->costs_d})
     if (p) (void)hipFree(p);
@@ -598,6 +599,12 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
     if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy"));
   }
   b->inst_h.resize(n);
+  struct Table {
+    int inst, kind;  // kind 0: per-step std_dev [T][C], 1: goal trajectory [H][O]
+    const float* host;
+    size_t count;
+  };
+  std::vector<Table> tables;
   int64_t ubuf = 0, sums = 0, costs = 0;
   int parts = 0;
   for (int i = 0; i < n; ++i) {
@@ -616,6 +623,8 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
       d.cost.map_height = map_h;
       d.cost.map_width = map_w;
     }
+    // host tables stand in for their device copies while the blocks are validated
+    if (d.goal_traj_host != nullptr) d.cost.goal_traj = d.goal_traj_host;
     int st = validate_sizes(d.num_rollouts, d.num_timesteps);
     if (st == MPPI_OK) st = validate_dynamics(d.dynamics);
     if (st == MPPI_OK) st = validate_cost(d.cost, d.dynamics, true);
@@ -656,6 +665,13 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
     sums += (int64_t)t64 * T * C;
     ubuf += (int64_t)((K + 255) / 256) * 256 * T * C;
     costs += K;
+    if (d.std_dev_t_host != nullptr) {
+      for (int j = 0; j < T * C; ++j)
+        if (!(d.std_dev_t_host[j] > 0.0f))
+          return bail(fail(MPPI_E_INVALID, "batch: instance " + std::to_string(i) + ": per-step std_dev must be > 0"));
+      tables.push_back({i, 0, d.std_dev_t_host, (size_t)T * C});
+    }
+    if (d.goal_traj_host != nullptr) tables.push_back({i, 1, d.goal_traj_host, (size_t)d.cost.goal_traj_len * O});
     b->control.emplace_back((size_t)T * C, 0.0f);
     if (d.init_control) std::copy(d.init_control, d.init_control + (size_t)T * C, b->control.back().begin());
     b->raw_mean.emplace_back((size_t)T * C, 0.0f);
@@ -693,7 +709,19 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
   A(host_alloc(&b->nominal_h, sizeof(float) * b->total_mean));
   A(host_alloc(&b->mean_out_h, sizeof(float) * b->total_mean));
   A(host_alloc(&b->stats_h, sizeof(mppi_weight_stats) * n));
+  size_t table_floats = 0;
+  for (const Table& t : tables) table_floats += t.count;
+  if (table_floats > 0) A(dev_alloc(&b->tables_d, sizeof(float) * table_floats));
   if (st != MPPI_OK) return bail(st);
+  size_t off = 0;
+  for (const Table& t : tables) {
+    e = hipMemcpy(b->tables_d + off, t.host, sizeof(float) * t.count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy"));
+    if (t.kind == 0) b->inst_h[t.inst].sp.std_dev_t = b->desc[t.inst].sampler.std_dev_t = b->tables_d + off;
+    else b->inst_h[t.inst].cost.goal_traj = b->desc[t.inst].cost.goal_traj = b->tables_d + off;
+    off += t.count;
+  }
+  for (mppi_instance_desc& d : b->desc) d.std_dev_t_host = d.goal_traj_host = nullptr;
   e = hipMemset(b->queue_d, 0, sizeof(int));
   if (e == hipSuccess) e = hipMemset(b->x0_d, 0, sizeof(float) * BATCH_MAX_STATE * n);
   if (e == hipSuccess) e = hipMemset(b->means_d, 0, sizeof(float) * b->total_mean);

@@ -116,6 +116,16 @@ int validate_cost(const mppi_cost_params& c, const mppi_dynamics_params& d, bool
     if (c.speed_coeff != 0.0f && (c.speed_index < 0 || c.speed_index >= c.output_dim))
       return fail(MPPI_E_INVALID, "speed term: speed_index out of range");
   }
+  return validate_goal_traj(c);
+}
+
+int validate_goal_traj(const mppi_cost_params& c)
+{
+  if (c.goal_traj == nullptr) return MPPI_OK;
+  if (c.cost != MPPI_COST_QUADRATIC_MAP)
+    return fail(MPPI_E_UNSUPPORTED, "goal trajectory: only the quadratic cost takes one");
+  if (c.goal_traj_len < 1) return fail(MPPI_E_INVALID, "goal trajectory: goal_traj_len must be >= 1");
+  if (c.goal_time < 0) return fail(MPPI_E_INVALID, "goal trajectory: goal_time must be >= 0");
   return MPPI_OK;
 }
 
@@ -253,6 +263,9 @@ struct mppi_controller {
   IterArgs* iter_d = nullptr;
   float* map_d = nullptr;
   float* nn_d = nullptr;
+  float* std_t_d = nullptr;  // [T][C] copy of the descriptor's per-step standard deviations
+  float* goal_d = nullptr;   // [goal_len][O] copy of the descriptor's goal trajectory
+  int goal_len = 0;
   // per-call exchange buffers: mapped fine-grained host memory (the *_hd members are their device-side addresses),
   // moved by copy_words launches on the controller's stream
   IterArgs* iter_h = nullptr;
@@ -468,22 +481,29 @@ int mppi_host_step(const mppi_dynamics_params* p, const float* x, const float* u
   return MPPI_OK;
 }
 
-int mppi_host_running_cost(const mppi_cost_params* c, const float* y, float* out)
+int mppi_host_running_cost_at(const mppi_cost_params* c, const float* y, int32_t t, float* out)
 {
   if (!c || !y || !out) return fail(MPPI_E_INVALID, "null argument");
+  if (t < 0) return fail(MPPI_E_INVALID, "running cost: step index must be >= 0");
   if (c->cost == MPPI_COST_QUADRATIC_MAP) {
     if (c->map == nullptr && c->map_coeff != 0.0f) return fail(MPPI_E_INVALID, "running cost: map is null");
+    MPPI_CHECK(validate_goal_traj(*c));
     switch (c->output_dim) {
-      case 4: *out = Cost<MPPI_COST_QUADRATIC_MAP, 4>::running(*c, y); return MPPI_OK;
-      case 7: *out = Cost<MPPI_COST_QUADRATIC_MAP, 7>::running(*c, y); return MPPI_OK;
+      case 4: *out = Cost<MPPI_COST_QUADRATIC_MAP, 4>::running(*c, y, t); return MPPI_OK;
+      case 7: *out = Cost<MPPI_COST_QUADRATIC_MAP, 7>::running(*c, y, t); return MPPI_OK;
       default: return fail(MPPI_E_UNSUPPORTED, "running cost: output_dim");
     }
   }
   if (c->cost == MPPI_COST_CARTPOLE_QUADRATIC && c->output_dim == 4) {
-    *out = Cost<MPPI_COST_CARTPOLE_QUADRATIC, 4>::running(*c, y);
+    *out = Cost<MPPI_COST_CARTPOLE_QUADRATIC, 4>::running(*c, y, t);
     return MPPI_OK;
   }
   return fail(MPPI_E_UNSUPPORTED, "running cost: unknown cost id");
+}
+
+int mppi_host_running_cost(const mppi_cost_params* c, const float* y, float* out)
+{
+  return mppi_host_running_cost_at(c, y, 0, out);
 }
 
 int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u)
@@ -634,7 +654,7 @@ static int controller_free(mppi_controller* c)
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
                  c->map_d, c->nn_d, c->loop_iter_d, c->prop_dmu_d, c->prop_sig_d, c->sigma_base_d,
-                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d};
+                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d, c->std_t_d, c->goal_d};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h};
@@ -642,6 +662,38 @@ static int controller_free(mppi_controller* c)
     if (p) (void)hipHostFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return MPPI_OK;
+}
+
+// Controller-owned device copies of the descriptor's host tables -- per-step standard deviations [T][C] and the goal
+// trajectory [goal_traj_len][O] -- pointed to from the parameter blocks of `n`. Without a host table the block's own
+// pointer stays (a caller's device table, or NULL for none).
+static int upload_tables(mppi_controller* c, const mppi_controller_desc* desc, mppi_controller_desc& n)
+{
+  if (desc->std_dev_t_host != nullptr) {
+    const size_t cnt = (size_t)c->T * c->C;
+    for (size_t i = 0; i < cnt; ++i)
+      if (!(desc->std_dev_t_host[i] > 0.0f)) return fail(MPPI_E_INVALID, "per-step std_dev must be > 0");
+    if (c->std_t_d == nullptr) MPPI_CHECK(dev_alloc(&c->std_t_d, sizeof(float) * cnt));
+    MPPI_HIP_CHECK(hipMemcpy(c->std_t_d, desc->std_dev_t_host, sizeof(float) * cnt, hipMemcpyHostToDevice));
+    n.sampler.std_dev_t = c->std_t_d;
+  }
+  if (desc->goal_traj_host != nullptr && !c->has_plugin) {
+    const int H = desc->cost.goal_traj_len;
+    if (H < 1) return fail(MPPI_E_INVALID, "goal trajectory: goal_traj_len must be >= 1");
+    const size_t cnt = (size_t)H * c->O;
+    if (c->goal_d == nullptr || H != c->goal_len) {
+      if (c->goal_d) MPPI_HIP_CHECK(hipFree(c->goal_d));
+      if (c->d.cost.goal_traj == c->goal_d) c->d.cost.goal_traj = nullptr;
+      c->goal_d = nullptr;
+      MPPI_CHECK(dev_alloc(&c->goal_d, sizeof(float) * cnt));
+      c->goal_len = H;
+    }
+    MPPI_HIP_CHECK(hipMemcpy(c->goal_d, desc->goal_traj_host, sizeof(float) * cnt, hipMemcpyHostToDevice));
+    n.cost.goal_traj = c->goal_d;
+  }
+  n.std_dev_t_host = nullptr;
+  n.goal_traj_host = nullptr;
   return MPPI_OK;
 }
 
@@ -697,6 +749,7 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
     MPPI_HIP_CHECK(hipMemcpy(c->map_d, desc->cost_map_host, sizeof(float) * n, hipMemcpyHostToDevice));
     c->d.cost.map = c->map_d;
   }
+  MPPI_CHECK(upload_tables(c, desc, c->d));
   if (!c->has_plugin) MPPI_CHECK(validate_cost(c->d.cost, c->d.dynamics, true));
   if (!c->has_plugin && desc->dynamics.model == MPPI_MODEL_RACER_DUBINS_NN) {
     if (desc->nn_theta_host == nullptr) return fail(MPPI_E_INVALID, "residual MLP: nn_theta_host is required");
@@ -981,6 +1034,8 @@ int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, in
   if (pd.S != c->S || pd.C != c->C || pd.T != c->T)
     return fail(MPPI_E_INVALID, "proposal: state/control/horizon dims do not match the controller");
   if (c->d.sampler.kind != MPPI_SAMPLER_GAUSSIAN) return fail(MPPI_E_UNSUPPORTED, "proposal: Gaussian sampler only");
+  if (c->d.sampler.std_dev_t != nullptr)
+    return fail(MPPI_E_UNSUPPORTED, "proposal: the base sampler has per-step std_dev (uniform std_dev only)");
   if (num_rollouts < 2 || num_rollouts > c->K)
     return fail(MPPI_E_INVALID, "proposal: num_rollouts must be in [2, controller num_rollouts]");
   const size_t tc = sizeof(float) * c->T * c->C;
@@ -1241,27 +1296,44 @@ int mppi_controller_set_params(mppi_controller* c, const mppi_controller_desc* d
     MPPI_CHECK(validate_dynamics(n.dynamics));
   }
   MPPI_CHECK(validate_sampler(n.sampler, n.dynamics));
-  // queued replays read the old parameters: wait for them, then update the device-side copies
+  // everything is checked before any device copy changes, so that a refused call leaves the controller as it was:
+  // the blocks are validated with the host tables standing in for the device copies they will become
+  const size_t nm = (size_t)n.cost.map_width * n.cost.map_height;
+  if (!c->has_plugin) {
+    if (desc->cost_map_host != nullptr && nm < 4) return fail(MPPI_E_INVALID, "cost map: width/height must be >= 2");
+    if (n.dynamics.model == MPPI_MODEL_RACER_DUBINS_NN && desc->nn_theta_host != nullptr &&
+        n.dynamics.nn_num_params != (int)c->nn_host.size())
+      return fail(MPPI_E_INVALID, "set_params: residual MLP parameter count changed");
+    mppi_cost_params probe = n.cost;
+    // the controller's map copy stays in use unless the caller passes a device map of its own
+    if (desc->cost_map_host != nullptr) probe.map = desc->cost_map_host;
+    else if (c->map_d != nullptr && n.cost.map == nullptr) probe.map = c->map_d;
+    if (desc->goal_traj_host != nullptr) probe.goal_traj = desc->goal_traj_host;
+    MPPI_CHECK(validate_cost(probe, n.dynamics, true));
+  }
+  if (desc->std_dev_t_host != nullptr)
+    for (size_t i = 0; i < (size_t)c->T * c->C; ++i)
+      if (!(desc->std_dev_t_host[i] > 0.0f)) return fail(MPPI_E_INVALID, "per-step std_dev must be > 0");
+  if (c->prop != nullptr && (desc->std_dev_t_host != nullptr || n.sampler.std_dev_t != nullptr))
+    return fail(MPPI_E_UNSUPPORTED, "set_params: per-step std_dev with a learned proposal attached");
+  // queued replays read the old parameters: wait for them; every captured graph holds the old parameter blocks as
+  // kernel arguments (and may point at copies replaced below)
   MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   if (!c->has_plugin) {
     if (desc->cost_map_host != nullptr) {
-      const size_t nm = (size_t)n.cost.map_width * n.cost.map_height;
-      if (nm < 4) return fail(MPPI_E_INVALID, "cost map: width/height must be >= 2");
       const size_t old = (size_t)o.cost.map_width * o.cost.map_height;
       if (c->map_d == nullptr || nm != old) {
         if (c->map_d) MPPI_HIP_CHECK(hipFree(c->map_d));
         c->map_d = nullptr;
+        c->d.cost.map = nullptr;
         MPPI_CHECK(dev_alloc(&c->map_d, sizeof(float) * nm));
       }
       MPPI_HIP_CHECK(hipMemcpy(c->map_d, desc->cost_map_host, sizeof(float) * nm, hipMemcpyHostToDevice));
     }
-    // the controller's map copy stays in use unless the caller passes a device map of its own
     if (c->map_d != nullptr && (desc->cost_map_host != nullptr || n.cost.map == nullptr)) n.cost.map = c->map_d;
-    MPPI_CHECK(validate_cost(n.cost, n.dynamics, true));
     if (n.dynamics.model == MPPI_MODEL_RACER_DUBINS_NN) {
       if (desc->nn_theta_host != nullptr) {
-        if (n.dynamics.nn_num_params != (int)c->nn_host.size())
-          return fail(MPPI_E_INVALID, "set_params: residual MLP parameter count changed");
         c->nn_host.assign(desc->nn_theta_host, desc->nn_theta_host + c->nn_host.size());
         MPPI_HIP_CHECK(hipMemcpy(c->nn_d, c->nn_host.data(), sizeof(float) * c->nn_host.size(),
                                  hipMemcpyHostToDevice));
@@ -1269,12 +1341,11 @@ int mppi_controller_set_params(mppi_controller* c, const mppi_controller_desc* d
       n.dynamics.nn_theta = c->nn_d;
     }
   }
+  MPPI_CHECK(upload_tables(c, desc, n));
   n.init_control = nullptr;
   n.cost_map_host = nullptr;
   n.nn_theta_host = nullptr;
   const bool reseed = n.sampler.seed != o.sampler.seed;
-  // every captured graph holds the old parameter blocks as kernel arguments
-  drop_graphs(c);
   c->d = n;
   if (reseed) {
     c->rng_counter = 0;

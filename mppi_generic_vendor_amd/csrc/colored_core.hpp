@@ -334,7 +334,11 @@ __device__ __forceinline__ ColoredTile colored_stage(const mppi_sampler_params& 
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int t = m * 16 + lk * 4 + v;
-        if (valid && t < T) srow[t * C] = sd * ((acc[q][v] - dpow[t] * xv) * inv);
+        // per-step standard deviations scale each step's noise (setGaussianControls after the colored draw,
+        // colored_noise.cu:367-386 with time_specific_std_dev)
+        const float sdt = sp.std_dev_t ? powf(sp.std_dev_decay, (float)a.iteration) * sp.std_dev_t[min(t, T - 1) * C + c]
+                                       : sd;
+        if (valid && t < T) srow[t * C] = sdt * ((acc[q][v] - dpow[t] * xv) * inv);
       }
     }
   }

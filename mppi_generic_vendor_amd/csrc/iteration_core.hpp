@@ -21,8 +21,9 @@ __device__ __forceinline__ IterArgs load_iter_args(const IterRef& r) { return r.
 // Rows k0 .. k0 + nrows - 1 of the Gaussian samples into tile[r ROW + col] (col = t C + c):
 //   u = mu_t                         if k == 0 or t < stride
 //   u = mu_t + dmu_t + sig_t eps     with a proposal (prop_l: dmu [TC4], sig [TC4] in LDS)
-//   u = s_c eps                      if k >= pstart      (s_c = std_dev_c decay^iteration)
-//   u = mu_t + s_c eps               otherwise,
+//   u = s_tc eps                     if k >= pstart      (s_tc = std_dev_c decay^iteration, or the per-step
+//                                                         std_dev_t[t][c] decay^iteration)
+//   u = mu_t + s_tc eps              otherwise,
 // then f(col, u) (the constraint). eps is normal e of the stream (seed, rng_offset) for element e = k T C + col, the
 // numbering of mppi_sample_gaussian: the tile is a contiguous span starting at element k0 T C (a multiple of 4), so
 // Philox block b covers tile elements 4 (b - base) .. 4 (b - base) + 3.
@@ -49,8 +50,8 @@ __device__ __forceinline__ void gaussian_tile_fill(const mppi_sampler_params& sp
         float v;
         if (k == 0 || t < a.stride) v = m;
         else if (PROP) v = m + prop_l[col] + prop_l[TC4 + col] * z.z[j];
-        else if (k >= pstart) v = decay * sp.std_dev[c] * z.z[j];
-        else v = m + decay * sp.std_dev[c] * z.z[j];
+        else if (k >= pstart) v = decay * step_std_dev(sp, t, c, C) * z.z[j];
+        else v = m + decay * step_std_dev(sp, t, c, C) * z.z[j];
         tile[r * ROW + col] = f(col, v);
       }
       ++col;

@@ -84,9 +84,10 @@ __global__ void __launch_bounds__(256) sample_gaussian_kernel(mppi_sampler_param
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const float mu = mean[t * C + c];
+      const float s = sp.std_dev_t ? decay * sp.std_dev_t[t * C + c] : sd[c];
       if (k == 0 || (int)t < a.stride) v[j] = mu;
-      else if ((int)k >= pstart) v[j] = sd[c] * eps[j];
-      else v[j] = mu + sd[c] * eps[j];
+      else if ((int)k >= pstart) v[j] = s * eps[j];
+      else v[j] = mu + s * eps[j];
       if (++c == (uint32_t)C) {
         c = 0;
         if (++t == (uint32_t)T) {

@@ -347,45 +347,46 @@ MPPI_HD float quad_state(const mppi_cost_params& c, const float* y)
   return s;
 }
 
+// The same term against the goal of step t: row min(goal_time + t, goal_traj_len - 1) of the goal trajectory
+// (QuadraticCostTrajectoryParams::getIndex, cost_functions/quadratic_cost/quadratic_cost.cuh:48-57), or the single
+// goal without one. t is uniform across the lanes that evaluate one step, so the row is read with scalar loads.
+template <int O>
+MPPI_HD float quad_state_at(const mppi_cost_params& c, const float* y, int t)
+{
+  if (c.goal_traj == nullptr) return quad_state<O>(c, y);
+  const int row = c.goal_time + t < c.goal_traj_len ? c.goal_time + t : c.goal_traj_len - 1;
+  const float* g = c.goal_traj + (size_t)row * O;
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < O; ++i) {
+    const float e = y[i] - g[i];
+    s += e * e * c.coeff[i];
+  }
+  return s;
+}
+
 template <int COST, int O>
 struct Cost;
 
 // cost_functions/cartpole/cartpole_quadratic_cost.cu:23-55
 template <int O>
 struct Cost<MPPI_COST_CARTPOLE_QUADRATIC, O> {
-  MPPI_HD static bool has_map(const mppi_cost_params&) { return false; }
-  MPPI_HD static float running_nomap(const mppi_cost_params& c, const float* y)
-  {
-    return quad_state<O>(c, y);
-  }
-  MPPI_HD static float running(const mppi_cost_params& c, const float* y) { return quad_state<O>(c, y); }
+  MPPI_HD static float running(const mppi_cost_params& c, const float* y, int) { return quad_state<O>(c, y); }
   MPPI_HD static float terminal(const mppi_cost_params& c, const float* y)
   {
     return quad_state<O>(c, y) * c.terminal_cost_coeff;
   }
 };
 
-// Quadratic goal term (cost_functions/quadratic_cost/quadratic_cost.cu:35-54) + cost-map term + speed term
-// (cost_functions/autorally/ar_standard_cost.cu:284-297); terminal = terminal_cost_coeff * quadratic term.
+// Quadratic goal term of step t (cost_functions/quadratic_cost/quadratic_cost.cu:35-54, the goal trajectory's row
+// for t when one is given) + cost-map term + speed term (cost_functions/autorally/ar_standard_cost.cu:284-297);
+// terminal = terminal_cost_coeff * quadratic term against the single goal (0 by default, as the reference's
+// QuadraticCost::terminalCost).
 template <int O>
 struct Cost<MPPI_COST_QUADRATIC_MAP, O> {
-  MPPI_HD static bool has_map(const mppi_cost_params& c)
+  MPPI_HD static float running(const mppi_cost_params& c, const float* y, int t)
   {
-    return c.map != nullptr && c.map_coeff != 0.0f;
-  }
-  // every term but the cost map (the map term is then added by a separate gather pass)
-  MPPI_HD static float running_nomap(const mppi_cost_params& c, const float* y)
-  {
-    float s = quad_state<O>(c, y);
-    if (c.speed_coeff != 0.0f) {
-      const float e = y[c.speed_index] - c.speed_target;
-      s += c.speed_coeff * e * e;
-    }
-    return s;
-  }
-  MPPI_HD static float running(const mppi_cost_params& c, const float* y)
-  {
-    float s = quad_state<O>(c, y);
+    float s = quad_state_at<O>(c, y, t);
     if (c.map != nullptr && c.map_coeff != 0.0f) s += c.map_coeff * map_query(c, y[c.map_x_index], y[c.map_y_index]);
     if (c.speed_coeff != 0.0f) {
       const float e = y[c.speed_index] - c.speed_target;
@@ -399,20 +400,37 @@ struct Cost<MPPI_COST_QUADRATIC_MAP, O> {
   }
 };
 
-// Likelihood-ratio control cost (sampling_distributions/gaussian/gaussian.cu:491-574,650-675):
-// 0.5 * lambda * (1 - alpha) * sum_c coeff_c * mu_c * (mu_c - 2 u_c) / sigma_c^2, mu = 0 for pure-noise samples.
+// Likelihood-ratio control cost of step t (sampling_distributions/gaussian/gaussian.cu:491-574,650-675):
+// 0.5 * lambda * (1 - alpha) * sum_c coeff_c * mu_c * (mu_c - 2 u_c) / sigma_tc^2, mu = 0 for pure-noise samples;
+// sigma_tc = std_dev_t[t C + c] with per-step standard deviations (gaussian.cu:665-668), std_dev[c] otherwise.
 template <int C>
 __device__ __forceinline__ float lr_cost(const mppi_sampler_params& s, const float* u, const float* mu, bool pure,
-                                         float lambda, float alpha)
+                                         float lambda, float alpha, int t)
 {
   float cost = 0.0f;
+  if (s.std_dev_t == nullptr) {
 #pragma unroll
-  for (int i = 0; i < C; ++i) {
-    const float m = pure ? 0.0f : mu[i];
-    // coeff / sigma^2 is uniform and hoisted out of the horizon loop
-    cost += m * (m - 2.0f * u[i]) * (s.control_cost_coeff[i] / (s.std_dev[i] * s.std_dev[i]));
+    for (int i = 0; i < C; ++i) {
+      const float m = pure ? 0.0f : mu[i];
+      // coeff / sigma^2 is uniform and hoisted out of the horizon loop
+      cost += m * (m - 2.0f * u[i]) * (s.control_cost_coeff[i] / (s.std_dev[i] * s.std_dev[i]));
+    }
+  } else {
+    // t is uniform across the lanes that evaluate one step: scalar loads of the step's row
+    const float* sd = s.std_dev_t + t * C;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      const float m = pure ? 0.0f : mu[i];
+      cost += m * (m - 2.0f * u[i]) * (s.control_cost_coeff[i] / (sd[i] * sd[i]));
+    }
   }
   return 0.5f * lambda * (1.0f - alpha) * cost;
+}
+
+// Standard deviation of control c at step t before the iteration's decay: the per-step table or std_dev.
+MPPI_HD float step_std_dev(const mppi_sampler_params& s, int t, int c, int C)
+{
+  return s.std_dev_t ? s.std_dev_t[t * C + c] : s.std_dev[c];
 }
 
 }  // namespace mppi_amd

@@ -113,9 +113,9 @@ struct BuiltinModel : Model<MODEL> {
 
 template <int COST, int O>
 struct BuiltinCost {
-  MPPI_HD static float running(const mppi_cost_params& c, const float* y, const float*, int)
+  MPPI_HD static float running(const mppi_cost_params& c, const float* y, const float*, int t)
   {
-    return Cost<COST, O>::running(c, y);
+    return Cost<COST, O>::running(c, y, t);
   }
   MPPI_HD static float terminal(const mppi_cost_params& c, const float* y) { return Cost<COST, O>::terminal(c, y); }
 };
@@ -237,8 +237,8 @@ __device__ __forceinline__ void rollout_tile_nn(const mppi_dynamics_params& dyn,
       const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
       const float* yb = ybuf + ((it - 1) & 1) * chunk * RC_LANES * (O | 1) + lane * (O | 1);
       for (int t = t0 + wave - NN_PRODUCERS; t < t1; t += NN_CONSUMERS) {
-        acc += CF::running(cost, yb + (t - t0) * RC_LANES * (O | 1)) +
-               lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
+        acc += CF::running(cost, yb + (t - t0) * RC_LANES * (O | 1), t) +
+               lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha, t);
         if (prop) acc_is += is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
       }
     }
@@ -367,8 +367,8 @@ __device__ __forceinline__ void rollout_tile_nn_split(const mppi_dynamics_params
       const int tb = t - t % NS_BLOCK;
       const int ts = tb + 4 * q + g;
       if (ts <= t) {
-        acc += CF::running(cost, hist + ((ts - tb) * NS_ROWS + n) * (O | 1)) +
-               lr_cost<C>(sp, row + ts * C, mu + ts * C, pure, lambda, alpha);
+        acc += CF::running(cost, hist + ((ts - tb) * NS_ROWS + n) * (O | 1), ts) +
+               lr_cost<C>(sp, row + ts * C, mu + ts * C, pure, lambda, alpha, ts);
         if (prop) acc_is += is_term<C>(*prop, row + ts * C, mu + ts * C, ts * C);
       }
     }
@@ -455,7 +455,7 @@ __device__ __forceinline__ void rollout_tile_t(const DP& dyn, const CP& cost, co
       for (int t = t0 + wave - 1; t < t1; t += RC_CONSUMERS) {
         // y and u are read in place from LDS (runtime-indexed cost terms become LDS addresses)
         acc += CF::running(cost, yb + (t - t0) * RC_LANES * (O | 1), crow + t * C, t) +
-               lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha);
+               lr_cost<C>(sp, crow + t * C, mu + t * C, pure, lambda, alpha, t);
         if (prop) acc_is += is_term<C>(*prop, crow + t * C, mu + t * C, t * C);
       }
     }

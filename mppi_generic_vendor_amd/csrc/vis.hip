@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(64) vis_rollout_kernel(mppi_dynamics_params dy
 #pragma unroll
     for (int i = 0; i < O; ++i) outputs[((size_t)j * T + t) * O + i] = x[i];
     step_costs[(size_t)j * T + t] =
-        (CF::running(cost, x) + lr_cost<C>(sp, u, mean + t * C, pure, lambda, alpha)) * invT;
+        (CF::running(cost, x, t) + lr_cost<C>(sp, u, mean + t * C, pure, lambda, alpha, t)) * invT;
   }
   terminal[j] = CF::terminal(cost, x) * invT;
 }

@@ -27,6 +27,10 @@ class Problem:
     x0: np.ndarray
     cost_map: np.ndarray | None = None
     nn_theta: np.ndarray | None = None
+    # per-step standard deviations [T][C] (time_specific_std_dev) and a goal trajectory [H][O] for the quadratic
+    # cost (QuadraticCostTrajectoryParams); host arrays, copied by the controller
+    std_dev_t: np.ndarray | None = None
+    goal_traj: np.ndarray | None = None
 
     @property
     def S(self) -> int:

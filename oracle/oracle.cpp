@@ -239,20 +239,29 @@ static R map_query(const mppi_cost_params& c, const float* map, R px, R py)
 }
 
 template <class R>
-static R quad_state(const mppi_cost_params& c, const R* y)
+static R quad_state(const mppi_cost_params& c, const R* y, const float* goal)
 {
   R s = 0;
   for (int i = 0; i < c.output_dim; ++i) {
-    const R e = y[i] - (R)c.goal[i];
+    const R e = y[i] - (R)goal[i];
     s += e * e * (R)c.coeff[i];
   }
   return s;
 }
 
-template <class R>
-static R running_cost(const mppi_cost_params& c, const float* map, const R* y)
+/* The goal of step t: row getIndex(t) = min(current_time + t, TIME_HORIZON - 1) of the goal trajectory
+ * (cost_functions/quadratic_cost/quadratic_cost.cuh:40-57; c.goal_traj is a host pointer here), else the goal. */
+static const float* goal_at(const mppi_cost_params& c, int t)
 {
-  R s = quad_state<R>(c, y);
+  if (c.goal_traj == nullptr) return c.goal;
+  const int row = std::min(c.goal_time + t, c.goal_traj_len - 1);
+  return c.goal_traj + (size_t)row * c.output_dim;
+}
+
+template <class R>
+static R running_cost(const mppi_cost_params& c, const float* map, const R* y, int t)
+{
+  R s = quad_state<R>(c, y, c.cost == MPPI_COST_QUADRATIC_MAP ? goal_at(c, t) : c.goal);
   if (c.cost == MPPI_COST_QUADRATIC_MAP) {
     if (map != nullptr && c.map_coeff != 0.0f) s += (R)c.map_coeff * map_query<R>(c, map, y[c.map_x_index], y[c.map_y_index]);
     if (c.speed_coeff != 0.0f) {
@@ -267,7 +276,7 @@ template <class R>
 static R terminal_cost(const mppi_cost_params& c, const R* y)
 {
   if (c.terminal_cost_coeff == 0.0f) return 0;
-  return quad_state<R>(c, y) * (R)c.terminal_cost_coeff;
+  return quad_state<R>(c, y, c.goal) * (R)c.terminal_cost_coeff;
 }
 
 /* The built-in models and costs as the rollout loop's system (rollout_loop.hpp). */
@@ -281,7 +290,7 @@ struct BuiltinSystem {
   int output_dim() const { return dyn.output_dim; }
   void enforce(R* u) const { enforce_constraints<R>(dyn, u); }
   void step(const R* x, R* xn, R* xd, const R* u, R* y, R dt) const { oracle::step<R>(dyn, x, xn, xd, u, y, dt); }
-  R running(const R* y, const R*, int) const { return running_cost<R>(cst, map, y); }
+  R running(const R* y, const R*, int t) const { return running_cost<R>(cst, map, y, t); }
   R terminal(const R* y) const { return terminal_cost<R>(cst, y); }
 };
 
@@ -344,7 +353,8 @@ void oracle_gaussian_transform(const mppi_sampler_params* s, int K, int T, int C
     for (int t = 0; t < T; ++t)
       for (int c = 0; c < C; ++c) {
         const size_t i = ((size_t)k * T + t) * C + c;
-        const float sd = decay * s->std_dev[c];
+        /* time_specific_std_dev: the step's row of the per-step table (gaussian.cu:38-43,313-318) */
+        const float sd = decay * (s->std_dev_t ? s->std_dev_t[t * C + c] : s->std_dev[c]);
         float v;
         if (k == 0 || t < stride) v = mean[t * C + c];
         else if (k >= pstart) v = sd * eps[i];
@@ -465,17 +475,17 @@ float oracle_map_query_f32(const mppi_cost_params* c, const float* map, float x,
   return map_query<float>(*c, map, x, y);
 }
 
-float oracle_running_cost_f32(const mppi_cost_params* c, const float* map, const float* y)
+float oracle_running_cost_f32(const mppi_cost_params* c, const float* map, const float* y, int t)
 {
-  return running_cost<float>(*c, map, y);
+  return running_cost<float>(*c, map, y, t);
 }
 
 float oracle_terminal_cost_f32(const mppi_cost_params* c, const float* y) { return terminal_cost<float>(*c, y); }
 
 float oracle_lr_cost_f32(const mppi_sampler_params* s, int C, const float* u, const float* mean_t, int pure,
-                         float lambda, float alpha)
+                         float lambda, float alpha, int t)
 {
-  return lr_cost<float>(*s, C, u, mean_t, pure != 0, lambda, alpha);
+  return lr_cost<float>(*s, C, u, mean_t, pure != 0, lambda, alpha, t);
 }
 
 /* Rollout of K samples; `samples` is updated in place with the clamped controls. map: host cost map or NULL.

@@ -45,9 +45,9 @@ _SIGS = {
     "oracle_adam": ([P, P, P, P, _i, _f, _f, _f, _f, _i], None),
     "oracle_is_cost": ([P, P, P, P, P, _i, _i, _i, _d, P], None),
     "oracle_map_query_f32": ([ct.POINTER(capi.CostParams), P, _f, _f], _f),
-    "oracle_running_cost_f32": ([ct.POINTER(capi.CostParams), P, P], _f),
+    "oracle_running_cost_f32": ([ct.POINTER(capi.CostParams), P, P, _i], _f),
     "oracle_terminal_cost_f32": ([ct.POINTER(capi.CostParams), P], _f),
-    "oracle_lr_cost_f32": ([ct.POINTER(capi.SamplerParams), _i, P, P, _i, _f, _f], _f),
+    "oracle_lr_cost_f32": ([ct.POINTER(capi.SamplerParams), _i, P, P, _i, _f, _f, _i], _f),
     "oracle_rollout_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
                             ct.POINTER(capi.SamplerParams), _i, _i, _f, _f, _f, P, P, P, P, _i], None),
     "oracle_rollout_f64": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
@@ -96,10 +96,11 @@ def normals(seed: int, offset: int, n: int) -> np.ndarray:
     return out
 
 
-def gaussian_transform(sp, K, T, C, mean, eps, stride, iteration) -> np.ndarray:
+def gaussian_transform(sp, K, T, C, mean, eps, stride, iteration, std_dev_t=None) -> np.ndarray:
     out = np.empty((K, T, C), dtype=np.float32)
     _a0 = np.ascontiguousarray(mean, np.float32)
     _a1 = np.ascontiguousarray(eps, np.float32)
+    sp = host_sampler(sp, std_dev_t)
     load().oracle_gaussian_transform(ct.byref(sp), K, T, C, ptr(_a0),
                                      ptr(_a1), stride, iteration, ptr(out))
     return out
@@ -113,11 +114,12 @@ def colored_scales(sp, T, C):
     return scales[: F.value], sigma, n.value, F.value
 
 
-def colored_samples(sp, K, T, C, mean, spectra, stride, iteration):
+def colored_samples(sp, K, T, C, mean, spectra, stride, iteration, std_dev_t=None):
     eps = np.empty((K, T, C), dtype=np.float32)
     out = np.empty((K, T, C), dtype=np.float32)
     _a0 = np.ascontiguousarray(mean, np.float32)
     _a1 = np.ascontiguousarray(spectra, np.float32)
+    sp = host_sampler(sp, std_dev_t)
     load().oracle_colored_samples(ct.byref(sp), K, T, C, ptr(_a0),
                                   ptr(_a1), stride, iteration, ptr(eps),
                                   ptr(out))
@@ -134,7 +136,7 @@ def rollout_f32(prob, samples: np.ndarray, mean: np.ndarray, nthreads: int = 1, 
     x0 = prob.x0 if x0 is None else x0
     _a0 = np.ascontiguousarray(x0, np.float32)
     _a1 = np.ascontiguousarray(mean, np.float32)
-    load().oracle_rollout_f32(ct.byref(dyn), ct.byref(prob.cost), ptr(m), ct.byref(prob.sampler), K, T, prob.dt,
+    load().oracle_rollout_f32(ct.byref(dyn), ct.byref(_host_cost(prob)), ptr(m), ct.byref(_host_smp(prob)), K, T, prob.dt,
                               prob.lambda_, prob.alpha, ptr(_a0),
                               ptr(_a1), ptr(s), ptr(costs), nthreads)
     return costs, s
@@ -148,7 +150,7 @@ def rollout_f64(prob, samples: np.ndarray, mean: np.ndarray, x0=None, nthreads: 
     x0 = prob.x0 if x0 is None else x0
     _a0 = np.ascontiguousarray(x0, np.float32)
     _a1 = np.ascontiguousarray(mean, np.float32)
-    load().oracle_rollout_f64(ct.byref(dyn), ct.byref(prob.cost), ptr(prob.cost_map), ct.byref(prob.sampler), K, T,
+    load().oracle_rollout_f64(ct.byref(dyn), ct.byref(_host_cost(prob)), ptr(prob.cost_map), ct.byref(_host_smp(prob)), K, T,
                               prob.dt, prob.lambda_, prob.alpha, ptr(_a0),
                               ptr(_a1), ptr(s), ptr(costs), nthreads)
     return costs, s
@@ -168,12 +170,43 @@ def rollout_trace(prob, samples: np.ndarray, mean: np.ndarray, x0=None, f64: boo
     _a0 = np.ascontiguousarray(x0, np.float32)
     _a1 = np.ascontiguousarray(mean, np.float32)
     fn = load().oracle_rollout_trace_f64 if f64 else load().oracle_rollout_trace_f32
-    fn(ct.byref(dyn), ct.byref(prob.cost), ptr(prob.cost_map), ct.byref(prob.sampler), K, T, prob.dt, prob.lambda_,
+    fn(ct.byref(dyn), ct.byref(_host_cost(prob)), ptr(prob.cost_map), ct.byref(_host_smp(prob)), K, T, prob.dt, prob.lambda_,
        prob.alpha, ptr(_a0), ptr(_a1), ptr(s), ptr(costs), ptr(outs), ptr(steps), ptr(term))
     return costs, outs, steps, term
 
 
 _keep = []
+
+
+def host_sampler(sp, std_dev_t=None):
+    """A copy of the sampler block whose std_dev_t points to host memory holding `std_dev_t` [T][C] (None: unset)."""
+    s = capi.SamplerParams.from_buffer_copy(sp)
+    if std_dev_t is None:
+        s.std_dev_t = None
+    else:
+        s._hold = np.ascontiguousarray(std_dev_t, dtype=np.float32)  # lives as long as the copy
+        s.std_dev_t = s._hold.ctypes.data
+    return s
+
+
+def host_cost(cost, goal_traj=None):
+    """A copy of the cost block whose goal_traj points to host memory holding `goal_traj` [H][O] (None: unset)."""
+    c = capi.CostParams.from_buffer_copy(cost)
+    if goal_traj is None:
+        c.goal_traj = None
+    else:
+        c._hold = np.ascontiguousarray(goal_traj, dtype=np.float32)
+        c.goal_traj = c._hold.ctypes.data
+        c.goal_traj_len = c._hold.shape[0]
+    return c
+
+
+def _host_smp(prob):
+    return host_sampler(prob.sampler, getattr(prob, "std_dev_t", None))
+
+
+def _host_cost(prob):
+    return host_cost(prob.cost, getattr(prob, "goal_traj", None))
 
 
 def _host_dyn(prob):

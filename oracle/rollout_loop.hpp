@@ -59,14 +59,16 @@ inline int pure_noise_start(int K, float pct)
   return k;
 }
 
-/* Likelihood-ratio control cost, gaussian.cu:650-675, mean = the mean the samples were drawn around. */
+/* Likelihood-ratio control cost of step t, gaussian.cu:650-675, mean = the mean the samples were drawn around; with
+ * time_specific_std_dev the step's standard deviations (:665-668; s.std_dev_t is a host pointer here). */
 template <class R>
-inline R lr_cost(const mppi_sampler_params& s, int C, const R* u, const float* mean_t, bool pure, R lambda, R alpha)
+inline R lr_cost(const mppi_sampler_params& s, int C, const R* u, const float* mean_t, bool pure, R lambda, R alpha,
+                 int t)
 {
   R cost = 0;
   for (int i = 0; i < C; ++i) {
     const R m = pure ? (R)0 : (R)mean_t[i];
-    const R sd = s.std_dev[i];
+    const R sd = s.std_dev_t ? s.std_dev_t[(size_t)t * C + i] : s.std_dev[i];
     cost += (R)s.control_cost_coeff[i] * m * (m - (R)2 * u[i]) / (sd * sd);
   }
   return (R)0.5 * lambda * ((R)1 - alpha) * cost;
@@ -100,7 +102,7 @@ void rollout_loop(const Sys& sys, const mppi_sampler_params& smp, int K, int T, 
       sys.enforce(u);
       for (int i = 0; i < C; ++i) ut[i] = (float)u[i];
       sys.step(x, xn, xd, u, y, dt);
-      const R c = sys.running(y, u, t) + lr_cost<R>(smp, C, u, mean + (size_t)t * C, k >= pstart, lambda, alpha);
+      const R c = sys.running(y, u, t) + lr_cost<R>(smp, C, u, mean + (size_t)t * C, k >= pstart, lambda, alpha, t);
       trace.step(k, t, y, sys.output_dim(), c / (R)T);
       run += c;
       for (int i = 0; i < S; ++i) x[i] = xn[i];

@@ -15,10 +15,12 @@ def banner(msg: str):
     sys.stdout.flush()
 
 
-def device_problem(prob):
-    """Copies of the problem's parameter blocks with device pointers for the cost map and the MLP parameters."""
+def device_problem(prob, with_sampler=False):
+    """Copies of the problem's parameter blocks with device pointers for the cost map, the MLP parameters and the
+    goal trajectory (and, with_sampler, a sampler copy pointing at the per-step standard deviations)."""
     dyn = capi.DynamicsParams.from_buffer_copy(prob.dyn)
     cost = capi.CostParams.from_buffer_copy(prob.cost)
+    smp = capi.SamplerParams.from_buffer_copy(prob.sampler)
     keep = []
     if prob.cost_map is not None:
         mb = DeviceBuffer.from_numpy(prob.cost_map.astype(np.float32))
@@ -28,6 +30,17 @@ def device_problem(prob):
         tb = DeviceBuffer.from_numpy(prob.nn_theta.astype(np.float32))
         keep.append(tb)
         dyn.nn_theta = tb.ptr
+    if prob.goal_traj is not None:
+        gb = DeviceBuffer.from_numpy(np.ascontiguousarray(prob.goal_traj, np.float32))
+        keep.append(gb)
+        cost.goal_traj = gb.ptr
+        cost.goal_traj_len = prob.goal_traj.shape[0]
+    if prob.std_dev_t is not None:
+        sb = DeviceBuffer.from_numpy(np.ascontiguousarray(prob.std_dev_t, np.float32))
+        keep.append(sb)
+        smp.std_dev_t = sb.ptr
+    if with_sampler:
+        return dyn, cost, smp, keep
     return dyn, cost, keep
 
 
@@ -38,7 +51,18 @@ def gpu_normals(seed: int, offset: int, n: int) -> np.ndarray:
     return b.to_numpy(n)
 
 
-def gpu_sample_gaussian(sp, K, T, C, mean, stride, iteration, rng_offset=0, eps=None):
+def _with_std_dev_t(sp, std_dev_t):
+    """A sampler copy whose std_dev_t points at a device copy of `std_dev_t` (None: the block as given)."""
+    if std_dev_t is None:
+        return sp, None
+    buf = DeviceBuffer.from_numpy(np.ascontiguousarray(std_dev_t, np.float32))
+    s = capi.SamplerParams.from_buffer_copy(sp)
+    s.std_dev_t = buf.ptr
+    return s, buf
+
+
+def gpu_sample_gaussian(sp, K, T, C, mean, stride, iteration, rng_offset=0, eps=None, std_dev_t=None):
+    sp, _sb = _with_std_dev_t(sp, std_dev_t)
     mb = DeviceBuffer.from_numpy(np.ascontiguousarray(mean, np.float32))
     out = DeviceBuffer(K * T * C * 4)
     eb = DeviceBuffer.from_numpy(np.ascontiguousarray(eps, np.float32)) if eps is not None else None
@@ -48,7 +72,8 @@ def gpu_sample_gaussian(sp, K, T, C, mean, stride, iteration, rng_offset=0, eps=
     return out.to_numpy((K, T, C))
 
 
-def gpu_sample_colored(sp, K, T, C, mean, stride, iteration, rng_offset=0, spectra=None):
+def gpu_sample_colored(sp, K, T, C, mean, stride, iteration, rng_offset=0, spectra=None, std_dev_t=None):
+    sp, _sb = _with_std_dev_t(sp, std_dev_t)
     mb = DeviceBuffer.from_numpy(np.ascontiguousarray(mean, np.float32))
     out = DeviceBuffer(K * T * C * 4)
     sb = DeviceBuffer.from_numpy(np.ascontiguousarray(spectra, np.float32)) if spectra is not None else None
@@ -60,13 +85,13 @@ def gpu_sample_colored(sp, K, T, C, mean, stride, iteration, rng_offset=0, spect
 
 def gpu_rollout(prob, samples, mean, x0=None):
     K, T, C = samples.shape
-    dyn, cost, keep = device_problem(prob)
+    dyn, cost, smp, keep = device_problem(prob, with_sampler=True)
     x0 = prob.x0 if x0 is None else x0
     xb = DeviceBuffer.from_numpy(np.ascontiguousarray(x0, np.float32))
     mb = DeviceBuffer.from_numpy(np.ascontiguousarray(mean, np.float32))
     sb = DeviceBuffer.from_numpy(np.ascontiguousarray(samples, np.float32))
     cb = DeviceBuffer(K * 4)
-    capi.call("mppi_rollout", ct.byref(dyn), ct.byref(cost), ct.byref(prob.sampler), K, T, prob.dt, prob.lambda_,
+    capi.call("mppi_rollout", ct.byref(dyn), ct.byref(cost), ct.byref(smp), K, T, prob.dt, prob.lambda_,
               prob.alpha, xb.ptr, mb.ptr, sb.ptr, cb.ptr, None)
     synchronize()
     return cb.to_numpy(K), sb.to_numpy((K, T, C))

@@ -379,7 +379,7 @@ def test_speed_term_closed_form(oracle):
     for lib in ("oracle", "engine"):
         def run(c, v):
             if lib == "oracle":
-                return oracle.oracle_running_cost_f32(ct.byref(c), None, v.ctypes.data)
+                return oracle.oracle_running_cost_f32(ct.byref(c), None, v.ctypes.data, 0)
             out = ct.c_float()
             capi.call("mppi_host_running_cost", ct.byref(c), v.ctypes.data, ct.byref(out))
             return out.value
@@ -476,8 +476,155 @@ def test_lr_cost_known_answers(oracle):
     mu = np.array([1.0, -1.0], np.float32)
     # 0.5 * lam * (1 - alpha) * [1*1*(1-2) + 0.5*(-1)*(-1-6)/4]
     want = 0.5 * 2.0 * (1 - 0.25) * (-1.0 + 0.5 * 7.0 / 4.0)
-    assert oracle.oracle_lr_cost_f32(ct.byref(sp), 2, u.ctypes.data, mu.ctypes.data, 0, 2.0, 0.25) == pytest.approx(want)
-    assert oracle.oracle_lr_cost_f32(ct.byref(sp), 2, u.ctypes.data, mu.ctypes.data, 1, 2.0, 0.25) == 0.0
+    assert oracle.oracle_lr_cost_f32(ct.byref(sp), 2, u.ctypes.data, mu.ctypes.data, 0, 2.0, 0.25, 0) == pytest.approx(want)
+    assert oracle.oracle_lr_cost_f32(ct.byref(sp), 2, u.ctypes.data, mu.ctypes.data, 1, 2.0, 0.25, 0) == 0.0
+
+
+def test_lr_cost_per_step_std_known_answers(oracle):
+    """time_specific_std_dev (gaussian.cu:665-668): step t divides by row t of the table."""
+    sp = capi.sampler_defaults(capi.SAMPLER_GAUSSIAN, 2)
+    sp.control_cost_coeff[0], sp.control_cost_coeff[1] = 1.0, 0.5
+    sp.std_dev[0], sp.std_dev[1] = 9.0, 9.0  # not used once the table is set
+    table = np.array([[1.0, 2.0], [2.0, 4.0], [0.5, 1.0]], np.float32)
+    hs = ol.host_sampler(sp, table)
+    u = np.array([1.0, 3.0], np.float32)
+    mu = np.array([1.0, -1.0], np.float32)
+    # t = 0: 0.5 * 2 * 0.75 * [1*1*(1-2)/1 + 0.5*(-1)*(-1-6)/4]; t = 1: [.../4 + .../16]; t = 2: [.../0.25 + .../1]
+    for t, (a, b) in enumerate(((1.0, 4.0), (4.0, 16.0), (0.25, 1.0))):
+        want = 0.5 * 2.0 * 0.75 * (-1.0 / a + 0.5 * 7.0 / b)
+        got = oracle.oracle_lr_cost_f32(ct.byref(hs), 2, u.ctypes.data, mu.ctypes.data, 0, 2.0, 0.25, t)
+        assert got == pytest.approx(want, rel=1e-6)
+
+
+def test_gaussian_transform_per_step_std():
+    """Per-step standard deviations: a table of identical rows equal to std_dev reproduces the uniform transform bit
+    for bit; rows scaled by powers of two scale the pure-noise samples of that step exactly and the exploration noise
+    of the other samples by the same factor."""
+    K, T, C = 200, 12, 2
+    sp = capi.sampler_defaults(capi.SAMPLER_GAUSSIAN, C)
+    sp.std_dev[0], sp.std_dev[1] = 0.5, 2.0
+    sp.std_dev_decay = 0.9
+    rng = np.random.default_rng(3)
+    mean = rng.normal(size=(T, C)).astype(np.float32)
+    eps = rng.normal(size=(K, T, C)).astype(np.float32)
+    uni = ol.gaussian_transform(sp, K, T, C, mean, eps, 2, 1)
+    same = np.tile(np.array([0.5, 2.0], np.float32), (T, 1))
+    assert np.array_equal(ol.gaussian_transform(sp, K, T, C, mean, eps, 2, 1, std_dev_t=same), uni)
+    scale = (2.0 ** rng.integers(-3, 4, size=(T, C))).astype(np.float32)
+    u = ol.gaussian_transform(sp, K, T, C, mean, eps, 2, 1, std_dev_t=same * scale)
+    pstart = capi.pure_noise_start(K, 0.01)
+    assert np.array_equal(u[pstart:, 2:], uni[pstart:, 2:] * scale[2:])  # exact: power-of-two scaling
+    assert np.array_equal(u[:, :2], uni[:, :2]) and np.array_equal(u[0], mean)
+    np.testing.assert_allclose((u[1:pstart, 2:] - mean[2:]) / scale[2:], uni[1:pstart, 2:] - mean[2:],
+                               rtol=1e-5, atol=1e-6)
+
+
+# QuadraticCostTrajectory known answers, tests/cost_functions/general_quadratic_costs.cu:19-39,41-94,98-120,183-199:
+# DoubleIntegrator outputs, y = 0, unit coefficients, the first five rows of the test's s_traj.
+QUAD_SINGLE_GOAL = [1.0, -5.0, 0.0, 0.5]
+QUAD_TRAJ = [[1, -5, 0, 0.5], [2, -5.5, 0, 1], [5, 5, 1, 10], [1, 1, 1, 1], [2, 3, 4, 5]]
+
+
+def test_goal_trajectory_known_answers(oracle, engine):
+    dyn = capi.dynamics_defaults(capi.MODEL_DOUBLE_INTEGRATOR)
+    base = capi.cost_defaults(capi.COST_QUADRATIC_MAP, dyn)
+    base.map_coeff = 0.0
+    for i in range(4):
+        base.coeff[i] = 1.0
+        base.goal[i] = QUAD_SINGLE_GOAL[i]
+    y = np.zeros(4, np.float32)
+
+    def run(lib, c, t):
+        if lib == "oracle":
+            return oracle.oracle_running_cost_f32(ct.byref(c), None, y.ctypes.data, t)
+        out = ct.c_float()
+        capi.call("mppi_host_running_cost_at", ct.byref(c), y.ctypes.data, t, ct.byref(out))
+        return out.value
+
+    for lib in ("oracle", "engine"):
+        assert run(lib, base, 0) == pytest.approx(26.25)  # SimpleStateCostCPU
+        assert run(lib, base, 10) == pytest.approx(26.25)  # LateStateCostCPU: a single goal holds at every step
+        w = capi.CostParams.from_buffer_copy(base)
+        for i, q in enumerate((10.0, 5.0, 7.0, 0.1)):
+            w.coeff[i] = q
+        assert run(lib, w, 0) == pytest.approx(10 + 5 * 5 * 5 + 0 + 0.025)  # WeightStateCostCPU
+        tr = ol.host_cost(base, np.array(QUAD_TRAJ, np.float32))
+        assert run(lib, tr, 2) == pytest.approx(25 + 25 + 1 + 100)  # MidTrajStateCostGPU
+        assert run(lib, tr, 10) == pytest.approx(4 + 9 + 16 + 25)  # getIndex clamps to the last row
+        tr.goal_time = 1
+        assert run(lib, tr, 1) == pytest.approx(151.0)  # current_time + t
+        tr.goal_time = 3
+        assert run(lib, tr, 0) == pytest.approx(4.0)
+        # the terminal term keeps the single goal
+        tr.terminal_cost_coeff = 2.0
+        assert oracle.oracle_terminal_cost_f32(ct.byref(tr), y.ctypes.data) == pytest.approx(2 * 26.25)
+    bad = ol.host_cost(base, np.array(QUAD_TRAJ, np.float32))
+    bad.goal_traj_len = 0
+    with pytest.raises(RuntimeError):
+        capi.call("mppi_host_running_cost_at", ct.byref(bad), y.ctypes.data, 0, ct.byref(ct.c_float()))
+
+
+def _di_problem(T=30, K=5):
+    prob = workloads.double_integrator_map(num_rollouts=K, num_timesteps=T, colored=False,
+                                           cost_map=np.zeros((8, 8), np.float32))
+    prob.cost.map_coeff = 0.0
+    prob.sampler.control_cost_coeff[0], prob.sampler.control_cost_coeff[1] = 0.3, 0.7
+    return prob
+
+
+def test_goal_trajectory_rollout_invariants():
+    """Rollout-level properties of the goal index min(goal_time + t, H - 1): a one-row table equal to the goal, a
+    table shifted by goal_time and a table padded with its last row all give the costs of their plain counterparts
+    bit for bit; a table whose rows differ changes them."""
+    T, K = 30, 5
+    prob = _di_problem(T, K)
+    rng = np.random.default_rng(8)
+    mean = (rng.normal(size=(T, prob.C)) * 0.3).astype(np.float32)
+    u = rng.normal(size=(K, T, prob.C)).astype(np.float32)
+    plain, _ = ol.rollout_f32(prob, u, mean)
+    prob.goal_traj = np.array([[prob.cost.goal[i] for i in range(4)]], np.float32)
+    one_row, _ = ol.rollout_f32(prob, u, mean)
+    assert np.array_equal(one_row, plain)
+    G = rng.normal(size=(12, 4)).astype(np.float32)
+    prob.goal_traj = G
+    prob.cost.goal_time = 4
+    shifted, _ = ol.rollout_f32(prob, u, mean)
+    prob.goal_traj = G[4:]
+    prob.cost.goal_time = 0
+    assert np.array_equal(ol.rollout_f32(prob, u, mean)[0], shifted)
+    prob.goal_traj = np.vstack([G[4:], np.repeat(G[-1:], T, axis=0)])
+    assert np.array_equal(ol.rollout_f32(prob, u, mean)[0], shifted)
+    assert not np.allclose(shifted, plain)
+
+
+def test_double_integrator_rollout_closed_form_time_varying():
+    """The prefix-sum DI rollout with a goal trajectory (row min(goal_time + t, H - 1) charged for y_{t+1}) and
+    per-step standard deviations in the likelihood-ratio term."""
+    T, K, C = 24, 4, 2
+    prob = _di_problem(T, K)
+    rng = np.random.default_rng(11)
+    G = rng.normal(size=(9, 4)).astype(np.float32)
+    sd_t = rng.uniform(0.3, 2.0, size=(T, C)).astype(np.float32)
+    prob.goal_traj, prob.std_dev_t = G, sd_t
+    prob.cost.goal_time = 3
+    mean = (rng.normal(size=(T, C)) * 0.5).astype(np.float32)
+    u = rng.normal(size=(K, T, C)).astype(np.float32)
+    costs, _ = ol.rollout_f64(prob, u, mean)
+    dt = np.float64(prob.dt)  # the FP64 loop takes dt as a double
+    x0 = prob.x0.astype(np.float64)
+    q = np.array([prob.cost.coeff[i] for i in range(4)], np.float64)
+    rows = G.astype(np.float64)[np.minimum(3 + np.arange(T), 8)]
+    pstart = capi.pure_noise_start(K, 0.01)
+    for k in range(K):
+        uc = u[k].astype(np.float64)
+        v = x0[2:4] + dt * np.cumsum(uc, axis=0)
+        p = x0[0:2] + dt * np.cumsum(np.vstack([x0[2:4], v[:-1]]), axis=0)
+        y = np.hstack([p, v])
+        run = np.sum(((y - rows) ** 2) @ q)
+        mu = mean.astype(np.float64) if k < pstart else np.zeros((T, C))
+        coeff = np.array([0.3, 0.7], np.float32).astype(np.float64)  # the FP32 coefficients the oracle charges
+        lr = 0.5 * np.sum(coeff * mu * (mu - 2 * uc) / sd_t.astype(np.float64) ** 2)
+        assert costs[k] == pytest.approx((run + lr) / T, rel=1e-9)
 
 
 # ----------------------------------------------------------------------------------------------------- cost map
