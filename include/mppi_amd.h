@@ -69,6 +69,12 @@ extern "C" {
 #define MPPI_FFT_LEN_2T 0
 #define MPPI_FFT_LEN_T 1
 
+/* Controller kinds: VanillaMPPIController (controllers/MPPI/mppi_controller.cu:152-237) and ColoredMPPIController
+ * (controllers/ColoredMPPI/colored_mppi_controller.cu:145-249: optional state leash, normExp or Tsallis weights,
+ * only control 1 clamped after smoothing). */
+#define MPPI_CONTROLLER_VANILLA 0
+#define MPPI_CONTROLLER_COLORED 1
+
 /* Precision of the residual MLP inside the rollout. */
 #define MPPI_NN_FP32 0 /* FP32 weights and activations (parity mode) */
 #define MPPI_NN_BF16 1 /* BF16 weights/activations on MFMA, FP32 accumulate */
@@ -222,6 +228,12 @@ int mppi_pure_noise_start(int32_t num_rollouts, float pure_noise_percentage, int
 int mppi_host_step(const mppi_dynamics_params* p, const float* x, const float* u, float dt, float* x_next,
                    float* x_dot, float* y);
 int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u);
+/* Dynamics::enforceLeash: per state, the nominal value when it lies within leash_values[i] of the true one, else the
+ * true value moved leash_values[i] towards it (dynamics/dynamics.cuh:445-463); the bicycle leashes its position in
+ * the body frame of the true yaw and its yaw along the shortest angle (dynamics/racer_dubins/racer_dubins.cu:
+ * 176-238). Host pointers; out may alias neither input. */
+int mppi_host_enforce_leash(const mppi_dynamics_params* p, const float* state_true, const float* state_nominal,
+                            const float* leash_values, float* state_out);
 /* Cost::computeRunningCost on the host (the kernels' cost code; c->map and c->goal_traj must point to host memory
  * here), at step 0 and at step t (the step index picks the goal_traj row). */
 int mppi_host_running_cost(const mppi_cost_params* c, const float* y, float* out);
@@ -267,6 +279,11 @@ int mppi_rollout(const mppi_dynamics_params* dyn, const mppi_cost_params* cost, 
  * overwritten in place with w_k = exp(-(cost_k - baseline)/lambda), the FP64 normalizer and sum of squares go to
  * stats_d. */
 int mppi_compute_weights(float* costs_d, int32_t num_rollouts, float lambda, mppi_weight_stats* stats_d, void* stream);
+/* The same with the Tsallis transform (TsallisKernel, core/mppi_common.cu:1004-1021,1469-1481):
+ * w_k = (1 - (cost_k - baseline) / gamma)^(1 / (r - 1)) where cost_k - baseline < gamma, else 0; gamma > 0, r != 1.
+ * lambda only enters the statistics (free energy). */
+int mppi_compute_weights_tsallis(float* costs_d, int32_t num_rollouts, float lambda, float gamma, float r,
+                                 mppi_weight_stats* stats_d, void* stream);
 
 /* Weighted control update (core/mppi_common.cu:746-773,1151-1196): mean_out[t][c] = sum_k (w_k / normalizer) *
  * samples[k][t][c], with the normalizer read from stats_d on the device. */
@@ -311,6 +328,13 @@ typedef struct mppi_controller_desc {
                                   sampler.std_dev_t */
   const float* goal_traj_host; /* optional host [cost.goal_traj_len][O] goal trajectory, copied and used in place of
                                   cost.goal_traj */
+  /* MPPI_CONTROLLER_VANILLA or MPPI_CONTROLLER_COLORED; the fields below apply to the colored controller
+   * (ColoredMPPIParams, controllers/ColoredMPPI/colored_mppi_controller.cuh:16-37) */
+  int32_t controller_kind;
+  float tsallis_gamma; /* gamma: > 0 together with tsallis_r != 0 selects the Tsallis weights; 0: normExp */
+  float tsallis_r;     /* r (2 in the reference's defaults) */
+  int32_t leash_active; /* leash_active_: leash x0 to the previous solution's state at the last slide's step count */
+  float state_leash_dist[MPPI_MAX_STATE]; /* state_leash_dist_ */
 } mppi_controller_desc;
 
 int mppi_controller_create(const mppi_controller_desc* desc, mppi_controller** out);
@@ -341,7 +365,8 @@ int mppi_controller_get_weight_stats(const mppi_controller* ctrl, mppi_weight_st
 int mppi_controller_get_free_energy(const mppi_controller* ctrl, mppi_free_energy* out_host);
 /* updateImportanceSampler / setControlSeq: set the nominal [T][C] sequence. */
 int mppi_controller_set_control_seq(mppi_controller* ctrl, const float* u_host);
-/* slideControlSequence(steps) (controllers/controller.cuh:609-621). */
+/* slideControlSequence(steps) (controllers/controller.cuh:609-621); for the colored controller it also sets the leash
+ * index to steps (ColoredMPPIController::slideControlSequence, colored_mppi_controller.cu:260-268). */
 int mppi_controller_slide_control_seq(mppi_controller* ctrl, int32_t steps);
 /* Device pointers to the controller's buffers for inspection: samples [K][T][C] (clamped samples of the last
  * iteration when keep_samples is set; otherwise unspecified: colored samples are drawn inside the rollout kernel

@@ -99,6 +99,15 @@ class Dynamics {
 #pragma unroll
     for (int i = 0; i < C_DIM; ++i) u[i] = derived().enforceControlConstraint(i, u[i]);
   }
+  // Dynamics::enforceLeash (dynamics/dynamics.cuh:445-463), host side: the nominal value per state when it lies within
+  // the leash of the true one, else the true value moved by the leash towards it
+  void enforceLeash(const state_array& xt, const state_array& xn, const state_array& lv, state_array& out) const
+  {
+    for (int i = 0; i < S_DIM; ++i) {
+      const float diff = xn[i] - xt[i];
+      out[i] = lv[i] < fabsf(diff) ? xt[i] + fminf(fmaxf(diff, -lv[i]), lv[i]) : xn[i];
+    }
+  }
   // Dynamics::step (dynamics/dynamics.cuh:278-285): derivative, update, output
   __host__ __device__ void step(const float* x, float* xn, float* xd, const float* u, float* y, float dt) const
   {

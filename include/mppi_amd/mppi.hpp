@@ -165,6 +165,15 @@ class DynamicsBase {
   {
     for (int i = 0; i < S; ++i) xn[i] = x[i] + xd[i] * dt;
   }
+  // Dynamics::enforceLeash (dynamics/dynamics.cuh:445-463; the bicycle's own, racer_dubins.cu:176-238)
+  void enforceLeash(const state_array& state_true, const state_array& state_nominal, const state_array& leash_values,
+                    state_array& state_output) const
+  {
+    const mppi_dynamics_params d = desc();
+    check(mppi_host_enforce_leash(&d, state_true.data(), state_nominal.data(), leash_values.data(),
+                                  state_output.data()),
+          "enforceLeash");
+  }
   void step(const state_array& x, state_array& xn, state_array& xd, const control_array& u, output_array& y,
             float /*t*/, float dt) const
   {
@@ -749,14 +758,14 @@ class VanillaMPPIController {
   {
     control_ = params.init_control_traj_;
   }
-  ~VanillaMPPIController() { release(); }
+  virtual ~VanillaMPPIController() { release(); }
   VanillaMPPIController(const VanillaMPPIController&) = delete;
   VanillaMPPIController& operator=(const VanillaMPPIController&) = delete;
 
-  std::string getControllerName() const { return "Vanilla MPPI"; }
+  virtual std::string getControllerName() const { return "Vanilla MPPI"; }
 
   // VanillaMPPIController::computeControl (controllers/MPPI/mppi_controller.cu:152-237)
-  void computeControl(const state_array& state, int optimization_stride = 1)
+  virtual void computeControl(const state_array& state, int optimization_stride = 1)
   {
     ensure();
     check(mppi_controller_set_control_seq(ctrl_, control_.data()), "updateImportanceSampler");
@@ -793,7 +802,7 @@ class VanillaMPPIController {
   MPPIFreeEnergyStatistics getFreeEnergyStatistics() const { return fe_; }
   void updateImportanceSampler(const control_trajectory& u) { control_ = u; }
   // Controller::slideControlSequence / slideControlSequenceHelper (controllers/controller.cuh:370-376,609-621)
-  void slideControlSequence(int steps)
+  virtual void slideControlSequence(int steps)
   {
     const int T = params_.num_timesteps_;
     for (int i = 0; i < T; ++i) {
@@ -885,6 +894,10 @@ class VanillaMPPIController {
   // rebuild the device side from scratch on the next computeControl
   void resetPlugins() { release(); }
 
+ protected:
+  // the controller-kind fields of the descriptor (none for the vanilla controller)
+  virtual void extendDesc(mppi_controller_desc&) const {}
+
  private:
   void enforce(control_array& u) const
   {
@@ -909,6 +922,7 @@ class VanillaMPPIController {
     add(&d.dt, sizeof(float) * 3);
     add(&d.num_iters, sizeof(int32_t) * 3);
     add(d.slide_control_scale, sizeof(d.slide_control_scale));
+    add(&d.controller_kind, sizeof(int32_t) * 4 + sizeof(d.state_leash_dist));
     if constexpr (is_device_plugin<DYN_T>::value) {
       add(static_cast<const void*>(model_), sizeof(DYN_T));
       add(static_cast<const void*>(cost_), sizeof(COST_T));
@@ -944,6 +958,7 @@ class VanillaMPPIController {
       d.cost = cost_->desc(*model_);
       d.goal_traj_host = cost_->goalTrajectoryHost();
     }
+    extendDesc(d);
     return d;
   }
 
@@ -988,7 +1003,10 @@ class VanillaMPPIController {
     state_fresh_ = false;
   }
 
+ protected:
   PARAMS_T params_;
+
+ private:
   mppi_controller* ctrl_ = nullptr;
   std::string snapshot_;              // the parameters the device side was last given
   unsigned long long map_version_ = 0;
@@ -998,6 +1016,97 @@ class VanillaMPPIController {
   mutable bool state_fresh_ = false;
   float baseline_ = 0.0f, normalizer_ = 0.0f;
   MPPIFreeEnergyStatistics fe_;
+};
+
+// ColoredMPPIParams (controllers/ColoredMPPI/colored_mppi_controller.cuh:16-37)
+template <int S_DIM, int C_DIM, int MAX_TIMESTEPS>
+struct ColoredMPPIParams : ControllerParams<S_DIM, C_DIM, MAX_TIMESTEPS> {
+  float r = 2.0f;
+  float gamma = 0.0f;
+  Mat<float, S_DIM, 1> state_leash_dist_ = Mat<float, S_DIM, 1>::Zero();
+};
+
+// ColoredMPPIController (controllers/ColoredMPPI/colored_mppi_controller.cuh:39-200, .cu:145-268): the vanilla
+// iteration with Tsallis weights when gamma and r are both nonzero, the state leash on x0 (applied here on the host
+// with the model's enforceLeash against the previous solution's state at the last slide's step count, before the
+// state goes to the device), and only control 1 clamped after smoothing.
+template <class DYN_T, class COST_T, class FB_T, int MAX_TIMESTEPS, int NUM_ROLLOUTS,
+          class SAMPLING_T = sampling_distributions::ColoredNoiseDistributionC<DYN_T::CONTROL_DIM>,
+          class PARAMS_T = ColoredMPPIParams<DYN_T::STATE_DIM, DYN_T::CONTROL_DIM, MAX_TIMESTEPS>>
+class ColoredMPPIController
+    : public VanillaMPPIController<DYN_T, COST_T, FB_T, MAX_TIMESTEPS, NUM_ROLLOUTS, SAMPLING_T, PARAMS_T> {
+  using BASE = VanillaMPPIController<DYN_T, COST_T, FB_T, MAX_TIMESTEPS, NUM_ROLLOUTS, SAMPLING_T, PARAMS_T>;
+
+ public:
+  using state_array = typename BASE::state_array;
+  using BASE::BASE;
+  std::string getControllerName() const override { return "Colored MPPI"; }
+
+  void computeControl(const state_array& state, int optimization_stride = 1) override
+  {
+    state_array local = state;
+    if (leash_active_ && solved_) {
+      const auto prev = this->getTargetStateSeq();
+      const int j = std::min(std::max(leash_jump_, 0), this->getNumTimesteps() - 1);
+      this->model_->enforceLeash(state, prev.col(j), this->params_.state_leash_dist_, local);
+    }
+    BASE::computeControl(local, optimization_stride);
+    solved_ = true;
+  }
+  void slideControlSequence(int steps) override
+  {
+    leash_jump_ = steps;
+    BASE::slideControlSequence(steps);
+  }
+
+  void setGamma(float gamma) { this->params_.gamma = gamma; }
+  float getGamma() const { return this->params_.gamma; }
+  void setRExp(float r) { this->params_.r = r; }
+  float getRExp() const { return this->params_.r; }
+  void setStateLeashLength(float new_state_leash, int index = 0) { this->params_.state_leash_dist_[index] = new_state_leash; }
+  float getStateLeashLength(int index) const { return this->params_.state_leash_dist_[index]; }
+  bool getLeashActive() const { return leash_active_; }
+  void setLeashActive(bool a) { leash_active_ = a; }
+  void setNoiseDecay(float decay)
+  {
+    auto p = this->sampler_->getParams();
+    p.std_dev_decay = decay;
+    this->sampler_->setParams(p);
+  }
+  void setOffsetDecayRate(float rate)
+  {
+    auto p = this->sampler_->getParams();
+    p.offset_decay_rate = rate;
+    this->sampler_->setParams(p);
+  }
+  float getOffsetDecayRate() const { return this->sampler_->getParams().offset_decay_rate; }
+  void setColoredNoiseExponents(const std::vector<float>& e)
+  {
+    auto p = this->sampler_->getParams();
+    for (size_t i = 0; i < e.size() && i < (size_t)DYN_T::CONTROL_DIM; ++i) p.exponents[i] = e[i];
+    this->sampler_->setParams(p);
+  }
+  float getColoredNoiseExponent(int index) const { return this->sampler_->getParams().exponents[index]; }
+  std::vector<float> getColoredNoiseExponents() const
+  {
+    std::vector<float> out;
+    for (int i = 0; i < DYN_T::CONTROL_DIM; ++i) out.push_back(this->sampler_->getParams().exponents[i]);
+    return out;
+  }
+
+ protected:
+  void extendDesc(mppi_controller_desc& d) const override
+  {
+    d.controller_kind = MPPI_CONTROLLER_COLORED;
+    d.tsallis_gamma = this->params_.gamma;
+    d.tsallis_r = this->params_.r;
+    d.leash_active = 0;  // leashed here
+  }
+
+ private:
+  int leash_jump_ = 1;
+  bool leash_active_ = false;
+  bool solved_ = false;
 };
 
 }  // namespace mppi_amd

@@ -39,6 +39,9 @@ SAMPLER_COLORED = 2
 FFT_LEN_2T = 0
 FFT_LEN_T = 1
 
+CONTROLLER_VANILLA = 0
+CONTROLLER_COLORED = 1
+
 NN_FP32 = 0
 NN_BF16 = 1
 
@@ -104,6 +107,8 @@ class ControllerDesc(ct.Structure):
         ("slide_control_scale", _f * MAX_CONTROL),
         ("init_control", ct.c_void_p), ("cost_map_host", ct.c_void_p), ("nn_theta_host", ct.c_void_p),
         ("std_dev_t_host", ct.c_void_p), ("goal_traj_host", ct.c_void_p),
+        ("controller_kind", _i), ("tsallis_gamma", _f), ("tsallis_r", _f), ("leash_active", _i),
+        ("state_leash_dist", _f * MAX_STATE),
     ]
 
 
@@ -135,6 +140,8 @@ _SIGS = {
     "mppi_host_enforce_constraints": ([ct.POINTER(DynamicsParams), P], ct.c_int),
     "mppi_host_running_cost": ([ct.POINTER(CostParams), P, ct.POINTER(_f)], ct.c_int),
     "mppi_host_running_cost_at": ([ct.POINTER(CostParams), P, _i, ct.POINTER(_f)], ct.c_int),
+    "mppi_host_enforce_leash": ([ct.POINTER(DynamicsParams), P, P, P, P], ct.c_int),
+    "mppi_compute_weights_tsallis": ([P, _i, _f, _f, _f, P, P], ct.c_int),
     "mppi_generate_normals": ([ct.c_uint64, ct.c_uint64, P, ct.c_int64, P], ct.c_int),
     "mppi_sample_gaussian": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),
     "mppi_sample_colored": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),

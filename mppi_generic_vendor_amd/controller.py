@@ -66,10 +66,14 @@ class VanillaMPPIController:
             d.init_control = ic.ctypes.data
         self._num_iters, self._use_graph, self._keep_samples = num_iters, d.use_graph, d.keep_samples
         self._slide = [d.slide_control_scale[i] for i in range(capi.MAX_CONTROL)]
+        self._extend_desc(d)
         h = ct.c_void_p()
         capi.call("mppi_controller_create", ct.byref(d), ct.byref(h))
         self._h = h
         self._lib = lib
+
+    def _extend_desc(self, d):
+        """Controller-kind fields of the descriptor (none for the vanilla controller)."""
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -150,6 +154,7 @@ class VanillaMPPIController:
         for i in range(capi.MAX_CONTROL):
             d.slide_control_scale[i] = self._slide[i]
         keep = _host_arrays(d, problem)  # noqa: F841 (alive across the call)
+        self._extend_desc(d)
         capi.call("mppi_controller_set_params", self._h, ct.byref(d))
         self.problem = problem
 
@@ -197,3 +202,78 @@ class VanillaMPPIController:
         v = ct.c_float()
         capi.call("mppi_controller_get_last_call_ms", self._h, ct.byref(v))
         return v.value
+
+
+class ColoredMPPIController(VanillaMPPIController):
+    """ColoredMPPIController (controllers/ColoredMPPI/colored_mppi_controller.cuh:39-200): the vanilla iteration with
+    the Tsallis weights when gamma and r are both nonzero (normExp otherwise), the state leash (x0 pulled towards the
+    previous solution's state at the last slide's step count), and only control 1 clamped after smoothing. Parameter
+    setters swap the new values in through set_params."""
+
+    def __init__(self, problem: Problem, num_iters: int = 1, use_graph: bool = True,
+                 init_control: np.ndarray | None = None, slide_control_scale=None, keep_samples: bool = False,
+                 gamma: float = 0.0, r: float = 2.0, leash_active: bool = False, state_leash_dist=None):
+        self._gamma, self._r, self._leash_active = float(gamma), float(r), bool(leash_active)
+        self._leash = np.zeros(capi.MAX_STATE, np.float32)
+        if state_leash_dist is not None:
+            self._leash[: len(state_leash_dist)] = state_leash_dist
+        super().__init__(problem, num_iters, use_graph, init_control, slide_control_scale, keep_samples)
+
+    def _extend_desc(self, d):
+        d.controller_kind = capi.CONTROLLER_COLORED
+        d.tsallis_gamma, d.tsallis_r, d.leash_active = self._gamma, self._r, int(self._leash_active)
+        for i in range(capi.MAX_STATE):
+            d.state_leash_dist[i] = float(self._leash[i])
+
+    def _push(self):
+        self.setParams(self.problem)
+
+    def getControllerName(self) -> str:
+        return "Colored MPPI"
+
+    def setGamma(self, gamma: float):
+        self._gamma = float(gamma)
+        self._push()
+
+    def getGamma(self) -> float:
+        return self._gamma
+
+    def setRExp(self, r: float):
+        self._r = float(r)
+        self._push()
+
+    def getRExp(self) -> float:
+        return self._r
+
+    def setLeashActive(self, active: bool):
+        self._leash_active = bool(active)
+        self._push()
+
+    def getLeashActive(self) -> bool:
+        return self._leash_active
+
+    def setStateLeashLength(self, length: float, index: int = 0):
+        self._leash[index] = length
+        self._push()
+
+    def getStateLeashLength(self, index: int) -> float:
+        return float(self._leash[index])
+
+    def setNoiseDecay(self, decay: float):
+        self.problem.sampler.std_dev_decay = decay
+        self._push()
+
+    def setOffsetDecayRate(self, rate: float):
+        self.problem.sampler.offset_decay_rate = rate
+        self._push()
+
+    def getOffsetDecayRate(self) -> float:
+        return self.problem.sampler.offset_decay_rate
+
+    def setColoredNoiseExponents(self, exponents):
+        for i, e in enumerate(exponents):
+            self.problem.sampler.exponents[i] = e
+        self._push()
+
+    def getColoredNoiseExponents(self):
+        return [self.problem.sampler.exponents[i] for i in range(self.C)]

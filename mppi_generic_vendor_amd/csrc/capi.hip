@@ -1,6 +1,7 @@
 // C-ABI of the engine: parameter defaults and validation, kernel-level entry points and the controller object that
 // runs whole MPPI iterations (VanillaMPPIController::computeControl, controllers/MPPI/mppi_controller.cu:152-237).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -117,6 +118,15 @@ int validate_cost(const mppi_cost_params& c, const mppi_dynamics_params& d, bool
       return fail(MPPI_E_INVALID, "speed term: speed_index out of range");
   }
   return validate_goal_traj(c);
+}
+
+int validate_tsallis(float gamma, float r, bool required)
+{
+  const bool on = gamma != 0.0f && r != 0.0f;
+  if (!on) return required ? fail(MPPI_E_INVALID, "Tsallis weights: gamma and r must be nonzero") : MPPI_OK;
+  if (!(gamma > 0.0f)) return fail(MPPI_E_INVALID, "Tsallis weights: gamma must be > 0");
+  if (r == 1.0f || !(r == r)) return fail(MPPI_E_INVALID, "Tsallis weights: r must differ from 1");
+  return MPPI_OK;
 }
 
 int validate_goal_traj(const mppi_cost_params& c)
@@ -266,6 +276,8 @@ struct mppi_controller {
   float* std_t_d = nullptr;  // [T][C] copy of the descriptor's per-step standard deviations
   float* goal_d = nullptr;   // [goal_len][O] copy of the descriptor's goal trajectory
   int goal_len = 0;
+  float* w_d = nullptr;      // [K] weights of the Tsallis iteration (the costs stay in costs_d)
+  int leash_jump = 1;        // the colored controller's leash index: the last slide's step count
   // per-call exchange buffers: mapped fine-grained host memory (the *_hd members are their device-side addresses),
   // moved by copy_words launches on the controller's stream
   IterArgs* iter_h = nullptr;
@@ -516,6 +528,54 @@ int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u)
   return MPPI_OK;
 }
 
+// normalizeAngle / shortestAngularDistance, float versions (utils/angle_utils.cuh:23-44)
+static float host_normalize_angle(float a)
+{
+  const float res = fmodf(a + (float)M_PI, (float)(2.0 * M_PI));
+  return res <= 0.0f ? res + (float)M_PI : res - (float)M_PI;
+}
+
+static void enforce_leash(const mppi_dynamics_params& p, const float* xt, const float* xn, const float* lv, float* out)
+{
+  const int S = p.state_dim;
+  const bool bicycle = p.model == MPPI_MODEL_RACER_DUBINS || p.model == MPPI_MODEL_RACER_DUBINS_NN;
+  for (int i = 0; i < S; ++i) out[i] = xt[i];
+  constexpr int YAW = 1, PX = 2, PY = 3;
+  if (bicycle) {
+    // position offset to the nominal in the body frame of the true yaw, clamped there, rotated back
+    const float yaw = xt[YAW];
+    float dx = xn[PX] - xt[PX], dy = xn[PY] - xt[PY];
+    float bx = dx * cosf(yaw) + dy * sinf(yaw);
+    float by = -dx * sinf(yaw) + dy * cosf(yaw);
+    bx = fminf(fmaxf(bx, -lv[PX]), lv[PX]);
+    by = fminf(fmaxf(by, -lv[PY]), lv[PY]);
+    dx = bx * cosf(yaw) + -by * sinf(yaw);
+    dy = bx * sinf(yaw) + by * cosf(yaw);
+    out[PX] += dx;
+    out[PY] += dy;
+  }
+  for (int i = 0; i < S; ++i) {
+    if (bicycle && (i == PX || i == PY)) continue;
+    const bool yaw = bicycle && i == YAW;
+    const float diff = yaw ? host_normalize_angle(xn[i] - xt[i]) : xn[i] - xt[i];
+    if (lv[i] < fabsf(diff)) {
+      out[i] = xt[i] + fminf(fmaxf(diff, -lv[i]), lv[i]);
+      if (yaw) out[i] = host_normalize_angle(out[i]);
+    } else {
+      out[i] = xn[i];
+    }
+  }
+}
+
+int mppi_host_enforce_leash(const mppi_dynamics_params* p, const float* state_true, const float* state_nominal,
+                            const float* leash_values, float* state_out)
+{
+  if (!p || !state_true || !state_nominal || !leash_values || !state_out) return fail(MPPI_E_INVALID, "null argument");
+  if (p->state_dim < 1 || p->state_dim > MPPI_MAX_STATE) return fail(MPPI_E_INVALID, "state_dim out of range");
+  enforce_leash(*p, state_true, state_nominal, leash_values, state_out);
+  return MPPI_OK;
+}
+
 int mppi_generate_normals(uint64_t seed, uint64_t offset, float* out_d, int64_t n, void* stream)
 {
   if (n < 0 || (n > 0 && !out_d)) return fail(MPPI_E_INVALID, "bad normals buffer");
@@ -578,6 +638,16 @@ int mppi_compute_weights(float* costs_d, int32_t K, float lambda, mppi_weight_st
   return launch_compute_weights(costs_d, K, lambda, stats_d, (hipStream_t)stream);
 }
 
+int mppi_compute_weights_tsallis(float* costs_d, int32_t K, float lambda, float gamma, float r,
+                                 mppi_weight_stats* stats_d, void* stream)
+{
+  if (!costs_d || !stats_d) return fail(MPPI_E_INVALID, "null argument");
+  if (K < 1) return fail(MPPI_E_INVALID, "num_rollouts must be >= 1");
+  if (!(lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
+  MPPI_CHECK(validate_tsallis(gamma, r, true));
+  return launch_weights(costs_d, costs_d, K, lambda, gamma, r, stats_d, nullptr, nullptr, 0, (hipStream_t)stream);
+}
+
 int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_d, const float* samples_d, int32_t K,
                          int32_t T, int32_t C, float* mean_out_d, void* stream)
 {
@@ -609,6 +679,26 @@ int mppi_weighted_update(const float* weights_d, const mppi_weight_stats* stats_
 
 // --------------------------------------------------------------------------------------------------- controller
 // The sample buffer receives the constrained samples of every iteration with keep_samples or the trajectory export.
+// The colored controller's Tsallis weights (ColoredMPPIParams gamma and r both nonzero).
+static bool uses_tsallis(const mppi_controller_desc& d)
+{
+  return d.controller_kind == MPPI_CONTROLLER_COLORED && d.tsallis_gamma != 0.0f && d.tsallis_r != 0.0f;
+}
+
+// The colored controller's own parameters: kind, Tsallis gamma / r, leash distances.
+static int validate_controller_kind(const mppi_controller_desc& d, bool plugin)
+{
+  if (d.controller_kind != MPPI_CONTROLLER_VANILLA && d.controller_kind != MPPI_CONTROLLER_COLORED)
+    return fail(MPPI_E_INVALID, "unknown controller_kind");
+  if (d.controller_kind != MPPI_CONTROLLER_COLORED) return MPPI_OK;
+  MPPI_CHECK(validate_tsallis(d.tsallis_gamma, d.tsallis_r, false));
+  if (plugin && uses_tsallis(d))
+    return fail(MPPI_E_UNSUPPORTED, "Tsallis weights: not available with a dynamics plugin");
+  for (int i = 0; i < MPPI_MAX_STATE; ++i)
+    if (!(d.state_leash_dist[i] >= 0.0f)) return fail(MPPI_E_INVALID, "state_leash_dist must be >= 0");
+  return MPPI_OK;
+}
+
 static bool keeps_samples(const mppi_controller* c) { return c->d.keep_samples != 0 || c->vis_random + c->vis_top > 0; }
 
 // Host post-processing of a plugin controller: the plugin's own constraint and step (Dynamics::enforceConstraints and
@@ -654,7 +744,7 @@ static int controller_free(mppi_controller* c)
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
                  c->map_d, c->nn_d, c->loop_iter_d, c->prop_dmu_d, c->prop_sig_d, c->sigma_base_d,
-                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d, c->std_t_d, c->goal_d};
+                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d, c->std_t_d, c->goal_d, c->w_d};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h};
@@ -720,6 +810,7 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
     MPPI_CHECK(validate_dynamics(desc->dynamics));
   }
   MPPI_CHECK(validate_sampler(desc->sampler, c->d.dynamics));
+  MPPI_CHECK(validate_controller_kind(*desc, c->has_plugin));
   if (!(desc->lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
   if (desc->num_iters < 1) return fail(MPPI_E_INVALID, "num_iters must be >= 1");
   c->S = c->d.dynamics.state_dim;
@@ -732,8 +823,12 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   MPPI_CHECK(dev_alloc(&c->samples_d, sizeof(float) * (size_t)K * T * C));
   MPPI_CHECK(dev_alloc(&c->costs_d, sizeof(float) * K));
   MPPI_CHECK(dev_alloc(&c->stats_d, sizeof(mppi_weight_stats)));
-  const size_t ws = fused_iteration_workspace_bytes(K, T, C);
+  // the workspace serves the fused iteration's partials and, for the colored controller, the weighted update's
+  const size_t ws = std::max(fused_iteration_workspace_bytes(K, T, C),
+                             desc->controller_kind == MPPI_CONTROLLER_COLORED ? weighted_update_workspace_bytes(K, T, C)
+                                                                              : (size_t)0);
   MPPI_CHECK(dev_alloc(&c->ws_d, ws));
+  if (desc->controller_kind == MPPI_CONTROLLER_COLORED) MPPI_CHECK(dev_alloc(&c->w_d, sizeof(float) * K));
   MPPI_CHECK(dev_alloc(&c->iter_d, sizeof(IterArgs) * desc->num_iters));
   MPPI_CHECK(dev_alloc(&c->loop_iter_d, sizeof(IterArgs) * desc->num_iters));
   MPPI_CHECK(host_alloc_mapped(&c->iter_h, &c->iter_hd, sizeof(IterArgs) * desc->num_iters));
@@ -848,6 +943,33 @@ static int enqueue_plugin_iteration(mppi_controller* c, IterRef ref, IterArgs* a
                         c->mean_d, advance, advance_step, copies, c->stream);
 }
 
+// One iteration of the colored controller with Tsallis weights, which need the global baseline before any weight
+// (so the fused kernel's per-workgroup log-sum-exp partials do not apply): samples into the sample buffer, the rollout
+// (costs, constrained samples written back), the weight transform against the global baseline, the weighted update
+// into the mean, then the mean to the host copy on the last iteration.
+static int enqueue_tsallis_iteration(mppi_controller* c, IterRef ref, IterArgs* advance, const ResultCopies* copies,
+                                     int advance_step, hipEvent_t between, int* partials_out)
+{
+  const mppi_controller_desc& d = c->d;
+  const int K = c->K, T = c->T, C = c->C;
+  if (d.sampler.kind == MPPI_SAMPLER_COLORED)
+    MPPI_CHECK(launch_sample_colored(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, c->stream));
+  else
+    MPPI_CHECK(launch_sample_gaussian(d.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, c->stream));
+  MPPI_CHECK(launch_rollout(d.dynamics, d.cost, d.sampler, K, T, d.dt, d.lambda, d.alpha, c->x0_d, c->mean_d,
+                            c->samples_d, c->costs_d, c->stream));
+  if (between != nullptr) MPPI_HIP_CHECK(hipEventRecord(between, c->stream));
+  if (partials_out != nullptr) *partials_out = 0;
+  MPPI_CHECK(launch_weights(c->costs_d, c->w_d, K, d.lambda, d.tsallis_gamma, d.tsallis_r, c->stats_d,
+                            copies != nullptr ? copies->stats : nullptr, advance, advance_step, c->stream));
+  MPPI_CHECK(launch_weighted_update(c->w_d, c->stats_d, c->samples_d, K, T, C, c->mean_d, c->ws_d, c->stream));
+  if (copies != nullptr && copies->mean != nullptr) {
+    const WordSpan m[] = {{c->mean_d, copies->mean, T * C}};
+    MPPI_CHECK(launch_copy_words(m, 1, c->stream));
+  }
+  return MPPI_OK;
+}
+
 // One MPPI iteration on the controller's stream: the fused sampling + rollout + weights + update (Gaussian samples
 // drawn inside the rollout kernel; colored samples too where the kernel's workgroup shape allows, else by the colored
 // sampler into the sample buffer first).
@@ -858,6 +980,7 @@ static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
   const mppi_controller_desc& d = c->d;
   const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
   if (c->has_plugin) return enqueue_plugin_iteration(c, ref, advance, copies, advance_step, between, partials_out);
+  if (uses_tsallis(d)) return enqueue_tsallis_iteration(c, ref, advance, copies, advance_step, between, partials_out);
   ColoredPlan plan;
   if (colored) MPPI_CHECK(colored_plan(d.sampler, c->K, c->T, c->C, c->stream, &plan));
   // the combine kernel writes the new mean over the sampling mean, as updateDistributionParamsFromDevice writes
@@ -967,7 +1090,17 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
     if (stride >= n) return fail(MPPI_E_INVALID, "colored sampler: optimization_stride out of range");
   }
   c->fe.previous_baseline = c->stats.baseline;
-  for (int i = 0; i < S; ++i) c->x0_h[i] = x0_host[i];
+  const bool colored_ctrl = d.controller_kind == MPPI_CONTROLLER_COLORED;
+  if (colored_ctrl && d.leash_active && c->computed && !c->has_plugin) {
+    // the leash pulls x0 towards the previous solution's state at the last slide's step count
+    // (ColoredMPPIController::computeControl, colored_mppi_controller.cu:151-154)
+    std::vector<float> prev((size_t)T * S);
+    MPPI_CHECK(mppi_controller_get_state_seq(c, prev.data()));
+    const int j = std::min(std::max(c->leash_jump, 0), T - 1);
+    enforce_leash(d.dynamics, x0_host, prev.data() + (size_t)j * S, d.state_leash_dist, c->x0_h);
+  } else {
+    for (int i = 0; i < S; ++i) c->x0_h[i] = x0_host[i];
+  }
   std::copy(c->control.begin(), c->control.end(), c->nominal_h);
   if (!c->ev0) {
     MPPI_HIP_CHECK(hipEventCreate(&c->ev0));
@@ -1007,8 +1140,19 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
   c->state_x0.assign(c->x0_h, c->x0_h + S);
   c->state_u = c->control;
   c->state_stale = true;
-  if (c->has_plugin) plugin_clamp_sequence(c, c->control.data());
-  else clamp_controls_seq(d.dynamics, c->control.data(), T);
+  if (colored_ctrl) {
+    // the colored controller clamps control 1 alone to its range (colored_mppi_controller.cu:226-231; a model with
+    // one control has none to clamp)
+    if (C >= 2)
+      for (int t = 0; t < T; ++t) {
+        float& u = c->control[(size_t)t * C + 1];
+        u = fminf(fmaxf(u, d.dynamics.control_min[1]), d.dynamics.control_max[1]);
+      }
+  } else if (c->has_plugin) {
+    plugin_clamp_sequence(c, c->control.data());
+  } else {
+    clamp_controls_seq(d.dynamics, c->control.data(), T);
+  }
   c->computed = true;
   return MPPI_OK;
 }
@@ -1036,6 +1180,7 @@ int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, in
   if (c->d.sampler.kind != MPPI_SAMPLER_GAUSSIAN) return fail(MPPI_E_UNSUPPORTED, "proposal: Gaussian sampler only");
   if (c->d.sampler.std_dev_t != nullptr)
     return fail(MPPI_E_UNSUPPORTED, "proposal: the base sampler has per-step std_dev (uniform std_dev only)");
+  if (uses_tsallis(c->d)) return fail(MPPI_E_UNSUPPORTED, "proposal: not available with Tsallis weights");
   if (num_rollouts < 2 || num_rollouts > c->K)
     return fail(MPPI_E_INVALID, "proposal: num_rollouts must be in [2, controller num_rollouts]");
   const size_t tc = sizeof(float) * c->T * c->C;
@@ -1241,6 +1386,7 @@ int mppi_controller_slide_control_seq(mppi_controller* c, int32_t steps)
   const int T = c->T, C = c->C;
   const float* zero = c->d.dynamics.zero_control;
   const float* scale = c->d.slide_control_scale;
+  c->leash_jump = steps;
   for (int i = 0; i < T; ++i) {
     const int ind = std::min(i + steps, T - 1);
     for (int j = 0; j < C; ++j) {
@@ -1280,10 +1426,14 @@ int mppi_controller_set_params(mppi_controller* c, const mppi_controller_desc* d
   const mppi_controller_desc& o = c->d;
   if (desc->num_rollouts != o.num_rollouts || desc->num_timesteps != o.num_timesteps ||
       desc->num_iters != o.num_iters || desc->sampler.kind != o.sampler.kind ||
+      desc->controller_kind != o.controller_kind ||
       (!c->has_plugin && (desc->dynamics.model != o.dynamics.model || desc->cost.cost != o.cost.cost)))
-    return fail(MPPI_E_INVALID, "set_params: sizes, iteration count, model, cost or sampler kind changed "
-                                "(create a new controller)");
+    return fail(MPPI_E_INVALID, "set_params: sizes, iteration count, model, cost, sampler or controller kind "
+                                "changed (create a new controller)");
   if (!(desc->lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
+  MPPI_CHECK(validate_controller_kind(*desc, c->has_plugin));
+  if (c->prop != nullptr && uses_tsallis(*desc))
+    return fail(MPPI_E_UNSUPPORTED, "set_params: Tsallis weights with a learned proposal attached");
   mppi_controller_desc n = *desc;
   if (c->has_plugin) {
     if (desc->dynamics.control_dim != c->C) return fail(MPPI_E_INVALID, "set_params: control_dim changed");
@@ -1440,11 +1590,19 @@ int mppi_controller_get_sampled_trajectories(mppi_controller* c, int32_t* count,
   if (terminal_costs) MPPI_HIP_CHECK(hipMemcpy(terminal_costs, term, sizeof(float) * n, hipMemcpyDeviceToHost));
   if (indices) std::copy(sel.begin(), sel.end(), indices);
   if (weights) {
-    // normalized weights w_k / eta of the last iteration (core/mppi_common.cu:994-1002 over its costs)
+    // normalized weights w_k / eta of the last iteration: the device's own weights with the Tsallis transform, else
+    // exp(-(c_k - b) / lambda) / eta over its costs (core/mppi_common.cu:994-1002)
     const double b = c->stats.baseline, eta = c->stats.normalizer, inv = 1.0 / (double)d.lambda;
+    std::vector<float> w;
+    if (uses_tsallis(d)) {
+      w.resize(c->K);
+      MPPI_HIP_CHECK(hipMemcpy(w.data(), c->w_d, sizeof(float) * c->K, hipMemcpyDeviceToHost));
+    }
     for (int i = 0; i < n; ++i) {
-      const double ck = cost[(size_t)sel[(size_t)i]];
-      weights[i] = (float)((ck <= FLT_MAX) ? std::exp(-inv * (ck - b)) / eta : 0.0);
+      const size_t k = (size_t)sel[(size_t)i];
+      const double ck = cost[k];
+      weights[i] = w.empty() ? (float)((ck <= FLT_MAX) ? std::exp(-inv * (ck - b)) / eta : 0.0)
+                             : (float)((double)w[k] / eta);
     }
   }
   *count = n;

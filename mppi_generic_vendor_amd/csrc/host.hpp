@@ -14,6 +14,8 @@ int model_dims(int model, int* S, int* C, int* O);
 int validate_dynamics(const mppi_dynamics_params& d);
 int validate_cost(const mppi_cost_params& c, const mppi_dynamics_params& d, bool need_map_ptr);
 int validate_goal_traj(const mppi_cost_params& c);
+// gamma / r of the Tsallis weights: off (either 0) unless `required`, else gamma > 0 and r != 1.
+int validate_tsallis(float gamma, float r, bool required);
 int validate_sampler(const mppi_sampler_params& s, const mppi_dynamics_params& d);
 int validate_sizes(int K, int T);
 

@@ -203,10 +203,16 @@ int launch_rollout(const mppi_dynamics_params& dyn, const mppi_cost_params& cost
 }
 
 // =============================================================================================== weight transform
-// fullGPUcomputeWeights (core/mppi_common.cu:1067-1089): one workgroup; baseline = first strict minimum, weights in
-// place, normalizer and sum of squares in FP64.
-__global__ void __launch_bounds__(1024) weights_kernel(float* __restrict__ costs, int K, float lambda,
-                                                       mppi_weight_stats* __restrict__ stats)
+// fullGPUcomputeWeights (core/mppi_common.cu:1067-1089): one workgroup; baseline = first strict minimum, weights
+// w_out[k] (may be costs itself), normalizer and sum of squares in FP64. The weights are the normExp transform
+// exp(-(c - b) / lambda) (:994-1002) or, with TSALLIS, (1 - (c - b) / gamma)^(1 / (r - 1)) for c - b < gamma and 0
+// beyond (TsallisTransform, :1004-1021, with its FP64 1.0 - (c - b) / gamma rounded to FP32 for logf). Optionally the
+// statistics also go to host memory and the iteration's rng offset is advanced (device-resident loops).
+template <bool TSALLIS>
+__global__ void __launch_bounds__(1024) weights_kernel(const float* costs, float* w_out, int K, float lambda,
+                                                       float gamma, float r, mppi_weight_stats* __restrict__ stats,
+                                                       mppi_weight_stats* __restrict__ stats_copy,
+                                                       IterArgs* __restrict__ advance, int advance_step)
 {
   __shared__ float sv[16];
   __shared__ int si[16];
@@ -238,8 +244,14 @@ __global__ void __launch_bounds__(1024) weights_kernel(float* __restrict__ costs
   for (int i = tid; i < K; i += blockDim.x) {
     const float c = costs[i];
     const bool fin = c <= FLT_MAX;
-    const float w = fin ? expf(-inv * (c - b)) : 0.0f;
-    costs[i] = w;
+    float w;
+    if (TSALLIS) {
+      const float dif = c - b;
+      w = dif < gamma ? expf(logf((float)(1.0 - (double)(dif / gamma))) / (r - 1.0f)) : 0.0f;
+    } else {
+      w = fin ? expf(-inv * (c - b)) : 0.0f;
+    }
+    w_out[i] = w;
     e += (double)w;
     e2 += (double)w * (double)w;
     if (fin) ec += (double)w * (double)c;
@@ -269,12 +281,28 @@ __global__ void __launch_bounds__(1024) weights_kernel(float* __restrict__ costs
     st.num_rollouts = K;
     st.weighted_cost = tec / te;
     *stats = st;
+    if (stats_copy != nullptr) *stats_copy = st;
+    if (advance != nullptr) advance->rng_offset += (uint64_t)advance_step;
   }
 }
 
 int launch_compute_weights(float* costs, int K, float lambda, mppi_weight_stats* stats, hipStream_t s)
 {
-  weights_kernel<<<1, 1024, 0, s>>>(costs, K, lambda, stats);
+  weights_kernel<false><<<1, 1024, 0, s>>>(costs, costs, K, lambda, 0.0f, 0.0f, stats, nullptr, nullptr, 0);
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
+int launch_weights(const float* costs, float* w_out, int K, float lambda, float gamma, float r,
+                   mppi_weight_stats* stats, mppi_weight_stats* stats_copy, IterArgs* advance, int advance_step,
+                   hipStream_t s)
+{
+  if (gamma != 0.0f && r != 0.0f)
+    weights_kernel<true><<<1, 1024, 0, s>>>(costs, w_out, K, lambda, gamma, r, stats, stats_copy, advance,
+                                           advance_step);
+  else
+    weights_kernel<false><<<1, 1024, 0, s>>>(costs, w_out, K, lambda, 0.0f, 0.0f, stats, stats_copy, advance,
+                                            advance_step);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }

@@ -27,6 +27,11 @@ int launch_rollout(const mppi_dynamics_params& dyn, const mppi_cost_params& cost
                    int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
                    float* samples, float* costs, hipStream_t s);
 int launch_compute_weights(float* costs, int K, float lambda, mppi_weight_stats* stats, hipStream_t s);
+// The weight transform into w_out (may be costs): normExp, or Tsallis when gamma != 0 and r != 0; the statistics also
+// to stats_copy and the rng offset of *advance advanced by advance_step when those are given.
+int launch_weights(const float* costs, float* w_out, int K, float lambda, float gamma, float r,
+                   mppi_weight_stats* stats, mppi_weight_stats* stats_copy, IterArgs* advance, int advance_step,
+                   hipStream_t s);
 // `ws` provides weighted_update_workspace_bytes(K, T, C) bytes of device memory.
 size_t weighted_update_workspace_bytes(int K, int T, int C);
 int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,

@@ -589,6 +589,66 @@ void oracle_free_energy(const double* w, int K, double baseline, double lambda, 
   *mod = lambda * (weird + 0.5 * weird * weird);
 }
 
+/* ColoredMPPIController's per-iteration host path (controllers/ColoredMPPI/colored_mppi_controller.cu:185-221):
+ * baseline = computeBaselineCost (first strict minimum, core/mppi_common.cu:894-935), the FP32 transform of the device
+ * (normExp, :994-1002, with lambda_inv = (float)(1.0 / lambda); or Tsallis when gamma != 0 and r != 0, :1004-1021),
+ * the host normalizer (an FP64 sum of the FP32 weights returned as float, :1091-1099) and computeFreeEnergy with its
+ * FP32 accumulators (:1101-1117). w receives the weights. */
+float oracle_colored_weights_f32(const float* costs, int K, float lambda, float gamma, float r, float* w,
+                                 float* baseline, float* fe_mean, float* fe_var, float* fe_mod)
+{
+  const float b = costs[oracle_best_index(costs, K)];
+  const bool tsallis = gamma != 0.0f && r != 0.0f;
+  const float lambda_inv = (float)(1.0 / lambda);
+  for (int i = 0; i < K; ++i) {
+    const float dif = costs[i] - b;
+    if (tsallis) w[i] = dif < gamma ? expf(logf((float)(1.0 - (double)(dif / gamma))) / (r - 1)) : 0.0f;
+    else w[i] = expf(-lambda_inv * dif);
+  }
+  double normalizer = 0.0;
+  for (int i = 0; i < K; ++i) normalizer += w[i];
+  float var = 0, norm = 0;
+  for (int i = 0; i < K; ++i) {
+    norm += w[i];
+    var += w[i] * w[i];
+  }
+  norm /= K;
+  *fe_mean = -lambda * logf(norm) + b;
+  *fe_var = lambda * (var / K - norm * norm);
+  const float weird = *fe_var / (norm * sqrtf(1.0f * K));
+  *fe_mod = lambda * (weird + 0.5f * weird * weird);
+  *baseline = b;
+  return (float)normalizer;
+}
+
+/* Dynamics::enforceLeash (dynamics/dynamics.cuh:445-463) and the bicycle's (dynamics/racer_dubins/racer_dubins.cu:
+ * 176-238: position leashed in the body frame of the true yaw, yaw along shortestAngularDistance and normalized). */
+void oracle_enforce_leash(const mppi_dynamics_params* p, const float* xt, const float* xn, const float* lv, float* out)
+{
+  const int S = p->state_dim;
+  const bool bicycle = p->model == MPPI_MODEL_RACER_DUBINS || p->model == MPPI_MODEL_RACER_DUBINS_NN;
+  for (int i = 0; i < S; ++i) out[i] = xt[i];
+  if (bicycle) {
+    const float c = cosf(xt[1]), sn = sinf(xt[1]);
+    float dx = xn[2] - xt[2], dy = xn[3] - xt[3];
+    const float bx = fminf(fmaxf(dx * c + dy * sn, -lv[2]), lv[2]);
+    const float by = fminf(fmaxf(-dx * sn + dy * c, -lv[3]), lv[3]);
+    out[2] += bx * c - by * sn;
+    out[3] += bx * sn + by * c;
+  }
+  for (int i = 0; i < S; ++i) {
+    if (bicycle && (i == 2 || i == 3)) continue;
+    const bool yaw = bicycle && i == 1;
+    const float diff = yaw ? normalize_angle<float>(xn[i] - xt[i]) : xn[i] - xt[i];
+    if (lv[i] < fabsf(diff)) {
+      out[i] = xt[i] + fminf(fmaxf(diff, -lv[i]), lv[i]);
+      if (yaw) out[i] = normalize_angle<float>(out[i]);
+    } else {
+      out[i] = xn[i];
+    }
+  }
+}
+
 /* ------------------------------------------------------------------------------------------ host post-process */
 void oracle_smooth(float* u, int T, int C)
 {

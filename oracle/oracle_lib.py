@@ -46,6 +46,8 @@ _SIGS = {
     "oracle_is_cost": ([P, P, P, P, P, _i, _i, _i, _d, P], None),
     "oracle_map_query_f32": ([ct.POINTER(capi.CostParams), P, _f, _f], _f),
     "oracle_running_cost_f32": ([ct.POINTER(capi.CostParams), P, P, _i], _f),
+    "oracle_colored_weights_f32": ([P, _i, _f, _f, _f, P, P, P, P, P], _f),
+    "oracle_enforce_leash": ([ct.POINTER(capi.DynamicsParams), P, P, P, P], None),
     "oracle_terminal_cost_f32": ([ct.POINTER(capi.CostParams), P], _f),
     "oracle_lr_cost_f32": ([ct.POINTER(capi.SamplerParams), _i, P, P, _i, _f, _f, _i], _f),
     "oracle_rollout_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
@@ -263,6 +265,24 @@ def state_trajectory_f64(prob, x0, u) -> np.ndarray:
     _a1 = np.ascontiguousarray(u, np.float32)
     load().oracle_state_trajectory_f64(ct.byref(dyn), ptr(_a0), ptr(_a1), T, float(np.float32(prob.dt)), ptr(out))
     return out
+
+
+def colored_weights(costs: np.ndarray, lam: float, gamma: float = 0.0, r: float = 0.0):
+    """The colored controller's host path: (FP32 weights, float normalizer, baseline, (fe mean, var, modified))."""
+    K = costs.shape[0]
+    c = np.ascontiguousarray(costs, dtype=np.float32)
+    w = np.empty(K, dtype=np.float32)
+    b, m, v, mod = _f(), _f(), _f(), _f()
+    eta = load().oracle_colored_weights_f32(ptr(c), K, lam, gamma, r, ptr(w), ct.byref(b), ct.byref(m), ct.byref(v),
+                                            ct.byref(mod))
+    return w, eta, b.value, (m.value, v.value, mod.value)
+
+
+def enforce_leash(dyn, x_true, x_nominal, leash):
+    out = np.zeros(capi.MAX_STATE, np.float32)
+    a = [np.ascontiguousarray(np.resize(np.asarray(v, np.float32), capi.MAX_STATE)) for v in (x_true, x_nominal, leash)]
+    load().oracle_enforce_leash(ct.byref(dyn), ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(out))
+    return out[: dyn.state_dim]
 
 
 def free_energy(w: np.ndarray, baseline: float, lam: float):

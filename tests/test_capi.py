@@ -122,4 +122,13 @@ def test_argument_validation_returns_codes(engine):
     h = ct.c_void_p()
     st = engine.mppi_controller_create(ct.byref(d), ct.byref(h))
     assert st == capi.MPPI_E_INVALID and not h.value
+    # the colored controller's parameters: kind, Tsallis gamma > 0 with r != 1, leash distances >= 0
+    d.lambda_ = 1.0
+    for kind, gamma, r, leash in ((7, 0.0, 2.0, 0.0), (1, -1.0, 2.0, 0.0), (1, 1.0, 1.0, 0.0), (1, 0.0, 2.0, -1.0)):
+        d.controller_kind, d.tsallis_gamma, d.tsallis_r, d.state_leash_dist[0] = kind, gamma, r, leash
+        st = engine.mppi_controller_create(ct.byref(d), ct.byref(h))
+        assert st == capi.MPPI_E_INVALID and not h.value, (kind, gamma, r, leash)
+    assert engine.mppi_compute_weights_tsallis(P, 10, 1.0, 0.0, 2.0, P, None) == capi.MPPI_E_INVALID
+    assert engine.mppi_compute_weights_tsallis(P, 10, 1.0, 1.0, 1.0, P, None) == capi.MPPI_E_INVALID
+    assert engine.mppi_compute_weights_tsallis(P, 10, 1.0, -2.0, 2.0, P, None) == capi.MPPI_E_INVALID
     assert engine.mppi_status_string(capi.MPPI_E_SIZE) == b"size exceeds a compiled limit"

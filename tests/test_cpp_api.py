@@ -12,7 +12,7 @@ PREBUILT = ROOT / "tests" / "cpp" / "_build" / "cartpole_closed_loop"
 
 
 def test_cpp_api_compiles_with_gcc(tmp_path, engine):
-    for name in ("cartpole_closed_loop", "plant_loop", "tracking_loop"):
+    for name in ("cartpole_closed_loop", "plant_loop", "tracking_loop", "colored_loop"):
         assert build_program(name, tmp_path).exists()
 
 
@@ -91,6 +91,20 @@ def test_cpp_goal_trajectory_tracking(tmp_path, hip, engine):
     current_time is held at 0."""
     exe = TRACKING if TRACKING.exists() else build_program("tracking_loop", tmp_path)
     print("[gpu] C++ goal-trajectory tracking", flush=True)
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0, res.stdout + res.stderr
+
+
+COLORED = ROOT / "tests" / "cpp" / "_build" / "colored_loop"
+
+
+@pytest.mark.gpu
+def test_cpp_colored_controller(tmp_path, hip, engine):
+    """ColoredMPPIController (tests/cpp/colored_loop.cpp): with normExp weights and no leash it reproduces the
+    vanilla controller bit for bit; with Tsallis weights and the leash it drives the double integrator to its goal."""
+    exe = COLORED if COLORED.exists() else build_program("colored_loop", tmp_path)
+    print("[gpu] C++ colored controller", flush=True)
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stdout + res.stderr

@@ -627,6 +627,79 @@ def test_double_integrator_rollout_closed_form_time_varying():
         assert costs[k] == pytest.approx((run + lr) / T, rel=1e-9)
 
 
+def test_colored_host_weights_closed_forms(oracle):
+    """The colored controller's host path: normExp equals the FP32 device transform; Tsallis with r = 2 is the linear
+    weight 1 - d / gamma, with r = 3 its square root, 0 at and beyond d = gamma, 1 at the baseline; the normalizer is
+    the FP64 sum of the FP32 weights; the FP32 free energy agrees with the FP64 one."""
+    rng = np.random.default_rng(12)
+    costs = rng.uniform(10.0, 14.0, 3000).astype(np.float32)
+    costs[17] = 9.5  # the baseline
+    K = costs.size
+    w, eta, b, fe = ol.colored_weights(costs, 0.7)
+    assert b == np.float32(9.5)
+    want = np.empty(K, np.float32)
+    oracle.oracle_weights_f32(costs.ctypes.data, K, 0.7, want.ctypes.data)
+    np.testing.assert_array_equal(w, want)
+    assert eta == np.float32(np.sum(w.astype(np.float64)))
+    w64, eta64, *_ = ol.weights(costs, 0.7)
+    m64 = ol.free_energy(w64, float(b), 0.7)
+    np.testing.assert_allclose(fe, m64, rtol=2e-5)
+    d = costs.astype(np.float64) - 9.5
+    for r, f in ((2.0, lambda q: q), (3.0, np.sqrt)):
+        w, eta, b, _ = ol.colored_weights(costs, 0.7, 2.0, r)
+        q = np.clip(1.0 - d / 2.0, 0.0, None)
+        np.testing.assert_allclose(w, f(q), rtol=2e-6, atol=2e-7)
+        assert w[17] == 1.0 and np.all(w[d >= 2.0] == 0.0)
+        assert eta == np.float32(np.sum(w.astype(np.float64)))
+
+
+def test_leash_known_answers(oracle, engine):
+    """Dynamics::enforceLeash: the reference's RacerDubins cases (tests/dynamics/racer_dubins_model_test.cu:164-255)
+    and the generic rule (dynamics/dynamics.cuh:445-463) on the double integrator, for the oracle and the engine."""
+    bic = capi.dynamics_defaults(capi.MODEL_RACER_DUBINS)
+    di = capi.dynamics_defaults(capi.MODEL_DOUBLE_INTEGRATOR)
+
+    def engine_leash(dyn, xt, xn, lv):
+        a = [np.resize(np.asarray(v, np.float32), capi.MAX_STATE).astype(np.float32) for v in (xt, xn, lv)]
+        out = np.zeros(capi.MAX_STATE, np.float32)
+        capi.call("mppi_host_enforce_leash", ct.byref(dyn), a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data,
+                  out.ctypes.data)
+        return out[: dyn.state_dim]
+
+    def norm(a):
+        a = np.float32(a)
+        res = np.fmod(a + np.float32(math.pi), np.float32(2 * math.pi))
+        return res + np.float32(math.pi) if res <= 0 else res - np.float32(math.pi)
+
+    rng = np.random.default_rng(5)
+    for leash in (ol.enforce_leash, engine_leash):
+        xt = rng.uniform(-1, 1, 7).astype(np.float32)
+        np.testing.assert_allclose(leash(bic, xt, xt, np.zeros(7)), xt, atol=1e-7)  # equal -> equal
+        np.testing.assert_allclose(leash(bic, xt, np.ones(7), np.zeros(7)), xt, atol=1e-7)  # zero leash -> true
+        np.testing.assert_allclose(leash(bic, xt, np.ones(7), 2 * np.ones(7)), np.ones(7), atol=1e-6)  # -> nominal
+        m3, p3 = -3 * np.ones(7, np.float32), 3 * np.ones(7, np.float32)
+        out = leash(bic, m3, p3, [0, 1.0, 0, 0, 0, 0, 0])  # yaw discontinuity inside the leash: the nominal yaw
+        np.testing.assert_allclose(out, [-3, 3, -3, -3, -3, -3, -3], rtol=2e-7)
+        out = leash(bic, m3, p3, [0, 0.15, 0, 0, 0, 0, 0])  # clamped across the discontinuity
+        np.testing.assert_allclose(out, [-3, norm(-3 - 0.15), -3, -3, -3, -3, -3], rtol=2e-7)
+        out = leash(bic, p3, m3, [0, 0.15, 0, 0, 0, 0, 0])
+        np.testing.assert_allclose(out, [3, norm(3 + 0.15), 3, 3, 3, 3, 3], rtol=2e-7)
+        for i in range(7):
+            xt, xn = np.zeros(7, np.float32), np.zeros(7, np.float32)
+            xt[i], xn[i] = 1.0, 1.1
+            want = np.zeros(7, np.float32)
+            want[i] = 1.1
+            np.testing.assert_allclose(leash(bic, xt, xn, np.ones(7)), want, atol=2e-7)
+        # body frame: true yaw pi/2, nominal 1 m ahead in x (the body's -y): a 0.2 m lateral leash moves x by 0.2
+        xt = np.array([0, math.pi / 2, 0, 0, 0, 0, 0], np.float32)
+        xn = np.array([0, math.pi / 2, 1.0, 0, 0, 0, 0], np.float32)
+        out = leash(bic, xt, xn, [0, 0, 5.0, 0.2, 0, 0, 0])
+        np.testing.assert_allclose(out[2:4], [0.2, 0.0], atol=1e-6)
+        # generic rule, double integrator: within the leash -> nominal, beyond -> true + leash towards it
+        out = leash(di, [0, 0, 0, 0], [0.5, -3.0, 0.1, 2.0], [1.0, 1.0, 0.0, 0.5])
+        np.testing.assert_allclose(out, [0.5, -1.0, 0.0, 0.5], atol=1e-7)
+
+
 # ----------------------------------------------------------------------------------------------------- cost map
 def test_bilinear_against_regular_grid_interpolator(oracle):
     """Texel centres at (i + 0.5) * res + origin, linear interpolation, clamp at the borders."""
