@@ -136,7 +136,9 @@ typedef struct mppi_cost_params {
   float terminal_cost_coeff;
   /* Cost-map term map_coeff * bilinear(map, y[map_x_index], y[map_y_index]) with the reference's CPU bilinear
    * semantics (utils/texture_helpers/two_d_texture_helper.cu:151-245, texture_helper.cu:94-134).
-   * World -> texel: q = ((p - origin) / resolution) / extent * extent - 0.5, clamped. map == NULL disables it. */
+   * World -> texel: q = ((R (p - origin)) / resolution) / extent * extent - 0.5, clamped, R = map_rotation (row-major
+   * 2 x 2, worldPoseToMapPose's rotation restricted to the plane) when map_rotated, else the identity.
+   * map == NULL disables it. */
   const float* map; /* device pointer [map_height][map_width] */
   int32_t map_width;
   int32_t map_height;
@@ -157,7 +159,8 @@ typedef struct mppi_cost_params {
   int32_t goal_traj_len;
   const float* goal_traj;
   int32_t goal_time;
-  int32_t _pad;
+  int32_t map_rotated;      /* 0: identity map frame rotation */
+  float map_rotation[4];    /* row-major 2 x 2 */
 } mppi_cost_params;
 
 typedef struct mppi_sampler_params {
@@ -228,6 +231,25 @@ int mppi_pure_noise_start(int32_t num_rollouts, float pure_noise_percentage, int
 int mppi_host_step(const mppi_dynamics_params* p, const float* x, const float* u, float dt, float* x_next,
                    float* x_dot, float* y);
 int mppi_host_enforce_constraints(const mppi_dynamics_params* p, float* u);
+/* Weight and map I/O from .npz archives (numpy.savez / savez_compressed; stored or deflate members, zip64) without a
+ * third-party loader.
+ * mppi_npz_read: array `key` (name without ".npy"; float32/64, int32/64, converted to FP32 in C order); out == NULL
+ * returns the shape (ndim <= 4, unused entries 0) and element count only. */
+int mppi_npz_read(const char* path, const char* key, float* out, int64_t max_count, int32_t* ndim, int64_t* shape,
+                  int64_t* count);
+/* FNNHelper::loadParams(prefix, npz) (utils/nn_helpers/fnn_helper.cu:44-127): layer sizes from dynamics_W1 /
+ * dynamics_b1 and dynamics_b2, ... (prefix "" or e.g. "model", a '/' appended as add_slash does), and the parameter
+ * vector in the FNN layout of mppi_dynamics_params (per layer W [out][in] row-major, then b [out]). theta == NULL
+ * returns the layer sizes and the parameter count only. */
+int mppi_fnn_load_npz(const char* path, const char* prefix, int32_t max_layers, int32_t* num_layers, int32_t* layers,
+                      float* theta, int32_t max_params, int32_t* num_params);
+/* ARStandardCost::loadTrackData's map (cost_functions/autorally/ar_standard_cost.cu:85-142): xBounds, yBounds,
+ * pixelsPerMeter give width = (x_max - x_min) ppm, height = (y_max - y_min) ppm, origin (x_min, y_min), resolution
+ * 1 / ppm; `channel` (NULL: "channel0") gives the [height][width] values in their stored order (the texture upload's
+ * rows). values == NULL returns the geometry only. */
+int mppi_track_map_load_npz(const char* path, const char* channel, float* values, int64_t max_count, int32_t* width,
+                            int32_t* height, float* origin_x, float* origin_y, float* resolution);
+
 /* Dynamics::enforceLeash: per state, the nominal value when it lies within leash_values[i] of the true one, else the
  * true value moved leash_values[i] towards it (dynamics/dynamics.cuh:445-463); the bicycle leashes its position in
  * the body frame of the true yaw and its yaw along the shortest angle (dynamics/racer_dubins/racer_dubins.cu:
@@ -365,6 +387,17 @@ int mppi_controller_get_weight_stats(const mppi_controller* ctrl, mppi_weight_st
 int mppi_controller_get_free_energy(const mppi_controller* ctrl, mppi_free_energy* out_host);
 /* updateImportanceSampler / setControlSeq: set the nominal [T][C] sequence. */
 int mppi_controller_set_control_seq(mppi_controller* ctrl, const float* u_host);
+/* Staged cost-map updates (TwoDTextureHelper::updateTexture / updateOrigin / updateResolution / updateRotation and
+ * copyToDevice, utils/texture_helpers/two_d_texture_helper.cu:21-60,88-101, texture_helper.cu:147-222,279-338):
+ * values [height][width] (or column-major [width][height] when column_major) and the map frame are copied into the
+ * controller's staging area at once (any thread) and applied at the start of its next computeControl (or, for the
+ * device-resident loop, its next mppi_controller_step_device). New values of the current extent are uploaded in place
+ * on the controller's stream, so captured graphs stay valid; a new extent or frame re-captures them. rotation: row-major
+ * 2 x 2 or NULL for the identity. Not available for plugin controllers. */
+int mppi_controller_update_cost_map(mppi_controller* ctrl, const float* values, int32_t width, int32_t height,
+                                    int32_t column_major);
+int mppi_controller_update_cost_map_frame(mppi_controller* ctrl, float origin_x, float origin_y, float resolution,
+                                          const float* rotation);
 /* slideControlSequence(steps) (controllers/controller.cuh:609-621); for the colored controller it also sets the leash
  * index to steps (ColoredMPPIController::slideControlSequence, colored_mppi_controller.cu:260-268). */
 int mppi_controller_slide_control_seq(mppi_controller* ctrl, int32_t steps);

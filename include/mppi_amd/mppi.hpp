@@ -317,6 +317,17 @@ class RacerDubinsNN : public RacerDubinsImpl<MPPI_MODEL_RACER_DUBINS_NN> {
     if ((int)theta.size() != desc_.nn_num_params) throw std::runtime_error("RacerDubinsNN: parameter count mismatch");
     theta_ = theta;
   }
+  // FNNHelper::loadParams(prefix, npz) (utils/nn_helpers/fnn_helper.cu:44-127) from an .npz file
+  void loadParams(const std::string& model_path, const std::string& prefix = "")
+  {
+    int32_t nl = 0, np = 0;
+    check(mppi_fnn_load_npz(model_path.c_str(), prefix.c_str(), 0, &nl, nullptr, nullptr, 0, &np), "loadParams");
+    std::vector<int32_t> layers(nl);
+    std::vector<float> theta(np);
+    check(mppi_fnn_load_npz(model_path.c_str(), prefix.c_str(), nl, nullptr, layers.data(), theta.data(), np, nullptr),
+          "loadParams");
+    updateModel(std::vector<int>(layers.begin(), layers.end()), theta);
+  }
   void setPrecision(int precision) { desc_.nn_precision = precision; }
   const std::vector<float>& getTheta() const { return theta_; }
   const float* hostParameters() const override { return theta_.data(); }
@@ -382,6 +393,8 @@ struct QuadraticMapCostParams {
   int map_x_index = 0, map_y_index = 1;
   float speed_coeff = 0.0f, speed_target = 0.0f;
   int speed_index = 0;
+  bool map_rotated = false;
+  float map_rotation[4] = {1.0f, 0.0f, 0.0f, 1.0f};  // row-major 2 x 2 map-frame rotation
   QuadraticMapCostParams()
   {
     for (int i = 0; i < DYN_T::OUTPUT_DIM; ++i) {
@@ -408,6 +421,55 @@ class QuadraticMapCost {
     height_ = height;
     ++map_version_;
   }
+  // TwoDTextureHelper's update interface (utils/texture_helpers/two_d_texture_helper.cu:21-60, texture_helper.cu:
+  // 279-316): new values of the current extent (row-major [height][width] or column-major [width][height]) or with a
+  // new extent, the map frame's origin, resolution and (planar, row-major 2 x 2) rotation. They take effect at the
+  // controller's next computeControl; values of an unchanged extent are uploaded in place.
+  void updateTexture(const std::vector<float>& values, bool column_major = false)
+  {
+    updateTexture(values, width_, height_, column_major);
+  }
+  void updateTexture(const std::vector<float>& values, int width, int height, bool column_major = false)
+  {
+    if (!column_major) {
+      setCostMap(values, width, height);
+      return;
+    }
+    if ((int)values.size() != width * height) throw std::runtime_error("updateTexture: bad map size");
+    std::vector<float> rm(values.size());
+    for (int j = 0; j < width; ++j)
+      for (int i = 0; i < height; ++i) rm[(size_t)i * width + j] = values[(size_t)j * height + i];
+    setCostMap(rm, width, height);
+  }
+  void updateOrigin(float x, float y)
+  {
+    params_.map_origin[0] = x;
+    params_.map_origin[1] = y;
+  }
+  void updateResolution(float resolution) { params_.map_resolution = resolution; }
+  void updateRotation(const std::array<float, 4>& rotation)
+  {
+    params_.map_rotated = true;
+    for (int i = 0; i < 4; ++i) params_.map_rotation[i] = rotation[i];
+  }
+  // ARStandardCost::loadTrackData (cost_functions/autorally/ar_standard_cost.cu:85-142): the map, origin and
+  // resolution of an .npz track map (xBounds, yBounds, pixelsPerMeter, channel)
+  void loadTrackData(const std::string& map_path, const std::string& channel = "channel0")
+  {
+    int32_t w = 0, h = 0;
+    float ox = 0.0f, oy = 0.0f, res = 1.0f;
+    check(mppi_track_map_load_npz(map_path.c_str(), channel.c_str(), nullptr, 0, &w, &h, &ox, &oy, &res),
+          "loadTrackData");
+    std::vector<float> v((size_t)w * h);
+    check(mppi_track_map_load_npz(map_path.c_str(), channel.c_str(), v.data(), (int64_t)v.size(), nullptr, nullptr,
+                                  nullptr, nullptr, nullptr),
+          "loadTrackData");
+    setCostMap(v, w, h);
+    updateOrigin(ox, oy);
+    updateResolution(res);
+  }
+  int getWidth() const { return width_; }
+  int getHeight() const { return height_; }
   // changes with every setCostMap: the controller uploads the new map on its next computeControl
   unsigned long long costMapVersion() const { return map_version_; }
   void GPUSetup() {}
@@ -435,6 +497,8 @@ class QuadraticMapCost {
     c.speed_coeff = params_.speed_coeff;
     c.speed_target = params_.speed_target;
     c.speed_index = params_.speed_index;
+    c.map_rotated = params_.map_rotated ? 1 : 0;
+    for (int i = 0; i < 4; ++i) c.map_rotation[i] = params_.map_rotation[i];
     return c;
   }
   const float* costMapHost() const { return map_.empty() ? nullptr : map_.data(); }
@@ -910,8 +974,8 @@ class VanillaMPPIController {
   }
 
   // Everything the device side depends on, as bytes: the parameter blocks (for plugin models the objects themselves,
-  // which are passed to the kernels by value), the controller parameters and the cost map's version.
-  std::string snapshot(const mppi_controller_desc& d) const
+  // which are passed to the kernels by value), the controller parameters and (with_map) the cost map's version.
+  std::string snapshot(const mppi_controller_desc& d, bool with_map = true) const
   {
     std::string s;
     auto add = [&s](const void* p, size_t n) { s.append(static_cast<const char*>(p), n); };
@@ -927,7 +991,7 @@ class VanillaMPPIController {
       add(static_cast<const void*>(model_), sizeof(DYN_T));
       add(static_cast<const void*>(cost_), sizeof(COST_T));
     } else {
-      const unsigned long long v = cost_->costMapVersion();
+      const unsigned long long v = with_map ? cost_->costMapVersion() : 0ull;
       add(&v, sizeof(v));
       if (d.goal_traj_host) add(d.goal_traj_host, sizeof(float) * d.cost.goal_traj_len * DYN_T::OUTPUT_DIM);
     }
@@ -975,11 +1039,22 @@ class VanillaMPPIController {
     if (ctrl_) {
       if (snap == snapshot_) return;
       if constexpr (!is_device_plugin<DYN_T>::value) {
+        // new values of an unchanged extent (and nothing else changed): staged in place, the graphs stay
+        if (cost_->costMapVersion() != map_version_ && cost_->costMapHost() != nullptr &&
+            snapshot(d, false) == snapshot_nomap_ && d.cost.map_width == map_w_ && d.cost.map_height == map_h_) {
+          check(mppi_controller_update_cost_map(ctrl_, cost_->costMapHost(), map_w_, map_h_, 0), "updateTexture");
+          map_version_ = cost_->costMapVersion();
+          snapshot_ = std::move(snap);
+          return;
+        }
         if (cost_->costMapVersion() != map_version_) d.cost_map_host = cost_->costMapHost();
         d.nn_theta_host = model_->hostParameters();
       }
       if (mppi_controller_set_params(ctrl_, &d) == MPPI_OK) {
         snapshot_ = std::move(snap);
+        snapshot_nomap_ = snapshot(d, false);
+        map_w_ = d.cost.map_width;
+        map_h_ = d.cost.map_height;
         if constexpr (!is_device_plugin<DYN_T>::value) map_version_ = cost_->costMapVersion();
         return;
       }
@@ -994,6 +1069,9 @@ class VanillaMPPIController {
       map_version_ = cost_->costMapVersion();
     }
     snapshot_ = std::move(snap);
+    snapshot_nomap_ = snapshot(d, false);
+    map_w_ = d.cost.map_width;
+    map_h_ = d.cost.map_height;
   }
   void release()
   {
@@ -1009,6 +1087,8 @@ class VanillaMPPIController {
  private:
   mppi_controller* ctrl_ = nullptr;
   std::string snapshot_;              // the parameters the device side was last given
+  std::string snapshot_nomap_;        // the same without the cost map's version
+  int map_w_ = 0, map_h_ = 0;         // the extent of the map the device side holds
   unsigned long long map_version_ = 0;
   std::shared_ptr<void> plugin_keep_;  // the plugin callback table's state (plugin models only)
   control_trajectory control_;

@@ -71,7 +71,8 @@ class CostParams(ct.Structure):
         ("map", ct.c_void_p), ("map_width", _i), ("map_height", _i), ("map_origin_x", _f), ("map_origin_y", _f),
         ("map_resolution", _f), ("map_coeff", _f), ("map_x_index", _i), ("map_y_index", _i),
         ("speed_coeff", _f), ("speed_target", _f), ("speed_index", _i),
-        ("goal_traj_len", _i), ("goal_traj", ct.c_void_p), ("goal_time", _i), ("_pad", _i),
+        ("goal_traj_len", _i), ("goal_traj", ct.c_void_p), ("goal_time", _i),
+        ("map_rotated", _i), ("map_rotation", _f * 4),
     ]
 
 
@@ -142,6 +143,11 @@ _SIGS = {
     "mppi_host_running_cost_at": ([ct.POINTER(CostParams), P, _i, ct.POINTER(_f)], ct.c_int),
     "mppi_host_enforce_leash": ([ct.POINTER(DynamicsParams), P, P, P, P], ct.c_int),
     "mppi_compute_weights_tsallis": ([P, _i, _f, _f, _f, P, P], ct.c_int),
+    "mppi_npz_read": ([ct.c_char_p, ct.c_char_p, P, ct.c_int64, P, P, P], ct.c_int),
+    "mppi_fnn_load_npz": ([ct.c_char_p, ct.c_char_p, _i, P, P, P, _i, P], ct.c_int),
+    "mppi_track_map_load_npz": ([ct.c_char_p, ct.c_char_p, P, ct.c_int64, P, P, P, P, P], ct.c_int),
+    "mppi_controller_update_cost_map": ([P, P, _i, _i, _i], ct.c_int),
+    "mppi_controller_update_cost_map_frame": ([P, _f, _f, _f, P], ct.c_int),
     "mppi_generate_normals": ([ct.c_uint64, ct.c_uint64, P, ct.c_int64, P], ct.c_int),
     "mppi_sample_gaussian": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),
     "mppi_sample_colored": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),
@@ -262,3 +268,43 @@ def pure_noise_start(num_rollouts: int, pct: float) -> int:
     out = _i()
     call("mppi_pure_noise_start", num_rollouts, pct, ct.byref(out))
     return out.value
+
+
+# ---------------------------------------------------------------------------------------------- weight / map I/O
+def npz_read(path: str, key: str):
+    """Array `key` of an .npz archive as float32 in C order (mppi_npz_read)."""
+    import numpy as np
+
+    nd, cnt = _i(), ct.c_int64()
+    shape = (ct.c_int64 * 4)()
+    call("mppi_npz_read", str(path).encode(), key.encode(), None, 0, ct.byref(nd), shape, ct.byref(cnt))
+    out = np.empty(cnt.value, np.float32)
+    call("mppi_npz_read", str(path).encode(), key.encode(), out.ctypes.data, cnt.value, None, None, None)
+    return out.reshape([shape[i] for i in range(nd.value)])
+
+
+def fnn_load_npz(path: str, prefix: str = ""):
+    """(layer sizes, FP32 parameter vector) of FNNHelper::loadParams(prefix, npz) (mppi_fnn_load_npz)."""
+    import numpy as np
+
+    nl, npar = _i(), _i()
+    call("mppi_fnn_load_npz", str(path).encode(), prefix.encode(), 0, ct.byref(nl), None, None, 0, ct.byref(npar))
+    layers = np.zeros(nl.value, np.int32)
+    theta = np.empty(npar.value, np.float32)
+    call("mppi_fnn_load_npz", str(path).encode(), prefix.encode(), nl.value, None, layers.ctypes.data,
+         theta.ctypes.data, npar.value, None)
+    return [int(v) for v in layers], theta
+
+
+def track_map_load_npz(path: str, channel: str = "channel0"):
+    """(values [height][width], (origin x, origin y), resolution) of a track map (mppi_track_map_load_npz)."""
+    import numpy as np
+
+    w, h = _i(), _i()
+    ox, oy, res = _f(), _f(), _f()
+    call("mppi_track_map_load_npz", str(path).encode(), channel.encode(), None, 0, ct.byref(w), ct.byref(h),
+         ct.byref(ox), ct.byref(oy), ct.byref(res))
+    vals = np.empty((h.value, w.value), np.float32)
+    call("mppi_track_map_load_npz", str(path).encode(), channel.encode(), vals.ctypes.data, vals.size, None, None,
+         None, None, None)
+    return vals, (ox.value, oy.value), res.value

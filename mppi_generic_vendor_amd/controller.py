@@ -158,6 +158,20 @@ class VanillaMPPIController:
         capi.call("mppi_controller_set_params", self._h, ct.byref(d))
         self.problem = problem
 
+    def updateCostMap(self, values: np.ndarray, column_major: bool = False):
+        """Stage new cost-map values (TwoDTextureHelper::updateTexture): [height][width], or [width][height] with
+        column_major; applied at the start of the next call (mppi_controller_update_cost_map)."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        h, w = (v.shape[1], v.shape[0]) if column_major else v.shape
+        capi.call("mppi_controller_update_cost_map", self._h, v.ctypes.data, int(w), int(h), int(column_major))
+
+    def updateCostMapFrame(self, origin, resolution: float, rotation=None):
+        """Stage a new map frame (updateOrigin / updateResolution / updateRotation): origin (x, y), resolution and an
+        optional row-major 2 x 2 rotation (mppi_controller_update_cost_map_frame)."""
+        rot = None if rotation is None else np.ascontiguousarray(rotation, dtype=np.float32).reshape(4)
+        capi.call("mppi_controller_update_cost_map_frame", self._h, float(origin[0]), float(origin[1]),
+                  float(resolution), None if rot is None else rot.ctypes.data)
+
     def setSeedCUDARandomNumberGen(self, seed: int):
         capi.call("mppi_controller_set_seed", self._h, ct.c_uint64(seed))
 

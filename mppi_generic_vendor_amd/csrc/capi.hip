@@ -278,6 +278,16 @@ struct mppi_controller {
   int goal_len = 0;
   float* w_d = nullptr;      // [K] weights of the Tsallis iteration (the costs stay in costs_d)
   int leash_jump = 1;        // the colored controller's leash index: the last slide's step count
+  // staged cost-map updates, applied at the start of the next call (map_mu guards the staging fields)
+  std::mutex map_mu;
+  std::vector<float> map_stage;  // [stage_h][stage_w]
+  int stage_w = 0, stage_h = 0;
+  bool stage_values = false, stage_frame = false;
+  float stage_origin[2] = {0.0f, 0.0f}, stage_res = 1.0f, stage_rot[4] = {1.0f, 0.0f, 0.0f, 1.0f};
+  int stage_rotated = 0;
+  float* map_pinned = nullptr;  // pinned source of the in-place upload
+  size_t map_pinned_n = 0;
+  hipEvent_t map_ev = nullptr;  // the last in-place upload (its source is reused only after it)
   // per-call exchange buffers: mapped fine-grained host memory (the *_hd members are their device-side addresses),
   // moved by copy_words launches on the controller's stream
   IterArgs* iter_h = nullptr;
@@ -747,7 +757,8 @@ static int controller_free(mppi_controller* c)
                  c->vis_mean_d, c->vis_idx_d, c->vis_buf_d, c->std_t_d, c->goal_d, c->w_d};
   for (void* p : dev)
     if (p) (void)hipFree(p);
-  void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h};
+  if (c->map_ev) (void)hipEventDestroy(c->map_ev);
+  void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h, c->map_pinned};
   for (void* p : host)
     if (p) (void)hipHostFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1079,6 +1090,96 @@ static void controller_set_iters(mppi_controller* c, int stride, int n)
   }
 }
 
+// Applies the staged cost-map updates: values of the current extent are copied in place on the controller's stream
+// (ordered after the work already queued there and before the call's own), so the captured graphs, which hold the map
+// pointer and frame by value, stay valid; a new extent or frame changes the cost block, so the graphs are dropped and
+// captured again on their next use.
+static int apply_staged_map(mppi_controller* c)
+{
+  std::lock_guard<std::mutex> lock(c->map_mu);
+  if (!c->stage_values && !c->stage_frame) return MPPI_OK;
+  mppi_cost_params& cost = c->d.cost;
+  const bool same_extent =
+      !c->stage_values || (c->map_d != nullptr && c->stage_w == cost.map_width && c->stage_h == cost.map_height);
+  if (c->stage_frame || !same_extent) {
+    MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+  }
+  if (c->stage_values) {
+    const size_t n = (size_t)c->stage_w * c->stage_h;
+    if (!same_extent) {
+      if (c->map_d) MPPI_HIP_CHECK(hipFree(c->map_d));
+      c->map_d = nullptr;
+      cost.map = nullptr;
+      MPPI_CHECK(dev_alloc(&c->map_d, sizeof(float) * n));
+    }
+    if (!c->map_ev) MPPI_HIP_CHECK(hipEventCreateWithFlags(&c->map_ev, hipEventDisableTiming));
+    else MPPI_HIP_CHECK(hipEventSynchronize(c->map_ev));  // the previous upload has read the pinned source
+    if (c->map_pinned_n < n) {
+      if (c->map_pinned) MPPI_HIP_CHECK(hipHostFree(c->map_pinned));
+      c->map_pinned = nullptr;
+      c->map_pinned_n = 0;
+      MPPI_CHECK(host_alloc(&c->map_pinned, sizeof(float) * n));
+      c->map_pinned_n = n;
+    }
+    std::memcpy(c->map_pinned, c->map_stage.data(), sizeof(float) * n);
+    MPPI_HIP_CHECK(hipMemcpyAsync(c->map_d, c->map_pinned, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    MPPI_HIP_CHECK(hipEventRecord(c->map_ev, c->stream));
+    cost.map = c->map_d;
+    cost.map_width = c->stage_w;
+    cost.map_height = c->stage_h;
+    c->stage_values = false;
+  }
+  if (c->stage_frame) {
+    cost.map_origin_x = c->stage_origin[0];
+    cost.map_origin_y = c->stage_origin[1];
+    cost.map_resolution = c->stage_res;
+    cost.map_rotated = c->stage_rotated;
+    for (int i = 0; i < 4; ++i) cost.map_rotation[i] = c->stage_rot[i];
+    c->stage_frame = false;
+  }
+  return MPPI_OK;
+}
+
+int mppi_controller_update_cost_map(mppi_controller* c, const float* values, int32_t w, int32_t h,
+                                    int32_t column_major)
+{
+  if (!c || !values) return fail(MPPI_E_INVALID, "null argument");
+  if (c->has_plugin) return fail(MPPI_E_UNSUPPORTED, "cost map update: plugin controllers have no built-in map");
+  if (w < 2 || h < 2) return fail(MPPI_E_INVALID, "cost map update: width/height must be >= 2");
+  if ((int64_t)w * h > ((int64_t)1 << 30)) return fail(MPPI_E_SIZE, "cost map update: map too large");
+  std::vector<float> v((size_t)w * h);
+  if (column_major) {
+    for (int j = 0; j < w; ++j)
+      for (int i = 0; i < h; ++i) v[(size_t)i * w + j] = values[(size_t)j * h + i];
+  } else {
+    std::memcpy(v.data(), values, sizeof(float) * v.size());
+  }
+  std::lock_guard<std::mutex> lock(c->map_mu);
+  c->map_stage.swap(v);
+  c->stage_w = w;
+  c->stage_h = h;
+  c->stage_values = true;
+  return MPPI_OK;
+}
+
+int mppi_controller_update_cost_map_frame(mppi_controller* c, float origin_x, float origin_y, float resolution,
+                                          const float* rotation)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  if (c->has_plugin) return fail(MPPI_E_UNSUPPORTED, "cost map update: plugin controllers have no built-in map");
+  if (!(resolution > 0.0f)) return fail(MPPI_E_INVALID, "cost map update: resolution must be > 0");
+  std::lock_guard<std::mutex> lock(c->map_mu);
+  c->stage_origin[0] = origin_x;
+  c->stage_origin[1] = origin_y;
+  c->stage_res = resolution;
+  c->stage_rotated = rotation != nullptr;
+  const float id[4] = {1.0f, 0.0f, 0.0f, 1.0f};
+  for (int i = 0; i < 4; ++i) c->stage_rot[i] = rotation ? rotation[i] : id[i];
+  c->stage_frame = true;
+  return MPPI_OK;
+}
+
 int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, int32_t stride)
 {
   if (!c || !x0_host) return fail(MPPI_E_INVALID, "null argument");
@@ -1089,6 +1190,7 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
     const int n = (d.sampler.fft_len_mode == MPPI_FFT_LEN_T) ? T : 2 * T;
     if (stride >= n) return fail(MPPI_E_INVALID, "colored sampler: optimization_stride out of range");
   }
+  MPPI_CHECK(apply_staged_map(c));
   c->fe.previous_baseline = c->stats.baseline;
   const bool colored_ctrl = d.controller_kind == MPPI_CONTROLLER_COLORED;
   if (colored_ctrl && d.leash_active && c->computed && !c->has_plugin) {
@@ -1272,6 +1374,7 @@ int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
   if (!c->computed)
     return fail(MPPI_E_INVALID, "step_device: call computeControl once first (the loop starts from its x0 and mean)");
   if (num_calls == 0) return MPPI_OK;
+  MPPI_CHECK(apply_staged_map(c));
   if (!c->loop_exec) {
     MPPI_CHECK(loop_args_init(c));
     MPPI_CHECK(controller_enqueue_loop(c));  // eager pass (sets kernel attributes before capture)
@@ -1481,7 +1584,14 @@ int mppi_controller_set_params(mppi_controller* c, const mppi_controller_desc* d
       }
       MPPI_HIP_CHECK(hipMemcpy(c->map_d, desc->cost_map_host, sizeof(float) * nm, hipMemcpyHostToDevice));
     }
-    if (c->map_d != nullptr && (desc->cost_map_host != nullptr || n.cost.map == nullptr)) n.cost.map = c->map_d;
+    if (c->map_d != nullptr && desc->cost_map_host == nullptr && n.cost.map == nullptr) {
+      // the controller's own copy stays, with its own extent (the one it was uploaded with)
+      n.cost.map = c->map_d;
+      n.cost.map_width = o.cost.map_width;
+      n.cost.map_height = o.cost.map_height;
+    } else if (c->map_d != nullptr && desc->cost_map_host != nullptr) {
+      n.cost.map = c->map_d;
+    }
     if (n.dynamics.model == MPPI_MODEL_RACER_DUBINS_NN) {
       if (desc->nn_theta_host != nullptr) {
         c->nn_host.assign(desc->nn_theta_host, desc->nn_theta_host + c->nn_host.size());

@@ -313,18 +313,26 @@ struct Model<MPPI_MODEL_RACER_DUBINS_NN> : Bicycle {
 
 // ---------------------------------------------------------------------------------------------------------- costs
 // Bilinear cost-map query with the reference's CPU semantics (utils/texture_helpers/two_d_texture_helper.cu:151-245
-// and the world->texture transform of texture_helper.cu:94-134 with identity rotation): clamp address mode,
-// texel centres at +0.5, x0 = min(floor(q), W - 2).
+// and the world->texture transform of texture_helper.cu:94-134, its rotation restricted to the plane): clamp address
+// mode, texel centres at +0.5, x0 = min(floor(q), W - 2).
 MPPI_HD float map_query(const mppi_cost_params& c, float px, float py)
 {
   const int W = c.map_width, H = c.map_height;
-  // texel coordinate (p - origin) / resolution - 0.5 (the reference's normalise-by-extent and scale-back cancel);
-  // the reciprocal of the uniform resolution is computed once per kernel
+  // map frame R (p - origin) (worldPoseToMapPose; R uniform, so the branch is too)
+  float dx = px - c.map_origin_x, dy = py - c.map_origin_y;
+  if (c.map_rotated) {
+    const float rx = c.map_rotation[0] * dx + c.map_rotation[1] * dy;
+    const float ry = c.map_rotation[2] * dx + c.map_rotation[3] * dy;
+    dx = rx;
+    dy = ry;
+  }
+  // texel coordinate m / resolution - 0.5 (the reference's normalise-by-extent and scale-back cancel); the reciprocal
+  // of the uniform resolution is computed once per kernel
   const float inv_res = 1.0f / c.map_resolution;
   // clamp to [0, W - 1] as the reference's if/else chain does for every non-NaN value; a NaN query maps to texel 0
   // instead of reaching the index computation
-  const float qx = fminf(fmaxf((px - c.map_origin_x) * inv_res - 0.5f, 0.0f), (float)(W - 1));
-  const float qy = fminf(fmaxf((py - c.map_origin_y) * inv_res - 0.5f, 0.0f), (float)(H - 1));
+  const float qx = fminf(fmaxf(dx * inv_res - 0.5f, 0.0f), (float)(W - 1));
+  const float qy = fminf(fmaxf(dy * inv_res - 0.5f, 0.0f), (float)(H - 1));
   const int x0 = min((int)floorf(qx), W - 2), x1 = x0 + 1;
   const int y0 = min((int)floorf(qy), H - 2), y1 = y0 + 1;
   const float* row0 = c.map + (size_t)y0 * W;

@@ -221,9 +221,17 @@ template <class R>
 static R map_query(const mppi_cost_params& c, const float* map, R px, R py)
 {
   const int W = c.map_width, H = c.map_height;
-  /* world -> map (identity rotation) -> normalised texture coordinate -> texel */
-  R nx = ((px - (R)c.map_origin_x) / (R)c.map_resolution) / (R)W;
-  R ny = ((py - (R)c.map_origin_y) / (R)c.map_resolution) / (R)H;
+  /* world -> map frame R (p - origin) (worldPoseToMapPose, texture_helper.cu:94-104, in the plane) -> normalised
+   * texture coordinate -> texel */
+  R mx = px - (R)c.map_origin_x, my = py - (R)c.map_origin_y;
+  if (c.map_rotated) {
+    const R rx = (R)c.map_rotation[0] * mx + (R)c.map_rotation[1] * my;
+    const R ry = (R)c.map_rotation[2] * mx + (R)c.map_rotation[3] * my;
+    mx = rx;
+    my = ry;
+  }
+  R nx = (mx / (R)c.map_resolution) / (R)W;
+  R ny = (my / (R)c.map_resolution) / (R)H;
   R qx = nx * (R)W - (R)0.5;
   R qy = ny * (R)H - (R)0.5;
   if (qx > (R)(W - 1)) qx = (R)(W - 1);

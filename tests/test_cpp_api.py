@@ -12,7 +12,7 @@ PREBUILT = ROOT / "tests" / "cpp" / "_build" / "cartpole_closed_loop"
 
 
 def test_cpp_api_compiles_with_gcc(tmp_path, engine):
-    for name in ("cartpole_closed_loop", "plant_loop", "tracking_loop", "colored_loop"):
+    for name in ("cartpole_closed_loop", "plant_loop", "tracking_loop", "colored_loop", "map_io"):
         assert build_program(name, tmp_path).exists()
 
 
@@ -106,6 +106,31 @@ def test_cpp_colored_controller(tmp_path, hip, engine):
     exe = COLORED if COLORED.exists() else build_program("colored_loop", tmp_path)
     print("[gpu] C++ colored controller", flush=True)
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0, res.stdout + res.stderr
+
+
+MAP_IO = ROOT / "tests" / "cpp" / "_build" / "map_io"
+
+
+@pytest.mark.gpu
+def test_cpp_weight_and_map_io(tmp_path, hip, engine):
+    """RacerDubinsNN::loadParams and QuadraticMapCost::loadTrackData from .npz files numpy writes here, and in-place
+    updateTexture under a running controller (tests/cpp/map_io.cpp)."""
+    from test_io import _reference_track_map
+
+    d, off = {}, 0
+    for i, (nin, nout) in enumerate(zip([9, 64, 64], [64, 64, 4]), start=1):
+        d[f"dynamics_W{i}"] = np.arange(off, off + nin * nout, dtype=np.float64).reshape(nout, nin)
+        off += nin * nout
+        d[f"dynamics_b{i}"] = np.arange(off, off + nout, dtype=np.float64)
+        off += nout
+    np.savez_compressed(tmp_path / "net.npz", **d)
+    np.savez(tmp_path / "track.npz", **_reference_track_map())
+    exe = MAP_IO if MAP_IO.exists() else build_program("map_io", tmp_path)
+    print("[gpu] C++ weight and map I/O", flush=True)
+    res = subprocess.run([str(exe), str(tmp_path / "net.npz"), str(tmp_path / "track.npz")], capture_output=True,
+                         text=True, timeout=240)
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stdout + res.stderr
 
