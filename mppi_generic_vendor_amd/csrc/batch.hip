@@ -44,12 +44,31 @@ struct BatchInstance {
   int32_t _pad;
 };
 
+// The final column sums run on up to SUM_WAVES wavefronts, each with a [64][8 C + 1] block of LDS.
+constexpr int SUM_WAVES = 4;
+__host__ __device__ constexpr int sum_block_floats(int C) { return 64 * (8 * C + 1); }
+
 // LDS floats of one workgroup: the work-item slot (4 floats), then per tile the mean, the sample ring
-// [3][chunk][R][C], the outputs [2][chunk][R][O], the consumer partial sums, terminal costs, costs and weights.
+// [3][chunk][R][C] and the outputs [2][chunk][R][O] (their room, at least the column sums' blocks, is reused by those
+// blocks after the rollout), the consumer partial sums, terminal costs, costs and weights.
 static size_t batch_lds_floats(int TCmax, int Cmax, int Omax, int NT, int R, int ch)
 {
   const int ncons = NT / 64 - 1;
-  return 4 + (size_t)((TCmax + 3) & ~3) + 3ull * ch * R * Cmax + 2ull * ch * R * Omax + (size_t)ncons * R + 3ull * R;
+  const size_t rollout = 3ull * ch * R * Cmax + 2ull * ch * R * Omax;
+  const size_t sums = (size_t)std::min(NT / 64, SUM_WAVES) * sum_block_floats(Cmax);
+  return 4 + (size_t)((TCmax + 3) & ~3) + std::max(rollout, sums) + (size_t)ncons * R + 3ull * R;
+}
+
+// v from lane (lane ^ MASK) for MASK = 8 (DPP row rotation by 8 inside each row of 16), 16 and 32 (row-swap
+// permutes); every lane of the wavefront must be active.
+template <int MASK>
+__device__ __forceinline__ float xor_partner(float v)
+{
+  static_assert(MASK == 8 || MASK == 16 || MASK == 32, "lane distance");
+  if constexpr (MASK == 8)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
+  else
+    return partner<MASK == 16 ? 4 : 5>(v);
 }
 
 template <int MODEL, int COST, int NT, int SPL>
@@ -71,7 +90,8 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
   float* mu = lds;
   float* ring = mu + ((TC + 3) & ~3);     // [3][ch][R][C]
   float* ybuf = ring + 3 * ch * R * C;    // [2][ch][R][O]
-  float* part = ybuf + 2 * ch * R * O;    // [NCONS][R]
+  // [NCONS][R], after the larger of the rollout buffers and the column sums' blocks (batch_lds_floats)
+  float* part = ring + max(3 * ch * R * C + 2 * ch * R * O, min(NW, SUM_WAVES) * sum_block_floats(C));
   float* term = part + NCONS * R;         // [R]
   float* cst = term + R;                  // [R]
   float* wl = cst + R;                    // [R]
@@ -83,32 +103,42 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
   __syncthreads();
 
   // Samples of chunk j into ring slot j % 3, by threads t_off .. t_off + nth - 1. Element (k, t, c) is normal
-  // k*TC + t*C + c of the instance's stream; thread item (r, q) evaluates Philox block q of row r's range.
+  // k*TC + t*C + c of the instance's stream; thread item (r, q) evaluates Philox block q of row r's range. Element
+  // indices stay below K T C <= 2^26 (BATCH_MAX_K, MPPI_MAX_TIMESTEPS, MPPI_MAX_CONTROL), so 32-bit arithmetic holds
+  // them and the Philox counter's high word is 0.
+  float sdc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) sdc[c] = decay * sp.std_dev[c];
+  const float* sdt = sp.std_dev_t;
   auto gen = [&](int j, int t_off, int nth) {
     const int t0 = j * ch, t1 = min(T, t0 + ch);
     // Philox blocks per row: one more when the row's range does not start on a block boundary (k0 * TC is a
     // multiple of 4 since R is, so only TC and t0 * C decide)
     const bool aligned = (TC % 4 == 0) && ((t0 * C) % 4 == 0);
     const int nb = ((t1 - t0) * C + 3) / 4 + (aligned ? 0 : 1);
+    // item / nb as (item + 1/2) / nb rounded down in FP32: exact, since the quotient lies at least 1/(2 nb) from an
+    // integer and item < 256 nb
+    const float inv_nb = 1.0f / (float)nb;
     float* slot = ring + (j % 3) * ch * R * C;
     for (int item = tid - t_off; item < R * nb; item += nth) {
-      const int r = item / nb, q = item - r * nb;
-      const int64_t kk = k0 + r;
-      const int64_t es = kk * TC + (int64_t)t0 * C, ee = kk * TC + (int64_t)t1 * C;
-      const int64_t b = (es >> 2) + q;
+      const int r = (int)(((float)item + 0.5f) * inv_nb), q = item - r * nb;
+      const int kk = k0 + r;
+      const int es = kk * TC + t0 * C, ee = kk * TC + t1 * C;
+      const int b = (es >> 2) + q;
       if (b > ((ee - 1) >> 2)) continue;
-      const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)b);
+      const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(uint32_t)b);
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
-        const int64_t e = 4 * b + jj;
+        const int e = 4 * b + jj;
         if (e < es || e >= ee) continue;
-        const int loc = (int)(e - kk * TC);
+        const int loc = e - kk * TC;
         const int t = loc / C, c = loc - t * C;
         const float m = mu[loc];
+        const float sd = sdt ? decay * sdt[loc] : sdc[c];
         float v;
         if (kk == 0 || t < a.stride) v = m;
-        else if (kk >= I.pstart) v = decay * step_std_dev(sp, t, c, C) * z.z[jj];
-        else v = m + decay * step_std_dev(sp, t, c, C) * z.z[jj];
+        else if (kk >= I.pstart) v = sd * z.z[jj];
+        else v = m + sd * z.z[jj];
         slot[((t - t0) * R + r) * C + c] = v;
       }
     }
@@ -246,18 +276,45 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
     }
   }
   __syncthreads();
-  // S_g[t][c] = sum_r w_r u[t][r][c]: wavefront per step, lanes over samples
-  for (int t = wave; t < T; t += NW) {
+  // S_g[t][c] = sum_r w_r u[t][r][c], P = 8 C outputs (8 steps) per wavefront pass: every lane issues the loads of its
+  // rows for the 8 steps at once (one memory latency per pass instead of one per output) and scales them by the rows'
+  // weights; the 64 x P block goes through LDS (row stride P + 1) so that lane (q, h) adds rows h P .. h P + P - 1 of
+  // output q, and lane exchanges add the 64 / P row groups. The rows of a tile are summed in a fixed order
+  // (deterministic).
+  constexpr int P = 8 * C, NSUM = NW < SUM_WAVES ? NW : SUM_WAVES;
+  if (wave < NSUM) {
+    float* blk = ring + wave * sum_block_floats(C);
+    const int q = lane % P, h = lane / P;
+    for (int t0 = wave * 8; t0 < T; t0 += NSUM * 8) {
+      float v[P];
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      float v = 0.0f;
+      for (int i = 0; i < P; ++i) v[i] = 0.0f;
 #pragma unroll
       for (int s = 0; s < SPL; ++s) {
         const int r = lane + 64 * s;
-        v += wl[r] * utile[((int64_t)t * R + r) * C + c];
+        const float wr = wl[r];
+        const float* src = utile + ((int64_t)t0 * R + r) * C;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if (t0 + j < T) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) v[j * C + c] += wr * src[(int64_t)j * R * C + c];
+          }
+        }
       }
-      v = wave_sum(v);
-      if (lane == 0) sums[I.sum_base + (int64_t)g * TC + t * C + c] = v;
+#pragma unroll
+      for (int i = 0; i < P; ++i) blk[lane * (P + 1) + i] = v[i];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      float acc_q = 0.0f;
+#pragma unroll
+      for (int i = 0; i < P; ++i) acc_q += blk[(h * P + i) * (P + 1) + q];
+      if constexpr (P <= 8) acc_q += xor_partner<8>(acc_q);
+      if constexpr (P <= 16) acc_q += xor_partner<16>(acc_q);
+      acc_q += xor_partner<32>(acc_q);
+      if (h == 0 && t0 * C + q < TC) sums[I.sum_base + (int64_t)g * TC + t0 * C + q] = acc_q;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
   }
 }

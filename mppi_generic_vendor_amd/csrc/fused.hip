@@ -160,12 +160,13 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
                                  samples);
 }
 
-// Combination of G partials: workgroup q owns COMBINE_COLS columns; every workgroup recomputes the global baseline
-// and the scales e_g = exp(-(b_g - b)/lambda) (G <= MAX_PARTIALS values), workgroup 0 also writes the statistics.
-// Each thread's first COMBINE_PREFETCH column values are requested before the partials, so that their latency
-// overlaps the baseline and normalizer reductions.
-constexpr int COMBINE_COLS = 8;
-constexpr int COMBINE_THREADS = 256;
+// Combination of G partials: workgroup q owns COMBINE_COLS consecutive columns (a 128-byte segment of every partial
+// row, so each cache line of the rows is fetched by one workgroup), read by COMBINE_THREADS / COMBINE_COLS row groups;
+// every workgroup recomputes the global baseline and the scales e_g = exp(-(b_g - b)/lambda) (G <= MAX_PARTIALS
+// values), workgroup 0 also writes the statistics. Each thread's first COMBINE_PREFETCH column values are requested
+// before the partials, so that their latency overlaps the baseline and normalizer reductions.
+constexpr int COMBINE_COLS = 32;
+constexpr int COMBINE_THREADS = 1024;
 constexpr int COMBINE_PREFETCH = 8;
 
 __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial* __restrict__ parts,
@@ -179,6 +180,7 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   __shared__ int si[COMBINE_THREADS / 64];
   __shared__ double se[COMBINE_THREADS / 64], se2[COMBINE_THREADS / 64], sec[COMBINE_THREADS / 64];
   __shared__ float acc_l[COMBINE_THREADS / COMBINE_COLS][COMBINE_COLS];
+  static_assert(MAX_PARTIALS % COMBINE_THREADS == 0, "partials per thread");
   constexpr int PER = MAX_PARTIALS / COMBINE_THREADS;
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const double inv = 1.0 / (double)lambda;
