@@ -167,8 +167,13 @@ def run_batch(args):
     probs, m = workloads.batch_mix(256, seed=7)
     KT = sum(p.num_rollouts * p.num_timesteps for p in probs)
     b = MPPIBatch(probs, cost_map=m)
-    cfg = b.autotune(reps=3)
-    log(f"batch256: {sum(p.num_rollouts for p in probs)} samples, {KT} sample-steps per call; autotuned "
+    if args.batch_shape:
+        # a fixed launch shape (profiled runs: the autotuner's timing is perturbed under a profiler)
+        b.setConfig(*[int(v) for v in args.batch_shape.split(",")])
+        cfg = b.getConfig()
+    else:
+        cfg = b.autotune(reps=3)
+    log(f"batch256: {sum(p.num_rollouts for p in probs)} samples, {KT} sample-steps per call; "
         f"threads={cfg.threads} samples/lane={cfg.samples_per_lane} chunk={cfg.chunk} blocks/CU={cfg.blocks_per_cu}")
     x0 = [p.x0 for p in probs]
     for _ in range(args.warmup):
@@ -366,6 +371,8 @@ def main():
     ap.add_argument("--host-calls", type=int, default=200,
                     help="timed computeControl calls of each configuration in `configs`")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--batch-shape", default="",
+                    help="threads,samples_per_lane,chunk of the batched kernel instead of the autotuner's choice")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time of the headline workload")
     ap.add_argument("--config-cpu-seconds", type=float, default=3.0,
                     help="CPU-baseline time of each configuration in `configs`")
