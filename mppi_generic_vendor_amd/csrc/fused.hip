@@ -160,21 +160,35 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
                                  samples);
 }
 
-// Combination of G partials: workgroup q owns COMBINE_COLS consecutive columns (a 128-byte segment of every partial
-// row, so each cache line of the rows is fetched by one workgroup), read by COMBINE_THREADS / COMBINE_COLS row groups;
-// every workgroup recomputes the global baseline and the scales e_g = exp(-(b_g - b)/lambda) (G <= MAX_PARTIALS
-// values), workgroup 0 also writes the statistics. Each thread's first COMBINE_PREFETCH column values are requested
-// before the partials, so that their latency overlaps the baseline and normalizer reductions.
-constexpr int COMBINE_COLS = 32;
-constexpr int COMBINE_THREADS = 1024;
+// Combination of G partials: workgroup q owns COLS consecutive columns, read by THREADS / COLS row groups; every
+// workgroup recomputes the global baseline and the scales e_g = exp(-(b_g - b)/lambda) (G <= MAX_PARTIALS values),
+// workgroup 0 also writes the statistics. Each thread's first COMBINE_PREFETCH column values are requested before the
+// partials, so that their latency overlaps the baseline and normalizer reductions. The kernel is latency-bound (one
+// dependent memory round trip, two block reductions): the launcher takes 8 columns on 256 threads up to 256 partials
+// and 16 columns on 512 threads above, the fastest measured shapes; wider column segments on more threads fetch fewer
+// bytes (each row line by fewer workgroups) but take longer.
 constexpr int COMBINE_PREFETCH = 8;
+constexpr int NUM_XCDS = 8;
 
-__global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial* __restrict__ parts,
-                                                                  const float* __restrict__ sums, int G, int K, int TC,
-                                                                  float lambda, mppi_weight_stats* __restrict__ stats,
-                                                                  float* __restrict__ mean_out, IterArgs* advance,
-                                                                  int advance_step, ResultCopies copies)
+// Workgroup w of a grid of n is dispatched to XCD w % 8 (round-robin). The logical block it takes is chosen so that
+// each XCD gets a contiguous run of blocks -- XCD x takes blocks start(x) .. start(x) + count(x) - 1 -- and the
+// workgroups whose column segments share a cache line of the partial rows sit behind the same L2.
+__device__ __forceinline__ int xcd_contiguous_block(int w, int n)
 {
+  const int x = w % NUM_XCDS, slot = w / NUM_XCDS;
+  const int base = n / NUM_XCDS, extra = n % NUM_XCDS;
+  return x * base + min(x, extra) + slot;
+}
+
+template <int COLS, int THREADS>
+__global__ void __launch_bounds__(THREADS) combine_kernel(const Partial* __restrict__ parts,
+                                                          const float* __restrict__ sums, int G, int K, int TC,
+                                                          float lambda, mppi_weight_stats* __restrict__ stats,
+                                                          float* __restrict__ mean_out, IterArgs* advance,
+                                                          int advance_step, ResultCopies copies)
+{
+  constexpr int COMBINE_COLS = COLS;
+  constexpr int COMBINE_THREADS = THREADS;
   __shared__ float escale[MAX_PARTIALS];
   __shared__ float sv[COMBINE_THREADS / 64];
   __shared__ int si[COMBINE_THREADS / 64];
@@ -187,7 +201,7 @@ __global__ void __launch_bounds__(COMBINE_THREADS) combine_kernel(const Partial*
   // columns of this workgroup: thread (grp, col) sums partials g = grp, grp + NG, ...
   constexpr int NG = COMBINE_THREADS / COMBINE_COLS;
   const int col = tid % COMBINE_COLS, grp = tid / COMBINE_COLS;
-  const int j = blockIdx.x * COMBINE_COLS + col;
+  const int j = xcd_contiguous_block(blockIdx.x, gridDim.x) * COMBINE_COLS + col;
   float pre[COMBINE_PREFETCH];
 #pragma unroll
   for (int i = 0; i < COMBINE_PREFETCH; ++i) {
@@ -452,9 +466,13 @@ int launch_combine(const Partial* parts, const float* sums, int G, int K, int TC
                    const ResultCopies* copies, hipStream_t s)
 {
   if (G < 1 || G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "combine: partial count out of range");
-  const int grid = (TC + COMBINE_COLS - 1) / COMBINE_COLS;
-  combine_kernel<<<grid, COMBINE_THREADS, 0, s>>>(parts, sums, G, K, TC, lambda, stats, mean_out, advance,
-                                                  advance_step, copies != nullptr ? *copies : ResultCopies{});
+  const ResultCopies rc = copies != nullptr ? *copies : ResultCopies{};
+  if (G <= 256)
+    combine_kernel<8, 256><<<(TC + 7) / 8, 256, 0, s>>>(parts, sums, G, K, TC, lambda, stats, mean_out, advance,
+                                                        advance_step, rc);
+  else
+    combine_kernel<16, 512><<<(TC + 15) / 16, 512, 0, s>>>(parts, sums, G, K, TC, lambda, stats, mean_out, advance,
+                                                           advance_step, rc);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
