@@ -620,6 +620,48 @@ int mppi_controller_get_last_call_ms(const mppi_controller* ctrl, float* ms);
 int mppi_controller_time_kernels(mppi_controller* ctrl, int32_t reps, float* rollout_ms, float* combine_ms,
                                  int32_t* num_partials);
 
+/* ------------------------------------------------------------------------------------------------------------------
+ * Two-system rollouts and the Tube-MPPI / RMPPI iterations: a pair of systems (distributions 0 and 1, each with its
+ * own initial state and mean) rolled out together over the same sample index, as the reference's rollout launches
+ * with blockDim.z = 2 (controllers/Tube-MPPI/tube_mppi_controller.cu:158-236, core/rmppi_kernels.cu:359-798).
+ * ---------------------------------------------------------------------------------------------------------------- */
+#define MPPI_TWO_SYSTEM_TUBE 0  /* two independent rollouts (rolloutKernel with z = 2, core/mppi_common.cu:35-153) */
+#define MPPI_TWO_SYSTEM_RMPPI 1 /* nominal + feedback-tracking real system (rolloutRMPPIKernel, rmppi_kernels.cu:666) */
+
+typedef struct mppi_two_system_params {
+  int32_t mode;                   /* MPPI_TWO_SYSTEM_TUBE or MPPI_TWO_SYSTEM_RMPPI */
+  int32_t nominal_index;          /* RMPPI: index (0 or 1) of the nominal system; the other one is the real system */
+  float value_function_threshold; /* RMPPI: value_func_threshold of the nominal cost */
+  int32_t _pad;
+  /* RMPPI feedback gains, device [T][S][C]: the real system's control is u + K_t (x_real - x_nominal) with
+   * (K_t e)_j = sum_i gains[(t S + i) C + j] e_i (DDPFeedbackState::fb_gain_traj_ read through its Eigen map,
+   * feedback_controllers/DDP/ddp.cuh:30-40,144-151). NULL: no feedback. */
+  const float* feedback_gains;
+} mppi_two_system_params;
+
+/* Two-system rollout. x0_d [2][S] initial states, mean_d [2][T][C] the means the samples were drawn around,
+ * samples_d [2][K][T][C] updated in place with the applied controls (the real system's with its feedback added; all
+ * constrained), costs_d [2][K] (system d's sample k at d K + k).
+ *   TUBE : cost_d = (sum_t [running(y_d,t) + lr_d(u_d,t)] ) / T + terminal(y_d) / T for each system alone -- equal to
+ *          two mppi_rollout calls.
+ *   RMPPI (n = nominal_index, r = 1 - n; rmppi_kernels.cu:487-664,666-798; the CPU form of
+ *          tests/include/kernel_tests/core/rmppi_kernel_test.cu:7-70):
+ *          fb_t = K_t (x_r - x_n); u_r += fb_t; both constrained; both stepped;
+ *          cost_r = (sum_t [running(y_r) + lr_r(u_r)] + terminal(y_r)) / T,
+ *          track_r = (sum_t [running(y_r) + fb_cost(fb_t)] + terminal(y_r)) / T,
+ *          state_n = (sum_t running(y_n) + terminal(y_n)) / T,  lr_n = sum_t lr_n(u_n) / T,
+ *          cost_n = 0.5 state_n + 0.5 max(min(track_r, value_function_threshold), state_n) + lr_n,
+ *          fb_cost = 0.5 lambda (1 - alpha) sum_c coeff_c fb_c^2 / sigma_c^2 (computeFeedbackCost, gaussian.cu:577-630).
+ * lr_d and fb_cost take their standard deviations (std_dev, or std_dev_t at step t) from distribution d's sampler:
+ * sampler0 for d = 0, sampler1 for d = 1 (NULL: sampler0); every other sampler field (control_cost_coeff, the
+ * pure-noise share) comes from sampler0, the one GaussianParams of both distributions (gaussian.cuh:21-42).
+ * Built-in models except the BF16 network. */
+int mppi_rollout_two_system(const mppi_dynamics_params* dyn, const mppi_cost_params* cost,
+                            const mppi_sampler_params* sampler0, const mppi_sampler_params* sampler1,
+                            const mppi_two_system_params* params, int32_t num_rollouts, int32_t num_timesteps, float dt,
+                            float lambda, float alpha, const float* x0_d, const float* mean_d, float* samples_d,
+                            float* costs_d, void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -41,6 +41,8 @@ FFT_LEN_T = 1
 
 CONTROLLER_VANILLA = 0
 CONTROLLER_COLORED = 1
+TWO_SYSTEM_TUBE = 0
+TWO_SYSTEM_RMPPI = 1
 
 NN_FP32 = 0
 NN_BF16 = 1
@@ -84,6 +86,11 @@ class SamplerParams(ct.Structure):
         ("exponents", _f * MAX_CONTROL), ("offset_decay_rate", _f), ("fmin", _f),
         ("fft_len_mode", _i), ("_pad", _i), ("seed", ct.c_uint64), ("std_dev_t", ct.c_void_p),
     ]
+
+
+class TwoSystemParams(ct.Structure):
+    _fields_ = [("mode", _i), ("nominal_index", _i), ("value_function_threshold", _f), ("_pad", _i),
+                ("feedback_gains", ct.c_void_p)]
 
 
 class WeightStats(ct.Structure):
@@ -153,6 +160,9 @@ _SIGS = {
     "mppi_sample_colored": ([ct.POINTER(SamplerParams), _i, _i, P, P, _i, _i, ct.c_uint64, P, P], ct.c_int),
     "mppi_rollout": ([ct.POINTER(DynamicsParams), ct.POINTER(CostParams), ct.POINTER(SamplerParams), _i, _i, _f, _f,
                       _f, P, P, P, P, P], ct.c_int),
+    "mppi_rollout_two_system": ([ct.POINTER(DynamicsParams), ct.POINTER(CostParams), ct.POINTER(SamplerParams),
+                                 ct.POINTER(SamplerParams), ct.POINTER(TwoSystemParams), _i, _i, _f, _f, _f, P, P, P,
+                                 P, P], ct.c_int),
     "mppi_compute_weights": ([P, _i, _f, P, P], ct.c_int),
     "mppi_weighted_update": ([P, P, P, _i, _i, _i, P, P], ct.c_int),
     "mppi_weighted_update_noalloc": ([P, P, P, _i, _i, _i, P, P], ct.c_int),

@@ -640,6 +640,33 @@ int mppi_rollout(const mppi_dynamics_params* dyn, const mppi_cost_params* cost, 
                         (hipStream_t)stream);
 }
 
+int mppi_rollout_two_system(const mppi_dynamics_params* dyn, const mppi_cost_params* cost,
+                            const mppi_sampler_params* sampler0, const mppi_sampler_params* sampler1,
+                            const mppi_two_system_params* params, int32_t K, int32_t T, float dt, float lambda,
+                            float alpha, const float* x0_d, const float* mean_d, float* samples_d, float* costs_d,
+                            void* stream)
+{
+  if (!dyn || !cost || !sampler0 || !params || !x0_d || !mean_d || !samples_d || !costs_d)
+    return fail(MPPI_E_INVALID, "null argument");
+  MPPI_CHECK(validate_sizes(K, T));
+  MPPI_CHECK(validate_dynamics(*dyn));
+  MPPI_CHECK(validate_cost(*cost, *dyn, true));
+  MPPI_CHECK(validate_sampler(*sampler0, *dyn));
+  if (dyn->model == MPPI_MODEL_RACER_DUBINS_NN && dyn->nn_theta == nullptr)
+    return fail(MPPI_E_INVALID, "residual MLP: nn_theta is null");
+  // distribution 1 contributes only its standard deviations
+  mppi_sampler_params sp1 = *sampler0;
+  if (sampler1 != nullptr) {
+    std::memcpy(sp1.std_dev, sampler1->std_dev, sizeof(sp1.std_dev));
+    sp1.std_dev_t = sampler1->std_dev_t;
+    MPPI_CHECK(validate_sampler(sp1, *dyn));
+  }
+  if (params->mode == MPPI_TWO_SYSTEM_RMPPI && !(params->value_function_threshold == params->value_function_threshold))
+    return fail(MPPI_E_INVALID, "two-system rollout: value_function_threshold is NaN");
+  return launch_two_system(*dyn, *cost, *sampler0, sp1, *params, K, T, dt, lambda, alpha, x0_d, mean_d, samples_d,
+                           costs_d, (hipStream_t)stream);
+}
+
 int mppi_compute_weights(float* costs_d, int32_t K, float lambda, mppi_weight_stats* stats_d, void* stream)
 {
   if (!costs_d || !stats_d) return fail(MPPI_E_INVALID, "null argument");

@@ -32,6 +32,11 @@ int launch_compute_weights(float* costs, int K, float lambda, mppi_weight_stats*
 int launch_weights(const float* costs, float* w_out, int K, float lambda, float gamma, float r,
                    mppi_weight_stats* stats, mppi_weight_stats* stats_copy, IterArgs* advance, int advance_step,
                    hipStream_t s);
+// Two-system rollout (Tube / RMPPI) over x0[2][S], mean[2][T][C], samples[2][K][T][C] (in place), costs[2][K].
+int launch_two_system(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp0,
+                      const mppi_sampler_params& sp1, const mppi_two_system_params& tp, int K, int T, float dt,
+                      float lambda, float alpha, const float* x0, const float* mean, float* samples, float* costs,
+                      hipStream_t s);
 // `ws` provides weighted_update_workspace_bytes(K, T, C) bytes of device memory.
 size_t weighted_update_workspace_bytes(int K, int T, int C);
 int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,

@@ -1,0 +1,314 @@
+// Two-system rollouts (Tube-MPPI, RMPPI): for every sample index k, system 0 and system 1 are rolled out together
+// over the horizon by one workgroup lane, as the reference's rollout kernels with blockDim.z = 2 do
+// (core/mppi_common.cu:35-153 for Tube-MPPI, controllers/Tube-MPPI/tube_mppi_controller.cu:158-236;
+// rolloutRMPPIKernel / rolloutRMPPIDynamicsKernel + rolloutRMPPICostKernel, core/rmppi_kernels.cu:359-798).
+//
+// One workgroup of TS_THREADS threads takes R samples (64, 32 or 16) of both distributions:
+//   - both control tiles [R][T C] and both means are staged in LDS (rows padded to T C + 1 floats), and for RMPPI the
+//     feedback gains [T][S][C];
+//   - the producer wavefront (one sample per lane) carries the two state chains: per step it forms the real system's
+//     feedback fb_t = K_t (x_r - x_n) (RMPPI), constrains both controls, writes them back into the tiles (the applied
+//     controls the weighted update averages), steps both systems and hands their outputs (and fb_t) to the consumers
+//     through a double-buffered LDS chunk;
+//   - the 15 consumer wavefronts evaluate the running, likelihood-ratio and feedback costs of the previous chunk,
+//     thread (w, lane) taking sample `lane` and the steps t0 + w, t0 + w + 15, ...;
+//   - wavefront 0 adds the consumer sums in a fixed order (deterministic) and forms the two costs; the tiles go back to
+//     the sample buffer with 16-byte stores.
+// Per sample-step the kernel reads and writes 2 C floats of each system (16 C bytes) -- the HBM traffic of the
+// reference's two rollouts -- and its VALU work is two dynamics steps and two cost evaluations.
+#include <algorithm>
+
+#include "iteration_core.hpp"
+#include "kernels.hpp"
+#include "models.hpp"
+#include "rollout_core.hpp"
+#include "tile.hpp"
+
+namespace mppi_amd {
+
+constexpr int TS_THREADS = RC_THREADS;
+constexpr int TS_CONSUMERS = TS_THREADS / 64 - 1;
+constexpr int TS_ACC = 4;  // consumer accumulators per lane (Tube uses 2)
+
+struct TwoSystemArgs {
+  float vft;  // RMPPI: value-function threshold
+  const float* gains;
+};
+
+// LDS floats for R rows and `chunk` steps per hand-off: tiles, means, gains (RMPPI), output chunks [2][chunk][64][2][O|1],
+// feedback chunks [2][chunk][64][C] (RMPPI), consumer partial sums [TS_ACC][15][64], terminal costs [2][64].
+template <int S, int C, int O>
+static size_t ts_lds_floats(int T, int R, int chunk, bool rmppi)
+{
+  const size_t TC = (size_t)T * C;
+  const size_t tiles = (2 * (size_t)R * (TC + 1) + 3) & ~(size_t)3;
+  const size_t means = 2 * ((TC + 3) & ~(size_t)3);
+  const size_t gains = rmppi ? (((size_t)T * S * C + 3) & ~(size_t)3) : 0;
+  const size_t outs = 2 * (size_t)chunk * 64 * 2 * (O | 1);
+  const size_t fbs = rmppi ? 2 * (size_t)chunk * 64 * C : 0;
+  return tiles + means + gains + outs + fbs + (size_t)TS_ACC * TS_CONSUMERS * 64 + 2 * 64;
+}
+
+// computeFeedbackCost (sampling_distributions/gaussian/gaussian.cu:577-630): 0.5 lambda (1 - alpha) sum_c coeff_c
+// fb_c^2 / sigma_c^2 with the distribution's standard deviations (std_dev, or the step's row of std_dev_t).
+template <int C>
+MPPI_HD float feedback_cost(const mppi_sampler_params& s, const float* fb, float lambda, float alpha, int t)
+{
+  float cost = 0.0f;
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const float sd = s.std_dev_t ? s.std_dev_t[t * C + i] : s.std_dev[i];
+    cost += s.control_cost_coeff[i] * (fb[i] * fb[i]) / (sd * sd);
+  }
+  return 0.5f * lambda * (1.0f - alpha) * cost;
+}
+
+// NOM = -1: Tube (two independent systems); NOM = 0 or 1: RMPPI with system NOM nominal and 1 - NOM real (compile-time,
+// so that the two state chains stay in registers).
+template <int MODEL, int COST, int NOM>
+__global__ void __launch_bounds__(TS_THREADS) two_system_rollout_kernel(
+    mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp0, mppi_sampler_params sp1,
+    TwoSystemArgs ta, int K, int T, float dt, float lambda, float alpha, const float* __restrict__ x0,
+    const float* __restrict__ mean, float* __restrict__ samples, float* __restrict__ costs, int pstart, int R,
+    int chunk)
+{
+  using M = BuiltinModel<MODEL>;
+  using CF = BuiltinCost<COST, M::O>;
+  constexpr int S = M::S, C = M::C, O = M::O, OP = O | 1;
+  constexpr bool RMPPI = NOM >= 0;
+  constexpr int nom = RMPPI ? NOM : 1, real = 1 - nom;
+  extern __shared__ float lds[];
+  const int TC = T * C, ROW = TC + 1, TC4 = (TC + 3) & ~3;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int k0 = blockIdx.x * R;
+  const int nrows = min(R, K - k0);
+  float* tile = lds;                                             // [2][R][ROW]
+  float* mu = tile + ((2 * R * ROW + 3) & ~3);                   // [2][TC4]
+  float* gl = mu + 2 * TC4;                                      // [T][S][C]
+  float* ybuf = gl + (RMPPI ? ((T * S * C + 3) & ~3) : 0);       // [2][chunk][64][2][OP]
+  float* fbuf = ybuf + 2 * chunk * 64 * 2 * OP;                  // [2][chunk][64][C]
+  float* part = fbuf + (RMPPI ? 2 * chunk * 64 * C : 0);         // [TS_ACC][15][64]
+  float* term = part + TS_ACC * TS_CONSUMERS * 64;               // [2][64]
+
+  for (int d = 0; d < 2; ++d)
+    tile_load<TS_THREADS>(samples + ((size_t)d * K + k0) * TC, tile + d * R * ROW, nrows, TC, ROW, tid);
+  for (int j = tid; j < 2 * TC; j += TS_THREADS) {
+    const int d = j >= TC ? 1 : 0;
+    mu[d * TC4 + (j - d * TC)] = mean[j];
+  }
+  if constexpr (RMPPI) {
+    for (int j = tid; j < T * S * C; j += TS_THREADS) gl[j] = ta.gains ? ta.gains[j] : 0.0f;
+  }
+  bool deadband = false;
+#pragma unroll
+  for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
+  __syncthreads();
+
+  // lanes beyond the tile's rows repeat its last row and write nothing back
+  const int lrow = min(lane, R - 1);
+  const bool writer = lane < R;
+  float* row0 = tile + lrow * ROW;
+  float* row1 = tile + R * ROW + lrow * ROW;
+  const int nch = (T + chunk - 1) / chunk;
+
+  float x[2][S], y[2][O];
+  if (wave == 0) {
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) x[d][i] = x0[d * S + i];
+      M::output(dyn, x[d], y[d]);
+    }
+  }
+  float acc[TS_ACC];
+#pragma unroll
+  for (int q = 0; q < TS_ACC; ++q) acc[q] = 0.0f;
+  const bool pure = (k0 + lrow) >= pstart;
+
+  for (int it = 0; it <= nch; ++it) {
+    if (wave == 0 && it < nch) {
+      const int t0 = it * chunk, t1 = min(T, t0 + chunk);
+      float* yb = ybuf + (it & 1) * chunk * 64 * 2 * OP + lane * 2 * OP;
+      float* fbb = fbuf + (it & 1) * chunk * 64 * C + lane * C;
+      for (int t = t0; t < t1; ++t) {
+        float u[2][C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          u[0][c] = row0[t * C + c];
+          u[1][c] = row1[t * C + c];
+        }
+        if constexpr (RMPPI) {
+          // fb_t = K_t (x_r - x_n); the gains of step t are uniform LDS reads
+          float fb[C];
+#pragma unroll
+          for (int c = 0; c < C; ++c) fb[c] = 0.0f;
+#pragma unroll
+          for (int i = 0; i < S; ++i) {
+            const float e = x[real][i] - x[nom][i];
+#pragma unroll
+            for (int c = 0; c < C; ++c) fb[c] += gl[(t * S + i) * C + c] * e;
+          }
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            u[real][c] += fb[c];
+            fbb[(t - t0) * 64 * C + c] = fb[c];
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          if (deadband) enforce_constraints<C>(dyn, u[d]);
+          else clamp_controls<C>(dyn, u[d]);
+        }
+        if (writer) {
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            row0[t * C + c] = u[0][c];
+            row1[t * C + c] = u[1][c];
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          float xd[S], xn[S];
+          M::deriv(dyn, x[d], u[d], xd);
+          M::update(dyn, x[d], xn, xd, dt);
+          M::output(dyn, xn, y[d]);
+#pragma unroll
+          for (int i = 0; i < O; ++i) yb[(t - t0) * 64 * 2 * OP + d * OP + i] = y[d][i];
+#pragma unroll
+          for (int i = 0; i < S; ++i) x[d][i] = xn[i];
+        }
+      }
+      if (it == nch - 1) {
+        term[lane] = CF::terminal(cost, y[0]);
+        term[64 + lane] = CF::terminal(cost, y[1]);
+      }
+    } else if (wave > 0 && it > 0) {
+      const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
+      const float* yb = ybuf + ((it - 1) & 1) * chunk * 64 * 2 * OP + lane * 2 * OP;
+      const float* fbb = fbuf + ((it - 1) & 1) * chunk * 64 * C + lane * C;
+      for (int t = t0 + wave - 1; t < t1; t += TS_CONSUMERS) {
+        const float* y0 = yb + (t - t0) * 64 * 2 * OP;
+        const float* u0 = row0 + t * C;
+        const float* u1 = row1 + t * C;
+        if constexpr (!RMPPI) {
+          acc[0] += CF::running(cost, y0, u0, t) + lr_cost<C>(sp0, u0, mu + t * C, pure, lambda, alpha, t);
+          acc[1] += CF::running(cost, y0 + OP, u1, t) + lr_cost<C>(sp1, u1, mu + TC4 + t * C, pure, lambda, alpha, t);
+        } else {
+          const float* yr = y0 + real * OP;
+          const float* yn = y0 + nom * OP;
+          const float* ur = real == 0 ? u0 : u1;
+          const float* un = real == 0 ? u1 : u0;
+          const mppi_sampler_params& spr = real == 0 ? sp0 : sp1;
+          const mppi_sampler_params& spn = real == 0 ? sp1 : sp0;  // compile-time selections
+          const float rc_r = CF::running(cost, yr, ur, t);
+          acc[0] += rc_r + lr_cost<C>(spr, ur, mu + real * TC4 + t * C, pure, lambda, alpha, t);
+          acc[1] += rc_r + feedback_cost<C>(spr, fbb + (t - t0) * 64 * C, lambda, alpha, t);
+          acc[2] += CF::running(cost, yn, un, t);
+          acc[3] += lr_cost<C>(spn, un, mu + nom * TC4 + t * C, pure, lambda, alpha, t);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int q = 0; q < TS_ACC; ++q) part[(q * TS_CONSUMERS + wave - 1) * 64 + lane] = acc[q];
+  }
+  for (int d = 0; d < 2; ++d)
+    tile_store<TS_THREADS>(samples + ((size_t)d * K + k0) * TC, tile + d * R * ROW, nrows, TC, ROW, tid);
+  __syncthreads();
+  if (wave == 0) {
+    float s[TS_ACC];
+#pragma unroll
+    for (int q = 0; q < TS_ACC; ++q) {
+      s[q] = 0.0f;
+#pragma unroll
+      for (int w = 0; w < TS_CONSUMERS; ++w) s[q] += part[(q * TS_CONSUMERS + w) * 64 + lane];
+    }
+    const float inv_t = 1.0f / (float)T;
+    float c0, c1;
+    if constexpr (!RMPPI) {
+      c0 = s[0] / (float)T + term[lane] / (float)T;
+      c1 = s[1] / (float)T + term[64 + lane] / (float)T;
+    } else {
+      const float tr = term[real * 64 + lane], tn = term[nom * 64 + lane];
+      const float cost_r = (s[0] + tr) * inv_t;
+      const float track_r = (s[1] + tr) * inv_t;
+      const float state_n = (s[2] + tn) * inv_t;
+      const float lr_n = s[3] * inv_t;
+      const float cost_n = 0.5f * state_n + 0.5f * fmaxf(fminf(track_r, ta.vft), state_n) + lr_n;
+      c0 = real == 0 ? cost_r : cost_n;
+      c1 = real == 0 ? cost_n : cost_r;
+    }
+    if (lane < nrows) {
+      costs[k0 + lane] = c0;
+      costs[K + k0 + lane] = c1;
+    }
+  }
+}
+
+// Samples per workgroup: the smallest of 64, 32, 16 whose grid still fits the CUs once (a small K spreads over more
+// CUs), then halved until the tiles and buffers fit the LDS; the chunk is the largest of 16, 8, 4 that fits.
+template <int MODEL, int COST, int NOM>
+static int launch_two_system_t(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                               const mppi_sampler_params& sp0, const mppi_sampler_params& sp1, const TwoSystemArgs& ta,
+                               int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
+                               float* samples, float* costs, hipStream_t s)
+{
+  using M = BuiltinModel<MODEL>;
+  constexpr bool RMPPI = NOM >= 0;
+  int cus = 0, dev = 0;
+  MPPI_HIP_CHECK(hipGetDevice(&dev));
+  MPPI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int R = fused_rows_for(K, cus > 0 ? cus : 256), chunk = 0;
+  const size_t limit = LDS_LIMIT / sizeof(float);
+  for (;;) {
+    for (int ch = 16; ch >= 4 && chunk == 0; ch >>= 1)
+      if (ts_lds_floats<M::S, M::C, M::O>(T, R, ch, RMPPI) <= limit) chunk = ch;
+    if (chunk != 0 || R == 16) break;
+    R /= 2;
+  }
+  if (chunk == 0) return fail(MPPI_E_SIZE, "two-system rollout: horizon too long for the LDS control tiles");
+  const size_t lds = sizeof(float) * ts_lds_floats<M::S, M::C, M::O>(T, R, chunk, RMPPI);
+  MPPI_CHECK(allow_max_dynamic_lds((const void*)two_system_rollout_kernel<MODEL, COST, NOM>));
+  const int pstart = pure_noise_start(K, sp0.pure_noise_trajectories_percentage);
+  const int grid = (K + R - 1) / R;
+  two_system_rollout_kernel<MODEL, COST, NOM><<<grid, TS_THREADS, lds, s>>>(
+      dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean, samples, costs, pstart, R, chunk);
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
+int launch_two_system(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp0,
+                      const mppi_sampler_params& sp1, const mppi_two_system_params& tp, int K, int T, float dt,
+                      float lambda, float alpha, const float* x0, const float* mean, float* samples, float* costs,
+                      hipStream_t s)
+{
+  if (tp.mode != MPPI_TWO_SYSTEM_TUBE && tp.mode != MPPI_TWO_SYSTEM_RMPPI)
+    return fail(MPPI_E_INVALID, "two-system rollout: mode must be MPPI_TWO_SYSTEM_TUBE or MPPI_TWO_SYSTEM_RMPPI");
+  if (tp.nominal_index != 0 && tp.nominal_index != 1)
+    return fail(MPPI_E_INVALID, "two-system rollout: nominal_index must be 0 or 1");
+  if (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
+    return fail(MPPI_E_UNSUPPORTED, "two-system rollout: the BF16 network model is not supported (use MPPI_NN_FP32)");
+  const TwoSystemArgs ta{tp.value_function_threshold, tp.feedback_gains};
+  const int nom = tp.mode == MPPI_TWO_SYSTEM_RMPPI ? tp.nominal_index : -1;
+#define MPPI_TS_CASE(MODEL, COST)                                                                                   \
+  if (dyn.model == MODEL && cost.cost == COST) {                                                                   \
+    if (nom < 0)                                                                                                   \
+      return launch_two_system_t<MODEL, COST, -1>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean,      \
+                                                  samples, costs, s);                                              \
+    if (nom == 0)                                                                                                  \
+      return launch_two_system_t<MODEL, COST, 0>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean,       \
+                                                 samples, costs, s);                                               \
+    return launch_two_system_t<MODEL, COST, 1>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean,         \
+                                               samples, costs, s);                                                 \
+  }
+  MPPI_TS_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
+  MPPI_TS_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
+  MPPI_TS_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)
+  MPPI_TS_CASE(MPPI_MODEL_RACER_DUBINS_NN, MPPI_COST_QUADRATIC_MAP)
+#undef MPPI_TS_CASE
+  return fail(MPPI_E_UNSUPPORTED, "two-system rollout: model/cost combination is not compiled into the kernel library");
+}
+
+}  // namespace mppi_amd

@@ -27,6 +27,9 @@
  *   - weights / normalizer    core/mppi_common.cu:894-1002,1091-1117
  *   - weighted update         core/mppi_common.cu:1151-1196
  *   - SG smoothing / slide    controllers/controller.cuh:573-621
+ *   - two-system rollouts     Tube-MPPI: controllers/Tube-MPPI/tube_mppi_controller.cu:158-236 (two independent
+ *                             rollouts); RMPPI: tests/include/kernel_tests/core/rmppi_kernel_test.cu:7-70,
+ *                             core/rmppi_kernels.cu:666-798, feedback cost gaussian.cu:632-647 (rollout_loop.hpp)
  *   - state trajectory        controllers/controller.cuh:664-684
  *   - Philox4x32-10           Salmon et al., "Parallel random numbers: as easy as 1, 2, 3", SC'11
  *   - Box-Muller              Box & Muller 1958
@@ -513,6 +516,26 @@ void oracle_rollout_f64(const mppi_dynamics_params* dyn, const mppi_cost_params*
   /* FP64 run of the same loop; the clamped controls written back are rounded to float. */
   const BuiltinSystem<double> sys{*dyn, *cst, map};
   rollout_loop_threads<double>(sys, *smp, K, T, dt, lambda, alpha, x0, mean, samples, costs, nthreads);
+}
+
+/* Two-system rollout (two_system_loop, rollout_loop.hpp): smp[2] are the two distributions' samplers, nom < 0 Tube,
+ * 0 / 1 RMPPI with that nominal index; gains host [T][S][C] or NULL; parts [K][4] or NULL. */
+void oracle_rollout_two_system_f32(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,
+                                   const mppi_sampler_params* smp, int nom, float vft, const float* gains, int K,
+                                   int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
+                                   float* samples, float* costs, float* parts, int nthreads)
+{
+  const BuiltinSystem<float> sys{*dyn, *cst, map};
+  const int nt = std::max(1, nthreads), chunk = (K + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (int w = 0; w < nt; ++w) {
+    const int b = w * chunk, e = std::min(K, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([&, b, e]() {
+      two_system_loop<float>(sys, smp, nom, vft, gains, K, T, dt, lambda, alpha, x0, mean, samples, costs, parts, b, e);
+    });
+  }
+  for (auto& t : th) t.join();
 }
 
 void oracle_rollout_trace_f32(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,

@@ -54,6 +54,8 @@ _SIGS = {
                             ct.POINTER(capi.SamplerParams), _i, _i, _f, _f, _f, P, P, P, P, _i], None),
     "oracle_rollout_f64": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
                             ct.POINTER(capi.SamplerParams), _i, _i, _d, _d, _d, P, P, P, P, _i], None),
+    "oracle_rollout_two_system_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P, P, _i, _f,
+                                       P, _i, _i, _f, _f, _f, P, P, P, P, P, _i], None),
     "oracle_rollout_trace_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
                                   ct.POINTER(capi.SamplerParams), _i, _i, _f, _f, _f, P, P, P, P, P, P, P], None),
     "oracle_rollout_trace_f64": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
@@ -142,6 +144,34 @@ def rollout_f32(prob, samples: np.ndarray, mean: np.ndarray, nthreads: int = 1, 
                               prob.lambda_, prob.alpha, ptr(_a0),
                               ptr(_a1), ptr(s), ptr(costs), nthreads)
     return costs, s
+
+
+def rollout_two_system(prob, samples: np.ndarray, means: np.ndarray, x0s: np.ndarray, nominal_index: int = -1,
+                       vft: float = 0.0, gains: np.ndarray | None = None, std_dev_1=None, std_dev_t_1=None,
+                       nthreads: int = 8, parts: bool = False):
+    """Two-system rollout (Tube: nominal_index < 0; RMPPI: 0 or 1) of samples [2][K][T][C] around means [2][T][C]
+    from x0s [2][S]: returns (costs [2][K], applied controls [2][K][T][C]) and with `parts` also RMPPI's
+    (cost_r, track_r, state_n, lr_n) [K][4]. Distribution 1's sampler is prob's with std_dev = std_dev_1 and
+    std_dev_t = std_dev_t_1 when given."""
+    K, T, C = samples.shape[1], samples.shape[2], samples.shape[3]
+    s = np.ascontiguousarray(samples, dtype=np.float32).copy()
+    costs = np.empty((2, K), dtype=np.float32)
+    pr = np.zeros((K, 4), np.float32)
+    smp = (capi.SamplerParams * 2)()
+    s0 = _host_smp(prob)
+    std_t1 = getattr(prob, "std_dev_t", None) if std_dev_t_1 is None else std_dev_t_1
+    s1 = host_sampler(prob.sampler, std_t1)
+    if std_dev_1 is not None:
+        for c in range(C):
+            s1.std_dev[c] = float(std_dev_1[c])
+    smp[0], smp[1] = s0, s1
+    g = None if gains is None else np.ascontiguousarray(gains, np.float32)
+    _a0 = np.ascontiguousarray(x0s, np.float32)
+    _a1 = np.ascontiguousarray(means, np.float32)
+    load().oracle_rollout_two_system_f32(ct.byref(_host_dyn(prob)), ct.byref(_host_cost(prob)), ptr(prob.cost_map),
+                                         smp, int(nominal_index), float(vft), ptr(g), K, T, prob.dt, prob.lambda_,
+                                         prob.alpha, ptr(_a0), ptr(_a1), ptr(s), ptr(costs), ptr(pr), nthreads)
+    return (costs, s, pr) if parts else (costs, s)
 
 
 def rollout_f64(prob, samples: np.ndarray, mean: np.ndarray, x0=None, nthreads: int = 8):

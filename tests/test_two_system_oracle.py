@@ -1,0 +1,133 @@
+"""The oracle's two-system rollout (Tube-MPPI and RMPPI, oracle/rollout_loop.hpp two_system_loop) pinned on the CPU
+against the already-pinned single-system rollout and closed forms:
+  - Tube mode equals two independent single-system rollouts bit for bit (each system with its own x0, mean and
+    distribution std_dev);
+  - RMPPI without feedback, both systems from the same state over the same samples, gives the MPPI rollout's cost for
+    both systems (the reference's ValidateCombinedRMPPIRolloutKernelAgainstMPPIRollout,
+    tests/mppi_core/rmppi_kernel_tests.cu:506-591, there within 1e-3);
+  - the value-function threshold's limits: far below every cost the nominal cost is its plain rollout cost; far
+    above, with alpha = 1 (no control costs), it is 0.5 A + 0.5 max(B, A) with A, B the plain costs of the nominal
+    and the real system;
+  - on the double integrator with wide control limits the applied feedback satisfies the error recursion
+    e_{t+1} = e_t + dt [e_vel; K_t e_t] exactly in the closed form, for non-symmetric gains [T][S][C] (a transposed
+    gain layout or a flipped error sign fails), and the tracking cost exceeds the real cost by the quadratic
+    feedback cost 0.5 lambda sum_c coeff_c fb_c^2 / sigma_c^2 of those controls (computeFeedbackCost,
+    sampling_distributions/gaussian/gaussian.cu:632-647), evaluated in numpy."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from mppi_generic_vendor_amd import capi, workloads
+
+
+def _di(K=48, T=40, map_coeff=1.0):
+    prob = workloads.double_integrator_map(num_rollouts=K, num_timesteps=T, colored=False,
+                                           cost_map=workloads.gaussian_blob_map(64, 64, blobs=5, seed=3))
+    prob.cost.map_resolution = 0.2
+    prob.cost.map_origin_x = prob.cost.map_origin_y = -6.4
+    prob.cost.map_coeff = map_coeff
+    prob.sampler.control_cost_coeff[0], prob.sampler.control_cost_coeff[1] = 0.4, 0.9
+    prob.sampler.pure_noise_trajectories_percentage = 0.1
+    return prob
+
+
+def _inputs(prob, rng, same=False):
+    K, T, C, S = prob.num_rollouts, prob.num_timesteps, prob.C, prob.S
+    u = rng.normal(size=(2, K, T, C)).astype(np.float32)
+    means = (rng.normal(size=(2, T, C)) * 0.4).astype(np.float32)
+    x0s = rng.uniform(-1, 1, size=(2, S)).astype(np.float32)
+    if same:
+        u[1], means[1], x0s[1] = u[0], means[0], x0s[0]
+    return u, means, x0s
+
+
+def test_tube_equals_two_single_rollouts(oracle):
+    prob = _di()
+    rng = np.random.default_rng(1)
+    u, means, x0s = _inputs(prob, rng)
+    std1 = np.array([0.7, 1.6], np.float32)
+    costs, applied = ol.rollout_two_system(prob, u, means, x0s, -1, std_dev_1=std1)
+    for d in range(2):
+        if d == 1:
+            prob.sampler.std_dev[0], prob.sampler.std_dev[1] = float(std1[0]), float(std1[1])
+        want, want_u = ol.rollout_f32(prob, u[d], means[d], x0=x0s[d])
+        assert np.array_equal(costs[d], want), d
+        assert np.array_equal(applied[d], want_u), d
+
+
+@pytest.mark.parametrize("nom", [0, 1])
+def test_rmppi_without_feedback_is_the_mppi_rollout(oracle, nom):
+    prob = _di()
+    rng = np.random.default_rng(2 + nom)
+    u, means, x0s = _inputs(prob, rng, same=True)
+    costs, applied = ol.rollout_two_system(prob, u, means, x0s, nom, vft=20.0)
+    want, want_u = ol.rollout_f32(prob, u[0], means[0], x0=x0s[0])
+    for d in range(2):
+        np.testing.assert_allclose(costs[d], want, rtol=2e-6, atol=1e-7)
+        assert np.array_equal(applied[d], want_u)
+
+
+def test_rmppi_value_function_threshold_limits(oracle):
+    prob = _di()
+    rng = np.random.default_rng(4)
+    u, means, x0s = _inputs(prob, rng)
+    x0s[1] = x0s[0]  # the same start: the two systems' plain costs then differ either way over their own samples
+    nom, real = 0, 1
+    # the plain costs of the two systems over their own samples
+    plain_n, _ = ol.rollout_f32(prob, u[nom], means[nom], x0=x0s[nom])
+    costs, _ = ol.rollout_two_system(prob, u, means, x0s, nom, vft=-1e30)
+    np.testing.assert_allclose(costs[nom], plain_n, rtol=2e-6, atol=1e-7)
+    prob.alpha = 1.0
+    A, _ = ol.rollout_f32(prob, u[nom], means[nom], x0=x0s[nom])
+    B, _ = ol.rollout_f32(prob, u[real], means[real], x0=x0s[real])
+    costs, _ = ol.rollout_two_system(prob, u, means, x0s, nom, vft=1e30)
+    np.testing.assert_allclose(costs[nom], 0.5 * A + 0.5 * np.maximum(B, A), rtol=2e-6, atol=1e-7)
+    np.testing.assert_allclose(costs[real], B, rtol=2e-6, atol=1e-7)
+    assert np.any(B > A) and np.any(B < A)  # both branches of the max occur
+
+
+def test_rmppi_feedback_closed_form(oracle):
+    """Double integrator x = (px, py, vx, vy), u = (ax, ay), Euler step dt: with the same samples for both systems
+    and no clamping, u_r - u_n = K_t e_t and e_{t+1} = (e_pos + dt e_vel, e_vel + dt K_t e_t)."""
+    T, K = 25, 16
+    prob = _di(K=K, T=T, map_coeff=0.0)
+    for c in range(2):
+        prob.dyn.control_min[c], prob.dyn.control_max[c] = -1e6, 1e6
+    rng = np.random.default_rng(5)
+    u, means, x0s = _inputs(prob, rng)
+    u[1] = u[0]
+    means[1] = means[0]
+    gains = rng.normal(size=(T, 4, 2)).astype(np.float32) * 0.8
+    for nom in (0, 1):
+        real = 1 - nom
+        _, applied = ol.rollout_two_system(prob, u, means, x0s, nom, vft=5.0, gains=gains)
+        e = (x0s[real] - x0s[nom]).astype(np.float64)
+        dt = float(np.float32(prob.dt))
+        for t in range(T):
+            fb = np.array([sum(gains[t, i, j] * e[i] for i in range(4)) for j in range(2)])
+            np.testing.assert_allclose(applied[real][:, t] - applied[nom][:, t], np.broadcast_to(fb, (K, 2)),
+                                       rtol=1e-4, atol=1e-4 * (1 + np.abs(fb).max()))
+            np.testing.assert_array_equal(applied[nom][:, t], u[nom][:, t])
+            e = np.array([e[0] + dt * e[2], e[1] + dt * e[3], e[2] + dt * fb[0], e[3] + dt * fb[1]])
+    # the tracking cost exceeds the real cost by the feedback cost minus the real likelihood-ratio cost
+    prob.sampler.pure_noise_trajectories_percentage = 0.0
+    zero = np.zeros_like(means)
+    costs, applied, parts = ol.rollout_two_system(prob, u, zero, x0s, 0, vft=5.0, gains=gains, parts=True)
+    fbv = (applied[1] - applied[0]).astype(np.float64)  # [K][T][C]
+    cc = np.array([prob.sampler.control_cost_coeff[c] for c in range(2)])
+    sd = np.array([prob.sampler.std_dev[c] for c in range(2)])
+    fb_cost = 0.5 * prob.lambda_ * (1 - prob.alpha) * (cc * fbv ** 2 / sd ** 2).sum(axis=2).sum(axis=1) / T
+    # with zero means the likelihood-ratio cost vanishes, so track_r - cost_r is the feedback cost alone
+    np.testing.assert_allclose(parts[:, 1] - parts[:, 0], fb_cost, rtol=1e-4, atol=1e-5)
+    assert fb_cost.min() > 0
+
+
+def test_two_system_known_sizes_and_modes(oracle, engine):
+    """The kernel-level entry point validates its arguments on the host (no GPU call is made for these)."""
+    prob = _di(K=8, T=4, map_coeff=0.0)
+    prob.cost.map = None
+    for mode, nom in ((5, 0), (capi.TWO_SYSTEM_RMPPI, 2)):
+        tp = capi.TwoSystemParams(mode, nom, 1.0, 0, None)
+        with pytest.raises(RuntimeError, match="two-system"):
+            capi.call("mppi_rollout_two_system", prob.dyn, prob.cost, prob.sampler, None, tp, 8, 4, 0.02, 1.0, 0.0,
+                      1, 1, 1, 1, None)
