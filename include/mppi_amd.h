@@ -662,6 +662,60 @@ int mppi_rollout_two_system(const mppi_dynamics_params* dyn, const mppi_cost_par
                             float lambda, float alpha, const float* x0_d, const float* mean_d, float* samples_d,
                             float* costs_d, void* stream);
 
+/* Two-system controller: TubeMPPIController (mode MPPI_TWO_SYSTEM_TUBE, controllers/Tube-MPPI/tube_mppi_controller.cu)
+ * or RobustMPPIController's optimisation (mode MPPI_TWO_SYSTEM_RMPPI, controllers/R-MPPI/robust_mppi_controller.cu:
+ * 635-756) over one sample buffer pair [2][K][T][C], the two-system rollout, and per system the weight transform and
+ * the weighted update. Both distributions draw the same normals (use_same_noise_for_all_distributions, the default of
+ * sampling_distributions/sampling_distribution.cuh:20) with the sampler's std_dev.
+ *   Tube : system 0 is the actual system (from the measured state, mean = actual control), system 1 the nominal one
+ *          (from the nominal trajectory's first state, mean = nominal control); after each iteration the nominal
+ *          trajectory and control are replaced by the actual ones when baseline_actual < baseline_nominal +
+ *          nominal_threshold (:261-276); then the nominal control is smoothed (SG) and both state trajectories are
+ *          propagated. slide propagates the nominal state one step with the nominal control and slides both sequences
+ *          (:318-331).
+ *   RMPPI: system 0 is the nominal system (from the nominal state, set with mppi_tube_set_nominal_state), system 1
+ *          the real one (from the measured state, with the feedback gains); both sample around the nominal control;
+ *          the updates give the nominal control (system 0) and the real control (system 1), both smoothed; the nominal
+ *          state trajectory is propagated. The nominal state is the caller's (the reference chooses it by its
+ *          candidate evaluation, robust_mppi_controller.cu:548-628, before each call); slide is a no-op as there
+ *          (robust_mppi_controller.cuh:190).
+ * Getters and setters take `nominal` = 1 for the nominal system, 0 for the actual / real one. */
+typedef struct mppi_tube_controller mppi_tube_controller;
+
+typedef struct mppi_tube_desc {
+  /* dynamics, cost, sampler, num_rollouts, num_timesteps, dt, lambda, alpha, num_iters, slide_control_scale,
+   * init_control (both sequences), cost_map_host, nn_theta_host, std_dev_t_host, goal_traj_host; other fields unused */
+  mppi_controller_desc base;
+  int32_t mode;                   /* MPPI_TWO_SYSTEM_TUBE or MPPI_TWO_SYSTEM_RMPPI */
+  float nominal_threshold;        /* Tube: TubeMPPIParams::nominal_threshold_ (tube_mppi_controller.cuh:17-21) */
+  float value_function_threshold; /* RMPPI: value_function_threshold (robust_mppi_controller.cuh) */
+  int32_t _pad;
+} mppi_tube_desc;
+
+int mppi_tube_create(const mppi_tube_desc* desc, mppi_tube_controller** out);
+int mppi_tube_destroy(mppi_tube_controller* ctrl);
+int mppi_tube_compute_control(mppi_tube_controller* ctrl, const float* x0_host, int32_t optimization_stride);
+int mppi_tube_get_control_seq(const mppi_tube_controller* ctrl, int32_t nominal, float* out_host); /* [T][C] */
+int mppi_tube_set_control_seq(mppi_tube_controller* ctrl, int32_t nominal, const float* u_host);
+int mppi_tube_get_state_seq(const mppi_tube_controller* ctrl, int32_t nominal, float* out_host);   /* [T][S] */
+int mppi_tube_get_weight_stats(const mppi_tube_controller* ctrl, int32_t nominal, mppi_weight_stats* out_host);
+int mppi_tube_get_free_energy(const mppi_tube_controller* ctrl, int32_t nominal, mppi_free_energy* out_host);
+/* Tube: 1 if the last iteration kept the nominal system, 0 if it was replaced by the actual one (nominal_state_used) */
+int mppi_tube_get_nominal_state_used(const mppi_tube_controller* ctrl, int32_t* out);
+int mppi_tube_slide_control_seq(mppi_tube_controller* ctrl, int32_t steps);
+/* The nominal state: RMPPI's nominal_state_, Tube's nominal trajectory start (set before the first call it replaces
+ * the initialisation from the measured state). */
+int mppi_tube_set_nominal_state(mppi_tube_controller* ctrl, const float* x_host);
+int mppi_tube_get_nominal_state(const mppi_tube_controller* ctrl, float* x_host);
+/* RMPPI feedback gains, host [T][S][C] (layout of mppi_two_system_params.feedback_gains); NULL clears them. */
+int mppi_tube_set_feedback_gains(mppi_tube_controller* ctrl, const float* gains_host);
+/* getFeedbackControl (controllers/controller.cuh:567-571, DDPFeedback::k_): u_fb = K_t (x - x_nominal). */
+int mppi_tube_get_feedback_control(const mppi_tube_controller* ctrl, const float* x, const float* x_nominal, int32_t t,
+                                   float* u_out);
+int mppi_tube_set_thresholds(mppi_tube_controller* ctrl, float nominal_threshold, float value_function_threshold);
+/* The last iteration's applied controls [2][K][T][C] and costs [2][K] (system index as above; either may be NULL). */
+int mppi_tube_get_samples(const mppi_tube_controller* ctrl, float* samples_host, float* costs_host);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

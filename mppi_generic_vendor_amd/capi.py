@@ -120,6 +120,11 @@ class ControllerDesc(ct.Structure):
     ]
 
 
+class TubeDesc(ct.Structure):
+    _fields_ = [("base", ControllerDesc), ("mode", _i), ("nominal_threshold", _f),
+                ("value_function_threshold", _f), ("_pad", _i)]
+
+
 class InstanceDesc(ct.Structure):
     _fields_ = [
         ("dynamics", DynamicsParams), ("cost", CostParams), ("sampler", SamplerParams),
@@ -215,6 +220,22 @@ _SIGS = {
                                             ct.POINTER(_f)], ct.c_int),
     "mppi_controller_get_last_call_ms": ([P, ct.POINTER(_f)], ct.c_int),
     "mppi_controller_time_kernels": ([P, _i, ct.POINTER(_f), ct.POINTER(_f), ct.POINTER(_i)], ct.c_int),
+    "mppi_tube_create": ([ct.POINTER(TubeDesc), ct.POINTER(P)], ct.c_int),
+    "mppi_tube_destroy": ([P], ct.c_int),
+    "mppi_tube_compute_control": ([P, P, _i], ct.c_int),
+    "mppi_tube_get_control_seq": ([P, _i, P], ct.c_int),
+    "mppi_tube_set_control_seq": ([P, _i, P], ct.c_int),
+    "mppi_tube_get_state_seq": ([P, _i, P], ct.c_int),
+    "mppi_tube_get_weight_stats": ([P, _i, P], ct.c_int),
+    "mppi_tube_get_free_energy": ([P, _i, P], ct.c_int),
+    "mppi_tube_get_nominal_state_used": ([P, ct.POINTER(_i)], ct.c_int),
+    "mppi_tube_slide_control_seq": ([P, _i], ct.c_int),
+    "mppi_tube_set_nominal_state": ([P, P], ct.c_int),
+    "mppi_tube_get_nominal_state": ([P, P], ct.c_int),
+    "mppi_tube_set_feedback_gains": ([P, P], ct.c_int),
+    "mppi_tube_get_feedback_control": ([P, P, P, _i, P], ct.c_int),
+    "mppi_tube_set_thresholds": ([P, _f, _f], ct.c_int),
+    "mppi_tube_get_samples": ([P, P, P], ct.c_int),
     "mppi_batch_time_kernels": ([P, _i, ct.POINTER(_f), ct.POINTER(_f)], ct.c_int),
 }
 

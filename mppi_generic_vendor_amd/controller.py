@@ -291,3 +291,170 @@ class ColoredMPPIController(VanillaMPPIController):
 
     def getColoredNoiseExponents(self):
         return [self.problem.sampler.exponents[i] for i in range(self.C)]
+
+
+class TubeMPPIController:
+    """TubeMPPIController (controllers/Tube-MPPI/tube_mppi_controller.cuh:23-116) over the two-system controller of
+    the C-ABI (mppi_tube_*): an actual and a nominal system optimised together, the nominal one replaced by the actual
+    one when the actual baseline is below the nominal baseline + nominal_threshold."""
+
+    MODE = capi.TWO_SYSTEM_TUBE
+
+    def __init__(self, problem: Problem, num_iters: int = 1, init_control: np.ndarray | None = None,
+                 nominal_threshold: float = 20.0, value_function_threshold: float = 1000.0, slide_control_scale=None):
+        self._lib = capi.load()
+        self.problem = problem
+        self.T, self.K, self.S, self.C = problem.num_timesteps, problem.num_rollouts, problem.S, problem.C
+        d = capi.TubeDesc()
+        b = d.base
+        b.dynamics, b.cost, b.sampler = problem.dyn, problem.cost, problem.sampler
+        b.num_rollouts, b.num_timesteps = self.K, self.T
+        b.dt, b.lambda_, b.alpha, b.num_iters = problem.dt, problem.lambda_, problem.alpha, num_iters
+        if slide_control_scale is not None:
+            for i, v in enumerate(slide_control_scale):
+                b.slide_control_scale[i] = v
+        self._keep = _host_arrays(b, problem)
+        if init_control is not None:
+            ic = np.ascontiguousarray(init_control, dtype=np.float32).reshape(self.T, self.C)
+            self._keep.append(ic)
+            b.init_control = ic.ctypes.data
+        d.mode = self.MODE
+        d.nominal_threshold = nominal_threshold
+        d.value_function_threshold = value_function_threshold
+        self._thresholds = [nominal_threshold, value_function_threshold]
+        h = ct.c_void_p()
+        capi.call("mppi_tube_create", ct.byref(d), ct.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.mppi_tube_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _get(self, name, nominal, shape):
+        out = np.empty(shape, dtype=np.float32)
+        capi.call(name, self._h, int(nominal), out.ctypes.data)
+        return out
+
+    # --------------------------------------------------------------------------------------------- reference API
+    def computeControl(self, state: np.ndarray, optimization_stride: int = 1):
+        x = np.ascontiguousarray(state, dtype=np.float32)
+        capi.call("mppi_tube_compute_control", self._h, x.ctypes.data, int(optimization_stride))
+
+    def getControlSeq(self) -> np.ndarray:
+        """The nominal control sequence (tube_mppi_controller.cuh:58-64)."""
+        return self._get("mppi_tube_get_control_seq", 1, (self.T, self.C))
+
+    def getActualControlSeq(self) -> np.ndarray:
+        return self._get("mppi_tube_get_control_seq", 0, (self.T, self.C))
+
+    def setControlSeq(self, u: np.ndarray, nominal: bool = True):
+        a = np.ascontiguousarray(u, dtype=np.float32).reshape(self.T, self.C)
+        capi.call("mppi_tube_set_control_seq", self._h, int(nominal), a.ctypes.data)
+
+    def getTargetStateSeq(self) -> np.ndarray:
+        """The nominal state trajectory (tube_mppi_controller.cuh:66-72)."""
+        return self._get("mppi_tube_get_state_seq", 1, (self.T, self.S))
+
+    def getActualStateSeq(self) -> np.ndarray:
+        return self._get("mppi_tube_get_state_seq", 0, (self.T, self.S))
+
+    def getWeightStats(self, nominal: bool) -> capi.WeightStats:
+        st = capi.WeightStats()
+        capi.call("mppi_tube_get_weight_stats", self._h, int(nominal), ct.byref(st))
+        return st
+
+    def getBaselineCost(self, idx: int = 0) -> float:
+        """Baseline of distribution idx in the reference's indexing (Tube: 0 actual, 1 nominal)."""
+        return self.getWeightStats(self._is_nominal(idx)).baseline
+
+    def getNormalizerCost(self, idx: int = 0) -> float:
+        return self.getWeightStats(self._is_nominal(idx)).normalizer
+
+    def _is_nominal(self, idx: int) -> bool:
+        return idx == 1
+
+    def getFreeEnergyStatistics(self, nominal: bool) -> capi.FreeEnergy:
+        fe = capi.FreeEnergy()
+        capi.call("mppi_tube_get_free_energy", self._h, int(nominal), ct.byref(fe))
+        return fe
+
+    def getNominalStateUsed(self) -> int:
+        v = ct.c_int32()
+        capi.call("mppi_tube_get_nominal_state_used", self._h, ct.byref(v))
+        return v.value
+
+    def slideControlSequence(self, steps: int):
+        capi.call("mppi_tube_slide_control_seq", self._h, int(steps))
+
+    def setNominalState(self, x: np.ndarray):
+        a = np.ascontiguousarray(x, dtype=np.float32)
+        capi.call("mppi_tube_set_nominal_state", self._h, a.ctypes.data)
+
+    def getNominalState(self) -> np.ndarray:
+        out = np.empty(self.S, np.float32)
+        capi.call("mppi_tube_get_nominal_state", self._h, out.ctypes.data)
+        return out
+
+    def setFeedbackGains(self, gains: np.ndarray | None):
+        if gains is None:
+            capi.call("mppi_tube_set_feedback_gains", self._h, None)
+            return
+        a = np.ascontiguousarray(gains, dtype=np.float32).reshape(self.T, self.S, self.C)
+        capi.call("mppi_tube_set_feedback_gains", self._h, a.ctypes.data)
+
+    def getFeedbackControl(self, state: np.ndarray, goal_state: np.ndarray, t: int) -> np.ndarray:
+        x = np.ascontiguousarray(state, np.float32)
+        g = np.ascontiguousarray(goal_state, np.float32)
+        out = np.empty(self.C, np.float32)
+        capi.call("mppi_tube_get_feedback_control", self._h, x.ctypes.data, g.ctypes.data, int(t), out.ctypes.data)
+        return out
+
+    def getNominalThreshold(self) -> float:
+        return self._thresholds[0]
+
+    def setNominalThreshold(self, v: float):
+        self._thresholds[0] = float(v)
+        capi.call("mppi_tube_set_thresholds", self._h, *self._thresholds)
+
+    def getValueFunctionThreshold(self) -> float:
+        return self._thresholds[1]
+
+    def setValueFunctionThreshold(self, v: float):
+        self._thresholds[1] = float(v)
+        capi.call("mppi_tube_set_thresholds", self._h, *self._thresholds)
+
+    def getSamples(self):
+        """The last iteration's applied controls [2][K][T][C] and costs [2][K] (buffer system order)."""
+        u = np.empty((2, self.K, self.T, self.C), np.float32)
+        c = np.empty((2, self.K), np.float32)
+        capi.call("mppi_tube_get_samples", self._h, u.ctypes.data, c.ctypes.data)
+        return u, c
+
+
+class RobustMPPIController(TubeMPPIController):
+    """The optimisation of RobustMPPIController (controllers/R-MPPI/robust_mppi_controller.cu:635-756): nominal
+    (distribution 0) and real (distribution 1) systems sampled around the nominal control, the real one tracking the
+    nominal with the feedback gains; the nominal state is set by the caller (setNominalState)."""
+
+    MODE = capi.TWO_SYSTEM_RMPPI
+
+    def __init__(self, problem: Problem, num_iters: int = 1, init_control: np.ndarray | None = None,
+                 value_function_threshold: float = 1000.0, slide_control_scale=None):
+        super().__init__(problem, num_iters, init_control, 20.0, value_function_threshold, slide_control_scale)
+
+    def _is_nominal(self, idx: int) -> bool:
+        return idx == 0
+
+    def getControlSeq(self) -> np.ndarray:
+        """The real system's control sequence."""
+        return self._get("mppi_tube_get_control_seq", 0, (self.T, self.C))
+
+    def getNominalControlSeq(self) -> np.ndarray:
+        return self._get("mppi_tube_get_control_seq", 1, (self.T, self.C))

@@ -1,0 +1,155 @@
+"""Tube-MPPI and RMPPI controllers on the GPU (mppi_tube_*: TubeMPPIController::computeControl,
+controllers/Tube-MPPI/tube_mppi_controller.cu:158-295, and RobustMPPIController's optimisation,
+controllers/R-MPPI/robust_mppi_controller.cu:635-756) against the oracle's chain on the same normals: the Gaussian
+transform of both distributions, the two-system rollout, the weights and the weighted update of each system, then the
+host rules (nominal replacement, SG smoothing, state trajectories, slide). Bars: costs within 1e-5 relative, samples
+within 1e-6, means within 1e-4 of the control scale."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_util import banner, gpu_normals, rel_err
+from mppi_generic_vendor_amd import capi, workloads
+from mppi_generic_vendor_amd.controller import RobustMPPIController, TubeMPPIController
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(K=2048, T=60):
+    prob = workloads.double_integrator_map(num_rollouts=K, num_timesteps=T, colored=False,
+                                           cost_map=workloads.gaussian_blob_map(256, 256, blobs=16, seed=7))
+    prob.sampler.control_cost_coeff[0], prob.sampler.control_cost_coeff[1] = 0.2, 0.2
+    return prob
+
+
+def _oracle_iteration(prob, means, x0s, offset, nom=-1, vft=0.0, gains=None, stride=1, iteration=0):
+    K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
+    eps = gpu_normals(prob.sampler.seed, offset, K * T * C).reshape(K, T, C)
+    u = np.stack([ol.gaussian_transform(prob.sampler, K, T, C, means[d], eps, stride, iteration) for d in range(2)])
+    costs, applied = ol.rollout_two_system(prob, u, means, x0s, nom, vft, gains)
+    new = []
+    for d in range(2):
+        w, eta, *_ = ol.weights(costs[d], prob.lambda_)
+        new.append(ol.weighted_update(w, eta, applied[d]).astype(np.float32))
+    return u, costs, applied, new
+
+
+def _close(got, want, tag):
+    err = float(np.max(np.abs(got - want))) / max(1e-3, float(np.max(np.abs(want))))
+    print(f"[tube] {tag}: rel err {err:.2e}")
+    assert err < 1e-4
+
+
+@pytest.mark.parametrize("threshold,replaced", [(1e9, True), (-1e9, False)])
+def test_tube_iteration_matches_oracle(hip, engine, threshold, replaced):
+    prob = _problem()
+    T, C, S = prob.num_timesteps, prob.C, prob.S
+    banner(f"Tube-MPPI controller, nominal threshold {threshold:g}")
+    rng = np.random.default_rng(3)
+    u_act = (rng.normal(size=(T, C)) * 0.3).astype(np.float32)
+    u_nom = (rng.normal(size=(T, C)) * 0.3).astype(np.float32)
+    x_nom = (prob.x0 + rng.normal(size=S).astype(np.float32) * 0.3).astype(np.float32)
+    ctrl = TubeMPPIController(prob, num_iters=1, init_control=u_act, nominal_threshold=threshold)
+    ctrl.setControlSeq(u_nom, nominal=True)
+    ctrl.setNominalState(x_nom)
+    ctrl.computeControl(prob.x0, 1)
+    means = np.stack([u_act, u_nom])
+    want_u, want_c, want_applied, new = _oracle_iteration(prob, means, np.stack([prob.x0, x_nom]), 0)
+    got_applied, got_c = ctrl.getSamples()
+    e = rel_err(got_c, want_c)
+    print(f"[tube] costs rel err {e:.2e}")
+    assert e < 1e-5
+    assert rel_err(got_applied, want_applied) < 1e-6
+    _close(ctrl.getActualControlSeq(), new[0], "actual control")
+    nominal_new = new[0] if replaced else new[1]
+    _close(ctrl.getControlSeq(), ol.smooth(nominal_new), "nominal control (smoothed)")
+    assert ctrl.getNominalStateUsed() == (0 if replaced else 1)
+    st_a, st_n = ctrl.getWeightStats(False), ctrl.getWeightStats(True)
+    assert st_a.baseline == pytest.approx(float(want_c[0].min()), rel=1e-5)
+    assert st_n.baseline == pytest.approx(float(want_c[1].min()), rel=1e-5)
+    # the state trajectories: the actual one from x0, the nominal one from the nominal start (x0 once replaced)
+    start = prob.x0 if replaced else x_nom
+    np.testing.assert_allclose(ctrl.getTargetStateSeq(), ol.state_trajectory(prob, start, ctrl.getControlSeq()),
+                               rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(ctrl.getActualStateSeq(),
+                               ol.state_trajectory(prob, prob.x0, ctrl.getActualControlSeq()), rtol=1e-4, atol=1e-4)
+    # slide: the nominal state advances one model step with the nominal control's first step; both sequences shift
+    un, ua = ctrl.getControlSeq(), ctrl.getActualControlSeq()
+    xn0 = ctrl.getNominalState()
+    ctrl.slideControlSequence(1)
+    xs = np.zeros(S, np.float32)
+    xd = np.zeros(S, np.float32)
+    y = np.zeros(prob.dyn.output_dim, np.float32)
+    u0 = un[0].copy()
+    capi.call("mppi_host_step", prob.dyn, xn0.ctypes.data, u0.ctypes.data, prob.dt, xs.ctypes.data, xd.ctypes.data,
+              y.ctypes.data)
+    np.testing.assert_array_equal(ctrl.getNominalState(), xs)
+    np.testing.assert_array_equal(ctrl.getControlSeq()[:-1], un[1:])
+    np.testing.assert_array_equal(ctrl.getActualControlSeq()[:-1], ua[1:])
+    # the next call draws the next normals (offset 1)
+    ctrl.computeControl(prob.x0, 1)
+    ctrl.close()
+
+
+def test_rmppi_iteration_matches_oracle(hip, engine):
+    prob = _problem(K=1536, T=50)
+    T, C, S = prob.num_timesteps, prob.C, prob.S
+    banner("RMPPI controller iteration with feedback gains")
+    rng = np.random.default_rng(4)
+    u_nom = (rng.normal(size=(T, C)) * 0.3).astype(np.float32)
+    x_nom = (prob.x0 + rng.normal(size=S).astype(np.float32) * 0.3).astype(np.float32)
+    gains = (rng.normal(size=(T, S, C)) * 0.5).astype(np.float32)
+    vft = 60.0
+    ctrl = RobustMPPIController(prob, num_iters=1, init_control=u_nom, value_function_threshold=vft)
+    ctrl.setNominalState(x_nom)
+    ctrl.setFeedbackGains(gains)
+    ctrl.computeControl(prob.x0, 1)
+    means = np.stack([u_nom, u_nom])
+    _, want_c, want_applied, new = _oracle_iteration(prob, means, np.stack([x_nom, prob.x0]), 0, nom=0, vft=vft,
+                                                     gains=gains)
+    got_applied, got_c = ctrl.getSamples()
+    e = rel_err(got_c, want_c)
+    print(f"[tube] rmppi costs rel err {e:.2e}")
+    assert e < 1e-4
+    assert rel_err(got_applied[0], want_applied[0]) < 1e-6  # the nominal system's controls carry no feedback
+    # the real system's u + K_t e_t: last-bit differences of the FMA-contracted feedback sum, on the control scale
+    du = float(np.max(np.abs(got_applied[1] - want_applied[1]))) / max(1.0, float(np.max(np.abs(want_applied[1]))))
+    print(f"[tube] rmppi real applied controls: {du:.2e} of the control scale")
+    assert du < 1e-5
+    _close(ctrl.getNominalControlSeq(), ol.smooth(new[0]), "rmppi nominal control")
+    _close(ctrl.getControlSeq(), ol.smooth(new[1]), "rmppi real control")
+    np.testing.assert_allclose(ctrl.getTargetStateSeq(), ol.state_trajectory(prob, x_nom, ctrl.getNominalControlSeq()),
+                               rtol=1e-4, atol=1e-4)
+    # getFeedbackControl = K_t (x - x_nominal)
+    x = rng.normal(size=S).astype(np.float32)
+    np.testing.assert_allclose(ctrl.getFeedbackControl(x, x_nom, 7), np.einsum("sc,s->c", gains[7], x - x_nom),
+                               rtol=1e-5, atol=1e-6)
+    assert ctrl.getBaselineCost(0) == pytest.approx(float(want_c[0].min()), rel=1e-4)
+    ctrl.close()
+
+
+def test_tube_closed_loop_tracks_goal(hip, engine):
+    """A closed loop on the double integrator (the state advanced with the first actual control, then slide): the
+    actual system reaches the goal region and the nominal system stays within the tube around it."""
+    prob = _problem(K=1024, T=50)
+    prob.cost.map_coeff = 0.0
+    banner("Tube-MPPI closed loop")
+    ctrl = TubeMPPIController(prob, num_iters=1, nominal_threshold=5.0)
+    x = prob.x0.copy()
+    dist0 = float(np.hypot(x[0] - 2.0, x[1] - 2.0))
+    y = np.zeros(prob.dyn.output_dim, np.float32)
+    xd = np.zeros(prob.S, np.float32)
+    for _ in range(150):
+        ctrl.computeControl(x, 1)
+        u = ctrl.getActualControlSeq()[0].copy()
+        xn = np.zeros(prob.S, np.float32)
+        capi.call("mppi_host_step", prob.dyn, x.ctypes.data, u.ctypes.data, prob.dt, xn.ctypes.data, xd.ctypes.data,
+                  y.ctypes.data)
+        x = xn
+        ctrl.slideControlSequence(1)
+    dist = float(np.hypot(x[0] - 2.0, x[1] - 2.0))
+    gap = float(np.linalg.norm(ctrl.getNominalState() - x))
+    print(f"[tube] closed loop: distance to goal {dist:.3f} (from {dist0:.3f}), nominal-actual gap {gap:.3f}")
+    assert dist < 0.5 * dist0
+    assert gap < dist0
+    ctrl.close()
