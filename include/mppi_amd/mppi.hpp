@@ -1189,4 +1189,270 @@ class ColoredMPPIController
   bool solved_ = false;
 };
 
+// =================================================================================== two-system controllers
+// Feedback gains K_t for RMPPI's real system and getFeedbackControl: gains [T][S][C] (the layout of
+// DDPFeedbackState::fb_gain_traj_, feedback_controllers/DDP/ddp.cuh:30-40), set by the caller (the DDP solver that
+// computes them in the reference is not part of this library). Fills the FB_T slot of the two-system controllers.
+template <class DYN_T, int FB_TIMESTEPS>
+class GainFeedback {
+ public:
+  static const int TEMPLATED_FB_TIMESTEPS = FB_TIMESTEPS;
+  GainFeedback(DYN_T* = nullptr, float = 0.0f) {}
+  void setGains(const std::vector<float>& g) { gains_ = g; }
+  const std::vector<float>& getGains() const { return gains_; }
+
+ private:
+  std::vector<float> gains_;
+};
+
+template <class T>
+struct is_gain_feedback : std::false_type {};
+template <class D, int N>
+struct is_gain_feedback<GainFeedback<D, N>> : std::true_type {};
+
+// TubeMPPIParams (controllers/Tube-MPPI/tube_mppi_controller.cuh:17-21); value_function_threshold_ is RMPPI's
+template <int S_DIM, int C_DIM, int MAX_TIMESTEPS>
+struct TubeMPPIParams : ControllerParams<S_DIM, C_DIM, MAX_TIMESTEPS> {
+  float nominal_threshold_ = 20.0f;
+  float value_function_threshold_ = 1000.0f;
+};
+
+// The two-system controllers over mppi_tube_* (both systems' sequences, statistics and state trajectories live in the
+// C-ABI object; parameter changes recreate it and restore the sequences and the nominal state).
+template <int MODE, class DYN_T, class COST_T, class FB_T, int MAX_TIMESTEPS, int NUM_ROLLOUTS, class SAMPLING_T,
+          class PARAMS_T>
+class TwoSystemController {
+ public:
+  static const int STATE_DIM = DYN_T::STATE_DIM, CONTROL_DIM = DYN_T::CONTROL_DIM;
+  using state_array = typename DYN_T::state_array;
+  using control_array = typename DYN_T::control_array;
+  using control_trajectory = Mat<float, CONTROL_DIM, MAX_TIMESTEPS>;
+  using state_trajectory = Mat<float, STATE_DIM, MAX_TIMESTEPS>;
+  using TEMPLATED_DYNAMICS = DYN_T;
+  using TEMPLATED_COSTS = COST_T;
+  using TEMPLATED_FEEDBACK = FB_T;
+  using TEMPLATED_SAMPLING = SAMPLING_T;
+  using TEMPLATED_PARAMS = PARAMS_T;
+
+  TwoSystemController(DYN_T* model, COST_T* cost, FB_T* fb_controller, SAMPLING_T* sampler, float dt, int max_iter,
+                      float lambda, float alpha, int num_timesteps = MAX_TIMESTEPS,
+                      const control_trajectory& init_control_traj = control_trajectory::Zero(), void* = nullptr)
+    : model_(model), cost_(cost), fb_(fb_controller), sampler_(sampler)
+  {
+    params_.dt_ = dt;
+    params_.num_iters_ = max_iter;
+    params_.lambda_ = lambda;
+    params_.alpha_ = alpha;
+    params_.num_timesteps_ = num_timesteps;
+    params_.init_control_traj_ = init_control_traj;
+    control_[0] = control_[1] = init_control_traj;
+  }
+  TwoSystemController(DYN_T* model, COST_T* cost, FB_T* fb_controller, SAMPLING_T* sampler, const PARAMS_T& params,
+                      void* = nullptr)
+    : model_(model), cost_(cost), fb_(fb_controller), sampler_(sampler), params_(params)
+  {
+    control_[0] = control_[1] = params.init_control_traj_;
+  }
+  virtual ~TwoSystemController() { release(); }
+  TwoSystemController(const TwoSystemController&) = delete;
+  TwoSystemController& operator=(const TwoSystemController&) = delete;
+
+  void computeControl(const state_array& state, int optimization_stride = 1)
+  {
+    ensure();
+    push_gains();
+    check(mppi_tube_compute_control(ctrl_, state.data(), optimization_stride), "computeControl");
+    for (int n = 0; n < 2; ++n) {
+      check(mppi_tube_get_control_seq(ctrl_, n, control_[n].data()), "getControlSeq");
+      check(mppi_tube_get_state_seq(ctrl_, n, state_[n].data()), "getStateSeq");
+      check(mppi_tube_get_weight_stats(ctrl_, n, &stats_[n]), "weights");
+      mppi_free_energy fe;
+      check(mppi_tube_get_free_energy(ctrl_, n, &fe), "free energy");
+      freeEnergyEstimate& e = n ? fe_.nominal_sys : fe_.real_sys;
+      e.freeEnergyMean = fe.free_energy_mean;
+      e.freeEnergyVariance = fe.free_energy_variance;
+      e.freeEnergyModifiedVariance = fe.free_energy_modified_variance;
+      e.normalizerPercent = fe.normalizer_percent;
+      e.increase = fe.increase;
+      e.previousBaseline = fe.previous_baseline;
+    }
+    int32_t used = 1;
+    check(mppi_tube_get_nominal_state_used(ctrl_, &used), "nominal_state_used");
+    fe_.nominal_state_used = used;
+    check(mppi_tube_get_nominal_state(ctrl_, nominal_state_.data()), "getNominalState");
+    have_nominal_ = true;
+  }
+  void slideControlSequence(int steps)
+  {
+    if (!ctrl_) return;
+    check(mppi_tube_slide_control_seq(ctrl_, steps), "slideControlSequence");
+    for (int n = 0; n < 2; ++n) check(mppi_tube_get_control_seq(ctrl_, n, control_[n].data()), "getControlSeq");
+    check(mppi_tube_get_nominal_state(ctrl_, nominal_state_.data()), "getNominalState");
+  }
+  // the nominal state (RMPPI: nominal_state_; Tube: the nominal trajectory's start)
+  void setNominalState(const state_array& x)
+  {
+    nominal_state_ = x;
+    have_nominal_ = true;
+    if (ctrl_) check(mppi_tube_set_nominal_state(ctrl_, x.data()), "setNominalState");
+  }
+  state_array getNominalState() const { return nominal_state_; }
+  control_trajectory getNominalControlSeq() const { return control_[1]; }
+  control_trajectory getActualControlSeq() const { return control_[0]; }
+  state_trajectory getNominalStateSeq() const { return state_[1]; }
+  state_trajectory getActualStateSeq() const { return state_[0]; }
+  // getBaselineCost / getNormalizerCost in the reference's distribution indexing (Tube: 0 actual, 1 nominal;
+  // RMPPI: 0 nominal, 1 real)
+  float getBaselineCost(int idx = 0) const { return stats_[is_nominal(idx)].baseline; }
+  float getNormalizerCost(int idx = 0) const { return (float)stats_[is_nominal(idx)].normalizer; }
+  MPPIFreeEnergyStatistics getFreeEnergyStatistics() const { return fe_; }
+  // getFeedbackControl (controllers/controller.cuh:567-571): K_t (x - x_goal) with the gains of the FB_T object
+  control_array getFeedbackControl(const state_array& x, const state_array& goal, int t) const
+  {
+    control_array u = control_array::Zero();
+    const std::vector<float>& g = gains();
+    if (g.empty()) return u;
+    for (int j = 0; j < CONTROL_DIM; ++j)
+      for (int i = 0; i < STATE_DIM; ++i) u[j] += g[((size_t)t * STATE_DIM + i) * CONTROL_DIM + j] * (x[i] - goal[i]);
+    return u;
+  }
+  PARAMS_T getParams() const { return params_; }
+  void setParams(const PARAMS_T& p) { params_ = p; }
+  int getNumTimesteps() const { return params_.num_timesteps_; }
+  float getDt() const { return params_.dt_; }
+  DYN_T* model_;
+  COST_T* cost_;
+  FB_T* fb_;
+  SAMPLING_T* sampler_;
+
+ protected:
+  virtual bool is_nominal(int idx) const = 0;
+  PARAMS_T params_;
+  control_trajectory control_[2];  // [0] actual / real, [1] nominal
+  state_trajectory state_[2];
+
+ private:
+  const std::vector<float>& gains() const
+  {
+    static const std::vector<float> none;
+    if constexpr (is_gain_feedback<FB_T>::value) {
+      return fb_ ? fb_->getGains() : none;
+    } else {
+      return none;
+    }
+  }
+  void push_gains()
+  {
+    const std::vector<float>& g = gains();
+    const size_t need = (size_t)params_.num_timesteps_ * STATE_DIM * CONTROL_DIM;
+    if (!g.empty() && g.size() < need) throw std::runtime_error("feedback gains hold fewer than T x S x C values");
+    check(mppi_tube_set_feedback_gains(ctrl_, g.empty() ? nullptr : g.data()), "setFeedbackGains");
+  }
+  mppi_tube_desc make_desc() const
+  {
+    mppi_tube_desc d;
+    std::memset(&d, 0, sizeof(d));
+    mppi_controller_desc& b = d.base;
+    b.sampler = sampler_->desc();
+    b.sampler.seed = params_.seed_;
+    b.num_rollouts = NUM_ROLLOUTS;
+    b.num_timesteps = params_.num_timesteps_;
+    b.dt = params_.dt_;
+    b.lambda = params_.lambda_;
+    b.alpha = params_.alpha_;
+    b.num_iters = params_.num_iters_;
+    for (int c = 0; c < CONTROL_DIM; ++c) b.slide_control_scale[c] = params_.slide_control_scale_[c];
+    b.std_dev_t_host = sampler_->stdDevTHost();
+    b.dynamics = model_->desc();
+    b.cost = cost_->desc(*model_);
+    b.goal_traj_host = cost_->goalTrajectoryHost();
+    d.mode = MODE;
+    d.nominal_threshold = params_.nominal_threshold_;
+    d.value_function_threshold = params_.value_function_threshold_;
+    return d;
+  }
+  std::string snapshot(const mppi_tube_desc& d) const
+  {
+    std::string s(reinterpret_cast<const char*>(&d), sizeof(d));
+    const unsigned long long v = cost_->costMapVersion();
+    s.append(reinterpret_cast<const char*>(&v), sizeof(v));
+    const mppi_controller_desc& b = d.base;
+    if (b.goal_traj_host) s.append((const char*)b.goal_traj_host, sizeof(float) * b.cost.goal_traj_len * DYN_T::OUTPUT_DIM);
+    if (b.std_dev_t_host) s.append((const char*)b.std_dev_t_host, sizeof(float) * b.num_timesteps * CONTROL_DIM);
+    return s;
+  }
+  // creates the device side, or recreates it after a parameter change with the sequences and nominal state restored
+  void ensure()
+  {
+    const int T = params_.num_timesteps_;
+    if (T < 1 || T > MAX_TIMESTEPS) throw std::runtime_error("num_timesteps out of range");
+    mppi_tube_desc d = make_desc();
+    d.base.init_control = control_[0].data();
+    std::string snap = snapshot(d);
+    if (ctrl_ && snap == snapshot_) return;
+    release();
+    d.base.cost_map_host = cost_->costMapHost();
+    d.base.nn_theta_host = model_->hostParameters();
+    check(mppi_tube_create(&d, &ctrl_), "mppi_tube_create");
+    check(mppi_tube_set_control_seq(ctrl_, 1, control_[1].data()), "setControlSeq");
+    if (have_nominal_) check(mppi_tube_set_nominal_state(ctrl_, nominal_state_.data()), "setNominalState");
+    snapshot_ = std::move(snap);
+  }
+  void release()
+  {
+    if (ctrl_) mppi_tube_destroy(ctrl_);
+    ctrl_ = nullptr;
+  }
+  mppi_tube_controller* ctrl_ = nullptr;
+  std::string snapshot_;
+  mppi_weight_stats stats_[2]{};
+  MPPIFreeEnergyStatistics fe_;
+  state_array nominal_state_ = state_array::Zero();
+  bool have_nominal_ = false;
+};
+
+// TubeMPPIController (controllers/Tube-MPPI/tube_mppi_controller.cuh:23-116): getControlSeq / getTargetStateSeq are
+// the nominal system's, getActualControlSeq the actual one's.
+template <class DYN_T, class COST_T, class FB_T, int MAX_TIMESTEPS, int NUM_ROLLOUTS,
+          class SAMPLING_T = sampling_distributions::GaussianDistributionC<DYN_T::CONTROL_DIM>,
+          class PARAMS_T = TubeMPPIParams<DYN_T::STATE_DIM, DYN_T::CONTROL_DIM, MAX_TIMESTEPS>>
+class TubeMPPIController : public TwoSystemController<MPPI_TWO_SYSTEM_TUBE, DYN_T, COST_T, FB_T, MAX_TIMESTEPS,
+                                                      NUM_ROLLOUTS, SAMPLING_T, PARAMS_T> {
+  using BASE =
+      TwoSystemController<MPPI_TWO_SYSTEM_TUBE, DYN_T, COST_T, FB_T, MAX_TIMESTEPS, NUM_ROLLOUTS, SAMPLING_T, PARAMS_T>;
+
+ public:
+  using BASE::BASE;
+  std::string getControllerName() const { return "Tube MPPI"; }
+  typename BASE::control_trajectory getControlSeq() const { return this->control_[1]; }
+  typename BASE::state_trajectory getTargetStateSeq() const { return this->state_[1]; }
+  float getNominalThreshold() const { return this->params_.nominal_threshold_; }
+  void setNominalThreshold(float v) { this->params_.nominal_threshold_ = v; }
+
+ protected:
+  bool is_nominal(int idx) const override { return idx == 1; }
+};
+
+// RobustMPPIController's optimisation (controllers/R-MPPI/robust_mppi_controller.cu:635-756) with the nominal state
+// supplied by the caller (setNominalState); getControlSeq is the real system's control.
+template <class DYN_T, class COST_T, class FB_T, int MAX_TIMESTEPS, int NUM_ROLLOUTS,
+          class SAMPLING_T = sampling_distributions::GaussianDistributionC<DYN_T::CONTROL_DIM>,
+          class PARAMS_T = TubeMPPIParams<DYN_T::STATE_DIM, DYN_T::CONTROL_DIM, MAX_TIMESTEPS>>
+class RobustMPPIController : public TwoSystemController<MPPI_TWO_SYSTEM_RMPPI, DYN_T, COST_T, FB_T, MAX_TIMESTEPS,
+                                                        NUM_ROLLOUTS, SAMPLING_T, PARAMS_T> {
+  using BASE = TwoSystemController<MPPI_TWO_SYSTEM_RMPPI, DYN_T, COST_T, FB_T, MAX_TIMESTEPS, NUM_ROLLOUTS,
+                                   SAMPLING_T, PARAMS_T>;
+
+ public:
+  using BASE::BASE;
+  std::string getControllerName() const { return "RMPPI"; }
+  typename BASE::control_trajectory getControlSeq() const { return this->control_[0]; }
+  typename BASE::state_trajectory getTargetStateSeq() const { return this->state_[1]; }
+  float getValueFunctionThreshold() const { return this->params_.value_function_threshold_; }
+  void setValueFunctionThreshold(float v) { this->params_.value_function_threshold_ = v; }
+
+ protected:
+  bool is_nominal(int idx) const override { return idx == 0; }
+};
+
 }  // namespace mppi_amd

@@ -12,7 +12,7 @@ PREBUILT = ROOT / "tests" / "cpp" / "_build" / "cartpole_closed_loop"
 
 
 def test_cpp_api_compiles_with_gcc(tmp_path, engine):
-    for name in ("cartpole_closed_loop", "plant_loop", "tracking_loop", "colored_loop", "map_io"):
+    for name in ("cartpole_closed_loop", "plant_loop", "tracking_loop", "colored_loop", "map_io", "tube_loop"):
         assert build_program(name, tmp_path).exists()
 
 
@@ -149,3 +149,18 @@ def test_host_running_cost_matches_oracle(engine, oracle):
         capi.call("mppi_host_running_cost", ct.byref(cost), y.ctypes.data, ct.byref(got))
         want = oracle.oracle_running_cost_f32(ct.byref(cost), m.ctypes.data, y.ctypes.data, 0)
         assert got.value == pytest.approx(want, rel=2e-6, abs=1e-6)
+
+
+TUBE = ROOT / "tests" / "cpp" / "_build" / "tube_loop"
+
+
+@pytest.mark.gpu
+def test_cpp_tube_and_rmppi_loops(tmp_path, hip, engine):
+    """TubeMPPIController and RobustMPPIController (include/mppi_amd/mppi.hpp) in disturbed closed loops on the double
+    integrator: goal reached, nominal replacement seen (Tube), PD feedback law and a bounded tube (RMPPI);
+    tests/cpp/tube_loop.cpp prints each check."""
+    exe = TUBE if TUBE.exists() else build_program("tube_loop", tmp_path)
+    print("[gpu] C++ Tube-MPPI / RMPPI loops", flush=True)
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0, res.stdout + res.stderr
