@@ -688,7 +688,9 @@ typedef struct mppi_tube_desc {
   mppi_controller_desc base;
   int32_t mode;                   /* MPPI_TWO_SYSTEM_TUBE or MPPI_TWO_SYSTEM_RMPPI */
   float nominal_threshold;        /* Tube: TubeMPPIParams::nominal_threshold_ (tube_mppi_controller.cuh:17-21) */
-  float value_function_threshold; /* RMPPI: value_function_threshold (robust_mppi_controller.cuh) */
+  float value_function_threshold; /* RMPPI: value_function_threshold (robust_mppi_controller.cuh:46-53) */
+  int32_t num_candidates;   /* RMPPI: num_candidate_nominal_states_, odd and >= 3 (0: 9) */
+  int32_t num_eval_samples; /* RMPPI: samples per candidate of the evaluation, eval_dyn_kernel_dim_.x (0: 64) */
   int32_t _pad;
 } mppi_tube_desc;
 
@@ -713,6 +715,20 @@ int mppi_tube_set_feedback_gains(mppi_tube_controller* ctrl, const float* gains_
 int mppi_tube_get_feedback_control(const mppi_tube_controller* ctrl, const float* x, const float* x_nominal, int32_t t,
                                    float* u_out);
 int mppi_tube_set_thresholds(mppi_tube_controller* ctrl, float nominal_threshold, float value_function_threshold);
+/* RMPPI: updateImportanceSamplingControl (robust_mppi_controller.cu:548-628), before computeControl. On the first call
+ * the nominal state is the measured state (stride 0). Afterwards N candidate nominal states are formed on the line
+ * through the nominal trajectory's first two states and the measured state (line-search weights, :480-498; candidates
+ * :351-363), each with the importance-sampler stride round(w1 s + w2 s) (:501-511); samples are drawn around the
+ * nominal control with optimization stride s, and every candidate rolls out the first M samples from its state with
+ * its shifted controls (initEvalKernel, core/rmppi_kernels.cu:12-358); the chosen candidate is the last whose free
+ * energy -lambda log(mean_j exp(-(c_ij - b) / lambda)) + b (b the smallest evaluation cost) is below the
+ * value-function threshold (:513-546), else the previous choice. The nominal state becomes that candidate, the nominal
+ * control is slid by its stride and the nominal trajectory is propagated. The feedback gains stay the caller's. */
+int mppi_tube_update_importance_sampling_control(mppi_tube_controller* ctrl, const float* x_host, int32_t stride);
+/* RMPPI: the last evaluation's candidate states [N][S], strides [N], evaluation costs [N][M], candidate free energies
+ * [N] and the chosen index (any pointer may be NULL). */
+int mppi_tube_get_candidates(const mppi_tube_controller* ctrl, float* states, int32_t* strides, float* costs,
+                             float* free_energies, int32_t* best_index);
 /* The last iteration's applied controls [2][K][T][C] and costs [2][K] (system index as above; either may be NULL). */
 int mppi_tube_get_samples(const mppi_tube_controller* ctrl, float* samples_host, float* costs_host);
 

@@ -1215,6 +1215,8 @@ template <int S_DIM, int C_DIM, int MAX_TIMESTEPS>
 struct TubeMPPIParams : ControllerParams<S_DIM, C_DIM, MAX_TIMESTEPS> {
   float nominal_threshold_ = 20.0f;
   float value_function_threshold_ = 1000.0f;
+  int num_candidate_nominal_states_ = 9;  // RobustMPPIParams (robust_mppi_controller.cuh:46-53)
+  int num_eval_samples_ = 64;             // eval_dyn_kernel_dim_.x
 };
 
 // The two-system controllers over mppi_tube_* (both systems' sequences, statistics and state trajectories live in the
@@ -1327,6 +1329,23 @@ class TwoSystemController {
 
  protected:
   virtual bool is_nominal(int idx) const = 0;
+  // the device side, created or refreshed, with the current gains
+  mppi_tube_controller* device()
+  {
+    ensure();
+    push_gains();
+    return ctrl_;
+  }
+  // the sequences, state trajectories and nominal state of the device side
+  void pull()
+  {
+    for (int n = 0; n < 2; ++n) {
+      check(mppi_tube_get_control_seq(ctrl_, n, control_[n].data()), "getControlSeq");
+      check(mppi_tube_get_state_seq(ctrl_, n, state_[n].data()), "getStateSeq");
+    }
+    check(mppi_tube_get_nominal_state(ctrl_, nominal_state_.data()), "getNominalState");
+    have_nominal_ = true;
+  }
   PARAMS_T params_;
   control_trajectory control_[2];  // [0] actual / real, [1] nominal
   state_trajectory state_[2];
@@ -1369,6 +1388,10 @@ class TwoSystemController {
     d.mode = MODE;
     d.nominal_threshold = params_.nominal_threshold_;
     d.value_function_threshold = params_.value_function_threshold_;
+    if (MODE == MPPI_TWO_SYSTEM_RMPPI) {
+      d.num_candidates = params_.num_candidate_nominal_states_;
+      d.num_eval_samples = params_.num_eval_samples_;
+    }
     return d;
   }
   std::string snapshot(const mppi_tube_desc& d) const
@@ -1450,9 +1473,24 @@ class RobustMPPIController : public TwoSystemController<MPPI_TWO_SYSTEM_RMPPI, D
   typename BASE::state_trajectory getTargetStateSeq() const { return this->state_[1]; }
   float getValueFunctionThreshold() const { return this->params_.value_function_threshold_; }
   void setValueFunctionThreshold(float v) { this->params_.value_function_threshold_ = v; }
+  // updateImportanceSamplingControl (robust_mppi_controller.cu:548-628): the candidate evaluation that chooses the
+  // nominal state and slides the nominal control, before computeControl
+  void updateImportanceSamplingControl(const typename BASE::state_array& state, int stride)
+  {
+    mppi_tube_controller* c = this->device();
+    check(mppi_tube_update_importance_sampling_control(c, state.data(), stride), "updateImportanceSamplingControl");
+    this->pull();
+    int32_t best = 0;
+    check(mppi_tube_get_candidates(c, nullptr, nullptr, nullptr, nullptr, &best), "getCandidates");
+    best_index_ = best;
+  }
+  int getBestIndex() const { return best_index_; }
 
  protected:
   bool is_nominal(int idx) const override { return idx == 0; }
+
+ private:
+  int best_index_ = 0;
 };
 
 }  // namespace mppi_amd

@@ -319,12 +319,16 @@ class TubeMPPIController:
             self._keep.append(ic)
             b.init_control = ic.ctypes.data
         d.mode = self.MODE
+        self._extend_tube_desc(d)
         d.nominal_threshold = nominal_threshold
         d.value_function_threshold = value_function_threshold
         self._thresholds = [nominal_threshold, value_function_threshold]
         h = ct.c_void_p()
         capi.call("mppi_tube_create", ct.byref(d), ct.byref(h))
         self._h = h
+
+    def _extend_tube_desc(self, d):
+        """RMPPI's candidate-evaluation sizes (unused by Tube-MPPI)."""
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -446,8 +450,31 @@ class RobustMPPIController(TubeMPPIController):
     MODE = capi.TWO_SYSTEM_RMPPI
 
     def __init__(self, problem: Problem, num_iters: int = 1, init_control: np.ndarray | None = None,
-                 value_function_threshold: float = 1000.0, slide_control_scale=None):
+                 value_function_threshold: float = 1000.0, slide_control_scale=None, num_candidates: int = 9,
+                 num_eval_samples: int = 64):
+        self._cand = (num_candidates, num_eval_samples)
         super().__init__(problem, num_iters, init_control, 20.0, value_function_threshold, slide_control_scale)
+
+    def _extend_tube_desc(self, d):
+        d.num_candidates, d.num_eval_samples = self._cand
+
+    def updateImportanceSamplingControl(self, state: np.ndarray, stride: int = 1):
+        """The candidate evaluation that picks the nominal state and slides the nominal control
+        (robust_mppi_controller.cu:548-628)."""
+        x = np.ascontiguousarray(state, dtype=np.float32)
+        capi.call("mppi_tube_update_importance_sampling_control", self._h, x.ctypes.data, int(stride))
+
+    def getCandidates(self):
+        """The last evaluation: candidate states [N][S], strides [N], costs [N][M], free energies [N], chosen index."""
+        N, M = self._cand
+        st = np.empty((N, self.S), np.float32)
+        sr = np.empty(N, np.int32)
+        co = np.empty((N, M), np.float32)
+        fe = np.empty(N, np.float32)
+        bi = ct.c_int32()
+        capi.call("mppi_tube_get_candidates", self._h, st.ctypes.data, sr.ctypes.data, co.ctypes.data, fe.ctypes.data,
+                  ct.byref(bi))
+        return st, sr, co, fe, bi.value
 
     def _is_nominal(self, idx: int) -> bool:
         return idx == 0

@@ -37,6 +37,12 @@ int launch_two_system(const mppi_dynamics_params& dyn, const mppi_cost_params& c
                       const mppi_sampler_params& sp1, const mppi_two_system_params& tp, int K, int T, float dt,
                       float lambda, float alpha, const float* x0, const float* mean, float* samples, float* costs,
                       hipStream_t s);
+// RMPPI candidate evaluation: N candidates x the first M of K samples [K][T][C], each candidate from states[i][S] with
+// its controls shifted by strides[i]; costs [N][M].
+int launch_candidate_eval(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                          const mppi_sampler_params& sp, int N, int M, int K, int T, float dt, float lambda,
+                          float alpha, const float* states, const int* strides, const float* mean,
+                          const float* samples, float* costs, hipStream_t s);
 // `ws` provides weighted_update_workspace_bytes(K, T, C) bytes of device memory.
 size_t weighted_update_workspace_bytes(int K, int T, int C);
 int launch_weighted_update(const float* w, const mppi_weight_stats* stats, const float* samples, int K, int T, int C,

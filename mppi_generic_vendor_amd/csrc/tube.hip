@@ -51,6 +51,16 @@ struct mppi_tube_controller {
   mppi_free_energy fe[2]{};
   uint64_t rng_counter = 0;
   bool have_samples = false;
+  // RMPPI candidate evaluation
+  int N = 0, M = 0;
+  float* cand_d = nullptr;   // [N][S]
+  int* stride_d = nullptr;   // [N]
+  float* eval_d = nullptr;   // [N][M]
+  std::vector<float> lsw;    // line-search weights [3][N]
+  std::vector<float> cand_h, eval_h, cand_fe;
+  std::vector<int> strides_h;
+  int best_index = 0, nominal_stride = 0;
+  bool isc_init = false;
   bool rmppi() const { return d.mode == MPPI_TWO_SYSTEM_RMPPI; }
   // index of the nominal system in the buffers
   int nom() const { return rmppi() ? 0 : 1; }
@@ -63,7 +73,7 @@ int tube_free(mppi_tube_controller* c)
   if (!c) return MPPI_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* devs[] = {c->x0_d, c->mean_d, c->out_d, c->samples_d, c->costs_d, c->w_d, c->gains_d, c->map_d, c->nn_d,
-                  c->std_t_d, c->goal_d, c->stats_d, c->ws_d};
+                  c->std_t_d, c->goal_d, c->stats_d, c->ws_d, c->cand_d, c->stride_d, c->eval_d};
   for (void* p : devs)
     if (p) (void)hipFree(p);
   if (c->host_io) (void)hipHostFree(c->host_io);
@@ -147,6 +157,31 @@ int tube_setup(mppi_tube_controller* c, const mppi_tube_desc* desc)
     c->states[d].assign((size_t)T * S, 0.0f);
   }
   c->nominal_x.assign(S, 0.0f);
+  if (c->rmppi()) {
+    c->N = desc->num_candidates ? desc->num_candidates : 9;
+    c->M = desc->num_eval_samples ? desc->num_eval_samples : 64;
+    if (c->N < 3 || c->N % 2 == 0 || c->N > 1023)
+      return fail(MPPI_E_INVALID, "RMPPI: num_candidates must be odd, >= 3 and < 1024");
+    if (c->M < 1 || c->M > K) return fail(MPPI_E_INVALID, "RMPPI: num_eval_samples must be in [1, num_rollouts]");
+    MPPI_CHECK(dev_alloc(&c->cand_d, sizeof(float) * (size_t)c->N * S));
+    MPPI_CHECK(dev_alloc(&c->stride_d, sizeof(int) * c->N));
+    MPPI_CHECK(dev_alloc(&c->eval_d, sizeof(float) * (size_t)c->N * c->M));
+    // computeLineSearchWeights (robust_mppi_controller.cu:480-498): [3][N]
+    const int h = c->N / 2;
+    c->lsw.assign((size_t)3 * c->N, 0.0f);
+    for (int i = 0; i < h + 1; ++i) {
+      c->lsw[0 * c->N + i] = 1 - i / float(h);
+      c->lsw[1 * c->N + i] = i / float(h);
+    }
+    for (int i = 1; i < h + 1; ++i) {
+      c->lsw[1 * c->N + h + i] = 1 - i / float(h);
+      c->lsw[2 * c->N + h + i] = i / float(h);
+    }
+    c->cand_h.assign((size_t)c->N * S, 0.0f);
+    c->strides_h.assign(c->N, 0);
+    c->eval_h.assign((size_t)c->N * c->M, 0.0f);
+    c->cand_fe.assign(c->N, 0.0f);
+  }
   b.init_control = nullptr;
   b.cost_map_host = nullptr;
   b.nn_theta_host = nullptr;
@@ -417,6 +452,97 @@ int mppi_tube_set_thresholds(mppi_tube_controller* c, float nominal_threshold, f
     return fail(MPPI_E_INVALID, "thresholds must not be NaN");
   c->d.nominal_threshold = nominal_threshold;
   c->d.value_function_threshold = value_function_threshold;
+  return MPPI_OK;
+}
+
+int mppi_tube_update_importance_sampling_control(mppi_tube_controller* c, const float* x, int32_t stride)
+{
+  if (!c || !x) return fail(MPPI_E_INVALID, "null argument");
+  if (!c->rmppi()) return fail(MPPI_E_INVALID, "updateImportanceSamplingControl: RMPPI controllers only");
+  if (stride < 0) return fail(MPPI_E_INVALID, "stride must be >= 0");
+  const mppi_controller_desc& b = c->b();
+  const int K = c->K, T = c->T, C = c->C, S = c->S, N = c->N, M = c->M, nom = c->nom();
+  const size_t TC = (size_t)T * C;
+  if (!c->isc_init) {
+    std::copy(x, x + S, c->nominal_x.begin());
+    c->isc_init = true;
+    c->nominal_stride = 0;
+  } else {
+    // candidates on the line through the nominal trajectory's first two states and the measured state (:351-363)
+    const float* xk = c->states[nom].data();
+    const float* xkp1 = c->states[nom].data() + (T > 1 ? S : 0);
+    for (int i = 0; i < N; ++i) {
+      const float w0 = c->lsw[i], w1 = c->lsw[N + i], w2 = c->lsw[2 * N + i];
+      for (int s = 0; s < S; ++s) c->cand_h[(size_t)i * S + s] = xk[s] * w0 + xkp1[s] * w1 + x[s] * w2;
+      // computeImportanceSamplerStride (:501-511): round((0, s, s) . w), half away from zero
+      c->strides_h[i] = (int)std::round(0.0f * w0 + (float)stride * w1 + (float)stride * w2);
+    }
+    float* mean_h = c->host_io + 2 * MPPI_MAX_STATE;
+    std::copy(c->control[nom].begin(), c->control[nom].end(), mean_h);
+    MPPI_HIP_CHECK(hipMemcpyAsync(c->cand_d, c->cand_h.data(), sizeof(float) * (size_t)N * S, hipMemcpyHostToDevice,
+                                  c->stream));
+    MPPI_HIP_CHECK(hipMemcpyAsync(c->stride_d, c->strides_h.data(), sizeof(int) * N, hipMemcpyHostToDevice,
+                                  c->stream));
+    MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, mean_h, sizeof(float) * TC, hipMemcpyHostToDevice, c->stream));
+    // generateSamples(stride, 0) around the nominal control (:593-597)
+    const IterRef ref{nullptr, {c->rng_counter, stride, 0}};
+    if (b.sampler.kind == MPPI_SAMPLER_COLORED)
+      MPPI_CHECK(launch_sample_colored(b.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, c->stream));
+    else
+      MPPI_CHECK(launch_sample_gaussian(b.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, c->stream));
+    MPPI_CHECK(launch_candidate_eval(b.dynamics, b.cost, b.sampler, N, M, K, T, b.dt, b.lambda, b.alpha, c->cand_d,
+                                     c->stride_d, c->mean_d, c->samples_d, c->eval_d, c->stream));
+    MPPI_HIP_CHECK(hipMemcpyAsync(c->eval_h.data(), c->eval_d, sizeof(float) * (size_t)N * M, hipMemcpyDeviceToHost,
+                                  c->stream));
+    MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->rng_counter++;
+    // computeCandidateBaseline + computeBestIndex (:513-546), in the reference's arithmetic
+    float base = c->eval_h[0];
+    for (int i = 1; i < N * M; ++i)
+      if (c->eval_h[i] < base) base = c->eval_h[i];
+    for (int i = 0; i < N; ++i) {
+      float fe = 0.0f;
+      for (int j = 0; j < M; ++j)
+        fe += expf((float)(-1.0 / (double)b.lambda * (double)(c->eval_h[(size_t)i * M + j] - base)));
+      fe = (float)(fe / (1.0 * M));
+      fe = -b.lambda * logf(fe) + base;
+      c->cand_fe[i] = fe;
+      if (fe < c->d.value_function_threshold) c->best_index = i;
+    }
+    c->nominal_state_used = c->best_index;
+    c->nominal_stride = c->strides_h[c->best_index];
+    std::copy(c->cand_h.begin() + (size_t)c->best_index * S, c->cand_h.begin() + (size_t)(c->best_index + 1) * S,
+              c->nominal_x.begin());
+  }
+  // slideControlSequenceHelper(nominal_stride, nominal control) and the nominal trajectory (:558-566)
+  {
+    std::vector<float>& u = c->control[nom];
+    const float* zero = b.dynamics.zero_control;
+    const float* scale = b.slide_control_scale;
+    const int steps = c->nominal_stride;
+    for (int i = 0; i < T; ++i) {
+      const int ind = std::min(i + steps, T - 1);
+      for (int j = 0; j < C; ++j) {
+        float v = u[(size_t)ind * C + j];
+        if (i + steps > T - 1) v = (v - zero[j]) * scale[j] + zero[j];
+        u[(size_t)i * C + j] = v;
+      }
+    }
+  }
+  propagate(c->dyn_host, c->nominal_x.data(), c->control[nom].data(), T, b.dt, c->states[nom].data());
+  return MPPI_OK;
+}
+
+int mppi_tube_get_candidates(const mppi_tube_controller* c, float* states, int32_t* strides, float* costs,
+                             float* free_energies, int32_t* best_index)
+{
+  if (!c) return fail(MPPI_E_INVALID, "null argument");
+  if (!c->rmppi()) return fail(MPPI_E_INVALID, "candidates: RMPPI controllers only");
+  if (states) std::copy(c->cand_h.begin(), c->cand_h.end(), states);
+  if (strides) std::copy(c->strides_h.begin(), c->strides_h.end(), strides);
+  if (costs) std::copy(c->eval_h.begin(), c->eval_h.end(), costs);
+  if (free_energies) std::copy(c->cand_fe.begin(), c->cand_fe.end(), free_energies);
+  if (best_index) *best_index = c->best_index;
   return MPPI_OK;
 }
 

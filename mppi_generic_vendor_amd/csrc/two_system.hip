@@ -279,6 +279,92 @@ static int launch_two_system_t(const mppi_dynamics_params& dyn, const mppi_cost_
   return MPPI_OK;
 }
 
+// ====================================================================================== RMPPI candidate evaluation
+// initEvalKernel (core/rmppi_kernels.cu:12-358, the fused form of initEvalDynKernel + initEvalCostKernel): candidate i
+// (workgroup row blockIdx.y) rolls out samples j < M of the sample buffer from its state states[i], reading sample j's
+// controls shifted by the candidate's stride, u_t = samples[j][min(t + stride_i, T - 1)], constrained; the running cost
+// plus the likelihood-ratio cost against the mean at t (the sample index for the pure-noise rule is i M + j);
+// cost = sum / T + terminal / T into costs[i M + j]. The shifted tile is staged in LDS and rolled out by the pipelined
+// rollout of rollout_core.hpp.
+template <int MODEL, int COST>
+__global__ void __launch_bounds__(rollout_threads<MODEL>()) candidate_eval_kernel(
+    mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp, int M, int T, float dt, float lambda,
+    float alpha, const float* __restrict__ states, const int* __restrict__ strides, const float* __restrict__ mean,
+    const float* __restrict__ samples, float* __restrict__ costs, int pstart, int chunk)
+{
+  constexpr int C = Model<MODEL>::C, S = Model<MODEL>::S;
+  constexpr int NT = rollout_threads<MODEL>();
+  extern __shared__ float lds[];
+  const int TC = T * C, ROW = TC + 1;
+  float* tile = lds;
+  float* mu = tile + RC_LANES * ROW;
+  float* x0_l = mu + ((TC + 3) & ~3);
+  float* scratch = x0_l + ((S + 3) & ~3);
+  const int tid = threadIdx.x;
+  const int cand = blockIdx.y;
+  const int k0 = blockIdx.x * RC_LANES;
+  const int nrows = min(RC_LANES, M - k0);
+  const int stride = strides[cand];
+  bool deadband = false;
+#pragma unroll
+  for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
+  for (int e = tid; e < nrows * TC; e += NT) {
+    const int r = e / TC, col = e - r * TC;
+    const int tt = col / C, c = col - tt * C;
+    const float v = samples[(size_t)(k0 + r) * TC + min(tt + stride, T - 1) * C + c];
+    tile[r * ROW + col] = constrain_control(dyn, c, v, deadband);
+  }
+  for (int j = tid; j < TC; j += NT) mu[j] = mean[j];
+  if (tid < S) x0_l[tid] = states[cand * S + tid];
+  __syncthreads();
+  float* cost_l;
+  float* raw_l;
+  rollout_tile<MODEL, COST>(dyn, cost, sp, T, dt, lambda, alpha, x0_l, tile, ROW, mu, pstart, cand * M + k0, scratch,
+                            chunk, cost_l, raw_l, nullptr, RC_LANES);
+  if (tid < nrows) costs[cand * M + k0 + tid] = cost_l[tid];
+}
+
+template <int MODEL, int COST>
+static int launch_candidate_eval_t(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                                   const mppi_sampler_params& sp, int N, int M, int K, int T, float dt, float lambda,
+                                   float alpha, const float* states, const int* strides, const float* mean,
+                                   const float* samples, float* costs, hipStream_t s)
+{
+  constexpr int C = Model<MODEL>::C, S = Model<MODEL>::S, O = Model<MODEL>::O;
+  const size_t base = (size_t)RC_LANES * (T * C + 1) + (size_t)((T * C + 3) & ~3) + (size_t)((S + 3) & ~3);
+  const size_t limit = LDS_LIMIT / sizeof(float);
+  const int chunk = base < limit ? rollout_chunk_for<MODEL, O>(limit - base) : 0;
+  if (chunk == 0) return fail(MPPI_E_SIZE, "candidate evaluation: horizon too long for the LDS control tile");
+  const size_t lds = sizeof(float) * (base + rollout_scratch_for<MODEL, O>(chunk));
+  MPPI_CHECK(allow_max_dynamic_lds((const void*)candidate_eval_kernel<MODEL, COST>));
+  const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
+  const dim3 grid((M + RC_LANES - 1) / RC_LANES, N);
+  candidate_eval_kernel<MODEL, COST><<<grid, rollout_threads<MODEL>(), lds, s>>>(
+      dyn, cost, sp, M, T, dt, lambda, alpha, states, strides, mean, samples, costs, pstart, chunk);
+  MPPI_HIP_CHECK(hipGetLastError());
+  return MPPI_OK;
+}
+
+int launch_candidate_eval(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                          const mppi_sampler_params& sp, int N, int M, int K, int T, float dt, float lambda,
+                          float alpha, const float* states, const int* strides, const float* mean,
+                          const float* samples, float* costs, hipStream_t s)
+{
+  if (N < 1 || M < 1 || M > K) return fail(MPPI_E_INVALID, "candidate evaluation: counts out of range");
+  if (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
+    return fail(MPPI_E_UNSUPPORTED, "candidate evaluation: the BF16 network model is not supported");
+#define MPPI_CE_CASE(MODEL, COST)                                                                                  \
+  if (dyn.model == MODEL && cost.cost == COST)                                                                    \
+    return launch_candidate_eval_t<MODEL, COST>(dyn, cost, sp, N, M, K, T, dt, lambda, alpha, states, strides,   \
+                                                mean, samples, costs, s);
+  MPPI_CE_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
+  MPPI_CE_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
+  MPPI_CE_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)
+  MPPI_CE_CASE(MPPI_MODEL_RACER_DUBINS_NN, MPPI_COST_QUADRATIC_MAP)
+#undef MPPI_CE_CASE
+  return fail(MPPI_E_UNSUPPORTED, "candidate evaluation: model/cost combination is not compiled into the library");
+}
+
 int launch_two_system(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp0,
                       const mppi_sampler_params& sp1, const mppi_two_system_params& tp, int K, int T, float dt,
                       float lambda, float alpha, const float* x0, const float* mean, float* samples, float* costs,

@@ -30,6 +30,7 @@
  *   - two-system rollouts     Tube-MPPI: controllers/Tube-MPPI/tube_mppi_controller.cu:158-236 (two independent
  *                             rollouts); RMPPI: tests/include/kernel_tests/core/rmppi_kernel_test.cu:7-70,
  *                             core/rmppi_kernels.cu:666-798, feedback cost gaussian.cu:632-647 (rollout_loop.hpp)
+ *   - RMPPI candidate eval    core/rmppi_kernels.cu:12-358, rmppi_kernel_test.cu:73-127 (rollout_loop.hpp)
  *   - state trajectory        controllers/controller.cuh:664-684
  *   - Philox4x32-10           Salmon et al., "Parallel random numbers: as easy as 1, 2, 3", SC'11
  *   - Box-Muller              Box & Muller 1958
@@ -536,6 +537,16 @@ void oracle_rollout_two_system_f32(const mppi_dynamics_params* dyn, const mppi_c
     });
   }
   for (auto& t : th) t.join();
+}
+
+/* RMPPI candidate evaluation (candidate_eval_loop, rollout_loop.hpp). */
+void oracle_candidate_eval_f32(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,
+                               const mppi_sampler_params* smp, int N, int M, int K, int T, float dt, float lambda,
+                               float alpha, const float* states, const int* strides, const float* mean,
+                               const float* samples, float* costs)
+{
+  const BuiltinSystem<float> sys{*dyn, *cst, map};
+  candidate_eval_loop<float>(sys, *smp, N, M, K, T, dt, lambda, alpha, states, strides, mean, samples, costs);
 }
 
 void oracle_rollout_trace_f32(const mppi_dynamics_params* dyn, const mppi_cost_params* cst, const float* map,

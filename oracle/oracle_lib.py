@@ -56,6 +56,8 @@ _SIGS = {
                             ct.POINTER(capi.SamplerParams), _i, _i, _d, _d, _d, P, P, P, P, _i], None),
     "oracle_rollout_two_system_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P, P, _i, _f,
                                        P, _i, _i, _f, _f, _f, P, P, P, P, P, _i], None),
+    "oracle_candidate_eval_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
+                                   ct.POINTER(capi.SamplerParams), _i, _i, _i, _i, _f, _f, _f, P, P, P, P, P], None),
     "oracle_rollout_trace_f32": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
                                   ct.POINTER(capi.SamplerParams), _i, _i, _f, _f, _f, P, P, P, P, P, P, P], None),
     "oracle_rollout_trace_f64": ([ct.POINTER(capi.DynamicsParams), ct.POINTER(capi.CostParams), P,
@@ -172,6 +174,21 @@ def rollout_two_system(prob, samples: np.ndarray, means: np.ndarray, x0s: np.nda
                                          smp, int(nominal_index), float(vft), ptr(g), K, T, prob.dt, prob.lambda_,
                                          prob.alpha, ptr(_a0), ptr(_a1), ptr(s), ptr(costs), ptr(pr), nthreads)
     return (costs, s, pr) if parts else (costs, s)
+
+
+def candidate_eval(prob, samples: np.ndarray, mean: np.ndarray, states: np.ndarray, strides, M: int):
+    """RMPPI candidate evaluation over the first M samples of samples [K][T][C]: costs [N][M]."""
+    K, T = samples.shape[0], samples.shape[1]
+    st = np.ascontiguousarray(states, np.float32)
+    N = st.shape[0]
+    sr = np.ascontiguousarray(strides, np.int32)
+    s = np.ascontiguousarray(samples, np.float32)
+    m = np.ascontiguousarray(mean, np.float32)
+    out = np.empty((N, M), np.float32)
+    load().oracle_candidate_eval_f32(ct.byref(_host_dyn(prob)), ct.byref(_host_cost(prob)), ptr(prob.cost_map),
+                                     ct.byref(_host_smp(prob)), N, M, K, T, prob.dt, prob.lambda_, prob.alpha,
+                                     ptr(st), ptr(sr), ptr(m), ptr(s), ptr(out))
+    return out
 
 
 def rollout_f64(prob, samples: np.ndarray, mean: np.ndarray, x0=None, nthreads: int = 8):

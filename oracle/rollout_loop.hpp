@@ -206,6 +206,36 @@ void two_system_loop(const Sys& sys, const mppi_sampler_params* smp, int nom, R 
   }
 }
 
+/* RMPPI candidate evaluation (initEvalKernel, core/rmppi_kernels.cu:12-358; the CPU form of
+ * tests/include/kernel_tests/core/rmppi_kernel_test.cu:73-127 with the GPU kernel's likelihood-ratio arguments, the
+ * mean at t and the sample index i M + j): candidate i rolls out samples j < M from states[i] with the controls
+ * samples[j][min(t + strides[i], T - 1)], constrained; costs[i M + j] = sum_t (running + lr) / T + terminal / T. */
+template <class R, class Sys>
+void candidate_eval_loop(const Sys& sys, const mppi_sampler_params& smp, int N, int M, int K, int T, R dt, R lambda,
+                         R alpha, const float* states, const int* strides, const float* mean, const float* samples,
+                         R* costs)
+{
+  const int S = sys.state_dim(), C = sys.control_dim();
+  const int pstart = pure_noise_start(K, smp.pure_noise_trajectories_percentage);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < M; ++j) {
+      R x[LOOP_MAX_DIM] = {0}, xn[LOOP_MAX_DIM] = {0}, xd[LOOP_MAX_DIM] = {0}, y[LOOP_MAX_DIM] = {0};
+      R u[LOOP_MAX_DIM];
+      for (int s = 0; s < S; ++s) x[s] = states[i * S + s];
+      R run = 0;
+      for (int t = 0; t < T; ++t) {
+        const float* ut = samples + ((size_t)j * T + std::min(t + strides[i], T - 1)) * C;
+        for (int c = 0; c < C; ++c) u[c] = ut[c];
+        sys.enforce(u);
+        sys.step(x, xn, xd, u, y, dt);
+        run += sys.running(y, u, t) +
+               lr_cost<R>(smp, C, u, mean + (size_t)t * C, i * M + j >= pstart, lambda, alpha, t);
+        for (int s = 0; s < S; ++s) x[s] = xn[s];
+      }
+      costs[(size_t)i * M + j] = run / (R)T + sys.terminal(y) / (R)T;
+    }
+}
+
 /* The loop split over `nthreads` std::thread workers by sample range (the multi-core CPU baseline). */
 template <class R, class Sys>
 void rollout_loop_threads(const Sys& sys, const mppi_sampler_params& smp, int K, int T, R dt, R lambda, R alpha,

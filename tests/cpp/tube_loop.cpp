@@ -2,11 +2,11 @@
 // controllers' model does not know) towards a goal at (2, 2):
 //   - TubeMPPIController: the actual system is controlled with getActualControlSeq's first control; the loop checks
 //     that the state approaches the goal and that the nominal system was replaced by the actual one at least once;
-//   - RobustMPPIController with PD feedback gains (GainFeedback): the nominal state advances along the nominal
-//     trajectory each step (the caller's choice of nominal state; the optimisation runs with stride 0 so that the
-//     first control is optimised too, the nominal sequence not being slid), the real system applies the real control
-//     plus getFeedbackControl(x, x_nominal, 0); the loop checks getFeedbackControl against the PD formula, that the
-//     real system approaches the goal and that it stays within a tube around the nominal one.
+//   - RobustMPPIController with PD feedback gains (GainFeedback), in the reference's order: updateImportanceSampling-
+//     Control (the candidate evaluation that picks the nominal state and slides the nominal control), computeControl,
+//     then the real system applies the real control plus getFeedbackControl(x, x_nominal, 0); the loop checks
+//     getFeedbackControl against the PD formula, that the real system approaches the goal, that it stays within a tube
+//     around the nominal one, and that the evaluation chose a candidate other than the first at some step.
 // Exit status 0 when every check holds.
 #include <cmath>
 #include <cstdio>
@@ -96,12 +96,15 @@ int main()
     fb.setGains(gains);
     RMPPI ctrl(&model, &cost, &fb, &sampler, dt, 1, 1.0f, 0.0f);
     ctrl.setValueFunctionThreshold(50.0f);
-    DYN::state_array x = DYN::state_array::Zero(), xn = DYN::state_array::Zero();
-    ctrl.setNominalState(xn);
+    DYN::state_array x = DYN::state_array::Zero(), xn;
     float max_gap = 0.0f;
     bool formula = true;
+    int moved = 0;
     for (int k = 0; k < 200; ++k) {
-      ctrl.computeControl(x, 0);
+      ctrl.updateImportanceSamplingControl(x, 1);
+      moved += ctrl.getBestIndex() != 0;
+      xn = ctrl.getNominalState();
+      ctrl.computeControl(x, 1);
       const auto u = ctrl.getControlSeq();
       const auto fbu = ctrl.getFeedbackControl(x, xn, 0);
       const float want0 = -kp * (x[0] - xn[0]) - kd * (x[2] - xn[2]);
@@ -111,14 +114,12 @@ int main()
       DYN::control_array ua;
       for (int c = 0; c < 2; ++c) ua[c] = u(c, 0) + fbu[c];
       x = plant_step(model, x, ua, dt);
-      // the nominal state moves along the nominal trajectory
-      const auto ns = ctrl.getNominalStateSeq();
-      for (int i = 0; i < DYN::STATE_DIM; ++i) xn[i] = ns(i, 1);
-      ctrl.setNominalState(xn);
       max_gap = std::fmax(max_gap, std::hypot(x[0] - xn[0], x[1] - xn[1]));
     }
     const float dist = std::hypot(x[0] - 2.0f, x[1] - 2.0f);
-    std::printf("[tube_loop] RMPPI: final distance %.3f, largest real-nominal gap %.3f\n", dist, max_gap);
+    std::printf("[tube_loop] RMPPI: final distance %.3f, largest real-nominal gap %.3f, candidate > 0 in %d calls\n",
+                dist, max_gap, moved);
+    check_that(moved > 0, "RMPPI candidate evaluation moved the nominal state along the line");
     check_that(formula, "getFeedbackControl is the PD law K_t (x - x_nominal)");
     check_that(dist < 0.35f * std::hypot(2.0f, 2.0f), "RMPPI real system approaches the goal");
     check_that(max_gap < 0.5f, "RMPPI real system stays in a tube around the nominal one");

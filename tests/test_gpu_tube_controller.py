@@ -153,3 +153,65 @@ def test_tube_closed_loop_tracks_goal(hip, engine):
     assert dist < 0.5 * dist0
     assert gap < dist0
     ctrl.close()
+
+
+def _line_search_weights(N):
+    h = N // 2
+    w = np.zeros((3, N), np.float32)
+    for i in range(h + 1):
+        w[0, i], w[1, i] = 1 - i / np.float32(h), i / np.float32(h)
+    for i in range(1, h + 1):
+        w[1, h + i], w[2, h + i] = 1 - i / np.float32(h), i / np.float32(h)
+    return w
+
+
+def test_rmppi_candidate_selection_matches_oracle(hip, engine):
+    """updateImportanceSamplingControl (robust_mppi_controller.cu:548-628): candidates on the line through the nominal
+    trajectory's first two states and the measured state, strides round(w1 s + w2 s), the evaluation costs against the
+    oracle's candidate rollouts on the same normals, the free energies and the chosen candidate (the last below the
+    value-function threshold), then the nominal state, the slid nominal control and the nominal trajectory."""
+    prob = _problem(K=1024, T=40)
+    T, C, S = prob.num_timesteps, prob.C, prob.S
+    N, M = 9, 64
+    banner("RMPPI candidate selection")
+    rng = np.random.default_rng(6)
+    u0 = (rng.normal(size=(T, C)) * 0.3).astype(np.float32)
+    ctrl = RobustMPPIController(prob, init_control=u0, value_function_threshold=1e9, num_candidates=N,
+                                num_eval_samples=M)
+    x = prob.x0.copy()
+    ctrl.updateImportanceSamplingControl(x, 1)  # the first call takes the measured state, stride 0
+    np.testing.assert_array_equal(ctrl.getNominalState(), x)
+    ctrl.computeControl(x, 1)  # consumes normals offset 0
+    nom_traj = ctrl.getTargetStateSeq()
+    nom_u = ctrl.getNominalControlSeq()
+    x2 = (x + rng.normal(size=S).astype(np.float32) * 0.05).astype(np.float32)
+    stride = 3
+    ctrl.updateImportanceSamplingControl(x2, stride)  # offset 1
+    st, sr, co, fe, best = ctrl.getCandidates()
+    w = _line_search_weights(N)
+    want_states = (nom_traj[0][:, None] * w[0] + nom_traj[1][:, None] * w[1] + x2[:, None] * w[2]).T
+    np.testing.assert_allclose(st, want_states, rtol=1e-6, atol=1e-6)
+    np.testing.assert_array_equal(sr, np.round(stride * w[1] + stride * w[2]).astype(np.int32))
+    K = prob.num_rollouts
+    eps = gpu_normals(prob.sampler.seed, 1, K * T * C).reshape(K, T, C)
+    samples = ol.gaussian_transform(prob.sampler, K, T, C, nom_u, eps, stride, 0)
+    want_costs = ol.candidate_eval(prob, samples, nom_u, st, sr, M)
+    e = rel_err(co, want_costs)
+    print(f"[tube] candidate evaluation costs rel err {e:.2e}, strides {sr.tolist()}")
+    assert e < 1e-5
+    b = float(co.min())
+    want_fe = -prob.lambda_ * np.log(np.mean(np.exp(-(co.astype(np.float64) - b) / prob.lambda_), axis=1)) + b
+    np.testing.assert_allclose(fe, want_fe, rtol=1e-5, atol=1e-5)
+    assert best == N - 1  # every free energy is below the threshold: the last candidate (the measured state)
+    np.testing.assert_allclose(ctrl.getNominalState(), st[best], rtol=0, atol=0)
+    s = int(sr[best])
+    np.testing.assert_array_equal(ctrl.getNominalControlSeq()[:T - s], nom_u[s:])
+    np.testing.assert_allclose(ctrl.getTargetStateSeq(), ol.state_trajectory(prob, st[best], ctrl.getNominalControlSeq()),
+                               rtol=1e-4, atol=1e-4)
+    # a threshold between the free energies chooses the last candidate below it
+    ctrl.setValueFunctionThreshold(float(np.median(fe)) + 1e-6)
+    ctrl.updateImportanceSamplingControl(x2, stride)
+    _, _, _, fe2, best2 = ctrl.getCandidates()
+    below = np.nonzero(fe2 < np.float32(ctrl.getValueFunctionThreshold()))[0]
+    assert best2 == (below[-1] if below.size else N - 1)
+    ctrl.close()

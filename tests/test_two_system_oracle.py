@@ -131,3 +131,24 @@ def test_two_system_known_sizes_and_modes(oracle, engine):
         with pytest.raises(RuntimeError, match="two-system"):
             capi.call("mppi_rollout_two_system", prob.dyn, prob.cost, prob.sampler, None, tp, 8, 4, 0.02, 1.0, 0.0,
                       1, 1, 1, 1, None)
+
+
+def test_candidate_eval_is_a_shifted_rollout(oracle):
+    """RMPPI's candidate evaluation (oracle candidate_eval_loop): candidate i with stride s over the first M samples is
+    the plain rollout from the candidate's state over samples shifted in numpy, u'[j][t] = u[j][min(t + s, T - 1)],
+    with the mean unshifted; bit for bit. (K >= N M: the evaluation's sample index i M + j takes part in the pure-noise
+    rule of the sampler's K samples, as in the reference.)"""
+    prob = _di(K=256, T=30)
+    prob.sampler.pure_noise_trajectories_percentage = 0.0
+    rng = np.random.default_rng(9)
+    K, T, C, S, M = 256, 30, 2, 4, 40
+    u = rng.normal(size=(K, T, C)).astype(np.float32)
+    mean = (rng.normal(size=(T, C)) * 0.4).astype(np.float32)
+    states = rng.uniform(-1, 1, size=(5, S)).astype(np.float32)
+    strides = np.array([0, 1, 2, 5, 29], np.int32)
+    got = ol.candidate_eval(prob, u, mean, states, strides, M)
+    for i in range(5):
+        idx = np.minimum(np.arange(T) + strides[i], T - 1)
+        shifted = np.ascontiguousarray(u[:M][:, idx])
+        want, _ = ol.rollout_f32(prob, shifted, mean, x0=states[i])
+        assert np.array_equal(got[i], want), i
