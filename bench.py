@@ -22,6 +22,7 @@ The same line carries, from the same run:
 from __future__ import annotations
 
 import argparse
+import ctypes as ct
 import json
 import os
 import statistics
@@ -359,6 +360,121 @@ def learned_comparison(episodes: int, use_graph: bool = True):
             "cost_gain_over_same_samples": round((s["cost"] - l["cost"]) / s["cost"], 6)}
 
 
+# -------------------------------------------------------------------------------------- two-system (Tube / RMPPI)
+def two_system_problem():
+    from mppi_generic_vendor_amd import workloads
+
+    return workloads.double_integrator_map(num_rollouts=8192, num_timesteps=100, colored=False)
+
+
+def pd_gains(T: int, kp: float = 4.0, kd: float = 2.0) -> np.ndarray:
+    """Feedback gains [T][S][C] of a PD law on the double integrator: a_x = -kp e_px - kd e_vx, a_y likewise."""
+    g = np.zeros((T, 4, 2), np.float32)
+    g[:, 0, 0] = g[:, 1, 1] = -kp
+    g[:, 2, 0] = g[:, 3, 1] = -kd
+    return g
+
+
+def run_two_system(args, host_calls: int, cpu_seconds: float):
+    """Tube-MPPI and RMPPI on the double integrator over the 1024x1024 64-blob map, K=8192, T=100: host-inclusive control
+    rates of both controllers (RMPPI: candidate evaluation + optimisation per call), the two-system rollout kernel's
+    duration in RMPPI mode (HIP events on its stream), its roofline over the algorithmic bytes of the two systems'
+    sample reads and writes and costs, 2 (8 K T C + 4 K) per launch, and the oracle's CPU loop."""
+    from mppi_generic_vendor_amd import capi, hip
+    from mppi_generic_vendor_amd.controller import RobustMPPIController, TubeMPPIController
+
+    prob = two_system_problem()
+    K, T, C, S = prob.num_rollouts, prob.num_timesteps, prob.C, prob.S
+    gains = pd_gains(T)
+    rates = {}
+    for name in ("tube", "rmppi"):
+        ctrl = TubeMPPIController(prob) if name == "tube" else RobustMPPIController(prob, value_function_threshold=50.0)
+        if name == "rmppi":
+            ctrl.setFeedbackGains(gains)
+        x = prob.x0.copy()
+
+        def call():
+            if name == "rmppi":
+                ctrl.updateImportanceSamplingControl(x, 1)
+            ctrl.computeControl(x, 1)
+            if name == "tube":
+                ctrl.slideControlSequence(1)
+
+        for _ in range(10):
+            call()
+        t0 = time.perf_counter()
+        for _ in range(host_calls):
+            call()
+        rates[name] = host_calls / (time.perf_counter() - t0)
+        ctrl.close()
+    # the kernel alone, RMPPI mode, on device buffers
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=(2, K, T, C)).astype(np.float32)
+    means = np.zeros((2, T, C), np.float32)
+    x0s = np.stack([prob.x0, prob.x0 + np.float32(0.1)]).astype(np.float32)
+    from mppi_generic_vendor_amd.hip import DeviceBuffer
+
+    mb_map = DeviceBuffer.from_numpy(prob.cost_map.astype(np.float32))
+    cost = capi.CostParams.from_buffer_copy(prob.cost)
+    cost.map = mb_map.ptr
+    gb, xb = DeviceBuffer.from_numpy(gains), DeviceBuffer.from_numpy(x0s)
+    mb, sb, cb = DeviceBuffer.from_numpy(means), DeviceBuffer.from_numpy(u), DeviceBuffer(2 * K * 4)
+    tp = capi.TwoSystemParams(capi.TWO_SYSTEM_RMPPI, 0, 50.0, 0, gb.ptr)
+    st = hip.Stream()
+    e0, e1 = hip.Event(), hip.Event()
+
+    def launch():
+        capi.call("mppi_rollout_two_system", ct.byref(prob.dyn), ct.byref(cost), ct.byref(prob.sampler), None,
+                  ct.byref(tp), K, T, prob.dt, prob.lambda_, prob.alpha, xb.ptr, mb.ptr, sb.ptr, cb.ptr, st.ptr)
+
+    for _ in range(3):
+        launch()
+    reps = max(args.kernel_reps, 5)
+    e0.record(st)
+    for _ in range(reps):
+        launch()
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    alg = 2 * (8 * K * T * C + 4 * K)
+    ach = alg / (ms * 1e-3) / 1e9
+    log(f"two-system: Tube {rates['tube']:.1f} Hz, RMPPI {rates['rmppi']:.1f} Hz host-inclusive; RMPPI rollout kernel "
+        f"{ms:.4f} ms = {ach:.1f} GB/s")
+    cpu = None
+    if cpu_seconds > 0:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle_lib as ol
+
+        Ks = 512
+        sub = np.ascontiguousarray(u[:, :Ks])
+        res = {}
+        for n in (host_threads(), 1):
+            reps_c, t0 = 0, time.perf_counter()
+            while True:
+                ol.rollout_two_system(prob, sub, means, x0s, 0, 50.0, gains, nthreads=n)
+                reps_c += 1
+                el = time.perf_counter() - t0
+                if el >= cpu_seconds / 2 or reps_c >= 5000:
+                    break
+            per = el / (reps_c * Ks)
+            res[n] = {"value": round(1.0 / (per * K), 4), "unit": "Hz", "cores": n, "kind": "port",
+                      "sample": f"RMPPI two-system rollout of {Ks} of the {K} samples (both systems, full T), "
+                                f"{reps_c} pass(es) in {el:.1f} s, scaled to K = {K}"}
+        cpu = res[host_threads()]
+        cpu["single_core"] = res[1]
+    mb_map.free()
+    return {"workload": "Tube-MPPI / RMPPI, double integrator over the 1024x1024 64-blob map, K=8192, T=100 (two "
+                        "systems per sample), PD feedback gains (RMPPI)",
+            "tube_control_hz_host": round(rates["tube"], 3), "rmppi_control_hz_host": round(rates["rmppi"], 3),
+            "kernels_ms": {"two_system_rollout_kernel<2, 2, 0>": round(ms, 5)},
+            "roofline": {"kernel": "two_system_rollout_kernel<2, 2, 0>", "bound": "hbm", "achieved": round(ach, 2),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                         "algorithmic_bytes": alg, "avg_ms": round(ms, 5),
+                         "traffic": measured_traffic("tube_di", "two_system_rollout_kernel")},
+            "utilisation": measured_utilisation("tube_di", ["two_system_rollout_kernel"]),
+            "cpu_baseline_rollout": cpu}
+
+
 CONTROLLER_WORKLOADS = ("cartpole", "di_colored_map", "bicycle_nn")
 
 
@@ -367,7 +483,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="batch256", choices=["batch256", *CONTROLLER_WORKLOADS])
+    ap.add_argument("--workload", default="batch256", choices=["batch256", "tube_di", *CONTROLLER_WORKLOADS])
     ap.add_argument("--host-calls", type=int, default=200,
                     help="timed computeControl calls of each configuration in `configs`")
     ap.add_argument("--kernel-reps", type=int, default=20)
@@ -407,6 +523,13 @@ def main():
 
     if args.workload == "batch256":
         line = run_batch(args)
+    elif args.workload == "tube_di":
+        r = run_two_system(args, args.steps, 0 if args.no_cpu else args.cpu_seconds)
+        line = {"metric": METRIC, "value": r["tube_control_hz_host"], "unit": "Hz", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32", "data": "synthetic (seeded cost map, Philox noise)",
+                "config": {"workload": r["workload"] + "; one step = one host-inclusive Tube computeControl(x, 1)"}}
+        line.update({k: v for k, v in r.items() if k != "workload"})
     else:
         r = run_controller(args.workload, args, args.steps, args.warmup, 0 if args.no_cpu else args.cpu_seconds)
         prob = build_problem(args.workload)
@@ -428,6 +551,9 @@ def main():
             r = run_controller(name, args, args.host_calls, 20, 0 if args.no_cpu else args.config_cpu_seconds)
             r.pop("_wall")
             line["configs"][name] = r
+        if args.workload != "tube_di":
+            line["two_system"] = run_two_system(args, min(args.host_calls, 100),
+                                                0 if args.no_cpu else args.config_cpu_seconds)
     line["learned_vs_plain"] = learned_comparison(args.learned_episodes, not args.learned_eager)
     print(json.dumps(line), flush=True)
 
