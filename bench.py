@@ -490,8 +490,8 @@ def run_two_system(args, host_calls: int, cpu_seconds: float):
             "roofline": {"kernel": "two_system_rollout_kernel<2, 2, 0>", "bound": "hbm", "achieved": round(ach, 2),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                          "algorithmic_bytes": alg, "avg_ms": round(ms, 5),
-                         "traffic": measured_traffic("tube_di", "two_system_rollout_kernel")},
-            "utilisation": measured_utilisation("tube_di", ["two_system_rollout_kernel"]),
+                         "traffic": measured_traffic("tube_di", "two_system_rollout_kernel<2, 2, 0>")},
+            "utilisation": measured_utilisation("tube_di", ["two_system_rollout_kernel", "candidate_eval_kernel"]),
             "cpu_baseline_rollout": cpu}
 
 
