@@ -176,19 +176,27 @@ def run_batch(args):
         cfg = b.autotune(reps=3)
     log(f"batch256: {sum(p.num_rollouts for p in probs)} samples, {KT} sample-steps per call; "
         f"threads={cfg.threads} samples/lane={cfg.samples_per_lane} chunk={cfg.chunk} blocks/CU={cfg.blocks_per_cu}")
-    x0 = [p.x0 for p in probs]
-    for _ in range(args.warmup):
-        b.computeControl(x0, 1)
+    x0 = b.packStates([p.x0 for p in probs])
+
+    def host_calls():
+        for _ in range(args.warmup):
+            b.computeControl(x0, 1)
+        hip.synchronize()
+        per = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            b.computeControl(x0, 1)
+            per.append(time.perf_counter() - t1)
+        hip.synchronize()
+        return time.perf_counter() - t0, per
+
+    # the same calls replayed from a captured hipGraph, for comparison with the direct launches
+    b.setGraphReplay(True)
+    graph_wall, _ = host_calls()
+    b.setGraphReplay(False)
     # timed region of `value`: exactly --steps host-inclusive control calls
-    hip.synchronize()
-    per = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t1 = time.perf_counter()
-        b.computeControl(x0, 1)
-        per.append(time.perf_counter() - t1)
-    hip.synchronize()
-    wall = time.perf_counter() - t0
+    wall, per = host_calls()
     value = args.steps / wall
     log(f"host-inclusive batched computeControl: {wall * 1e3 / args.steps:.4f} ms/call")
     for i in range(len(probs)):
@@ -234,14 +242,15 @@ def run_batch(args):
         "data": "synthetic (seeded map, states, goals; Philox noise)",
         "config": {"workload": "batch256: 256 MPPI instances (double integrator / bicycle 50/50), "
                                "K~U[512,8192]/64, T~U[50,150], 1024x1024 cost map; one step = one host-inclusive "
-                               "batched computeControl (x0 upload, persistent rollout + combine, control download "
-                               "and smoothing)",
+                               "batched computeControl (states from host memory, persistent rollout + combine with "
+                               "smoothing and constraints, controls back to host memory)",
                    "instances": len(probs), "sample_steps_per_call": KT,
                    "launch_shape": {"threads": cfg.threads, "samples_per_lane": cfg.samples_per_lane,
                                     "chunk": cfg.chunk, "blocks_per_cu": cfg.blocks_per_cu}},
         "rollout_steps_per_s": round(value * KT, 1),
         "instance_controls_per_s": round(value * len(probs), 1),
         "control_hz_host_median": round(1.0 / statistics.median(per), 3),
+        "control_hz_host_graph": round(args.steps / graph_wall, 3),
         "device_loop_hz": round(n_dev / dev_wall, 3),
         "device_loop_rollout_steps_per_s": round(n_dev / dev_wall * KT, 1),
         "kernels_ms": {kname: round(roll_ms, 5), "batch_combine_kernel": round(comb_ms, 5)},
@@ -264,18 +273,26 @@ def run_controller(name: str, args, steps: int, warmup: int, cpu_seconds: float)
 
     prob = build_problem(name)
     K, T, C = prob.num_rollouts, prob.num_timesteps, prob.C
-    ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True)
     x = prob.x0.copy()
-    for _ in range(warmup):
-        ctrl.computeControl(x, 1)
-    hip.synchronize()
-    per = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        t1 = time.perf_counter()
-        ctrl.computeControl(x, 1)
-        per.append(time.perf_counter() - t1)
-    wall = time.perf_counter() - t0
+
+    def host_rate(ctrl):
+        for _ in range(warmup):
+            ctrl.computeControl(x, 1)
+        hip.synchronize()
+        per = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            ctrl.computeControl(x, 1)
+            per.append(time.perf_counter() - t1)
+        return time.perf_counter() - t0, per
+
+    # the same calls replayed from a captured hipGraph, for comparison with the direct launches
+    gctrl = VanillaMPPIController(prob, num_iters=1, use_graph=True)
+    graph_wall, _ = host_rate(gctrl)
+    gctrl.close()
+    ctrl = VanillaMPPIController(prob, num_iters=1, use_graph=False)
+    wall, per = host_rate(ctrl)
     st = ctrl.getWeightStats()
     if not np.isfinite(st.normalizer) or st.normalizer <= 0:
         raise SystemExit(f"{name}: invalid normalizer")
@@ -318,6 +335,7 @@ def run_controller(name: str, args, steps: int, warmup: int, cpu_seconds: float)
         "workload": f"{name}: K={K}, T={T}, S={prob.S}, C={C}, dt={prob.dt}, num_iters=1",
         "control_hz_host": round(steps / wall, 3),
         "control_hz_host_median": round(1.0 / statistics.median(per), 3),
+        "control_hz_host_graph": round(steps / graph_wall, 3),
         "rollout_steps_per_s": round(steps / wall * K * T, 1),
         "device_loop_hz": round(n_dev / dev_wall, 3),
         "device_loop_rollout_steps_per_s": round(n_dev / dev_wall * K * T, 1),

@@ -539,8 +539,14 @@ int mppi_batch_destroy(mppi_batch* batch);
 int mppi_batch_autotune(mppi_batch* batch, int32_t reps, mppi_batch_config* best);
 int mppi_batch_set_config(mppi_batch* batch, const mppi_batch_config* config);
 int mppi_batch_get_config(const mppi_batch* batch, mppi_batch_config* config);
-/* One computeControl for every instance: x0_host is [num_instances][MPPI_MAX_STATE]. */
+/* One computeControl for every instance: x0_host is [num_instances][MPPI_MAX_STATE]. The inputs are read by the GPU
+ * from mapped host memory, the new means are smoothed and constrained on the GPU (the next call's nominal stays in
+ * device memory unless the host changes it) and the results written back to mapped host memory; the host polls a
+ * completion word written by the last kernel. */
 int mppi_batch_compute_control(mppi_batch* batch, const float* x0_host, int32_t optimization_stride);
+/* 1: computeControl replays its launches from a captured hipGraph; 0 (default): direct launches, which measured
+ * faster on ROCm 7 (a hipGraphLaunch costs more host time than the three launches it replaces). */
+int mppi_batch_set_graph_replay(mppi_batch* batch, int32_t enable);
 /* num_calls device-resident batched iterations from a captured hipGraph (x0 and means already in device memory,
  * generator counters advanced on the device). */
 int mppi_batch_step_device(mppi_batch* batch, int32_t num_calls);

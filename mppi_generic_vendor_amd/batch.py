@@ -50,11 +50,25 @@ class MPPIBatch:
         self.n = n
 
     # ------------------------------------------------------------------------------------------------ control
-    def computeControl(self, x0: np.ndarray, stride: int = 1):
+    def packStates(self, x0) -> np.ndarray:
+        """The instances' states (a list of per-instance vectors) as the [n][MAX_STATE] array computeControl takes."""
         x = np.zeros((self.n, capi.MAX_STATE), np.float32)
         for i, p in enumerate(self.problems):
             x[i, :p.S] = np.asarray(x0[i], np.float32)[:p.S]
+        return x
+
+    def computeControl(self, x0, stride: int = 1):
+        """x0: a float32 [n][MAX_STATE] array (used as is), or a list of per-instance state vectors."""
+        if isinstance(x0, np.ndarray) and x0.dtype == np.float32 and x0.shape == (self.n, capi.MAX_STATE) \
+                and x0.flags.c_contiguous:
+            x = x0
+        else:
+            x = self.packStates(x0)
         capi.call("mppi_batch_compute_control", self._h, x.ctypes.data, stride)
+
+    def setGraphReplay(self, enable: bool):
+        """Replay computeControl's launches from a captured hipGraph (default: direct launches)."""
+        capi.call("mppi_batch_set_graph_replay", self._h, 1 if enable else 0)
 
     def stepDevice(self, num_calls: int):
         capi.call("mppi_batch_step_device", self._h, num_calls)

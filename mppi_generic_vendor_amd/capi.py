@@ -195,6 +195,7 @@ _SIGS = {
     "mppi_batch_set_config": ([P, ct.POINTER(BatchConfig)], ct.c_int),
     "mppi_batch_get_config": ([P, ct.POINTER(BatchConfig)], ct.c_int),
     "mppi_batch_compute_control": ([P, P, _i], ct.c_int),
+    "mppi_batch_set_graph_replay": ([P, _i], ct.c_int),
     "mppi_batch_step_device": ([P, _i], ct.c_int),
     "mppi_batch_get_control_seq": ([P, _i, P], ct.c_int),
     "mppi_batch_get_raw_mean": ([P, _i, P], ct.c_int),

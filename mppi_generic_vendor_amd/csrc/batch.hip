@@ -360,18 +360,43 @@ __global__ void __launch_bounds__(NT) batch_rollout_kernel(const BatchInstance* 
   }
 }
 
+// Where the combine also writes a call's results (device-side aliases of mapped host memory; null members are
+// skipped): raw means and constrained controls [total_mean], statistics [n], then the host completion signal.
+struct BatchCopies {
+  float* mean = nullptr;
+  float* control = nullptr;
+  mppi_weight_stats* stats = nullptr;
+  HostSignal signal;
+};
+
+// Savitzky-Golay taps of smooth_control (controllers/controller.cuh:573-607), in the same FP32 order with products and
+// sums rounded separately (no FMA contraction), so the device's smoothed sequence is bit-identical to the host's.
+__device__ __forceinline__ float sg_smooth_at(const float* u, int t, int c, int T, int C)
+{
+#pragma clang fp contract(off)
+  const float coef[5] = {-3.0f / 35.0f, 12.0f / 35.0f, 17.0f / 35.0f, 12.0f / 35.0f, -3.0f / 35.0f};
+  if (t == 0) return u[c];
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) s = s + coef[k] * u[min(max(t + k - 2, 0), T - 1) * C + c];
+  return s;
+}
+
 // One workgroup per instance: global baseline (first strict minimum over the tiles' minima), FP64 normalizer and
-// sum of squares, new mean S / eta written in place, statistics; optionally advances the instance's generator
-// counter. Workgroup 0 resets the work-queue counter for the next launch.
+// sum of squares, new mean S / eta, statistics; optionally advances the instance's generator counter. The new mean is
+// then Savitzky-Golay smoothed and constrained (computeControl's post-processing, controllers/controller.cuh:573-607
+// and controllers/MPPI/mppi_controller.cu:224-227) into the instance's mean in place, the next call's nominal.
+// Workgroup 0 resets the work-queue counter for the next launch.
 constexpr int BATCH_COMBINE_THREADS = 256;
 constexpr int BATCH_MAX_TILES = BATCH_MAX_K / 64;
 
 __global__ void __launch_bounds__(BATCH_COMBINE_THREADS) batch_combine_kernel(
     const BatchInstance* __restrict__ inst, const Partial* __restrict__ parts, const float* __restrict__ sums,
     float* __restrict__ means, mppi_weight_stats* __restrict__ stats, IterArgs* iters, int advance,
-    int* __restrict__ queue)
+    int* __restrict__ queue, BatchCopies out)
 {
   __shared__ float escale[BATCH_MAX_TILES];
+  __shared__ float raw[MPPI_MAX_TIMESTEPS * MPPI_MAX_CONTROL];
   __shared__ float sv[BATCH_COMBINE_THREADS / 64];
   __shared__ int si[BATCH_COMBINE_THREADS / 64];
   __shared__ double se[BATCH_COMBINE_THREADS / 64], se2[BATCH_COMBINE_THREADS / 64], sec[BATCH_COMBINE_THREADS / 64];
@@ -422,7 +447,17 @@ __global__ void __launch_bounds__(BATCH_COMBINE_THREADS) batch_combine_kernel(
   for (int j = tid; j < TC; j += BATCH_COMBINE_THREADS) {
     float acc = 0.0f;
     for (int q = 0; q < G; ++q) acc += escale[q] * Sg[(int64_t)q * TC + j];
-    means[I.mean_base + j] = (float)((double)acc / eta);
+    const float m = (float)((double)acc / eta);
+    raw[j] = m;
+    if (out.mean != nullptr) out.mean[I.mean_base + j] = m;
+  }
+  __syncthreads();
+  const int T = I.T, C = I.C;
+  for (int j = tid; j < TC; j += BATCH_COMBINE_THREADS) {
+    const int t = j / C, c = j - t * C;
+    const float u = constrain_control(I.dyn, c, sg_smooth_at(raw, t, c, T, C), true);
+    means[I.mean_base + j] = u;
+    if (out.control != nullptr) out.control[I.mean_base + j] = u;
   }
   if (tid == 0) {
     mppi_weight_stats st;
@@ -434,8 +469,32 @@ __global__ void __launch_bounds__(BATCH_COMBINE_THREADS) batch_combine_kernel(
     st.num_rollouts = I.K;
     st.weighted_cost = wcost / eta;
     stats[i] = st;
+    if (out.stats != nullptr) out.stats[i] = st;
     if (advance) iters[i].rng_offset += 1;
     if (i == 0) *queue = 0;
+  }
+  signal_host_done(out.signal);
+}
+
+// The per-call inputs from mapped host memory, one workgroup per instance: iteration arguments, x0 and, where the
+// host changed the control sequence since the last call (upload[i]), the nominal sequence. Otherwise the mean the
+// last combine left in place (the same values the host holds) stays.
+__global__ void __launch_bounds__(BATCH_COMBINE_THREADS) batch_stage_kernel(
+    const BatchInstance* __restrict__ inst, const IterArgs* __restrict__ iters_h, IterArgs* __restrict__ iters_d,
+    const float* __restrict__ x0_h, float* __restrict__ x0_d, const float* __restrict__ nominal_h,
+    const int* __restrict__ upload_h, float* __restrict__ means)
+{
+  const int i = blockIdx.x, tid = threadIdx.x;
+  constexpr int W = (int)(sizeof(IterArgs) / 4);
+  if (tid < W)
+    reinterpret_cast<uint32_t*>(iters_d + i)[tid] = reinterpret_cast<const uint32_t*>(iters_h + i)[tid];
+  else if (tid >= 64 && tid < 64 + BATCH_MAX_STATE)
+    x0_d[i * BATCH_MAX_STATE + tid - 64] = x0_h[i * BATCH_MAX_STATE + tid - 64];
+  const int up = __builtin_amdgcn_readfirstlane(upload_h[i]);
+  if (up != 0) {
+    const BatchInstance& I = inst[i];
+    const int TC = I.T * I.C;
+    for (int j = tid; j < TC; j += BATCH_COMBINE_THREADS) means[I.mean_base + j] = nominal_h[I.mean_base + j];
   }
 }
 
@@ -512,6 +571,24 @@ struct mppi_batch {
   float* means_backup_d = nullptr;
   float* nominal_h = nullptr;
   float* mean_out_h = nullptr;
+  float* control_out_h = nullptr;
+  int* upload_h = nullptr;
+  // device-side aliases of the mapped host buffers above
+  IterArgs* iters_hd = nullptr;
+  float* x0_hd = nullptr;
+  float* nominal_hd = nullptr;
+  float* mean_out_hd = nullptr;
+  float* control_out_hd = nullptr;
+  int* upload_hd = nullptr;
+  mppi_weight_stats* stats_hd = nullptr;
+  // completion signal: device workgroup count and epoch, the mapped host word, signals enqueued
+  uint32_t* sig_d = nullptr;
+  uint32_t* done_h = nullptr;
+  uint32_t* done_hd = nullptr;
+  uint32_t signals = 0;
+  hipEvent_t ev_end = nullptr;
+  bool use_graph = false;
+  std::vector<char> dirty;  // the host's control sequence differs from the device's nominal
   std::vector<int> mean_base;
   int total_mean = 0;
   mppi_weight_stats* stats_d = nullptr;
@@ -535,7 +612,7 @@ struct mppi_batch {
   std::vector<std::vector<float>> control, raw_mean;
   // per instance, propagated on the first request after a call from that call's x0 and smoothed controls
   mutable std::vector<std::vector<float>> state_seq;
-  std::vector<std::vector<float>> state_x0, state_u;
+  std::vector<std::vector<float>> state_x0;
   mutable std::vector<char> state_stale;
   std::vector<mppi_weight_stats> stats;
   std::vector<mppi_free_energy> fe;
@@ -560,10 +637,13 @@ static int batch_free(mppi_batch* b)
   batch_drop_graphs(b);
   for (void* p : {(void*)b->map_d, (void*)b->tables_d, (void*)b->inst_d, (void*)b->work_d, (void*)b->queue_d, (void*)b->iters_d,
                   (void*)b->loop_iters_d, (void*)b->x0_d, (void*)b->means_d, (void*)b->means_backup_d,
-                  (void*)b->stats_d, (void*)b->ubuf_d, (void*)b->parts_d, (void*)b->sums_d, (void*)b->costs_d})
+                  (void*)b->stats_d, (void*)b->ubuf_d, (void*)b->parts_d, (void*)b->sums_d, (void*)b->costs_d,
+                  (void*)b->sig_d})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)b->iters_h, (void*)b->x0_h, (void*)b->nominal_h, (void*)b->mean_out_h, (void*)b->stats_h})
+  for (void* p : {(void*)b->iters_h, (void*)b->x0_h, (void*)b->nominal_h, (void*)b->mean_out_h, (void*)b->stats_h,
+                  (void*)b->control_out_h, (void*)b->upload_h, (void*)b->done_h})
     if (p) (void)hipHostFree(p);
+  if (b->ev_end) (void)hipEventDestroy(b->ev_end);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return MPPI_OK;
@@ -608,28 +688,36 @@ static int batch_apply_config(mppi_batch* b, const mppi_batch_config& c)
 
 // One batched iteration: persistent rollout over every tile, then the per-instance combine.
 // `between` (optional) is recorded between the persistent rollout launch and the combine launch.
-static int batch_enqueue_iteration(mppi_batch* b, IterArgs* iters, int advance, hipEvent_t between = nullptr)
+// `out` (optional): the call's host copies and completion signal.
+static int batch_enqueue_iteration(mppi_batch* b, IterArgs* iters, int advance, hipEvent_t between = nullptr,
+                                   const BatchCopies* out = nullptr)
 {
   MPPI_CHECK(launch_batch(b->cfg.threads, b->cfg.samples_per_lane, b->inst_d, b->work_d, b->nwork, b->queue_d, iters,
                           b->x0_d, b->means_d, b->ubuf_d, b->parts_d, b->sums_d, b->costs_d, b->cfg.chunk,
                           b->lds_bytes, b->blocks, b->stream));
   if (between != nullptr) MPPI_HIP_CHECK(hipEventRecord(between, b->stream));
   batch_combine_kernel<<<b->n, BATCH_COMBINE_THREADS, 0, b->stream>>>(b->inst_d, b->parts_d, b->sums_d, b->means_d,
-                                                                      b->stats_d, iters, advance, b->queue_d);
+                                                                      b->stats_d, iters, advance, b->queue_d,
+                                                                      out != nullptr ? *out : BatchCopies{});
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
 
+// One computeControl of the whole batch: the inputs staged from mapped host memory, the iteration, the results and
+// the completion signal written to mapped host memory by the combine.
 static int batch_enqueue_call(mppi_batch* b)
 {
-  const hipStream_t s = b->stream;
-  MPPI_HIP_CHECK(hipMemcpyAsync(b->iters_d, b->iters_h, sizeof(IterArgs) * b->n, hipMemcpyHostToDevice, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(b->x0_d, b->x0_h, sizeof(float) * BATCH_MAX_STATE * b->n, hipMemcpyHostToDevice, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(b->means_d, b->nominal_h, sizeof(float) * b->total_mean, hipMemcpyHostToDevice, s));
-  MPPI_CHECK(batch_enqueue_iteration(b, b->iters_d, 0));
-  MPPI_HIP_CHECK(hipMemcpyAsync(b->mean_out_h, b->means_d, sizeof(float) * b->total_mean, hipMemcpyDeviceToHost, s));
-  MPPI_HIP_CHECK(hipMemcpyAsync(b->stats_h, b->stats_d, sizeof(mppi_weight_stats) * b->n, hipMemcpyDeviceToHost, s));
-  return MPPI_OK;
+  batch_stage_kernel<<<b->n, BATCH_COMBINE_THREADS, 0, b->stream>>>(b->inst_d, b->iters_hd, b->iters_d, b->x0_hd,
+                                                                    b->x0_d, b->nominal_hd, b->upload_hd, b->means_d);
+  MPPI_HIP_CHECK(hipGetLastError());
+  BatchCopies out;
+  out.mean = b->mean_out_hd;
+  out.control = b->control_out_hd;
+  out.stats = b->stats_hd;
+  out.signal.count = b->sig_d;
+  out.signal.epoch = b->sig_d + 1;
+  out.signal.flag = b->done_hd;
+  return batch_enqueue_iteration(b, b->iters_d, 0, nullptr, &out);
 }
 
 extern "C" {
@@ -733,8 +821,7 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
     if (d.init_control) std::copy(d.init_control, d.init_control + (size_t)T * C, b->control.back().begin());
     b->raw_mean.emplace_back((size_t)T * C, 0.0f);
     b->state_seq.emplace_back((size_t)T * S, 0.0f);
-    b->state_x0.emplace_back();
-    b->state_u.emplace_back();
+    b->state_x0.emplace_back((size_t)BATCH_MAX_STATE, 0.0f);
   }
   b->state_stale.assign(n, 0);
   b->stats.assign(n, mppi_weight_stats{});
@@ -761,11 +848,16 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
   A(dev_alloc(&b->parts_d, sizeof(Partial) * (size_t)parts));
   A(dev_alloc(&b->sums_d, sizeof(float) * (size_t)sums));
   A(dev_alloc(&b->costs_d, sizeof(float) * (size_t)costs));
-  A(host_alloc(&b->iters_h, sizeof(IterArgs) * n));
-  A(host_alloc(&b->x0_h, sizeof(float) * BATCH_MAX_STATE * n));
-  A(host_alloc(&b->nominal_h, sizeof(float) * b->total_mean));
-  A(host_alloc(&b->mean_out_h, sizeof(float) * b->total_mean));
-  A(host_alloc(&b->stats_h, sizeof(mppi_weight_stats) * n));
+  A(host_alloc_mapped(&b->iters_h, &b->iters_hd, sizeof(IterArgs) * n));
+  A(host_alloc_mapped(&b->x0_h, &b->x0_hd, sizeof(float) * BATCH_MAX_STATE * n));
+  A(host_alloc_mapped(&b->nominal_h, &b->nominal_hd, sizeof(float) * b->total_mean));
+  A(host_alloc_mapped(&b->mean_out_h, &b->mean_out_hd, sizeof(float) * b->total_mean));
+  A(host_alloc_mapped(&b->control_out_h, &b->control_out_hd, sizeof(float) * b->total_mean));
+  A(host_alloc_mapped(&b->upload_h, &b->upload_hd, sizeof(int) * n));
+  A(host_alloc_mapped(&b->stats_h, &b->stats_hd, sizeof(mppi_weight_stats) * n));
+  A(host_alloc_mapped(&b->done_h, &b->done_hd, sizeof(uint32_t)));
+  A(dev_alloc(&b->sig_d, 2 * sizeof(uint32_t)));
+  b->dirty.assign(n, 1);
   size_t table_floats = 0;
   for (const Table& t : tables) table_floats += t.count;
   if (table_floats > 0) A(dev_alloc(&b->tables_d, sizeof(float) * table_floats));
@@ -780,6 +872,8 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
   }
   for (mppi_instance_desc& d : b->desc) d.std_dev_t_host = d.goal_traj_host = nullptr;
   e = hipMemset(b->queue_d, 0, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(b->sig_d, 0, 2 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_end, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMemset(b->x0_d, 0, sizeof(float) * BATCH_MAX_STATE * n);
   if (e == hipSuccess) e = hipMemset(b->means_d, 0, sizeof(float) * b->total_mean);
   if (e != hipSuccess) return bail(hip_fail(e, "hipMemset"));
@@ -826,6 +920,7 @@ int mppi_batch_autotune(mppi_batch* b, int32_t reps, mppi_batch_config* best_out
   if (!b) return fail(MPPI_E_INVALID, "null argument");
   if (reps < 1) return fail(MPPI_E_INVALID, "batch: reps must be >= 1");
   MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->dirty.assign(b->n, 1);
   MPPI_HIP_CHECK(hipMemcpy(b->means_backup_d, b->means_d, sizeof(float) * b->total_mean, hipMemcpyDeviceToDevice));
   for (int i = 0; i < b->n; ++i) b->iters_h[i] = IterArgs{b->rng_counter, 1, 0};
   MPPI_HIP_CHECK(hipMemcpy(b->iters_d, b->iters_h, sizeof(IterArgs) * b->n, hipMemcpyHostToDevice));
@@ -873,11 +968,13 @@ int mppi_batch_compute_control(mppi_batch* b, const float* x0_host, int32_t stri
   for (int i = 0; i < b->n; ++i) {
     b->fe[i].previous_baseline = b->stats[i].baseline;
     for (int j = 0; j < BATCH_MAX_STATE; ++j) b->x0_h[i * BATCH_MAX_STATE + j] = x0_host[i * BATCH_MAX_STATE + j];
-    std::copy(b->control[i].begin(), b->control[i].end(), b->nominal_h + b->mean_base[i]);
+    // the nominal is uploaded only where the host changed the sequence since the device computed it
+    if (b->dirty[i]) std::copy(b->control[i].begin(), b->control[i].end(), b->nominal_h + b->mean_base[i]);
+    b->upload_h[i] = b->dirty[i] ? 1 : 0;
     b->iters_h[i] = IterArgs{b->rng_counter, stride, 0};
   }
   b->rng_counter++;
-  if (b->eager_runs > 0) {
+  if (b->use_graph && b->eager_runs > 0) {
     if (!b->graph_exec) {
       MPPI_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
       const int st = batch_enqueue_call(b);
@@ -896,20 +993,37 @@ int mppi_batch_compute_control(mppi_batch* b, const float* x0_host, int32_t stri
     MPPI_CHECK(batch_enqueue_call(b));
     b->eager_runs++;
   }
-  MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  const uint32_t want = ++b->signals;
+  MPPI_HIP_CHECK(hipEventRecord(b->ev_end, b->stream));
+  const int st = wait_host_signal(b->done_h, want, b->ev_end);
+  if (st != MPPI_OK) {
+    b->signals = *b->done_h;
+    return st;
+  }
+  // the combine smoothed and constrained the new means on the device (into the next call's nominal as well)
   for (int i = 0; i < b->n; ++i) {
-    const int T = b->T[i], C = b->C[i];
+    const size_t base = (size_t)b->mean_base[i], tc = (size_t)b->T[i] * b->C[i];
     b->stats[i] = b->stats_h[i];
     free_energy_from_stats(b->stats[i], b->desc[i].lambda, b->K[i], &b->fe[i]);
-    b->raw_mean[i].assign(b->mean_out_h + b->mean_base[i], b->mean_out_h + b->mean_base[i] + (size_t)T * C);
-    b->control[i] = b->raw_mean[i];
-    smooth_control(b->control[i], T, C);
-    b->state_x0[i].assign(b->x0_h + i * BATCH_MAX_STATE, b->x0_h + (i + 1) * BATCH_MAX_STATE);
-    b->state_u[i] = b->control[i];
+    std::memcpy(b->raw_mean[i].data(), b->mean_out_h + base, sizeof(float) * tc);
+    std::memcpy(b->control[i].data(), b->control_out_h + base, sizeof(float) * tc);
+    std::memcpy(b->state_x0[i].data(), b->x0_h + (size_t)i * BATCH_MAX_STATE, sizeof(float) * BATCH_MAX_STATE);
     b->state_stale[i] = 1;
-    clamp_controls_seq(b->desc[i].dynamics, b->control[i].data(), T);
+    b->dirty[i] = 0;
   }
   b->computed = true;
+  return MPPI_OK;
+}
+
+int mppi_batch_set_graph_replay(mppi_batch* b, int32_t enable)
+{
+  if (!b) return fail(MPPI_E_INVALID, "null argument");
+  MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->use_graph = enable != 0;
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
   return MPPI_OK;
 }
 
@@ -932,6 +1046,7 @@ int mppi_batch_step_device(mppi_batch* b, int32_t num_calls)
   if (!b->computed)
     return fail(MPPI_E_INVALID, "batch step_device: call computeControl once first (the loop starts from its state)");
   if (num_calls == 0) return MPPI_OK;
+  b->dirty.assign(b->n, 1);  // the loop moves the device means away from the host's sequences
   if (!b->loop_exec) {
     MPPI_CHECK(batch_loop_args_init(b));
     MPPI_CHECK(batch_enqueue_iteration(b, b->loop_iters_d, 1));  // eager pass before capture
@@ -957,6 +1072,7 @@ int mppi_batch_time_kernels(mppi_batch* b, int32_t reps, float* rollout_ms, floa
   if (!b || !rollout_ms || !combine_ms || reps < 1) return fail(MPPI_E_INVALID, "bad argument");
   if (!b->computed) return fail(MPPI_E_INVALID, "batch time_kernels: call computeControl once first");
   MPPI_CHECK(batch_loop_args_init(b));
+  b->dirty.assign(b->n, 1);
   std::vector<hipEvent_t> ev((size_t)(2 * reps + 1), nullptr);
   int st = MPPI_OK;
   for (auto& e : ev)
@@ -1015,8 +1131,11 @@ int mppi_batch_get_state_seq(const mppi_batch* b, int32_t i, float* out)
   MPPI_CHECK(batch_index(b, i));
   if (!out) return fail(MPPI_E_INVALID, "null argument");
   if (b->state_stale[i]) {
-    propagate(b->desc[i].dynamics, b->state_x0[i].data(), b->state_u[i].data(), b->T[i], b->desc[i].dt,
-              b->state_seq[i].data());
+    // computeStateTrajectory of the smoothed, not yet constrained controls (the device smoothing is the host
+    // function's arithmetic)
+    std::vector<float> u(b->raw_mean[i]);
+    smooth_control(u, b->T[i], b->C[i]);
+    propagate(b->desc[i].dynamics, b->state_x0[i].data(), u.data(), b->T[i], b->desc[i].dt, b->state_seq[i].data());
     b->state_stale[i] = 0;
   }
   std::copy(b->state_seq[i].begin(), b->state_seq[i].end(), out);
@@ -1044,6 +1163,7 @@ int mppi_batch_set_control_seq(mppi_batch* b, int32_t i, const float* u)
   MPPI_CHECK(batch_index(b, i));
   if (!u) return fail(MPPI_E_INVALID, "null argument");
   std::copy(u, u + b->control[i].size(), b->control[i].begin());
+  b->dirty[i] = 1;
   return MPPI_OK;
 }
 
@@ -1053,6 +1173,7 @@ int mppi_batch_slide_control_seq(mppi_batch* b, int32_t steps)
   if (steps < 0) return fail(MPPI_E_INVALID, "steps must be >= 0");
   // slideControlSequenceHelper (controllers/controller.cuh:609-621) per instance
   for (int k = 0; k < b->n; ++k) {
+    b->dirty[k] = 1;
     const int T = b->T[k], C = b->C[k];
     const float* zero = b->desc[k].dynamics.zero_control;
     const float* scale = b->desc[k].slide_control_scale;

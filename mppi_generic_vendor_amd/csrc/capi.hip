@@ -1,6 +1,8 @@
 // C-ABI of the engine: parameter defaults and validation, kernel-level entry points and the controller object that
 // runs whole MPPI iterations (VanillaMPPIController::computeControl, controllers/MPPI/mppi_controller.cu:152-237).
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -34,6 +36,23 @@ int hip_fail(hipError_t e, const char* what)
 {
   set_last_error(std::string(what) + ": " + hipGetErrorString(e));
   return MPPI_E_HIP;
+}
+
+int wait_host_signal(const uint32_t* flag, uint32_t want, hipEvent_t ev)
+{
+  const volatile uint32_t* f = flag;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t n = 1;; ++n) {
+    if (*f == want) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      return MPPI_OK;
+    }
+    if ((n & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(HOST_SPIN_MS)) break;
+    __builtin_ia32_pause();
+  }
+  MPPI_HIP_CHECK(hipEventSynchronize(ev));
+  if (*f != want) return fail(MPPI_E_HIP, "control call completed without its completion signal");
+  return MPPI_OK;
 }
 
 // The attribute belongs to the kernel on the device that is current when it is set, so it is recorded per
@@ -345,7 +364,14 @@ struct mppi_controller {
   float* vis_buf_d = nullptr;  // controls [n][T][C], outputs [n][T][O], step costs [n][T], terminal [n]
   // stream time of the last computeControl from the first upload to the last download
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  float last_gpu_ms = 0.0f;
+  mutable float last_gpu_ms = 0.0f;
+  mutable bool timing_pending = false;  // last_gpu_ms is read from ev0 / ev1 on request
+  // completion signal of a call: the device workgroup count and epoch (sig_d[0], sig_d[1]) and the mapped host word
+  // the call's last kernel publishes the epoch to; `signals` counts the signals enqueued
+  uint32_t* sig_d = nullptr;
+  uint32_t* done_h = nullptr;
+  uint32_t* done_hd = nullptr;
+  uint32_t signals = 0;
 };
 
 extern "C" {
@@ -781,11 +807,11 @@ static int controller_free(mppi_controller* c)
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
                  c->map_d, c->nn_d, c->loop_iter_d, c->prop_dmu_d, c->prop_sig_d, c->sigma_base_d,
-                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d, c->std_t_d, c->goal_d, c->w_d};
+                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d, c->std_t_d, c->goal_d, c->w_d, c->sig_d};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (c->map_ev) (void)hipEventDestroy(c->map_ev);
-  void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h, c->map_pinned};
+  void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h, c->map_pinned, c->done_h};
   for (void* p : host)
     if (p) (void)hipHostFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -874,6 +900,10 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   MPPI_CHECK(host_alloc_mapped(&c->nominal_h, &c->nominal_hd, sizeof(float) * T * C));
   MPPI_CHECK(host_alloc_mapped(&c->mean_out_h, &c->mean_out_hd, sizeof(float) * T * C));
   MPPI_CHECK(host_alloc_mapped(&c->stats_h, &c->stats_hd, sizeof(mppi_weight_stats)));
+  MPPI_CHECK(host_alloc_mapped(&c->done_h, &c->done_hd, sizeof(uint32_t)));
+  MPPI_CHECK(dev_alloc(&c->sig_d, 2 * sizeof(uint32_t)));
+  MPPI_HIP_CHECK(hipMemsetAsync(c->sig_d, 0, 2 * sizeof(uint32_t), c->stream));
+  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
   // plugin-owned device copies of the cost map and the MLP parameters
   if (!c->has_plugin && desc->cost_map_host != nullptr) {
     const size_t n = (size_t)desc->cost.map_width * desc->cost.map_height;
@@ -1003,7 +1033,7 @@ static int enqueue_tsallis_iteration(mppi_controller* c, IterRef ref, IterArgs* 
   MPPI_CHECK(launch_weighted_update(c->w_d, c->stats_d, c->samples_d, K, T, C, c->mean_d, c->ws_d, c->stream));
   if (copies != nullptr && copies->mean != nullptr) {
     const WordSpan m[] = {{c->mean_d, copies->mean, T * C}};
-    MPPI_CHECK(launch_copy_words(m, 1, c->stream));
+    MPPI_CHECK(launch_copy_words(m, 1, c->stream, &copies->signal));
   }
   return MPPI_OK;
 }
@@ -1029,6 +1059,16 @@ static int enqueue_iteration(mppi_controller* c, IterRef ref, IterArgs* advance,
                                 colored ? &plan : nullptr, advance_step, between, partials_out);
 }
 
+// The completion signal of a control call (given by its last kernel).
+static HostSignal call_signal(const mppi_controller* c)
+{
+  HostSignal h;
+  h.count = c->sig_d;
+  h.epoch = c->sig_d + 1;
+  h.flag = c->done_hd;
+  return h;
+}
+
 // The work of one computeControl call, enqueued on the controller's stream (captured once when use_graph is set).
 static int controller_enqueue(mppi_controller* c)
 {
@@ -1046,6 +1086,7 @@ static int controller_enqueue(mppi_controller* c)
   ResultCopies out;
   out.mean = c->mean_out_hd;
   out.stats = c->stats_hd;
+  out.signal = call_signal(c);
   for (int it = 0; it < d.num_iters; ++it) {
     if (vis && d.num_iters > 1 && it + 1 == d.num_iters) {
       const WordSpan m[] = {{c->mean_d, c->vis_mean_d, T * C}};
@@ -1073,14 +1114,15 @@ static int controller_enqueue_proposal(mppi_controller* c)
   out.stats = c->stats_hd;
   out.cost0_src = c->costs_d;
   out.cost0 = c->cost0_hd;
+  out.signal = call_signal(c);
   return launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->prop_K, T, d.dt, d.lambda, d.alpha, c->x0_d,
                                 c->mean_d, c->samples_d, true, d.keep_samples != 0, IterRef{c->iter_d, {0, 0, 0}},
                                 c->costs_d, c->ws_d, c->stats_d, c->mean_d, nullptr, s, c->prop_dmu_d, c->prop_sig_d,
                                 &out);
 }
 
-// Runs one enqueue function eagerly on its first use and from a captured hipGraph afterwards (when use_graph), then
-// records ev1 on the stream (the end of the call's GPU work) and waits for it.
+// Runs one enqueue function eagerly on its first use and from a captured hipGraph afterwards (when use_graph), records
+// ev1 on the stream (the end of the call's GPU work) and waits for the call's completion signal.
 static int controller_run(mppi_controller* c, int (*enqueue)(mppi_controller*), hipGraph_t* graph,
                           hipGraphExec_t* exec, int* runs)
 {
@@ -1103,9 +1145,12 @@ static int controller_run(mppi_controller* c, int (*enqueue)(mppi_controller*), 
     MPPI_CHECK(enqueue(c));
     (*runs)++;
   }
+  const uint32_t want = ++c->signals;
   MPPI_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
-  MPPI_HIP_CHECK(hipEventSynchronize(c->ev1));
-  return MPPI_OK;
+  c->timing_pending = true;
+  const int st = wait_host_signal(c->done_h, want, c->ev1);
+  if (st != MPPI_OK) c->signals = *c->done_h;  // resynchronised with what the device gave
+  return st;
 }
 
 static void controller_set_iters(mppi_controller* c, int stride, int n)
@@ -1258,8 +1303,6 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
     MPPI_CHECK(controller_run(c, controller_enqueue, &c->graph, &c->graph_exec, &c->eager_runs));
     K_used = K;
   }
-  // stream time of the call: from ev0 (before the first launch) to ev1 (recorded right after the last one)
-  MPPI_HIP_CHECK(hipEventElapsedTime(&c->last_gpu_ms, c->ev0, c->ev1));
   c->stats = *c->stats_h;
   c->raw_mean.assign(c->mean_out_h, c->mean_out_h + (size_t)T * C);
   c->control = c->raw_mean;
@@ -1330,6 +1373,12 @@ int mppi_controller_set_proposal(mppi_controller* c, mppi_proposal* proposal, in
 int mppi_controller_get_last_call_ms(const mppi_controller* c, float* ms)
 {
   if (!c || !ms) return fail(MPPI_E_INVALID, "null argument");
+  if (c->timing_pending) {
+    // stream time of the call: from ev0 (before the first launch) to ev1 (recorded right after the last one)
+    MPPI_HIP_CHECK(hipEventSynchronize(c->ev1));
+    MPPI_HIP_CHECK(hipEventElapsedTime(&c->last_gpu_ms, c->ev0, c->ev1));
+    c->timing_pending = false;
+  }
   *ms = c->last_gpu_ms;
   return MPPI_OK;
 }

@@ -181,4 +181,36 @@ __device__ __forceinline__ float sanitize_cost(float c) { return (c <= FLT_MAX) 
 // Pure-noise start index: first k with (float)k >= (1 - p) * K in FP32 (sampling_distributions/gaussian/gaussian.cu:96).
 int pure_noise_start(int K, float pct);
 
+// ----------------------------------------------------------------------------------------- host completion signal
+// The end of a control call as seen by a host that polls a word of mapped host memory instead of blocking on an event:
+// the call's last kernel, once the results of all its workgroups are visible to the host, advances a device epoch and
+// stores it to `flag` (the device-side alias of that word). `count` (zeroed device word) counts the workgroups that
+// are done, `epoch` (zeroed device word) is the number of signals given. Null `flag`: no signal.
+struct HostSignal {
+  uint32_t* count = nullptr;
+  uint32_t* epoch = nullptr;
+  uint32_t* flag = nullptr;
+};
+
+// Every thread of every workgroup calls this at the very end of the kernel. Each thread's own stores (results in
+// mapped host memory among them) are made visible at system scope, then the last workgroup to arrive resets the count,
+// advances the epoch and publishes it with a system-scope release store.
+__device__ __forceinline__ void signal_host_done(const HostSignal& s)
+{
+  if (s.flag == nullptr) return;
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
+    const uint32_t blocks = gridDim.x * gridDim.y * gridDim.z;
+    const uint32_t before = __hip_atomic_fetch_add(s.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (before + 1u == blocks) {
+      *s.count = 0u;
+      const uint32_t e = *s.epoch + 1u;
+      *s.epoch = e;
+      __threadfence_system();
+      __hip_atomic_store(s.flag, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
+
 }  // namespace mppi_amd

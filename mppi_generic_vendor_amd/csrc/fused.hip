@@ -290,6 +290,7 @@ __global__ void __launch_bounds__(THREADS) combine_kernel(const Partial* __restr
     if (copies.cost0 != nullptr) *copies.cost0 = *copies.cost0_src;
     if (advance != nullptr) advance->rng_offset += (uint64_t)advance_step;
   }
+  signal_host_done(copies.signal);
 }
 
 // Compute units of the current device (cached per device; 0 if the query fails).

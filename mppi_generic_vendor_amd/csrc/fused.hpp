@@ -19,11 +19,13 @@ size_t fused_iteration_workspace_bytes(int K, int T, int C);
 
 // Optional copies of an iteration's results that the combine kernel also writes, typically to mapped host memory
 // (device-side addresses): the new mean [T C], the statistics, and cost0_src[0] to cost0. Null members are skipped.
+// `signal` (optional) is given once all of these are visible to the host.
 struct ResultCopies {
   float* mean = nullptr;
   mppi_weight_stats* stats = nullptr;
   const float* cost0_src = nullptr;
   float* cost0 = nullptr;
+  HostSignal signal;
 };
 
 // One MPPI iteration in two launches. With `generate` the Gaussian control samples are drawn inside the rollout

@@ -28,6 +28,12 @@ void clamp_controls_seq(const mppi_dynamics_params& p, float* u, int T);
 // Free-energy statistics (core/mppi_common.cu:1101-1117) from the FP64 weight sums; fe->previous_baseline is input.
 void free_energy_from_stats(const mppi_weight_stats& st, double lambda, int K, mppi_free_energy* fe);
 
+// Waits for the completion signal `want` in the mapped word `flag` (HostSignal): polls it, since waking from a
+// blocking wait costs about as much as a short control call, and after HOST_SPIN_MS blocks on `ev` (recorded after
+// the call's last kernel), which also reports a failed call.
+constexpr int HOST_SPIN_MS = 20;
+int wait_host_signal(const uint32_t* flag, uint32_t want, hipEvent_t ev);
+
 template <class Tp>
 int dev_alloc(Tp** p, size_t bytes)
 {

@@ -281,7 +281,10 @@ __global__ void __launch_bounds__(1024) weights_kernel(const float* costs, float
     st.num_rollouts = K;
     st.weighted_cost = tec / te;
     *stats = st;
-    if (stats_copy != nullptr) *stats_copy = st;
+    if (stats_copy != nullptr) {
+      *stats_copy = st;
+      __threadfence_system();  // in host memory before any later kernel's completion signal
+    }
     if (advance != nullptr) advance->rng_offset += (uint64_t)advance_step;
   }
 }
@@ -381,29 +384,31 @@ struct WordSpans {
   WordSpan s[MAX_WORD_SPANS];
 };
 
-__global__ void __launch_bounds__(64 * MAX_WORD_SPANS) copy_words_kernel(WordSpans l, int count)
+__global__ void __launch_bounds__(64 * MAX_WORD_SPANS) copy_words_kernel(WordSpans l, int count, HostSignal sig)
 {
   const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (k >= count) return;
-  const WordSpan sp = k == 0 ? l.s[0] : (k == 1 ? l.s[1] : (k == 2 ? l.s[2] : l.s[3]));
-  const uint32_t* __restrict__ src = static_cast<const uint32_t*>(sp.src);
-  uint32_t* __restrict__ dst = static_cast<uint32_t*>(sp.dst);
-  for (int i0 = 0; i0 < sp.words; i0 += 4 * 64) {
-    uint32_t v[4];
+  if (k < count) {
+    const WordSpan sp = k == 0 ? l.s[0] : (k == 1 ? l.s[1] : (k == 2 ? l.s[2] : l.s[3]));
+    const uint32_t* __restrict__ src = static_cast<const uint32_t*>(sp.src);
+    uint32_t* __restrict__ dst = static_cast<uint32_t*>(sp.dst);
+    for (int i0 = 0; i0 < sp.words; i0 += 4 * 64) {
+      uint32_t v[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int i = i0 + 64 * j + lane;
-      v[j] = i < sp.words ? src[i] : 0u;
-    }
+      for (int j = 0; j < 4; ++j) {
+        const int i = i0 + 64 * j + lane;
+        v[j] = i < sp.words ? src[i] : 0u;
+      }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int i = i0 + 64 * j + lane;
-      if (i < sp.words) dst[i] = v[j];
+      for (int j = 0; j < 4; ++j) {
+        const int i = i0 + 64 * j + lane;
+        if (i < sp.words) dst[i] = v[j];
+      }
     }
   }
+  signal_host_done(sig);
 }
 
-int launch_copy_words(const WordSpan* spans, int count, hipStream_t s)
+int launch_copy_words(const WordSpan* spans, int count, hipStream_t s, const HostSignal* sig)
 {
   if (count < 1 || count > MAX_WORD_SPANS) return fail(MPPI_E_INVALID, "copy_words: 1 to 4 spans");
   WordSpans l{};
@@ -411,7 +416,7 @@ int launch_copy_words(const WordSpan* spans, int count, hipStream_t s)
     if (spans[i].words < 0) return fail(MPPI_E_INVALID, "copy_words: negative span");
     l.s[i] = spans[i];
   }
-  copy_words_kernel<<<1, 64 * count, 0, s>>>(l, count);
+  copy_words_kernel<<<1, 64 * count, 0, s>>>(l, count, sig != nullptr ? *sig : HostSignal{});
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }

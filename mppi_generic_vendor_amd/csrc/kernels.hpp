@@ -63,7 +63,8 @@ struct WordSpan {
   int words;
 };
 constexpr int MAX_WORD_SPANS = 4;
-int launch_copy_words(const WordSpan* spans, int count, hipStream_t s);
+// `sig` (optional): the host completion signal, given after the copies.
+int launch_copy_words(const WordSpan* spans, int count, hipStream_t s, const HostSignal* sig = nullptr);
 
 // 32-bit words of n objects of type Tp.
 template <class Tp>

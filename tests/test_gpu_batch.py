@@ -98,3 +98,24 @@ def test_batch_autotune_and_device_loop(hip, engine):
         st = b.getWeightStats(i)
         assert np.isfinite(st.normalizer) and st.normalizer >= 1.0
     b.close()
+
+
+def test_batch_device_nominal_and_graph_replay(hip, engine):
+    """Calls whose nominal stays on the device (the host did not change the sequence since the combine smoothed and
+    constrained it) and calls replayed from a captured hipGraph match the oracle around the host's sequence."""
+    banner("batch device-resident nominal + graph replay")
+    probs = _mixed_batch()
+    b = MPPIBatch(probs, cost_map=SMALL_MAP)
+    b.setConfig(256, 1, 4)
+    x0 = b.packStates([p.x0 for p in probs])
+    b.computeControl(x0, 1)
+    for call in range(1, 5):
+        if call == 2:
+            b.setGraphReplay(True)  # call 2 runs eagerly and is captured, calls 3 and 4 replay the graph
+        nominal = [b.getControlSeq(i) for i in range(len(probs))]
+        b.computeControl(x0, 1)
+        for i, p in enumerate(probs):
+            _check_instance(b, i, p, nominal[i], rng_offset=call)
+            # the device smoothing and constraints are the oracle's FP32 arithmetic, operation for operation
+            np.testing.assert_array_equal(b.getControlSeq(i), ol.enforce(p, ol.smooth(b.getRawMean(i))))
+    b.close()
