@@ -103,43 +103,49 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
   __syncthreads();
 
   // Samples of chunk j into ring slot j % 3, by threads t_off .. t_off + nth - 1. Element (k, t, c) is normal
-  // k*TC + t*C + c of the instance's stream; thread item (r, q) evaluates Philox block q of row r's range. Element
-  // indices stay below K T C <= 2^26 (BATCH_MAX_K, MPPI_MAX_TIMESTEPS, MPPI_MAX_CONTROL), so 32-bit arithmetic holds
-  // them and the Philox counter's high word is 0.
+  // k*TC + t*C + c of the instance's stream. C divides 4, so a Philox block (4 normals) holds SPB = 4 / C whole steps of
+  // one row: thread item (q, r) evaluates block q of row r's range, whose first step lies (e mod 4) / C steps before
+  // t0 when the row's first element e is not on a block boundary. Consecutive items are consecutive rows of one block
+  // index, so a wavefront's LDS stores are row-contiguous and its mean reads are broadcasts. Element indices stay
+  // below K T C <= 2^26 (BATCH_MAX_K, MPPI_MAX_TIMESTEPS, MPPI_MAX_CONTROL), so 32-bit arithmetic holds them and the
+  // Philox counter's high word is 0.
+  static_assert(4 % C == 0, "a Philox block holds whole steps");
+  constexpr int SPB = 4 / C;
   float sdc[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) sdc[c] = decay * sp.std_dev[c];
   const float* sdt = sp.std_dev_t;
   auto gen = [&](int j, int t_off, int nth) {
-    const int t0 = j * ch, t1 = min(T, t0 + ch);
-    // Philox blocks per row: one more when the row's range does not start on a block boundary (k0 * TC is a
-    // multiple of 4 since R is, so only TC and t0 * C decide)
+    const int t0 = j * ch, n = min(T, t0 + ch) - t0;
     const bool aligned = (TC % 4 == 0) && ((t0 * C) % 4 == 0);
-    const int nb = ((t1 - t0) * C + 3) / 4 + (aligned ? 0 : 1);
-    // item / nb as (item + 1/2) / nb rounded down in FP32: exact, since the quotient lies at least 1/(2 nb) from an
-    // integer and item < 256 nb
-    const float inv_nb = 1.0f / (float)nb;
+    const int nb = (n + SPB - 1) / SPB + (aligned ? 0 : 1);
     float* slot = ring + (j % 3) * ch * R * C;
     for (int item = tid - t_off; item < R * nb; item += nth) {
-      const int r = (int)(((float)item + 0.5f) * inv_nb), q = item - r * nb;
+      const int r = item & (R - 1), q = item / R;
       const int kk = k0 + r;
-      const int es = kk * TC + t0 * C, ee = kk * TC + t1 * C;
-      const int b = (es >> 2) + q;
-      if (b > ((ee - 1) >> 2)) continue;
-      const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(uint32_t)b);
+      const int es = kk * TC + t0 * C;
+      const int s0 = q * SPB - (es & 3) / C;  // chunk step of the block's first normal
+      if (s0 >= n) continue;
+      const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(uint32_t)((es >> 2) + q));
+      const bool keep = kk == 0, pure_row = kk >= I.pstart;
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj) {
-        const int e = 4 * b + jj;
-        if (e < es || e >= ee) continue;
-        const int loc = e - kk * TC;
-        const int t = loc / C, c = loc - t * C;
-        const float m = mu[loc];
-        const float sd = sdt ? decay * sdt[loc] : sdc[c];
-        float v;
-        if (kk == 0 || t < a.stride) v = m;
-        else if (kk >= I.pstart) v = sd * z.z[jj];
-        else v = m + sd * z.z[jj];
-        slot[((t - t0) * R + r) * C + c] = v;
+      for (int h = 0; h < SPB; ++h) {
+        const int s = s0 + h;
+        if (s < 0 || s >= n) continue;
+        const int t = t0 + s;
+        float v[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const float m = mu[t * C + c];
+          const float sd = sdt ? decay * sdt[t * C + c] : sdc[c];
+          if (keep || t < a.stride) v[c] = m;
+          else if (pure_row) v[c] = sd * z.z[h * C + c];
+          else v[c] = m + sd * z.z[h * C + c];
+        }
+        float* dst = slot + (s * R + r) * C;
+        if constexpr (C == 2) *reinterpret_cast<float2*>(dst) = make_float2(v[0], v[1]);
+        else if constexpr (C == 4) *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+        else dst[0] = v[0];
       }
     }
   };
