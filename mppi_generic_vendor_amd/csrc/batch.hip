@@ -44,6 +44,11 @@ struct BatchInstance {
   int32_t _pad;
 };
 
+// Cost-map queries of one consumer wavefront whose texel loads are issued together (per sample of a lane).
+#ifndef BATCH_MAP_PREFETCH
+#define BATCH_MAP_PREFETCH 1
+#endif
+
 // The final column sums run on up to SUM_WAVES wavefronts, each with a [64][8 C + 1] block of LDS.
 constexpr int SUM_WAVES = 4;
 __host__ __device__ constexpr int sum_block_floats(int C) { return 64 * (8 * C + 1); }
@@ -215,15 +220,43 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
         const int t0 = (it - 1) * ch, t1 = min(T, t0 + ch);
         const float* rs = ring + ((it - 1) % 3) * ch * R * C;
         const float* yb = ybuf + ((it - 1) & 1) * ch * R * O;
-        for (int t = t0 + wave - 1; t < t1; t += NCONS) {
+        // the wavefront's steps t0 + wave - 1 + j NCONS in groups of PF: the cost-map texels of a group's steps are
+        // fetched first, so that their loads are in flight together, then the group's costs are added in step order
+        constexpr int PF = BATCH_MAP_PREFETCH / SPL > 0 ? BATCH_MAP_PREFETCH / SPL : 1;
+        for (int tb = t0 + wave - 1; tb < t1; tb += NCONS * PF) {
+          MapTap tap[PF][SPL];
+          if constexpr (COST == MPPI_COST_QUADRATIC_MAP) {
+            if (CF::uses_map(I.cost)) {
 #pragma unroll
-          for (int s = 0; s < SPL; ++s) {
-            const int r = lane + 64 * s;
-            // y is read in place from LDS: the cost's runtime-indexed terms (map position, speed) become LDS
-            // addresses instead of register selects
-            const float* yy = yb + ((t - t0) * R + r) * O;
-            const float* u = rs + ((t - t0) * R + r) * C;
-            acc[s] += CF::running(I.cost, yy, t) + lr_cost<C>(sp, u, mu + t * C, pure[s], I.lambda, I.alpha, t);
+              for (int g = 0; g < PF; ++g) {
+                const int t = tb + g * NCONS;
+                if (t < t1) {
+#pragma unroll
+                  for (int s = 0; s < SPL; ++s) {
+                    const float* yy = yb + ((t - t0) * R + lane + 64 * s) * O;
+                    tap[g][s] = map_fetch(I.cost, yy[I.cost.map_x_index], yy[I.cost.map_y_index]);
+                  }
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int g = 0; g < PF; ++g) {
+            const int t = tb + g * NCONS;
+            if (t < t1) {
+#pragma unroll
+              for (int s = 0; s < SPL; ++s) {
+                const int r = lane + 64 * s;
+                // y is read in place from LDS: the cost's runtime-indexed terms (map position, speed) become LDS
+                // addresses instead of register selects
+                const float* yy = yb + ((t - t0) * R + r) * O;
+                const float* u = rs + ((t - t0) * R + r) * C;
+                float run;
+                if constexpr (COST == MPPI_COST_QUADRATIC_MAP) run = CF::running_tap(I.cost, yy, t, tap[g][s]);
+                else run = CF::running(I.cost, yy, t);
+                acc[s] += run + lr_cost<C>(sp, u, mu + t * C, pure[s], I.lambda, I.alpha, t);
+              }
+            }
           }
         }
         // clamped samples of the chunk to the tile's [T][R][C] region (contiguous, 16-byte stores)

@@ -314,8 +314,15 @@ struct Model<MPPI_MODEL_RACER_DUBINS_NN> : Bicycle {
 // ---------------------------------------------------------------------------------------------------------- costs
 // Bilinear cost-map query with the reference's CPU semantics (utils/texture_helpers/two_d_texture_helper.cu:151-245
 // and the world->texture transform of texture_helper.cu:94-134, its rotation restricted to the plane): clamp address
-// mode, texel centres at +0.5, x0 = min(floor(q), W - 2).
-MPPI_HD float map_query(const mppi_cost_params& c, float px, float py)
+// mode, texel centres at +0.5, x0 = min(floor(q), W - 2). Split into the fetch of the four texels (map_fetch) and their
+// blend (map_blend), so that a caller can have the fetches of several queries in flight before it blends them.
+struct MapTap {
+  float q11, q12, q21, q22;  // texels (x0, y0), (x1, y0), (x0, y1), (x1, y1)
+  float qx, qy;              // clamped texel coordinates
+  int x0, y0;
+};
+
+MPPI_HD MapTap map_fetch(const mppi_cost_params& c, float px, float py)
 {
   const int W = c.map_width, H = c.map_height;
   // map frame R (p - origin) (worldPoseToMapPose; R uniform, so the branch is too)
@@ -331,17 +338,34 @@ MPPI_HD float map_query(const mppi_cost_params& c, float px, float py)
   const float inv_res = 1.0f / c.map_resolution;
   // clamp to [0, W - 1] as the reference's if/else chain does for every non-NaN value; a NaN query maps to texel 0
   // instead of reaching the index computation
-  const float qx = fminf(fmaxf(dx * inv_res - 0.5f, 0.0f), (float)(W - 1));
-  const float qy = fminf(fmaxf(dy * inv_res - 0.5f, 0.0f), (float)(H - 1));
-  const int x0 = min((int)floorf(qx), W - 2), x1 = x0 + 1;
-  const int y0 = min((int)floorf(qy), H - 2), y1 = y0 + 1;
-  const float* row0 = c.map + (size_t)y0 * W;
-  const float* row1 = c.map + (size_t)y1 * W;
-  const float q11 = row0[x0], q12 = row0[x1], q21 = row1[x0], q22 = row1[x1];
-  const float ya = q11 * ((float)x1 - qx) + q12 * (qx - (float)x0);
-  const float yb = q21 * ((float)x1 - qx) + q22 * (qx - (float)x0);
-  return ya * ((float)y1 - qy) + yb * (qy - (float)y0);
+  MapTap m;
+  m.qx = fminf(fmaxf(dx * inv_res - 0.5f, 0.0f), (float)(W - 1));
+  m.qy = fminf(fmaxf(dy * inv_res - 0.5f, 0.0f), (float)(H - 1));
+  m.x0 = min((int)floorf(m.qx), W - 2);
+  m.y0 = min((int)floorf(m.qy), H - 2);
+  // 32-bit texel index (maps hold at most 2^30 values); on the device the loads are global-memory loads
+  const uint32_t i0 = (uint32_t)m.y0 * (uint32_t)W + (uint32_t)m.x0, i1 = i0 + (uint32_t)W;
+#ifdef __HIP_DEVICE_COMPILE__
+  const __attribute__((address_space(1))) float* map = (const __attribute__((address_space(1))) float*)c.map;
+#else
+  const float* map = c.map;
+#endif
+  m.q11 = map[i0];
+  m.q12 = map[i0 + 1];
+  m.q21 = map[i1];
+  m.q22 = map[i1 + 1];
+  return m;
 }
+
+MPPI_HD float map_blend(const MapTap& m)
+{
+  const int x1 = m.x0 + 1, y1 = m.y0 + 1;
+  const float ya = m.q11 * ((float)x1 - m.qx) + m.q12 * (m.qx - (float)m.x0);
+  const float yb = m.q21 * ((float)x1 - m.qx) + m.q22 * (m.qx - (float)m.x0);
+  return ya * ((float)y1 - m.qy) + yb * (m.qy - (float)m.y0);
+}
+
+MPPI_HD float map_query(const mppi_cost_params& c, float px, float py) { return map_blend(map_fetch(c, px, py)); }
 
 template <int O>
 MPPI_HD float quad_state(const mppi_cost_params& c, const float* y)
@@ -392,15 +416,23 @@ struct Cost<MPPI_COST_CARTPOLE_QUADRATIC, O> {
 // QuadraticCost::terminalCost).
 template <int O>
 struct Cost<MPPI_COST_QUADRATIC_MAP, O> {
-  MPPI_HD static float running(const mppi_cost_params& c, const float* y, int t)
+  MPPI_HD static bool uses_map(const mppi_cost_params& c) { return c.map != nullptr && c.map_coeff != 0.0f; }
+  // the running cost with the map term's texels already fetched (tap, read only when uses_map)
+  MPPI_HD static float running_tap(const mppi_cost_params& c, const float* y, int t, const MapTap& tap)
   {
     float s = quad_state_at<O>(c, y, t);
-    if (c.map != nullptr && c.map_coeff != 0.0f) s += c.map_coeff * map_query(c, y[c.map_x_index], y[c.map_y_index]);
+    if (uses_map(c)) s += c.map_coeff * map_blend(tap);
     if (c.speed_coeff != 0.0f) {
       const float e = y[c.speed_index] - c.speed_target;
       s += c.speed_coeff * e * e;
     }
     return s;
+  }
+  MPPI_HD static float running(const mppi_cost_params& c, const float* y, int t)
+  {
+    MapTap tap{};
+    if (uses_map(c)) tap = map_fetch(c, y[c.map_x_index], y[c.map_y_index]);
+    return running_tap(c, y, t, tap);
   }
   MPPI_HD static float terminal(const mppi_cost_params& c, const float* y)
   {
