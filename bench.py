@@ -351,7 +351,7 @@ def run_controller(name: str, args, steps: int, warmup: int, cpu_seconds: float)
     }
 
 
-def learned_comparison(episodes: int, use_graph: bool = True):
+def learned_comparison(episodes: int, use_graph: bool = False):
     """Learned vs plain MPPI on the bicycle + residual-network task with the committed trained proposal, closed loop
     from held-out starts (executed-trajectory cost, stream and host-inclusive rates per control step):
       - plain MPPI's convergence curve at 8192 samples and 1, 2, 3, 5 iterations;
@@ -534,8 +534,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     ap.add_argument("--learned-episodes", type=int, default=4,
                     help="closed-loop episodes of the learned-proposal comparison (0: skip)")
-    ap.add_argument("--learned-eager", action="store_true",
-                    help="run the learned-proposal comparison with eager launches instead of graph replay")
+    ap.add_argument("--learned-graph", action="store_true",
+                    help="run the learned-proposal comparison from captured hipGraphs instead of direct launches")
     ap.add_argument("--calibration-mib", type=int, default=256,
                     help="device-to-device copy of this size, a plain copy kernel of known traffic for counter scaling")
     args = ap.parse_args()
@@ -592,7 +592,7 @@ def main():
         if args.workload != "tube_di":
             line["two_system"] = run_two_system(args, min(args.host_calls, 100),
                                                 0 if args.no_cpu else args.config_cpu_seconds)
-    line["learned_vs_plain"] = learned_comparison(args.learned_episodes, not args.learned_eager)
+    line["learned_vs_plain"] = learned_comparison(args.learned_episodes, args.learned_graph)
     print(json.dumps(line), flush=True)
 
 

@@ -339,7 +339,9 @@ typedef struct mppi_controller_desc {
   float lambda;
   float alpha;
   int32_t num_iters;
-  int32_t use_graph;    /* 1: capture one computeControl call into a hipGraph on its second use and replay it */
+  int32_t use_graph;    /* 1: capture one computeControl call into a hipGraph on its second use and replay it;
+                           0 (faster on ROCm 7 for host-inclusive calls): direct launches, see
+                           mppi_controller_compute_control */
   int32_t keep_samples; /* 1: store the clamped control samples [K][T][C] in the sample buffer every iteration */
   float slide_control_scale[MPPI_MAX_CONTROL]; /* ControllerParams::slide_control_scale_ */
   const float* init_control; /* host [T][C] initial nominal sequence or NULL for zeros */
@@ -362,9 +364,14 @@ typedef struct mppi_controller_desc {
 int mppi_controller_create(const mppi_controller_desc* desc, mppi_controller** out);
 int mppi_controller_destroy(mppi_controller* ctrl);
 
-/* One MPC solve: H2D x0, num_iters x (sample, rollout, weights, update), D2H of the new mean, Savitzky-Golay smoothing
- * keeping u[0] (controllers/controller.cuh:573-607), host state-trajectory propagation (:664-684) and a control clamp
- * (controllers/MPPI/mppi_controller.cu:224-227). */
+/* One MPC solve: x0 to the device, num_iters x (sample, rollout, weights, update), the new mean to the host,
+ * Savitzky-Golay smoothing keeping u[0] (controllers/controller.cuh:573-607), the state trajectory (:664-684,
+ * propagated on the host when first requested) and a control clamp (controllers/MPPI/mppi_controller.cu:224-227).
+ * The inputs and results travel through mapped host memory and the host polls a completion word written by the last
+ * kernel. Without use_graph, for the built-in models and costs with normExp weights, the calls are direct launches:
+ * the iteration arguments are kernel arguments, the rollout reads x0 from host memory, the smoothing and the clamp
+ * run in the last kernel (bit-identical to the host functions), and the smoothed sequence stays on the device as the
+ * next call's nominal unless the host changes it (set_control_seq, slide_control_seq, set_params). */
 int mppi_controller_compute_control(mppi_controller* ctrl, const float* x0_host, int32_t optimization_stride);
 /* Enqueue `num_calls` device-resident repetitions of the GPU work of one computeControl call (num_iters x sample,
  * rollout, weights, update) on the controller's stream, replayed from captured hipGraphs of 32 and 8 calls

@@ -767,7 +767,7 @@ struct ControllerParams {
   dim3 cost_rollout_dim_;      // accepted, unused
   Mat<float, C_DIM, MAX_TIMESTEPS> init_control_traj_ = Mat<float, C_DIM, MAX_TIMESTEPS>::Zero();
   Mat<float, C_DIM, 1> slide_control_scale_ = Mat<float, C_DIM, 1>::Zero();
-  bool use_graph_ = true;
+  bool use_graph_ = false;
 };
 
 // User-written device plugins (include/mppi_amd/device_plugin.hpp) mark themselves with DEVICE_PLUGIN; a controller

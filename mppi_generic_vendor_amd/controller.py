@@ -36,7 +36,7 @@ def _host_arrays(d, problem: Problem) -> list:
 
 
 class VanillaMPPIController:
-    def __init__(self, problem: Problem, num_iters: int = 1, use_graph: bool = True,
+    def __init__(self, problem: Problem, num_iters: int = 1, use_graph: bool = False,
                  init_control: np.ndarray | None = None, slide_control_scale=None, keep_samples: bool = False):
         lib = capi.load()
         self.problem = problem
@@ -224,7 +224,7 @@ class ColoredMPPIController(VanillaMPPIController):
     previous solution's state at the last slide's step count), and only control 1 clamped after smoothing. Parameter
     setters swap the new values in through set_params."""
 
-    def __init__(self, problem: Problem, num_iters: int = 1, use_graph: bool = True,
+    def __init__(self, problem: Problem, num_iters: int = 1, use_graph: bool = False,
                  init_control: np.ndarray | None = None, slide_control_scale=None, keep_samples: bool = False,
                  gamma: float = 0.0, r: float = 2.0, leash_active: bool = False, state_leash_dist=None):
         self._gamma, self._r, self._leash_active = float(gamma), float(r), bool(leash_active)

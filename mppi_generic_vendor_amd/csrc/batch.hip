@@ -408,19 +408,6 @@ struct BatchCopies {
   HostSignal signal;
 };
 
-// Savitzky-Golay taps of smooth_control (controllers/controller.cuh:573-607), in the same FP32 order with products and
-// sums rounded separately (no FMA contraction), so the device's smoothed sequence is bit-identical to the host's.
-__device__ __forceinline__ float sg_smooth_at(const float* u, int t, int c, int T, int C)
-{
-#pragma clang fp contract(off)
-  const float coef[5] = {-3.0f / 35.0f, 12.0f / 35.0f, 17.0f / 35.0f, 12.0f / 35.0f, -3.0f / 35.0f};
-  if (t == 0) return u[c];
-  float s = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) s = s + coef[k] * u[min(max(t + k - 2, 0), T - 1) * C + c];
-  return s;
-}
-
 // One workgroup per instance: global baseline (first strict minimum over the tiles' minima), FP64 normalizer and
 // sum of squares, new mean S / eta, statistics; optionally advances the instance's generator counter. The new mean is
 // then Savitzky-Golay smoothed and constrained (computeControl's post-processing, controllers/controller.cuh:573-607

@@ -323,7 +323,8 @@ struct mppi_controller {
   // getTargetStateSeq: propagated on the first request after a call, from that call's x0 and smoothed (not yet
   // constrained) controls
   mutable std::vector<float> state_seq;
-  std::vector<float> state_x0, state_u;
+  std::vector<float> state_x0;
+  mutable std::vector<float> state_u;
   mutable bool state_stale = false;
   mppi_weight_stats stats{};
   mppi_free_energy fe{};
@@ -372,6 +373,16 @@ struct mppi_controller {
   uint32_t* done_h = nullptr;
   uint32_t* done_hd = nullptr;
   uint32_t signals = 0;
+  // direct calls (no graph, built-in model and cost, normExp weights): the last combine smooths and constrains the new
+  // mean into control_out (mapped) and nominal_d, the next call's sampling mean, which the host uploads only when it
+  // has changed its sequence since (nominal_dirty)
+  float* nominal_d = nullptr;
+  float* control_out_h = nullptr;
+  float* control_out_hd = nullptr;
+  bool nominal_dirty = true;
+  bool x0_d_stale = false;  // a direct call read x0 from host memory: x0_d is refreshed before its next use
+  // state_u holds the raw mean, smoothed when the trajectory is first requested
+  mutable bool state_smooth_pending = false;
 };
 
 extern "C" {
@@ -807,11 +818,13 @@ static int controller_free(mppi_controller* c)
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* dev[] = {c->x0_d, c->mean_d, c->samples_d, c->costs_d, c->stats_d, c->ws_d, c->iter_d,
                  c->map_d, c->nn_d, c->loop_iter_d, c->prop_dmu_d, c->prop_sig_d, c->sigma_base_d,
-                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d, c->std_t_d, c->goal_d, c->w_d, c->sig_d};
+                 c->vis_mean_d, c->vis_idx_d, c->vis_buf_d, c->std_t_d, c->goal_d, c->w_d, c->sig_d,
+                 c->nominal_d};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (c->map_ev) (void)hipEventDestroy(c->map_ev);
-  void* host[] = {c->iter_h, c->x0_h, c->nominal_h, c->mean_out_h, c->stats_h, c->cost0_h, c->map_pinned, c->done_h};
+  void* host[] = {c->iter_h,  c->x0_h,       c->nominal_h, c->mean_out_h,   c->stats_h,
+                  c->cost0_h, c->map_pinned, c->done_h,    c->control_out_h};
   for (void* p : host)
     if (p) (void)hipHostFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -901,6 +914,8 @@ static int controller_setup(mppi_controller* c, const mppi_controller_desc* desc
   MPPI_CHECK(host_alloc_mapped(&c->mean_out_h, &c->mean_out_hd, sizeof(float) * T * C));
   MPPI_CHECK(host_alloc_mapped(&c->stats_h, &c->stats_hd, sizeof(mppi_weight_stats)));
   MPPI_CHECK(host_alloc_mapped(&c->done_h, &c->done_hd, sizeof(uint32_t)));
+  MPPI_CHECK(host_alloc_mapped(&c->control_out_h, &c->control_out_hd, sizeof(float) * T * C));
+  MPPI_CHECK(dev_alloc(&c->nominal_d, sizeof(float) * T * C));
   MPPI_CHECK(dev_alloc(&c->sig_d, 2 * sizeof(uint32_t)));
   MPPI_HIP_CHECK(hipMemsetAsync(c->sig_d, 0, 2 * sizeof(uint32_t), c->stream));
   MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1098,6 +1113,66 @@ static int controller_enqueue(mppi_controller* c)
   return MPPI_OK;
 }
 
+// x0_d from the last call's x0 (host staging) where a direct call left it behind, on the controller's stream.
+static int refresh_x0_d(mppi_controller* c)
+{
+  if (!c->x0_d_stale) return MPPI_OK;
+  const WordSpan w[] = {{c->x0_hd, c->x0_d, c->S}};
+  MPPI_CHECK(launch_copy_words(w, 1, c->stream));
+  c->x0_d_stale = false;
+  return MPPI_OK;
+}
+
+// Whether computeControl takes the direct path: launches without a graph, the built-in models and costs and the
+// fused normExp iteration (the device post-processing mirrors the vanilla controller's).
+static bool direct_path(const mppi_controller* c)
+{
+  const mppi_controller_desc& d = c->d;
+  return !d.use_graph && !c->has_plugin && d.controller_kind == MPPI_CONTROLLER_VANILLA && !uses_tsallis(d);
+}
+
+// The direct path's call: the iteration arguments by value and x0 read from mapped host memory by the rollout, the
+// nominal uploaded only when the host changed it (else the one the last call left in nominal_d), and the last
+// combine's post-processing and completion signal.
+static int controller_enqueue_direct(mppi_controller* c)
+{
+  const int T = c->T, C = c->C;
+  const mppi_controller_desc& d = c->d;
+  const bool vis = c->vis_random + c->vis_top > 0;
+  const bool colored = d.sampler.kind == MPPI_SAMPLER_COLORED;
+  ColoredPlan plan;
+  if (colored) MPPI_CHECK(colored_plan(d.sampler, c->K, T, C, c->stream, &plan));
+  WordSpan in[2];
+  int n = 0;
+  if (c->nominal_dirty) in[n++] = WordSpan{c->nominal_hd, c->nominal_d, T * C};
+  // with the trajectory export and one iteration, the nominal is also the last iteration's sampling mean
+  if (vis && d.num_iters == 1)
+    in[n++] = WordSpan{c->nominal_dirty ? (const void*)c->nominal_hd : (const void*)c->nominal_d, c->vis_mean_d, T * C};
+  if (n > 0) MPPI_CHECK(launch_copy_words(in, n, c->stream));
+  ResultCopies out;
+  out.mean = c->mean_out_hd;
+  out.stats = c->stats_hd;
+  out.signal = call_signal(c);
+  out.control = c->control_out_hd;
+  out.nominal = c->nominal_d;
+  out.limits = control_limits(d.dynamics);
+  out.T = T;
+  out.C = C;
+  c->x0_d_stale = true;
+  for (int it = 0; it < d.num_iters; ++it) {
+    if (vis && d.num_iters > 1 && it + 1 == d.num_iters) {
+      const WordSpan m[] = {{c->mean_d, c->vis_mean_d, T * C}};
+      MPPI_CHECK(launch_copy_words(m, 1, c->stream));
+    }
+    const IterRef ref{nullptr, c->iter_h[it]};
+    MPPI_CHECK(launch_fused_iteration(d.dynamics, d.cost, d.sampler, c->K, T, d.dt, d.lambda, d.alpha, c->x0_hd,
+                                      it == 0 ? c->nominal_d : c->mean_d, c->samples_d, !colored, keeps_samples(c),
+                                      ref, c->costs_d, c->ws_d, c->stats_d, c->mean_d, nullptr, c->stream, nullptr,
+                                      nullptr, it + 1 == d.num_iters ? &out : nullptr, colored ? &plan : nullptr));
+  }
+  return MPPI_OK;
+}
+
 // The proposal call: network inference on the uploaded state and nominal sequence, then one proposal iteration of
 // prop_K samples (importance-weight corrected costs), the new mean, statistics and the warm start's cost to the host.
 static int controller_enqueue_proposal(mppi_controller* c)
@@ -1126,7 +1201,7 @@ static int controller_enqueue_proposal(mppi_controller* c)
 static int controller_run(mppi_controller* c, int (*enqueue)(mppi_controller*), hipGraph_t* graph,
                           hipGraphExec_t* exec, int* runs)
 {
-  if (c->d.use_graph && *runs > 0) {
+  if (c->d.use_graph && graph != nullptr && *runs > 0) {
     if (!*exec) {
       MPPI_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       const int st = enqueue(c);
@@ -1298,20 +1373,35 @@ int mppi_controller_compute_control(mppi_controller* c, const float* x0_host, in
     if (plain) c->n_fallback++;
     K_used = c->prop_K;
   }
+  bool direct = false;
   if (plain) {
     controller_set_iters(c, stride, d.num_iters);
-    MPPI_CHECK(controller_run(c, controller_enqueue, &c->graph, &c->graph_exec, &c->eager_runs));
+    direct = direct_path(c);
+    if (direct) MPPI_CHECK(controller_run(c, controller_enqueue_direct, nullptr, nullptr, &c->eager_runs));
+    else MPPI_CHECK(controller_run(c, controller_enqueue, &c->graph, &c->graph_exec, &c->eager_runs));
     K_used = K;
   }
   c->stats = *c->stats_h;
   c->raw_mean.assign(c->mean_out_h, c->mean_out_h + (size_t)T * C);
-  c->control = c->raw_mean;
   free_energy_from_stats(c->stats, d.lambda, K_used, &c->fe);
-  smooth_control(c->control, T, C);
   // the state trajectory of the smoothed controls (computeStateTrajectory) is propagated when first requested
   c->state_x0.assign(c->x0_h, c->x0_h + S);
-  c->state_u = c->control;
   c->state_stale = true;
+  if (direct) {
+    // smoothed and constrained on the device, bit-identical to the host functions; the device holds it as the next
+    // nominal
+    c->control.assign(c->control_out_h, c->control_out_h + (size_t)T * C);
+    c->state_u = c->raw_mean;
+    c->state_smooth_pending = true;
+    c->nominal_dirty = false;
+    c->computed = true;
+    return MPPI_OK;
+  }
+  c->nominal_dirty = true;
+  c->control = c->raw_mean;
+  smooth_control(c->control, T, C);
+  c->state_u = c->control;
+  c->state_smooth_pending = false;
   if (colored_ctrl) {
     // the colored controller clamps control 1 alone to its range (colored_mppi_controller.cu:226-231; a model with
     // one control has none to clamp)
@@ -1451,6 +1541,7 @@ int mppi_controller_step_device(mppi_controller* c, int32_t num_calls)
     return fail(MPPI_E_INVALID, "step_device: call computeControl once first (the loop starts from its x0 and mean)");
   if (num_calls == 0) return MPPI_OK;
   MPPI_CHECK(apply_staged_map(c));
+  MPPI_CHECK(refresh_x0_d(c));
   if (!c->loop_exec) {
     MPPI_CHECK(loop_args_init(c));
     MPPI_CHECK(controller_enqueue_loop(c));  // eager pass (sets kernel attributes before capture)
@@ -1473,6 +1564,7 @@ int mppi_controller_time_kernels(mppi_controller* c, int32_t reps, float* rollou
   if (!c || !rollout_ms || !combine_ms || reps < 1) return fail(MPPI_E_INVALID, "bad argument");
   if (!c->computed) return fail(MPPI_E_INVALID, "time_kernels: call computeControl once first");
   MPPI_CHECK(loop_args_init(c));
+  MPPI_CHECK(refresh_x0_d(c));
   std::vector<hipEvent_t> ev((size_t)(2 * reps + 1), nullptr);
   int st = MPPI_OK;
   for (auto& e : ev)
@@ -1524,6 +1616,10 @@ int mppi_controller_get_state_seq(const mppi_controller* c, float* out)
 {
   if (!c || !out) return fail(MPPI_E_INVALID, "null argument");
   if (c->state_stale) {
+    if (c->state_smooth_pending) {
+      smooth_control(c->state_u, c->T, c->C);
+      c->state_smooth_pending = false;
+    }
     if (c->has_plugin) {
       plugin_propagate(c, c->state_x0.data(), c->state_u.data(), c->state_seq.data());
     } else {
@@ -1555,6 +1651,7 @@ int mppi_controller_set_control_seq(mppi_controller* c, const float* u)
 {
   if (!c || !u) return fail(MPPI_E_INVALID, "null argument");
   c->control.assign(u, u + (size_t)c->T * c->C);
+  c->nominal_dirty = true;
   return MPPI_OK;
 }
 
@@ -1566,6 +1663,7 @@ int mppi_controller_slide_control_seq(mppi_controller* c, int32_t steps)
   const float* zero = c->d.dynamics.zero_control;
   const float* scale = c->d.slide_control_scale;
   c->leash_jump = steps;
+  c->nominal_dirty = true;
   for (int i = 0; i < T; ++i) {
     const int ind = std::min(i + steps, T - 1);
     for (int j = 0; j < C; ++j) {
@@ -1611,6 +1709,7 @@ int mppi_controller_set_params(mppi_controller* c, const mppi_controller_desc* d
                                 "changed (create a new controller)");
   if (!(desc->lambda > 0.0f)) return fail(MPPI_E_INVALID, "lambda must be > 0");
   MPPI_CHECK(validate_controller_kind(*desc, c->has_plugin));
+  c->nominal_dirty = true;  // the next call uploads the host's sequence under the new parameters
   if (c->prop != nullptr && uses_tsallis(*desc))
     return fail(MPPI_E_UNSUPPORTED, "set_params: Tsallis weights with a learned proposal attached");
   mppi_controller_desc n = *desc;
@@ -1767,6 +1866,7 @@ int mppi_controller_get_sampled_trajectories(mppi_controller* c, int32_t* count,
   float* out = ctl + (size_t)n * T * C;
   float* stp = out + (size_t)n * T * O;
   float* term = stp + (size_t)n * T;
+  MPPI_CHECK(refresh_x0_d(c));
   MPPI_CHECK(launch_vis_rollouts(d.dynamics, d.cost, d.sampler, K, T, d.dt, d.lambda, d.alpha, c->x0_d, c->vis_mean_d,
                                  c->samples_d, c->vis_idx_d, n, ctl, out, stp, term, c->stream));
   MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));

@@ -192,25 +192,46 @@ struct HostSignal {
   uint32_t* flag = nullptr;
 };
 
-// Every thread of every workgroup calls this at the very end of the kernel. Each thread's own stores (results in
-// mapped host memory among them) are made visible at system scope, then the last workgroup to arrive resets the count,
-// advances the epoch and publishes it with a system-scope release store.
-__device__ __forceinline__ void signal_host_done(const HostSignal& s)
+// Called by every thread of every workgroup at the end of a kernel whose results the host reads. Each thread's own
+// stores (results in mapped host memory among them) are made visible at system scope, then the workgroup counts
+// itself done; true, in every thread of the workgroup, for the last workgroup to arrive, which then also sees every
+// other workgroup's stores.
+__device__ __forceinline__ bool last_block_arrive(const HostSignal& s)
 {
-  if (s.flag == nullptr) return;
+  __shared__ uint32_t last;
   __threadfence_system();
   __syncthreads();
   if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
     const uint32_t blocks = gridDim.x * gridDim.y * gridDim.z;
     const uint32_t before = __hip_atomic_fetch_add(s.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (before + 1u == blocks) {
-      *s.count = 0u;
-      const uint32_t e = *s.epoch + 1u;
-      *s.epoch = e;
-      __threadfence_system();
-      __hip_atomic_store(s.flag, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    last = (before + 1u == blocks) ? 1u : 0u;
   }
+  __syncthreads();
+  const bool is_last = last != 0u;
+  if (is_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return is_last;
+}
+
+// Called by every thread of the last workgroup (last_block_arrive) once its own stores are done: resets the count,
+// advances the epoch and publishes it with a system-scope release store.
+__device__ __forceinline__ void signal_publish(const HostSignal& s)
+{
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
+    *s.count = 0u;
+    const uint32_t e = *s.epoch + 1u;
+    *s.epoch = e;
+    __threadfence_system();
+    __hip_atomic_store(s.flag, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// The whole signal at the very end of a kernel (every thread of every workgroup calls it). Null flag: nothing.
+__device__ __forceinline__ void signal_host_done(const HostSignal& s)
+{
+  if (s.flag == nullptr) return;
+  if (last_block_arrive(s)) signal_publish(s);
 }
 
 }  // namespace mppi_amd

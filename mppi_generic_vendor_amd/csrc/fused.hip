@@ -80,13 +80,14 @@ __global__ void __launch_bounds__((ru_threads<MODEL, SPLIT>())) rollout_update_k
   const int k0 = g * rows;
   const int nrows = min(rows, K - k0);
   // the mean and x0 are requested first: their loads then retire with the tile's, whose waits come later; x0 is
-  // staged in the local-weight slots, which are free until the rollout has read it
+  // staged in the local-weight slots, which are free until the rollout has read it. x0 may live in mapped host
+  // memory, so only wavefront 0 reads it (one request per workgroup).
   float mv[MV];
   if constexpr (CNTT == 0) {
 #pragma unroll
     for (int i = 0; i < MV; ++i) mv[i] = mean[min(tid + i * NT, TC - 1)];
   }
-  const float xv = x0[min(tid, M::S - 1)];
+  const float xv = tid < 64 ? x0[min(tid, M::S - 1)] : 0.0f;
   float* x0_l = wl;
   bool deadband = false;
 #pragma unroll
@@ -290,7 +291,20 @@ __global__ void __launch_bounds__(THREADS) combine_kernel(const Partial* __restr
     if (copies.cost0 != nullptr) *copies.cost0 = *copies.cost0_src;
     if (advance != nullptr) advance->rng_offset += (uint64_t)advance_step;
   }
-  signal_host_done(copies.signal);
+  if (copies.signal.flag != nullptr && last_block_arrive(copies.signal)) {
+    // the last workgroup, which sees every column of the new mean: computeControl's post-processing
+    // (controllers/controller.cuh:573-607 smoothing, controllers/MPPI/mppi_controller.cu:224-227 constraints) into
+    // the host's copy and the next call's nominal
+    if (copies.control != nullptr) {
+      for (int q = tid; q < TC; q += COMBINE_THREADS) {
+        const int t = q / copies.C, c = q - t * copies.C;
+        const float u = constrain_with(copies.limits, c, sg_smooth_at(mean_out, t, c, copies.T, copies.C));
+        copies.control[q] = u;
+        copies.nominal[q] = u;
+      }
+    }
+    signal_publish(copies.signal);
+  }
 }
 
 // Compute units of the current device (cached per device; 0 if the query fails).

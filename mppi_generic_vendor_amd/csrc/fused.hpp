@@ -3,6 +3,7 @@
 
 #include "colored_core.hpp"
 #include "kernels.hpp"
+#include "models.hpp"
 
 namespace mppi_amd {
 
@@ -19,13 +20,19 @@ size_t fused_iteration_workspace_bytes(int K, int T, int C);
 
 // Optional copies of an iteration's results that the combine kernel also writes, typically to mapped host memory
 // (device-side addresses): the new mean [T C], the statistics, and cost0_src[0] to cost0. Null members are skipped.
-// `signal` (optional) is given once all of these are visible to the host.
+// `signal` (optional) is given once all of these are visible to the host. With a signal and `control`, the combine's
+// last workgroup also writes the new mean Savitzky-Golay smoothed and constrained by `limits` to `control` [T C]
+// (host) and `nominal` [T C] (device).
 struct ResultCopies {
   float* mean = nullptr;
   mppi_weight_stats* stats = nullptr;
   const float* cost0_src = nullptr;
   float* cost0 = nullptr;
   HostSignal signal;
+  float* control = nullptr;
+  float* nominal = nullptr;
+  ControlLimits limits{};
+  int T = 0, C = 0;
 };
 
 // One MPPI iteration in two launches. With `generate` the Gaussian control samples are drawn inside the rollout

@@ -115,6 +115,44 @@ MPPI_HD void clamp_controls(const mppi_dynamics_params& p, float* u)
   for (int i = 0; i < C; ++i) u[i] = fminf(fmaxf(p.control_min[i], u[i]), p.control_max[i]);
 }
 
+// Savitzky-Golay taps of smooth_control (controllers/controller.cuh:573-607), in the same FP32 order with products and
+// sums rounded separately (no FMA contraction), so the device's smoothed sequence is bit-identical to the host's.
+MPPI_HD float sg_smooth_at(const float* u, int t, int c, int T, int C)
+{
+#pragma clang fp contract(off)
+  const float coef[5] = {-3.0f / 35.0f, 12.0f / 35.0f, 17.0f / 35.0f, 12.0f / 35.0f, -3.0f / 35.0f};
+  if (t == 0) return u[c];
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) s = s + coef[k] * u[min(max(t + k - 2, 0), T - 1) * C + c];
+  return s;
+}
+
+// enforce_constraints of control i from copied limits (the same arithmetic), for kernels that constrain a whole
+// control sequence without the dynamics parameter block.
+struct ControlLimits {
+  float lo[MPPI_MAX_CONTROL], hi[MPPI_MAX_CONTROL], dead[MPPI_MAX_CONTROL], zero[MPPI_MAX_CONTROL];
+};
+
+inline ControlLimits control_limits(const mppi_dynamics_params& p)
+{
+  ControlLimits L;
+  for (int i = 0; i < MPPI_MAX_CONTROL; ++i) {
+    L.lo[i] = p.control_min[i];
+    L.hi[i] = p.control_max[i];
+    L.dead[i] = p.control_deadband[i];
+    L.zero[i] = p.zero_control[i];
+  }
+  return L;
+}
+
+MPPI_HD float constrain_with(const ControlLimits& L, int i, float v)
+{
+  if (fabsf(v) < L.dead[i]) v = L.zero[i];
+  else v += L.dead[i] * -(v >= 0.0f ? 1.0f : -1.0f);
+  return fminf(fmaxf(L.lo[i], v), L.hi[i]);
+}
+
 template <int MODEL>
 struct Model;
 

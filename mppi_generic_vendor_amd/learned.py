@@ -40,6 +40,16 @@ def random_start(rng: np.random.Generator) -> np.ndarray:
     return x
 
 
+def kick(rng: np.random.Generator, x: np.ndarray) -> np.ndarray:
+    """A disturbance of the running system: new speed ~ U(2, 10) m/s, heading turned by ~ U(-1, 1) rad and steering
+    angle ~ U(-0.3, 0.3). Right after one the warm start (the previous plan, slid by a step) no longer fits the state."""
+    y = x.copy()
+    y[0] = rng.uniform(2.0, 10.0)
+    y[1] = y[1] + rng.uniform(-1.0, 1.0)
+    y[4] = rng.uniform(-0.3, 0.3)
+    return y.astype(np.float32)
+
+
 class Plant:
     """The true system for closed-loop runs: the same model stepped on the host (mppi_host_step)."""
 
@@ -87,8 +97,9 @@ class Dataset:
 
 def generate_pairs(num_pairs: int, seed: int = 1, num_rollouts: int = 8192, iters: int = 3,
                    steps_per_episode: int = 50, workers: int = 4, cost_map: np.ndarray | None = None,
-                   log=None) -> Dataset:
-    """Closed-loop plain MPPI from seeded random starts, `workers` controllers on their own streams."""
+                   log=None, kick_every: int = 0) -> Dataset:
+    """Closed-loop plain MPPI from seeded random starts, `workers` controllers on their own streams; with kick_every
+    > 0 the system is disturbed (`kick`) every kick_every control steps."""
     m = workloads.gaussian_blob_map() if cost_map is None else cost_map
     per = [num_pairs // workers + (1 if w < num_pairs % workers else 0) for w in range(workers)]
     out = [None] * workers
@@ -98,16 +109,18 @@ def generate_pairs(num_pairs: int, seed: int = 1, num_rollouts: int = 8192, iter
         try:
             prob = task(num_rollouts, cost_map=m, seed=1000 * seed + w)
             plant = Plant(prob)
-            ctrl = VanillaMPPIController(prob, num_iters=iters, use_graph=True)
+            ctrl = VanillaMPPIController(prob, num_iters=iters)
             rng = np.random.default_rng(seed * 7919 + w)
             T, C = prob.num_timesteps, prob.C
             xs, ps, us = [], [], []
             while len(xs) < per[w]:
                 x = random_start(rng)
                 ctrl.updateImportanceSampler(np.zeros((T, C), np.float32))
-                for _ in range(steps_per_episode):
+                for step in range(steps_per_episode):
                     if len(xs) >= per[w]:
                         break
+                    if kick_every > 0 and step > 0 and step % kick_every == 0:
+                        x = kick(rng, x)
                     prev = ctrl.getControlSeq()
                     ctrl.computeControl(x, 1)
                     xs.append(x.copy())
@@ -149,17 +162,21 @@ def train(ds: Dataset, sigma: np.ndarray, epochs: int = 30, batch: int = 512, lr
 
 
 def closed_loop(prob: workloads.Problem, ctrl: VanillaMPPIController, starts: list[np.ndarray], steps: int,
-                wall: list | None = None):
+                wall: list | None = None, kick_every: int = 0, kick_seed: int = 5):
     """Executed-trajectory cost per episode and the stream time of every control step (appended to `wall`: the
-    host wall time of every computeControl call)."""
+    host wall time of every computeControl call). With kick_every > 0 the system is disturbed every kick_every
+    steps by the same seeded sequence of kicks for every controller."""
     plant = Plant(prob)
     T, C = prob.num_timesteps, prob.C
     costs, ms = [], []
+    krng = np.random.default_rng(kick_seed)
     for x0 in starts:
         x = x0.copy()
         ctrl.updateImportanceSampler(np.zeros((T, C), np.float32))
         total = 0.0
-        for _ in range(steps):
+        for step in range(steps):
+            if kick_every > 0 and step > 0 and step % kick_every == 0:
+                x = kick(krng, x)
             t0 = time.perf_counter()
             ctrl.computeControl(x, 1)
             if wall is not None:
@@ -175,7 +192,7 @@ def closed_loop(prob: workloads.Problem, ctrl: VanillaMPPIController, starts: li
 def evaluate(net: ProposalNet | None, episodes: int = 10, steps: int = 100, seed: int = 99,
              plain_rollouts: int = 8192, plain_iters: int = 3, proposal_rollouts: int = 2048,
              fallback_tolerance: float = 0.005, cost_map: np.ndarray | None = None, log=None,
-             use_graph: bool = True) -> dict:
+             use_graph: bool = False, kick_every: int = 0) -> dict:
     m = workloads.gaussian_blob_map() if cost_map is None else cost_map
     rng = np.random.default_rng(seed)
     starts = [random_start(rng) for _ in range(episodes)]
@@ -191,7 +208,7 @@ def evaluate(net: ProposalNet | None, episodes: int = 10, steps: int = 100, seed
         ctrl.computeControl(starts[0], 1)  # warm-up (graph capture)
         ctrl.computeControl(starts[0], 1)
         wall = []
-        c, ms = closed_loop(prob, ctrl, starts, steps, wall)
+        c, ms = closed_loop(prob, ctrl, starts, steps, wall, kick_every=kick_every)
         entry = {"cost": float(c.mean()), "costs": c.tolist(), "ms_per_call": float(np.mean(ms)),
                  "ms_median": float(statistics.median(ms)), "hz": 1e3 / float(np.mean(ms)),
                  "ms_host_median": float(statistics.median(wall)), "hz_host": 1e3 / float(statistics.median(wall)),
@@ -232,7 +249,7 @@ def cold_start_quality(net: ProposalNet | None, starts: int = 32, seed: int = 11
     for name, K, iters, learned_mode in modes:
         prob = task(K, cost_map=m, seed=4243)
         plant = Plant(prob)
-        ctrl = VanillaMPPIController(prob, num_iters=iters, use_graph=True)
+        ctrl = VanillaMPPIController(prob, num_iters=iters)
         if learned_mode:
             ctrl.setProposal(net, proposal_rollouts, fallback_tolerance=None)
         T, C = prob.num_timesteps, prob.C
