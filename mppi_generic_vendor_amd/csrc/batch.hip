@@ -596,14 +596,12 @@ struct mppi_batch {
   float* means_d = nullptr;
   float* means_backup_d = nullptr;
   float* nominal_h = nullptr;
-  float* mean_out_h = nullptr;
   float* control_out_h = nullptr;
   int* upload_h = nullptr;
   // device-side aliases of the mapped host buffers above
   IterArgs* iters_hd = nullptr;
   float* x0_hd = nullptr;
   float* nominal_hd = nullptr;
-  float* mean_out_hd = nullptr;
   float* control_out_hd = nullptr;
   int* upload_hd = nullptr;
   mppi_weight_stats* stats_hd = nullptr;
@@ -615,6 +613,13 @@ struct mppi_batch {
   hipEvent_t ev_end = nullptr;
   bool use_graph = false;
   std::vector<char> dirty;  // the host's control sequence differs from the device's nominal
+  // The last call's results are read where the combine left them: the controls and statistics in mapped host memory,
+  // the raw means in raw_d (copied to the host on the first request). ctl_host[i]: control[i] (host) replaces the
+  // mapped controls after a setControlSeq / slide.
+  float* raw_d = nullptr;
+  std::vector<char> ctl_host;
+  mutable std::vector<char> raw_fresh;
+  std::vector<float> prev_baseline;
   std::vector<int> mean_base;
   int total_mean = 0;
   mppi_weight_stats* stats_d = nullptr;
@@ -635,13 +640,11 @@ struct mppi_batch {
   bool computed = false;    // computeControl has run once (the device loop starts from its x0 and means)
   bool loop_ready = false;  // loop_iters_d holds the device loop's arguments
   uint64_t rng_counter = 0;
-  std::vector<std::vector<float>> control, raw_mean;
+  std::vector<std::vector<float>> control;
+  mutable std::vector<std::vector<float>> raw_mean;
   // per instance, propagated on the first request after a call from that call's x0 and smoothed controls
   mutable std::vector<std::vector<float>> state_seq;
-  std::vector<std::vector<float>> state_x0;
   mutable std::vector<char> state_stale;
-  std::vector<mppi_weight_stats> stats;
-  std::vector<mppi_free_energy> fe;
 };
 
 static void batch_drop_graphs(mppi_batch* b)
@@ -664,10 +667,10 @@ static int batch_free(mppi_batch* b)
   for (void* p : {(void*)b->map_d, (void*)b->tables_d, (void*)b->inst_d, (void*)b->work_d, (void*)b->queue_d, (void*)b->iters_d,
                   (void*)b->loop_iters_d, (void*)b->x0_d, (void*)b->means_d, (void*)b->means_backup_d,
                   (void*)b->stats_d, (void*)b->ubuf_d, (void*)b->parts_d, (void*)b->sums_d, (void*)b->costs_d,
-                  (void*)b->sig_d})
+                  (void*)b->sig_d, (void*)b->raw_d})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)b->iters_h, (void*)b->x0_h, (void*)b->nominal_h, (void*)b->mean_out_h, (void*)b->stats_h,
-                  (void*)b->control_out_h, (void*)b->upload_h, (void*)b->done_h})
+  for (void* p : {(void*)b->iters_h, (void*)b->x0_h, (void*)b->nominal_h, (void*)b->stats_h, (void*)b->control_out_h,
+                  (void*)b->upload_h, (void*)b->done_h})
     if (p) (void)hipHostFree(p);
   if (b->ev_end) (void)hipEventDestroy(b->ev_end);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -737,7 +740,7 @@ static int batch_enqueue_call(mppi_batch* b)
                                                                     b->x0_d, b->nominal_hd, b->upload_hd, b->means_d);
   MPPI_HIP_CHECK(hipGetLastError());
   BatchCopies out;
-  out.mean = b->mean_out_hd;
+  out.mean = b->raw_d;
   out.control = b->control_out_hd;
   out.stats = b->stats_hd;
   out.signal.count = b->sig_d;
@@ -847,11 +850,8 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
     if (d.init_control) std::copy(d.init_control, d.init_control + (size_t)T * C, b->control.back().begin());
     b->raw_mean.emplace_back((size_t)T * C, 0.0f);
     b->state_seq.emplace_back((size_t)T * S, 0.0f);
-    b->state_x0.emplace_back((size_t)BATCH_MAX_STATE, 0.0f);
   }
   b->state_stale.assign(n, 0);
-  b->stats.assign(n, mppi_weight_stats{});
-  b->fe.assign(n, mppi_free_energy{});
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -877,13 +877,16 @@ int mppi_batch_create(const mppi_instance_desc* instances, int32_t n, const floa
   A(host_alloc_mapped(&b->iters_h, &b->iters_hd, sizeof(IterArgs) * n));
   A(host_alloc_mapped(&b->x0_h, &b->x0_hd, sizeof(float) * BATCH_MAX_STATE * n));
   A(host_alloc_mapped(&b->nominal_h, &b->nominal_hd, sizeof(float) * b->total_mean));
-  A(host_alloc_mapped(&b->mean_out_h, &b->mean_out_hd, sizeof(float) * b->total_mean));
+  A(dev_alloc(&b->raw_d, sizeof(float) * b->total_mean));
   A(host_alloc_mapped(&b->control_out_h, &b->control_out_hd, sizeof(float) * b->total_mean));
   A(host_alloc_mapped(&b->upload_h, &b->upload_hd, sizeof(int) * n));
   A(host_alloc_mapped(&b->stats_h, &b->stats_hd, sizeof(mppi_weight_stats) * n));
   A(host_alloc_mapped(&b->done_h, &b->done_hd, sizeof(uint32_t)));
   A(dev_alloc(&b->sig_d, 2 * sizeof(uint32_t)));
   b->dirty.assign(n, 1);
+  b->ctl_host.assign(n, 1);
+  b->raw_fresh.assign(n, 1);
+  b->prev_baseline.assign(n, 0.0f);
   size_t table_floats = 0;
   for (const Table& t : tables) table_floats += t.count;
   if (table_floats > 0) A(dev_alloc(&b->tables_d, sizeof(float) * table_floats));
@@ -991,11 +994,15 @@ int mppi_batch_compute_control(mppi_batch* b, const float* x0_host, int32_t stri
   if (!b || !x0_host) return fail(MPPI_E_INVALID, "null argument");
   for (int i = 0; i < b->n; ++i)
     if (stride < 0 || stride > b->T[i]) return fail(MPPI_E_INVALID, "optimization_stride out of range");
+  std::memcpy(b->x0_h, x0_host, sizeof(float) * BATCH_MAX_STATE * b->n);
   for (int i = 0; i < b->n; ++i) {
-    b->fe[i].previous_baseline = b->stats[i].baseline;
-    for (int j = 0; j < BATCH_MAX_STATE; ++j) b->x0_h[i * BATCH_MAX_STATE + j] = x0_host[i * BATCH_MAX_STATE + j];
-    // the nominal is uploaded only where the host changed the sequence since the device computed it
-    if (b->dirty[i]) std::copy(b->control[i].begin(), b->control[i].end(), b->nominal_h + b->mean_base[i]);
+    b->prev_baseline[i] = b->stats_h[i].baseline;
+    // the nominal is uploaded only where the device's differs from the host's sequence
+    if (b->dirty[i]) {
+      const size_t base = (size_t)b->mean_base[i], tc = (size_t)b->T[i] * b->C[i];
+      const float* src = b->ctl_host[i] ? b->control[i].data() : b->control_out_h + base;
+      std::memmove(b->nominal_h + base, src, sizeof(float) * tc);
+    }
     b->upload_h[i] = b->dirty[i] ? 1 : 0;
     b->iters_h[i] = IterArgs{b->rng_counter, stride, 0};
   }
@@ -1026,17 +1033,12 @@ int mppi_batch_compute_control(mppi_batch* b, const float* x0_host, int32_t stri
     b->signals = *b->done_h;
     return st;
   }
-  // the combine smoothed and constrained the new means on the device (into the next call's nominal as well)
-  for (int i = 0; i < b->n; ++i) {
-    const size_t base = (size_t)b->mean_base[i], tc = (size_t)b->T[i] * b->C[i];
-    b->stats[i] = b->stats_h[i];
-    free_energy_from_stats(b->stats[i], b->desc[i].lambda, b->K[i], &b->fe[i]);
-    std::memcpy(b->raw_mean[i].data(), b->mean_out_h + base, sizeof(float) * tc);
-    std::memcpy(b->control[i].data(), b->control_out_h + base, sizeof(float) * tc);
-    std::memcpy(b->state_x0[i].data(), b->x0_h + (size_t)i * BATCH_MAX_STATE, sizeof(float) * BATCH_MAX_STATE);
-    b->state_stale[i] = 1;
-    b->dirty[i] = 0;
-  }
+  // the combine smoothed and constrained the new means on the device (into the next call's nominal as well) and
+  // wrote them and the statistics to mapped host memory, where the getters read them
+  std::fill(b->ctl_host.begin(), b->ctl_host.end(), 0);
+  std::fill(b->raw_fresh.begin(), b->raw_fresh.end(), 0);
+  std::fill(b->state_stale.begin(), b->state_stale.end(), 1);
+  std::fill(b->dirty.begin(), b->dirty.end(), 0);
   b->computed = true;
   return MPPI_OK;
 }
@@ -1136,11 +1138,23 @@ static int batch_index(const mppi_batch* b, int32_t i)
   return MPPI_OK;
 }
 
+// The raw mean of instance i's last call, copied from the device on its first request after the call.
+static int batch_raw_mean(const mppi_batch* b, int i)
+{
+  if (b->raw_fresh[i]) return MPPI_OK;
+  MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
+  MPPI_HIP_CHECK(hipMemcpy(b->raw_mean[i].data(), b->raw_d + b->mean_base[i], sizeof(float) * b->raw_mean[i].size(),
+                           hipMemcpyDeviceToHost));
+  b->raw_fresh[i] = 1;
+  return MPPI_OK;
+}
+
 int mppi_batch_get_control_seq(const mppi_batch* b, int32_t i, float* out)
 {
   MPPI_CHECK(batch_index(b, i));
   if (!out) return fail(MPPI_E_INVALID, "null argument");
-  std::copy(b->control[i].begin(), b->control[i].end(), out);
+  const size_t tc = b->control[i].size();
+  std::memcpy(out, b->ctl_host[i] ? b->control[i].data() : b->control_out_h + b->mean_base[i], sizeof(float) * tc);
   return MPPI_OK;
 }
 
@@ -1148,6 +1162,7 @@ int mppi_batch_get_raw_mean(const mppi_batch* b, int32_t i, float* out)
 {
   MPPI_CHECK(batch_index(b, i));
   if (!out) return fail(MPPI_E_INVALID, "null argument");
+  MPPI_CHECK(batch_raw_mean(b, i));
   std::copy(b->raw_mean[i].begin(), b->raw_mean[i].end(), out);
   return MPPI_OK;
 }
@@ -1158,10 +1173,12 @@ int mppi_batch_get_state_seq(const mppi_batch* b, int32_t i, float* out)
   if (!out) return fail(MPPI_E_INVALID, "null argument");
   if (b->state_stale[i]) {
     // computeStateTrajectory of the smoothed, not yet constrained controls (the device smoothing is the host
-    // function's arithmetic)
+    // function's arithmetic) from the call's x0
+    MPPI_CHECK(batch_raw_mean(b, i));
     std::vector<float> u(b->raw_mean[i]);
     smooth_control(u, b->T[i], b->C[i]);
-    propagate(b->desc[i].dynamics, b->state_x0[i].data(), u.data(), b->T[i], b->desc[i].dt, b->state_seq[i].data());
+    propagate(b->desc[i].dynamics, b->x0_h + (size_t)i * BATCH_MAX_STATE, u.data(), b->T[i], b->desc[i].dt,
+              b->state_seq[i].data());
     b->state_stale[i] = 0;
   }
   std::copy(b->state_seq[i].begin(), b->state_seq[i].end(), out);
@@ -1172,7 +1189,7 @@ int mppi_batch_get_weight_stats(const mppi_batch* b, int32_t i, mppi_weight_stat
 {
   MPPI_CHECK(batch_index(b, i));
   if (!out) return fail(MPPI_E_INVALID, "null argument");
-  *out = b->stats[i];
+  *out = b->stats_h[i];
   return MPPI_OK;
 }
 
@@ -1180,7 +1197,10 @@ int mppi_batch_get_free_energy(const mppi_batch* b, int32_t i, mppi_free_energy*
 {
   MPPI_CHECK(batch_index(b, i));
   if (!out) return fail(MPPI_E_INVALID, "null argument");
-  *out = b->fe[i];
+  mppi_free_energy fe{};
+  fe.previous_baseline = b->prev_baseline[i];
+  free_energy_from_stats(b->stats_h[i], b->desc[i].lambda, b->K[i], &fe);
+  *out = fe;
   return MPPI_OK;
 }
 
@@ -1189,6 +1209,7 @@ int mppi_batch_set_control_seq(mppi_batch* b, int32_t i, const float* u)
   MPPI_CHECK(batch_index(b, i));
   if (!u) return fail(MPPI_E_INVALID, "null argument");
   std::copy(u, u + b->control[i].size(), b->control[i].begin());
+  b->ctl_host[i] = 1;
   b->dirty[i] = 1;
   return MPPI_OK;
 }
@@ -1199,6 +1220,9 @@ int mppi_batch_slide_control_seq(mppi_batch* b, int32_t steps)
   if (steps < 0) return fail(MPPI_E_INVALID, "steps must be >= 0");
   // slideControlSequenceHelper (controllers/controller.cuh:609-621) per instance
   for (int k = 0; k < b->n; ++k) {
+    if (!b->ctl_host[k])
+      std::memcpy(b->control[k].data(), b->control_out_h + b->mean_base[k], sizeof(float) * b->control[k].size());
+    b->ctl_host[k] = 1;
     b->dirty[k] = 1;
     const int T = b->T[k], C = b->C[k];
     const float* zero = b->desc[k].dynamics.zero_control;
