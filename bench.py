@@ -357,7 +357,9 @@ def learned_comparison(episodes: int, use_graph: bool = False):
       - plain MPPI's convergence curve at 8192 samples and 1, 2, 3, 5 iterations;
       - the learned proposal at 2048 and at 256 samples (one iteration, fallback above 0.5 %) beside plain MPPI with
         the same samples and one iteration;
-      - the cheapest plain setting within 1 % of the converged cost (8192 x 5), and the learned rate against it."""
+      - the cheapest plain setting within 1 % of the converged cost (8192 x 5), and the learned rate against it;
+      - the same closed loop disturbed every 10 control steps (learned.kick): plain 8192 x 1, 8192 x 5, 256 x 1 and
+        the learned proposal at 256 samples."""
     from mppi_generic_vendor_amd import learned, train_proposal, workloads
 
     if episodes <= 0 or not train_proposal.DEFAULT_OUT.exists():
@@ -380,6 +382,13 @@ def learned_comparison(episodes: int, use_graph: bool = False):
         r = ev(net=net, plain_rollouts=8192, plain_iters=1, proposal_rollouts=P)
         regimes[str(P)] = {"learned": r["learned"], "plain_same_samples": r["plain_small"]}
         curve[f"{P}x1"] = r["plain_small"]
+    # a disturbed closed loop: every 10 control steps the system is kicked (new speed, heading turned, steering reset),
+    # so the warm start no longer fits the state right after each kick
+    kicked = {}
+    for K, iters in ((8192, 1), (8192, 5), (256, 1)):
+        kicked[f"{K}x{iters}"] = ev(plain_rollouts=K, plain_iters=iters, kick_every=10)["plain"]
+    kicked["learned_256"] = ev(net=net, plain_rollouts=8192, plain_iters=1, proposal_rollouts=256,
+                               kick_every=10)["learned"]
     net.close()
     converged = curve["8192x5"]["cost"]
     within = {k: v for k, v in curve.items() if v["cost"] <= converged * 1.01}
@@ -395,7 +404,8 @@ def learned_comparison(episodes: int, use_graph: bool = False):
             "learned_256_cost_gain_over_same_samples":
                 round(1 - l256["cost"] / regimes["256"]["plain_same_samples"]["cost"], 6),
             "learned_256_rate_vs_plain_8192x3_host": round(l256["hz_host"] / curve["8192x3"]["hz_host"], 4),
-            "learned_256_rate_vs_cheapest_within_1pct_host": round(l256["hz_host"] / cheapest[1]["hz_host"], 4)}
+            "learned_256_rate_vs_cheapest_within_1pct_host": round(l256["hz_host"] / cheapest[1]["hz_host"], 4),
+            "disturbed_every_10_steps": kicked}
 
 
 # -------------------------------------------------------------------------------------- two-system (Tube / RMPPI)

@@ -38,7 +38,19 @@ struct mppi_tube_controller {
   float* goal_d = nullptr;
   mppi_weight_stats* stats_d = nullptr;  // [2]
   void* ws_d = nullptr;
-  float* host_io = nullptr;  // pinned: x0 [2][MAX_STATE], means [2][T][C], outs [2][T][C], stats
+  // mapped pinned exchange buffers (the *_hd members are their device-side addresses): x0 [2][MAX_STATE], means
+  // [2][T][C], outs [2][T][C], stats [2]; for RMPPI's candidate evaluation candidates [N][S], strides [N], costs [N][M]
+  float* host_io = nullptr;
+  float* host_io_hd = nullptr;
+  float* isc_io = nullptr;
+  float* isc_io_hd = nullptr;
+  bool gains_dirty = false;  // the host gains changed since they were uploaded
+  // completion signal of a call (device workgroup count and epoch, mapped host word), signals enqueued, fallback event
+  uint32_t* sig_d = nullptr;
+  uint32_t* done_h = nullptr;
+  uint32_t* done_hd = nullptr;
+  uint32_t signals = 0;
+  hipEvent_t ev = nullptr;
   std::vector<float> control[2];  // system d's control sequence (Tube: 0 actual, 1 nominal; RMPPI: 0 nominal, 1 real)
   std::vector<float> states[2];   // system d's state trajectory
   std::vector<float> gains;       // host copy [T][S][C] (empty: none)
@@ -73,16 +85,38 @@ int tube_free(mppi_tube_controller* c)
   if (!c) return MPPI_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* devs[] = {c->x0_d, c->mean_d, c->out_d, c->samples_d, c->costs_d, c->w_d, c->gains_d, c->map_d, c->nn_d,
-                  c->std_t_d, c->goal_d, c->stats_d, c->ws_d, c->cand_d, c->stride_d, c->eval_d};
+                  c->std_t_d, c->goal_d, c->stats_d, c->ws_d, c->cand_d, c->stride_d, c->eval_d, c->sig_d};
   for (void* p : devs)
     if (p) (void)hipFree(p);
-  if (c->host_io) (void)hipHostFree(c->host_io);
+  for (void* p : {(void*)c->host_io, (void*)c->isc_io, (void*)c->done_h})
+    if (p) (void)hipHostFree(p);
+  if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MPPI_OK;
 }
 
 size_t io_floats(int T, int C) { return 2 * MPPI_MAX_STATE + 4 * (size_t)T * C; }
+
+// The completion signal of the call's last kernel.
+HostSignal tube_signal(const mppi_tube_controller* c)
+{
+  HostSignal h;
+  h.count = c->sig_d;
+  h.epoch = c->sig_d + 1;
+  h.flag = c->done_hd;
+  return h;
+}
+
+// Waits for the signal of the work just enqueued (given by its last kernel).
+int tube_wait(mppi_tube_controller* c)
+{
+  const uint32_t want = ++c->signals;
+  MPPI_HIP_CHECK(hipEventRecord(c->ev, c->stream));
+  const int st = wait_host_signal(c->done_h, want, c->ev);
+  if (st != MPPI_OK) c->signals = *c->done_h;
+  return st;
+}
 
 int tube_setup(mppi_tube_controller* c, const mppi_tube_desc* desc)
 {
@@ -117,7 +151,12 @@ int tube_setup(mppi_tube_controller* c, const mppi_tube_desc* desc)
   MPPI_CHECK(dev_alloc(&c->ws_d, weighted_update_workspace_bytes(K, T, C)));
   MPPI_CHECK(dev_alloc(&c->gains_d, sizeof(float) * (size_t)T * S * C));
   MPPI_HIP_CHECK(hipMemset(c->gains_d, 0, sizeof(float) * (size_t)T * S * C));
-  MPPI_CHECK(host_alloc(&c->host_io, sizeof(float) * io_floats(T, C) + 2 * sizeof(mppi_weight_stats)));
+  MPPI_CHECK(host_alloc_mapped(&c->host_io, &c->host_io_hd,
+                               sizeof(float) * io_floats(T, C) + 2 * sizeof(mppi_weight_stats)));
+  MPPI_CHECK(host_alloc_mapped(&c->done_h, &c->done_hd, sizeof(uint32_t)));
+  MPPI_CHECK(dev_alloc(&c->sig_d, 2 * sizeof(uint32_t)));
+  MPPI_HIP_CHECK(hipMemset(c->sig_d, 0, 2 * sizeof(uint32_t)));
+  MPPI_HIP_CHECK(hipEventCreateWithFlags(&c->ev, hipEventDisableTiming));
   if (b.cost_map_host != nullptr) {
     const size_t n = (size_t)b.cost.map_width * b.cost.map_height;
     if (n < 4) return fail(MPPI_E_INVALID, "cost map: width/height must be >= 2");
@@ -177,6 +216,8 @@ int tube_setup(mppi_tube_controller* c, const mppi_tube_desc* desc)
       c->lsw[1 * c->N + h + i] = 1 - i / float(h);
       c->lsw[2 * c->N + h + i] = i / float(h);
     }
+    MPPI_CHECK(host_alloc_mapped(&c->isc_io, &c->isc_io_hd,
+                                 sizeof(float) * ((size_t)c->N * S + (size_t)c->N + (size_t)c->N * c->M)));
     c->cand_h.assign((size_t)c->N * S, 0.0f);
     c->strides_h.assign(c->N, 0);
     c->eval_h.assign((size_t)c->N * c->M, 0.0f);
@@ -200,6 +241,10 @@ int tube_iteration(mppi_tube_controller* c, const float* x0_host, int stride, in
   float* mean_h = x0_h + 2 * MPPI_MAX_STATE;
   float* out_h = mean_h + 2 * TC;
   mppi_weight_stats* st_h = reinterpret_cast<mppi_weight_stats*>(c->host_io + io_floats(T, C));
+  float* x0_hd = c->host_io_hd;
+  float* mean_hd = x0_hd + 2 * MPPI_MAX_STATE;
+  float* out_hd = mean_hd + 2 * TC;
+  mppi_weight_stats* st_hd = reinterpret_cast<mppi_weight_stats*>(c->host_io_hd + io_floats(T, C));
   const int nom = c->nom(), act = 1 - nom;
   // initial states: the measured state for the actual / real system, the nominal state for the nominal one
   for (int i = 0; i < S; ++i) {
@@ -211,11 +256,14 @@ int tube_iteration(mppi_tube_controller* c, const float* x0_host, int stride, in
     const std::vector<float>& m = c->rmppi() ? c->control[nom] : c->control[d];
     std::copy(m.begin(), m.end(), mean_h + d * TC);
   }
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->x0_d, x0_h, sizeof(float) * 2 * S, hipMemcpyHostToDevice, c->stream));
-  MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, mean_h, sizeof(float) * 2 * TC, hipMemcpyHostToDevice, c->stream));
-  if (c->rmppi() && !c->gains.empty())
+  // the states and means from the mapped buffers by one copy kernel; the gains (pageable) only when they changed
+  const WordSpan in[] = {{x0_hd, c->x0_d, 2 * S}, {mean_hd, c->mean_d, (int)(2 * TC)}};
+  MPPI_CHECK(launch_copy_words(in, 2, c->stream));
+  if (c->rmppi() && !c->gains.empty() && c->gains_dirty) {
     MPPI_HIP_CHECK(hipMemcpyAsync(c->gains_d, c->gains.data(), sizeof(float) * c->gains.size(),
                                   hipMemcpyHostToDevice, c->stream));
+    c->gains_dirty = false;
+  }
   const IterRef ref{nullptr, {c->rng_counter, stride, iter}};
   for (int d = 0; d < 2; ++d) {
     float* sd = c->samples_d + (size_t)d * K * TC;
@@ -237,9 +285,11 @@ int tube_iteration(mppi_tube_controller* c, const float* x0_host, int stride, in
     MPPI_CHECK(launch_weighted_update(c->w_d + (size_t)d * K, c->stats_d + d, c->samples_d + (size_t)d * K * TC, K, T,
                                       C, c->out_d + d * TC, c->ws_d, c->stream));
   }
-  MPPI_HIP_CHECK(hipMemcpyAsync(out_h, c->out_d, sizeof(float) * 2 * TC, hipMemcpyDeviceToHost, c->stream));
-  MPPI_HIP_CHECK(hipMemcpyAsync(st_h, c->stats_d, sizeof(mppi_weight_stats) * 2, hipMemcpyDeviceToHost, c->stream));
-  MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  // both new means and statistics to the mapped buffers, then the completion signal
+  const WordSpan res[] = {{c->out_d, out_hd, (int)(2 * TC)}, {c->stats_d, st_hd, words_of<mppi_weight_stats>(2)}};
+  const HostSignal sig = tube_signal(c);
+  MPPI_CHECK(launch_copy_words(res, 2, c->stream, &sig));
+  MPPI_CHECK(tube_wait(c));
   c->rng_counter++;
   c->have_samples = true;
   for (int d = 0; d < 2; ++d) {
@@ -427,6 +477,7 @@ int mppi_tube_set_feedback_gains(mppi_tube_controller* c, const float* gains)
   for (size_t i = 0; i < n; ++i)
     if (!std::isfinite(gains[i])) return fail(MPPI_E_INVALID, "feedback gains must be finite");
   c->gains.assign(gains, gains + n);
+  c->gains_dirty = true;
   return MPPI_OK;
 }
 
@@ -477,13 +528,19 @@ int mppi_tube_update_importance_sampling_control(mppi_tube_controller* c, const 
       // computeImportanceSamplerStride (:501-511): round((0, s, s) . w), half away from zero
       c->strides_h[i] = (int)std::round(0.0f * w0 + (float)stride * w1 + (float)stride * w2);
     }
+    // candidates, strides and the nominal control through the mapped buffers
     float* mean_h = c->host_io + 2 * MPPI_MAX_STATE;
+    float* cand_io = c->isc_io;
+    int* stride_io = reinterpret_cast<int*>(cand_io + (size_t)N * S);
+    float* eval_io = cand_io + (size_t)N * S + N;
+    float* cand_iod = c->isc_io_hd;
     std::copy(c->control[nom].begin(), c->control[nom].end(), mean_h);
-    MPPI_HIP_CHECK(hipMemcpyAsync(c->cand_d, c->cand_h.data(), sizeof(float) * (size_t)N * S, hipMemcpyHostToDevice,
-                                  c->stream));
-    MPPI_HIP_CHECK(hipMemcpyAsync(c->stride_d, c->strides_h.data(), sizeof(int) * N, hipMemcpyHostToDevice,
-                                  c->stream));
-    MPPI_HIP_CHECK(hipMemcpyAsync(c->mean_d, mean_h, sizeof(float) * TC, hipMemcpyHostToDevice, c->stream));
+    std::copy(c->cand_h.begin(), c->cand_h.end(), cand_io);
+    std::copy(c->strides_h.begin(), c->strides_h.end(), stride_io);
+    const WordSpan in[] = {{cand_iod, c->cand_d, N * S},
+                           {cand_iod + (size_t)N * S, c->stride_d, N},
+                           {c->host_io_hd + 2 * MPPI_MAX_STATE, c->mean_d, (int)TC}};
+    MPPI_CHECK(launch_copy_words(in, 3, c->stream));
     // generateSamples(stride, 0) around the nominal control (:593-597)
     const IterRef ref{nullptr, {c->rng_counter, stride, 0}};
     if (b.sampler.kind == MPPI_SAMPLER_COLORED)
@@ -492,9 +549,11 @@ int mppi_tube_update_importance_sampling_control(mppi_tube_controller* c, const 
       MPPI_CHECK(launch_sample_gaussian(b.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, c->stream));
     MPPI_CHECK(launch_candidate_eval(b.dynamics, b.cost, b.sampler, N, M, K, T, b.dt, b.lambda, b.alpha, c->cand_d,
                                      c->stride_d, c->mean_d, c->samples_d, c->eval_d, c->stream));
-    MPPI_HIP_CHECK(hipMemcpyAsync(c->eval_h.data(), c->eval_d, sizeof(float) * (size_t)N * M, hipMemcpyDeviceToHost,
-                                  c->stream));
-    MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+    const WordSpan res[] = {{c->eval_d, cand_iod + (size_t)N * S + N, N * M}};
+    const HostSignal sig = tube_signal(c);
+    MPPI_CHECK(launch_copy_words(res, 1, c->stream, &sig));
+    MPPI_CHECK(tube_wait(c));
+    std::copy(eval_io, eval_io + (size_t)N * M, c->eval_h.begin());
     c->rng_counter++;
     // computeCandidateBaseline + computeBestIndex (:513-546), in the reference's arithmetic
     float base = c->eval_h[0];
