@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "fused.hpp"
 #include "host.hpp"
 #include "kernels.hpp"
 #include "models.hpp"
@@ -45,6 +46,10 @@ struct mppi_tube_controller {
   float* isc_io = nullptr;
   float* isc_io_hd = nullptr;
   bool gains_dirty = false;  // the host gains changed since they were uploaded
+  // fused iteration (Gaussian samples): per system, the workgroups' Partials [2][gstride] and rows [2][gstride][T C]
+  Partial* parts_d = nullptr;
+  float* psums_d = nullptr;
+  int gstride = 0;
   // completion signal of a call (device workgroup count and epoch, mapped host word), signals enqueued, fallback event
   uint32_t* sig_d = nullptr;
   uint32_t* done_h = nullptr;
@@ -85,7 +90,8 @@ int tube_free(mppi_tube_controller* c)
   if (!c) return MPPI_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* devs[] = {c->x0_d, c->mean_d, c->out_d, c->samples_d, c->costs_d, c->w_d, c->gains_d, c->map_d, c->nn_d,
-                  c->std_t_d, c->goal_d, c->stats_d, c->ws_d, c->cand_d, c->stride_d, c->eval_d, c->sig_d};
+                  c->std_t_d, c->goal_d, c->stats_d, c->ws_d, c->cand_d, c->stride_d, c->eval_d, c->sig_d,
+                  c->parts_d, c->psums_d};
   for (void* p : devs)
     if (p) (void)hipFree(p);
   for (void* p : {(void*)c->host_io, (void*)c->isc_io, (void*)c->done_h})
@@ -108,10 +114,11 @@ HostSignal tube_signal(const mppi_tube_controller* c)
   return h;
 }
 
-// Waits for the signal of the work just enqueued (given by its last kernel).
-int tube_wait(mppi_tube_controller* c)
+// Waits for the last of the `n` signals of the work just enqueued.
+int tube_wait(mppi_tube_controller* c, uint32_t n = 1)
 {
-  const uint32_t want = ++c->signals;
+  c->signals += n;
+  const uint32_t want = c->signals;
   MPPI_HIP_CHECK(hipEventRecord(c->ev, c->stream));
   const int st = wait_host_signal(c->done_h, want, c->ev);
   if (st != MPPI_OK) c->signals = *c->done_h;
@@ -149,6 +156,10 @@ int tube_setup(mppi_tube_controller* c, const mppi_tube_desc* desc)
   MPPI_CHECK(dev_alloc(&c->w_d, sizeof(float) * 2 * K));
   MPPI_CHECK(dev_alloc(&c->stats_d, sizeof(mppi_weight_stats) * 2));
   MPPI_CHECK(dev_alloc(&c->ws_d, weighted_update_workspace_bytes(K, T, C)));
+  // partials of the fused iteration: at most ceil(K / 16) workgroups per system
+  c->gstride = (K + 15) / 16;
+  MPPI_CHECK(dev_alloc(&c->parts_d, sizeof(Partial) * 2 * (size_t)c->gstride));
+  MPPI_CHECK(dev_alloc(&c->psums_d, sizeof(float) * 2 * (size_t)c->gstride * T * C));
   MPPI_CHECK(dev_alloc(&c->gains_d, sizeof(float) * (size_t)T * S * C));
   MPPI_HIP_CHECK(hipMemset(c->gains_d, 0, sizeof(float) * (size_t)T * S * C));
   MPPI_CHECK(host_alloc_mapped(&c->host_io, &c->host_io_hd,
@@ -265,31 +276,50 @@ int tube_iteration(mppi_tube_controller* c, const float* x0_host, int stride, in
     c->gains_dirty = false;
   }
   const IterRef ref{nullptr, {c->rng_counter, stride, iter}};
-  for (int d = 0; d < 2; ++d) {
-    float* sd = c->samples_d + (size_t)d * K * TC;
-    if (b.sampler.kind == MPPI_SAMPLER_COLORED)
-      MPPI_CHECK(launch_sample_colored(b.sampler, K, T, C, c->mean_d + d * TC, sd, ref, nullptr, c->stream));
-    else
-      MPPI_CHECK(launch_sample_gaussian(b.sampler, K, T, C, c->mean_d + d * TC, sd, ref, nullptr, c->stream));
-  }
   mppi_two_system_params tp{};
   tp.mode = c->d.mode;
   tp.nominal_index = nom;
   tp.value_function_threshold = c->d.value_function_threshold;
   tp.feedback_gains = (c->rmppi() && !c->gains.empty()) ? c->gains_d : nullptr;
-  MPPI_CHECK(launch_two_system(b.dynamics, b.cost, b.sampler, b.sampler, tp, K, T, b.dt, b.lambda, b.alpha, c->x0_d,
-                               c->mean_d, c->samples_d, c->costs_d, c->stream));
-  for (int d = 0; d < 2; ++d) {
-    MPPI_CHECK(launch_weights(c->costs_d + (size_t)d * K, c->w_d + (size_t)d * K, K, b.lambda, 0.0f, 0.0f,
-                              c->stats_d + d, nullptr, nullptr, 0, c->stream));
-    MPPI_CHECK(launch_weighted_update(c->w_d + (size_t)d * K, c->stats_d + d, c->samples_d + (size_t)d * K * TC, K, T,
-                                      C, c->out_d + d * TC, c->ws_d, c->stream));
+  if (b.sampler.kind == MPPI_SAMPLER_GAUSSIAN) {
+    // the fused iteration: both systems' samples drawn in the rollout kernel, per-workgroup partials, and one combine
+    // per system writing its new mean and statistics to the mapped buffers (each combine signals; the second one's
+    // epoch ends the call)
+    int G = 0;
+    MPPI_CHECK(launch_two_system_fused(b.dynamics, b.cost, b.sampler, tp, K, T, b.dt, b.lambda, b.alpha, c->x0_d,
+                                       c->mean_d, c->samples_d, c->costs_d, ref, c->parts_d, c->psums_d, c->gstride,
+                                       &G, c->stream));
+    for (int d = 0; d < 2; ++d) {
+      ResultCopies rc;
+      rc.mean = out_hd + d * TC;
+      rc.stats = st_hd + d;
+      rc.signal = tube_signal(c);
+      MPPI_CHECK(launch_combine(c->parts_d + (size_t)d * c->gstride, c->psums_d + (size_t)d * c->gstride * TC, G, K,
+                                (int)TC, b.lambda, c->stats_d + d, c->out_d + d * TC, nullptr, 1, &rc, c->stream));
+    }
+    MPPI_CHECK(tube_wait(c, 2));
+  } else {
+    for (int d = 0; d < 2; ++d) {
+      float* sd = c->samples_d + (size_t)d * K * TC;
+      if (b.sampler.kind == MPPI_SAMPLER_COLORED)
+        MPPI_CHECK(launch_sample_colored(b.sampler, K, T, C, c->mean_d + d * TC, sd, ref, nullptr, c->stream));
+      else
+        MPPI_CHECK(launch_sample_gaussian(b.sampler, K, T, C, c->mean_d + d * TC, sd, ref, nullptr, c->stream));
+    }
+    MPPI_CHECK(launch_two_system(b.dynamics, b.cost, b.sampler, b.sampler, tp, K, T, b.dt, b.lambda, b.alpha, c->x0_d,
+                                 c->mean_d, c->samples_d, c->costs_d, c->stream));
+    for (int d = 0; d < 2; ++d) {
+      MPPI_CHECK(launch_weights(c->costs_d + (size_t)d * K, c->w_d + (size_t)d * K, K, b.lambda, 0.0f, 0.0f,
+                                c->stats_d + d, nullptr, nullptr, 0, c->stream));
+      MPPI_CHECK(launch_weighted_update(c->w_d + (size_t)d * K, c->stats_d + d, c->samples_d + (size_t)d * K * TC, K, T,
+                                        C, c->out_d + d * TC, c->ws_d, c->stream));
+    }
+    // both new means and statistics to the mapped buffers, then the completion signal
+    const WordSpan res[] = {{c->out_d, out_hd, (int)(2 * TC)}, {c->stats_d, st_hd, words_of<mppi_weight_stats>(2)}};
+    const HostSignal sig = tube_signal(c);
+    MPPI_CHECK(launch_copy_words(res, 2, c->stream, &sig));
+    MPPI_CHECK(tube_wait(c));
   }
-  // both new means and statistics to the mapped buffers, then the completion signal
-  const WordSpan res[] = {{c->out_d, out_hd, (int)(2 * TC)}, {c->stats_d, st_hd, words_of<mppi_weight_stats>(2)}};
-  const HostSignal sig = tube_signal(c);
-  MPPI_CHECK(launch_copy_words(res, 2, c->stream, &sig));
-  MPPI_CHECK(tube_wait(c));
   c->rng_counter++;
   c->have_samples = true;
   for (int d = 0; d < 2; ++d) {

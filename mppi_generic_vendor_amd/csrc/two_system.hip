@@ -16,6 +16,12 @@
 //     the sample buffer with 16-byte stores.
 // Per sample-step the kernel reads and writes 2 C floats of each system (16 C bytes) -- the HBM traffic of the
 // reference's two rollouts -- and its VALU work is two dynamics steps and two cost evaluations.
+//
+// The fused form (FUSED, Gaussian samples) draws both tiles in LDS from the same normals around each system's mean
+// (mppi_sample_gaussian's numbering and transform with distribution 0's parameters, as the two sampler launches it
+// replaces) and ends like the single-system fused iteration: per system, the tile's local baseline, weights and FP64
+// sums and the weighted column sums of the applied controls as a Partial and a partial row, which combine_kernel
+// merges into the new mean. The applied controls and the costs are still written out.
 #include <algorithm>
 
 #include "iteration_core.hpp"
@@ -46,7 +52,61 @@ static size_t ts_lds_floats(int T, int R, int chunk, bool rmppi)
   const size_t gains = rmppi ? (((size_t)T * S * C + 3) & ~(size_t)3) : 0;
   const size_t outs = 2 * (size_t)chunk * 64 * 2 * (O | 1);
   const size_t fbs = rmppi ? 2 * (size_t)chunk * 64 * C : 0;
-  return tiles + means + gains + outs + fbs + (size_t)TS_ACC * TS_CONSUMERS * 64 + 2 * 64;
+  return tiles + means + gains + outs + fbs + (size_t)TS_ACC * TS_CONSUMERS * 64 + 2 * 64 + 3 * 64;
+}
+
+// Both systems' sample rows k0 .. k0 + nrows - 1 from the same normals, around each system's mean mu_d (LDS, [TC]):
+// tile_d[r ROW + col] = mu_t if k == 0 or t < stride, s_tc eps if k >= pstart, mu_t + s_tc eps otherwise -- the
+// transform of gaussian_tile_fill (unconstrained: the producer constrains after the feedback), evaluated once per
+// normal for both tiles.
+template <int C, int NT>
+__device__ __forceinline__ void two_tile_fill(const mppi_sampler_params& sp, const IterArgs a, int k0, int nrows,
+                                              int TC, int ROW, int pstart, const float* mu0, const float* mu1,
+                                              float* tile0, float* tile1)
+{
+  const int tid = threadIdx.x;
+  const float decay = powf(sp.std_dev_decay, (float)a.iteration);
+  const int64_t base = ((int64_t)k0 * TC) >> 2;
+  const int n = nrows * TC;
+  for (int e0 = tid * 4; e0 < n; e0 += NT * 4) {
+    const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(base + (e0 >> 2)));
+    int r = e0 / TC;
+    int col = e0 - r * TC;
+    int t = col / C;
+    int c = col - t * C;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (e0 + j < n) {
+        const int k = k0 + r;
+        const float sd = decay * step_std_dev(sp, t, c, C);
+        const float m0 = mu0[col], m1 = mu1[col];
+        float v0, v1;
+        if (k == 0 || t < a.stride) {
+          v0 = m0;
+          v1 = m1;
+        } else if (k >= pstart) {
+          v0 = sd * z.z[j];
+          v1 = v0;
+        } else {
+          v0 = m0 + sd * z.z[j];
+          v1 = m1 + sd * z.z[j];
+        }
+        tile0[r * ROW + col] = v0;
+        tile1[r * ROW + col] = v1;
+      }
+      ++col;
+      if (++c == C) {
+        c = 0;
+        ++t;
+      }
+      if (col == TC) {
+        col = 0;
+        t = 0;
+        c = 0;
+        ++r;
+      }
+    }
+  }
 }
 
 // computeFeedbackCost (sampling_distributions/gaussian/gaussian.cu:577-630): 0.5 lambda (1 - alpha) sum_c coeff_c
@@ -65,12 +125,12 @@ MPPI_HD float feedback_cost(const mppi_sampler_params& s, const float* fb, float
 
 // NOM = -1: Tube (two independent systems); NOM = 0 or 1: RMPPI with system NOM nominal and 1 - NOM real (compile-time,
 // so that the two state chains stay in registers).
-template <int MODEL, int COST, int NOM>
+template <int MODEL, int COST, int NOM, bool FUSED>
 __global__ void __launch_bounds__(TS_THREADS) two_system_rollout_kernel(
     mppi_dynamics_params dyn, mppi_cost_params cost, mppi_sampler_params sp0, mppi_sampler_params sp1,
     TwoSystemArgs ta, int K, int T, float dt, float lambda, float alpha, const float* __restrict__ x0,
     const float* __restrict__ mean, float* __restrict__ samples, float* __restrict__ costs, int pstart, int R,
-    int chunk)
+    int chunk, IterRef itr, Partial* __restrict__ parts, float* __restrict__ psums, int gstride)
 {
   using M = BuiltinModel<MODEL>;
   using CF = BuiltinCost<COST, M::O>;
@@ -89,15 +149,24 @@ __global__ void __launch_bounds__(TS_THREADS) two_system_rollout_kernel(
   float* fbuf = ybuf + 2 * chunk * 64 * 2 * OP;                  // [2][chunk][64][C]
   float* part = fbuf + (RMPPI ? 2 * chunk * 64 * C : 0);         // [TS_ACC][15][64]
   float* term = part + TS_ACC * TS_CONSUMERS * 64;               // [2][64]
+  float* cl = term + 2 * 64;                                     // [2][64] costs (FUSED)
+  float* wl = cl + 2 * 64;                                       // [64] local weights (FUSED)
 
-  for (int d = 0; d < 2; ++d)
-    tile_load<TS_THREADS>(samples + ((size_t)d * K + k0) * TC, tile + d * R * ROW, nrows, TC, ROW, tid);
+  if constexpr (!FUSED) {
+    for (int d = 0; d < 2; ++d)
+      tile_load<TS_THREADS>(samples + ((size_t)d * K + k0) * TC, tile + d * R * ROW, nrows, TC, ROW, tid);
+  }
   for (int j = tid; j < 2 * TC; j += TS_THREADS) {
     const int d = j >= TC ? 1 : 0;
     mu[d * TC4 + (j - d * TC)] = mean[j];
   }
   if constexpr (RMPPI) {
     for (int j = tid; j < T * S * C; j += TS_THREADS) gl[j] = ta.gains ? ta.gains[j] : 0.0f;
+  }
+  if constexpr (FUSED) {
+    __syncthreads();
+    two_tile_fill<C, TS_THREADS>(sp0, load_iter_args(itr), k0, nrows, TC, ROW, pstart, mu, mu + TC4, tile,
+                                 tile + R * ROW);
   }
   bool deadband = false;
 #pragma unroll
@@ -214,8 +283,10 @@ __global__ void __launch_bounds__(TS_THREADS) two_system_rollout_kernel(
 #pragma unroll
     for (int q = 0; q < TS_ACC; ++q) part[(q * TS_CONSUMERS + wave - 1) * 64 + lane] = acc[q];
   }
-  for (int d = 0; d < 2; ++d)
-    tile_store<TS_THREADS>(samples + ((size_t)d * K + k0) * TC, tile + d * R * ROW, nrows, TC, ROW, tid);
+  if constexpr (!FUSED) {
+    for (int d = 0; d < 2; ++d)
+      tile_store<TS_THREADS>(samples + ((size_t)d * K + k0) * TC, tile + d * R * ROW, nrows, TC, ROW, tid);
+  }
   __syncthreads();
   if (wave == 0) {
     float s[TS_ACC];
@@ -240,20 +311,35 @@ __global__ void __launch_bounds__(TS_THREADS) two_system_rollout_kernel(
       c0 = real == 0 ? cost_r : cost_n;
       c1 = real == 0 ? cost_n : cost_r;
     }
-    if (lane < nrows) {
+    if constexpr (FUSED) {
+      cl[lane] = c0;
+      cl[64 + lane] = c1;
+    } else if (lane < nrows) {
       costs[k0 + lane] = c0;
       costs[K + k0 + lane] = c1;
+    }
+  }
+  if constexpr (FUSED) {
+    // per system: costs and applied controls out, the tile's partial of the weighted update
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      tile_partials<TS_THREADS, true, false>(cl + 64 * d, cl + 64 * d, wl, tile + d * R * ROW, ROW, TC, k0, nrows,
+                                             blockIdx.x, lambda, costs + (size_t)d * K, parts + (size_t)d * gstride,
+                                             psums + (size_t)d * gstride * TC, samples + (size_t)d * K * TC);
+      __syncthreads();
     }
   }
 }
 
 // Samples per workgroup: the smallest of 64, 32, 16 whose grid still fits the CUs once (a small K spreads over more
 // CUs), then halved until the tiles and buffers fit the LDS; the chunk is the largest of 16, 8, 4 that fits.
-template <int MODEL, int COST, int NOM>
+template <int MODEL, int COST, int NOM, bool FUSED>
 static int launch_two_system_t(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                                const mppi_sampler_params& sp0, const mppi_sampler_params& sp1, const TwoSystemArgs& ta,
                                int K, int T, float dt, float lambda, float alpha, const float* x0, const float* mean,
-                               float* samples, float* costs, hipStream_t s)
+                               float* samples, float* costs, hipStream_t s, IterRef itr = IterRef{nullptr, {0, 0, 0}},
+                               Partial* parts = nullptr, float* psums = nullptr, int gstride = 0, int* G_out = nullptr)
 {
   using M = BuiltinModel<MODEL>;
   constexpr bool RMPPI = NOM >= 0;
@@ -270,11 +356,14 @@ static int launch_two_system_t(const mppi_dynamics_params& dyn, const mppi_cost_
   }
   if (chunk == 0) return fail(MPPI_E_SIZE, "two-system rollout: horizon too long for the LDS control tiles");
   const size_t lds = sizeof(float) * ts_lds_floats<M::S, M::C, M::O>(T, R, chunk, RMPPI);
-  MPPI_CHECK(allow_max_dynamic_lds((const void*)two_system_rollout_kernel<MODEL, COST, NOM>));
+  MPPI_CHECK(allow_max_dynamic_lds((const void*)two_system_rollout_kernel<MODEL, COST, NOM, FUSED>));
   const int pstart = pure_noise_start(K, sp0.pure_noise_trajectories_percentage);
   const int grid = (K + R - 1) / R;
-  two_system_rollout_kernel<MODEL, COST, NOM><<<grid, TS_THREADS, lds, s>>>(
-      dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean, samples, costs, pstart, R, chunk);
+  if (FUSED && grid > gstride) return fail(MPPI_E_SIZE, "two-system iteration: partial workspace too small");
+  if (G_out != nullptr) *G_out = grid;
+  two_system_rollout_kernel<MODEL, COST, NOM, FUSED><<<grid, TS_THREADS, lds, s>>>(
+      dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean, samples, costs, pstart, R, chunk, itr, parts, psums,
+      gstride);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }
@@ -381,13 +470,13 @@ int launch_two_system(const mppi_dynamics_params& dyn, const mppi_cost_params& c
 #define MPPI_TS_CASE(MODEL, COST)                                                                                   \
   if (dyn.model == MODEL && cost.cost == COST) {                                                                   \
     if (nom < 0)                                                                                                   \
-      return launch_two_system_t<MODEL, COST, -1>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean,      \
-                                                  samples, costs, s);                                              \
+      return launch_two_system_t<MODEL, COST, -1, false>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0,     \
+                                                         mean, samples, costs, s);                                 \
     if (nom == 0)                                                                                                  \
-      return launch_two_system_t<MODEL, COST, 0>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean,       \
-                                                 samples, costs, s);                                               \
-    return launch_two_system_t<MODEL, COST, 1>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean,         \
-                                               samples, costs, s);                                                 \
+      return launch_two_system_t<MODEL, COST, 0, false>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0,      \
+                                                        mean, samples, costs, s);                                  \
+    return launch_two_system_t<MODEL, COST, 1, false>(dyn, cost, sp0, sp1, ta, K, T, dt, lambda, alpha, x0, mean,  \
+                                                      samples, costs, s);                                          \
   }
   MPPI_TS_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
   MPPI_TS_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
@@ -395,6 +484,40 @@ int launch_two_system(const mppi_dynamics_params& dyn, const mppi_cost_params& c
   MPPI_TS_CASE(MPPI_MODEL_RACER_DUBINS_NN, MPPI_COST_QUADRATIC_MAP)
 #undef MPPI_TS_CASE
   return fail(MPPI_E_UNSUPPORTED, "two-system rollout: model/cost combination is not compiled into the kernel library");
+}
+
+int launch_two_system_fused(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
+                            const mppi_sampler_params& sp, const mppi_two_system_params& tp, int K, int T, float dt,
+                            float lambda, float alpha, const float* x0, const float* mean, float* samples,
+                            float* costs, IterRef itr, Partial* parts, float* psums, int gstride, int* G,
+                            hipStream_t s)
+{
+  if (tp.mode != MPPI_TWO_SYSTEM_TUBE && tp.mode != MPPI_TWO_SYSTEM_RMPPI)
+    return fail(MPPI_E_INVALID, "two-system iteration: mode must be MPPI_TWO_SYSTEM_TUBE or MPPI_TWO_SYSTEM_RMPPI");
+  if (tp.nominal_index != 0 && tp.nominal_index != 1)
+    return fail(MPPI_E_INVALID, "two-system iteration: nominal_index must be 0 or 1");
+  if (sp.kind != MPPI_SAMPLER_GAUSSIAN) return fail(MPPI_E_INVALID, "two-system iteration: Gaussian samples only");
+  if (dyn.model == MPPI_MODEL_RACER_DUBINS_NN && dyn.nn_precision == MPPI_NN_BF16)
+    return fail(MPPI_E_UNSUPPORTED, "two-system iteration: the BF16 network model is not supported (use MPPI_NN_FP32)");
+  const TwoSystemArgs ta{tp.value_function_threshold, tp.feedback_gains};
+  const int nom = tp.mode == MPPI_TWO_SYSTEM_RMPPI ? tp.nominal_index : -1;
+#define MPPI_TSF_CASE(MODEL, COST)                                                                                  \
+  if (dyn.model == MODEL && cost.cost == COST) {                                                                   \
+    if (nom < 0)                                                                                                   \
+      return launch_two_system_t<MODEL, COST, -1, true>(dyn, cost, sp, sp, ta, K, T, dt, lambda, alpha, x0, mean,  \
+                                                        samples, costs, s, itr, parts, psums, gstride, G);         \
+    if (nom == 0)                                                                                                  \
+      return launch_two_system_t<MODEL, COST, 0, true>(dyn, cost, sp, sp, ta, K, T, dt, lambda, alpha, x0, mean,   \
+                                                       samples, costs, s, itr, parts, psums, gstride, G);          \
+    return launch_two_system_t<MODEL, COST, 1, true>(dyn, cost, sp, sp, ta, K, T, dt, lambda, alpha, x0, mean,     \
+                                                     samples, costs, s, itr, parts, psums, gstride, G);            \
+  }
+  MPPI_TSF_CASE(MPPI_MODEL_CARTPOLE, MPPI_COST_CARTPOLE_QUADRATIC)
+  MPPI_TSF_CASE(MPPI_MODEL_DOUBLE_INTEGRATOR, MPPI_COST_QUADRATIC_MAP)
+  MPPI_TSF_CASE(MPPI_MODEL_RACER_DUBINS, MPPI_COST_QUADRATIC_MAP)
+  MPPI_TSF_CASE(MPPI_MODEL_RACER_DUBINS_NN, MPPI_COST_QUADRATIC_MAP)
+#undef MPPI_TSF_CASE
+  return fail(MPPI_E_UNSUPPORTED, "two-system iteration: model/cost combination is not compiled into the library");
 }
 
 }  // namespace mppi_amd
