@@ -425,9 +425,12 @@ def pd_gains(T: int, kp: float = 4.0, kd: float = 2.0) -> np.ndarray:
 
 def run_two_system(args, host_calls: int, cpu_seconds: float):
     """Tube-MPPI and RMPPI on the double integrator over the 1024x1024 64-blob map, K=8192, T=100: host-inclusive control
-    rates of both controllers (RMPPI: candidate evaluation + optimisation per call), the two-system rollout kernel's
-    duration in RMPPI mode (HIP events on its stream), its roofline over the algorithmic bytes of the two systems'
-    sample reads and writes and costs, 2 (8 K T C + 4 K) per launch, and the oracle's CPU loop."""
+    rates of both controllers (RMPPI: candidate evaluation + optimisation per call); the durations (HIP events on the
+    controller's stream) of the RMPPI controller's fused two-system iteration kernel and its combine launch, with the
+    fused kernel's roofline over its algorithmic bytes per launch -- the applied controls and costs of both systems
+    written, 2 (4 K T C + 4 K), plus the per-workgroup partials, 2 G (4 T C + 32); the kernel-level two-system rollout
+    entry point in RMPPI mode on device buffers (samples read and written, 2 (8 K T C + 4 K)); and the oracle's CPU
+    loop."""
     from mppi_generic_vendor_amd import capi, hip
     from mppi_generic_vendor_amd.controller import RobustMPPIController, TubeMPPIController
 
@@ -454,6 +457,8 @@ def run_two_system(args, host_calls: int, cpu_seconds: float):
         for _ in range(host_calls):
             call()
         rates[name] = host_calls / (time.perf_counter() - t0)
+        if name == "rmppi":
+            fused_ms, comb_ms, G = ctrl.timeKernels(max(args.kernel_reps, 5))
         ctrl.close()
     # the kernel alone, RMPPI mode, on device buffers
     rng = np.random.default_rng(3)
@@ -486,8 +491,11 @@ def run_two_system(args, host_calls: int, cpu_seconds: float):
     ms = e0.elapsed_ms(e1) / reps
     alg = 2 * (8 * K * T * C + 4 * K)
     ach = alg / (ms * 1e-3) / 1e9
-    log(f"two-system: Tube {rates['tube']:.1f} Hz, RMPPI {rates['rmppi']:.1f} Hz host-inclusive; RMPPI rollout kernel "
-        f"{ms:.4f} ms = {ach:.1f} GB/s")
+    alg_f = 2 * (4 * K * T * C + 4 * K) + 2 * G * (4 * T * C + 32)
+    ach_f = alg_f / (fused_ms * 1e-3) / 1e9
+    log(f"two-system: Tube {rates['tube']:.1f} Hz, RMPPI {rates['rmppi']:.1f} Hz host-inclusive; RMPPI fused iteration "
+        f"kernel {fused_ms:.4f} ms = {ach_f:.1f} GB/s + combine {comb_ms:.4f} ms ({G} partials per system); "
+        f"kernel-level rollout {ms:.4f} ms = {ach:.1f} GB/s")
     cpu = None
     if cpu_seconds > 0:
         sys.path.insert(0, str(ROOT / "oracle"))
@@ -514,11 +522,18 @@ def run_two_system(args, host_calls: int, cpu_seconds: float):
     return {"workload": "Tube-MPPI / RMPPI, double integrator over the 1024x1024 64-blob map, K=8192, T=100 (two "
                         "systems per sample), PD feedback gains (RMPPI)",
             "tube_control_hz_host": round(rates["tube"], 3), "rmppi_control_hz_host": round(rates["rmppi"], 3),
-            "kernels_ms": {"two_system_rollout_kernel<2, 2, 0>": round(ms, 5)},
-            "roofline": {"kernel": "two_system_rollout_kernel<2, 2, 0>", "bound": "hbm", "achieved": round(ach, 2),
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
-                         "algorithmic_bytes": alg, "avg_ms": round(ms, 5),
-                         "traffic": measured_traffic("tube_di", "two_system_rollout_kernel<2, 2, 0>")},
+            "kernels_ms": {"two_system_rollout_kernel<2, 2, 0, true>": round(fused_ms, 5),
+                           "combine_kernel (two systems, one launch)": round(comb_ms, 5),
+                           "two_system_rollout_kernel<2, 2, 0, false>": round(ms, 5)},
+            "roofline": {"kernel": "two_system_rollout_kernel<2, 2, 0, true>", "bound": "hbm",
+                         "achieved": round(ach_f, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach_f / HBM_PEAK_GBS, 5), "algorithmic_bytes": alg_f, "avg_ms": round(fused_ms, 5),
+                         "partials_per_system": G,
+                         "traffic": measured_traffic("tube_di", "two_system_rollout_kernel<2, 2, 0, true>")},
+            "kernel_level_rollout": {"kernel": "two_system_rollout_kernel<2, 2, 0, false>", "achieved": round(ach, 2),
+                                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "algorithmic_bytes": alg,
+                                     "avg_ms": round(ms, 5),
+                                     "traffic": measured_traffic("tube_di", "two_system_rollout_kernel<2, 2, 0, false>")},
             "utilisation": measured_utilisation("tube_di", ["two_system_rollout_kernel", "candidate_eval_kernel"]),
             "cpu_baseline_rollout": cpu}
 

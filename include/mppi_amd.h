@@ -744,6 +744,14 @@ int mppi_tube_get_candidates(const mppi_tube_controller* ctrl, float* states, in
                              float* free_energies, int32_t* best_index);
 /* The last iteration's applied controls [2][K][T][C] and costs [2][K] (system index as above; either may be NULL). */
 int mppi_tube_get_samples(const mppi_tube_controller* ctrl, float* samples_host, float* costs_host);
+/* Benchmark utility (no counterpart in the reference): `reps` repeats of the last iteration run (same states, means,
+ * generator offset, stride and iteration; Gaussian sampler only) launched eagerly with HIP events on the controller's
+ * stream before the fused two-system rollout kernel, between it and the combine launch (both systems), and after it;
+ * the average durations of the rollout launch and of the combine launch are returned, and the number of per-workgroup
+ * partials per system (num_partials may be NULL). The repeats rewrite the samples and costs with the same
+ * values; the host-side sequences and statistics are unchanged. */
+int mppi_tube_time_kernels(mppi_tube_controller* ctrl, int32_t reps, float* rollout_ms, float* combine_ms,
+                           int32_t* num_partials);
 
 #ifdef __cplusplus
 } /* extern "C" */

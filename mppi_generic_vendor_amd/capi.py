@@ -237,6 +237,7 @@ _SIGS = {
     "mppi_tube_get_feedback_control": ([P, P, P, _i, P], ct.c_int),
     "mppi_tube_set_thresholds": ([P, _f, _f], ct.c_int),
     "mppi_tube_get_samples": ([P, P, P], ct.c_int),
+    "mppi_tube_time_kernels": ([P, _i, ct.POINTER(_f), ct.POINTER(_f), ct.POINTER(_i)], ct.c_int),
     "mppi_tube_update_importance_sampling_control": ([P, P, _i], ct.c_int),
     "mppi_tube_get_candidates": ([P, P, P, P, P, ct.POINTER(_i)], ct.c_int),
     "mppi_batch_time_kernels": ([P, _i, ct.POINTER(_f), ct.POINTER(_f)], ct.c_int),

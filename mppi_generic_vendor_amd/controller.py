@@ -441,6 +441,14 @@ class TubeMPPIController:
         capi.call("mppi_tube_get_samples", self._h, u.ctypes.data, c.ctypes.data)
         return u, c
 
+    def timeKernels(self, reps: int = 50):
+        """Average (fused two-system rollout kernel, two-system combine) durations in ms over `reps` repeats of the last
+        iteration (Gaussian sampler), from HIP events on the controller's stream, and the number of per-workgroup
+        partials each combine reduces."""
+        a, b, g = ct.c_float(), ct.c_float(), ct.c_int32()
+        capi.call("mppi_tube_time_kernels", self._h, int(reps), ct.byref(a), ct.byref(b), ct.byref(g))
+        return a.value, b.value, g.value
+
 
 class RobustMPPIController(TubeMPPIController):
     """The optimisation of RobustMPPIController (controllers/R-MPPI/robust_mppi_controller.cu:635-756): nominal

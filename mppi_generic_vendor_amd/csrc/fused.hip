@@ -186,8 +186,16 @@ __global__ void __launch_bounds__(THREADS) combine_kernel(const Partial* __restr
                                                           const float* __restrict__ sums, int G, int K, int TC,
                                                           float lambda, mppi_weight_stats* __restrict__ stats,
                                                           float* __restrict__ mean_out, IterArgs* advance,
-                                                          int advance_step, ResultCopies copies)
+                                                          int advance_step, ResultCopies copies, int pstride)
 {
+  // system blockIdx.y of a multi-system combine: its partials, rows and results lie one stride further on
+  const int sys = blockIdx.y;
+  parts += (size_t)sys * pstride;
+  sums += (size_t)sys * pstride * TC;
+  stats += sys;
+  mean_out += (size_t)sys * TC;
+  if (copies.mean != nullptr) copies.mean += (size_t)sys * TC;
+  if (copies.stats != nullptr) copies.stats += sys;
   constexpr int COMBINE_COLS = COLS;
   constexpr int COMBINE_THREADS = THREADS;
   __shared__ float escale[MAX_PARTIALS];
@@ -288,8 +296,8 @@ __global__ void __launch_bounds__(THREADS) combine_kernel(const Partial* __restr
     st.weighted_cost = wcost / eta;
     *stats = st;
     if (copies.stats != nullptr) *copies.stats = st;
-    if (copies.cost0 != nullptr) *copies.cost0 = *copies.cost0_src;
-    if (advance != nullptr) advance->rng_offset += (uint64_t)advance_step;
+    if (sys == 0 && copies.cost0 != nullptr) *copies.cost0 = *copies.cost0_src;
+    if (sys == 0 && advance != nullptr) advance->rng_offset += (uint64_t)advance_step;
   }
   if (copies.signal.flag != nullptr && last_block_arrive(copies.signal)) {
     // the last workgroup, which sees every column of the new mean: computeControl's post-processing
@@ -478,16 +486,20 @@ int launch_fused_iteration(const mppi_dynamics_params& dyn, const mppi_cost_para
 
 int launch_combine(const Partial* parts, const float* sums, int G, int K, int TC, float lambda,
                    mppi_weight_stats* stats, float* mean_out, IterArgs* advance, int advance_step,
-                   const ResultCopies* copies, hipStream_t s)
+                   const ResultCopies* copies, hipStream_t s, int systems, int pstride)
 {
   if (G < 1 || G > MAX_PARTIALS) return fail(MPPI_E_SIZE, "combine: partial count out of range");
+  if (systems < 1 || systems > 2 || (systems > 1 && pstride < G))
+    return fail(MPPI_E_INVALID, "combine: one or two systems, partial stride >= partial count");
   const ResultCopies rc = copies != nullptr ? *copies : ResultCopies{};
+  if (systems > 1 && rc.control != nullptr)
+    return fail(MPPI_E_INVALID, "combine: the smoothed-control copy takes one system");
   if (G <= 256)
-    combine_kernel<8, 256><<<(TC + 7) / 8, 256, 0, s>>>(parts, sums, G, K, TC, lambda, stats, mean_out, advance,
-                                                        advance_step, rc);
+    combine_kernel<8, 256><<<dim3((TC + 7) / 8, systems), 256, 0, s>>>(parts, sums, G, K, TC, lambda, stats, mean_out,
+                                                                       advance, advance_step, rc, pstride);
   else
-    combine_kernel<16, 512><<<(TC + 15) / 16, 512, 0, s>>>(parts, sums, G, K, TC, lambda, stats, mean_out, advance,
-                                                           advance_step, rc);
+    combine_kernel<16, 512><<<dim3((TC + 15) / 16, systems), 512, 0, s>>>(parts, sums, G, K, TC, lambda, stats,
+                                                                          mean_out, advance, advance_step, rc, pstride);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }

@@ -61,14 +61,17 @@ constexpr int MAX_PARTIALS = 4096;
 // The fused two-system iteration (Tube-MPPI / RMPPI, Gaussian samples of `sp` for both systems, both drawn from the
 // same normals around mean[d][T][C]): rollout of both systems, then per system d the Partials parts[d gstride + g] and
 // partial rows psums[(d gstride + g) TC ..] of the G workgroups (G <= gstride, returned), the costs [2][K] and the
-// applied controls [2][K][T][C]. combine_kernel (launch_combine) forms each system's new mean from them.
+// applied controls [2][K][T][C]. launch_combine with systems = 2 and pstride = gstride forms both new means.
 int launch_two_system_fused(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                             const mppi_sampler_params& sp, const mppi_two_system_params& tp, int K, int T, float dt,
                             float lambda, float alpha, const float* x0, const float* mean, float* samples,
                             float* costs, IterRef itr, Partial* parts, float* psums, int gstride, int* G,
                             hipStream_t s);
+// With systems = 2 one launch combines two independent sets: set y's partials at parts[y pstride ..], its rows at
+// sums[y pstride TC ..], its statistics at stats[y], its mean at mean_out[y TC ..] (copies.mean / copies.stats
+// likewise); `advance` and cost0 belong to set 0, and the smoothed-control copy is not available.
 int launch_combine(const Partial* parts, const float* sums, int G, int K, int TC, float lambda,
                    mppi_weight_stats* stats, float* mean_out, IterArgs* advance, int advance_step,
-                   const ResultCopies* copies, hipStream_t s);
+                   const ResultCopies* copies, hipStream_t s, int systems = 1, int pstride = 0);
 
 }  // namespace mppi_amd

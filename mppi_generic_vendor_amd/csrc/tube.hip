@@ -67,6 +67,7 @@ struct mppi_tube_controller {
   mppi_weight_stats stats[2]{};
   mppi_free_energy fe[2]{};
   uint64_t rng_counter = 0;
+  IterArgs last_args{};  // generator offset, stride and iteration of the last iteration run
   bool have_samples = false;
   // RMPPI candidate evaluation
   int N = 0, M = 0;
@@ -276,6 +277,7 @@ int tube_iteration(mppi_tube_controller* c, const float* x0_host, int stride, in
     c->gains_dirty = false;
   }
   const IterRef ref{nullptr, {c->rng_counter, stride, iter}};
+  c->last_args = ref.val;
   mppi_two_system_params tp{};
   tp.mode = c->d.mode;
   tp.nominal_index = nom;
@@ -283,21 +285,18 @@ int tube_iteration(mppi_tube_controller* c, const float* x0_host, int stride, in
   tp.feedback_gains = (c->rmppi() && !c->gains.empty()) ? c->gains_d : nullptr;
   if (b.sampler.kind == MPPI_SAMPLER_GAUSSIAN) {
     // the fused iteration: both systems' samples drawn in the rollout kernel, per-workgroup partials, and one combine
-    // per system writing its new mean and statistics to the mapped buffers (each combine signals; the second one's
-    // epoch ends the call)
+    // launch over both systems writing the new means and statistics to the mapped buffers, then the signal
     int G = 0;
     MPPI_CHECK(launch_two_system_fused(b.dynamics, b.cost, b.sampler, tp, K, T, b.dt, b.lambda, b.alpha, c->x0_d,
                                        c->mean_d, c->samples_d, c->costs_d, ref, c->parts_d, c->psums_d, c->gstride,
                                        &G, c->stream));
-    for (int d = 0; d < 2; ++d) {
-      ResultCopies rc;
-      rc.mean = out_hd + d * TC;
-      rc.stats = st_hd + d;
-      rc.signal = tube_signal(c);
-      MPPI_CHECK(launch_combine(c->parts_d + (size_t)d * c->gstride, c->psums_d + (size_t)d * c->gstride * TC, G, K,
-                                (int)TC, b.lambda, c->stats_d + d, c->out_d + d * TC, nullptr, 1, &rc, c->stream));
-    }
-    MPPI_CHECK(tube_wait(c, 2));
+    ResultCopies rc;
+    rc.mean = out_hd;
+    rc.stats = st_hd;
+    rc.signal = tube_signal(c);
+    MPPI_CHECK(launch_combine(c->parts_d, c->psums_d, G, K, (int)TC, b.lambda, c->stats_d, c->out_d, nullptr, 1, &rc,
+                              c->stream, 2, c->gstride));
+    MPPI_CHECK(tube_wait(c));
   } else {
     for (int d = 0; d < 2; ++d) {
       float* sd = c->samples_d + (size_t)d * K * TC;
@@ -645,6 +644,59 @@ int mppi_tube_get_samples(const mppi_tube_controller* c, float* samples_host, fl
   if (costs_host)
     MPPI_HIP_CHECK(hipMemcpyAsync(costs_host, c->costs_d, sizeof(float) * 2 * c->K, hipMemcpyDeviceToHost, c->stream));
   MPPI_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return MPPI_OK;
+}
+
+int mppi_tube_time_kernels(mppi_tube_controller* c, int32_t reps, float* rollout_ms, float* combine_ms,
+                           int32_t* num_partials)
+{
+  if (!c || !rollout_ms || !combine_ms || reps < 1) return fail(MPPI_E_INVALID, "bad argument");
+  if (!c->have_samples) return fail(MPPI_E_INVALID, "time_kernels: call computeControl once first");
+  const mppi_controller_desc& b = c->b();
+  if (b.sampler.kind != MPPI_SAMPLER_GAUSSIAN)
+    return fail(MPPI_E_INVALID, "time_kernels: the fused two-system iteration takes the Gaussian sampler");
+  const int K = c->K;
+  const size_t TC = (size_t)c->T * c->C;
+  // the last iteration again: x0_d and mean_d still hold its inputs
+  const IterRef ref{nullptr, c->last_args};
+  mppi_two_system_params tp{};
+  tp.mode = c->d.mode;
+  tp.nominal_index = c->nom();
+  tp.value_function_threshold = c->d.value_function_threshold;
+  tp.feedback_gains = (c->rmppi() && !c->gains.empty()) ? c->gains_d : nullptr;
+  std::vector<hipEvent_t> ev((size_t)(2 * reps + 1), nullptr);
+  int st = MPPI_OK;
+  for (auto& e : ev)
+    if (st == MPPI_OK && hipEventCreate(&e) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventCreate");
+  if (st == MPPI_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventRecord");
+  for (int r = 0; r < reps && st == MPPI_OK; ++r) {
+    int G = 0;
+    st = launch_two_system_fused(b.dynamics, b.cost, b.sampler, tp, K, c->T, b.dt, b.lambda, b.alpha, c->x0_d,
+                                 c->mean_d, c->samples_d, c->costs_d, ref, c->parts_d, c->psums_d, c->gstride, &G,
+                                 c->stream);
+    if (num_partials != nullptr) *num_partials = G;
+    if (st == MPPI_OK && hipEventRecord(ev[2 * r + 1], c->stream) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventRecord");
+    if (st == MPPI_OK)
+      st = launch_combine(c->parts_d, c->psums_d, G, K, (int)TC, b.lambda, c->stats_d, c->out_d, nullptr, 1, nullptr,
+                          c->stream, 2, c->gstride);
+    if (st == MPPI_OK && hipEventRecord(ev[2 * r + 2], c->stream) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventRecord");
+  }
+  float roll = 0.0f, comb = 0.0f;
+  if (st == MPPI_OK && hipEventSynchronize(ev.back()) != hipSuccess) st = fail(MPPI_E_HIP, "hipEventSynchronize");
+  for (int r = 0; r < reps && st == MPPI_OK; ++r) {
+    float x = 0.0f, y = 0.0f;
+    if (hipEventElapsedTime(&x, ev[2 * r], ev[2 * r + 1]) != hipSuccess ||
+        hipEventElapsedTime(&y, ev[2 * r + 1], ev[2 * r + 2]) != hipSuccess)
+      st = fail(MPPI_E_HIP, "hipEventElapsedTime");
+    roll += x;
+    comb += y;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st != MPPI_OK) return st;
+  *rollout_ms = roll / reps;
+  *combine_ms = comb / reps;
   return MPPI_OK;
 }
 

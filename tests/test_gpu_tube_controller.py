@@ -215,3 +215,38 @@ def test_rmppi_candidate_selection_matches_oracle(hip, engine):
     below = np.nonzero(fe2 < np.float32(ctrl.getValueFunctionThreshold()))[0]
     assert best2 == (below[-1] if below.size else N - 1)
     ctrl.close()
+
+
+def test_rmppi_time_kernels_repeats_the_last_iteration(hip, engine):
+    """mppi_tube_time_kernels reruns the last iteration (same offset, stride, iteration): the samples and costs it
+    rewrites equal the call's, the host sequences are untouched, and the next call is unaffected."""
+    prob = _problem(K=1536, T=50)
+    T, C, S = prob.num_timesteps, prob.C, prob.S
+    banner("RMPPI time_kernels repeats")
+    rng = np.random.default_rng(5)
+    gains = (rng.normal(size=(T, S, C)) * 0.5).astype(np.float32)
+
+    def make():
+        c = RobustMPPIController(prob, num_iters=1, value_function_threshold=60.0)
+        c.setNominalState(prob.x0)
+        c.setFeedbackGains(gains)
+        return c
+
+    a, b = make(), make()
+    for c in (a, b):
+        c.computeControl(prob.x0, 2)
+    u0, c0 = a.getSamples()
+    seq = a.getControlSeq()
+    roll, comb, G = a.timeKernels(3)
+    print(f"[tube] fused rollout {roll:.4f} ms, combines {comb:.4f} ms, {G} partials")
+    assert roll > 0 and comb > 0 and 1 <= G <= (prob.num_rollouts + 15) // 16
+    u1, c1 = a.getSamples()
+    np.testing.assert_array_equal(u1, u0)
+    np.testing.assert_array_equal(c1, c0)
+    np.testing.assert_array_equal(a.getControlSeq(), seq)
+    for c in (a, b):
+        c.computeControl(prob.x0, 1)
+    np.testing.assert_array_equal(a.getControlSeq(), b.getControlSeq())
+    np.testing.assert_array_equal(a.getNominalControlSeq(), b.getNominalControlSeq())
+    for c in (a, b):
+        c.close()
