@@ -23,6 +23,7 @@
 // sums and the weighted column sums of the applied controls as a Partial and a partial row, which combine_kernel
 // merges into the new mean. The applied controls and the costs are still written out.
 #include <algorithm>
+#include <type_traits>
 
 #include "iteration_core.hpp"
 #include "kernels.hpp"
@@ -173,88 +174,176 @@ __global__ void __launch_bounds__(TS_THREADS) two_system_rollout_kernel(
   for (int c = 0; c < C; ++c) deadband |= dyn.control_deadband[c] != 0.0f;
   __syncthreads();
 
-  // lanes beyond the tile's rows repeat its last row and write nothing back
+  // lanes beyond the tile's rows repeat its last row: they compute the same values as its lane and store them to the
+  // same words
   const int lrow = min(lane, R - 1);
-  const bool writer = lane < R;
   float* row0 = tile + lrow * ROW;
   float* row1 = tile + R * ROW + lrow * ROW;
   const int nch = (T + chunk - 1) / chunk;
+  // Tube with at most 32 samples per workgroup: the producer splits its wavefront by system, lane (sd, srow) =
+  // (lane / 32, lane % 32) carrying system sd of sample srow alone in x[0] and storing its outputs to the slots of
+  // sample srow (half the instructions per step). Otherwise each lane carries both systems of its sample (RMPPI's
+  // coupled feedback is cheaper within one lane than through the cross-half exchange of the split form). Either way
+  // slot r of the chunk buffers and the terminal costs belongs to sample row r.
+  const bool split = !RMPPI && R <= 32;
+  const int sd = lane >> 5;
+  const int srow = min(lane & 31, R - 1);
+  float* rows = tile + sd * R * ROW + srow * ROW;
 
   float x[2][S], y[2][O];
   if (wave == 0) {
+    if (split) {
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
+      for (int i = 0; i < S; ++i) x[0][i] = x0[sd * S + i];
+      M::output(dyn, x[0], y[0]);
+    } else {
 #pragma unroll
-      for (int i = 0; i < S; ++i) x[d][i] = x0[d * S + i];
-      M::output(dyn, x[d], y[d]);
+      for (int d = 0; d < 2; ++d) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) x[d][i] = x0[d * S + i];
+        M::output(dyn, x[d], y[d]);
+      }
     }
   }
   float acc[TS_ACC];
 #pragma unroll
   for (int q = 0; q < TS_ACC; ++q) acc[q] = 0.0f;
   const bool pure = (k0 + lrow) >= pstart;
+  // the producer's operands of the next step (controls of both systems, RMPPI gains), loaded one step ahead so that
+  // their LDS latency passes while the current step is computed; step t + 1's words are not written before step t + 1
+  constexpr int GN = RMPPI ? S * C : 1;
+  float un[2][C], gn[GN];
+  auto load_gains = [&](int t) {
+    if constexpr (RMPPI) {
+#pragma unroll
+      for (int j = 0; j < GN; ++j) gn[j] = gl[t * S * C + j];
+    }
+  };
+  auto load_step = [&](int t) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      un[0][c] = row0[t * C + c];
+      un[1][c] = row1[t * C + c];
+    }
+    load_gains(t);
+  };
+  auto load_step_split = [&](int t) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) un[0][c] = rows[t * C + c];
+  };
+  if (wave == 0) {
+    if (split) load_step_split(0);
+    else load_step(0);
+  }
+  // Before a chunk's steps every load still in flight is waited for (the chunk's first prefetch among them): the
+  // parameter loads issued ahead of the chunk loop are then known to be complete inside the step loop, which otherwise
+  // waits for them -- and with them for the step's own prefetch -- at every step.
+  auto settle_loads = [] { __builtin_amdgcn_s_waitcnt(0); };
+  // the producer's steps t0 .. t1 - 1, with the constraint form (deadband or plain clamp) fixed outside the loop
+  auto produce = [&](auto deadband_tag, int t0, int t1, float* yb, float* fbb) {
+    settle_loads();
+    for (int t = t0; t < t1; ++t) {
+      float u[2][C], g[GN], fbs[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        u[0][c] = un[0][c];
+        u[1][c] = un[1][c];
+      }
+#pragma unroll
+      for (int j = 0; j < GN; ++j) g[j] = gn[j];
+      load_step(min(t + 1, T - 1));  // at the last step a harmless reload
+      if constexpr (RMPPI) {
+        // fb_t = K_t (x_r - x_n) with the gains of step t
+        float fb[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) fb[c] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          const float e = x[real][i] - x[nom][i];
+#pragma unroll
+          for (int c = 0; c < C; ++c) fb[c] += g[i * C + c] * e;
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          u[real][c] += fb[c];
+          fbs[c] = fb[c];
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        if constexpr (decltype(deadband_tag)::value) enforce_constraints<C>(dyn, u[d]);
+        else clamp_controls<C>(dyn, u[d]);
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        row0[t * C + c] = u[0][c];
+        row1[t * C + c] = u[1][c];
+      }
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        float xd[S], xn[S];
+        M::deriv(dyn, x[d], u[d], xd);
+        M::update(dyn, x[d], xn, xd, dt);
+        M::output(dyn, xn, y[d]);
+#pragma unroll
+        for (int i = 0; i < O; ++i) yb[(t - t0) * 64 * 2 * OP + d * OP + i] = y[d][i];
+#pragma unroll
+        for (int i = 0; i < S; ++i) x[d][i] = xn[i];
+      }
+      // the feedback's hand-off last: its registers are then not reused while the store is in flight
+      if constexpr (RMPPI) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) fbb[(t - t0) * 64 * C + c] = fbs[c];
+      }
+    }
+  };
+  // the split form (Tube): system sd of sample srow per lane, yb at the sample's slot
+  auto produce_split = [&](auto deadband_tag, int t0, int t1, float* yb) {
+    settle_loads();
+    for (int t = t0; t < t1; ++t) {
+      float u[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) u[c] = un[0][c];
+      load_step_split(min(t + 1, T - 1));  // at the last step a harmless reload
+      if constexpr (decltype(deadband_tag)::value) enforce_constraints<C>(dyn, u);
+      else clamp_controls<C>(dyn, u);
+#pragma unroll
+      for (int c = 0; c < C; ++c) rows[t * C + c] = u[c];
+      float xd[S], xn[S];
+      M::deriv(dyn, x[0], u, xd);
+      M::update(dyn, x[0], xn, xd, dt);
+      M::output(dyn, xn, y[0]);
+#pragma unroll
+      for (int i = 0; i < O; ++i) yb[(t - t0) * 64 * 2 * OP + sd * OP + i] = y[0][i];
+#pragma unroll
+      for (int i = 0; i < S; ++i) x[0][i] = xn[i];
+    }
+  };
 
   for (int it = 0; it <= nch; ++it) {
     if (wave == 0 && it < nch) {
       const int t0 = it * chunk, t1 = min(T, t0 + chunk);
-      float* yb = ybuf + (it & 1) * chunk * 64 * 2 * OP + lane * 2 * OP;
-      float* fbb = fbuf + (it & 1) * chunk * 64 * C + lane * C;
-      for (int t = t0; t < t1; ++t) {
-        float u[2][C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          u[0][c] = row0[t * C + c];
-          u[1][c] = row1[t * C + c];
-        }
-        if constexpr (RMPPI) {
-          // fb_t = K_t (x_r - x_n); the gains of step t are uniform LDS reads
-          float fb[C];
-#pragma unroll
-          for (int c = 0; c < C; ++c) fb[c] = 0.0f;
-#pragma unroll
-          for (int i = 0; i < S; ++i) {
-            const float e = x[real][i] - x[nom][i];
-#pragma unroll
-            for (int c = 0; c < C; ++c) fb[c] += gl[(t * S + i) * C + c] * e;
-          }
-#pragma unroll
-          for (int c = 0; c < C; ++c) {
-            u[real][c] += fb[c];
-            fbb[(t - t0) * 64 * C + c] = fb[c];
-          }
-        }
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          if (deadband) enforce_constraints<C>(dyn, u[d]);
-          else clamp_controls<C>(dyn, u[d]);
-        }
-        if (writer) {
-#pragma unroll
-          for (int c = 0; c < C; ++c) {
-            row0[t * C + c] = u[0][c];
-            row1[t * C + c] = u[1][c];
-          }
-        }
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          float xd[S], xn[S];
-          M::deriv(dyn, x[d], u[d], xd);
-          M::update(dyn, x[d], xn, xd, dt);
-          M::output(dyn, xn, y[d]);
-#pragma unroll
-          for (int i = 0; i < O; ++i) yb[(t - t0) * 64 * 2 * OP + d * OP + i] = y[d][i];
-#pragma unroll
-          for (int i = 0; i < S; ++i) x[d][i] = xn[i];
-        }
+      float* ybs = ybuf + (it & 1) * chunk * 64 * 2 * OP;
+      float* fbs = fbuf + (it & 1) * chunk * 64 * C;
+      if (split) {
+        if (deadband) produce_split(std::true_type{}, t0, t1, ybs + srow * 2 * OP);
+        else produce_split(std::false_type{}, t0, t1, ybs + srow * 2 * OP);
+      } else {
+        if (deadband) produce(std::true_type{}, t0, t1, ybs + lane * 2 * OP, fbs + lane * C);
+        else produce(std::false_type{}, t0, t1, ybs + lane * 2 * OP, fbs + lane * C);
       }
       if (it == nch - 1) {
-        term[lane] = CF::terminal(cost, y[0]);
-        term[64 + lane] = CF::terminal(cost, y[1]);
+        if (split) {
+          term[sd * 64 + srow] = CF::terminal(cost, y[0]);
+        } else {
+          term[lane] = CF::terminal(cost, y[0]);
+          term[64 + lane] = CF::terminal(cost, y[1]);
+        }
       }
     } else if (wave > 0 && it > 0) {
       const int t0 = (it - 1) * chunk, t1 = min(T, t0 + chunk);
-      const float* yb = ybuf + ((it - 1) & 1) * chunk * 64 * 2 * OP + lane * 2 * OP;
-      const float* fbb = fbuf + ((it - 1) & 1) * chunk * 64 * C + lane * C;
+      const float* yb = ybuf + ((it - 1) & 1) * chunk * 64 * 2 * OP + lrow * 2 * OP;
+      const float* fbb = fbuf + ((it - 1) & 1) * chunk * 64 * C + lrow * C;
       for (int t = t0 + wave - 1; t < t1; t += TS_CONSUMERS) {
         const float* y0 = yb + (t - t0) * 64 * 2 * OP;
         const float* u0 = row0 + t * C;
@@ -299,10 +388,10 @@ __global__ void __launch_bounds__(TS_THREADS) two_system_rollout_kernel(
     const float inv_t = 1.0f / (float)T;
     float c0, c1;
     if constexpr (!RMPPI) {
-      c0 = s[0] / (float)T + term[lane] / (float)T;
-      c1 = s[1] / (float)T + term[64 + lane] / (float)T;
+      c0 = s[0] / (float)T + term[lrow] / (float)T;
+      c1 = s[1] / (float)T + term[64 + lrow] / (float)T;
     } else {
-      const float tr = term[real * 64 + lane], tn = term[nom * 64 + lane];
+      const float tr = term[real * 64 + lrow], tn = term[nom * 64 + lrow];
       const float cost_r = (s[0] + tr) * inv_t;
       const float track_r = (s[1] + tr) * inv_t;
       const float state_n = (s[2] + tn) * inv_t;

@@ -91,8 +91,9 @@ def test_tube_iteration_matches_oracle(hip, engine, threshold, replaced):
     ctrl.close()
 
 
-def test_rmppi_iteration_matches_oracle(hip, engine):
-    prob = _problem(K=1536, T=50)
+@pytest.mark.parametrize("K", [1536, 9000])
+def test_rmppi_iteration_matches_oracle(hip, engine, K):
+    prob = _problem(K=K, T=50)
     T, C, S = prob.num_timesteps, prob.C, prob.S
     banner("RMPPI controller iteration with feedback gains")
     rng = np.random.default_rng(4)

@@ -64,8 +64,10 @@ def _inputs(prob, seed, scale=1.0):
     return u, means, x0s
 
 
+# K = 9000 fills the CUs with 64-sample workgroups (one lane per sample carrying both systems); the smaller K run
+# 16- or 32-sample workgroups, whose producer lanes carry one system each
 @pytest.mark.parametrize("name,K,T", [("di_map", 2000, 60), ("cartpole", 300, 100), ("bicycle", 1000, 50),
-                                      ("di_map", 512, 256)])
+                                      ("di_map", 512, 256), ("di_map", 9000, 40)])
 def test_tube_two_system_matches_oracle(hip, engine, name, K, T):
     prob = _problem(name, K, T)
     banner(f"two-system Tube rollout {name} K={K} T={T}")
@@ -83,7 +85,8 @@ def test_tube_two_system_matches_oracle(hip, engine, name, K, T):
 
 
 @pytest.mark.parametrize("name,K,T,nom", [("di_map", 2000, 60, 0), ("cartpole", 700, 100, 1), ("bicycle", 1000, 50, 0),
-                                          ("bicycle", 96, 40, 1), ("di_map", 256, 256, 1)])
+                                          ("bicycle", 96, 40, 1), ("di_map", 256, 256, 1), ("di_map", 9000, 40, 1),
+                                          ("bicycle", 9000, 30, 0)])
 def test_rmppi_two_system_matches_oracle(hip, engine, name, K, T, nom):
     prob = _problem(name, K, T)
     banner(f"two-system RMPPI rollout {name} K={K} T={T} nominal={nom}")
