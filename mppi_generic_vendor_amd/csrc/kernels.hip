@@ -107,15 +107,17 @@ __global__ void __launch_bounds__(256) sample_gaussian_kernel(mppi_sampler_param
 }
 
 int launch_sample_gaussian(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
-                           IterRef it, const float* eps_in, hipStream_t s)
+                           IterRef it, const float* eps_in, hipStream_t s, int rows)
 {
   const int pstart = pure_noise_start(K, sp.pure_noise_trajectories_percentage);
-  const int64_t n = (int64_t)K * T * C;
+  const int Kw = (rows >= 0 && rows < K) ? rows : K;  // samples written
+  if (Kw == 0) return MPPI_OK;
+  const int64_t n = (int64_t)Kw * T * C;
   const int grid = grid_for((n + 3) / 4, 256, 8192);
   if (((uintptr_t)samples & 15) == 0)
-    sample_gaussian_kernel<true><<<grid, 256, 0, s>>>(sp, K, T, C, mean, samples, it, eps_in, pstart);
+    sample_gaussian_kernel<true><<<grid, 256, 0, s>>>(sp, Kw, T, C, mean, samples, it, eps_in, pstart);
   else
-    sample_gaussian_kernel<false><<<grid, 256, 0, s>>>(sp, K, T, C, mean, samples, it, eps_in, pstart);
+    sample_gaussian_kernel<false><<<grid, 256, 0, s>>>(sp, Kw, T, C, mean, samples, it, eps_in, pstart);
   MPPI_HIP_CHECK(hipGetLastError());
   return MPPI_OK;
 }

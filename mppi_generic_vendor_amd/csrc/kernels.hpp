@@ -19,8 +19,10 @@ struct IterRef {
 };
 
 int launch_generate_normals(uint64_t seed, uint64_t offset, float* out, int64_t n, hipStream_t s);
+// With rows >= 0 only the first `rows` samples of the K-sample draw are written (the same values: the pure-noise
+// share still follows K).
 int launch_sample_gaussian(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
-                           IterRef it, const float* eps_in, hipStream_t s);
+                           IterRef it, const float* eps_in, hipStream_t s, int rows = -1);
 int launch_sample_colored(const mppi_sampler_params& sp, int K, int T, int C, const float* mean, float* samples,
                           IterRef it, const float* spectra_in, hipStream_t s);
 int launch_rollout(const mppi_dynamics_params& dyn, const mppi_cost_params& cost, const mppi_sampler_params& sp,

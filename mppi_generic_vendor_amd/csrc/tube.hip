@@ -570,12 +570,14 @@ int mppi_tube_update_importance_sampling_control(mppi_tube_controller* c, const 
                            {cand_iod + (size_t)N * S, c->stride_d, N},
                            {c->host_io_hd + 2 * MPPI_MAX_STATE, c->mean_d, (int)TC}};
     MPPI_CHECK(launch_copy_words(in, 3, c->stream));
-    // generateSamples(stride, 0) around the nominal control (:593-597)
+    // generateSamples(stride, 0) around the nominal control (:593-597); the evaluation reads samples j < M, so of a
+    // Gaussian draw only those rows are written
     const IterRef ref{nullptr, {c->rng_counter, stride, 0}};
     if (b.sampler.kind == MPPI_SAMPLER_COLORED)
       MPPI_CHECK(launch_sample_colored(b.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, c->stream));
     else
-      MPPI_CHECK(launch_sample_gaussian(b.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, c->stream));
+      MPPI_CHECK(launch_sample_gaussian(b.sampler, K, T, C, c->mean_d, c->samples_d, ref, nullptr, c->stream,
+                                        std::min(M, K)));
     MPPI_CHECK(launch_candidate_eval(b.dynamics, b.cost, b.sampler, N, M, K, T, b.dt, b.lambda, b.alpha, c->cand_d,
                                      c->stride_d, c->mean_d, c->samples_d, c->eval_d, c->stream));
     const WordSpan res[] = {{c->eval_d, cand_iod + (size_t)N * S + N, N * M}};
