@@ -138,7 +138,7 @@ typedef struct mppi_cost_params {
    * semantics (utils/texture_helpers/two_d_texture_helper.cu:151-245, texture_helper.cu:94-134).
    * World -> texel: q = ((R (p - origin)) / resolution) / extent * extent - 0.5, clamped, R = map_rotation (row-major
    * 2 x 2, worldPoseToMapPose's rotation restricted to the plane) when map_rotated, else the identity.
-   * map == NULL disables it. */
+   * map == NULL disables it. At most 2^30 texels (MPPI_E_SIZE otherwise). */
   const float* map; /* device pointer [map_height][map_width] */
   int32_t map_width;
   int32_t map_height;

@@ -128,6 +128,9 @@ int validate_cost(const mppi_cost_params& c, const mppi_dynamics_params& d, bool
     if (c.map != nullptr || !need_map_ptr) {
       if (c.map_coeff != 0.0f && (c.map != nullptr || !need_map_ptr)) {
         if (c.map_width < 2 || c.map_height < 2) return fail(MPPI_E_INVALID, "cost map: width/height must be >= 2");
+        // the kernels index texels with 32-bit offsets
+        if ((int64_t)c.map_width * c.map_height > ((int64_t)1 << 30))
+          return fail(MPPI_E_SIZE, "cost map: at most 2^30 texels");
         if (!(c.map_resolution > 0.0f)) return fail(MPPI_E_INVALID, "cost map: resolution must be > 0");
         if (c.map_x_index < 0 || c.map_x_index >= c.output_dim || c.map_y_index < 0 || c.map_y_index >= c.output_dim)
           return fail(MPPI_E_INVALID, "cost map: position indices out of range");

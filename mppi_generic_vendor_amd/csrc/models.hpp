@@ -381,17 +381,29 @@ MPPI_HD MapTap map_fetch(const mppi_cost_params& c, float px, float py)
   m.qy = fminf(fmaxf(dy * inv_res - 0.5f, 0.0f), (float)(H - 1));
   m.x0 = min((int)floorf(m.qx), W - 2);
   m.y0 = min((int)floorf(m.qy), H - 2);
-  // 32-bit texel index (maps hold at most 2^30 values); on the device the loads are global-memory loads
+  // 32-bit texel index (maps hold at most 2^30 values); on the device each row's texel pair (x0, x0 + 1) is one
+  // 8-byte global load at 4-byte alignment
   const uint32_t i0 = (uint32_t)m.y0 * (uint32_t)W + (uint32_t)m.x0, i1 = i0 + (uint32_t)W;
 #ifdef __HIP_DEVICE_COMPILE__
-  const __attribute__((address_space(1))) float* map = (const __attribute__((address_space(1))) float*)c.map;
+  struct __attribute__((aligned(4))) TexelPair {
+    float a, b;
+  };
+  using GlobalPair = const __attribute__((address_space(1))) TexelPair;
+  using GlobalFloat = const __attribute__((address_space(1))) float;
+  GlobalFloat* map = (GlobalFloat*)c.map;
+  GlobalPair* r0 = (GlobalPair*)(map + i0);
+  GlobalPair* r1 = (GlobalPair*)(map + i1);
+  m.q11 = r0->a;
+  m.q12 = r0->b;
+  m.q21 = r1->a;
+  m.q22 = r1->b;
 #else
   const float* map = c.map;
-#endif
   m.q11 = map[i0];
   m.q12 = map[i0 + 1];
   m.q21 = map[i1];
   m.q22 = map[i1 + 1];
+#endif
   return m;
 }
 
