@@ -527,7 +527,8 @@ typedef struct mppi_instance_desc {
 
 /* Launch shape of the persistent kernel: threads per workgroup (256, 512, 1024: one state-chain wavefront, the rest
  * evaluate costs and draw the next chunk's samples), samples per lane of the state-chain wavefront (1, 2 or 4: 64,
- * 128 or 256 samples per tile), horizon chunk (4, 8, 16 steps), and the resulting workgroups per CU. iteration_us is the
+ * 128 or 256 samples per tile), horizon chunk (2 to 16 steps; the autotuner tries 3, 4, 6, 8, 12 and 16, and a chunk
+ * that the cost wavefronts divide evenly balances them), and the resulting workgroups per CU. iteration_us is the
  * measured time of one batched iteration (0 if not measured). */
 typedef struct mppi_batch_config {
   int32_t threads;

@@ -932,7 +932,7 @@ int mppi_batch_set_config(mppi_batch* b, const mppi_batch_config* c)
     return fail(MPPI_E_INVALID, "batch: samples_per_lane must be 1, 2 or 4");
   if (c->threads != 256 && c->threads != 512 && c->threads != 1024)
     return fail(MPPI_E_INVALID, "batch: threads must be 256, 512 or 1024");
-  if (c->chunk != 4 && c->chunk != 8 && c->chunk != 16) return fail(MPPI_E_INVALID, "batch: chunk must be 4, 8 or 16");
+  if (c->chunk < 2 || c->chunk > 16) return fail(MPPI_E_INVALID, "batch: chunk must be in [2, 16]");
   MPPI_HIP_CHECK(hipStreamSynchronize(b->stream));
   return batch_apply_config(b, *c);
 }
@@ -961,7 +961,7 @@ int mppi_batch_autotune(mppi_batch* b, int32_t reps, mppi_batch_config* best_out
   std::vector<mppi_batch_config> shapes;
   for (int nt : {256, 512, 1024})
     for (int spl : {1, 2, 4})
-      for (int ch : {4, 8, 16}) shapes.push_back(mppi_batch_config{nt, spl, ch, 0, 0.0f});
+      for (int ch : {3, 4, 6, 8, 12, 16}) shapes.push_back(mppi_batch_config{nt, spl, ch, 0, 0.0f});
   std::vector<float> best_us(shapes.size(), -1.0f);
   int st = MPPI_OK;
   for (int round = 0; round < 2 && st == MPPI_OK; ++round)

@@ -374,13 +374,20 @@ MPPI_HD MapTap map_fetch(const mppi_cost_params& c, float px, float py)
   // texel coordinate m / resolution - 0.5 (the reference's normalise-by-extent and scale-back cancel); the reciprocal
   // of the uniform resolution is computed once per kernel
   const float inv_res = 1.0f / c.map_resolution;
+  const float qx = dx * inv_res - 0.5f, qy = dy * inv_res - 0.5f;
   // clamp to [0, W - 1] as the reference's if/else chain does for every non-NaN value; a NaN query maps to texel 0
-  // instead of reaching the index computation
+  // instead of reaching the index computation (v_med3_f32 with a NaN operand returns the minimum of the others); the
+  // clamped coordinate is >= 0, so truncation gives the floor
   MapTap m;
-  m.qx = fminf(fmaxf(dx * inv_res - 0.5f, 0.0f), (float)(W - 1));
-  m.qy = fminf(fmaxf(dy * inv_res - 0.5f, 0.0f), (float)(H - 1));
-  m.x0 = min((int)floorf(m.qx), W - 2);
-  m.y0 = min((int)floorf(m.qy), H - 2);
+#ifdef __HIP_DEVICE_COMPILE__
+  m.qx = __builtin_amdgcn_fmed3f(qx, 0.0f, (float)(W - 1));
+  m.qy = __builtin_amdgcn_fmed3f(qy, 0.0f, (float)(H - 1));
+#else
+  m.qx = fminf(fmaxf(qx, 0.0f), (float)(W - 1));
+  m.qy = fminf(fmaxf(qy, 0.0f), (float)(H - 1));
+#endif
+  m.x0 = min((int)m.qx, W - 2);
+  m.y0 = min((int)m.qy, H - 2);
   // 32-bit texel index (maps hold at most 2^30 values); on the device each row's texel pair (x0, x0 + 1) is one
   // 8-byte global load at 4-byte alignment
   const uint32_t i0 = (uint32_t)m.y0 * (uint32_t)W + (uint32_t)m.x0, i1 = i0 + (uint32_t)W;

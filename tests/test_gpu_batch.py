@@ -58,7 +58,8 @@ def _check_instance(b, i, p, nominal, rng_offset, stride=1):
     assert em < 1e-4
 
 
-@pytest.mark.parametrize("shape", [(1024, 1, 16), (256, 1, 4), (512, 2, 8), (1024, 2, 4), (256, 4, 4)])
+@pytest.mark.parametrize("shape", [(1024, 1, 16), (256, 1, 4), (512, 2, 8), (1024, 2, 4), (256, 4, 4), (256, 1, 6),
+                                   (512, 1, 12), (256, 1, 3), (256, 2, 5)])
 def test_batch_matches_oracle(hip, engine, shape):
     banner(f"batch mixed 6 instances shape={shape}")
     probs = _mixed_batch()
