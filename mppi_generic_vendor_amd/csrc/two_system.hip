@@ -422,7 +422,8 @@ __global__ void __launch_bounds__(TS_THREADS) two_system_rollout_kernel(
 }
 
 // Samples per workgroup: the smallest of 64, 32, 16 whose grid still fits the CUs once (a small K spreads over more
-// CUs), then halved until the tiles and buffers fit the LDS; the chunk is the largest of 16, 8, 4 that fits.
+// CUs), then halved until the tiles and buffers fit the LDS; the chunk is the first of 15 (one step per cost
+// wavefront), 8 and 4 that fits.
 template <int MODEL, int COST, int NOM, bool FUSED>
 static int launch_two_system_t(const mppi_dynamics_params& dyn, const mppi_cost_params& cost,
                                const mppi_sampler_params& sp0, const mppi_sampler_params& sp1, const TwoSystemArgs& ta,
@@ -438,8 +439,8 @@ static int launch_two_system_t(const mppi_dynamics_params& dyn, const mppi_cost_
   int R = fused_rows_for(K, cus > 0 ? cus : 256), chunk = 0;
   const size_t limit = LDS_LIMIT / sizeof(float);
   for (;;) {
-    for (int ch = 16; ch >= 4 && chunk == 0; ch >>= 1)
-      if (ts_lds_floats<M::S, M::C, M::O>(T, R, ch, RMPPI) <= limit) chunk = ch;
+    for (int ch : {TS_CONSUMERS, 8, 4})
+      if (chunk == 0 && ts_lds_floats<M::S, M::C, M::O>(T, R, ch, RMPPI) <= limit) chunk = ch;
     if (chunk != 0 || R == 16) break;
     R /= 2;
   }
