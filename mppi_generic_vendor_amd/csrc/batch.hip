@@ -54,12 +54,12 @@ constexpr int SUM_WAVES = 4;
 __host__ __device__ constexpr int sum_block_floats(int C) { return 64 * (8 * C + 1); }
 
 // LDS floats of one workgroup: the work-item slot (4 floats), then per tile the mean, the sample ring
-// [3][chunk][R][C] and the outputs [2][chunk][R][O] (their room, at least the column sums' blocks, is reused by those
+// [4][chunk][R][C] and the outputs [2][chunk][R][O] (their room, at least the column sums' blocks, is reused by those
 // blocks after the rollout), the consumer partial sums, terminal costs, costs and weights.
 static size_t batch_lds_floats(int TCmax, int Cmax, int Omax, int NT, int R, int ch)
 {
   const int ncons = NT / 64 - 1;
-  const size_t rollout = 3ull * ch * R * Cmax + 2ull * ch * R * Omax;
+  const size_t rollout = 4ull * ch * R * Cmax + 2ull * ch * R * Omax;
   const size_t sums = (size_t)std::min(NT / 64, SUM_WAVES) * sum_block_floats(Cmax);
   return 4 + (size_t)((TCmax + 3) & ~3) + std::max(rollout, sums) + (size_t)ncons * R + 3ull * R;
 }
@@ -93,10 +93,10 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
   const int k0 = g * R;
   const int nrows = min(R, K - k0);
   float* mu = lds;
-  float* ring = mu + ((TC + 3) & ~3);     // [3][ch][R][C]
-  float* ybuf = ring + 3 * ch * R * C;    // [2][ch][R][O]
+  float* ring = mu + ((TC + 3) & ~3);     // [4][ch][R][C]
+  float* ybuf = ring + 4 * ch * R * C;    // [2][ch][R][O]
   // [NCONS][R], after the larger of the rollout buffers and the column sums' blocks (batch_lds_floats)
-  float* part = ring + max(3 * ch * R * C + 2 * ch * R * O, min(NW, SUM_WAVES) * sum_block_floats(C));
+  float* part = ring + max(4 * ch * R * C + 2 * ch * R * O, min(NW, SUM_WAVES) * sum_block_floats(C));
   float* term = part + NCONS * R;         // [R]
   float* cst = term + R;                  // [R]
   float* wl = cst + R;                    // [R]
@@ -107,13 +107,17 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
   const int nch = (T + ch - 1) / ch;
   __syncthreads();
 
-  // Samples of chunk j into ring slot j % 3, by threads t_off .. t_off + nth - 1. Element (k, t, c) is normal
+  // Samples of chunk j into ring slot j % 4, by threads t_off .. t_off + nth - 1. Element (k, t, c) is normal
   // k*TC + t*C + c of the instance's stream. C divides 4, so a Philox block (4 normals) holds SPB = 4 / C whole steps of
-  // one row: thread item (q, r) evaluates block q of row r's range, whose first step lies (e mod 4) / C steps before
-  // t0 when the row's first element e is not on a block boundary. Consecutive items are consecutive rows of one block
-  // index, so a wavefront's LDS stores are row-contiguous and its mean reads are broadcasts. Element indices stay
-  // below K T C <= 2^26 (BATCH_MAX_K, MPPI_MAX_TIMESTEPS, MPPI_MAX_CONTROL), so 32-bit arithmetic holds them and the
-  // Philox counter's high word is 0.
+  // one row: thread item (qi, r) evaluates a block of row r's range, whose first step lies m = (e mod 4) / C steps
+  // before t0 when the row's first element e is not on a block boundary. A block that runs past the chunk's end (its
+  // last steps belong to chunk j + 1) also stores those steps, into slot (j + 1) % 4, which holds no live chunk then
+  // (the producer works on j - 1 and the cost wavefronts on j - 2); chunk j + 1 then skips that block. So every chunk
+  // after the first needs ceil(n / SPB) blocks per row whatever the alignment (a chunk of at least SPB steps keeps
+  // the stored steps inside chunk j + 1). Consecutive items are consecutive rows of one block index, so a wavefront's
+  // LDS stores are row-contiguous and its mean reads are broadcasts. Element indices stay below K T C <= 2^26
+  // (BATCH_MAX_K, MPPI_MAX_TIMESTEPS, MPPI_MAX_CONTROL), so 32-bit arithmetic holds them and the Philox counter's high
+  // word is 0.
   static_assert(4 % C == 0, "a Philox block holds whole steps");
   constexpr int SPB = 4 / C;
   float sdc[C];
@@ -123,31 +127,34 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
   auto gen = [&](int j, int t_off, int nth) {
     const int t0 = j * ch, n = min(T, t0 + ch) - t0;
     const bool aligned = (TC % 4 == 0) && ((t0 * C) % 4 == 0);
-    const int nb = (n + SPB - 1) / SPB + (aligned ? 0 : 1);
-    float* slot = ring + (j % 3) * ch * R * C;
+    const int nb = (n + SPB - 1) / SPB + ((j == 0 && !aligned) ? 1 : 0);
+    float* slot = ring + (j % 4) * ch * R * C;
+    float* next = ring + ((j + 1) % 4) * ch * R * C;
     for (int item = tid - t_off; item < R * nb; item += nth) {
-      const int r = item & (R - 1), q = item / R;
+      const int r = item & (R - 1), qi = item / R;
       const int kk = k0 + r;
       const int es = kk * TC + t0 * C;
-      const int s0 = q * SPB - (es & 3) / C;  // chunk step of the block's first normal
+      const int m = (es & 3) / C;                     // steps of the row's first block before the chunk
+      const int q = qi + ((j > 0 && m > 0) ? 1 : 0);  // block 0 of a misaligned row came with chunk j - 1
+      const int s0 = q * SPB - m;                     // chunk step of the block's first normal
       if (s0 >= n) continue;
       const Normal4 z = normals4(sp.seed, a.rng_offset, (uint64_t)(uint32_t)((es >> 2) + q));
       const bool keep = kk == 0, pure_row = kk >= I.pstart;
 #pragma unroll
       for (int h = 0; h < SPB; ++h) {
         const int s = s0 + h;
-        if (s < 0 || s >= n) continue;
         const int t = t0 + s;
+        if (s < 0 || t >= T) continue;
         float v[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const float m = mu[t * C + c];
+          const float mv = mu[t * C + c];
           const float sd = sdt ? decay * sdt[t * C + c] : sdc[c];
-          if (keep || t < a.stride) v[c] = m;
+          if (keep || t < a.stride) v[c] = mv;
           else if (pure_row) v[c] = sd * z.z[h * C + c];
-          else v[c] = m + sd * z.z[h * C + c];
+          else v[c] = mv + sd * z.z[h * C + c];
         }
-        float* dst = slot + (s * R + r) * C;
+        float* dst = s < n ? slot + (s * R + r) * C : next + ((s - n) * R + r) * C;
         if constexpr (C == 2) *reinterpret_cast<float2*>(dst) = make_float2(v[0], v[1]);
         else if constexpr (C == 4) *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
         else dst[0] = v[0];
@@ -179,7 +186,7 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
   for (int it = 0; it <= nch; ++it) {
     if (wave == 0 && it < nch) {
       const int t0 = it * ch, n = min(T, t0 + ch) - t0;
-      float* rs = ring + (it % 3) * ch * R * C;
+      float* rs = ring + (it % 4) * ch * R * C;
       float* yb = ybuf + (it & 1) * ch * R * O;
       float un[SPL][C];
 #pragma unroll
@@ -218,7 +225,7 @@ __device__ __forceinline__ void batch_tile(const BatchInstance& I, const IterArg
     } else if (wave > 0) {
       if (it > 0) {
         const int t0 = (it - 1) * ch, t1 = min(T, t0 + ch);
-        const float* rs = ring + ((it - 1) % 3) * ch * R * C;
+        const float* rs = ring + ((it - 1) % 4) * ch * R * C;
         const float* yb = ybuf + ((it - 1) & 1) * ch * R * O;
         // the wavefront's steps t0 + wave - 1 + j NCONS in groups of PF: the cost-map texels of a group's steps are
         // fetched first, so that their loads are in flight together, then the group's costs are added in step order
@@ -683,6 +690,9 @@ static int batch_free(mppi_batch* b)
 static int batch_apply_config(mppi_batch* b, const mppi_batch_config& c)
 {
   const int R = 64 * c.samples_per_lane;
+  // the sample generation stores a Philox block's steps past a chunk into the next chunk only: chunk >= 4 / C
+  for (int Ci : b->C)
+    if (c.chunk * Ci < 4) return fail(MPPI_E_INVALID, "batch: chunk must be at least 4 / control_dim steps");
   const size_t lds = sizeof(float) * batch_lds_floats(b->TCmax, b->Cmax, b->Omax, c.threads, R, c.chunk);
   if (lds > LDS_LIMIT) return fail(MPPI_E_SIZE, "batch: launch shape exceeds the LDS");
   int occ = 0;
